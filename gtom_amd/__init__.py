@@ -1,0 +1,371 @@
+"""Python host binding of the MI355X template-matching library (libgtomx.so).
+
+The product is the C ABI declared in include/gtomx.h; this module binds it with
+ctypes and exposes functions named after the reference's C++ API
+(d_PickSubTomograms, d_rlnProject, d_FFTR2C, ...) that take torch tensors as
+device memory. Every call goes to the HIP library; there is no fallback: if the
+library is missing or the call fails, a RuntimeError is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgtomx.so")
+_lib = None
+
+
+class GtomxError(RuntimeError):
+    def __init__(self, status, where, message):
+        super().__init__(f"{where}: status {status} ({message})")
+        self.status = status
+
+
+class int3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("z", ctypes.c_int)]
+
+
+class float3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float)]
+
+
+class CtfParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("pixelsize", "Cs", "voltage", "defocus", "amplitude", "phaseshift", "Bfactor", "scale")]
+
+    @classmethod
+    def make(cls, pixelsize=1e-10, Cs=2e-3, voltage=300e3, defocus=-3e-6, amplitude=0.07,
+             phaseshift=0.0, Bfactor=0.0, scale=1.0):
+        return cls(pixelsize, Cs, voltage, defocus, amplitude, phaseshift, Bfactor, scale)
+
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_UNSUPPORTED = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_DEVICE = 4
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U = ctypes.c_uint
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGS = {
+    "gtomx_version": [],
+    "gtomx_status_string": [_I],
+    "gtomx_last_error": [],
+    "gtomx_debug_fail_alloc_after": [_I],
+    "gtomx_equal_angular_spacing": [_F, _F, _F, _F, _F, _F, _F, _P, _I, ctypes.POINTER(_I)],
+    "gtomx_fibonacci_angles": [_I, _I, _P],
+    "gtomx_projector_matrices": [_P, _U, _F, _P],
+    "gtomx_fft_r2c": [_P, _P, _I, int3, _U, _P],
+    "gtomx_ifft_c2r": [_P, _P, _I, int3, _U, _I, _P],
+    "gtomx_pad": [_P, _P, int3, int3, _F, _U, _P],
+    "gtomx_remap_full2fullfft": [_P, _P, int3, _U, _P],
+    "gtomx_project": [_P, int3, _P, int3, _U, _P, _U, _P],
+    "gtomx_project_angles": [_P, int3, _P, int3, _P, _F, _U, _P],
+    "gtomx_ctf_3d": [_P, _U, int3, _P, _P],
+    "gtomx_complex_multiply_by_vector": [_P, _P, _P, _SZ, _U, _P],
+    "gtomx_complex_multiply_by_conj_vector": [_P, _P, _P, _SZ, _U, _P],
+    "gtomx_update_max_argmax": [_P, _SZ, _U, _I, _P, _P, _P],
+    "gtomx_divide_safe_by_vector": [_P, _P, _P, _SZ, _U, _P],
+    "gtomx_sphere_mask": [_P, _P, int3, _F, _F, _U, _P],
+    "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
+    "gtomx_pick_plan_create": [ctypes.POINTER(_P), int3, int3, _U],
+    "gtomx_pick_plan_destroy": [_P],
+    "gtomx_pick_plan_info": [_P, ctypes.POINTER(_U), ctypes.POINTER(_SZ)],
+    "gtomx_pick_plan_enable_timing": [_P, _I],
+    "gtomx_pick_plan_timings": [_P, _P, _P, _I],
+    "gtomx_pick_subtomograms": [_P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _F, _P, _P, _P, _P],
+    "gtomx_pick_begin": [_P, int3, _U, _P, _P, _P],
+    "gtomx_pick_update": [_P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P],
+    "gtomx_pick_finish": [_P, _P, int3, _U, _F, _P, _P, _P, _P],
+}
+_RESTYPES = {"gtomx_version": ctypes.c_char_p, "gtomx_status_string": ctypes.c_char_p,
+             "gtomx_last_error": ctypes.c_char_p, "gtomx_debug_fail_alloc_after": None}
+
+
+def _load_torch_runtime_first():
+    """PyTorch wheels carry their own HIP runtime under the same soname. Importing
+    torch before libgtomx.so is loaded makes the library bind to that runtime, so
+    the process holds one HIP/HSA runtime and torch's device pointers are valid in
+    the library's calls. Loading libgtomx.so first would map a second runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
+def lib():
+    """Loads libgtomx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gtom_amd.build`")
+        _load_torch_runtime_first()
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def last_error():
+    return lib().gtomx_last_error().decode()
+
+
+def check(status, where):
+    if status != OK:
+        raise GtomxError(status, where, last_error())
+    return status
+
+
+def i3(d):
+    if isinstance(d, int3):
+        return d
+    d = tuple(int(v) for v in d)
+    return int3(*(d + (1,) * (3 - len(d))))
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor")
+    if not t.is_cuda:
+        raise ValueError("tensor must be on the GPU")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _complex_view(t):
+    import torch
+    if t.dtype == torch.complex64:
+        return t
+    raise TypeError("complex tensors must be complex64 (interleaved float pairs)")
+
+
+# ---------------------------------------------------------------------------
+# host utilities
+# ---------------------------------------------------------------------------
+def equal_angular_spacing(phirange, thetarange, psirange, increment):
+    """GetEqualAngularSpacing: (n, 3) float32 array of (rot, tilt, psi) in radians."""
+    L = lib()
+    cnt = _I(0)
+    a = [np.float32(phirange[0]), np.float32(phirange[1]), np.float32(thetarange[0]),
+         np.float32(thetarange[1]), np.float32(psirange[0]), np.float32(psirange[1]),
+         np.float32(increment)]
+    check(L.gtomx_equal_angular_spacing(*a, None, 0, ctypes.byref(cnt)), "equal_angular_spacing")
+    out = np.empty((cnt.value, 3), np.float32)
+    check(L.gtomx_equal_angular_spacing(*a, out.ctypes.data, cnt.value, ctypes.byref(cnt)),
+          "equal_angular_spacing")
+    return out
+
+
+def fibonacci_angles(ndirs, npsi):
+    out = np.empty((ndirs * npsi, 3), np.float32)
+    check(lib().gtomx_fibonacci_angles(ndirs, npsi, out.ctypes.data), "fibonacci_angles")
+    return out
+
+
+def projector_matrices(angles, oversample):
+    a = np.ascontiguousarray(angles, np.float32).reshape(-1, 3)
+    out = np.empty((a.shape[0], 3, 3), np.float32)
+    check(lib().gtomx_projector_matrices(a.ctypes.data, a.shape[0], oversample, out.ctypes.data),
+          "projector_matrices")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# device operations (torch tensors as device memory)
+# ---------------------------------------------------------------------------
+def d_FFTR2C(x, ndims=3, stream=None):
+    """Unnormalised R2C of real (..., nz, ny, nx) float32 -> complex64 (..., nz, ny, nx/2+1)."""
+    import torch
+    nz, ny, nx = x.shape[-3:] if ndims == 3 else ((1,) + tuple(x.shape[-2:]) if ndims == 2 else (1, 1, x.shape[-1]))
+    batch = x.numel() // (nx * ny * nz)
+    out = torch.empty(x.shape[:-1] + (nx // 2 + 1,), dtype=torch.complex64, device=x.device)
+    check(lib().gtomx_fft_r2c(_ptr(x), _ptr(out), ndims, int3(nx, ny, nz), batch, _stream(stream)),
+          "gtomx_fft_r2c")
+    return out
+
+
+def d_IFFTC2R(X, nx, ndims=3, renormalize=True, stream=None):
+    """C2R of complex64 half spectra to width nx (renormalize: multiply by 1/N)."""
+    import torch
+    X = _complex_view(X)
+    shp = X.shape[:-1] + (nx,)
+    nz, ny = (shp[-3], shp[-2]) if ndims == 3 else ((1, shp[-2]) if ndims == 2 else (1, 1))
+    batch = int(np.prod(shp)) // (nx * ny * nz)
+    out = torch.empty(shp, dtype=torch.float32, device=X.device)
+    check(lib().gtomx_ifft_c2r(_ptr(X), _ptr(out), ndims, int3(nx, ny, nz), batch, int(bool(renormalize)),
+                               _stream(stream)), "gtomx_ifft_c2r")
+    return out
+
+
+def d_Pad(x, outdims, value=0.0, stream=None):
+    import torch
+    ind = tuple(x.shape[-3:])
+    batch = x.numel() // int(np.prod(ind))
+    out = torch.empty(x.shape[:-3] + tuple(outdims), dtype=torch.float32, device=x.device)
+    check(lib().gtomx_pad(_ptr(x), _ptr(out), int3(ind[2], ind[1], ind[0]),
+                          int3(outdims[2], outdims[1], outdims[0]), value, batch, _stream(stream)), "gtomx_pad")
+    return out
+
+
+def d_RemapFull2FullFFT(x, stream=None):
+    import torch
+    d = tuple(x.shape[-3:])
+    batch = x.numel() // int(np.prod(d))
+    out = torch.empty_like(x)
+    check(lib().gtomx_remap_full2fullfft(_ptr(x), _ptr(out), int3(d[2], d[1], d[0]), batch, _stream(stream)),
+          "gtomx_remap_full2fullfft")
+    return out
+
+
+def prepare_projector(template, p, stream=None):
+    """Projector half spectrum R2C(ifftshift(pad_centre(template, p))) (complex64, (p, p, p/2+1))."""
+    padded = d_Pad(template, (p, p, p), 0.0, stream)
+    return d_FFTR2C(d_RemapFull2FullFFT(padded, stream), 3, stream)
+
+
+def d_rlnProject(volume_ft, p, n, matrices=None, angles=None, oversample=1.0, rmax=None, stream=None):
+    """Project a (p, p, p/2+1) half spectrum to `batch` (n, n, n/2+1) half spectra."""
+    import torch
+    if matrices is None:
+        matrices = torch.from_numpy(projector_matrices(angles, oversample)).to(volume_ft.device)
+    matrices = matrices.reshape(-1, 9).contiguous().float()
+    batch = matrices.shape[0]
+    out = torch.empty((batch, n, n, n // 2 + 1), dtype=torch.complex64, device=volume_ft.device)
+    check(lib().gtomx_project(_ptr(volume_ft), int3(p, p, p), _ptr(out), int3(n, n, n),
+                              n // 2 if rmax is None else rmax, _ptr(matrices), batch, _stream(stream)),
+          "gtomx_project")
+    return out
+
+
+def ctf_3d(params, n, device="cuda", stream=None):
+    import torch
+    params = list(params) if isinstance(params, (list, tuple)) else [params]
+    arr = (CtfParams * len(params))(*params)
+    out = torch.empty((len(params), n, n, n // 2 + 1), dtype=torch.float32, device=device)
+    check(lib().gtomx_ctf_3d(arr, len(params), int3(n, n, n), _ptr(out), _stream(stream)), "gtomx_ctf_3d")
+    return out
+
+
+def d_UpdateMaxArgmax(corr, best, bestidx, batchoffset, stream=None):
+    batch = corr.shape[0]
+    L = best.numel()
+    check(lib().gtomx_update_max_argmax(_ptr(corr), L, batch, batchoffset, _ptr(best), _ptr(bestidx),
+                                        _stream(stream)), "gtomx_update_max_argmax")
+
+
+def d_LocalStd(vol, radius, with_mean=False, stream=None):
+    import torch
+    nz, ny, nx = vol.shape[-3:]
+    sd = torch.empty_like(vol)
+    mean = torch.empty_like(vol) if with_mean else None
+    check(lib().gtomx_local_std(_ptr(vol), int3(nx, ny, nz), radius, _ptr(sd), _ptr(mean), _stream(stream)),
+          "gtomx_local_std")
+    return (sd, mean) if with_mean else sd
+
+
+class PickPlan:
+    """Owns a gtomx_pick_plan (scratch for d_PickSubTomograms on n^3 volumes)."""
+
+    def __init__(self, p, n, batch=0):
+        self.p, self.n = p, n
+        h = _P()
+        check(lib().gtomx_pick_plan_create(ctypes.byref(h), int3(p, p, p), int3(n, n, n), batch),
+              "gtomx_pick_plan_create")
+        self.handle = h
+        b, nbytes = _U(0), _SZ(0)
+        check(lib().gtomx_pick_plan_info(h, ctypes.byref(b), ctypes.byref(nbytes)), "gtomx_pick_plan_info")
+        self.batch, self.device_bytes = b.value, nbytes.value
+
+    def close(self):
+        if self.handle:
+            lib().gtomx_pick_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def enable_timing(self, on=True):
+        check(lib().gtomx_pick_plan_enable_timing(self.handle, int(bool(on))), "enable_timing")
+
+    def timings(self, reset=True):
+        ms = (ctypes.c_double * 3)()
+        cnt = (ctypes.c_ulonglong * 3)()
+        check(lib().gtomx_pick_plan_timings(self.handle, ms, cnt, int(bool(reset))), "timings")
+        return list(ms), list(cnt)
+
+
+def _angles_arr(angles):
+    a = np.ascontiguousarray(angles, np.float32).reshape(-1, 3)
+    return a
+
+
+def d_PickSubTomograms(plan, projector_ft, oversample, exp_ft, ctf, angles, maskradius,
+                       ctf_volume_stride=0, best=None, bestidx=None, bestangle=None, stream=None):
+    """Exhaustive template match; exp_ft is (nvol, n, n, n/2+1) complex64.
+
+    Returns (best, bestidx[, bestangle]) with best normalised by the local std.
+    """
+    import torch
+    n, p = plan.n, plan.p
+    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    dev = exp_ft.device
+    if best is None:
+        best = torch.empty((nvol, n, n, n), dtype=torch.float32, device=dev)
+    if bestidx is None:
+        bestidx = torch.empty((nvol, n, n, n), dtype=torch.int32, device=dev)
+    a = _angles_arr(angles)
+    check(lib().gtomx_pick_subtomograms(plan.handle, _ptr(projector_ft), oversample, int3(p, p, p), _ptr(exp_ft),
+                                        _ptr(ctf), ctf_volume_stride, int3(n, n, n), nvol, a.ctypes.data,
+                                        a.shape[0], maskradius, _ptr(best), _ptr(bestidx), _ptr(bestangle),
+                                        _stream(stream)), "gtomx_pick_subtomograms")
+    return best, bestidx
+
+
+def pick_begin(plan, nvol, best, bestidx, stream=None):
+    n = plan.n
+    check(lib().gtomx_pick_begin(plan.handle, int3(n, n, n), nvol, _ptr(best), _ptr(bestidx), _stream(stream)),
+          "gtomx_pick_begin")
+
+
+def pick_update(plan, projector_ft, oversample, exp_ft, ctf, angles, angle_offset, best, bestidx,
+                ctf_volume_stride=0, stream=None):
+    n, p = plan.n, plan.p
+    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    a = _angles_arr(angles)
+    check(lib().gtomx_pick_update(plan.handle, _ptr(projector_ft), oversample, int3(p, p, p), _ptr(exp_ft),
+                                  _ptr(ctf), ctf_volume_stride, int3(n, n, n), nvol, a.ctypes.data, a.shape[0],
+                                  angle_offset, _ptr(best), _ptr(bestidx), _stream(stream)), "gtomx_pick_update")
+
+
+def pick_finish(plan, exp_ft, maskradius, best, bestidx, bestangle=None, stream=None):
+    n = plan.n
+    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    check(lib().gtomx_pick_finish(plan.handle, _ptr(exp_ft), int3(n, n, n), nvol, maskradius, _ptr(best),
+                                  _ptr(bestidx), _ptr(bestangle), _stream(stream)), "gtomx_pick_finish")

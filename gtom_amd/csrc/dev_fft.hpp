@@ -1,0 +1,185 @@
+// Device-side FFT building blocks for gfx950 (wave64).
+//
+// A length-N transform is split as N = R1 * R2. A thread holds R1 points
+// (stride R2 apart), runs a register DFT-R1, applies the W_N^{j*k1} twiddles,
+// exchanges through LDS, and finishes with R1/R2 register DFT-R2s. With R2 = 1
+// the whole transform stays in one thread's registers. This is the Stockham
+// two-step (four-step) factorisation; no reference source exists for the FFT
+// arithmetic (the reference calls closed cuFFT, src/FFT/FFT.cu:17-42).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gx {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (DIR * i): DIR = -1 forward (multiply by -i), +1 inverse (multiply by +i)
+template <int DIR> __device__ __forceinline__ float2 mul_pj(float2 a) {
+  return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// Twiddle table: tw[m] = exp(-2 pi i m / TW_LEN) evaluated in extended
+// precision on the host and rounded to float.
+constexpr int TW_LEN = 4096;
+template <int DIR, int N> __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) {
+  static_assert(TW_LEN % N == 0, "twiddle table too short");
+  float2 w = tw[(m & (N - 1)) * (TW_LEN / N)];
+  return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+// ---- register DFTs, natural order in and out, unnormalised ----
+template <int DIR> __device__ __forceinline__ void dft2(float2& a0, float2& a1) {
+  float2 t = a0;
+  a0 = cadd(t, a1);
+  a1 = csub(t, a1);
+}
+template <int DIR> __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_pj<DIR>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+template <int DIR> __device__ __forceinline__ void dft8(float2* a) {
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  constexpr float h = 0.70710678118654752440f;  // sqrt(1/2)
+  o1 = cmul(o1, make_float2(h, DIR * h));
+  o2 = mul_pj<DIR>(o2);
+  o3 = cmul(o3, make_float2(-h, DIR * h));
+  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+  a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+  a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+  a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+}
+template <int DIR> __device__ __forceinline__ void dft16(float2* a) {
+  float2 e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    e[i] = a[2 * i];
+    o[i] = a[2 * i + 1];
+  }
+  dft8<DIR>(e);
+  dft8<DIR>(o);
+  // W16^k = cos(pi k/8) + DIR i sin(pi k/8)
+  constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+  constexpr float h = 0.70710678118654752440f;
+  o[1] = cmul(o[1], make_float2(c1, DIR * s1));
+  o[2] = cmul(o[2], make_float2(h, DIR * h));
+  o[3] = cmul(o[3], make_float2(s1, DIR * c1));
+  o[4] = mul_pj<DIR>(o[4]);
+  o[5] = cmul(o[5], make_float2(-s1, DIR * c1));
+  o[6] = cmul(o[6], make_float2(-h, DIR * h));
+  o[7] = cmul(o[7], make_float2(-c1, DIR * s1));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = cadd(e[k], o[k]);
+    a[k + 8] = csub(e[k], o[k]);
+  }
+}
+template <int R, int DIR> __device__ __forceinline__ void dft(float2* a) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    dft2<DIR>(a[0], a[1]);
+  } else if constexpr (R == 4) {
+    dft4<DIR>(a[0], a[1], a[2], a[3]);
+  } else if constexpr (R == 8) {
+    dft8<DIR>(a);
+  } else if constexpr (R == 16) {
+    dft16<DIR>(a);
+  } else {
+    static_assert(R == 16, "unsupported radix");
+  }
+}
+
+// ---- length factorisations ----
+// Column transforms (strided axes): TX adjacent columns per workgroup, R2
+// threads per column, R1 points per thread.
+template <int N> struct ColShape;
+template <> struct ColShape<8>   { static constexpr int R1 = 8,  R2 = 1,  TX = 64; };
+template <> struct ColShape<16>  { static constexpr int R1 = 16, R2 = 1,  TX = 64; };
+template <> struct ColShape<32>  { static constexpr int R1 = 8,  R2 = 4,  TX = 16; };
+template <> struct ColShape<64>  { static constexpr int R1 = 8,  R2 = 8,  TX = 16; };
+template <> struct ColShape<128> { static constexpr int R1 = 16, R2 = 8,  TX = 16; };
+template <> struct ColShape<256> { static constexpr int R1 = 16, R2 = 16, TX = 16; };
+
+// Row transforms (contiguous axis, complex length NC): R2 threads per row,
+// R1 points per thread.
+template <int NC> struct RowShape;
+template <> struct RowShape<4>   { static constexpr int R1 = 4,  R2 = 1; };
+template <> struct RowShape<8>   { static constexpr int R1 = 8,  R2 = 1; };
+template <> struct RowShape<16>  { static constexpr int R1 = 16, R2 = 1; };
+template <> struct RowShape<32>  { static constexpr int R1 = 8,  R2 = 4; };
+template <> struct RowShape<64>  { static constexpr int R1 = 8,  R2 = 8; };
+template <> struct RowShape<128> { static constexpr int R1 = 16, R2 = 8; };
+
+// Index of the k-th output held by thread j after fft_two_step:
+// k = (j*(R1/R2) + u) + R1*k2 for slot u*R2 + k2.
+template <int R1, int R2> __device__ __forceinline__ int out_index(int j, int slot) {
+  if constexpr (R2 == 1) {
+    return slot;
+  } else {
+    int u = slot / R2, k2 = slot % R2;
+    return (j * (R1 / R2) + u) + R1 * k2;
+  }
+}
+
+// Two-step transform of length N = R1*R2 for one thread of a group of R2
+// threads (index j) sharing an LDS exchange area. On entry v[m] holds input
+// point j + R2*m; on exit v[slot] holds output point out_index(j, slot).
+// xch(k1, jj) returns the LDS address used for the exchange of stage-1
+// output k1 of thread jj; it must be private to the group. The caller
+// synchronises the workgroup before the call if xch's memory was in use.
+template <int N, int R1, int R2, int DIR, class Xch>
+__device__ __forceinline__ void fft_two_step(float2 (&v)[R1], int j, const float2* __restrict__ tw, Xch xch) {
+  static_assert(R1 * R2 == N, "bad factorisation");
+  dft<R1, DIR>(v);
+  if constexpr (R2 > 1) {
+    static_assert(R1 % R2 == 0, "R1 must be a multiple of R2");
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], twiddle<DIR, N>(tw, j * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) *xch(k1, j) = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < R1 / R2; ++u) {
+      float2 z[R2];
+      int k1 = j * (R1 / R2) + u;
+#pragma unroll
+      for (int jj = 0; jj < R2; ++jj) z[jj] = *xch(k1, jj);
+      dft<R2, DIR>(z);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) v[u * R2 + k2] = z[k2];
+    }
+  }
+}
+
+// Workgroup sum of one float per thread (all threads must call). Returns the
+// total in thread 0. red must hold one float per wave.
+__device__ __forceinline__ float block_sum(float v, float* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  float s = 0;
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += red[w];
+  }
+  __syncthreads();
+  return s;
+}
+
+}  // namespace gx

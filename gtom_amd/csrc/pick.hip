@@ -1,0 +1,593 @@
+// The exhaustive template-matching search, d_PickSubTomograms
+// (src/Correlation/SubTomograms.cu:19-125), as three fused passes per batch
+// of B orientations over one n^3 volume:
+//
+//  pass 1 (k_pick_project_z): for a tile of TX x-columns at one y and all z:
+//     R = project(projector, M_o) * ctf        (src/Relion/Project.cu:89-177,
+//                                               src/Generics/Arithmetics.cu:275-286)
+//     R' = Herm(R) on the x = 0 and x = n/2 planes, DC = 0, and the partial
+//     sum of |R'|^2 for the unit-RMS normalisation; this is the Fourier form of
+//     C2R -> mean0/std1 -> R2C -> /RMS (src/Correlation/SubTomograms.cu:66-70,
+//     src/ImageManipulation/Norm.cu:242-284, 476-509)
+//     Q = E_v * conj(R')                       (src/Correlation/SubTomograms.cu:127-136)
+//     inverse DFT along z, written to the batch buffer G[o].
+//     E_v and the CTF are read once per batch and kept in registers.
+//  pass 2 (k_pick_y): inverse DFT along y in place; one workgroup per
+//     orientation also reduces the partial sums to the scale 1/RMS.
+//  pass 3 (k_pick_x_max): C2R along x (real part of the Hermitian-completed
+//     inverse), times the scale, then the running max/argmax with strict '>'
+//     in increasing orientation index (src/Correlation/SubTomograms.cu:138-158)
+//     for all B orientations, best/argmax held in registers.
+// After the last orientation, pick_finish divides by the local standard
+// deviation of C2R(E_v) (src/Correlation/SubTomograms.cu:103-118).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dev_fft.hpp"
+#include "dev_project.hpp"
+#include "runtime.hpp"
+
+namespace gx {
+
+// ---------------------------------------------------------------------------
+// pass 1
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_project_z(
+    Projector P, const float* __restrict__ mats, int nb, const float2* __restrict__ E,
+    const float* __restrict__ ctf, float2* __restrict__ G, float* __restrict__ partial,
+    const float2* __restrict__ tw) {
+  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
+  constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
+  constexpr size_t H = (size_t)NH * N * N;
+  constexpr int NT = TX * R2;
+  __shared__ float2 xch[R2 > 1 ? TX * N : 1];
+  __shared__ float red[(NT + 63) / 64];
+
+  const int t = threadIdx.x, c = t % TX, j = t / TX;
+  const int ix = blockIdx.x * TX + c;
+  const int iy = blockIdx.y;
+  const bool valid = ix < NH;
+  const int y = signed_freq(iy, N);
+  const bool herm = valid && (ix == 0 || ix == HALF);
+  const int iyp = (N - iy) & (N - 1);
+  const int yp = signed_freq(iyp, N);
+  const int nwg = gridDim.x * gridDim.y;
+  const int wg = blockIdx.y * gridDim.x + blockIdx.x;
+
+  float2 e[R1];
+  float cf[R1], cfp[R1];
+#pragma unroll
+  for (int m = 0; m < R1; ++m) {
+    const int iz = j + R2 * m;
+    const size_t idx = ((size_t)iz * N + iy) * NH + ix;
+    e[m] = valid ? E[idx] : make_float2(0.f, 0.f);
+    cf[m] = valid ? (ctf ? ctf[idx] : 1.f) : 0.f;
+    cfp[m] = 0.f;
+    if (herm) {
+      const int izp = (N - iz) & (N - 1);
+      cfp[m] = ctf ? ctf[((size_t)izp * N + iyp) * NH + ix] : 1.f;
+    }
+  }
+
+  for (int o = 0; o < nb; ++o) {
+    float M[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+    float acc = 0.f;
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const int iz = j + R2 * m;
+      const int z = signed_freq(iz, N);
+      float2 r = make_float2(0.f, 0.f);
+      if (valid && ix * ix + y * y + z * z <= RMAX2) r = cscale(project_sample(P, M, ix, y, z), cf[m]);
+      if (herm) {
+        const int izp = (N - iz) & (N - 1);
+        const int zp = signed_freq(izp, N);
+        float2 rp = make_float2(0.f, 0.f);
+        if (ix * ix + yp * yp + zp * zp <= RMAX2) rp = cscale(project_sample(P, M, ix, yp, zp), cfp[m]);
+        r = make_float2(0.5f * (r.x + rp.x), 0.5f * (r.y - rp.y));
+        if (ix == 0 && iy == 0 && iz == 0) r = make_float2(0.f, 0.f);
+      }
+      acc += r.x * r.x + r.y * r.y;
+      v[m] = cmulc(e[m], r);
+    }
+    __syncthreads();
+    fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
+    if (valid) {
+      float2* Go = G + (size_t)o * H;
+#pragma unroll
+      for (int s = 0; s < R1; ++s) Go[((size_t)out_index<R1, R2>(j, s) * N + iy) * NH + ix] = v[s];
+    }
+    const float tot = block_sum(acc, red);
+    if (t == 0) partial[(size_t)o * nwg + wg] = tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
+    float2* __restrict__ G, const float* __restrict__ partial, int nwg1, float* __restrict__ scale,
+    const float2* __restrict__ tw) {
+  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
+  constexpr int NH = N / 2 + 1;
+  constexpr size_t H = (size_t)NH * N * N;
+  constexpr int NT = TX * R2;
+  __shared__ float2 xch[R2 > 1 ? TX * N : 1];
+  __shared__ double dred[NT];
+
+  const int t = threadIdx.x, c = t % TX, j = t / TX;
+  const int ix = blockIdx.x * TX + c;
+  const int z = blockIdx.y, o = blockIdx.z;
+  const bool valid = ix < NH;
+  float2* base = G + (size_t)o * H + (size_t)z * N * NH + ix;
+  float2 v[R1];
+#pragma unroll
+  for (int m = 0; m < R1; ++m) v[m] = valid ? base[(size_t)(j + R2 * m) * NH] : make_float2(0.f, 0.f);
+  fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
+  if (valid) {
+#pragma unroll
+    for (int s = 0; s < R1; ++s) base[(size_t)out_index<R1, R2>(j, s) * NH] = v[s];
+  }
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    // unit-RMS scale over the H half-spectrum bins (src/ImageManipulation/Norm.cu:476-509)
+    double s = 0;
+    for (int i = t; i < nwg1; i += NT) s += (double)partial[(size_t)o * nwg1 + i];
+    dred[t] = s;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (t < w) dred[t] += dred[t + w];
+      __syncthreads();
+    }
+    if (t == 0) {
+      double rms = sqrt(dred[0] / (double)H);
+      scale[o] = (float)(1.0 / fmax(rms, 1e-20));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 3
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G, int nb, int angle0,
+                                                    const float* __restrict__ scale, float* __restrict__ best,
+                                                    int32_t* __restrict__ bidx, const float2* __restrict__ tw) {
+  constexpr int NC = N / 2, NH = N / 2 + 1;
+  constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
+  constexpr int GS = R2 * (R1 + 1);
+  constexpr size_t H = (size_t)NH * N * N;
+  __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
+  const int t = threadIdx.x, r = t / R2, j = t % R2;
+  const int row = blockIdx.x * RW + r;
+  const bool valid = row < N * N;
+
+  float bv[2 * R1];
+  int32_t bi[2 * R1];
+  float2* bp = reinterpret_cast<float2*>(best + (size_t)row * N);
+  int2* ip = reinterpret_cast<int2*>(bidx + (size_t)row * N);
+#pragma unroll
+  for (int s = 0; s < R1; ++s) {
+    const int m = out_index<R1, R2>(j, s);
+    float2 b = valid ? bp[m] : make_float2(0.f, 0.f);
+    int2 a = valid ? ip[m] : make_int2(0, 0);
+    bv[2 * s] = b.x;
+    bv[2 * s + 1] = b.y;
+    bi[2 * s] = a.x;
+    bi[2 * s + 1] = a.y;
+  }
+
+  for (int o = 0; o < nb; ++o) {
+    const float sc = scale[o];
+    const float2* X = G + (size_t)o * H + (size_t)row * NH;
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const int k = j + R2 * m;
+      float2 a = valid ? X[k] : make_float2(0.f, 0.f);
+      float2 b = valid ? X[NC - k] : make_float2(0.f, 0.f);
+      if (k == 0) {
+        a.y = 0.f;
+        b.y = 0.f;
+      }
+      const float2 bc = cconj(b);
+      v[m] = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle<1, N>(tw, k), csub(a, bc))));
+    }
+    __syncthreads();
+    fft_two_step<NC, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[r * GS + jj * (R1 + 1) + k1]; });
+    const int32_t oi = angle0 + o;
+#pragma unroll
+    for (int s = 0; s < R1; ++s) {
+      const float c0 = v[s].x * sc, c1 = v[s].y * sc;
+      if (c0 > bv[2 * s]) {
+        bv[2 * s] = c0;
+        bi[2 * s] = oi;
+      }
+      if (c1 > bv[2 * s + 1]) {
+        bv[2 * s + 1] = c1;
+        bi[2 * s + 1] = oi;
+      }
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int s = 0; s < R1; ++s) {
+      const int m = out_index<R1, R2>(j, s);
+      bp[m] = make_float2(bv[2 * s], bv[2 * s + 1]);
+      ip[m] = make_int2(bi[2 * s], bi[2 * s + 1]);
+    }
+  }
+}
+
+}  // namespace gx
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+struct gtomx_pick_plan {
+  int n = 0, p = 0;
+  unsigned B = 0;
+  int device = 0;
+  size_t H = 0, V = 0;
+  int ntiles = 0, nwg1 = 0;
+  float2* G = nullptr;
+  float* partial = nullptr;
+  float* scale = nullptr;
+  float* mats = nullptr;
+  size_t mats_cap = 0;   // orientations
+  float* h_mats = nullptr;  // pinned staging
+  size_t h_mats_cap = 0;
+  hipEvent_t mats_ev = nullptr;
+  bool mats_ev_recorded = false;
+  void* ls_mem = nullptr;
+  gx::LocalStdScratch ls{};
+  float* ebuf = nullptr;
+  size_t device_bytes = 0;
+  // kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Pending { int stage; size_t a, b; };
+  std::vector<Pending> pending;
+  double ms[3] = {0, 0, 0};
+  unsigned long long launches[3] = {0, 0, 0};
+};
+
+namespace gx {
+namespace {
+
+void plan_release(gtomx_pick_plan* pl) {
+  if (!pl) return;
+  dev_free(pl->G);
+  dev_free(pl->partial);
+  dev_free(pl->scale);
+  dev_free(pl->mats);
+  dev_free(pl->ls_mem);
+  dev_free(pl->ebuf);
+  if (pl->h_mats) (void)hipHostFree(pl->h_mats);
+  if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
+  for (hipEvent_t e : pl->ev_pool) (void)hipEventDestroy(e);
+  delete pl;
+}
+
+template <int N> constexpr int col_tiles() { return (N / 2 + 1 + ColShape<N>::TX - 1) / ColShape<N>::TX; }
+
+int tiles_for(int n) {
+  switch (n) {
+    case 8: return col_tiles<8>();
+    case 16: return col_tiles<16>();
+    case 32: return col_tiles<32>();
+    case 64: return col_tiles<64>();
+    case 128: return col_tiles<128>();
+    case 256: return col_tiles<256>();
+  }
+  return 0;
+}
+
+int event_at(gtomx_pick_plan* pl, size_t i, hipEvent_t* ev) {
+  while (pl->ev_pool.size() <= i) {
+    hipEvent_t e;
+    GX_HIP(hipEventCreate(&e));
+    pl->ev_pool.push_back(e);
+  }
+  *ev = pl->ev_pool[i];
+  return GTOMX_OK;
+}
+
+// Runs `launch` bracketed by timing events when the plan's timing is enabled.
+template <class F> int timed(gtomx_pick_plan* pl, int stage, hipStream_t s, F launch) {
+  if (!pl->timing) return launch();
+  hipEvent_t a, b;
+  size_t ia = pl->ev_used, ib = pl->ev_used + 1;
+  GX_TRY(event_at(pl, ia, &a));
+  GX_TRY(event_at(pl, ib, &b));
+  pl->ev_used += 2;
+  GX_HIP(hipEventRecord(a, s));
+  GX_TRY(launch());
+  GX_HIP(hipEventRecord(b, s));
+  pl->pending.push_back({stage, ia, ib});
+  return GTOMX_OK;
+}
+
+template <int N>
+int run_batch(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb, const float2* E,
+              const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
+  constexpr int TX = ColShape<N>::TX, R2 = ColShape<N>::R2;
+  constexpr int RW = 256 / RowShape<N / 2>::R2;
+  const int ntiles = col_tiles<N>();
+  GX_TRY(timed(pl, 0, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick_project_z<N>, dim3(ntiles, N), dim3(TX * R2), 0, s, P, mats, nb, E, ctf, pl->G,
+                       pl->partial, tw);
+    GX_LAUNCHED("k_pick_project_z");
+    return GTOMX_OK;
+  }));
+  GX_TRY(timed(pl, 1, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick_y<N>, dim3(ntiles, N, nb), dim3(TX * R2), 0, s, pl->G, pl->partial, ntiles * N,
+                       pl->scale, tw);
+    GX_LAUNCHED("k_pick_y");
+    return GTOMX_OK;
+  }));
+  GX_TRY(timed(pl, 2, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick_x_max<N>, dim3((N * N + RW - 1) / RW), dim3(256), 0, s, pl->G, nb, angle0, pl->scale,
+                       best, idx, tw);
+    GX_LAUNCHED("k_pick_x_max");
+    return GTOMX_OK;
+  }));
+  return GTOMX_OK;
+}
+
+int run_batch_n(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb, const float2* E,
+                const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
+  switch (pl->n) {
+    case 8: return run_batch<8>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    case 16: return run_batch<16>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    case 32: return run_batch<32>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    case 64: return run_batch<64>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    case 128: return run_batch<128>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    case 256: return run_batch<256>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+  }
+  return fail(GTOMX_ERR_UNSUPPORTED, "volume side not supported");
+}
+
+int check_plan_dims(const gtomx_pick_plan* pl, gtomx_int3 dimsvolume, unsigned nvolumes) {
+  if (!pl) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (dimsvolume.x != pl->n || dimsvolume.y != pl->n || dimsvolume.z != pl->n)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "dimsvolume does not match the plan");
+  if (nvolumes == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "nvolumes must be positive");
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  if (dev != pl->device) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan was created on another device");
+  return GTOMX_OK;
+}
+
+int upload_matrices(gtomx_pick_plan* pl, const gtomx_float3* h_angles, unsigned nangles, float oversample,
+                    hipStream_t s) {
+  if (pl->mats_ev_recorded) {
+    GX_HIP(hipEventSynchronize(pl->mats_ev));
+    pl->mats_ev_recorded = false;
+  }
+  if (pl->mats_cap < nangles) {
+    dev_free(pl->mats);
+    pl->mats = nullptr;
+    pl->device_bytes -= pl->mats_cap * 9 * sizeof(float);
+    pl->mats_cap = 0;
+    void* d = nullptr;
+    GX_TRY(dev_alloc(&d, (size_t)nangles * 9 * sizeof(float)));
+    pl->mats = static_cast<float*>(d);
+    pl->mats_cap = nangles;
+    pl->device_bytes += pl->mats_cap * 9 * sizeof(float);
+  }
+  if (pl->h_mats_cap < nangles) {
+    if (pl->h_mats) (void)hipHostFree(pl->h_mats);
+    pl->h_mats = nullptr;
+    pl->h_mats_cap = 0;
+    void* h = nullptr;
+    hipError_t e = hipHostMalloc(&h, (size_t)nangles * 9 * sizeof(float), hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(matrices)");
+    pl->h_mats = static_cast<float*>(h);
+    pl->h_mats_cap = nangles;
+  }
+  GX_TRY(gtomx_projector_matrices(h_angles, nangles, oversample, pl->h_mats));
+  GX_HIP(hipMemcpyAsync(pl->mats, pl->h_mats, (size_t)nangles * 9 * sizeof(float), hipMemcpyHostToDevice, s));
+  GX_HIP(hipEventRecord(pl->mats_ev, s));
+  pl->mats_ev_recorded = true;
+  return GTOMX_OK;
+}
+
+}  // namespace
+}  // namespace gx
+
+using namespace gx;
+
+extern "C" {
+
+int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, unsigned batch) {
+  if (!out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan pointer is NULL");
+  *out = nullptr;
+  if (dv.x != dv.y || dv.x != dv.z) return fail(GTOMX_ERR_UNSUPPORTED, "volumes must be cubes");
+  if (!fft_len_ok(dv.x)) return fail(GTOMX_ERR_UNSUPPORTED, "volume side must be a power of two in [8, 256]");
+  if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 1024)
+    return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 1024");
+  if (batch > 256) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 256");
+  auto* pl = new gtomx_pick_plan();
+  pl->n = dv.x;
+  pl->p = dp.x;
+  pl->B = batch ? batch : (dv.x >= 128 ? 8u : 16u);
+  pl->H = elements_ft(dv);
+  pl->V = elements(dv);
+  pl->ntiles = tiles_for(pl->n);
+  pl->nwg1 = pl->ntiles * pl->n;
+  int s = GTOMX_OK;
+  hipError_t he = hipGetDevice(&pl->device);
+  if (he != hipSuccess) {
+    plan_release(pl);
+    return hip_fail(he, "hipGetDevice");
+  }
+  void* d = nullptr;
+  auto alloc = [&](void** dst, size_t bytes) {
+    if (s != GTOMX_OK) return;
+    s = dev_alloc(dst, bytes);
+    if (s == GTOMX_OK) pl->device_bytes += bytes;
+  };
+  alloc(&d, (size_t)pl->B * pl->H * sizeof(float2));
+  pl->G = static_cast<float2*>(d);
+  d = nullptr;
+  alloc(&d, (size_t)pl->B * pl->nwg1 * sizeof(float));
+  pl->partial = static_cast<float*>(d);
+  d = nullptr;
+  alloc(&d, (size_t)pl->B * sizeof(float));
+  pl->scale = static_cast<float*>(d);
+  d = nullptr;
+  alloc(&pl->ls_mem, local_std_scratch_bytes(dv));
+  alloc(&d, pl->V * sizeof(float));
+  pl->ebuf = static_cast<float*>(d);
+  if (s != GTOMX_OK) {
+    std::string msg = gtomx_last_error();
+    plan_release(pl);
+    return fail(s, msg);
+  }
+  pl->ls = local_std_carve(pl->ls_mem, dv);
+  he = hipEventCreateWithFlags(&pl->mats_ev, hipEventDisableTiming);
+  if (he != hipSuccess) {
+    plan_release(pl);
+    return hip_fail(he, "hipEventCreate");
+  }
+  const float2* tw = nullptr;
+  s = twiddles(&tw);
+  if (s != GTOMX_OK) {
+    std::string msg = gtomx_last_error();
+    plan_release(pl);
+    return fail(s, msg);
+  }
+  *out = pl;
+  return GTOMX_OK;
+}
+
+int gtomx_pick_plan_destroy(gtomx_pick_plan* plan) {
+  if (!plan) return GTOMX_OK;
+  // Buffers may still be in use by enqueued work.
+  (void)hipDeviceSynchronize();
+  plan_release(plan);
+  return GTOMX_OK;
+}
+
+int gtomx_pick_plan_info(const gtomx_pick_plan* plan, unsigned* batch, size_t* device_bytes) {
+  if (!plan) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (batch) *batch = plan->B;
+  if (device_bytes) *device_bytes = plan->device_bytes;
+  return GTOMX_OK;
+}
+
+int gtomx_pick_plan_enable_timing(gtomx_pick_plan* plan, int enable) {
+  if (!plan) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
+  plan->timing = enable != 0;
+  return GTOMX_OK;
+}
+
+int gtomx_pick_plan_timings(gtomx_pick_plan* plan, double* ms3, unsigned long long* launches3, int reset) {
+  if (!plan) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
+  for (const auto& pd : plan->pending) {
+    GX_HIP(hipEventSynchronize(plan->ev_pool[pd.b]));
+    float ms = 0.f;
+    GX_HIP(hipEventElapsedTime(&ms, plan->ev_pool[pd.a], plan->ev_pool[pd.b]));
+    plan->ms[pd.stage] += ms;
+    plan->launches[pd.stage] += 1;
+  }
+  plan->pending.clear();
+  plan->ev_used = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (ms3) ms3[k] = plan->ms[k];
+    if (launches3) launches3[k] = plan->launches[k];
+    if (reset) {
+      plan->ms[k] = 0;
+      plan->launches[k] = 0;
+    }
+  }
+  return GTOMX_OK;
+}
+
+int gtomx_pick_begin(gtomx_pick_plan* plan, gtomx_int3 dimsvolume, unsigned nvolumes, float* d_best,
+                     int32_t* d_idx, void* stream) {
+  GX_TRY(check_plan_dims(plan, dimsvolume, nvolumes));
+  if (!d_best || !d_idx) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL output buffer");
+  hipStream_t s = as_stream(stream);
+  // d_ValueFill best = -1e30, angle = 0 (src/Correlation/SubTomograms.cu:41-42)
+  GX_TRY(fill_f32(d_best, plan->V * nvolumes, -1e30f, s));
+  GX_TRY(fill_i32(d_idx, plan->V * nvolumes, 0, s));
+  return GTOMX_OK;
+}
+
+int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, float oversample,
+                      gtomx_int3 dimsprojector, const gtomx_complex* d_expft, const float* d_ctf,
+                      size_t ctf_volume_stride, gtomx_int3 dimsvolume, unsigned nvolumes,
+                      const gtomx_float3* h_angles, unsigned nangles, unsigned angle_offset, float* d_best,
+                      int32_t* d_idx, void* stream) {
+  GX_TRY(check_plan_dims(plan, dimsvolume, nvolumes));
+  if (dimsprojector.x != plan->p || dimsprojector.y != plan->p || dimsprojector.z != plan->p)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "dimsprojector does not match the plan");
+  if (!d_projectorft || !d_expft || !d_best || !d_idx)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL device buffer");
+  if (nangles == 0) return GTOMX_OK;
+  if (!h_angles) return fail(GTOMX_ERR_INVALID_ARGUMENT, "h_angles is NULL");
+  if ((unsigned long long)angle_offset + nangles > 0x7fffffffULL)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "orientation index exceeds int32");
+  if (!(oversample > 0.f) || !std::isfinite(oversample))
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "projectoroversample must be positive");
+  hipStream_t s = as_stream(stream);
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  GX_TRY(upload_matrices(plan, h_angles, nangles, oversample, s));
+  Projector P{reinterpret_cast<const float2*>(d_projectorft), plan->p, plan->p / 2 + 1,
+              (plan->p / 2 + 1) * plan->p * plan->p - 1};
+  for (unsigned v = 0; v < nvolumes; ++v) {
+    const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
+    const float* cv = d_ctf ? d_ctf + (size_t)v * ctf_volume_stride : nullptr;
+    float* bv = d_best + (size_t)v * plan->V;
+    int32_t* iv = d_idx + (size_t)v * plan->V;
+    for (unsigned b0 = 0; b0 < nangles; b0 += plan->B) {
+      int nb = (int)std::min<unsigned>(plan->B, nangles - b0);
+      GX_TRY(run_batch_n(plan, P, plan->mats + 9 * (size_t)b0, nb, Ev, cv, bv, iv, (int)(angle_offset + b0), tw, s));
+    }
+  }
+  return GTOMX_OK;
+}
+
+int gtomx_pick_finish(gtomx_pick_plan* plan, const gtomx_complex* d_expft, gtomx_int3 dimsvolume,
+                      unsigned nvolumes, float maskradius, float* d_best, int32_t* d_idx, float* d_bestangle,
+                      void* stream) {
+  GX_TRY(check_plan_dims(plan, dimsvolume, nvolumes));
+  if (!d_expft || !d_best || !d_idx) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL device buffer");
+  if (!(maskradius >= 0.f) || !std::isfinite(maskradius))
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "maskradius must be finite and non-negative");
+  hipStream_t s = as_stream(stream);
+  GX_TRY(local_std_prepare(plan->ls, dimsvolume, maskradius, s));
+  for (unsigned v = 0; v < nvolumes; ++v) {
+    const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
+    // e = C2R_u(E_v) (src/Correlation/SubTomograms.cu:105); the batch buffer is the workspace.
+    GX_TRY(fft_c2r(Ev, plan->G, plan->ebuf, dimsvolume, 1, 1.0f, s));
+    GX_TRY(local_std_apply(plan->ls, plan->ebuf, dimsvolume, nullptr, nullptr, d_best + (size_t)v * plan->V,
+                           d_idx + (size_t)v * plan->V, d_bestangle ? d_bestangle + (size_t)v * plan->V : nullptr,
+                           s));
+  }
+  return GTOMX_OK;
+}
+
+int gtomx_pick_subtomograms(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, float oversample,
+                            gtomx_int3 dimsprojector, const gtomx_complex* d_expft, const float* d_ctf,
+                            size_t ctf_volume_stride, gtomx_int3 dimsvolume, unsigned nvolumes,
+                            const gtomx_float3* h_angles, unsigned nangles, float maskradius, float* d_best,
+                            int32_t* d_idx, float* d_bestangle, void* stream) {
+  if (nangles > 0 && !h_angles) return fail(GTOMX_ERR_INVALID_ARGUMENT, "h_angles is NULL");
+  GX_TRY(gtomx_pick_begin(plan, dimsvolume, nvolumes, d_best, d_idx, stream));
+  GX_TRY(gtomx_pick_update(plan, d_projectorft, oversample, dimsprojector, d_expft, d_ctf, ctf_volume_stride,
+                           dimsvolume, nvolumes, h_angles, nangles, 0, d_best, d_idx, stream));
+  GX_TRY(gtomx_pick_finish(plan, d_expft, dimsvolume, nvolumes, maskradius, d_best, d_idx, d_bestangle, stream));
+  return GTOMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// Host-side runtime shared by every entry point: status codes, the calling
+// thread's last error message, checked device allocation (with a test hook
+// that forces allocation failure), and per-device constant tables.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/gtomx.h"
+
+namespace gx {
+
+// Length of the device twiddle table (must equal TW_LEN in dev_fft.hpp).
+constexpr int TW_LEN_HOST = 4096;
+
+// Records a message for gtomx_last_error() and returns the status.
+int fail(int status, const std::string& msg);
+int ok();
+
+// Maps a HIP error to GTOMX_ERR_DEVICE / GTOMX_ERR_OUT_OF_MEMORY with context.
+int hip_fail(hipError_t e, const char* what);
+
+#define GX_HIP(call)                                       \
+  do {                                                     \
+    hipError_t gx_e_ = (call);                             \
+    if (gx_e_ != hipSuccess) return ::gx::hip_fail(gx_e_, #call); \
+  } while (0)
+
+// Checks hipGetLastError after a kernel launch.
+#define GX_LAUNCHED(name)                                  \
+  do {                                                     \
+    hipError_t gx_e_ = hipGetLastError();                  \
+    if (gx_e_ != hipSuccess) return ::gx::hip_fail(gx_e_, name); \
+  } while (0)
+
+#define GX_TRY(expr)              \
+  do {                            \
+    int gx_s_ = (expr);           \
+    if (gx_s_ != GTOMX_OK) return gx_s_; \
+  } while (0)
+
+// Device allocation through the library's allocator (honours the test hook).
+int dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
+
+// Twiddle table exp(-2 pi i m / 4096), m in [0, 4096), for the current device.
+int twiddles(const float2** table);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// Axis extent accepted by the FFT kernels: power of two in [8, 256].
+inline bool fft_len_ok(int n) { return is_pow2(n) && n >= 8 && n <= 256; }
+
+// Validates dims for an ndims-dimensional transform; returns GTOMX_OK or an error.
+int check_fft_dims(int ndims, gtomx_int3 dims);
+
+inline size_t elements(gtomx_int3 d) { return (size_t)d.x * d.y * d.z; }
+inline size_t elements_ft(gtomx_int3 d) { return (size_t)(d.x / 2 + 1) * d.y * d.z; }
+
+// Scratch device buffer that frees itself.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { dev_free(p); }
+  int alloc(size_t b) {
+    dev_free(p);
+    p = nullptr;
+    bytes = 0;
+    int s = dev_alloc(&p, b);
+    if (s == GTOMX_OK) bytes = b;
+    return s;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---- internal host API shared between translation units ----
+// Batched 3D/2D transforms (dims with dims.z == 1 for 2D). `work` must hold
+// elements_ft(dims) * batch complex values for c2r (the input is copied
+// there first so it is left unmodified); r2c needs no workspace.
+int fft_r2c(const float* in, float2* out, gtomx_int3 dims, unsigned batch, hipStream_t s);
+int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 dims, unsigned batch,
+            float scale, hipStream_t s);
+// In-place inverse along z then y of half spectra; used by fft_c2r.
+int fft_c2r_inplace_yz(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
+
+// Element-wise helpers (ops.hip).
+int fill_f32(float* d, size_t n, float v, hipStream_t s);
+int fill_i32(int32_t* d, size_t n, int32_t v, hipStream_t s);
+int sphere_mask_ifftshift(float* out, gtomx_int3 dims, float radius, float sigma, hipStream_t s);
+int sum_f64(const float* d, size_t n, double* d_result, hipStream_t s);
+int mul_conj_inplace(float2* a, const float2* b, size_t n, hipStream_t s);
+int square_f32(const float* in, float* out, size_t n, hipStream_t s);
+// std = sqrt(max(0, M*c2/N - (c1/N)^2)) / M (double), optional mean = c1/(N*M);
+// with best != nullptr also best = |std| > 1e-15 ? best/std : 0 and the float index copy.
+int local_std_combine(const float* c1, const float* c2, const double* d_masksum, size_t n,
+                      float* std_out, float* mean_out, float* best, const int32_t* idx,
+                      float* angle_out, hipStream_t s);
+
+// Local standard deviation chain (ops.hip). The scratch holds the mask FT,
+// two spectra, three real volumes and a few doubles.
+struct LocalStdScratch {
+  float2* maskft;
+  float2* aft;
+  float2* bft;
+  float* rbuf;
+  float* c1;
+  float* c2;
+  double* dsum;  // 257 doubles; the mask sum is dsum[256]
+};
+size_t local_std_scratch_bytes(gtomx_int3 d);
+LocalStdScratch local_std_carve(void* base, gtomx_int3 d);
+int local_std_prepare(const LocalStdScratch& sc, gtomx_int3 d, float radius, hipStream_t s);
+int local_std_apply(const LocalStdScratch& sc, const float* map, gtomx_int3 d, float* std_out, float* mean_out,
+                    float* best, const int32_t* idx, float* angle_out, hipStream_t s);
+
+// Standalone projector launch (ops.hip).
+int project_launch(const float2* vol, int p, float2* out, int n, unsigned rmax, const float* d_mats,
+                   unsigned batch, hipStream_t s);
+
+}  // namespace gx

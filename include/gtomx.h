@@ -1,0 +1,264 @@
+/*
+ * gtomx.h - C ABI of the MI355X (gfx950) implementation of GTOM's FFT
+ * template-matching path.
+ *
+ * Conventions shared by every entry point:
+ *  - All d_ pointers are device pointers (HIP), h_ pointers are host pointers.
+ *    The caller owns every buffer passed in.
+ *  - Real volumes are x-fastest, element (x, y, z) at (z*dims.y + y)*dims.x + x.
+ *  - Half spectra ("ft") hold x in [0, dims.x/2]: element (x, y, z) at
+ *    (z*dims.y + y)*(dims.x/2 + 1) + x, with y and z in FFT order. Complex
+ *    values are interleaved float pairs (gtomx_complex, the layout of float2 /
+ *    tcomplex in the reference, include/Prerequisites.cuh:42).
+ *  - `stream` is a hipStream_t passed as void*; NULL selects the default
+ *    stream. Entry points enqueue work on that stream and return without
+ *    waiting for it unless documented otherwise.
+ *  - Every function returns a gtomx_status; no C++ exception crosses the ABI.
+ *    gtomx_last_error() gives a human-readable message for the calling thread.
+ *  - Supported transform sizes: every axis a power of two in [8, 256]
+ *    (an axis of extent 1 is allowed for 2D and 1D data).
+ *
+ * The reference interface each function replaces is cited as file:line of
+ * the reference tree (dtegunov/gtom).
+ */
+#ifndef GTOMX_H
+#define GTOMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { int x, y, z; } gtomx_int3;       /* int3 */
+typedef struct { float x, y; } gtomx_complex;     /* tcomplex, include/Prerequisites.cuh:42 */
+typedef struct { float x, y, z; } gtomx_float3;   /* tfloat3, include/Prerequisites.cuh:62-73 */
+
+typedef enum {
+  GTOMX_OK = 0,
+  GTOMX_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad dims, zero batch, ... */
+  GTOMX_ERR_UNSUPPORTED = 2,      /* valid request outside the supported sizes */
+  GTOMX_ERR_OUT_OF_MEMORY = 3,    /* device or pinned-host allocation failed */
+  GTOMX_ERR_DEVICE = 4,           /* a HIP runtime call or kernel launch failed */
+  GTOMX_ERR_INTERNAL = 5
+} gtomx_status;
+
+/* Library version and diagnostics (no reference counterpart). */
+const char* gtomx_version(void);
+const char* gtomx_status_string(int status);
+const char* gtomx_last_error(void);
+
+/* Fault-injection hook for host-side tests of the error paths (no reference
+ * counterpart): makes the n-th following device allocation made by this
+ * library fail with GTOMX_ERR_OUT_OF_MEMORY (n = 0 disables). */
+void gtomx_debug_fail_alloc_after(int n);
+
+/* ------------------------------------------------------------------------ */
+/* Angles and rotation matrices (host)                                       */
+/* ------------------------------------------------------------------------ */
+
+/* GetEqualAngularSpacing, src/Helpers/AngularSpacing.cu:46-67 (restated with
+ * the reference's float32 loop counters). Writes min(count, capacity)
+ * (rot, tilt, psi) triples in radians to h_angles (may be NULL) and the total
+ * count to *count. */
+int gtomx_equal_angular_spacing(float phimin, float phimax, float thetamin, float thetamax,
+                                float psimin, float psimax, float increment,
+                                gtomx_float3* h_angles, int capacity, int* count);
+
+/* Fibonacci-sphere directions x in-plane steps (no reference counterpart):
+ * direction i of ndirs has z_i = 1 - (2i+1)/ndirs, phi_i = i*pi*(3 - sqrt 5);
+ * psi_j = j * 2*pi/npsi. Output index i*npsi + j, as ZYZ (rot, tilt, psi).
+ * Computed in double. h_angles must hold ndirs*npsi triples. */
+int gtomx_fibonacci_angles(int ndirs, int npsi, gtomx_float3* h_angles);
+
+/* transpose(Matrix3Euler(angles)) * oversample for each angle, row-major
+ * 3x3 per orientation, computed in double and rounded to float
+ * (src/Relion/Project.cu:12-17, src/Helpers/Euler.cu:39-68). */
+int gtomx_projector_matrices(const gtomx_float3* h_angles, unsigned n, float oversample,
+                             float* h_matrices);
+
+/* ------------------------------------------------------------------------ */
+/* FFT (replaces the cuFFT wrappers)                                         */
+/* ------------------------------------------------------------------------ */
+
+/* d_FFTR2C, include/FFT.cuh:70 / src/FFT/FFT.cu:7-42: unnormalised forward
+ * transform of `batch` real volumes into half spectra. */
+int gtomx_fft_r2c(const float* d_input, gtomx_complex* d_output, int ndims, gtomx_int3 dims,
+                  unsigned batch, void* stream);
+
+/* d_IFFTC2R, include/FFT.cuh:150 / src/FFT/IFFT.cu:8-56: inverse transform of
+ * half spectra; renormalize != 0 multiplies by 1/N. The result is the real
+ * part of the inverse DFT of the Hermitian-completed spectrum. The input is
+ * not modified (the reference's cuFFT call may clobber it). */
+int gtomx_ifft_c2r(const gtomx_complex* d_input, float* d_output, int ndims, gtomx_int3 dims,
+                   unsigned batch, int renormalize, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Projector preparation and projection                                      */
+/* ------------------------------------------------------------------------ */
+
+/* d_Pad with T_PAD_VALUE, include/Generics.cuh:813 / src/Generics/Padding.cu:22-45:
+ * centred copy of `batch` real volumes into outputdims, filling with value. */
+int gtomx_pad(const float* d_input, float* d_output, gtomx_int3 inputdims, gtomx_int3 outputdims,
+              float value, unsigned batch, void* stream);
+
+/* d_RemapFull2FullFFT, include/FFT.cuh:287 / src/FFT/FFTRemap.cu:89-106:
+ * ifftshift of real volumes (d_input != d_output required). */
+int gtomx_remap_full2fullfft(const float* d_input, float* d_output, gtomx_int3 dims,
+                             unsigned batch, void* stream);
+
+/* d_rlnProject (device-array form), include/Relion.cuh:35 /
+ * src/Relion/Project.cu:25-43, kernel src/Relion/Project.cu:89-177: samples
+ * the (p/2+1, p, p) projector half spectrum at M_b * k for every half-spectrum
+ * bin k of a dimsproj volume with |k| <= rmax, for `batch` row-major 3x3
+ * matrices (device memory, 9 floats each). 3D projections only. */
+int gtomx_project(const gtomx_complex* d_volumeft, gtomx_int3 dimsvolume, gtomx_complex* d_proj,
+                  gtomx_int3 dimsproj, unsigned rmax, const float* d_matrices, unsigned batch,
+                  void* stream);
+
+/* d_rlnProject (angles form), include/Relion.cuh:34 / src/Relion/Project.cu:12-23:
+ * matrices from host angles (transpose(Euler) * supersample), rmax = dimsproj.x/2. */
+int gtomx_project_angles(const gtomx_complex* d_volumeft, gtomx_int3 dimsvolume,
+                         gtomx_complex* d_proj, gtomx_int3 dimsproj,
+                         const gtomx_float3* h_angles, float supersample, unsigned batch,
+                         void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* CTF                                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* CTF parameters in SI units, the subset of CTFParams (include/CTF.cuh:11-43)
+ * used by the radially symmetric 3D form. */
+typedef struct {
+  float pixelsize;   /* m (default 1e-10) */
+  float Cs;          /* m (2e-3) */
+  float voltage;     /* V (300e3) */
+  float defocus;     /* m, negative = underfocus (-3e-6) */
+  float amplitude;   /* amplitude contrast (0.07) */
+  float phaseshift;  /* rad (0) */
+  float Bfactor;     /* m^2 (0) */
+  float scale;       /* (1) */
+} gtomx_ctf_params;
+
+/* 3D CTF on the half spectrum of a cube of side n (dims.x = dims.y = dims.z),
+ * one array of (n/2+1)*n*n floats per parameter set: d_GetCTF<false,false>
+ * (include/CTF.cuh:195-223) with CTFParamsLean (include/CTF.cuh:100-134) and
+ * the 3D frequency mapping of src/CTF/Wiener.cu:99-110. Evaluated in double. */
+int gtomx_ctf_3d(const gtomx_ctf_params* h_params, unsigned batch, gtomx_int3 dims, float* d_ctf,
+                 void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Element-wise pieces of the path (drop-ins for the unfused reference)      */
+/* ------------------------------------------------------------------------ */
+
+/* d_ComplexMultiplyByVector (real multiplicators), include/Generics.cuh:71 /
+ * src/Generics/Arithmetics.cu:192-197: out[b][i] = in[b][i] * v[i]. */
+int gtomx_complex_multiply_by_vector(const gtomx_complex* d_input, const float* d_multiplicators,
+                                     gtomx_complex* d_output, size_t elements, unsigned batch,
+                                     void* stream);
+
+/* d_ComplexMultiplyByConjVector, include/Generics.cuh:150 /
+ * src/Generics/Arithmetics.cu:220-225: out[b][i] = in[b][i] * conj(v[i]). */
+int gtomx_complex_multiply_by_conj_vector(const gtomx_complex* d_input,
+                                          const gtomx_complex* d_multiplicators,
+                                          gtomx_complex* d_output, size_t elements,
+                                          unsigned batch, void* stream);
+
+/* UpdateCorrelationKernel, src/Correlation/SubTomograms.cu:138-158: running
+ * max over `batch` maps of `vectorlength` floats with strict '>', the first
+ * maximum winning; d_bestindex receives batchoffset + b (int32). */
+int gtomx_update_max_argmax(const float* d_correlation, size_t vectorlength, unsigned batch,
+                            int batchoffset, float* d_best, int32_t* d_bestindex, void* stream);
+
+/* d_DivideSafeByVector, include/Generics.cuh:190 / src/Generics/Arithmetics.cu:478-493:
+ * out = |div| > 1e-15 ? in / div : 0. */
+int gtomx_divide_safe_by_vector(const float* d_input, const float* d_divisors, float* d_output,
+                                size_t elements, unsigned batch, void* stream);
+
+/* d_SphereMask (real, not decentered), include/Masking.cuh:20 /
+ * src/Masking/SphereMask.cu:18-33, 49-106: soft sphere with raised-cosine
+ * edge of width sigma around (dims/2). */
+int gtomx_sphere_mask(const float* d_input, float* d_output, gtomx_int3 dims, float radius,
+                      float sigma, unsigned batch, void* stream);
+
+/* d_LocalStd without a Fourier mask, include/Generics.cuh:782 /
+ * src/Generics/LocalStd.cu:10-118: local standard deviation (and, if d_mean
+ * is not NULL, mean) of d_map under a soft sphere of localradius. The
+ * variance is combined in double. Allocates and frees its own scratch. */
+int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, float* d_std,
+                    float* d_mean, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* The template-matching path                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gtomx_pick_plan gtomx_pick_plan;
+
+/* Scratch and configuration for d_PickSubTomograms on cubes of side n =
+ * dimsvolume.x against a projector of side p = dimsprojector.x. The plan
+ * owns every device buffer the search needs (the reference allocates them
+ * per call, src/Correlation/SubTomograms.cu:44-56). batch is the number of
+ * orientations processed per pass (0 selects the default). */
+int gtomx_pick_plan_create(gtomx_pick_plan** plan, gtomx_int3 dimsprojector,
+                           gtomx_int3 dimsvolume, unsigned batch);
+int gtomx_pick_plan_destroy(gtomx_pick_plan* plan);
+/* Orientations per pass and device bytes owned by the plan. */
+int gtomx_pick_plan_info(const gtomx_pick_plan* plan, unsigned* batch, size_t* device_bytes);
+
+/* Kernel timing: when enabled, every launch of the search kernels is bracketed
+ * by HIP events on the plan's stream; gtomx_pick_plan_timings synchronises and
+ * returns, per stage k in {0: project+z, 1: y, 2: x+max}, the summed
+ * milliseconds and launch count since the last reset. */
+int gtomx_pick_plan_enable_timing(gtomx_pick_plan* plan, int enable);
+int gtomx_pick_plan_timings(gtomx_pick_plan* plan, double* ms3, unsigned long long* launches3,
+                            int reset);
+
+/* d_PickSubTomograms, include/Correlation.cuh:97-109 /
+ * src/Correlation/SubTomograms.cu:19-125, with the device-array projector
+ * (the texture pair is replaced by d_projectorft, a (p/2+1, p, p) half
+ * spectrum in FFT order). For every volume v and orientation o:
+ *   R = project(projector, transpose(Euler(h_angles[o])) * oversample) * ctf_v,
+ *   R' = R normalised as C2R -> mean0/std1 -> R2C -> unit RMS,
+ *   C = C2R(E_v * conj(R')), best = max over o (strict '>', first wins),
+ * then best /= local std of C2R(E_v) under a soft sphere of maskradius.
+ * d_ctf holds (n/2+1)*n*n floats; ctf_volume_stride is the element offset
+ * between the CTFs of consecutive volumes (0: one CTF for all volumes, as in
+ * the reference). d_experimentalft is not modified. d_bestindex is int32;
+ * d_bestangle (optional, may be NULL) receives the same index as float, the
+ * reference's output type. Asynchronous on `stream`. */
+int gtomx_pick_subtomograms(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
+                            float projectoroversample, gtomx_int3 dimsprojector,
+                            const gtomx_complex* d_experimentalft, const float* d_ctf,
+                            size_t ctf_volume_stride, gtomx_int3 dimsvolume, unsigned nvolumes,
+                            const gtomx_float3* h_angles, unsigned nangles, float maskradius,
+                            float* d_bestcorrelation, int32_t* d_bestindex, float* d_bestangle,
+                            void* stream);
+
+/* Resumable pieces of the same search (no reference counterpart; they split
+ * d_PickSubTomograms at its natural seams so a caller can checkpoint or
+ * spread an orientation list over several calls):
+ *  - begin: fills best with -1e30 and the index with 0;
+ *  - update: searches h_angles[0..nangles) and records indices as
+ *    angle_offset + o, keeping strictly better maxima only;
+ *  - finish: divides best by the local standard deviation of C2R(E_v).
+ * begin; update(list[0:k], 0); update(list[k:], k); finish equals one call of
+ * gtomx_pick_subtomograms on the whole list. */
+int gtomx_pick_begin(gtomx_pick_plan* plan, gtomx_int3 dimsvolume, unsigned nvolumes,
+                     float* d_bestcorrelation, int32_t* d_bestindex, void* stream);
+int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
+                      float projectoroversample, gtomx_int3 dimsprojector,
+                      const gtomx_complex* d_experimentalft, const float* d_ctf,
+                      size_t ctf_volume_stride, gtomx_int3 dimsvolume, unsigned nvolumes,
+                      const gtomx_float3* h_angles, unsigned nangles, unsigned angle_offset,
+                      float* d_bestcorrelation, int32_t* d_bestindex, void* stream);
+int gtomx_pick_finish(gtomx_pick_plan* plan, const gtomx_complex* d_experimentalft,
+                      gtomx_int3 dimsvolume, unsigned nvolumes, float maskradius,
+                      float* d_bestcorrelation, int32_t* d_bestindex, float* d_bestangle,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GTOMX_H */

@@ -1,0 +1,616 @@
+// CPU oracle for GTOM's FFT template-matching path (d_PickSubTomograms).
+//
+// TEST INFRASTRUCTURE ONLY. This file is a plain restatement of the reference
+// algorithm in double precision (the correctness oracle) and, instantiated at
+// float, the host-core baseline that is timed beside the GPU path. Only the
+// test suite, the smoke check and the CPU-baseline leg of the benchmark load
+// it; the product library never links or calls it.
+//
+// Every function cites the reference file:line it restates. Conventions:
+//  * real volumes are x-fastest, index (z*ny + y)*nx + x;
+//  * half spectra hold x in [0, nx/2], index (z*ny + y)*(nx/2+1) + x, y and z
+//    in FFT order (signed frequency = i <= n/2 ? i : i - n);
+//  * R2C is the unnormalised forward DFT (exp(-2 pi i k.r/n));
+//  * C2R_u is the unnormalised inverse, defined as the real part of the inverse
+//    DFT of the Hermitian-completed spectrum (the imaginary parts of the x=0
+//    and x=nx/2 bins are ignored after the y/z transforms); C2R_n = C2R_u / N.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+template <class T> using cpx = std::complex<T>;
+
+// ---------------------------------------------------------------------------
+// 1D complex FFT: iterative radix-2 for powers of two, direct DFT otherwise.
+// Twiddles are evaluated in long double and rounded to T.
+// Replaces the closed cuFFT plans behind src/FFT/FFT.cu:17-42 and
+// src/FFT/IFFT.cu:18-56 (no reference source exists for the arithmetic).
+// ---------------------------------------------------------------------------
+template <class T> struct Fft1D {
+  int n = 0;
+  bool pow2 = true;
+  std::vector<cpx<T>> tw;  // exp(-2 pi i k / n), k in [0, n)
+  std::vector<int> rev;
+
+  explicit Fft1D(int n_) : n(n_) {
+    pow2 = n > 0 && (n & (n - 1)) == 0;
+    tw.resize(std::max(n, 1));
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < n; ++k) {
+      long double a = -twopi * (long double)k / (long double)n;
+      tw[k] = cpx<T>((T)std::cos(a), (T)std::sin(a));
+    }
+    if (pow2) {
+      rev.resize(n);
+      int lg = 0;
+      while ((1 << lg) < n) ++lg;
+      for (int i = 0; i < n; ++i) {
+        int r = 0;
+        for (int b = 0; b < lg; ++b) r |= ((i >> b) & 1) << (lg - 1 - b);
+        rev[i] = r;
+      }
+    }
+  }
+
+  // sign = -1: forward (exp(-i...)), sign = +1: inverse (exp(+i...)), unnormalised.
+  void run(cpx<T>* a, int sign, cpx<T>* scratch) const {
+    if (n <= 1) return;
+    if (!pow2) {
+      for (int k = 0; k < n; ++k) {
+        cpx<T> acc(0, 0);
+        for (int j = 0; j < n; ++j) {
+          cpx<T> w = tw[(int)(((long long)j * k) % n)];
+          if (sign > 0) w = std::conj(w);
+          acc += a[j] * w;
+        }
+        scratch[k] = acc;
+      }
+      for (int k = 0; k < n; ++k) a[k] = scratch[k];
+      return;
+    }
+    for (int i = 0; i < n; ++i)
+      if (i < rev[i]) std::swap(a[i], a[rev[i]]);
+    for (int len = 2; len <= n; len <<= 1) {
+      int half = len >> 1, step = n / len;
+      for (int i = 0; i < n; i += len) {
+        for (int j = 0; j < half; ++j) {
+          cpx<T> w = tw[j * step];
+          if (sign > 0) w = std::conj(w);
+          cpx<T> u = a[i + j], v = a[i + j + half] * w;
+          a[i + j] = u + v;
+          a[i + j + half] = u - v;
+        }
+      }
+    }
+  }
+};
+
+inline bool in_parallel() {
+#ifdef _OPENMP
+  return omp_in_parallel();
+#else
+  return false;
+#endif
+}
+
+// Transform every line along one axis of a complex array with dims (dx, dy, dz).
+// axis 0: x (contiguous), 1: y (stride dx), 2: z (stride dx*dy).
+template <class T>
+void fft_axis(cpx<T>* data, int dx, int dy, int dz, int axis, int sign) {
+  int len = axis == 0 ? dx : (axis == 1 ? dy : dz);
+  if (len <= 1) return;
+  Fft1D<T> plan(len);
+  long long stride = axis == 0 ? 1 : (axis == 1 ? dx : (long long)dx * dy);
+  long long nlines = (long long)dx * dy * dz / len;
+#pragma omp parallel if (!in_parallel())
+  {
+    std::vector<cpx<T>> line(len), scratch(len);
+#pragma omp for schedule(static)
+    for (long long l = 0; l < nlines; ++l) {
+      long long base;
+      if (axis == 0) base = l * dx;
+      else if (axis == 1) base = (l / dx) * (long long)dx * dy + (l % dx);
+      else base = l;  // l indexes (x, y) in [0, dx*dy)
+      for (int i = 0; i < len; ++i) line[i] = data[base + i * stride];
+      plan.run(line.data(), sign, scratch.data());
+      for (int i = 0; i < len; ++i) data[base + i * stride] = line[i];
+    }
+  }
+}
+
+// R2C: unnormalised forward transform of a real (nx, ny, nz) volume into the
+// (nx/2+1, ny, nz) half spectrum. Follows cufftExecR2C as called at
+// src/FFT/FFT.cu:36-42 (dims reversed to {z,y,x} at src/FFT/FFT.cu:21).
+template <class T>
+void r2c3(const T* in, cpx<T>* out, int nx, int ny, int nz) {
+  int nh = nx / 2 + 1;
+  Fft1D<T> plan(nx);
+  long long rows = (long long)ny * nz;
+#pragma omp parallel if (!in_parallel())
+  {
+    std::vector<cpx<T>> line(nx), scratch(nx);
+#pragma omp for schedule(static)
+    for (long long r = 0; r < rows; ++r) {
+      for (int x = 0; x < nx; ++x) line[x] = cpx<T>(in[r * nx + x], 0);
+      plan.run(line.data(), -1, scratch.data());
+      for (int x = 0; x < nh; ++x) out[r * nh + x] = line[x];
+    }
+  }
+  fft_axis(out, nh, ny, nz, 1, -1);
+  fft_axis(out, nh, ny, nz, 2, -1);
+}
+
+// C2R_u: unnormalised inverse of a half spectrum, real-part semantics (see the
+// file header). Follows cufftExecC2R at src/FFT/IFFT.cu:48-56 (no 1/N).
+template <class T>
+void c2r3(const cpx<T>* in, T* out, int nx, int ny, int nz) {
+  int nh = nx / 2 + 1;
+  long long H = (long long)nh * ny * nz;
+  std::vector<cpx<T>> tmp(in, in + H);
+  fft_axis(tmp.data(), nh, ny, nz, 2, +1);
+  fft_axis(tmp.data(), nh, ny, nz, 1, +1);
+  Fft1D<T> plan(nx);
+  long long rows = (long long)ny * nz;
+#pragma omp parallel if (!in_parallel())
+  {
+    std::vector<cpx<T>> line(nx), scratch(nx);
+#pragma omp for schedule(static)
+    for (long long r = 0; r < rows; ++r) {
+      const cpx<T>* h = &tmp[r * nh];
+      line[0] = cpx<T>(h[0].real(), 0);
+      for (int k = 1; k < nh; ++k) {
+        if (2 * k == nx) {
+          line[k] = cpx<T>(h[k].real(), 0);
+        } else {
+          line[k] = h[k];
+          line[nx - k] = std::conj(h[k]);
+        }
+      }
+      plan.run(line.data(), +1, scratch.data());
+      for (int x = 0; x < nx; ++x) out[r * nx + x] = line[x].real();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fourier-space projector: restates Project3DArraytoNDKernel<3>
+// (src/Relion/Project.cu:89-177) for one orientation. M is the row-major 3x3
+// matrix applied to the integer frequency (x, y, z); the reference builds it as
+// transpose(Matrix3Euler(angles)) * Matrix3Scale(oversample)
+// (src/Relion/Project.cu:12-17). The reference reads the projector at the
+// linear address ((z*p + y)*(p/2+1) + x) without a bounds check; x1 = p/2+1
+// therefore reaches into the next row. The same linear address is used here and
+// clamped to the array, which changes nothing for in-range reads.
+// ---------------------------------------------------------------------------
+template <class T>
+void project1(const cpx<T>* vol, int p, cpx<T>* out, int n, const double* M,
+              int rmax) {
+  int ph = p / 2 + 1, nh = n / 2 + 1;
+  long long total = (long long)ph * p * p;
+  int rmax2 = rmax * rmax;
+  long long H = (long long)nh * n * n;
+  auto at = [&](long long xx, long long yy, long long zz) -> cpx<T> {
+    long long i = (zz * p + yy) * ph + xx;
+    if (i < 0) i = 0;
+    if (i >= total) i = total - 1;
+    return vol[i];
+  };
+  auto lerp = [](cpx<T> a, cpx<T> b, T t) -> cpx<T> {
+    return cpx<T>(std::fma(b.real() - a.real(), t, a.real()),
+                  std::fma(b.imag() - a.imag(), t, a.imag()));
+  };
+#pragma omp parallel for schedule(static) if (!in_parallel())
+  for (long long id = 0; id < H; ++id) {
+    int ix = (int)(id % nh);
+    int iy = (int)((id / nh) % n);
+    int iz = (int)(id / ((long long)nh * n));
+    int x = ix;
+    int y = iy <= n / 2 ? iy : iy - n;
+    int z = iz <= n / 2 ? iz : iz - n;
+    int r2 = z * z + y * y + x * x;
+    if (r2 > rmax2) {
+      out[id] = cpx<T>(0, 0);
+      continue;
+    }
+    T px = (T)(M[0] * x + M[1] * y + M[2] * z);
+    T py = (T)(M[3] * x + M[4] * y + M[5] * z);
+    T pz = (T)(M[6] * x + M[7] * y + M[8] * z);
+    T neg = 1;
+    if (px < (T)-1e-5) {
+      px = std::fabs(px);
+      py = -py;
+      pz = -pz;
+      neg = -1;
+    }
+    T fx = std::floor(px + (T)1e-5);
+    long long x0 = (long long)fx, x1 = x0 + 1;
+    T tx = px - fx;
+    T fy = std::floor(py);
+    long long y0 = (long long)fy, y1 = y0 + 1;
+    if (y0 < 0) y0 += p;
+    if (y1 < 0) y1 += p;
+    T ty = py - fy;
+    T fz = std::floor(pz);
+    long long z0 = (long long)fz, z1 = z0 + 1;
+    if (z0 < 0) z0 += p;
+    if (z1 < 0) z1 += p;
+    T tz = pz - fz;
+    cpx<T> d000 = at(x0, y0, z0), d001 = at(x1, y0, z0);
+    cpx<T> d010 = at(x0, y1, z0), d011 = at(x1, y1, z0);
+    cpx<T> d100 = at(x0, y0, z1), d101 = at(x1, y0, z1);
+    cpx<T> d110 = at(x0, y1, z1), d111 = at(x1, y1, z1);
+    cpx<T> dx00 = lerp(d000, d001, tx), dx01 = lerp(d010, d011, tx);
+    cpx<T> dx10 = lerp(d100, d101, tx), dx11 = lerp(d110, d111, tx);
+    cpx<T> dxy0 = lerp(dx00, dx01, ty), dxy1 = lerp(dx10, dx11, ty);
+    cpx<T> val = lerp(dxy0, dxy1, tz);
+    out[id] = cpx<T>(val.real(), val.imag() * neg);
+  }
+}
+
+// Matrix3Euler (src/Helpers/Euler.cu:39-68): ZYZ, RELION Euler_angles2matrix
+// layout, returned row-major (A[r][c] = A[3r + c]).
+void euler_matrix(double rot, double tilt, double psi, double* A) {
+  double ca = std::cos(rot), sa = std::sin(rot);
+  double cb = std::cos(tilt), sb = std::sin(tilt);
+  double cg = std::cos(psi), sg = std::sin(psi);
+  double cc = cb * ca, cs = cb * sa, sc = sb * ca, ss = sb * sa;
+  A[0] = cg * cc - sg * sa;  A[1] = cg * cs + sg * ca;  A[2] = -cg * sb;
+  A[3] = -sg * cc - cg * sa; A[4] = -sg * cs + cg * ca; A[5] = sg * sb;
+  A[6] = sc;                 A[7] = ss;                 A[8] = cb;
+}
+
+// The projector matrix transpose(A) * s (src/Relion/Project.cu:16).
+void projector_matrix(double rot, double tilt, double psi, double s, double* M) {
+  double A[9];
+  euler_matrix(rot, tilt, psi, A);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) M[3 * r + c] = s * A[3 * c + r];
+}
+
+// Soft sphere mask value, SphereMaskKernel with decentered = false
+// (src/Masking/SphereMask.cu:49-106), centre (n/2, n/2, n/2), sigma 1.
+double soft_sphere(int x, int y, int z, int nx, int ny, int nz, double radius,
+                   double sigma) {
+  double xx = x - nx / 2, yy = y - ny / 2, zz = nz > 1 ? z - nz / 2 : 0;
+  double len = std::sqrt(xx * xx + yy * yy + zz * zz);
+  if (len < radius) return 1.0;
+  if (sigma > 0) {
+    double v = (std::cos(std::min(1.0, (len - radius) / sigma) * M_PI) + 1.0) * 0.5;
+    return std::max(0.0, v);
+  }
+  return 0.0;
+}
+
+// IFFTShift(x, d) = (x + (d+1)/2) % d (include/Prerequisites.cuh:227);
+// RemapFull2FullFFTKernel writes out[ifftshift(r)] = in[r]
+// (src/FFT/FFTRemap.cu:227-240).
+template <class T>
+void remap_full2fullfft(const T* in, T* out, int nx, int ny, int nz) {
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        int rx = (x + (nx + 1) / 2) % nx, ry = (y + (ny + 1) / 2) % ny,
+            rz = (z + (nz + 1) / 2) % nz;
+        out[((long long)rz * ny + ry) * nx + rx] = in[((long long)z * ny + y) * nx + x];
+      }
+}
+
+// d_LocalStd without a Fourier mask (src/Generics/LocalStd.cu:10-118):
+// sigma = sqrt(max(0, M*conv(map^2) - conv(map)^2)) / M, with conv = C2R_n of
+// the product with the conjugate FT of the ifftshifted soft sphere and
+// M = sum of the mask. Optional local mean = conv(map) / M.
+template <class T>
+void local_std(const T* map, int nx, int ny, int nz, double radius, T* std_out,
+               T* mean_out) {
+  long long V = (long long)nx * ny * nz;
+  int nh = nx / 2 + 1;
+  long long H = (long long)nh * ny * nz;
+  std::vector<T> mask(V), maskr(V);
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x)
+        mask[((long long)z * ny + y) * nx + x] =
+            (T)soft_sphere(x, y, z, nx, ny, nz, radius, 1.0);
+  remap_full2fullfft(mask.data(), maskr.data(), nx, ny, nz);
+  double msum = 0;
+  for (long long i = 0; i < V; ++i) msum += maskr[i];
+  std::vector<cpx<T>> mft(H), aft(H), bft(H);
+  r2c3(maskr.data(), mft.data(), nx, ny, nz);
+  std::vector<T> sq(V);
+  for (long long i = 0; i < V; ++i) sq[i] = map[i] * map[i];
+  r2c3(map, aft.data(), nx, ny, nz);
+  r2c3(sq.data(), bft.data(), nx, ny, nz);
+  for (long long i = 0; i < H; ++i) {
+    aft[i] *= std::conj(mft[i]);
+    bft[i] *= std::conj(mft[i]);
+  }
+  std::vector<T> c1(V), c2(V);
+  c2r3(aft.data(), c1.data(), nx, ny, nz);
+  c2r3(bft.data(), c2.data(), nx, ny, nz);
+  T invN = (T)(1.0 / (double)V);
+  T M = (T)msum;
+  for (long long i = 0; i < V; ++i) {
+    T a = c1[i] * invN, b = c2[i] * invN;
+    if (mean_out) mean_out[i] = a / M;
+    T v = b * M - a * a;
+    v = std::max(v, (T)0);
+    std_out[i] = std::sqrt(v) / M;
+  }
+}
+
+// 3D CTF on the half spectrum: CTFParamsLean (include/CTF.cuh:100-134) with
+// d_GetCTF<false,false> (include/CTF.cuh:195-223) and the 3D frequency mapping
+// k = |r| * ny, ny = 1/(n * pixelsize) (src/CTF/Wiener.cu:99-110).
+// Lengths in SI units as in CTFParams (include/CTF.cuh:11-43).
+template <class T>
+void ctf3(T* out, int n, double pixelsize_m, double voltage_V, double Cs_m,
+          double defocus_m, double amplitude, double phaseshift,
+          double bfactor_m2, double scale) {
+  int nh = n / 2 + 1;
+  double ny = 1.0 / ((double)n * (pixelsize_m * 1e10));
+  double lambda = 12.2643247 / std::sqrt(voltage_V * (1.0 + voltage_V * 0.978466e-6));
+  double defocus = defocus_m * 1e10;
+  double Cs = Cs_m * 1e10;
+  double K1 = M_PI * lambda;
+  double K2 = M_PI_2 * Cs * lambda * lambda * lambda;
+  double K3 = std::atan(amplitude / std::sqrt(1 - amplitude * amplitude));
+  double B = bfactor_m2 * 0.25e20;
+  for (int iz = 0; iz < n; ++iz)
+    for (int iy = 0; iy < n; ++iy)
+      for (int ix = 0; ix < nh; ++ix) {
+        int y = ((iy + n / 2) % n) - n / 2, z = ((iz + n / 2) % n) - n / 2;
+        double r = std::sqrt((double)ix * ix + (double)y * y + (double)z * z);
+        double k = r * ny, k2 = k * k, k4 = k2 * k2;
+        double gamma = K1 * defocus * k2 + K2 * k4 - phaseshift - K3;
+        double v = -std::sin(gamma);
+        if (B != 0) v *= std::exp(B * k2);
+        v *= scale;
+        out[((long long)iz * n + iy) * nh + ix] = (T)v;
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Full path: restates d_PickSubTomograms (src/Correlation/SubTomograms.cu:19-125)
+// with the device-array projector, for one volume at a time. Per orientation:
+//  R = project(P, M_o) (src/Relion/Project.cu:89-177)
+//  R *= ctf_v                      (src/Generics/Arithmetics.cu:275-286)
+//  r = C2R_u(R); r = (r - mean)/std (src/ImageManipulation/Norm.cu:242-284)
+//  Y = R2C(r); Y /= max(sqrt(sum|Y|^2 / H), 1e-20)
+//                                  (src/ImageManipulation/Norm.cu:476-509)
+//  Q = E_v * conj(Y)               (src/Correlation/SubTomograms.cu:127-136)
+//  C = C2R_u(Q)                    (src/Correlation/SubTomograms.cu:81)
+//  best = max over o, strict '>' in increasing o (SubTomograms.cu:138-158)
+// then sigma = local_std(C2R_u(E_v), maskradius) and
+// out = |sigma| > 1e-15 ? best / sigma : 0 (src/Generics/Arithmetics.cu:478-493).
+// The oracle also keeps the second-best CC per voxel.
+// ---------------------------------------------------------------------------
+template <class T>
+double pick(const T* projft, int p, double oversample, const T* expft,
+            const T* ctf, int ctf_per_volume, int n, int nvol,
+            const double* angles, int nangles, double maskradius, T* best_out,
+            int32_t* idx_out, T* top1_out, T* top2_out, T* sigma_out) {
+  const cpx<T>* P = reinterpret_cast<const cpx<T>*>(projft);
+  const cpx<T>* E = reinterpret_cast<const cpx<T>*>(expft);
+  int nh = n / 2 + 1;
+  long long V = (long long)n * n * n, H = (long long)nh * n * n;
+  std::vector<cpx<T>> R(H), Y(H), Q(H);
+  std::vector<T> r(V), C(V);
+  std::vector<T> top1(V * nvol, (T)-1e30), top2(V * nvol, (T)-1e30);
+  std::vector<int32_t> idx(V * nvol, 0);
+  double loop_seconds = 0;
+  for (int v = 0; v < nvol; ++v) {
+    const T* c = ctf + (ctf_per_volume ? (long long)v * H : 0);
+    const cpx<T>* Ev = E + (long long)v * H;
+    T* t1 = &top1[v * V];
+    T* t2 = &top2[v * V];
+    int32_t* ix = &idx[v * V];
+    auto t0 = std::chrono::steady_clock::now();
+    for (int o = 0; o < nangles; ++o) {
+      double M[9];
+      projector_matrix(angles[3 * o], angles[3 * o + 1], angles[3 * o + 2], oversample, M);
+      project1(P, p, R.data(), n, M, n / 2);
+      for (long long i = 0; i < H; ++i) R[i] *= c[i];
+      c2r3(R.data(), r.data(), n, n, n);
+      double s1 = 0, s2 = 0;
+      for (long long i = 0; i < V; ++i) {
+        double val = r[i];
+        s1 += val;
+        s2 += val * val;
+      }
+      double mean = s1 / (double)V;
+      double sd = std::sqrt((double)V * s2 - s1 * s1) / (double)V;
+      T tmean = (T)mean, tinv = (T)(sd > 0 ? 1.0 / sd : 0.0);
+      for (long long i = 0; i < V; ++i) r[i] = (r[i] - tmean) * tinv;
+      r2c3(r.data(), Y.data(), n, n, n);
+      double ss = 0;
+      for (long long i = 0; i < H; ++i) ss += (double)std::norm(Y[i]);
+      double rms = std::sqrt(ss / (double)H);
+      T inv = (T)(1.0 / std::max(rms, 1e-20));
+      for (long long i = 0; i < H; ++i) {
+        Y[i] *= inv;
+        Q[i] = Ev[i] * std::conj(Y[i]);
+      }
+      c2r3(Q.data(), C.data(), n, n, n);
+      for (long long i = 0; i < V; ++i) {
+        T cc = C[i];
+        if (cc > t1[i]) {
+          t2[i] = t1[i];
+          t1[i] = cc;
+          ix[i] = o;
+        } else if (cc > t2[i]) {
+          t2[i] = cc;
+        }
+      }
+    }
+    loop_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // Local-std normalisation (SubTomograms.cu:103-118)
+    std::vector<T> e(V), sig(V);
+    c2r3(Ev, e.data(), n, n, n);
+    local_std(e.data(), n, n, n, maskradius, sig.data(), (T*)nullptr);
+    for (long long i = 0; i < V; ++i) {
+      T s = sig[i];
+      T out = std::fabs(s) > (T)1e-15 ? t1[i] / s : (T)0;
+      if (best_out) best_out[v * V + i] = out;
+      if (sigma_out) sigma_out[v * V + i] = s;
+    }
+  }
+  if (idx_out) std::memcpy(idx_out, idx.data(), sizeof(int32_t) * V * nvol);
+  if (top1_out) std::memcpy(top1_out, top1.data(), sizeof(T) * V * nvol);
+  if (top2_out) std::memcpy(top2_out, top2.data(), sizeof(T) * V * nvol);
+  return loop_seconds;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI for ctypes. Complex arrays are interleaved (re, im).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_num_threads(int t) {
+#ifdef _OPENMP
+  if (t > 0) omp_set_num_threads(t);
+#else
+  (void)t;
+#endif
+}
+
+void orc_fft_c2c_1d_f64(double* data, int n, int sign) {
+  Fft1D<double> plan(n);
+  std::vector<cpx<double>> s(n);
+  plan.run(reinterpret_cast<cpx<double>*>(data), sign, s.data());
+}
+
+void orc_r2c_3d_f64(const double* in, double* out, int nx, int ny, int nz, int batch) {
+  long long V = (long long)nx * ny * nz, H = (long long)(nx / 2 + 1) * ny * nz;
+  for (int b = 0; b < batch; ++b)
+    r2c3(in + b * V, reinterpret_cast<cpx<double>*>(out) + b * H, nx, ny, nz);
+}
+void orc_c2r_3d_f64(const double* in, double* out, int nx, int ny, int nz, int batch) {
+  long long V = (long long)nx * ny * nz, H = (long long)(nx / 2 + 1) * ny * nz;
+  for (int b = 0; b < batch; ++b)
+    c2r3(reinterpret_cast<const cpx<double>*>(in) + b * H, out + b * V, nx, ny, nz);
+}
+void orc_r2c_3d_f32(const float* in, float* out, int nx, int ny, int nz, int batch) {
+  long long V = (long long)nx * ny * nz, H = (long long)(nx / 2 + 1) * ny * nz;
+  for (int b = 0; b < batch; ++b)
+    r2c3(in + b * V, reinterpret_cast<cpx<float>*>(out) + b * H, nx, ny, nz);
+}
+void orc_c2r_3d_f32(const float* in, float* out, int nx, int ny, int nz, int batch) {
+  long long V = (long long)nx * ny * nz, H = (long long)(nx / 2 + 1) * ny * nz;
+  for (int b = 0; b < batch; ++b)
+    c2r3(reinterpret_cast<const cpx<float>*>(in) + b * H, out + b * V, nx, ny, nz);
+}
+
+void orc_euler_matrix(double rot, double tilt, double psi, double* A) {
+  euler_matrix(rot, tilt, psi, A);
+}
+void orc_projector_matrix(double rot, double tilt, double psi, double s, double* M) {
+  projector_matrix(rot, tilt, psi, s, M);
+}
+
+// One orientation per call; M row-major (includes the oversampling factor).
+void orc_project_f64(const double* vol, int p, double* out, int n, const double* M, int rmax) {
+  project1(reinterpret_cast<const cpx<double>*>(vol), p, reinterpret_cast<cpx<double>*>(out), n, M, rmax);
+}
+
+// GetEqualAngularSpacing (src/Helpers/AngularSpacing.cu:46-67), restated with
+// the reference's float32 loop counters. Writes up to maxcount (rot, tilt, psi)
+// triples and returns the total count.
+int orc_equal_angular_spacing(float phimin, float phimax, float thetamin, float thetamax,
+                              float psimin, float psimax, float increment, float* out,
+                              int maxcount) {
+  const float PI2f = 6.283185307179586476925286766559f;
+  int count = 0;
+  for (float tilt = thetamin; tilt <= thetamax + 1e-5f; tilt += increment) {
+    int rot_nstep = tilt > 0 ? (int)std::ceil(PI2f * std::sin(tilt) / increment) : 1;
+    float rot_sam = PI2f / (float)rot_nstep;
+    for (float rot = phimin; rot <= phimax - rot_sam + 1e-5f; rot += rot_sam)
+      for (float psi = psimin; psi <= psimax - increment + 1e-5f; psi += increment) {
+        if (out && count < maxcount) {
+          out[3 * count] = rot;
+          out[3 * count + 1] = tilt;
+          out[3 * count + 2] = psi;
+        }
+        count++;
+      }
+  }
+  return count;
+}
+
+void orc_ctf3_f64(double* out, int n, double pixelsize_m, double voltage_V, double Cs_m,
+                  double defocus_m, double amplitude, double phaseshift, double bfactor_m2,
+                  double scale) {
+  ctf3(out, n, pixelsize_m, voltage_V, Cs_m, defocus_m, amplitude, phaseshift, bfactor_m2, scale);
+}
+
+void orc_sphere_mask_f64(double* out, int nx, int ny, int nz, double radius, double sigma) {
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x)
+        out[((long long)z * ny + y) * nx + x] = soft_sphere(x, y, z, nx, ny, nz, radius, sigma);
+}
+
+void orc_remap_full2fullfft_f64(const double* in, double* out, int nx, int ny, int nz) {
+  remap_full2fullfft(in, out, nx, ny, nz);
+}
+
+void orc_local_std_f64(const double* map, int nx, int ny, int nz, double radius,
+                       double* std_out, double* mean_out) {
+  local_std(map, nx, ny, nz, radius, std_out, mean_out);
+}
+
+// Running max/argmax over a batch of maps (UpdateCorrelationKernel,
+// src/Correlation/SubTomograms.cu:138-158), int32 index.
+void orc_update_max_argmax_f64(const double* corr, long long vectorlength, int batch,
+                               int batchoffset, double* best, int32_t* bestidx) {
+  for (long long i = 0; i < vectorlength; ++i) {
+    double b = best[i];
+    int32_t a = bestidx[i];
+    for (int k = 0; k < batch; ++k) {
+      double c = corr[k * vectorlength + i];
+      if (c > b) {
+        b = c;
+        a = k + batchoffset;
+      }
+    }
+    best[i] = b;
+    bestidx[i] = a;
+  }
+}
+
+// Returns the wall time of the orientation loop in seconds.
+double orc_pick_f64(const double* projft, int p, double oversample, const double* expft,
+                    const double* ctf, int ctf_per_volume, int n, int nvol,
+                    const double* angles, int nangles, double maskradius, double* best_out,
+                    int32_t* idx_out, double* top1_out, double* top2_out, double* sigma_out) {
+  return pick<double>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
+                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+double orc_pick_f32(const float* projft, int p, double oversample, const float* expft,
+                    const float* ctf, int ctf_per_volume, int n, int nvol,
+                    const double* angles, int nangles, double maskradius, float* best_out,
+                    int32_t* idx_out, float* top1_out, float* top2_out, float* sigma_out) {
+  return pick<float>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
+                     nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+"""ctypes wrapper around the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the CPU-baseline leg of bench.py, always as the checker or the timed host
+baseline, never as part of the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_d = ctypes.c_double
+_f = ctypes.c_float
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+_p = ctypes.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_num_threads.restype = _i
+        L.orc_set_num_threads.argtypes = [_i]
+        L.orc_fft_c2c_1d_f64.argtypes = [_p, _i, _i]
+        for nm in ("orc_r2c_3d_f64", "orc_c2r_3d_f64", "orc_r2c_3d_f32", "orc_c2r_3d_f32"):
+            getattr(L, nm).argtypes = [_p, _p, _i, _i, _i, _i]
+        L.orc_euler_matrix.argtypes = [_d, _d, _d, _p]
+        L.orc_projector_matrix.argtypes = [_d, _d, _d, _d, _p]
+        L.orc_project_f64.argtypes = [_p, _i, _p, _i, _p, _i]
+        L.orc_equal_angular_spacing.argtypes = [_f, _f, _f, _f, _f, _f, _f, _p, _i]
+        L.orc_equal_angular_spacing.restype = _i
+        L.orc_ctf3_f64.argtypes = [_p, _i, _d, _d, _d, _d, _d, _d, _d, _d]
+        L.orc_sphere_mask_f64.argtypes = [_p, _i, _i, _i, _d, _d]
+        L.orc_remap_full2fullfft_f64.argtypes = [_p, _p, _i, _i, _i]
+        L.orc_local_std_f64.argtypes = [_p, _i, _i, _i, _d, _p, _p]
+        L.orc_update_max_argmax_f64.argtypes = [_p, _ll, _i, _i, _p, _p]
+        pick_args = [_p, _i, _d, _p, _p, _i, _i, _i, _p, _i, _d, _p, _p, _p, _p, _p]
+        L.orc_pick_f64.argtypes = pick_args
+        L.orc_pick_f64.restype = _d
+        L.orc_pick_f32.argtypes = pick_args
+        L.orc_pick_f32.restype = _d
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def set_num_threads(t):
+    lib().orc_set_num_threads(int(t))
+
+
+def fft_1d(x, sign=-1):
+    """Unnormalised complex DFT of a 1D array (sign -1 forward, +1 inverse)."""
+    a = _c(x, np.complex128).copy()
+    lib().orc_fft_c2c_1d_f64(_ptr(a), a.size, sign)
+    return a
+
+
+def r2c(vol, dtype=np.float64):
+    """Unnormalised forward R2C of a (nz, ny, nx) or (batch, nz, ny, nx) volume."""
+    v = _c(vol, dtype)
+    shp = v.shape if v.ndim == 4 else (1,) + v.shape
+    b, nz, ny, nx = shp
+    cdt = np.complex128 if dtype == np.float64 else np.complex64
+    out = np.empty((b, nz, ny, nx // 2 + 1), cdt)
+    fn = lib().orc_r2c_3d_f64 if dtype == np.float64 else lib().orc_r2c_3d_f32
+    fn(_ptr(v), _ptr(out), nx, ny, nz, b)
+    return out if vol.ndim == 4 else out[0]
+
+
+def c2r(spec, nx, dtype=np.float64):
+    """Unnormalised inverse C2R (real-part semantics) to width nx."""
+    cdt = np.complex128 if dtype == np.float64 else np.complex64
+    s = _c(spec, cdt)
+    shp = s.shape if s.ndim == 4 else (1,) + s.shape
+    b, nz, ny, nh = shp
+    assert nh == nx // 2 + 1
+    out = np.empty((b, nz, ny, nx), dtype)
+    fn = lib().orc_c2r_3d_f64 if dtype == np.float64 else lib().orc_c2r_3d_f32
+    fn(_ptr(s), _ptr(out), nx, ny, nz, b)
+    return out if spec.ndim == 4 else out[0]
+
+
+def euler_matrix(rot, tilt, psi):
+    A = np.empty(9)
+    lib().orc_euler_matrix(rot, tilt, psi, _ptr(A))
+    return A.reshape(3, 3)
+
+
+def projector_matrix(rot, tilt, psi, s):
+    M = np.empty(9)
+    lib().orc_projector_matrix(rot, tilt, psi, s, _ptr(M))
+    return M.reshape(3, 3)
+
+
+def project(vol_ft, p, n, M, rmax=None):
+    """Project the (p/2+1, p, p) half spectrum with row-major matrix M to (n/2+1, n, n)."""
+    v = _c(vol_ft, np.complex128)
+    assert v.shape == (p, p, p // 2 + 1)
+    out = np.empty((n, n, n // 2 + 1), np.complex128)
+    Mm = _c(M, np.float64).reshape(9)
+    lib().orc_project_f64(_ptr(v), p, _ptr(out), n, _ptr(Mm), n // 2 if rmax is None else int(rmax))
+    return out
+
+
+def equal_angular_spacing(phirange, thetarange, psirange, increment):
+    L = lib()
+    args = [np.float32(phirange[0]), np.float32(phirange[1]), np.float32(thetarange[0]),
+            np.float32(thetarange[1]), np.float32(psirange[0]), np.float32(psirange[1]),
+            np.float32(increment)]
+    cnt = L.orc_equal_angular_spacing(*args, None, 0)
+    out = np.empty((cnt, 3), np.float32)
+    L.orc_equal_angular_spacing(*args, _ptr(out), cnt)
+    return out
+
+
+def ctf3(n, pixelsize_m, voltage_V, Cs_m, defocus_m, amplitude, phaseshift=0.0, bfactor_m2=0.0,
+         scale=1.0):
+    out = np.empty((n, n, n // 2 + 1))
+    lib().orc_ctf3_f64(_ptr(out), n, pixelsize_m, voltage_V, Cs_m, defocus_m, amplitude,
+                       phaseshift, bfactor_m2, scale)
+    return out
+
+
+def sphere_mask(shape, radius, sigma=1.0):
+    nz, ny, nx = shape
+    out = np.empty(shape)
+    lib().orc_sphere_mask_f64(_ptr(out), nx, ny, nz, radius, sigma)
+    return out
+
+
+def remap_full2fullfft(vol):
+    v = _c(vol, np.float64)
+    out = np.empty_like(v)
+    nz, ny, nx = v.shape
+    lib().orc_remap_full2fullfft_f64(_ptr(v), _ptr(out), nx, ny, nz)
+    return out
+
+
+def local_std(vol, radius, with_mean=False):
+    v = _c(vol, np.float64)
+    nz, ny, nx = v.shape
+    sd = np.empty_like(v)
+    mean = np.empty_like(v) if with_mean else None
+    lib().orc_local_std_f64(_ptr(v), nx, ny, nz, radius, _ptr(sd), _ptr(mean) if with_mean else None)
+    return (sd, mean) if with_mean else sd
+
+
+def update_max_argmax(corr, best, bestidx, batchoffset):
+    c = _c(corr, np.float64)
+    b = best  # updated in place
+    assert b.dtype == np.float64 and bestidx.dtype == np.int32
+    batch = c.shape[0]
+    L = c[0].size
+    lib().orc_update_max_argmax_f64(_ptr(c), L, batch, batchoffset, _ptr(b), _ptr(bestidx))
+
+
+def pick(projft, p, oversample, expft, ctf, n, angles, maskradius, ctf_per_volume=False,
+         dtype=np.float64):
+    """Run the whole path. Returns dict(best, idx, top1, top2, sigma, loop_seconds)."""
+    cdt = np.complex128 if dtype == np.float64 else np.complex64
+    P = _c(projft, cdt)
+    E = _c(expft, cdt)
+    nvol = E.shape[0] if E.ndim == 4 else 1
+    C = _c(ctf, dtype)
+    A = _c(angles, np.float64).reshape(-1, 3)
+    V = n ** 3
+    best = np.empty((nvol, n, n, n), dtype)
+    idx = np.empty((nvol, n, n, n), np.int32)
+    top1 = np.empty((nvol, n, n, n), dtype)
+    top2 = np.empty((nvol, n, n, n), dtype)
+    sigma = np.empty((nvol, n, n, n), dtype)
+    fn = lib().orc_pick_f64 if dtype == np.float64 else lib().orc_pick_f32
+    secs = fn(_ptr(P), p, oversample, _ptr(E), _ptr(C), int(bool(ctf_per_volume)), n, nvol,
+              _ptr(A), A.shape[0], maskradius, _ptr(best), _ptr(idx), _ptr(top1), _ptr(top2),
+              _ptr(sigma))
+    del V
+    return dict(best=best, idx=idx, top1=top1, top2=top2, sigma=sigma, loop_seconds=secs)
