@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Throughput of the template-matching path on one MI355X.
+
+Workload (the 256^3 configuration): a 512x512x256 tomogram processed as two
+256^3 blocks with 40 planted copies of a 48^3 random-blob template (SNR 0.05),
+a per-block CTF (defocus U(2,5) um, 10 A/px, 300 kV), and 9,000 orientations
+(500 Fibonacci-sphere directions x 18 in-plane steps); projector box p = 96
+(oversampling 0.375), local-std radius 24. One step is one full
+d_PickSubTomograms call over both blocks and all orientations, with the block
+spectra, CTFs and projector already resident in HBM.
+
+Prints one JSON line. `value` is voxel-orientations per second (2 * 256^3 *
+9000 per step); `roofline` describes the kernel that takes the most time, from
+HIP events recorded around every launch on the stream the search runs on;
+`cpu_baseline` times the oracle's float32 restatement on this host's cores over
+a bounded sample (one block, the first K orientations).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "voxel-orientations/s, 256³ tomo template match; learned vs plain at ≥99% recall"
+HBM_PEAK_GBS = 8000.0
+STAGES = ("k_pick_project_z", "k_pick_y", "k_pick_x_max")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=0, help="orientations per pass (0: library default)")
+    ap.add_argument("--angles", type=int, default=0,
+                    help="search only the first N orientations (profiling runs; not a bench line)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--seed", type=int, default=2)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_probe():
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: 5 x {4 * n} bytes read and written")
+
+
+def algorithmic_bytes(stage, n, nb):
+    """Minimum HBM bytes of one launch of a stage for nb orientations of one n^3 volume."""
+    H = (n // 2 + 1) * n * n
+    V = n ** 3
+    if stage == 0:  # read E (8 B/bin) and the CTF (4 B/bin) once, write nb spectra
+        return 12 * H + 8 * H * nb
+    if stage == 1:  # read and write nb spectra
+        return 16 * H * nb
+    return 8 * H * nb + 16 * V  # read nb spectra, read+write best and argmax
+
+
+def cpu_baseline(w, target_seconds):
+    from oracle import pyoracle as po
+    import torch
+    n, p = w["n"], w["p"]
+    # Same float32 inputs as the GPU run, block 0.
+    E = w["exp_ft"][:1].cpu().numpy()
+    P = w["projft"].cpu().numpy()
+    C = w["ctf"][0].cpu().numpy()
+    angles = w["angles"].astype(np.float64)
+    threads = po.num_threads()
+    probe = max(1, min(4, threads // 4))
+    r = po.pick(P, p, w["oversample"], E, C, n, angles[:probe], w["maskradius"], dtype=np.float32)
+    per = r["loop_seconds"] / probe
+    k = int(max(2, min(len(angles), math.ceil(target_seconds / max(per, 1e-6)))))
+    log(f"cpu baseline: {per:.3f} s/orientation on {threads} threads, timing {k} orientations")
+    r = po.pick(P, p, w["oversample"], E, C, n, angles[:k], w["maskradius"], dtype=np.float32)
+    del torch
+    return {"value": n ** 3 * k / r["loop_seconds"], "unit": "voxel-orientations/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 block 256^3 x first {k} orientations, float32 oracle restatement, "
+                      f"orientation loop {r['loop_seconds']:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (orientation sharding across "
+              "GPUs is a design note, not built)", file=sys.stderr)
+        return 2
+    if args.copy_probe:
+        copy_probe()
+        return 0
+    import torch
+    import gtom_amd as gx
+    from gtom_amd import synth
+
+    t0 = time.time()
+    w = synth.cfg2(seed=args.seed)
+    log(f"synthetic workload built in {time.time() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    n, p = w["n"], w["p"]
+    nvol = w["blocks"].shape[0]
+    angles = w["angles"] if args.angles <= 0 else w["angles"][:args.angles]
+    H = (n // 2 + 1) * n * n
+    blocks = torch.from_numpy(w["blocks"]).to(dev)
+    exp_ft = gx.d_FFTR2C(blocks)
+    del blocks
+    projft = gx.prepare_projector(torch.from_numpy(w["template"]).to(dev), p)
+    ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=w["pixelsize"], Cs=2.7e-3, voltage=300e3, defocus=-float(d),
+                                       amplitude=0.07) for d in w["defoci"]], n, device=dev)
+    plan = gx.PickPlan(p, n, args.batch)
+    best = torch.empty((nvol, n, n, n), dtype=torch.float32, device=dev)
+    idx = torch.empty((nvol, n, n, n), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        gx.d_PickSubTomograms(plan, projft, w["oversample"], exp_ft, ctf, angles, w["maskradius"],
+                              ctf_volume_stride=H, best=best, bestidx=idx, stream=stream)
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    plan.enable_timing(True)
+    plan.timings(reset=True)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for i in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = ev0.elapsed_time(ev1)
+    ms, cnt = plan.timings(reset=True)
+    ms_per_step = total_ms / args.steps
+    work = nvol * n ** 3 * len(angles)
+    value = work / (ms_per_step / 1000.0)
+    log(f"{args.steps} steps, {ms_per_step:.1f} ms/step, {value:.4g} vo/s; stage ms {ms}, launches {cnt}")
+
+    # dominant kernel and its roofline
+    k = int(np.argmax(ms))
+    nb_full = plan.batch
+    nbatches = cnt[k]
+    # exact bytes summed over all launches of the stage in the timed region
+    per_vol = [algorithmic_bytes(k, n, min(nb_full, len(angles) - b0)) for b0 in range(0, len(angles), nb_full)]
+    total_bytes = sum(per_vol) * nvol * args.steps
+    assert nbatches == len(per_vol) * nvol * args.steps, (nbatches, len(per_vol))
+    avg_ms = ms[k] / nbatches
+    bytes_per_launch = total_bytes / nbatches
+    achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
+    roof = {"bound": "hbm", "kernel": STAGES[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "stage_ms_per_step": {STAGES[i]: round(ms[i] / args.steps, 2) for i in range(3)}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        w2 = dict(n=n, p=p, oversample=w["oversample"], maskradius=w["maskradius"], angles=angles,
+                  exp_ft=exp_ft, projft=projft, ctf=ctf)
+        cpu = cpu_baseline(w2, args.cpu_seconds)
+
+    out = {"metric": METRIC, "value": value, "unit": "voxel-orientations/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded noise + planted blob templates)",
+           "config": {"workload": "plain exhaustive search: 512x512x256 tomogram as 2 blocks of 256^3, 48^3 "
+                                  "template (projector 96^3, oversample 0.375), 9000 orientations (500 Fibonacci "
+                                  "directions x 18 psi), per-block 3D CTF, local-std radius 24",
+                      "blocks": nvol, "block": n, "orientations": len(angles), "batch": plan.batch,
+                      "search": "plain"},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    plan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -27,13 +27,37 @@ template <int DIR> __device__ __forceinline__ float2 mul_pj(float2 a) {
   return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
-// Twiddle table: tw[m] = exp(-2 pi i m / TW_LEN) evaluated in extended
+// Twiddle tables (one allocation): tw[m] = exp(-2 pi i m / 4096) for
+// m in [0, 4096) serves power-of-two lengths; tw[4096 + m] = exp(-2 pi i m / 3072)
+// for m in [0, 3072) serves lengths 3 * 2^k. Both are evaluated in extended
 // precision on the host and rounded to float.
 constexpr int TW_LEN = 4096;
+constexpr int TW3_LEN = 3072;
+constexpr int TW3_OFF = 4096;
+constexpr bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 template <int DIR, int N> __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) {
-  static_assert(TW_LEN % N == 0, "twiddle table too short");
-  float2 w = tw[(m & (N - 1)) * (TW_LEN / N)];
+  float2 w;
+  if constexpr (pow2(N)) {
+    static_assert(TW_LEN % N == 0, "twiddle table too short");
+    w = tw[(m & (N - 1)) * (TW_LEN / N)];
+  } else {
+    static_assert(TW3_LEN % N == 0, "length must be 3 * 2^k <= 3072");
+    w = tw[TW3_OFF + (m % N) * (TW3_LEN / N)];
+  }
   return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+// exp(DIR * 2 pi i m / R) for R dividing 24, from the 24th roots of unity.
+template <int DIR, int R> __device__ __forceinline__ float2 root24(int m) {
+  static_assert(24 % R == 0, "R must divide 24");
+  constexpr float c15 = 0.96592582628906828675f, s15 = 0.25881904510252076235f;
+  constexpr float c30 = 0.86602540378443864676f, h = 0.70710678118654752440f;
+  constexpr float C[24] = {1.f, c15, c30, h, 0.5f, s15, 0.f, -s15, -0.5f, -h, -c30, -c15,
+                           -1.f, -c15, -c30, -h, -0.5f, -s15, 0.f, s15, 0.5f, h, c30, c15};
+  constexpr float S[24] = {0.f, s15, 0.5f, h, c30, c15, 1.f, c15, c30, h, 0.5f, s15,
+                           0.f, -s15, -0.5f, -h, -c30, -c15, -1.f, -c15, -c30, -h, -0.5f, -s15};
+  const int k = (m * (24 / R)) % 24;
+  return make_float2(C[k], DIR * S[k]);
 }
 
 // ---- register DFTs, natural order in and out, unnormalised ----
@@ -88,16 +112,60 @@ template <int DIR> __device__ __forceinline__ void dft16(float2* a) {
     a[k + 8] = csub(e[k], o[k]);
   }
 }
+template <int DIR> __device__ __forceinline__ void dft3(float2& a0, float2& a1, float2& a2) {
+  constexpr float s3 = 0.86602540378443864676f;  // sin(2 pi / 3)
+  const float2 t = cadd(a1, a2);
+  const float2 m = make_float2(a0.x - 0.5f * t.x, a0.y - 0.5f * t.y);
+  const float2 u = mul_pj<DIR>(cscale(csub(a1, a2), s3));
+  a0 = cadd(a0, t);
+  a1 = cadd(m, u);
+  a2 = csub(m, u);
+}
+
+template <int R, int DIR> __device__ __forceinline__ void dft(float2* a);
+
+// Register DFT of length A*B: A-point DFTs over x[b + B*a], twiddles
+// W_{AB}^{b*k1}, then B-point DFTs; X[k1 + A*k2].
+template <int A, int B, int DIR> __device__ __forceinline__ void dft_mixed(float2* x) {
+  float2 y[B][A];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    float2 t[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) t[a] = x[b + B * a];
+    dft<A, DIR>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) y[b][k1] = (b * k1 == 0) ? t[k1] : cmul(t[k1], root24<DIR, A * B>(b * k1));
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) {
+    float2 t[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) t[b] = y[b][k1];
+    dft<B, DIR>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) x[k1 + A * k2] = t[k2];
+  }
+}
+
 template <int R, int DIR> __device__ __forceinline__ void dft(float2* a) {
   if constexpr (R == 1) {
   } else if constexpr (R == 2) {
     dft2<DIR>(a[0], a[1]);
+  } else if constexpr (R == 3) {
+    dft3<DIR>(a[0], a[1], a[2]);
   } else if constexpr (R == 4) {
     dft4<DIR>(a[0], a[1], a[2], a[3]);
+  } else if constexpr (R == 6) {
+    dft_mixed<2, 3, DIR>(a);
   } else if constexpr (R == 8) {
     dft8<DIR>(a);
+  } else if constexpr (R == 12) {
+    dft_mixed<4, 3, DIR>(a);
   } else if constexpr (R == 16) {
     dft16<DIR>(a);
+  } else if constexpr (R == 24) {
+    dft_mixed<8, 3, DIR>(a);
   } else {
     static_assert(R == 16, "unsupported radix");
   }
@@ -108,20 +176,30 @@ template <int R, int DIR> __device__ __forceinline__ void dft(float2* a) {
 // threads per column, R1 points per thread.
 template <int N> struct ColShape;
 template <> struct ColShape<8>   { static constexpr int R1 = 8,  R2 = 1,  TX = 64; };
+template <> struct ColShape<12>  { static constexpr int R1 = 12, R2 = 1,  TX = 64; };
 template <> struct ColShape<16>  { static constexpr int R1 = 16, R2 = 1,  TX = 64; };
+template <> struct ColShape<24>  { static constexpr int R1 = 24, R2 = 1,  TX = 64; };
 template <> struct ColShape<32>  { static constexpr int R1 = 8,  R2 = 4,  TX = 16; };
+template <> struct ColShape<48>  { static constexpr int R1 = 12, R2 = 4,  TX = 16; };
 template <> struct ColShape<64>  { static constexpr int R1 = 8,  R2 = 8,  TX = 16; };
+template <> struct ColShape<96>  { static constexpr int R1 = 24, R2 = 4,  TX = 16; };
 template <> struct ColShape<128> { static constexpr int R1 = 16, R2 = 8,  TX = 16; };
+template <> struct ColShape<192> { static constexpr int R1 = 24, R2 = 8,  TX = 16; };
 template <> struct ColShape<256> { static constexpr int R1 = 16, R2 = 16, TX = 16; };
 
 // Row transforms (contiguous axis, complex length NC): R2 threads per row,
 // R1 points per thread.
 template <int NC> struct RowShape;
 template <> struct RowShape<4>   { static constexpr int R1 = 4,  R2 = 1; };
+template <> struct RowShape<6>   { static constexpr int R1 = 6,  R2 = 1; };
 template <> struct RowShape<8>   { static constexpr int R1 = 8,  R2 = 1; };
+template <> struct RowShape<12>  { static constexpr int R1 = 12, R2 = 1; };
 template <> struct RowShape<16>  { static constexpr int R1 = 16, R2 = 1; };
+template <> struct RowShape<24>  { static constexpr int R1 = 12, R2 = 2; };
 template <> struct RowShape<32>  { static constexpr int R1 = 8,  R2 = 4; };
+template <> struct RowShape<48>  { static constexpr int R1 = 12, R2 = 4; };
 template <> struct RowShape<64>  { static constexpr int R1 = 8,  R2 = 8; };
+template <> struct RowShape<96>  { static constexpr int R1 = 24, R2 = 4; };
 template <> struct RowShape<128> { static constexpr int R1 = 16, R2 = 8; };
 
 // Index of the k-th output held by thread j after fft_two_step:

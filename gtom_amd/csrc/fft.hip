@@ -1,4 +1,4 @@
-// Batched R2C / C2R transforms of 3D, 2D and 1D volumes (power-of-two axes,
+// Batched R2C / C2R transforms of 3D, 2D and 1D volumes (axes 2^a or 3*2^a,
 // 8..256) for gfx950. Replace the cuFFT plans wrapped by d_FFTR2C and
 // d_IFFTC2R (src/FFT/FFT.cu:7-42, src/FFT/IFFT.cu:8-56).
 //
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_rows_r2c(const float* __restrict__ in, 
     for (int m = 0; m < R1; ++m) {
       const int k = j + R2 * m;
       const float2 zk = zrow[r * ZS + k];
-      const float2 zn = cconj(zrow[r * ZS + ((NC - k) & (NC - 1))]);
+      const float2 zn = cconj(zrow[r * ZS + ((NC - k) % NC)]);
       const float2 s = cadd(zk, zn), d = csub(zk, zn);
       X[k] = cscale(csub(s, mul_pj<1>(cmul(twiddle<-1, N>(tw, k), d))), 0.5f);
     }
@@ -135,12 +135,11 @@ int launch_cols_n(int n, float2* data, int width, long long axis_stride, long lo
     return GTOMX_OK;
   };
   switch (n) {
-    case 8: return go(k_cols<8, DIR>, ColShape<8>::TX, ColShape<8>::R2);
-    case 16: return go(k_cols<16, DIR>, ColShape<16>::TX, ColShape<16>::R2);
-    case 32: return go(k_cols<32, DIR>, ColShape<32>::TX, ColShape<32>::R2);
-    case 64: return go(k_cols<64, DIR>, ColShape<64>::TX, ColShape<64>::R2);
-    case 128: return go(k_cols<128, DIR>, ColShape<128>::TX, ColShape<128>::R2);
-    case 256: return go(k_cols<256, DIR>, ColShape<256>::TX, ColShape<256>::R2);
+#define GX_CASE(N) \
+  case N:          \
+    return go(k_cols<N, DIR>, ColShape<N>::TX, ColShape<N>::R2);
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
     default: return fail(GTOMX_ERR_UNSUPPORTED, "column FFT length not supported");
   }
 }
@@ -156,12 +155,11 @@ int launch_rows_c2r(int n, const float2* in, float* out, long long nrows, float 
     return GTOMX_OK;
   };
   switch (n) {
-    case 8: return go(k_rows_c2r<8>, rows_per_block<8>());
-    case 16: return go(k_rows_c2r<16>, rows_per_block<16>());
-    case 32: return go(k_rows_c2r<32>, rows_per_block<32>());
-    case 64: return go(k_rows_c2r<64>, rows_per_block<64>());
-    case 128: return go(k_rows_c2r<128>, rows_per_block<128>());
-    case 256: return go(k_rows_c2r<256>, rows_per_block<256>());
+#define GX_CASE(N) \
+  case N:          \
+    return go(k_rows_c2r<N>, rows_per_block<N>());
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
     default: return fail(GTOMX_ERR_UNSUPPORTED, "row C2R length not supported");
   }
 }
@@ -174,12 +172,11 @@ int launch_rows_r2c(int n, const float* in, float2* out, long long nrows, const 
     return GTOMX_OK;
   };
   switch (n) {
-    case 8: return go(k_rows_r2c<8>, rows_per_block<8>());
-    case 16: return go(k_rows_r2c<16>, rows_per_block<16>());
-    case 32: return go(k_rows_r2c<32>, rows_per_block<32>());
-    case 64: return go(k_rows_r2c<64>, rows_per_block<64>());
-    case 128: return go(k_rows_r2c<128>, rows_per_block<128>());
-    case 256: return go(k_rows_r2c<256>, rows_per_block<256>());
+#define GX_CASE(N) \
+  case N:          \
+    return go(k_rows_r2c<N>, rows_per_block<N>());
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
     default: return fail(GTOMX_ERR_UNSUPPORTED, "row R2C length not supported");
   }
 }

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
   const bool valid = ix < NH;
   const int y = signed_freq(iy, N);
   const bool herm = valid && (ix == 0 || ix == HALF);
-  const int iyp = (N - iy) & (N - 1);
+  const int iyp = (N - iy) % N;
   const int yp = signed_freq(iyp, N);
   const int nwg = gridDim.x * gridDim.y;
   const int wg = blockIdx.y * gridDim.x + blockIdx.x;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
     cf[m] = valid ? (ctf ? ctf[idx] : 1.f) : 0.f;
     cfp[m] = 0.f;
     if (herm) {
-      const int izp = (N - iz) & (N - 1);
+      const int izp = (N - iz) % N;
       cfp[m] = ctf ? ctf[((size_t)izp * N + iyp) * NH + ix] : 1.f;
     }
   }
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
       float2 r = make_float2(0.f, 0.f);
       if (valid && ix * ix + y * y + z * z <= RMAX2) r = cscale(project_sample(P, M, ix, y, z), cf[m]);
       if (herm) {
-        const int izp = (N - iz) & (N - 1);
+        const int izp = (N - iz) % N;
         const int zp = signed_freq(izp, N);
         float2 rp = make_float2(0.f, 0.f);
         if (ix * ix + yp * yp + zp * zp <= RMAX2) rp = cscale(project_sample(P, M, ix, yp, zp), cfp[m]);
@@ -279,12 +279,11 @@ template <int N> constexpr int col_tiles() { return (N / 2 + 1 + ColShape<N>::TX
 
 int tiles_for(int n) {
   switch (n) {
-    case 8: return col_tiles<8>();
-    case 16: return col_tiles<16>();
-    case 32: return col_tiles<32>();
-    case 64: return col_tiles<64>();
-    case 128: return col_tiles<128>();
-    case 256: return col_tiles<256>();
+#define GX_CASE(N) \
+  case N:          \
+    return col_tiles<N>();
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
   }
   return 0;
 }
@@ -344,12 +343,11 @@ int run_batch(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb
 int run_batch_n(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb, const float2* E,
                 const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
   switch (pl->n) {
-    case 8: return run_batch<8>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
-    case 16: return run_batch<16>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
-    case 32: return run_batch<32>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
-    case 64: return run_batch<64>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
-    case 128: return run_batch<128>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
-    case 256: return run_batch<256>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+#define GX_CASE(N) \
+  case N:          \
+    return run_batch<N>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
   }
   return fail(GTOMX_ERR_UNSUPPORTED, "volume side not supported");
 }
@@ -410,7 +408,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   if (!out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan pointer is NULL");
   *out = nullptr;
   if (dv.x != dv.y || dv.x != dv.z) return fail(GTOMX_ERR_UNSUPPORTED, "volumes must be cubes");
-  if (!fft_len_ok(dv.x)) return fail(GTOMX_ERR_UNSUPPORTED, "volume side must be a power of two in [8, 256]");
+  if (!fft_len_ok(dv.x)) return fail(GTOMX_ERR_UNSUPPORTED, "volume side must be 2^a or 3*2^a in [8, 256]");
   if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 1024)
     return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 1024");
   if (batch > 256) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 256");

@@ -65,16 +65,21 @@ int twiddles(const float2** table) {
   std::lock_guard<std::mutex> lk(g_tables_mu);
   if ((int)g_tables.size() <= dev) g_tables.resize(dev + 1);
   if (!g_tables[dev].tw) {
-    std::vector<float2> h(TW_LEN_HOST);
+    const int total = TW_LEN_HOST + TW3_LEN_HOST;
+    std::vector<float2> h(total);
     const long double twopi = 6.283185307179586476925286766559005768L;
     for (int m = 0; m < TW_LEN_HOST; ++m) {
       long double a = -twopi * (long double)m / (long double)TW_LEN_HOST;
       h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
+    for (int m = 0; m < TW3_LEN_HOST; ++m) {
+      long double a = -twopi * (long double)m / (long double)TW3_LEN_HOST;
+      h[TW_LEN_HOST + m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
     void* d = nullptr;
-    hipError_t e = hipMalloc(&d, sizeof(float2) * TW_LEN_HOST);
+    hipError_t e = hipMalloc(&d, sizeof(float2) * total);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
-    e = hipMemcpy(d, h.data(), sizeof(float2) * TW_LEN_HOST, hipMemcpyHostToDevice);
+    e = hipMemcpy(d, h.data(), sizeof(float2) * total, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       (void)hipFree(d);
       return hip_fail(e, "hipMemcpy(twiddles)");
@@ -91,9 +96,9 @@ int check_fft_dims(int ndims, gtomx_int3 d) {
   int want = 3 - (d.z == 1) - (d.z == 1 && d.y == 1);
   if (ndims != want)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "ndims does not match dims (trailing extents must be 1)");
-  if (!fft_len_ok(d.x)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.x must be a power of two in [8, 256]");
-  if (d.y != 1 && !fft_len_ok(d.y)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.y must be 1 or a power of two in [8, 256]");
-  if (d.z != 1 && !fft_len_ok(d.z)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.z must be 1 or a power of two in [8, 256]");
+  if (!fft_len_ok(d.x)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.x must be 2^a or 3*2^a in [8, 256]");
+  if (d.y != 1 && !fft_len_ok(d.y)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.y must be 1, 2^a or 3*2^a in [8, 256]");
+  if (d.z != 1 && !fft_len_ok(d.z)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.z must be 1, 2^a or 3*2^a in [8, 256]");
   return GTOMX_OK;
 }
 

@@ -12,8 +12,10 @@
 
 namespace gx {
 
-// Length of the device twiddle table (must equal TW_LEN in dev_fft.hpp).
+// Device twiddle tables (must match TW_LEN / TW3_LEN / TW3_OFF in dev_fft.hpp):
+// exp(-2 pi i m/4096) for m < 4096, then exp(-2 pi i m/3072) for m < 3072.
 constexpr int TW_LEN_HOST = 4096;
+constexpr int TW3_LEN_HOST = 3072;
 
 // Records a message for gtomx_last_error() and returns the status.
 int fail(int status, const std::string& msg);
@@ -51,8 +53,12 @@ int twiddles(const float2** table);
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-// Axis extent accepted by the FFT kernels: power of two in [8, 256].
-inline bool fft_len_ok(int n) { return is_pow2(n) && n >= 8 && n <= 256; }
+// Axis extents accepted by the FFT kernels: 2^a or 3 * 2^a in [8, 256].
+inline bool fft_len_ok(int n) {
+  return n >= 8 && n <= 256 && (is_pow2(n) || (n % 3 == 0 && is_pow2(n / 3)));
+}
+// Expands X(n) for every supported transform length.
+#define GX_FOR_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
 // Validates dims for an ndims-dimensional transform; returns GTOMX_OK or an error.
 int check_fft_dims(int ndims, gtomx_int3 dims);

@@ -10,7 +10,7 @@ FFT_TOL = 2e-6  # normwise relative, FP32 transform vs FP64
 
 
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (8, 16, 32), (32, 32, 32), (64, 64, 64),
-                                   (128, 32, 16), (16, 256, 8)])
+                                   (128, 32, 16), (16, 256, 8), (96, 96, 96), (12, 24, 48), (192, 8, 24)])
 def test_fft_r2c_matches_oracle(gx, orc, cuda, shape):
     import torch
     rng = np.random.default_rng(hash(shape) % 2**32)
@@ -31,7 +31,8 @@ def test_fft_r2c_2d_1d(gx, orc, cuda, shape):
     assert rel_linf(to_np(X), ref) < FFT_TOL
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 8), (64, 64, 64), (8, 128, 256)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 8), (64, 64, 64), (8, 128, 256), (96, 48, 96),
+                                   (24, 192, 12)])
 def test_fft_c2r_nonhermitian_matches_oracle(gx, orc, cuda, shape):
     """Arbitrary (non-Hermitian) half spectra: the result is the real part of the completed inverse."""
     import torch
