@@ -1,0 +1,160 @@
+"""Pins the oracle's Euler matrices, orientation lists and projector."""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+
+def rz(t):  # passive rotation about z, RELION convention
+    c, s = math.cos(t), math.sin(t)
+    return np.array([[c, s, 0], [-s, c, 0], [0, 0, 1.0]])
+
+
+def ry(t):
+    c, s = math.cos(t), math.sin(t)
+    return np.array([[c, 0, -s], [0, 1, 0], [s, 0, c]])
+
+
+@pytest.mark.parametrize("ang", [(0.3, 1.1, -2.0), (np.deg2rad(-13), np.deg2rad(-20), np.deg2rad(-161)),
+                                 (0, 0, 0), (2.5, np.pi, 0.7)])
+def test_euler_is_zyz_composition(orc, ang):
+    A = orc.euler_matrix(*ang)
+    assert np.allclose(A, rz(ang[2]) @ ry(ang[1]) @ rz(ang[0]), atol=1e-15)
+    assert np.allclose(A @ A.T, np.eye(3), atol=1e-14)
+    assert math.isclose(np.linalg.det(A), 1.0, rel_tol=1e-14)
+
+
+def test_euler_known_vector():
+    """Matrix3Euler(0, 20 deg, 0) applied to x (GTOMTest/Euler.cpp:17-19) gives (cos 20, 0, sin 20)."""
+    from oracle import pyoracle as orc
+    A = orc.euler_matrix(0.0, np.deg2rad(20), 0.0)
+    assert np.allclose(A @ [1, 0, 0], [math.cos(np.deg2rad(20)), 0, math.sin(np.deg2rad(20))], atol=1e-15)
+
+
+@pytest.mark.parametrize("deg,count", [(30, 588), (15, 4536), (12.5, 7616)])
+def test_equal_angular_spacing_counts(orc, deg, count):
+    a = orc.equal_angular_spacing((0, 2 * np.pi), (0, np.pi), (0, 2 * np.pi), np.deg2rad(deg))
+    assert a.shape == (count, 3)
+    # first ring is the pole with a single rot step; psi runs fastest
+    npsi = int(math.floor((360 - deg) / deg + 1e-6)) + 1  # psi in [0, 2 pi - increment]
+    assert np.all(a[:npsi, 1] == 0) and np.all(a[:npsi, 0] == 0)
+    assert a[npsi, 1] > 0
+    assert np.allclose(np.diff(a[:npsi, 2]), np.float32(np.deg2rad(deg)), atol=1e-6)
+
+
+def test_projector_matrix_is_scaled_transpose(orc):
+    ang = (0.4, 2.2, -1.3)
+    M = orc.projector_matrix(*ang, 0.375)
+    assert np.allclose(M, 0.375 * orc.euler_matrix(*ang).T, atol=1e-16)
+
+
+def _projector(orc, rng, p):
+    return orc.r2c(rng.standard_normal((p, p, p)))
+
+
+def _signed(i, n):
+    return i if i <= n // 2 else i - n
+
+
+def test_project_identity_returns_stored_bins(orc):
+    rng = np.random.default_rng(3)
+    n = p = 16
+    P = _projector(orc, rng, p)
+    out = orc.project(P, p, n, np.eye(3))
+    for iz in range(n):
+        for iy in range(n):
+            for ix in range(n // 2 + 1):
+                z, y = _signed(iz, n), _signed(iy, n)
+                if ix * ix + y * y + z * z > (n // 2) ** 2:
+                    assert out[iz, iy, ix] == 0
+                else:
+                    assert out[iz, iy, ix] == P[iz, iy, ix]
+
+
+def test_project_quarter_turn_permutes_bins(orc):
+    """rot = 90 deg about z maps q = (k_y, -k_x, k_z); negative q_x is taken from the Hermitian mate."""
+    rng = np.random.default_rng(4)
+    n = p = 16
+    P = _projector(orc, rng, p)
+    M = orc.projector_matrix(np.pi / 2, 0.0, 0.0, 1.0)
+    out = orc.project(P, p, n, M)
+    for iz in range(n):
+        for iy in range(n):
+            for ix in range(n // 2 + 1):
+                x, y, z = ix, _signed(iy, n), _signed(iz, n)
+                if x * x + y * y + z * z > (n // 2) ** 2:
+                    continue
+                q = np.rint(M @ [x, y, z]).astype(int)
+                if q[0] < 0:
+                    ref = np.conj(P[(-q[2]) % p, (-q[1]) % p, -q[0]])
+                else:
+                    ref = P[q[2] % p, q[1] % p, q[0]]
+                assert abs(out[iz, iy, ix] - ref) < 1e-12 * abs(P).max()
+
+
+def test_project_matches_scipy_trilinear(orc):
+    """Independent trilinear interpolation (scipy, periodic in y/z) of the projector at q = M k."""
+    rng = np.random.default_rng(5)
+    n, p = 16, 32
+    P = _projector(orc, rng, p)
+    M = orc.projector_matrix(0.7, 1.2, -0.4, 1.7)  # |q| <= 1.7 * 8 < p/2 - 1: no x wrap
+    out = orc.project(P, p, n, M)
+    iz, iy, ix = np.meshgrid(np.arange(n), np.arange(n), np.arange(n // 2 + 1), indexing="ij")
+    z = np.where(iz <= n // 2, iz, iz - n)
+    y = np.where(iy <= n // 2, iy, iy - n)
+    x = ix
+    inside = x * x + y * y + z * z <= (n // 2) ** 2
+    q = np.einsum("ij,jzyx->izyx", M, np.stack([x, y, z]).astype(float))
+    flip = q[0] < -1e-5
+    q = np.where(flip[None], -q, q)
+    coords = np.stack([np.mod(q[2], p), np.mod(q[1], p), q[0]]).reshape(3, -1)
+    # extend x by one column so that x1 = p/2 + 1 never leaves the array (weights are ~0 there)
+    Pe = np.concatenate([P, P[:, :, :1]], axis=2)
+    re = ndimage.map_coordinates(Pe.real, coords, order=1, mode="grid-wrap").reshape(q.shape[1:])
+    im = ndimage.map_coordinates(Pe.imag, coords, order=1, mode="grid-wrap").reshape(q.shape[1:])
+    ref = np.where(flip, re - 1j * im, re + 1j * im)
+    ref = np.where(inside, ref, 0)
+    assert np.abs(out - ref).max() < 1e-10 * np.abs(P).max()
+
+
+def _gauss_ft_template(orc, n, A, sig):
+    """R2C(ifftshift(t_rot)) for t_rot(r) = g(A^T r), g an anisotropic Gaussian at the box centre."""
+    c = n // 2
+    g = np.arange(n) - c
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X, Y, Z]).reshape(3, -1)
+    src = A.T @ pts
+    v = np.exp(-0.5 * ((src[0] / sig[0]) ** 2 + (src[1] / sig[1]) ** 2 + (src[2] / sig[2]) ** 2))
+    return orc.r2c(np.fft.ifftshift(v.reshape(n, n, n)))
+
+
+def test_project_rotated_gaussian_matches_closed_form(orc):
+    """Projecting the oversampled FT of an anisotropic Gaussian at rotation A equals the FT of the
+    Gaussian rotated in real space by A (t(A^T r)), within interpolation error; the transposed
+    rotation does not. Pins the rotation direction and the oversampling factor."""
+    n, p = 32, 64
+    sig = (1.6, 2.4, 3.6)
+    ang = (0.6, 1.0, 2.1)
+    A = orc.euler_matrix(*ang)
+    P = _gauss_ft_template(orc, p, np.eye(3), sig)  # unrotated template in the p box
+    out = orc.project(P, p, n, orc.projector_matrix(*ang, p / n))
+    ref = _gauss_ft_template(orc, n, A, sig)
+    iz, iy, ix = np.meshgrid(np.arange(n), np.arange(n), np.arange(n // 2 + 1), indexing="ij")
+    z = np.where(iz <= n // 2, iz, iz - n)
+    y = np.where(iy <= n // 2, iy, iy - n)
+    inside = ix ** 2 + y ** 2 + z ** 2 <= (n // 2) ** 2
+    err = np.abs(out - ref)[inside].max() / np.abs(ref).max()
+    assert err < 0.02, err
+    wrong = _gauss_ft_template(orc, n, A.T, sig)
+    err_t = np.abs(out - wrong)[inside].max() / np.abs(ref).max()
+    assert err_t > 0.2, err_t
+
+
+def test_project_reads_next_row_with_zero_weight(orc):
+    """q_x = p/2 exactly: x1 = p/2 + 1 is read at the next linear address with weight 0."""
+    rng = np.random.default_rng(6)
+    n = p = 16
+    P = _projector(orc, rng, p)
+    out = orc.project(P, p, n, np.eye(3))
+    assert out[0, 0, n // 2] == P[0, 0, p // 2]
