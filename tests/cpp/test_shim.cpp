@@ -1,0 +1,133 @@
+// Exercises the header-only gtom:: drop-in (include/gtom_pick.hpp) from C++,
+// the way a host program written against the reference API would call it.
+// Prints one line per case and exits non-zero on the first failure.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "gtom_pick.hpp"
+
+using namespace gtom;
+
+#define EXPECT(cond, ...)                          \
+  do {                                             \
+    if (!(cond)) {                                 \
+      std::printf("FAIL %s:%d: ", __FILE__, __LINE__); \
+      std::printf(__VA_ARGS__);                    \
+      std::printf("\n");                           \
+      std::exit(1);                                \
+    }                                              \
+  } while (0)
+
+template <class T> T* dev(size_t n) {
+  T* p = nullptr;
+  EXPECT(hipMalloc((void**)&p, n * sizeof(T)) == hipSuccess, "hipMalloc");
+  return p;
+}
+
+static void fft_roundtrip() {
+  int3 dims = make_int3(32, 16, 8);
+  size_t V = 32 * 16 * 8, H = 17 * 16 * 8;
+  std::vector<float> h(V);
+  std::mt19937 rng(1);
+  std::normal_distribution<float> nd;
+  for (auto& v : h) v = nd(rng);
+  float* d_x = dev<float>(V);
+  float* d_y = dev<float>(V);
+  tcomplex* d_X = dev<tcomplex>(H);
+  hipMemcpy(d_x, h.data(), V * sizeof(float), hipMemcpyHostToDevice);
+  d_FFTR2C(d_x, d_X, 3, dims);
+  d_IFFTC2R(d_X, d_y, 3, dims);  // renormalised
+  std::vector<float> back(V);
+  hipMemcpy(back.data(), d_y, V * sizeof(float), hipMemcpyDeviceToHost);
+  double err = 0;
+  for (size_t i = 0; i < V; ++i) err = std::max(err, (double)std::fabs(back[i] - h[i]));
+  EXPECT(err < 1e-5, "round trip error %g", err);
+  std::printf("ok fft_roundtrip (max error %.2e)\n", err);
+  hipFree(d_x); hipFree(d_y); hipFree(d_X);
+}
+
+// The centred template moved by a known circular shift s is found at its
+// centre n/2 + s with the identity orientation (index 0 of the 30-degree list).
+static void pick_finds_shift() {
+  const int n = 32, box = 16, p = 32;
+  std::vector<float> tmpl((size_t)box * box * box);
+  for (int z = 0; z < box; ++z)
+    for (int y = 0; y < box; ++y)
+      for (int x = 0; x < box; ++x) {
+        double dx = x - box / 2 - 1.5, dy = y - box / 2 + 1.0, dz = z - box / 2;
+        tmpl[((size_t)z * box + y) * box + x] =
+            (float)(std::exp(-(dx * dx / 4 + dy * dy / 9 + dz * dz / 2.5) / 2) +
+                    0.5 * std::exp(-((dx - 3) * (dx - 3) + dy * dy + (dz + 2) * (dz + 2)) / 3));
+      }
+  float* d_t = dev<float>(tmpl.size());
+  hipMemcpy(d_t, tmpl.data(), tmpl.size() * sizeof(float), hipMemcpyHostToDevice);
+  float* d_pad = dev<float>((size_t)p * p * p);
+  float* d_shift = dev<float>((size_t)p * p * p);
+  tcomplex* d_proj = dev<tcomplex>((size_t)(p / 2 + 1) * p * p);
+  d_Pad(d_t, d_pad, make_int3(box, box, box), make_int3(p, p, p), 0.f);
+  d_RemapFull2FullFFT(d_pad, d_shift, make_int3(p, p, p));
+  d_FFTR2C(d_shift, d_proj, 3, make_int3(p, p, p));
+
+  // tomogram: the padded (centred) template moved by (sx, sy, sz)
+  std::vector<float> centred((size_t)p * p * p), tomo((size_t)n * n * n);
+  hipMemcpy(centred.data(), d_pad, centred.size() * sizeof(float), hipMemcpyDeviceToHost);
+  const int sx = 5, sy = -3, sz = 7;
+  for (int z = 0; z < n; ++z)
+    for (int y = 0; y < n; ++y)
+      for (int x = 0; x < n; ++x)
+        tomo[((size_t)((z + sz + n) % n) * n + (y + sy + n) % n) * n + (x + sx + n) % n] =
+            centred[((size_t)z * n + y) * n + x];
+  float* d_tomo = dev<float>(tomo.size());
+  hipMemcpy(d_tomo, tomo.data(), tomo.size() * sizeof(float), hipMemcpyHostToDevice);
+  tcomplex* d_E = dev<tcomplex>((size_t)(n / 2 + 1) * n * n);
+  d_FFTR2C(d_tomo, d_E, 3, make_int3(n, n, n));
+  std::vector<float> ctf((size_t)(n / 2 + 1) * n * n, 1.f);
+  float* d_ctf = dev<float>(ctf.size());
+  hipMemcpy(d_ctf, ctf.data(), ctf.size() * sizeof(float), hipMemcpyHostToDevice);
+
+  const float pi = 3.14159265358979f;
+  std::vector<float3> angles = GetEqualAngularSpacing(make_float2(0, 2 * pi), make_float2(0, pi),
+                                                      make_float2(0, 2 * pi), 30.f * pi / 180.f);
+  EXPECT(angles.size() == 588, "angle count %zu", angles.size());
+  float* d_best = dev<float>((size_t)n * n * n);
+  float* d_angle = dev<float>((size_t)n * n * n);
+  d_PickSubTomograms(d_proj, (float)p / n, make_int3(p, p, p), d_E, d_ctf, make_int3(n, n, n), 1, angles.data(),
+                     (uint)angles.size(), 6.f, d_best, d_angle);
+  std::vector<float> best((size_t)n * n * n), ang((size_t)n * n * n);
+  hipMemcpy(best.data(), d_best, best.size() * sizeof(float), hipMemcpyDeviceToHost);
+  hipMemcpy(ang.data(), d_angle, ang.size() * sizeof(float), hipMemcpyDeviceToHost);
+  size_t arg = 0;
+  for (size_t i = 1; i < best.size(); ++i)
+    if (best[i] > best[arg]) arg = i;
+  int ax = (int)(arg % n), ay = (int)((arg / n) % n), az = (int)(arg / ((size_t)n * n));
+  EXPECT(ax == (n / 2 + sx + n) % n && ay == (n / 2 + sy + n) % n && az == (n / 2 + sz + n) % n,
+         "peak at (%d,%d,%d)", ax, ay, az);
+  EXPECT(ang[arg] == 0.f, "orientation index %g", ang[arg]);
+  std::printf("ok pick_finds_shift (peak %.3f at (%d,%d,%d), orientation %g)\n", best[arg], ax, ay, az, ang[arg]);
+  hipFree(d_t); hipFree(d_pad); hipFree(d_shift); hipFree(d_proj); hipFree(d_tomo); hipFree(d_E);
+  hipFree(d_ctf); hipFree(d_best); hipFree(d_angle);
+}
+
+static void errors_throw() {
+  bool threw = false;
+  try {
+    d_FFTR2C(nullptr, nullptr, 3, make_int3(100, 100, 100));
+  } catch (const std::runtime_error& e) {
+    threw = true;
+  }
+  EXPECT(threw, "unsupported size did not throw");
+  std::printf("ok errors_throw\n");
+}
+
+int main() {
+  fft_roundtrip();
+  pick_finds_shift();
+  errors_throw();
+  std::printf("all passed\n");
+  return 0;
+}
