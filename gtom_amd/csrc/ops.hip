@@ -189,6 +189,7 @@ __global__ void k_project(Projector P, float2* __restrict__ out, int n, int rmax
 #pragma unroll
   for (int k = 0; k < 9; ++k) M[k] = mats[9 * b + k];
   float2* o = out + b * H;
+  const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
   for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < H; id += (size_t)gridDim.x * blockDim.x) {
     int ix = (int)(id % nh), iy = (int)((id / nh) % n), iz = (int)(id / ((size_t)nh * n));
     int x = ix, y = signed_freq(iy, n), z = signed_freq(iz, n);
@@ -196,7 +197,7 @@ __global__ void k_project(Projector P, float2* __restrict__ out, int n, int rmax
       o[id] = make_float2(0.f, 0.f);
       continue;
     }
-    o[id] = project_sample(P, M, x, y, z);
+    o[id] = project_sample(rs, P.p, P.ph, M, x, y, z);
   }
 }
 
@@ -317,7 +318,7 @@ int local_std_apply(const LocalStdScratch& sc, const float* map, gtomx_int3 d, f
 
 int project_launch(const float2* vol, int p, float2* out, int n, unsigned rmax, const float* d_mats,
                    unsigned batch, hipStream_t s) {
-  Projector P{vol, p, p / 2 + 1, (p / 2 + 1) * p * p - 1};
+  Projector P{vol, p, p / 2 + 1, (p / 2 + 1) * p * p * 8};
   size_t H = (size_t)(n / 2 + 1) * n * n;
   unsigned gx_ = (unsigned)std::min<size_t>((H + TPB - 1) / TPB, 4096);
   hipLaunchKernelGGL(k_project, dim3(gx_, batch), dim3(TPB), 0, s, P, out, n, (int)(rmax * rmax), d_mats, batch);
@@ -367,8 +368,8 @@ int gtomx_project(const gtomx_complex* d_volumeft, gtomx_int3 dimsvolume, gtomx_
   if (!d_volumeft || !d_proj || !d_matrices || batch == 0)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer or zero batch");
   if (dimsvolume.x != dimsvolume.y || dimsvolume.x != dimsvolume.z || dimsvolume.x < 2 || dimsvolume.x % 2 ||
-      dimsvolume.x > 1024)
-    return fail(GTOMX_ERR_UNSUPPORTED, "projector volume must be an even cube of side <= 1024");
+      dimsvolume.x > 512)
+    return fail(GTOMX_ERR_UNSUPPORTED, "projector volume must be an even cube of side <= 512");
   if (dimsproj.x != dimsproj.y || dimsproj.x != dimsproj.z || dimsproj.x < 2 || dimsproj.x % 2 || dimsproj.x > 1024)
     return fail(GTOMX_ERR_UNSUPPORTED, "projections must be 3D even cubes of side <= 1024 (2D is not built)");
   rmax = std::min<unsigned>(rmax, (unsigned)dimsproj.x / 2);  // src/Relion/Project.cu:32

@@ -5,15 +5,15 @@
 //  pass 1 (k_pick_project_z): for a tile of TX x-columns at one y and all z:
 //     R = project(projector, M_o) * ctf        (src/Relion/Project.cu:89-177,
 //                                               src/Generics/Arithmetics.cu:275-286)
-//     R' = Herm(R) on the x = 0 and x = n/2 planes, DC = 0, and the partial
-//     sum of |R'|^2 for the unit-RMS normalisation; this is the Fourier form of
-//     C2R -> mean0/std1 -> R2C -> /RMS (src/Correlation/SubTomograms.cu:66-70,
-//     src/ImageManipulation/Norm.cu:242-284, 476-509)
-//     Q = E_v * conj(R')                       (src/Correlation/SubTomograms.cu:127-136)
-//     inverse DFT along z, written to the batch buffer G[o].
-//     E_v and the CTF are read once per batch and kept in registers.
+//     Q = E_v * conj(R) with DC = 0            (src/Correlation/SubTomograms.cu:127-136)
+//     and the partial sums of |R|^2; inverse DFT along z, written to the
+//     batch buffer G[o].
+//  k_pick_herm_norm: the correction of those sums on the x = 0 and x = n/2
+//     planes, so that together they are sum |R'|^2 for R' = Herm(R) with DC
+//     zeroed: the Fourier form of C2R -> mean0/std1 -> R2C -> /RMS
+//     (src/Correlation/SubTomograms.cu:66-70, src/ImageManipulation/Norm.cu:242-284, 476-509).
 //  pass 2 (k_pick_y): inverse DFT along y in place; one workgroup per
-//     orientation also reduces the partial sums to the scale 1/RMS.
+//     orientation also reduces the sums to the scale 1/RMS.
 //  pass 3 (k_pick_x_max): C2R along x (real part of the Hermitian-completed
 //     inverse), times the scale, then the running max/argmax with strict '>'
 //     in increasing orientation index (src/Correlation/SubTomograms.cu:138-158)
@@ -31,8 +31,23 @@
 
 namespace gx {
 
+// Row pitch (complex elements) of the batch buffer G: the half-spectrum width
+// n/2+1 rounded up to whole column tiles, so that every tile row starts on a
+// 128-byte boundary. Columns past n/2 are never read or written.
+template <int N> constexpr int g_pitch() {
+  return (N / 2 + 1 + ColShape<N>::TX - 1) / ColShape<N>::TX * ColShape<N>::TX;
+}
+template <int N> constexpr size_t g_stride() { return (size_t)g_pitch<N>() * N * N; }
+
 // ---------------------------------------------------------------------------
 // pass 1
+//
+// The Hermitian projection of the x = 0 and x = n/2 planes does not change
+// C2R(E * conj R) when E is the R2C of a real volume (C2R keeps the real part
+// of the completed inverse, and Herm(E conj R) = E conj Herm(R)), so the
+// samples here are not projected; only the unit-RMS sum needs Herm(R), and
+// k_pick_herm_norm supplies its correction on the two planes. The DC bin is
+// zeroed (the reference subtracts the mean).
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_project_z(
@@ -41,8 +56,10 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
     const float2* __restrict__ tw) {
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
-  constexpr size_t H = (size_t)NH * N * N;
+  constexpr int PITCH = g_pitch<N>();
   constexpr int NT = TX * R2;
+  constexpr int GRP = 4;  // projector samples per group of in-flight gathers
+  static_assert(R1 % GRP == 0, "R1 must be a multiple of the sample group");
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
   __shared__ float red[(NT + 63) / 64];
 
@@ -51,56 +68,71 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
   const int iy = blockIdx.y;
   const bool valid = ix < NH;
   const int y = signed_freq(iy, N);
-  const bool herm = valid && (ix == 0 || ix == HALF);
-  const int iyp = (N - iy) % N;
-  const int yp = signed_freq(iyp, N);
   const int nwg = gridDim.x * gridDim.y;
   const int wg = blockIdx.y * gridDim.x + blockIdx.x;
-
-  float2 e[R1];
-  float cf[R1], cfp[R1];
-#pragma unroll
-  for (int m = 0; m < R1; ++m) {
-    const int iz = j + R2 * m;
-    const size_t idx = ((size_t)iz * N + iy) * NH + ix;
-    e[m] = valid ? E[idx] : make_float2(0.f, 0.f);
-    cf[m] = valid ? (ctf ? ctf[idx] : 1.f) : 0.f;
-    cfp[m] = 0.f;
-    if (herm) {
-      const int izp = (N - iz) % N;
-      cfp[m] = ctf ? ctf[((size_t)izp * N + iyp) * NH + ix] : 1.f;
-    }
-  }
+  const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
+  // Element (x, y, z) of E or the CTF sits at (z*N + y)*NH + x; the z part of
+  // this thread's bins is the wave-uniform constant m*R2*N*NH. E and the CTF
+  // are re-read for every orientation; the workgroup's tile stays in L2.
+  constexpr unsigned HB = (unsigned)(NH * N * N);
+  const __amdgpu_buffer_rsrc_t rE = make_rsrc(E, HB * 8u);
+  const __amdgpu_buffer_rsrc_t rC = make_rsrc(ctf, ctf ? HB * 4u : 0u);
+  const int ebase = valid ? (j * N + iy) * NH + ix : (int)HB;  // invalid lanes read nothing (0)
+  const bool dc_thread = ix == 0 && iy == 0 && j == 0;        // holds the DC bin as m = 0
+  // Output: element (x, y, z) of G[o] at (z*N + y)*PITCH + x.
+  const int gbase = (j * (R1 / R2) * N + iy) * PITCH + ix;
 
   for (int o = 0; o < nb; ++o) {
     float M[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+    // q = M (x, y, z) = q0 + z * M[:, 2]
+    const float q0x = M[0] * ix + M[1] * y, q0y = M[3] * ix + M[4] * y, q0z = M[6] * ix + M[7] * y;
     float acc = 0.f;
     float2 v[R1];
+    // Samples in groups of GRP: the group's gathers are in flight together, the
+    // scheduling barrier keeps the next group's loads from being hoisted (which
+    // would exhaust the registers), and a group whose bins all lie outside the
+    // sphere in this wave is skipped.
 #pragma unroll
-    for (int m = 0; m < R1; ++m) {
-      const int iz = j + R2 * m;
-      const int z = signed_freq(iz, N);
-      float2 r = make_float2(0.f, 0.f);
-      if (valid && ix * ix + y * y + z * z <= RMAX2) r = cscale(project_sample(P, M, ix, y, z), cf[m]);
-      if (herm) {
-        const int izp = (N - iz) % N;
-        const int zp = signed_freq(izp, N);
-        float2 rp = make_float2(0.f, 0.f);
-        if (ix * ix + yp * yp + zp * zp <= RMAX2) rp = cscale(project_sample(P, M, ix, yp, zp), cfp[m]);
-        r = make_float2(0.5f * (r.x + rp.x), 0.5f * (r.y - rp.y));
-        if (ix == 0 && iy == 0 && iz == 0) r = make_float2(0.f, 0.f);
+    for (int g = 0; g < R1 / GRP; ++g) {
+      bool ins[GRP];
+      bool any = false;
+#pragma unroll
+      for (int u = 0; u < GRP; ++u) {
+        const int z = signed_freq(j + R2 * (g * GRP + u), N);
+        ins[u] = valid && ix * ix + y * y + z * z <= RMAX2;
+        any |= ins[u];
       }
-      acc += r.x * r.x + r.y * r.y;
-      v[m] = cmulc(e[m], r);
+      if (__any(any)) {
+#pragma unroll
+        for (int u = 0; u < GRP; ++u) {
+          const int m = g * GRP + u;
+          const float z = (float)signed_freq(j + R2 * m, N);
+          const float cm = ctf ? buf_load1(rC, ebase * 4, m * R2 * N * NH * 4) : 1.f;
+          const float2 r = cscale(project_at(rs, P.p, P.ph, fmaf(M[2], z, q0x), fmaf(M[5], z, q0y),
+                                             fmaf(M[8], z, q0z), ins[u]), cm);
+          const float2 e = buf_load2(rE, ebase * 8, m * R2 * N * NH * 8);
+          acc += r.x * r.x + r.y * r.y;
+          v[m] = cmulc(e, r);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < GRP; ++u) v[g * GRP + u] = make_float2(0.f, 0.f);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
+    // DC of Q is zero (the reference's mean subtraction); its |R|^2 term is
+    // removed from the sum by k_pick_herm_norm.
+    if (dc_thread) v[0] = make_float2(0.f, 0.f);
     __syncthreads();
     fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
-    if (valid) {
-      float2* Go = G + (size_t)o * H;
+    // Columns past n/2 (invalid lanes) land in the row padding, which is never read.
+    const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * g_stride<N>(), (unsigned)(g_stride<N>() * 8));
 #pragma unroll
-      for (int s = 0; s < R1; ++s) Go[((size_t)out_index<R1, R2>(j, s) * N + iy) * NH + ix] = v[s];
+    for (int s = 0; s < R1; ++s) {
+      const int u = s / R2, k2 = s % R2;  // out_index(j, s) = j*(R1/R2) + u + R1*k2
+      buf_store2(v[s], rG, gbase * 8, (u + R1 * k2) * N * PITCH * 8);
     }
     const float tot = block_sum(acc, red);
     if (t == 0) partial[(size_t)o * nwg + wg] = tot;
@@ -108,15 +140,58 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
 }
 
 // ---------------------------------------------------------------------------
+// Unit-RMS correction on the x = 0 and x = n/2 planes: for every bin k of the
+// two planes, |Herm R(k)|^2 - |R(k)|^2 with Herm R(k) = (R(k) + conj R(-k))/2
+// (R(-k) sampled at the mirrored stored bin) and Herm R(0) = 0. Added to the
+// pass-1 sums it gives sum |R'|^2 for R' = Herm(R) with DC zeroed, the
+// spectrum that C2R -> mean0/std1 -> R2C produces in the reference.
+// ---------------------------------------------------------------------------
+constexpr int HERM_BLOCKS = 32;
+
+template <int N>
+__global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float* __restrict__ mats,
+                                                        const float* __restrict__ ctf, float* __restrict__ corr) {
+  constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
+  __shared__ float red[4];
+  const int o = blockIdx.y;
+  float M[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+  const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
+  float acc = 0.f;
+  for (int b = blockIdx.x * 256 + threadIdx.x; b < 2 * N * N; b += gridDim.x * 256) {
+    const int ix = b < N * N ? 0 : HALF;
+    const int r = b % (N * N), iy = r % N, iz = r / N;
+    const int y = signed_freq(iy, N), z = signed_freq(iz, N);
+    const int iyp = (N - iy) % N, izp = (N - iz) % N;
+    const int yp = signed_freq(iyp, N), zp = signed_freq(izp, N);
+    const bool in = ix * ix + y * y + z * z <= RMAX2;  // the same radius for k and -k
+    const float c = ctf ? ctf[((size_t)iz * N + iy) * NH + ix] : 1.f;
+    const float cp = ctf ? ctf[((size_t)izp * N + iyp) * NH + ix] : 1.f;
+    const float2 R = cscale(project_at(rs, P.p, P.ph, M[0] * ix + M[1] * y + M[2] * z,
+                                       M[3] * ix + M[4] * y + M[5] * z, M[6] * ix + M[7] * y + M[8] * z, in), c);
+    const float2 Rp = cscale(project_at(rs, P.p, P.ph, M[0] * ix + M[1] * yp + M[2] * zp,
+                                        M[3] * ix + M[4] * yp + M[5] * zp, M[6] * ix + M[7] * yp + M[8] * zp, in),
+                             cp);
+    float2 h = make_float2(0.5f * (R.x + Rp.x), 0.5f * (R.y - Rp.y));
+    if (b == 0) h = make_float2(0.f, 0.f);  // DC
+    acc += (h.x * h.x + h.y * h.y) - (R.x * R.x + R.y * R.y);
+  }
+  const float tot = block_sum(acc, red);
+  if (threadIdx.x == 0) corr[(size_t)o * gridDim.x + blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
 // pass 2
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
-    float2* __restrict__ G, const float* __restrict__ partial, int nwg1, float* __restrict__ scale,
-    const float2* __restrict__ tw) {
+    float2* __restrict__ G, const float* __restrict__ partial, int nwg1, const float* __restrict__ corr,
+    float* __restrict__ scale, const float2* __restrict__ tw) {
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1;
   constexpr size_t H = (size_t)NH * N * N;
+  constexpr int PITCH = g_pitch<N>();
   constexpr int NT = TX * R2;
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
   __shared__ double dred[NT];
@@ -125,19 +200,20 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
   const int ix = blockIdx.x * TX + c;
   const int z = blockIdx.y, o = blockIdx.z;
   const bool valid = ix < NH;
-  float2* base = G + (size_t)o * H + (size_t)z * N * NH + ix;
+  float2* base = G + (size_t)o * g_stride<N>() + (size_t)z * N * PITCH + ix;
   float2 v[R1];
 #pragma unroll
-  for (int m = 0; m < R1; ++m) v[m] = valid ? base[(size_t)(j + R2 * m) * NH] : make_float2(0.f, 0.f);
+  for (int m = 0; m < R1; ++m) v[m] = valid ? base[(size_t)(j + R2 * m) * PITCH] : make_float2(0.f, 0.f);
   fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
   if (valid) {
 #pragma unroll
-    for (int s = 0; s < R1; ++s) base[(size_t)out_index<R1, R2>(j, s) * NH] = v[s];
+    for (int s = 0; s < R1; ++s) base[(size_t)out_index<R1, R2>(j, s) * PITCH] = v[s];
   }
   if (blockIdx.x == 0 && blockIdx.y == 0) {
     // unit-RMS scale over the H half-spectrum bins (src/ImageManipulation/Norm.cu:476-509)
     double s = 0;
     for (int i = t; i < nwg1; i += NT) s += (double)partial[(size_t)o * nwg1 + i];
+    for (int i = t; i < HERM_BLOCKS; i += NT) s += (double)corr[(size_t)o * HERM_BLOCKS + i];
     dred[t] = s;
     __syncthreads();
     for (int w = NT / 2; w > 0; w >>= 1) {
@@ -158,10 +234,10 @@ template <int N>
 __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G, int nb, int angle0,
                                                     const float* __restrict__ scale, float* __restrict__ best,
                                                     int32_t* __restrict__ bidx, const float2* __restrict__ tw) {
-  constexpr int NC = N / 2, NH = N / 2 + 1;
+  constexpr int NC = N / 2;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
   constexpr int GS = R2 * (R1 + 1);
-  constexpr size_t H = (size_t)NH * N * N;
+  constexpr int PITCH = g_pitch<N>();
   __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
   const int row = blockIdx.x * RW + r;
@@ -184,7 +260,7 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
 
   for (int o = 0; o < nb; ++o) {
     const float sc = scale[o];
-    const float2* X = G + (size_t)o * H + (size_t)row * NH;
+    const float2* X = G + (size_t)o * g_stride<N>() + (size_t)row * PITCH;
     float2 v[R1];
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
@@ -237,6 +313,7 @@ struct gtomx_pick_plan {
   int ntiles = 0, nwg1 = 0;
   float2* G = nullptr;
   float* partial = nullptr;
+  float* corr = nullptr;  // B x HERM_BLOCKS plane corrections of the unit-RMS sums
   float* scale = nullptr;
   float* mats = nullptr;
   size_t mats_cap = 0;   // orientations
@@ -265,6 +342,7 @@ void plan_release(gtomx_pick_plan* pl) {
   if (!pl) return;
   dev_free(pl->G);
   dev_free(pl->partial);
+  dev_free(pl->corr);
   dev_free(pl->scale);
   dev_free(pl->mats);
   dev_free(pl->ls_mem);
@@ -276,6 +354,17 @@ void plan_release(gtomx_pick_plan* pl) {
 }
 
 template <int N> constexpr int col_tiles() { return (N / 2 + 1 + ColShape<N>::TX - 1) / ColShape<N>::TX; }
+
+size_t g_stride_for(int n) {
+  switch (n) {
+#define GX_CASE(N) \
+  case N:          \
+    return g_stride<N>();
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
+  }
+  return 0;
+}
 
 int tiles_for(int n) {
   switch (n) {
@@ -325,9 +414,11 @@ int run_batch(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb
     GX_LAUNCHED("k_pick_project_z");
     return GTOMX_OK;
   }));
+  hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
+  GX_LAUNCHED("k_pick_herm_norm");
   GX_TRY(timed(pl, 1, s, [&]() -> int {
     hipLaunchKernelGGL(k_pick_y<N>, dim3(ntiles, N, nb), dim3(TX * R2), 0, s, pl->G, pl->partial, ntiles * N,
-                       pl->scale, tw);
+                       pl->corr, pl->scale, tw);
     GX_LAUNCHED("k_pick_y");
     return GTOMX_OK;
   }));
@@ -409,8 +500,8 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   *out = nullptr;
   if (dv.x != dv.y || dv.x != dv.z) return fail(GTOMX_ERR_UNSUPPORTED, "volumes must be cubes");
   if (!fft_len_ok(dv.x)) return fail(GTOMX_ERR_UNSUPPORTED, "volume side must be 2^a or 3*2^a in [8, 256]");
-  if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 1024)
-    return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 1024");
+  if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 512)
+    return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 512");
   if (batch > 256) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 256");
   auto* pl = new gtomx_pick_plan();
   pl->n = dv.x;
@@ -432,11 +523,15 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
     s = dev_alloc(dst, bytes);
     if (s == GTOMX_OK) pl->device_bytes += bytes;
   };
-  alloc(&d, (size_t)pl->B * pl->H * sizeof(float2));
+  // The batch buffer also serves as the C2R workspace in pick_finish (H elements).
+  alloc(&d, std::max((size_t)pl->B * g_stride_for(pl->n), pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->B * pl->nwg1 * sizeof(float));
   pl->partial = static_cast<float*>(d);
+  d = nullptr;
+  alloc(&d, (size_t)pl->B * HERM_BLOCKS * sizeof(float));
+  pl->corr = static_cast<float*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->B * sizeof(float));
   pl->scale = static_cast<float*>(d);
@@ -541,7 +636,7 @@ int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
   GX_TRY(twiddles(&tw));
   GX_TRY(upload_matrices(plan, h_angles, nangles, oversample, s));
   Projector P{reinterpret_cast<const float2*>(d_projectorft), plan->p, plan->p / 2 + 1,
-              (plan->p / 2 + 1) * plan->p * plan->p - 1};
+              (plan->p / 2 + 1) * plan->p * plan->p * 8};
   for (unsigned v = 0; v < nvolumes; ++v) {
     const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
     const float* cv = d_ctf ? d_ctf + (size_t)v * ctf_volume_stride : nullptr;

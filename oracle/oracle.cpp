@@ -188,8 +188,9 @@ void c2r3(const cpx<T>* in, T* out, int nx, int ny, int nz) {
 // transpose(Matrix3Euler(angles)) * Matrix3Scale(oversample)
 // (src/Relion/Project.cu:12-17). The reference reads the projector at the
 // linear address ((z*p + y)*(p/2+1) + x) without a bounds check; x1 = p/2+1
-// therefore reaches into the next row. The same linear address is used here and
-// clamped to the array, which changes nothing for in-range reads.
+// therefore reaches into the next row. The same linear address is used here;
+// an address outside the array reads 0 (the reference's behaviour there is
+// undefined), which changes nothing for in-range reads.
 // ---------------------------------------------------------------------------
 template <class T>
 void project1(const cpx<T>* vol, int p, cpx<T>* out, int n, const double* M,
@@ -200,8 +201,7 @@ void project1(const cpx<T>* vol, int p, cpx<T>* out, int n, const double* M,
   long long H = (long long)nh * n * n;
   auto at = [&](long long xx, long long yy, long long zz) -> cpx<T> {
     long long i = (zz * p + yy) * ph + xx;
-    if (i < 0) i = 0;
-    if (i >= total) i = total - 1;
+    if (i < 0 || i >= total) return cpx<T>(0, 0);
     return vol[i];
   };
   auto lerp = [](cpx<T> a, cpx<T> b, T t) -> cpx<T> {
