@@ -31,13 +31,47 @@
 
 namespace gx {
 
-// Row pitch (complex elements) of the batch buffer G: the half-spectrum width
-// n/2+1 rounded up to whole column tiles, so that every tile row starts on a
-// 128-byte boundary. Columns past n/2 are never read or written.
-template <int N> constexpr int g_pitch() {
-  return (N / 2 + 1 + ColShape<N>::TX - 1) / ColShape<N>::TX * ColShape<N>::TX;
-}
-template <int N> constexpr size_t g_stride() { return (size_t)g_pitch<N>() * N * N; }
+// Layout of the batch buffer G (one block of STRIDE complex values per
+// orientation). When n/2 is a multiple of the column-tile width TX ("split"),
+// the x < n/2 columns form rows of exactly PITCH = n/2 values, so every tile
+// row is a whole number of 128-byte lines, and the x = n/2 (Nyquist) column is
+// stored after them as a compact [z][y] array. Otherwise rows hold all n/2+1
+// columns padded to whole tiles (PITCH), and the padding is never read.
+template <int N> struct GLayout {
+  static constexpr int TX = ColShape<N>::TX;
+  static constexpr bool SPLIT = (N / 2) % TX == 0;
+  static constexpr int PITCH = SPLIT ? N / 2 : (N / 2 + 1 + TX - 1) / TX * TX;
+  static constexpr size_t MAIN = (size_t)PITCH * N * N;
+  static constexpr size_t STRIDE = MAIN + (SPLIT ? (size_t)N * N : 0);
+};
+
+// Workgroup layout of the column passes (1 and 2). In the split layout the
+// first NQ workgroups take the Nyquist column of TX rows at once, one row per
+// lane (their strided accesses are slower, so they start first), and the
+// remaining FT*N workgroups take TX full columns of one row each. Otherwise
+// every row has FT tiles and lanes past x = n/2 idle.
+template <int N> struct ColMap {
+  static constexpr int TX = ColShape<N>::TX;
+  static constexpr bool SPLIT = GLayout<N>::SPLIT;
+  static constexpr int FT = SPLIT ? (N / 2) / TX : (N / 2 + TX) / TX;
+  static constexpr int NQ = SPLIT ? N / TX : 0;
+  static constexpr int NWG = FT * N + NQ;
+  // (workgroup w, lane column c) -> (x, row); row is y in pass 1 and z in pass 2.
+  __device__ static void map(int w, int c, int& x, int& row) {
+    if (w < NQ) {
+      x = N / 2;
+      row = w * TX + c;
+    } else {
+      x = ((w - NQ) % FT) * TX + c;
+      row = (w - NQ) / FT;
+    }
+  }
+};
+
+// Cache-policy bits of buffer accesses: non-temporal (streaming) data should
+// not displace the projector from L2.
+constexpr int CPOL_NT = 2;
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // pass 1
@@ -54,9 +88,9 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
     Projector P, const float* __restrict__ mats, int nb, const float2* __restrict__ E,
     const float* __restrict__ ctf, float2* __restrict__ G, float* __restrict__ partial,
     const float2* __restrict__ tw) {
+  using L = GLayout<N>;
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
-  constexpr int PITCH = g_pitch<N>();
   constexpr int NT = TX * R2;
   constexpr int GRP = 4;  // projector samples per group of in-flight gathers
   static_assert(R1 % GRP == 0, "R1 must be a multiple of the sample group");
@@ -64,23 +98,32 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
   __shared__ float red[(NT + 63) / 64];
 
   const int t = threadIdx.x, c = t % TX, j = t / TX;
-  const int ix = blockIdx.x * TX + c;
-  const int iy = blockIdx.y;
+  int ix, iy;
+  ColMap<N>::map(blockIdx.x, c, ix, iy);
+  const bool nyq_wg = (int)blockIdx.x < ColMap<N>::NQ;
   const bool valid = ix < NH;
   const int y = signed_freq(iy, N);
-  const int nwg = gridDim.x * gridDim.y;
-  const int wg = blockIdx.y * gridDim.x + blockIdx.x;
   const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
-  // Element (x, y, z) of E or the CTF sits at (z*N + y)*NH + x; the z part of
-  // this thread's bins is the wave-uniform constant m*R2*N*NH. E and the CTF
-  // are re-read for every orientation; the workgroup's tile stays in L2.
+
+  // E_v and the CTF at this thread's bins, read once per batch (element
+  // (x, y, z) at (z*N + y)*NH + x; the z part is the constant m*R2*N*NH).
   constexpr unsigned HB = (unsigned)(NH * N * N);
   const __amdgpu_buffer_rsrc_t rE = make_rsrc(E, HB * 8u);
   const __amdgpu_buffer_rsrc_t rC = make_rsrc(ctf, ctf ? HB * 4u : 0u);
   const int ebase = valid ? (j * N + iy) * NH + ix : (int)HB;  // invalid lanes read nothing (0)
-  const bool dc_thread = ix == 0 && iy == 0 && j == 0;        // holds the DC bin as m = 0
-  // Output: element (x, y, z) of G[o] at (z*N + y)*PITCH + x.
-  const int gbase = (j * (R1 / R2) * N + iy) * PITCH + ix;
+  float2 e[R1];
+  float cf[R1];
+#pragma unroll
+  for (int m = 0; m < R1; ++m) {
+    auto ve = __builtin_amdgcn_raw_buffer_load_b64(rE, ebase * 8, m * R2 * N * NH * 8, CPOL_NT);
+    __builtin_memcpy(&e[m], &ve, 8);
+    cf[m] = ctf ? buf_load1(rC, ebase * 4, m * R2 * N * NH * 4) : (valid ? 1.f : 0.f);
+  }
+  const bool dc_thread = ix == 0 && iy == 0 && j == 0;  // holds the DC bin as m = 0
+  // Output of this thread's first bin (z = j*(R1/R2)) and the step between its outputs.
+  const int zrow0 = j * (R1 / R2);
+  const int gbase = nyq_wg ? (int)L::MAIN + zrow0 * N + iy : (zrow0 * N + iy) * L::PITCH + ix;
+  const int gstep = nyq_wg ? N : N * L::PITCH;  // per unit of output z
 
   for (int o = 0; o < nb; ++o) {
     float M[9];
@@ -109,12 +152,10 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
         for (int u = 0; u < GRP; ++u) {
           const int m = g * GRP + u;
           const float z = (float)signed_freq(j + R2 * m, N);
-          const float cm = ctf ? buf_load1(rC, ebase * 4, m * R2 * N * NH * 4) : 1.f;
           const float2 r = cscale(project_at(rs, P.p, P.ph, fmaf(M[2], z, q0x), fmaf(M[5], z, q0y),
-                                             fmaf(M[8], z, q0z), ins[u]), cm);
-          const float2 e = buf_load2(rE, ebase * 8, m * R2 * N * NH * 8);
+                                             fmaf(M[8], z, q0z), ins[u]), cf[m]);
           acc += r.x * r.x + r.y * r.y;
-          v[m] = cmulc(e, r);
+          v[m] = cmulc(e[m], r);
         }
       } else {
 #pragma unroll
@@ -127,15 +168,16 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
     if (dc_thread) v[0] = make_float2(0.f, 0.f);
     __syncthreads();
     fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
-    // Columns past n/2 (invalid lanes) land in the row padding, which is never read.
-    const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * g_stride<N>(), (unsigned)(g_stride<N>() * 8));
+    // Lanes past x = n/2 (unsplit layout) land in the row padding, which is never read.
+    const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * L::STRIDE, (unsigned)(L::STRIDE * 8));
 #pragma unroll
     for (int s = 0; s < R1; ++s) {
-      const int u = s / R2, k2 = s % R2;  // out_index(j, s) = j*(R1/R2) + u + R1*k2
-      buf_store2(v[s], rG, gbase * 8, (u + R1 * k2) * N * PITCH * 8);
+      const int u = s / R2, k2 = s % R2;  // output z = j*(R1/R2) + u + R1*k2
+      const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
+      __builtin_amdgcn_raw_buffer_store_b64(vs, rG, gbase * 8, (u + R1 * k2) * gstep * 8, CPOL_NT);
     }
     const float tot = block_sum(acc, red);
-    if (t == 0) partial[(size_t)o * nwg + wg] = tot;
+    if (t == 0) partial[(size_t)o * ColMap<N>::NWG + blockIdx.x] = tot;
   }
 }
 
@@ -186,35 +228,46 @@ __global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
-    float2* __restrict__ G, const float* __restrict__ partial, int nwg1, const float* __restrict__ corr,
+    float2* __restrict__ G, const float* __restrict__ partial, const float* __restrict__ corr,
     float* __restrict__ scale, const float2* __restrict__ tw) {
+  using L = GLayout<N>;
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1;
   constexpr size_t H = (size_t)NH * N * N;
-  constexpr int PITCH = g_pitch<N>();
   constexpr int NT = TX * R2;
+  constexpr int NWG = ColMap<N>::NWG;
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
   __shared__ double dred[NT];
 
   const int t = threadIdx.x, c = t % TX, j = t / TX;
-  const int ix = blockIdx.x * TX + c;
-  const int z = blockIdx.y, o = blockIdx.z;
-  const bool valid = ix < NH;
-  float2* base = G + (size_t)o * g_stride<N>() + (size_t)z * N * PITCH + ix;
+  int ix, z;
+  ColMap<N>::map(blockIdx.x, c, ix, z);
+  const int o = blockIdx.y;
+  const bool nyq_wg = (int)blockIdx.x < ColMap<N>::NQ;
+  // Column (x, z) along y: main rows (stride PITCH) or the compact Nyquist column (stride 1).
+  const int base = nyq_wg ? (int)L::MAIN + z * N : z * N * L::PITCH + ix;
+  const int ystep = nyq_wg ? 1 : L::PITCH;
+  const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * L::STRIDE, (unsigned)(L::STRIDE * 8));
   float2 v[R1];
 #pragma unroll
-  for (int m = 0; m < R1; ++m) v[m] = valid ? base[(size_t)(j + R2 * m) * PITCH] : make_float2(0.f, 0.f);
-  fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
-  if (valid) {
-#pragma unroll
-    for (int s = 0; s < R1; ++s) base[(size_t)out_index<R1, R2>(j, s) * PITCH] = v[s];
+  for (int m = 0; m < R1; ++m) {
+    auto vl = __builtin_amdgcn_raw_buffer_load_b64(rG, (base + j * ystep) * 8, m * R2 * ystep * 8, CPOL_NT);
+    __builtin_memcpy(&v[m], &vl, 8);
   }
-  if (blockIdx.x == 0 && blockIdx.y == 0) {
+  fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
+  const int obase = base + j * (R1 / R2) * ystep;
+#pragma unroll
+  for (int s = 0; s < R1; ++s) {
+    const int u = s / R2, k2 = s % R2;  // output y = j*(R1/R2) + u + R1*k2
+    const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
+    __builtin_amdgcn_raw_buffer_store_b64(vs, rG, obase * 8, (u + R1 * k2) * ystep * 8, CPOL_NT);
+  }
+  if (blockIdx.x == 0) {
     // unit-RMS scale over the H half-spectrum bins (src/ImageManipulation/Norm.cu:476-509)
-    double s = 0;
-    for (int i = t; i < nwg1; i += NT) s += (double)partial[(size_t)o * nwg1 + i];
-    for (int i = t; i < HERM_BLOCKS; i += NT) s += (double)corr[(size_t)o * HERM_BLOCKS + i];
-    dred[t] = s;
+    double sum = 0;
+    for (int i = t; i < NWG; i += NT) sum += (double)partial[(size_t)o * NWG + i];
+    for (int i = t; i < HERM_BLOCKS; i += NT) sum += (double)corr[(size_t)o * HERM_BLOCKS + i];
+    dred[t] = sum;
     __syncthreads();
     for (int w = NT / 2; w > 0; w >>= 1) {
       if (t < w) dred[t] += dred[t + w];
@@ -234,13 +287,13 @@ template <int N>
 __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G, int nb, int angle0,
                                                     const float* __restrict__ scale, float* __restrict__ best,
                                                     int32_t* __restrict__ bidx, const float2* __restrict__ tw) {
+  using L = GLayout<N>;
   constexpr int NC = N / 2;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
   constexpr int GS = R2 * (R1 + 1);
-  constexpr int PITCH = g_pitch<N>();
   __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
-  const int row = blockIdx.x * RW + r;
+  const int row = blockIdx.x * RW + r;  // row = z*N + y
   const bool valid = row < N * N;
 
   float bv[2 * R1];
@@ -257,22 +310,25 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
     bi[2 * s] = a.x;
     bi[2 * s + 1] = a.y;
   }
+  // Row bins X[0..n/2-1] in the main rows, X[n/2] in the Nyquist column
+  // (split layout) or at the end of the row.
+  const int xrow = valid ? row * L::PITCH : (int)L::STRIDE;  // invalid rows read nothing (0)
+  const int xnyq = valid ? (L::SPLIT ? (int)L::MAIN + row : row * L::PITCH + NC) : (int)L::STRIDE;
 
   for (int o = 0; o < nb; ++o) {
     const float sc = scale[o];
-    const float2* X = G + (size_t)o * g_stride<N>() + (size_t)row * PITCH;
+    const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * L::STRIDE, (unsigned)(L::STRIDE * 8));
     float2 v[R1];
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const int k = j + R2 * m;
-      float2 a = valid ? X[k] : make_float2(0.f, 0.f);
-      float2 b = valid ? X[NC - k] : make_float2(0.f, 0.f);
-      if (k == 0) {
-        a.y = 0.f;
-        b.y = 0.f;
-      }
-      const float2 bc = cconj(b);
-      v[m] = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle<1, N>(tw, k), csub(a, bc))));
+      // a = X[k]; b = X[n/2 - k], which is X[n/2] for k = 0
+      const float2 a = buf_load2(rG, (xrow + j) * 8, R2 * m * 8);
+      const float2 b = buf_load2(rG, (m == 0 && j == 0 ? xnyq : xrow + NC - k) * 8, 0);
+      // the imaginary parts of X[0] and X[n/2] do not enter the real inverse
+      const float2 a0 = k == 0 ? make_float2(a.x, 0.f) : a;
+      const float2 bc = k == 0 ? make_float2(b.x, 0.f) : cconj(b);
+      v[m] = cadd(cadd(a0, bc), mul_pj<1>(cmul(twiddle<1, N>(tw, k), csub(a0, bc))));
     }
     __syncthreads();
     fft_two_step<NC, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[r * GS + jj * (R1 + 1) + k1]; });
@@ -310,7 +366,7 @@ struct gtomx_pick_plan {
   unsigned B = 0;
   int device = 0;
   size_t H = 0, V = 0;
-  int ntiles = 0, nwg1 = 0;
+  int nwg1 = 0;  // pass-1 workgroups (partial sums) per orientation
   float2* G = nullptr;
   float* partial = nullptr;
   float* corr = nullptr;  // B x HERM_BLOCKS plane corrections of the unit-RMS sums
@@ -353,24 +409,22 @@ void plan_release(gtomx_pick_plan* pl) {
   delete pl;
 }
 
-template <int N> constexpr int col_tiles() { return (N / 2 + 1 + ColShape<N>::TX - 1) / ColShape<N>::TX; }
-
 size_t g_stride_for(int n) {
   switch (n) {
 #define GX_CASE(N) \
   case N:          \
-    return g_stride<N>();
+    return GLayout<N>::STRIDE;
     GX_FOR_SIZES(GX_CASE)
 #undef GX_CASE
   }
   return 0;
 }
 
-int tiles_for(int n) {
+int nwg_for(int n) {
   switch (n) {
 #define GX_CASE(N) \
   case N:          \
-    return col_tiles<N>();
+    return ColMap<N>::NWG;
     GX_FOR_SIZES(GX_CASE)
 #undef GX_CASE
   }
@@ -407,9 +461,8 @@ int run_batch(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb
               const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
   constexpr int TX = ColShape<N>::TX, R2 = ColShape<N>::R2;
   constexpr int RW = 256 / RowShape<N / 2>::R2;
-  const int ntiles = col_tiles<N>();
   GX_TRY(timed(pl, 0, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick_project_z<N>, dim3(ntiles, N), dim3(TX * R2), 0, s, P, mats, nb, E, ctf, pl->G,
+    hipLaunchKernelGGL(k_pick_project_z<N>, dim3(ColMap<N>::NWG), dim3(TX * R2), 0, s, P, mats, nb, E, ctf, pl->G,
                        pl->partial, tw);
     GX_LAUNCHED("k_pick_project_z");
     return GTOMX_OK;
@@ -417,7 +470,7 @@ int run_batch(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb
   hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
   GX_LAUNCHED("k_pick_herm_norm");
   GX_TRY(timed(pl, 1, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick_y<N>, dim3(ntiles, N, nb), dim3(TX * R2), 0, s, pl->G, pl->partial, ntiles * N,
+    hipLaunchKernelGGL(k_pick_y<N>, dim3(ColMap<N>::NWG, nb), dim3(TX * R2), 0, s, pl->G, pl->partial,
                        pl->corr, pl->scale, tw);
     GX_LAUNCHED("k_pick_y");
     return GTOMX_OK;
@@ -509,8 +562,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   pl->B = batch ? batch : (dv.x >= 128 ? 8u : 16u);
   pl->H = elements_ft(dv);
   pl->V = elements(dv);
-  pl->ntiles = tiles_for(pl->n);
-  pl->nwg1 = pl->ntiles * pl->n;
+  pl->nwg1 = nwg_for(pl->n);
   int s = GTOMX_OK;
   hipError_t he = hipGetDevice(&pl->device);
   if (he != hipSuccess) {
