@@ -60,15 +60,40 @@ def copy_probe():
     log(f"copy probe: 5 x {4 * n} bytes read and written")
 
 
+def disc_bins(n):
+    """Half-spectrum bins in the columns (x, y) with x^2 + y^2 <= (n/2)^2: the
+    only columns that hold bins of the sphere |k| <= n/2, and so the only ones
+    pass 1 reads E and the CTF for and writes, and pass 2 reads."""
+    x = np.arange(n // 2 + 1)[None, :]
+    y = np.fft.fftfreq(n, 1.0 / n)[:, None]
+    return int(np.count_nonzero(x * x + y * y <= (n // 2) ** 2)) * n
+
+
 def algorithmic_bytes(stage, n, nb):
     """Minimum HBM bytes of one launch of a stage for nb orientations of one n^3 volume."""
     H = (n // 2 + 1) * n * n
+    Hd = disc_bins(n)
     V = n ** 3
-    if stage == 0:  # read E (8 B/bin) and the CTF (4 B/bin) once, write nb spectra
-        return 12 * H + 8 * H * nb
-    if stage == 1:  # read and write nb spectra
-        return 16 * H * nb
+    if stage == 0:  # read E (8 B/bin) and the CTF (4 B/bin) once, write nb column-transformed spectra
+        return 12 * Hd + 8 * Hd * nb
+    if stage == 1:  # read the disc columns, write the full y-transformed spectra
+        return 8 * Hd * nb + 8 * H * nb
     return 8 * H * nb + 16 * V  # read nb spectra, read+write best and argmax
+
+
+def measured_traffic(kernel, n, batch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/pick_cfg2_traffic.json, calibrated against a plain copy), when it was taken at the
+    same launch shape; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pick_cfg2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        js = json.load(f)
+    if js.get("block") != n or js.get("batch") != batch:
+        return None
+    rec = js["kernels"].get(f"gx::{kernel}<{n}>")
+    return int(rec["traffic_bytes"]) if rec else None
 
 
 def cpu_baseline(w, target_seconds):
@@ -163,7 +188,8 @@ def main():
     bytes_per_launch = total_bytes / nbatches
     achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
     roof = {"bound": "hbm", "kernel": STAGES[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(STAGES[k], n, nb_full),
             "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch),
             "stage_ms_per_step": {STAGES[i]: round(ms[i] / args.steps, 2) for i in range(3)}}
 

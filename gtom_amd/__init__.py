@@ -74,6 +74,7 @@ _SIGS = {
     "gtomx_divide_safe_by_vector": [_P, _P, _P, _SZ, _U, _P],
     "gtomx_sphere_mask": [_P, _P, int3, _F, _F, _U, _P],
     "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
+    "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
     "gtomx_pick_plan_create": [ctypes.POINTER(_P), int3, int3, _U],
     "gtomx_pick_plan_destroy": [_P],
     "gtomx_pick_plan_info": [_P, ctypes.POINTER(_U), ctypes.POINTER(_SZ)],
@@ -275,6 +276,28 @@ def d_UpdateMaxArgmax(corr, best, bestidx, batchoffset, stream=None):
     L = best.numel()
     check(lib().gtomx_update_max_argmax(_ptr(corr), L, batch, batchoffset, _ptr(best), _ptr(bestidx),
                                         _stream(stream)), "gtomx_update_max_argmax")
+
+
+def d_LocalPeaks(vol, localextent, threshold, stream=None):
+    """d_LocalPeaks (src/Correlation/LocalPeaks.cu:24-62) on a float32 CUDA
+    tensor of shape (..., z, y, x): one (k, 3) int64 numpy array of (x, y, z)
+    per volume, in raster order."""
+    import numpy as np
+    nz, ny, nx = vol.shape[-3:]
+    batch = max(1, vol.numel() // (nx * ny * nz))
+    counts = np.zeros(batch, np.uint64)
+    cap = 4096
+    while True:
+        peaks = np.zeros((batch, cap, 3), np.int32)
+        check(lib().gtomx_local_peaks(_ptr(vol), int3(nx, ny, nz), int(localextent), float(threshold), batch,
+                                      ctypes.c_void_p(peaks.ctypes.data), cap,
+                                      ctypes.c_void_p(counts.ctypes.data), _stream(stream)),
+              "gtomx_local_peaks")
+        need = int(counts.max())
+        if need <= cap:
+            break
+        cap = need
+    return [peaks[b, :int(counts[b])].astype(np.int64) for b in range(batch)]
 
 
 def d_LocalStd(vol, radius, with_mean=False, stream=None):

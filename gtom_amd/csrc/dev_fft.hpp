@@ -219,14 +219,49 @@ template <int R1, int R2> __device__ __forceinline__ int out_index(int j, int sl
 // xch(k1, jj) returns the LDS address used for the exchange of stage-1
 // output k1 of thread jj; it must be private to the group. The caller
 // synchronises the workgroup before the call if xch's memory was in use.
-template <int N, int R1, int R2, int DIR, class Xch>
-__device__ __forceinline__ void fft_two_step(float2 (&v)[R1], int j, const float2* __restrict__ tw, Xch xch) {
+// Twiddle source tags for fft_two_step: the global table above, or an LDS
+// copy holding exp(-2 pi i m / N) for m in [0, N).
+struct TwiddleGlobal {};
+struct TwiddleLDS {};
+template <int DIR, int N>
+__device__ __forceinline__ float2 twiddle_from(const float2* tw, int m, TwiddleGlobal) {
+  return twiddle<DIR, N>(tw, m);
+}
+template <int DIR, int N>
+__device__ __forceinline__ float2 twiddle_from(const float2* tw, int m, TwiddleLDS) {
+  const float2 w = tw[m % N];
+  return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+// Or twiddles the thread holds in registers: w[k1] = exp(DIR 2 pi i j k1 / N)
+// for its j (w[0] unused), loaded once by load_two_step_twiddles.
+template <int R1> struct TwiddleRegs {
+  const float2 (&w)[R1];
+};
+template <int N, int R1, int DIR>
+__device__ __forceinline__ void load_two_step_twiddles(float2 (&w)[R1], int j, const float2* __restrict__ tw) {
+  w[0] = make_float2(1.f, 0.f);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) w[k1] = twiddle<DIR, N>(tw, j * k1);
+}
+template <int DIR, int N, class Src>
+__device__ __forceinline__ float2 stage_twiddle(const float2* tw, int j, int k1, Src src) {
+  return twiddle_from<DIR, N>(tw, j * k1, src);
+}
+template <int DIR, int N, int R1>
+__device__ __forceinline__ float2 stage_twiddle(const float2*, int, int k1, TwiddleRegs<R1> src) {
+  return src.w[k1];
+}
+
+template <int N, int R1, int R2, int DIR, class Xch, class Src = TwiddleGlobal>
+__device__ __forceinline__ void fft_two_step(float2 (&v)[R1], int j, const float2* __restrict__ tw, Xch xch,
+                                             Src src = Src{}) {
   static_assert(R1 * R2 == N, "bad factorisation");
   dft<R1, DIR>(v);
   if constexpr (R2 > 1) {
     static_assert(R1 % R2 == 0, "R1 must be a multiple of R2");
 #pragma unroll
-    for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], twiddle<DIR, N>(tw, j * k1));
+    for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], stage_twiddle<DIR, N>(tw, j, k1, src));
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) *xch(k1, j) = v[k1];
     __syncthreads();

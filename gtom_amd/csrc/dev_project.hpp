@@ -107,6 +107,56 @@ __device__ __forceinline__ float2 project_at(__amdgpu_buffer_rsrc_t rs, int p, i
   return val;
 }
 
+// The projector re-laid for the search kernels, so that the sampler needs no
+// wrap logic and one address per sample: planes and rows in centred order
+// with a one-voxel margin (row c holds the FFT-order row (c - OFF) mod p,
+// OFF = p/2 + 1, c in [0, p + 3)), and one extra column x = p/2 + 1 holding
+// the element that follows the row at the reference's linear address (x = 0
+// of the next FFT-order row, or 0 past the end of the array). For every q
+// with |q| < p/2 + 1/2 this reads exactly what project_at reads. Built by
+// k_centre_projector in the search plan.
+struct CentredProjector {
+  const float2* __restrict__ data;
+  int bytes;     // PX * PY * PY * 8
+  int row8;      // PX * 8: byte step of y
+  int plane8;    // PX * PY * 8: byte step of z
+  float base8;   // (OFF * PY + OFF) * PX * 8: byte offset of (x, y, z) = (0, 0, 0)
+};
+
+__device__ __forceinline__ void proj_quad(__amdgpu_buffer_rsrc_t rs, int off, int soff, float2& a, float2& b) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, soff, 0);
+  __builtin_memcpy(&a, &v, 8);
+  __builtin_memcpy(&b, reinterpret_cast<const char*>(&v) + 8, 8);
+}
+
+// project_at on the centred layout (same flip, floors, lerp order and result).
+// The byte offset of the (x0, y0, z0) corner is computed in float, exact since
+// every term is an integer below 2^24.
+__device__ __forceinline__ float2 project_centred(__amdgpu_buffer_rsrc_t rs, const CentredProjector& C, float px,
+                                                  float py, float pz, bool inside = true) {
+  const float sgn = px < -1e-5f ? -1.f : 1.f;
+  px *= sgn;
+  py *= sgn;
+  pz *= sgn;
+  const float fx = floorf(px + 1e-5f);
+  const float fy = floorf(py);
+  const float fz = floorf(pz);
+  const float tx = px - fx, ty = py - fy, tz = pz - fz;
+  const float fo = fmaf(fz, (float)C.plane8, fmaf(fy, (float)C.row8, fmaf(fx, 8.f, C.base8)));
+  const int o = inside ? (int)fo : OUT_OF_RANGE;
+  float2 d000, d001, d010, d011, d100, d101, d110, d111;
+  proj_quad(rs, o, 0, d000, d001);
+  proj_quad(rs, o, C.row8, d010, d011);
+  proj_quad(rs, o, C.plane8, d100, d101);
+  proj_quad(rs, o, C.plane8 + C.row8, d110, d111);
+  const float2 dx00 = lerp2(d000, d001, tx), dx01 = lerp2(d010, d011, tx);
+  const float2 dx10 = lerp2(d100, d101, tx), dx11 = lerp2(d110, d111, tx);
+  const float2 dxy0 = lerp2(dx00, dx01, ty), dxy1 = lerp2(dx10, dx11, ty);
+  float2 val = lerp2(dxy0, dxy1, tz);
+  val.y *= sgn;
+  return val;
+}
+
 // M: row-major 3x3 (rotation^T * oversample); (x, y, z): signed integer frequency.
 __device__ __forceinline__ float2 project_sample(__amdgpu_buffer_rsrc_t rs, int p, int ph, const float* M, int x,
                                                  int y, int z) {

@@ -83,9 +83,29 @@ typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 // k_pick_herm_norm supplies its correction on the two planes. The DC bin is
 // zeroed (the reference subtracts the mean).
 // ---------------------------------------------------------------------------
-template <int N>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sampler_rsrc(const Projector& P) { return projector_rsrc(P); }
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sampler_rsrc(const CentredProjector& C) {
+  return make_rsrc(C.data, (unsigned)C.bytes);
+}
+__device__ __forceinline__ float2 sample(__amdgpu_buffer_rsrc_t rs, const Projector& P, float px, float py, float pz,
+                                         bool in) {
+  return project_at(rs, P.p, P.ph, px, py, pz, in);
+}
+__device__ __forceinline__ float2 sample(__amdgpu_buffer_rsrc_t rs, const CentredProjector& C, float px, float py,
+                                         float pz, bool in) {
+  return project_centred(rs, C, px, py, pz, in);
+}
+
+// Columns (x, y) with x^2 + y^2 > (n/2)^2 hold no bin of the sphere, so Q is
+// zero along all of z there and stays zero through the z transform: pass 1
+// writes nothing for them and pass 2 reads zeros in their place.
+template <int N> __device__ __forceinline__ bool column_in_disc(int x, int y) {
+  return x * x + y * y <= (N / 2) * (N / 2);
+}
+
+template <int N, class PJ>
 __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_project_z(
-    Projector P, const float* __restrict__ mats, int nb, const float2* __restrict__ E,
+    PJ P, const float* __restrict__ mats, int nb, const float2* __restrict__ E,
     const float* __restrict__ ctf, float2* __restrict__ G, float* __restrict__ partial,
     const float2* __restrict__ tw) {
   using L = GLayout<N>;
@@ -101,9 +121,19 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
   int ix, iy;
   ColMap<N>::map(blockIdx.x, c, ix, iy);
   const bool nyq_wg = (int)blockIdx.x < ColMap<N>::NQ;
-  const bool valid = ix < NH;
   const int y = signed_freq(iy, N);
-  const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
+  {
+    // A workgroup whose columns all lie outside the disc only reports zero sums.
+    int x0, row0;
+    ColMap<N>::map(blockIdx.x, 0, x0, row0);
+    const bool wg_out = nyq_wg ? blockIdx.x != 0 : !column_in_disc<N>(x0, signed_freq(row0, N));
+    if (wg_out) {
+      for (int o = t; o < nb; o += NT) partial[(size_t)o * ColMap<N>::NWG + blockIdx.x] = 0.f;
+      return;
+    }
+  }
+  const bool valid = ix < NH && column_in_disc<N>(ix, y);
+  const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
 
   // E_v and the CTF at this thread's bins, read once per batch (element
   // (x, y, z) at (z*N + y)*NH + x; the z part is the constant m*R2*N*NH).
@@ -119,10 +149,15 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
     __builtin_memcpy(&e[m], &ve, 8);
     cf[m] = ctf ? buf_load1(rC, ebase * 4, m * R2 * N * NH * 4) : (valid ? 1.f : 0.f);
   }
+  // The N twiddles of the z transform, staged once per workgroup in LDS.
+  __shared__ float2 twl[N];
+  for (int i = t; i < N; i += NT) twl[i] = twiddle<-1, N>(tw, i);
   const bool dc_thread = ix == 0 && iy == 0 && j == 0;  // holds the DC bin as m = 0
   // Output of this thread's first bin (z = j*(R1/R2)) and the step between its outputs.
   const int zrow0 = j * (R1 / R2);
-  const int gbase = nyq_wg ? (int)L::MAIN + zrow0 * N + iy : (zrow0 * N + iy) * L::PITCH + ix;
+  // Lanes of columns outside the disc store nothing (out-of-range offset).
+  const int gbase8 = !valid ? OUT_OF_RANGE
+                            : (nyq_wg ? (int)L::MAIN + zrow0 * N + iy : (zrow0 * N + iy) * L::PITCH + ix) * 8;
   const int gstep = nyq_wg ? N : N * L::PITCH;  // per unit of output z
 
   for (int o = 0; o < nb; ++o) {
@@ -152,8 +187,8 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
         for (int u = 0; u < GRP; ++u) {
           const int m = g * GRP + u;
           const float z = (float)signed_freq(j + R2 * m, N);
-          const float2 r = cscale(project_at(rs, P.p, P.ph, fmaf(M[2], z, q0x), fmaf(M[5], z, q0y),
-                                             fmaf(M[8], z, q0z), ins[u]), cf[m]);
+          const float2 r = cscale(sample(rs, P, fmaf(M[2], z, q0x), fmaf(M[5], z, q0y), fmaf(M[8], z, q0z), ins[u]),
+                                  cf[m]);
           acc += r.x * r.x + r.y * r.y;
           v[m] = cmulc(e[m], r);
         }
@@ -167,14 +202,14 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
     // removed from the sum by k_pick_herm_norm.
     if (dc_thread) v[0] = make_float2(0.f, 0.f);
     __syncthreads();
-    fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
-    // Lanes past x = n/2 (unsplit layout) land in the row padding, which is never read.
+    fft_two_step<N, R1, R2, 1>(v, j, twl, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; },
+                               TwiddleLDS{});
     const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * L::STRIDE, (unsigned)(L::STRIDE * 8));
 #pragma unroll
     for (int s = 0; s < R1; ++s) {
       const int u = s / R2, k2 = s % R2;  // output z = j*(R1/R2) + u + R1*k2
       const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
-      __builtin_amdgcn_raw_buffer_store_b64(vs, rG, gbase * 8, (u + R1 * k2) * gstep * 8, CPOL_NT);
+      __builtin_amdgcn_raw_buffer_store_b64(vs, rG, gbase8, (u + R1 * k2) * gstep * 8, CPOL_NT);
     }
     const float tot = block_sum(acc, red);
     if (t == 0) partial[(size_t)o * ColMap<N>::NWG + blockIdx.x] = tot;
@@ -236,8 +271,10 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
   constexpr size_t H = (size_t)NH * N * N;
   constexpr int NT = TX * R2;
   constexpr int NWG = ColMap<N>::NWG;
-  __shared__ float2 xch[R2 > 1 ? TX * N : 1];
-  __shared__ double dred[NT];
+  // The unit-RMS reduction reuses the exchange area after the transform.
+  static_assert(R2 == 1 || TX * N * sizeof(float2) >= NT * sizeof(double), "exchange area too small");
+  __shared__ float2 xch[R2 > 1 ? TX * N : NT];
+  double* dred = reinterpret_cast<double*>(xch);
 
   const int t = threadIdx.x, c = t % TX, j = t / TX;
   int ix, z;
@@ -251,10 +288,18 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
   float2 v[R1];
 #pragma unroll
   for (int m = 0; m < R1; ++m) {
-    auto vl = __builtin_amdgcn_raw_buffer_load_b64(rG, (base + j * ystep) * 8, m * R2 * ystep * 8, CPOL_NT);
+    // bins of columns outside the disc were not written by pass 1 and are zero
+    const int voff = column_in_disc<N>(ix, signed_freq(j + R2 * m, N)) ? (base + j * ystep) * 8 : OUT_OF_RANGE;
+    auto vl = __builtin_amdgcn_raw_buffer_load_b64(rG, voff, m * R2 * ystep * 8, CPOL_NT);
     __builtin_memcpy(&v[m], &vl, 8);
   }
-  fft_two_step<N, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
+  // The N twiddles of this length, staged once per workgroup in LDS (one
+  // global load per thread instead of R1 - 1).
+  __shared__ float2 twl[N];
+  for (int i = t; i < N; i += NT) twl[i] = twiddle<-1, N>(tw, i);
+  __syncthreads();
+  fft_two_step<N, R1, R2, 1>(v, j, twl, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; },
+                             TwiddleLDS{});
   const int obase = base + j * (R1 / R2) * ystep;
 #pragma unroll
   for (int s = 0; s < R1; ++s) {
@@ -264,6 +309,7 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
   }
   if (blockIdx.x == 0) {
     // unit-RMS scale over the H half-spectrum bins (src/ImageManipulation/Norm.cu:476-509)
+    __syncthreads();  // the exchange area is free again
     double sum = 0;
     for (int i = t; i < NWG; i += NT) sum += (double)partial[(size_t)o * NWG + i];
     for (int i = t; i < HERM_BLOCKS; i += NT) sum += (double)corr[(size_t)o * HERM_BLOCKS + i];
@@ -312,6 +358,13 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
   }
   // Row bins X[0..n/2-1] in the main rows, X[n/2] in the Nyquist column
   // (split layout) or at the end of the row.
+  // This thread's twiddles, the same for every orientation, held in
+  // registers: exp(2 pi i k/N) for its pre-processing bins k = j + R2*m and
+  // the stage twiddles of the half-length transform.
+  float2 twp[R1], twh[R1];
+#pragma unroll
+  for (int m = 0; m < R1; ++m) twp[m] = twiddle<1, N>(tw, j + R2 * m);
+  load_two_step_twiddles<NC, R1, 1>(twh, j, tw);
   const int xrow = valid ? row * L::PITCH : (int)L::STRIDE;  // invalid rows read nothing (0)
   const int xnyq = valid ? (L::SPLIT ? (int)L::MAIN + row : row * L::PITCH + NC) : (int)L::STRIDE;
 
@@ -328,10 +381,11 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
       // the imaginary parts of X[0] and X[n/2] do not enter the real inverse
       const float2 a0 = k == 0 ? make_float2(a.x, 0.f) : a;
       const float2 bc = k == 0 ? make_float2(b.x, 0.f) : cconj(b);
-      v[m] = cadd(cadd(a0, bc), mul_pj<1>(cmul(twiddle<1, N>(tw, k), csub(a0, bc))));
+      v[m] = cadd(cadd(a0, bc), mul_pj<1>(cmul(twp[m], csub(a0, bc))));
     }
     __syncthreads();
-    fft_two_step<NC, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[r * GS + jj * (R1 + 1) + k1]; });
+    fft_two_step<NC, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[r * GS + jj * (R1 + 1) + k1]; },
+                                TwiddleRegs<R1>{twh});
     const int32_t oi = angle0 + o;
 #pragma unroll
     for (int s = 0; s < R1; ++s) {
@@ -355,6 +409,27 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// The centred projector layout (CentredProjector): element (cx, cy, cz) of a
+// PX x PY x PY array, PX = p/2 + 2, PY = p + 3, is the reference's projector
+// at linear index (iz*p + iy)*(p/2+1) + cx with iy = (cy - OFF) mod p,
+// iz = (cz - OFF) mod p, OFF = p/2 + 1; cx = p/2 + 1 continues into the next
+// row at that linear address, and indices past the array read 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_centre_projector(const float2* __restrict__ P, int p,
+                                                          float2* __restrict__ out) {
+  const int ph = p / 2 + 1, PX = ph + 1, PY = p + 3, OFF = p / 2 + 1;
+  const int total = PX * PY * PY;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const int cx = i % PX, cy = (i / PX) % PY, cz = i / (PX * PY);
+  const int iy = ((cy - OFF) % p + p) % p, iz = ((cz - OFF) % p + p) % p;
+  const int lin = (iz * p + iy) * ph + cx;
+  out[i] = lin < ph * p * p ? P[lin] : make_float2(0.f, 0.f);
+}
+
+inline int centred_bytes(int p) { return (p / 2 + 2) * (p + 3) * (p + 3) * 8; }
 
 }  // namespace gx
 
@@ -380,6 +455,7 @@ struct gtomx_pick_plan {
   void* ls_mem = nullptr;
   gx::LocalStdScratch ls{};
   float* ebuf = nullptr;
+  float2* cproj = nullptr;  // centred projector layout
   size_t device_bytes = 0;
   // kernel timing
   bool timing = false;
@@ -403,6 +479,7 @@ void plan_release(gtomx_pick_plan* pl) {
   dev_free(pl->mats);
   dev_free(pl->ls_mem);
   dev_free(pl->ebuf);
+  dev_free(pl->cproj);
   if (pl->h_mats) (void)hipHostFree(pl->h_mats);
   if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
   for (hipEvent_t e : pl->ev_pool) (void)hipEventDestroy(e);
@@ -456,14 +533,14 @@ template <class F> int timed(gtomx_pick_plan* pl, int stage, hipStream_t s, F la
   return GTOMX_OK;
 }
 
-template <int N>
-int run_batch(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb, const float2* E,
+template <int N, class PJ>
+int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float* mats, int nb, const float2* E,
               const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
   constexpr int TX = ColShape<N>::TX, R2 = ColShape<N>::R2;
   constexpr int RW = 256 / RowShape<N / 2>::R2;
   GX_TRY(timed(pl, 0, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick_project_z<N>, dim3(ColMap<N>::NWG), dim3(TX * R2), 0, s, P, mats, nb, E, ctf, pl->G,
-                       pl->partial, tw);
+    hipLaunchKernelGGL((k_pick_project_z<N, PJ>), dim3(ColMap<N>::NWG), dim3(TX * R2), 0, s, PS, mats, nb, E, ctf,
+                       pl->G, pl->partial, tw);
     GX_LAUNCHED("k_pick_project_z");
     return GTOMX_OK;
   }));
@@ -484,12 +561,17 @@ int run_batch(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb
   return GTOMX_OK;
 }
 
-int run_batch_n(gtomx_pick_plan* pl, const Projector& P, const float* mats, int nb, const float2* E,
-                const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
+// Pass 1 samples the centred layout when every sampled q stays within
+// |q| < p/2 + 1/2 (oversample * n/2 <= p/2 + 1/4, which leaves room for the
+// rounding of q), and the reference's layout otherwise.
+int run_batch_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projector& P, const float* mats, int nb,
+                const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
+                hipStream_t s) {
   switch (pl->n) {
-#define GX_CASE(N) \
-  case N:          \
-    return run_batch<N>(pl, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+#define GX_CASE(N)                                                                            \
+  case N:                                                                                     \
+    return C ? run_batch<N>(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, tw, s)            \
+             : run_batch<N>(pl, P, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
     GX_FOR_SIZES(GX_CASE)
 #undef GX_CASE
   }
@@ -559,7 +641,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   auto* pl = new gtomx_pick_plan();
   pl->n = dv.x;
   pl->p = dp.x;
-  pl->B = batch ? batch : (dv.x >= 128 ? 8u : 16u);
+  pl->B = batch ? batch : (dv.x >= 128 ? 32u : 16u);
   pl->H = elements_ft(dv);
   pl->V = elements(dv);
   pl->nwg1 = nwg_for(pl->n);
@@ -591,6 +673,9 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   alloc(&pl->ls_mem, local_std_scratch_bytes(dv));
   alloc(&d, pl->V * sizeof(float));
   pl->ebuf = static_cast<float*>(d);
+  d = nullptr;
+  alloc(&d, (size_t)centred_bytes(pl->p));
+  pl->cproj = static_cast<float2*>(d);
   if (s != GTOMX_OK) {
     std::string msg = gtomx_last_error();
     plan_release(pl);
@@ -689,6 +774,14 @@ int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
   GX_TRY(upload_matrices(plan, h_angles, nangles, oversample, s));
   Projector P{reinterpret_cast<const float2*>(d_projectorft), plan->p, plan->p / 2 + 1,
               (plan->p / 2 + 1) * plan->p * plan->p * 8};
+  CentredProjector C{};
+  const bool centred = (double)oversample * (plan->n / 2) <= plan->p / 2 + 0.25;
+  if (centred) {
+    const int p = plan->p, PX = p / 2 + 2, PY = p + 3, OFF = p / 2 + 1;
+    hipLaunchKernelGGL(k_centre_projector, dim3((PX * PY * PY + 255) / 256), dim3(256), 0, s, P.data, p, plan->cproj);
+    GX_LAUNCHED("k_centre_projector");
+    C = CentredProjector{plan->cproj, centred_bytes(p), PX * 8, PX * PY * 8, (float)((OFF * PY + OFF) * PX * 8)};
+  }
   for (unsigned v = 0; v < nvolumes; ++v) {
     const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
     const float* cv = d_ctf ? d_ctf + (size_t)v * ctf_volume_stride : nullptr;
@@ -696,7 +789,8 @@ int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
     int32_t* iv = d_idx + (size_t)v * plan->V;
     for (unsigned b0 = 0; b0 < nangles; b0 += plan->B) {
       int nb = (int)std::min<unsigned>(plan->B, nangles - b0);
-      GX_TRY(run_batch_n(plan, P, plan->mats + 9 * (size_t)b0, nb, Ev, cv, bv, iv, (int)(angle_offset + b0), tw, s));
+      GX_TRY(run_batch_n(plan, centred ? &C : nullptr, P, plan->mats + 9 * (size_t)b0, nb, Ev, cv, bv, iv,
+                         (int)(angle_offset + b0), tw, s));
     }
   }
   return GTOMX_OK;

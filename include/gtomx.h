@@ -15,7 +15,7 @@
  *    waiting for it unless documented otherwise.
  *  - Every function returns a gtomx_status; no C++ exception crosses the ABI.
  *    gtomx_last_error() gives a human-readable message for the calling thread.
- *  - Supported transform sizes: every axis a power of two in [8, 256]
+ *  - Supported transform sizes: every axis 2^a or 3*2^a in [8, 256]
  *    (an axis of extent 1 is allowed for 2D and 1D data).
  *
  * The reference interface each function replaces is cited as file:line of
@@ -188,6 +188,22 @@ int gtomx_sphere_mask(const float* d_input, float* d_output, gtomx_int3 dims, fl
  * variance is combined in double. Allocates and frees its own scratch. */
 int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, float* d_std,
                     float* d_mean, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Peak extraction (the consumer of the search output)                       */
+/* ------------------------------------------------------------------------ */
+
+/* d_LocalPeaks, include/Correlation.cuh:81 / src/Correlation/LocalPeaks.cu:24-62
+ * (kernel :98-143): for each of `batch` volumes, the voxels whose value is not
+ * below threshold and not smaller than any voxel within the Euclidean ball of
+ * radius localextent (clamped at the volume faces, no wrap). Peaks of volume
+ * b are written as (x, y, z) in raster order (z, then y, then x) to
+ * h_peaks + b*capacity, at most `capacity` of them; h_counts[b] receives the
+ * full count, so a caller whose capacity was too small can call again.
+ * Synchronous: returns after the results are on the host. */
+int gtomx_local_peaks(const float* d_input, gtomx_int3 dims, int localextent, float threshold,
+                      unsigned batch, gtomx_int3* h_peaks, size_t capacity, size_t* h_counts,
+                      void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* The template-matching path                                                */

@@ -596,6 +596,42 @@ void orc_update_max_argmax_f64(const double* corr, long long vectorlength, int b
   }
 }
 
+// d_LocalPeaks (src/Correlation/LocalPeaks.cu:24-62) with LocalPeaksKernel
+// (src/Correlation/LocalPeaks.cu:98-143), one volume: voxel (x, y, z) is a
+// peak unless its value is below the threshold or some other voxel within the
+// box clamped to the volume, at squared distance <= localextent^2 (the
+// kernel's sqdist > r^2 + 1e-5 test on integers), holds a strictly larger
+// value. Writes up to `capacity` linear indices in raster order (z, then y,
+// then x, the reference's host scan at :47-51) and returns the peak count.
+long long orc_local_peaks_f32(const float* in, int nx, int ny, int nz, int localextent, float threshold,
+                              long long* out, long long capacity) {
+  const long long r2 = (long long)localextent * localextent;
+  long long count = 0;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const float v = in[((long long)z * ny + y) * nx + x];
+        if (v < threshold) continue;
+        bool peak = true;
+        for (int zz = std::max(0, z - localextent); peak && zz <= std::min(nz - 1, z + localextent); ++zz)
+          for (int yy = std::max(0, y - localextent); peak && yy <= std::min(ny - 1, y + localextent); ++yy)
+            for (int xx = std::max(0, x - localextent); xx <= std::min(nx - 1, x + localextent); ++xx) {
+              const long long d2 = (long long)(zz - z) * (zz - z) + (long long)(yy - y) * (yy - y) +
+                                   (long long)(xx - x) * (xx - x);
+              if (d2 > r2 || d2 == 0) continue;
+              if (v < in[((long long)zz * ny + yy) * nx + xx]) {
+                peak = false;
+                break;
+              }
+            }
+        if (peak) {
+          if (count < capacity) out[count] = ((long long)z * ny + y) * nx + x;
+          ++count;
+        }
+      }
+  return count;
+}
+
 // Returns the wall time of the orientation loop in seconds.
 double orc_pick_f64(const double* projft, int p, double oversample, const double* expft,
                     const double* ctf, int ctf_per_volume, int n, int nvol,

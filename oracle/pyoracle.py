@@ -41,6 +41,8 @@ def lib():
         L.orc_remap_full2fullfft_f64.argtypes = [_p, _p, _i, _i, _i]
         L.orc_local_std_f64.argtypes = [_p, _i, _i, _i, _d, _p, _p]
         L.orc_update_max_argmax_f64.argtypes = [_p, _ll, _i, _i, _p, _p]
+        L.orc_local_peaks_f32.argtypes = [_p, _i, _i, _i, _i, _f, _p, _ll]
+        L.orc_local_peaks_f32.restype = _ll
         pick_args = [_p, _i, _d, _p, _p, _i, _i, _i, _p, _i, _d, _p, _p, _p, _p, _p]
         L.orc_pick_f64.argtypes = pick_args
         L.orc_pick_f64.restype = _d
@@ -170,6 +172,21 @@ def update_max_argmax(corr, best, bestidx, batchoffset):
     batch = c.shape[0]
     L = c[0].size
     lib().orc_update_max_argmax_f64(_ptr(c), L, batch, batchoffset, _ptr(b), _ptr(bestidx))
+
+
+def local_peaks(vol, localextent, threshold):
+    """d_LocalPeaks on one float32 volume (nz, ny, nx): (k, 3) int array of (x, y, z) in raster order."""
+    v = _c(vol, np.float32)
+    nz, ny, nx = v.shape
+    cap = 4096
+    while True:
+        out = np.empty(cap, np.int64)
+        k = lib().orc_local_peaks_f32(_ptr(v), nx, ny, nz, int(localextent), float(threshold), _ptr(out), cap)
+        if k <= cap:
+            break
+        cap = k
+    lin = out[:k]
+    return np.stack([lin % nx, (lin // nx) % ny, lin // (nx * ny)], axis=1).astype(np.int64)
 
 
 def pick(projft, p, oversample, expft, ctf, n, angles, maskradius, ctf_per_volume=False,

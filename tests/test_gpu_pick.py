@@ -72,6 +72,19 @@ def test_pick_two_volumes_per_volume_ctf(gx, orc, cuda):
     _check(best, idx, ref)
 
 
+@pytest.mark.parametrize("factor", [0.8, 1.5])
+def test_pick_other_oversampling(gx, orc, cuda, factor):
+    """Oversampling other than p/n. At 0.8 * p/n every sample stays inside the
+    projector (the centred projector layout); at 1.5 * p/n samples run past
+    its Nyquist and read the reference's wrapped and linear addresses (the
+    reference's own layout)."""
+    w = _small_workload(gx, seed=13)
+    w["oversample"] = w["oversample"] * factor
+    ctf = [synth_ctf(orc, w["n"], -3e-6)]
+    best, idx, ref = _run_both(gx, orc, cuda, w, ctf, nang=40)
+    _check(best, idx, ref)
+
+
 @pytest.mark.parametrize("batch", [1, 5, 16])
 def test_pick_batch_sizes_agree(gx, orc, cuda, batch):
     """Results do not depend on the number of orientations per pass (bit-exact)."""
