@@ -22,6 +22,26 @@ class GtomxError(RuntimeError):
         self.status = status
 
 
+class MrcHeader(ctypes.Structure):
+    """gtomx_mrc_header: HeaderMRC (include/IO.cuh:26-96), the 1024-byte MRC header."""
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("mode", ctypes.c_int32),
+                ("startx", ctypes.c_int32), ("starty", ctypes.c_int32), ("startz", ctypes.c_int32),
+                ("mx", ctypes.c_int32), ("my", ctypes.c_int32), ("mz", ctypes.c_int32),
+                ("cellx", ctypes.c_float), ("celly", ctypes.c_float), ("cellz", ctypes.c_float),
+                ("alpha", ctypes.c_float), ("beta", ctypes.c_float), ("gamma", ctypes.c_float),
+                ("mapc", ctypes.c_int32), ("mapr", ctypes.c_int32), ("maps", ctypes.c_int32),
+                ("dmin", ctypes.c_float), ("dmax", ctypes.c_float), ("dmean", ctypes.c_float),
+                ("ispg", ctypes.c_int32), ("nsymbt", ctypes.c_int32),
+                ("creatid", ctypes.c_int16), ("extra1", ctypes.c_uint8 * 30),
+                ("nint", ctypes.c_int16), ("nreal", ctypes.c_int16), ("extra2", ctypes.c_uint8 * 28),
+                ("idtype", ctypes.c_int16), ("lens", ctypes.c_int16), ("nd1", ctypes.c_int16),
+                ("nd2", ctypes.c_int16), ("vd1", ctypes.c_int16), ("vd2", ctypes.c_int16),
+                ("tiltoriginal", ctypes.c_float * 3), ("tiltcurrent", ctypes.c_float * 3),
+                ("origin", ctypes.c_float * 3), ("cmap", ctypes.c_char * 4), ("stamp", ctypes.c_uint8 * 4),
+                ("rms", ctypes.c_float), ("nlabl", ctypes.c_int32), ("labels", (ctypes.c_char * 80) * 10)]
+
+
 class int3(ctypes.Structure):
     _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("z", ctypes.c_int)]
 
@@ -75,6 +95,20 @@ _SIGS = {
     "gtomx_sphere_mask": [_P, _P, int3, _F, _F, _U, _P],
     "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
     "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
+    "gtomx_fft_full_pad": [_P, _P, int3, int3, _U, _P],
+    "gtomx_mrc_header_default": [_P],
+    "gtomx_mrc_read_header": [ctypes.c_char_p, _P],
+    "gtomx_mrc_read": [ctypes.c_char_p, _I, _P, _SZ, _P],
+    "gtomx_mrc_write": [_P, _P, ctypes.c_char_p],
+    "gtomx_mrc_write_f32": [_P, int3, ctypes.c_char_p],
+    "gtomx_mrc_write_device_f32": [_P, int3, ctypes.c_char_p, _P],
+    "gtomx_mrc_to_device_f32": [_P, _I, _SZ, _P, _P],
+    "gtomx_norm_mean01std": [_P, _P, _SZ, _U, _P],
+    "gtomx_picker_create": [ctypes.POINTER(_P), _P, int3, _P, int3, _U, _P],
+    "gtomx_picker_destroy": [_P],
+    "gtomx_picker_set_image": [_P, _P, _P, _P],
+    "gtomx_picker_correlate": [_P, _F, _P, _P, _P],
+    "gtomx_picker_correlate_angles": [_P, _P, _U, _P, _P, _P],
     "gtomx_pick_plan_create": [ctypes.POINTER(_P), int3, int3, _U],
     "gtomx_pick_plan_destroy": [_P],
     "gtomx_pick_plan_info": [_P, ctypes.POINTER(_U), ctypes.POINTER(_SZ)],
@@ -342,6 +376,137 @@ class PickPlan:
         cnt = (ctypes.c_ulonglong * 3)()
         check(lib().gtomx_pick_plan_timings(self.handle, ms, cnt, int(bool(reset))), "timings")
         return list(ms), list(cnt)
+
+
+_MRC_DTYPES = {0: np.uint8, 1: np.int16, 2: np.float32, 3: np.dtype([("re", "<i2"), ("im", "<i2")]),
+               4: np.complex64, 6: np.uint16}
+
+
+def read_mrc_header(path):
+    """ReadMRCHeader (src/IO/mrc.cpp:38-66): an MrcHeader."""
+    h = MrcHeader()
+    check(lib().gtomx_mrc_read_header(os.fsencode(path), ctypes.byref(h)), "gtomx_mrc_read_header")
+    return h
+
+
+def read_mrc(path, nframe=-1):
+    """ReadMRC (src/IO/mrc.cpp:7-36): (header, numpy array (z, y, x) in the file's mode);
+    one z section when nframe >= 0."""
+    h = read_mrc_header(path)
+    if h.mode not in _MRC_DTYPES:
+        raise GtomxError(2, "read_mrc", f"MRC mode {h.mode} is not read into numpy here")
+    dt = np.dtype(_MRC_DTYPES[h.mode])
+    shape = (1 if nframe >= 0 else h.nz, h.ny, h.nx)
+    out = np.empty(shape, dt)
+    check(lib().gtomx_mrc_read(os.fsencode(path), int(nframe), ctypes.c_void_p(out.ctypes.data), out.nbytes,
+                               ctypes.byref(h)), "gtomx_mrc_read")
+    return h, out
+
+
+def write_mrc(path, data, header=None):
+    """WriteMRC: float32 data with the default header and its min/max
+    (src/IO/mrc.cpp:85-106), or any supported mode with the given header
+    (src/IO/mrc.cpp:68-83)."""
+    a = np.ascontiguousarray(data)
+    if header is None:
+        a = np.ascontiguousarray(a, np.float32)
+        nz, ny, nx = a.shape[-3:] if a.ndim >= 3 else (1,) * (3 - a.ndim) + a.shape
+        check(lib().gtomx_mrc_write_f32(ctypes.c_void_p(a.ctypes.data), int3(nx, ny, nz), os.fsencode(path)),
+              "gtomx_mrc_write_f32")
+    else:
+        check(lib().gtomx_mrc_write(ctypes.c_void_p(a.ctypes.data), ctypes.byref(header), os.fsencode(path)),
+              "gtomx_mrc_write")
+
+
+def mrc_header_default():
+    h = MrcHeader()
+    lib().gtomx_mrc_header_default(ctypes.byref(h))
+    return h
+
+
+def d_WriteMRC(x, path, stream=None):
+    """d_WriteMRC (src/IO/mrc.cpp:108-115) of a float32 CUDA tensor (z, y, x)."""
+    nz, ny, nx = x.shape[-3:]
+    check(lib().gtomx_mrc_write_device_f32(_ptr(x), int3(nx, ny, nz), os.fsencode(path), _stream(stream)),
+          "gtomx_mrc_write_device_f32")
+
+
+def mrc_to_device(data, mode, device="cuda", stream=None):
+    """MixedToDeviceTfloat (src/Helpers/Memory.cu:424-436): MRC mode 0/1/2/6 data to a
+    float32 CUDA tensor, widened on the GPU."""
+    import torch
+    a = np.ascontiguousarray(data)
+    out = torch.empty(a.shape, dtype=torch.float32, device=device)
+    check(lib().gtomx_mrc_to_device_f32(ctypes.c_void_p(a.ctypes.data), int(mode), a.size, _ptr(out),
+                                        _stream(stream)), "gtomx_mrc_to_device_f32")
+    return out
+
+
+def d_FFTFullPad(x, newshape, stream=None):
+    """d_FFTFullPad (src/FFT/FFTResize.cu:92-110) of real volumes (..., z, y, x) in FFT
+    order to newshape (z, y, x)."""
+    import torch
+    oz, oy, ox = x.shape[-3:]
+    nz, ny, nx = newshape
+    batch = max(1, x.numel() // (ox * oy * oz))
+    out = torch.empty(tuple(x.shape[:-3]) + (nz, ny, nx), dtype=torch.float32, device=x.device)
+    check(lib().gtomx_fft_full_pad(_ptr(x), _ptr(out), int3(ox, oy, oz), int3(nx, ny, nz), batch, _stream(stream)),
+          "gtomx_fft_full_pad")
+    return out
+
+
+def d_NormMean01Std(x, batch=1, out=None, stream=None):
+    """d_Norm with T_NORM_MEAN01STD (src/ImageManipulation/Norm.cu:36-70) over `batch`
+    equal parts of x."""
+    import torch
+    out = torch.empty_like(x) if out is None else out
+    check(lib().gtomx_norm_mean01std(_ptr(x), _ptr(out), x.numel() // batch, batch, _stream(stream)),
+          "gtomx_norm_mean01std")
+    return out
+
+
+class Picker:
+    """The Picker class (include/Correlation.cuh:136-164): initialize with a cubic
+    reference (z, y, x) and the image shape; set_image; perform_correlation."""
+
+    def __init__(self, ref, imageshape, refmask=None, batch=0, stream=None):
+        nr = ref.shape[-1]
+        nz, ny, nx = imageshape
+        h = _P()
+        check(lib().gtomx_picker_create(ctypes.byref(h), _ptr(ref), int3(nr, nr, nr), _ptr(refmask),
+                                        int3(nx, ny, nz), batch, _stream(stream)), "gtomx_picker_create")
+        self.handle = h
+        self.imageshape = tuple(imageshape)
+
+    def close(self):
+        if self.handle:
+            lib().gtomx_picker_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_image(self, image, ctf, stream=None):
+        """Normalises `image` in place and keeps `ctf`; afterwards `image` holds its
+        square, as in the reference. Keep `ctf` alive while correlating."""
+        self._ctf = ctf
+        check(lib().gtomx_picker_set_image(self.handle, _ptr(image), _ptr(ctf), _stream(stream)),
+              "gtomx_picker_set_image")
+
+    def perform_correlation(self, best, bestangle, anglestep=None, angles=None, stream=None):
+        """Updates best (z, y, x) and bestangle (z, y, x, 3) in place over the
+        orientations of GetEqualAngularSpacing(anglestep) or an explicit list."""
+        if angles is not None:
+            a = _angles_arr(angles)
+            check(lib().gtomx_picker_correlate_angles(self.handle, ctypes.c_void_p(a.ctypes.data), len(a),
+                                                      _ptr(best), _ptr(bestangle), _stream(stream)),
+                  "gtomx_picker_correlate_angles")
+        else:
+            check(lib().gtomx_picker_correlate(self.handle, float(anglestep), _ptr(best), _ptr(bestangle),
+                                               _stream(stream)), "gtomx_picker_correlate")
 
 
 def _angles_arr(angles):

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "dev_fft.hpp"
@@ -103,11 +104,25 @@ template <int N> __device__ __forceinline__ bool column_in_disc(int x, int y) {
   return x * x + y * y <= (N / 2) * (N / 2);
 }
 
+// Pass 1 runs as one wave of resident workgroups over the work items
+// (tile, orientation), tile = a column tile with at least one column inside
+// the disc (tiles[] lists their workgroup indices w of ColMap), in tile-major
+// order: workgroup g takes the contiguous items [g*I/G, (g+1)*I/G) of the
+// I = ntiles*nb items, so every workgroup does the same share of the work
+// (one launch of one workgroup per tile leaves the last round mostly idle).
+// E and the CTF are re-read when a workgroup moves to its next tile.
+// Resident pass-1 workgroups per CU asked of the compiler: 3 at n = 256 with
+// the centred sampler (at most 168 VGPRs, 12 waves per CU); no bound otherwise.
+template <int N, class PJ> struct Pass1Occupancy {
+  static constexpr int MIN_BLOCKS = (N == 256 && std::is_same<PJ, CentredProjector>::value) ? 3 : 1;
+};
+
 template <int N, class PJ>
-__global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_project_z(
+__global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2, (Pass1Occupancy<N, PJ>::MIN_BLOCKS)) void
+k_pick_project_z(
     PJ P, const float* __restrict__ mats, int nb, const float2* __restrict__ E,
     const float* __restrict__ ctf, float2* __restrict__ G, float* __restrict__ partial,
-    const float2* __restrict__ tw) {
+    const float2* __restrict__ tw, const int* __restrict__ tiles, int ntiles) {
   using L = GLayout<N>;
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
@@ -116,104 +131,123 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_proj
   static_assert(R1 % GRP == 0, "R1 must be a multiple of the sample group");
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
   __shared__ float red[(NT + 63) / 64];
-
+  // The N twiddles of the z transform, staged once per workgroup in LDS (the
+  // first barrier before their use is the one ahead of the first transform).
+  __shared__ float2 twl[N];
   const int t = threadIdx.x, c = t % TX, j = t / TX;
-  int ix, iy;
-  ColMap<N>::map(blockIdx.x, c, ix, iy);
-  const bool nyq_wg = (int)blockIdx.x < ColMap<N>::NQ;
-  const int y = signed_freq(iy, N);
-  {
-    // A workgroup whose columns all lie outside the disc only reports zero sums.
-    int x0, row0;
-    ColMap<N>::map(blockIdx.x, 0, x0, row0);
-    const bool wg_out = nyq_wg ? blockIdx.x != 0 : !column_in_disc<N>(x0, signed_freq(row0, N));
-    if (wg_out) {
-      for (int o = t; o < nb; o += NT) partial[(size_t)o * ColMap<N>::NWG + blockIdx.x] = 0.f;
-      return;
-    }
-  }
-  const bool valid = ix < NH && column_in_disc<N>(ix, y);
-  const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
+  for (int i = t; i < N; i += NT) twl[i] = twiddle<-1, N>(tw, i);
 
-  // E_v and the CTF at this thread's bins, read once per batch (element
-  // (x, y, z) at (z*N + y)*NH + x; the z part is the constant m*R2*N*NH).
+  const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
   constexpr unsigned HB = (unsigned)(NH * N * N);
   const __amdgpu_buffer_rsrc_t rE = make_rsrc(E, HB * 8u);
   const __amdgpu_buffer_rsrc_t rC = make_rsrc(ctf, ctf ? HB * 4u : 0u);
-  const int ebase = valid ? (j * N + iy) * NH + ix : (int)HB;  // invalid lanes read nothing (0)
-  float2 e[R1];
-  float cf[R1];
-#pragma unroll
-  for (int m = 0; m < R1; ++m) {
-    auto ve = __builtin_amdgcn_raw_buffer_load_b64(rE, ebase * 8, m * R2 * N * NH * 8, CPOL_NT);
-    __builtin_memcpy(&e[m], &ve, 8);
-    cf[m] = ctf ? buf_load1(rC, ebase * 4, m * R2 * N * NH * 4) : (valid ? 1.f : 0.f);
-  }
-  // The N twiddles of the z transform, staged once per workgroup in LDS.
-  __shared__ float2 twl[N];
-  for (int i = t; i < N; i += NT) twl[i] = twiddle<-1, N>(tw, i);
-  const bool dc_thread = ix == 0 && iy == 0 && j == 0;  // holds the DC bin as m = 0
-  // Output of this thread's first bin (z = j*(R1/R2)) and the step between its outputs.
-  const int zrow0 = j * (R1 / R2);
-  // Lanes of columns outside the disc store nothing (out-of-range offset).
-  const int gbase8 = !valid ? OUT_OF_RANGE
-                            : (nyq_wg ? (int)L::MAIN + zrow0 * N + iy : (zrow0 * N + iy) * L::PITCH + ix) * 8;
-  const int gstep = nyq_wg ? N : N * L::PITCH;  // per unit of output z
+  const long long items = (long long)ntiles * nb;
+  const long long i1 = items * (blockIdx.x + 1) / gridDim.x;
+  for (long long it = items * blockIdx.x / gridDim.x; it < i1;) {
+    const int ti = (int)(it / nb);
+    const int o0 = (int)(it % nb);
+    const int o1 = (int)min((long long)nb, o0 + (i1 - it));
+    it += o1 - o0;
+    const int w = tiles[ti];
+    int ix, iy;
+    ColMap<N>::map(w, c, ix, iy);
+    const bool nyq_wg = w < ColMap<N>::NQ;
+    const int y = signed_freq(iy, N);
+    const bool valid = ix < NH && column_in_disc<N>(ix, y);
 
-  for (int o = 0; o < nb; ++o) {
-    float M[9];
+    // E_v and the CTF at this thread's bins of the tile (element (x, y, z) at
+    // (z*N + y)*NH + x; the z part is the constant m*R2*N*NH).
+    const int ebase = valid ? (j * N + iy) * NH + ix : (int)HB;  // invalid lanes read nothing (0)
+    float2 e[R1];
+    float cf[R1];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
-    // q = M (x, y, z) = q0 + z * M[:, 2]
-    const float q0x = M[0] * ix + M[1] * y, q0y = M[3] * ix + M[4] * y, q0z = M[6] * ix + M[7] * y;
-    float acc = 0.f;
-    float2 v[R1];
-    // Samples in groups of GRP: the group's gathers are in flight together, the
-    // scheduling barrier keeps the next group's loads from being hoisted (which
-    // would exhaust the registers), and a group whose bins all lie outside the
-    // sphere in this wave is skipped.
+    for (int m = 0; m < R1; ++m) {
+      auto ve = __builtin_amdgcn_raw_buffer_load_b64(rE, ebase * 8, m * R2 * N * NH * 8, CPOL_NT);
+      __builtin_memcpy(&e[m], &ve, 8);
+      cf[m] = ctf ? buf_load1(rC, ebase * 4, m * R2 * N * NH * 4) : (valid ? 1.f : 0.f);
+    }
+    const bool dc_thread = ix == 0 && iy == 0 && j == 0;  // holds the DC bin as m = 0
+    // Output of this thread's first bin (z = j*(R1/R2)) and the step between its outputs.
+    const int zrow0 = j * (R1 / R2);
+    // Lanes of columns outside the disc store nothing (out-of-range offset).
+    const int gbase8 = !valid ? OUT_OF_RANGE
+                              : (nyq_wg ? (int)L::MAIN + zrow0 * N + iy : (zrow0 * N + iy) * L::PITCH + ix) * 8;
+    const int gstep = nyq_wg ? N : N * L::PITCH;  // per unit of output z
+
+    for (int o = o0; o < o1; ++o) {
+      float M[9];
 #pragma unroll
-    for (int g = 0; g < R1 / GRP; ++g) {
-      bool ins[GRP];
-      bool any = false;
+      for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+      // q = M (x, y, z) = q0 + z * M[:, 2]
+      const float q0x = M[0] * ix + M[1] * y, q0y = M[3] * ix + M[4] * y, q0z = M[6] * ix + M[7] * y;
+      float acc = 0.f;
+      float2 v[R1];
+      // Samples in groups of GRP: the group's gathers are in flight together, the
+      // scheduling barrier keeps the next group's loads from being hoisted (which
+      // would exhaust the registers), and a group whose bins all lie outside the
+      // sphere in this wave is skipped.
 #pragma unroll
-      for (int u = 0; u < GRP; ++u) {
-        const int z = signed_freq(j + R2 * (g * GRP + u), N);
-        ins[u] = valid && ix * ix + y * y + z * z <= RMAX2;
-        any |= ins[u];
-      }
-      if (__any(any)) {
+      for (int g = 0; g < R1 / GRP; ++g) {
+        bool ins[GRP];
+        bool any = false;
 #pragma unroll
         for (int u = 0; u < GRP; ++u) {
-          const int m = g * GRP + u;
-          const float z = (float)signed_freq(j + R2 * m, N);
-          const float2 r = cscale(sample(rs, P, fmaf(M[2], z, q0x), fmaf(M[5], z, q0y), fmaf(M[8], z, q0z), ins[u]),
-                                  cf[m]);
-          acc += r.x * r.x + r.y * r.y;
-          v[m] = cmulc(e[m], r);
+          const int z = signed_freq(j + R2 * (g * GRP + u), N);
+          ins[u] = valid && ix * ix + y * y + z * z <= RMAX2;
+          any |= ins[u];
         }
-      } else {
+        if (__any(any)) {
 #pragma unroll
-        for (int u = 0; u < GRP; ++u) v[g * GRP + u] = make_float2(0.f, 0.f);
+          for (int u = 0; u < GRP; ++u) {
+            const int m = g * GRP + u;
+            const float z = (float)signed_freq(j + R2 * m, N);
+            const float2 r = cscale(
+                sample(rs, P, fmaf(M[2], z, q0x), fmaf(M[5], z, q0y), fmaf(M[8], z, q0z), ins[u]), cf[m]);
+            acc += r.x * r.x + r.y * r.y;
+            v[m] = cmulc(e[m], r);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < GRP; ++u) v[g * GRP + u] = make_float2(0.f, 0.f);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    // DC of Q is zero (the reference's mean subtraction); its |R|^2 term is
-    // removed from the sum by k_pick_herm_norm.
-    if (dc_thread) v[0] = make_float2(0.f, 0.f);
-    __syncthreads();
-    fft_two_step<N, R1, R2, 1>(v, j, twl, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; },
-                               TwiddleLDS{});
-    const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * L::STRIDE, (unsigned)(L::STRIDE * 8));
+      // DC of Q is zero (the reference's mean subtraction); its |R|^2 term is
+      // removed from the sum by k_pick_herm_norm.
+      if (dc_thread) v[0] = make_float2(0.f, 0.f);
+      __syncthreads();
+      fft_two_step<N, R1, R2, 1>(v, j, twl, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; },
+                                 TwiddleLDS{});
+      const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * L::STRIDE, (unsigned)(L::STRIDE * 8));
 #pragma unroll
-    for (int s = 0; s < R1; ++s) {
-      const int u = s / R2, k2 = s % R2;  // output z = j*(R1/R2) + u + R1*k2
-      const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
-      __builtin_amdgcn_raw_buffer_store_b64(vs, rG, gbase8, (u + R1 * k2) * gstep * 8, CPOL_NT);
+      for (int s = 0; s < R1; ++s) {
+        const int u = s / R2, k2 = s % R2;  // output z = j*(R1/R2) + u + R1*k2
+        const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
+        __builtin_amdgcn_raw_buffer_store_b64(vs, rG, gbase8, (u + R1 * k2) * gstep * 8, CPOL_NT);
+      }
+      const float tot = block_sum(acc, red);
+      if (t == 0) partial[(size_t)o * ntiles + ti] = tot;
     }
-    const float tot = block_sum(acc, red);
-    if (t == 0) partial[(size_t)o * ColMap<N>::NWG + blockIdx.x] = tot;
   }
+}
+
+// Workgroup indices w of the pass-1 tiles with at least one column inside
+// the disc, in increasing order (the same test as column_in_disc).
+template <int N> std::vector<int> disc_tiles() {
+  using CM = ColMap<N>;
+  std::vector<int> out;
+  for (int w = 0; w < CM::NWG; ++w) {
+    bool in;
+    if (w < CM::NQ) {
+      in = w == 0;  // the Nyquist column (n/2, y) meets the disc only at y = 0
+    } else {
+      const int x0 = ((w - CM::NQ) % CM::FT) * CM::TX, row = (w - CM::NQ) / CM::FT;
+      const int y = row <= N / 2 ? row : row - N;
+      in = x0 * x0 + y * y <= (N / 2) * (N / 2);
+    }
+    if (in) out.push_back(w);
+  }
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -263,14 +297,13 @@ __global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
-    float2* __restrict__ G, const float* __restrict__ partial, const float* __restrict__ corr,
+    float2* __restrict__ G, const float* __restrict__ partial, int ntiles, const float* __restrict__ corr,
     float* __restrict__ scale, const float2* __restrict__ tw) {
   using L = GLayout<N>;
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1;
   constexpr size_t H = (size_t)NH * N * N;
   constexpr int NT = TX * R2;
-  constexpr int NWG = ColMap<N>::NWG;
   // The unit-RMS reduction reuses the exchange area after the transform.
   static_assert(R2 == 1 || TX * N * sizeof(float2) >= NT * sizeof(double), "exchange area too small");
   __shared__ float2 xch[R2 > 1 ? TX * N : NT];
@@ -311,7 +344,7 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
     // unit-RMS scale over the H half-spectrum bins (src/ImageManipulation/Norm.cu:476-509)
     __syncthreads();  // the exchange area is free again
     double sum = 0;
-    for (int i = t; i < NWG; i += NT) sum += (double)partial[(size_t)o * NWG + i];
+    for (int i = t; i < ntiles; i += NT) sum += (double)partial[(size_t)o * ntiles + i];
     for (int i = t; i < HERM_BLOCKS; i += NT) sum += (double)corr[(size_t)o * HERM_BLOCKS + i];
     dred[t] = sum;
     __syncthreads();
@@ -441,7 +474,9 @@ struct gtomx_pick_plan {
   unsigned B = 0;
   int device = 0;
   size_t H = 0, V = 0;
-  int nwg1 = 0;  // pass-1 workgroups (partial sums) per orientation
+  int ntiles = 0;        // pass-1 column tiles inside the disc (partial sums per orientation)
+  int* tiles = nullptr;  // their workgroup indices
+  int grid1[2] = {0, 0}; // resident pass-1 workgroups: reference layout, centred layout
   float2* G = nullptr;
   float* partial = nullptr;
   float* corr = nullptr;  // B x HERM_BLOCKS plane corrections of the unit-RMS sums
@@ -474,6 +509,7 @@ void plan_release(gtomx_pick_plan* pl) {
   if (!pl) return;
   dev_free(pl->G);
   dev_free(pl->partial);
+  dev_free(pl->tiles);
   dev_free(pl->corr);
   dev_free(pl->scale);
   dev_free(pl->mats);
@@ -497,15 +533,34 @@ size_t g_stride_for(int n) {
   return 0;
 }
 
-int nwg_for(int n) {
+std::vector<int> tiles_for(int n) {
   switch (n) {
 #define GX_CASE(N) \
   case N:          \
-    return ColMap<N>::NWG;
+    return disc_tiles<N>();
     GX_FOR_SIZES(GX_CASE)
 #undef GX_CASE
   }
-  return 0;
+  return {};
+}
+
+// Workgroups of pass 1 that are resident at once on the whole device.
+template <int N, class PJ> int resident_grid(int cus) {
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pick_project_z<N, PJ>,
+                                                              ColShape<N>::TX * ColShape<N>::R2, 0);
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  return per_cu * cus;
+}
+int resident_grid_for(int n, bool centred, int cus) {
+  switch (n) {
+#define GX_CASE(N) \
+  case N:          \
+    return centred ? resident_grid<N, CentredProjector>(cus) : resident_grid<N, Projector>(cus);
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
+  }
+  return cus;
 }
 
 int event_at(gtomx_pick_plan* pl, size_t i, hipEvent_t* ev) {
@@ -539,15 +594,16 @@ int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float
   constexpr int TX = ColShape<N>::TX, R2 = ColShape<N>::R2;
   constexpr int RW = 256 / RowShape<N / 2>::R2;
   GX_TRY(timed(pl, 0, s, [&]() -> int {
-    hipLaunchKernelGGL((k_pick_project_z<N, PJ>), dim3(ColMap<N>::NWG), dim3(TX * R2), 0, s, PS, mats, nb, E, ctf,
-                       pl->G, pl->partial, tw);
+    const int g = pl->grid1[std::is_same<PJ, CentredProjector>::value ? 1 : 0];
+    hipLaunchKernelGGL((k_pick_project_z<N, PJ>), dim3(g), dim3(TX * R2), 0, s, PS, mats, nb, E, ctf, pl->G,
+                       pl->partial, tw, pl->tiles, pl->ntiles);
     GX_LAUNCHED("k_pick_project_z");
     return GTOMX_OK;
   }));
   hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
   GX_LAUNCHED("k_pick_herm_norm");
   GX_TRY(timed(pl, 1, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick_y<N>, dim3(ColMap<N>::NWG, nb), dim3(TX * R2), 0, s, pl->G, pl->partial,
+    hipLaunchKernelGGL(k_pick_y<N>, dim3(ColMap<N>::NWG, nb), dim3(TX * R2), 0, s, pl->G, pl->partial, pl->ntiles,
                        pl->corr, pl->scale, tw);
     GX_LAUNCHED("k_pick_y");
     return GTOMX_OK;
@@ -644,7 +700,8 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   pl->B = batch ? batch : (dv.x >= 128 ? 32u : 16u);
   pl->H = elements_ft(dv);
   pl->V = elements(dv);
-  pl->nwg1 = nwg_for(pl->n);
+  const std::vector<int> tiles = tiles_for(pl->n);
+  pl->ntiles = (int)tiles.size();
   int s = GTOMX_OK;
   hipError_t he = hipGetDevice(&pl->device);
   if (he != hipSuccess) {
@@ -661,8 +718,11 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   alloc(&d, std::max((size_t)pl->B * g_stride_for(pl->n), pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
-  alloc(&d, (size_t)pl->B * pl->nwg1 * sizeof(float));
+  alloc(&d, (size_t)pl->B * pl->ntiles * sizeof(float));
   pl->partial = static_cast<float*>(d);
+  d = nullptr;
+  alloc(&d, (size_t)pl->ntiles * sizeof(int));
+  pl->tiles = static_cast<int*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->B * HERM_BLOCKS * sizeof(float));
   pl->corr = static_cast<float*>(d);
@@ -682,6 +742,15 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
     return fail(s, msg);
   }
   pl->ls = local_std_carve(pl->ls_mem, dv);
+  he = hipMemcpy(pl->tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice);
+  int cus = 0;
+  if (he == hipSuccess) he = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
+  if (he != hipSuccess) {
+    plan_release(pl);
+    return hip_fail(he, "plan set-up");
+  }
+  pl->grid1[0] = resident_grid_for(pl->n, false, cus);
+  pl->grid1[1] = resident_grid_for(pl->n, true, cus);
   he = hipEventCreateWithFlags(&pl->mats_ev, hipEventDisableTiming);
   if (he != hipSuccess) {
     plan_release(pl);

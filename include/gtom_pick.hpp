@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -139,6 +140,37 @@ inline void d_LocalStd(tfloat* d_map, int3 dimsmap, tfloat* d_fouriermask, tfloa
   detail::check(gtomx_local_std(d_map, detail::i3(dimsmap), localradius, d_std, d_mean, nullptr), "d_LocalStd");
 }
 
+// d_LocalPeaks, include/Correlation.cuh:81 / src/Correlation/LocalPeaks.cu:24-62:
+// h_peaks[b] is malloc'ed (free() it) when volume b has peaks, as in the
+// reference; h_peaksnum[b] receives the count.
+inline void d_LocalPeaks(tfloat* d_input, int3** h_peaks, int* h_peaksnum, int3 dims, int localextent,
+                         tfloat threshold, int batch = 1) {
+  if (batch < 1) throw std::runtime_error("d_LocalPeaks: batch must be positive");
+  size_t cap = 4096;
+  std::vector<gtomx_int3> buf;
+  std::vector<size_t> counts((size_t)batch);
+  for (;;) {
+    buf.resize(cap * (size_t)batch);
+    detail::check(gtomx_local_peaks(d_input, detail::i3(dims), localextent, threshold, (unsigned)batch, buf.data(),
+                                    cap, counts.data(), nullptr),
+                  "d_LocalPeaks");
+    size_t need = 0;
+    for (size_t c : counts) need = c > need ? c : need;
+    if (need <= cap) break;
+    cap = need;
+  }
+  for (int b = 0; b < batch; ++b) {
+    h_peaksnum[b] = (int)counts[(size_t)b];
+    if (counts[(size_t)b] == 0) continue;
+    h_peaks[b] = static_cast<int3*>(std::malloc(counts[(size_t)b] * sizeof(int3)));
+    if (!h_peaks[b]) throw std::runtime_error("d_LocalPeaks: out of host memory");
+    for (size_t k = 0; k < counts[(size_t)b]; ++k) {
+      const gtomx_int3 q = buf[(size_t)b * cap + k];
+      h_peaks[b][k] = make_int3(q.x, q.y, q.z);
+    }
+  }
+}
+
 // GetEqualAngularSpacing, src/Helpers/AngularSpacing.cu:46-67.
 inline std::vector<float3> GetEqualAngularSpacing(float2 phirange, float2 thetarange, float2 psirange,
                                                   float increment) {
@@ -181,6 +213,119 @@ inline void d_PickSubTomograms(tcomplex* d_projectordata, tfloat projectoroversa
   gtomx_pick_plan_destroy(plan);
   detail::check(s, "d_PickSubTomograms");
 }
+
+// MRC I/O, include/IO.cuh:13-96, 165-171. HeaderMRC keeps the reference's
+// member names over the 1024-byte MRC header (the file format fixes the layout).
+enum MRC_DATATYPE { MRC_BYTE = 0, MRC_SHORT = 1, MRC_FLOAT = 2, MRC_SHORTCOMPLEX = 3, MRC_FLOATCOMPLEX = 4,
+                    MRC_UNSIGNEDSHORT = 6, MRC_RGB = 16 };
+struct HeaderMRC {
+  int3 dimensions;
+  MRC_DATATYPE mode;
+  int3 startsubimage;
+  int3 griddimensions;
+  float3 pixelsize;
+  float3 angles;
+  int3 maporder;
+  float minvalue, maxvalue, meanvalue;
+  int spacegroup;
+  int extendedbytes;
+  short creatid;
+  unsigned char extradata1[30];
+  short nint, nreal;
+  unsigned char extradata2[28];
+  short idtype, lens, nd1, nd2, vd1, vd2;
+  float3 tiltoriginal, tiltcurrent, origin;
+  unsigned char cmap[4];
+  unsigned char stamp[4];
+  float stddevvalue;
+  int numlabels;
+  unsigned char labels[10][80];
+  HeaderMRC() { gtomx_mrc_header_default(reinterpret_cast<gtomx_mrc_header*>(this)); }
+};
+static_assert(sizeof(HeaderMRC) == sizeof(gtomx_mrc_header), "HeaderMRC is the 1024-byte MRC header");
+
+inline HeaderMRC ReadMRCHeader(std::string path) {
+  HeaderMRC h;
+  detail::check(gtomx_mrc_read_header(path.c_str(), reinterpret_cast<gtomx_mrc_header*>(&h)), "ReadMRCHeader");
+  return h;
+}
+// *data receives pinned host memory (hipHostFree it), as the reference's cudaMallocHost.
+inline void ReadMRC(std::string path, void** data, int nframe = -1) {
+  HeaderMRC h = ReadMRCHeader(path);
+  static const size_t sizes[17] = {1, 2, 4, 4, 8, 1, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3};
+  if ((int)h.mode < 0 || (int)h.mode > 16) throw std::runtime_error("ReadMRC: unknown mode");
+  size_t bytes = (size_t)h.dimensions.x * h.dimensions.y * sizes[(int)h.mode] * (nframe >= 0 ? 1 : h.dimensions.z);
+  if (hipHostMalloc(data, bytes, hipHostMallocDefault) != hipSuccess) throw std::runtime_error("ReadMRC: out of memory");
+  detail::check(gtomx_mrc_read(path.c_str(), nframe, *data, bytes, nullptr), "ReadMRC");
+}
+inline void WriteMRC(void* data, HeaderMRC header, std::string path) {
+  detail::check(gtomx_mrc_write(data, reinterpret_cast<const gtomx_mrc_header*>(&header), path.c_str()), "WriteMRC");
+}
+inline void WriteMRC(tfloat* data, int3 dims, std::string path) {
+  detail::check(gtomx_mrc_write_f32(data, detail::i3(dims), path.c_str()), "WriteMRC");
+}
+inline void d_WriteMRC(tfloat* d_data, int3 dims, std::string path) {
+  detail::check(gtomx_mrc_write_device_f32(d_data, detail::i3(dims), path.c_str(), nullptr), "d_WriteMRC");
+}
+// MixedToDeviceTfloat(void*, tfloat*, MRC_DATATYPE, size_t), src/Helpers/Memory.cu:424-436.
+inline void MixedToDeviceTfloat(void* h_input, tfloat* d_output, MRC_DATATYPE datatype, size_t elements) {
+  detail::check(gtomx_mrc_to_device_f32(h_input, (int)datatype, elements, d_output, nullptr), "MixedToDeviceTfloat");
+}
+
+// d_FFTFullPad on real volumes, include/FFT.cuh:297.
+inline void d_FFTFullPad(tfloat* d_input, tfloat* d_output, int3 olddims, int3 newdims, int batch = 1) {
+  detail::check(gtomx_fft_full_pad(d_input, d_output, detail::i3(olddims), detail::i3(newdims), (unsigned)batch,
+                                   nullptr),
+                "d_FFTFullPad");
+  detail::sync();
+}
+
+// d_Norm with T_NORM_MEAN01STD and no mask, include/ImageManipulation.cuh:48
+// (other modes and masks are not built).
+inline void d_NormMean01Std(tfloat* d_input, tfloat* d_output, size_t elements, int batch = 1) {
+  detail::check(gtomx_norm_mean01std(d_input, d_output, elements, (unsigned)batch, nullptr), "d_Norm");
+}
+
+// class Picker, include/Correlation.cuh:136-164. The reference's members
+// (cuFFT plans, textures, scratch) live inside the library object; the public
+// dimensions are kept. 3D only.
+class Picker {
+ public:
+  int3 dimsref{}, dimsimage{};
+  uint ndims = 0;
+
+  Picker() = default;
+  Picker(const Picker&) = delete;
+  Picker& operator=(const Picker&) = delete;
+  ~Picker() { gtomx_picker_destroy(handle_); }
+
+  void Initialize(tfloat* _d_ref, int3 _dimsref, tfloat* _d_refmask, int3 _dimsimage) {
+    gtomx_picker_destroy(handle_);
+    handle_ = nullptr;
+    dimsref = _dimsref;
+    dimsimage = _dimsimage;
+    ndims = (uint)detail::ndims(_dimsref);
+    detail::check(gtomx_picker_create(&handle_, _d_ref, detail::i3(_dimsref), _d_refmask, detail::i3(_dimsimage), 0,
+                                      nullptr),
+                  "Picker::Initialize");
+    detail::sync();
+  }
+  // Normalises _d_image and leaves it holding its square, as the reference does;
+  // _d_ctf is kept by pointer.
+  void SetImage(tfloat* _d_image, tfloat* _d_ctf) {
+    detail::check(gtomx_picker_set_image(handle_, _d_image, _d_ctf, nullptr), "Picker::SetImage");
+    detail::sync();
+  }
+  void PerformCorrelation(tfloat anglestep, tfloat* d_bestccf, tfloat3* d_bestangle) {
+    detail::check(gtomx_picker_correlate(handle_, anglestep, d_bestccf, reinterpret_cast<gtomx_float3*>(d_bestangle),
+                                         nullptr),
+                  "Picker::PerformCorrelation");
+    detail::sync();
+  }
+
+ private:
+  gtomx_picker* handle_ = nullptr;
+};
 
 }  // namespace gtom
 

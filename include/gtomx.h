@@ -190,6 +190,60 @@ int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, floa
                     float* d_mean, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* The Picker class (whole-volume matcher)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* d_FFTFullPad on real volumes, include/FFT.cuh:297 / src/FFT/FFTResize.cu:92-110
+ * (kernel :212-238): pads `batch` volumes in FFT order (origin at index 0)
+ * from olddims to newdims around the origin, zero elsewhere. In place
+ * (d_input == d_output, buffer sized for newdims) is allowed and then
+ * synchronises the stream. */
+int gtomx_fft_full_pad(const float* d_input, float* d_output, gtomx_int3 olddims, gtomx_int3 newdims,
+                       unsigned batch, void* stream);
+
+/* d_Norm with T_NORM_MEAN01STD and no mask, include/ImageManipulation.cuh:48 /
+ * src/ImageManipulation/Norm.cu:36-70 (statistics of d_Dev, src/Generics/Dev.cu:13-72):
+ * each of `batch` arrays of `elements` floats becomes (x - mean)/std with the
+ * population std (x - mean when std = 0), statistics accumulated in double.
+ * Synchronises the stream (its scratch is freed on return). */
+int gtomx_norm_mean01std(const float* d_input, float* d_output, size_t elements, unsigned batch,
+                         void* stream);
+
+typedef struct gtomx_picker gtomx_picker;
+
+/* Picker::Initialize, include/Correlation.cuh:161 / src/Correlation/Picker.cu:30-116:
+ * a cubic 3D reference d_ref (centred, real) for images of dimsimage. The
+ * reference is transformed once (R2C of its ifftshift). d_refmask is accepted
+ * and ignored: the reference prepares it but never uses it
+ * (src/Correlation/Picker.cu:56-88, 193-221). batch = orientations per pass
+ * (0 = default). The reference's constructor throws (src/Correlation/Picker.cu:27);
+ * this object runs the pipeline its code describes. */
+int gtomx_picker_create(gtomx_picker** picker, const float* d_ref, gtomx_int3 dimsref,
+                        const float* d_refmask, gtomx_int3 dimsimage, unsigned batch, void* stream);
+int gtomx_picker_destroy(gtomx_picker* picker);
+
+/* Picker::SetImage, include/Correlation.cuh:162 / src/Correlation/Picker.cu:143-156:
+ * normalises d_image to mean 0 / std 1 and transforms it; as in the reference
+ * the caller's d_image is left holding the square of the normalised image.
+ * d_ctf ((nref/2+1)*nref*nref floats) is kept by pointer and must stay valid. */
+int gtomx_picker_set_image(gtomx_picker* picker, float* d_image, const float* d_ctf, void* stream);
+
+/* Picker::PerformCorrelation, include/Correlation.cuh:163 /
+ * src/Correlation/Picker.cu:158-247: for every orientation of
+ * GetEqualAngularSpacing((0, 2pi), (0, pi), (0, 2pi), anglestep), with the
+ * rotation the reference left commented out done by the device-array
+ * projector at the reference size:
+ *   C = C2R(imageft * conj(R2C(norm(FullPad(C2R_n(project(refft) * ctf)))))),
+ * and where C > d_bestccf: d_bestccf = C, d_bestangle = the (rot, tilt, psi)
+ * triple (PickingUpdateKernel, src/Correlation/Picker.cu:249-276). The caller
+ * initialises d_bestccf and d_bestangle, as in the reference. */
+int gtomx_picker_correlate(gtomx_picker* picker, float anglestep, float* d_bestccf,
+                           gtomx_float3* d_bestangle, void* stream);
+/* The same over an explicit orientation list (no reference counterpart). */
+int gtomx_picker_correlate_angles(gtomx_picker* picker, const gtomx_float3* h_angles, unsigned nangles,
+                                  float* d_bestccf, gtomx_float3* d_bestangle, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Peak extraction (the consumer of the search output)                       */
 /* ------------------------------------------------------------------------ */
 
@@ -204,6 +258,62 @@ int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, floa
 int gtomx_local_peaks(const float* d_input, gtomx_int3 dims, int localextent, float threshold,
                       unsigned batch, gtomx_int3* h_peaks, size_t capacity, size_t* h_counts,
                       void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* MRC files (the on-disk format of tomograms, templates and output maps)    */
+/* ------------------------------------------------------------------------ */
+
+/* HeaderMRC, include/IO.cuh:26-96: the 1024-byte MRC header, field for field
+ * (little-endian). cell* are the reference's `pixelsize`, mx..mz its
+ * `griddimensions`, nsymbt its `extendedbytes`. */
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t mode;                 /* 0 byte, 1 short, 2 float, 3 short complex, 4 float complex, 6 ushort, 16 RGB */
+  int32_t startx, starty, startz;
+  int32_t mx, my, mz;
+  float cellx, celly, cellz;
+  float alpha, beta, gamma;
+  int32_t mapc, mapr, maps;
+  float dmin, dmax, dmean;
+  int32_t ispg;
+  int32_t nsymbt;
+  int16_t creatid;
+  uint8_t extra1[30];
+  int16_t nint, nreal;
+  uint8_t extra2[28];
+  int16_t idtype, lens, nd1, nd2, vd1, vd2;
+  float tiltoriginal[3], tiltcurrent[3], origin[3];
+  char cmap[4];
+  uint8_t stamp[4];
+  float rms;
+  int32_t nlabl;
+  char labels[10][80];
+} gtomx_mrc_header;
+
+/* HeaderMRC() defaults (include/IO.cuh:72-95), plus the "MAP " identifier and a
+ * little-endian machine stamp, which the reference leaves unset. */
+void gtomx_mrc_header_default(gtomx_mrc_header* header);
+/* ReadMRCHeader, include/IO.cuh:166 / src/IO/mrc.cpp:38-66. */
+int gtomx_mrc_read_header(const char* path, gtomx_mrc_header* header);
+/* ReadMRC, include/IO.cuh:165 / src/IO/mrc.cpp:7-36: the raw data of z section nframe (nframe >= 0)
+ * or of the whole volume (nframe < 0), in the file's mode, after the header
+ * and the extended header. h_data must hold `capacity` >= the data bytes;
+ * header (may be NULL) receives the header. */
+int gtomx_mrc_read(const char* path, int nframe, void* h_data, size_t capacity, gtomx_mrc_header* header);
+/* WriteMRC(void*, HeaderMRC, path), include/IO.cuh:168 / src/IO/mrc.cpp:68-83: the header and
+ * nx*ny*nz voxels of its mode. No extended header is written, so nsymbt is
+ * written as 0 (the reference writes the field as given). */
+int gtomx_mrc_write(const void* h_data, const gtomx_mrc_header* header, const char* path);
+/* WriteMRC(tfloat*, int3, path), include/IO.cuh:169 / src/IO/mrc.cpp:85-106: float volume with the
+ * default header and its min/max. */
+int gtomx_mrc_write_f32(const float* h_data, gtomx_int3 dims, const char* path);
+/* d_WriteMRC(tfloat*, int3, path), include/IO.cuh:171 / src/IO/mrc.cpp:108-115. Synchronous. */
+int gtomx_mrc_write_device_f32(const float* d_data, gtomx_int3 dims, const char* path, void* stream);
+/* MixedToDeviceTfloat(void*, tfloat*, MRC_DATATYPE, size_t),
+ * src/Helpers/Memory.cu:424-436: modes 0 (as unsigned bytes), 1, 2 and 6 to
+ * float on the device; the raw bytes cross PCIe and are widened on the GPU.
+ * Synchronous. */
+int gtomx_mrc_to_device_f32(const void* h_data, int mode, size_t elements, float* d_output, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* The template-matching path                                                */

@@ -469,6 +469,93 @@ double pick(const T* projft, int p, double oversample, const T* expft,
   return loop_seconds;
 }
 
+// d_FFTFullPad on real volumes (src/FFT/FFTResize.cu:92-110, FFTFullPadEvenKernel
+// :212-238): both volumes in FFT order (origin at index 0). Output element x
+// (per axis) has centred coordinate nr = (x + new/2) % new, which maps to
+// or = nr + (old - new - ((old & (1 - (new & 1))) % 2)) / 2 in the input's centred
+// coordinates (the kernel's `old & 1 - (new & 1)` parses as old & (1 - (new & 1)));
+// inside [0, old) the value is read at (or + (old+1)/2) % old, outside it is 0.
+template <class T>
+void full_pad(const T* in, T* out, int ox, int oy, int oz, int nx, int ny, int nz) {
+  auto shift = [](int o, int n) { return (o - n - ((o & (1 - (n & 1))) % 2)) / 2; };
+  const int sx = shift(ox, nx), sy = shift(oy, ny), sz = shift(oz, nz);
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const int rx = (x + nx / 2) % nx + sx, ry = (y + ny / 2) % ny + sy, rz = (z + nz / 2) % nz + sz;
+        T v = 0;
+        if (rx >= 0 && rx < ox && ry >= 0 && ry < oy && rz >= 0 && rz < oz) {
+          const int ix = (rx + (ox + 1) / 2) % ox, iy = (ry + (oy + 1) / 2) % oy, iz = (rz + (oz + 1) / 2) % oz;
+          v = in[((long long)iz * oy + iy) * ox + ix];
+        }
+        out[((long long)z * ny + y) * nx + x] = v;
+      }
+}
+
+// d_Norm with T_NORM_MEAN01STD and no mask (src/ImageManipulation/Norm.cu:36-70,
+// NormMeanStdDevKernel :192-212, statistics from d_Dev src/Generics/Dev.cu:13-72):
+// mean = sum/N, variance = sum((x - mean)^2)/N (population), out = (x - mean)/std,
+// or x - mean when std = 0. Accumulated in double.
+template <class T> void norm_mean01std(const T* in, T* out, long long n) {
+  double s = 0;
+  for (long long i = 0; i < n; ++i) s += in[i];
+  const double mean = s / (double)n;
+  double v = 0;
+  for (long long i = 0; i < n; ++i) v += ((double)in[i] - mean) * ((double)in[i] - mean);
+  const double sd = std::sqrt(v / (double)n);
+  for (long long i = 0; i < n; ++i) out[i] = (T)(sd == 0 ? (double)in[i] - mean : ((double)in[i] - mean) / sd);
+}
+
+// The Picker class (include/Correlation.cuh:136-164, src/Correlation/Picker.cu:25-276)
+// as its code intends, with the rotation its author left commented out
+// (src/Correlation/Picker.cu:181-187) done by the device-array projector:
+//  Initialize: refft = R2C(ifftshift(ref)) at the reference size (:92-94, :112-114);
+//  SetImage: image -> mean0/std1 (:148), imageft = R2C(image) (:152);
+//  PerformCorrelation, for every orientation (:172-245):
+//    R = project(refft, transpose(Euler(angle)), rmax = nref/2) (:176, :185),
+//    R *= ctf (:189), r = C2R_n(R) (:190), padded = FullPad(r) (:225),
+//    padded -> mean0/std1 (:226), P = R2C(padded) (:227),
+//    C = C2R_u(imageft * conj(P)) (:228-229), best/angle updated where
+//    C > best (PickingUpdateKernel :249-276).
+// One cubic reference of side nr, one image nx x ny x nz, ctf of
+// (nr/2+1)*nr*nr. best/idx/top2 are updated in place (the caller fills them),
+// idx receives the index into `angles`.
+template <class T>
+void picker(const T* ref, int nr, const T* image, int nx, int ny, int nz, const T* ctf, const double* angles,
+            int nangles, T* best, int32_t* idx, T* top2) {
+  const int nrh = nr / 2 + 1;
+  const long long Vr = (long long)nr * nr * nr, Hr = (long long)nrh * nr * nr;
+  const long long V = (long long)nx * ny * nz, H = (long long)(nx / 2 + 1) * ny * nz;
+  std::vector<T> shifted(Vr), r(Vr), padded(V), C(V), img(V);
+  std::vector<cpx<T>> refft(Hr), R(Hr), imageft(H), P(H);
+  remap_full2fullfft(ref, shifted.data(), nr, nr, nr);
+  r2c3(shifted.data(), refft.data(), nr, nr, nr);
+  norm_mean01std(image, img.data(), V);
+  r2c3(img.data(), imageft.data(), nx, ny, nz);
+  for (int o = 0; o < nangles; ++o) {
+    double M[9];
+    projector_matrix(angles[3 * o], angles[3 * o + 1], angles[3 * o + 2], 1.0, M);
+    project1(refft.data(), nr, R.data(), nr, M, nr / 2);
+    for (long long i = 0; i < Hr; ++i) R[i] *= ctf[i];
+    c2r3(R.data(), r.data(), nr, nr, nr);
+    for (long long i = 0; i < Vr; ++i) r[i] /= (T)Vr;
+    full_pad(r.data(), padded.data(), nr, nr, nr, nx, ny, nz);
+    norm_mean01std(padded.data(), padded.data(), V);
+    r2c3(padded.data(), P.data(), nx, ny, nz);
+    for (long long i = 0; i < H; ++i) P[i] = imageft[i] * std::conj(P[i]);
+    c2r3(P.data(), C.data(), nx, ny, nz);
+    for (long long i = 0; i < V; ++i) {
+      if (C[i] > best[i]) {
+        if (top2) top2[i] = best[i];
+        best[i] = C[i];
+        idx[i] = o;
+      } else if (top2 && C[i] > top2[i]) {
+        top2[i] = C[i];
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -630,6 +717,17 @@ long long orc_local_peaks_f32(const float* in, int nx, int ny, int nz, int local
         }
       }
   return count;
+}
+
+void orc_full_pad_f64(const double* in, double* out, int ox, int oy, int oz, int nx, int ny, int nz) {
+  full_pad(in, out, ox, oy, oz, nx, ny, nz);
+}
+
+void orc_norm_mean01std_f64(const double* in, double* out, long long n) { norm_mean01std(in, out, n); }
+
+void orc_picker_f64(const double* ref, int nr, const double* image, int nx, int ny, int nz, const double* ctf,
+                    const double* angles, int nangles, double* best, int32_t* idx, double* top2) {
+  picker(ref, nr, image, nx, ny, nz, ctf, angles, nangles, best, idx, top2);
 }
 
 // Returns the wall time of the orientation loop in seconds.

@@ -43,6 +43,9 @@ def lib():
         L.orc_update_max_argmax_f64.argtypes = [_p, _ll, _i, _i, _p, _p]
         L.orc_local_peaks_f32.argtypes = [_p, _i, _i, _i, _i, _f, _p, _ll]
         L.orc_local_peaks_f32.restype = _ll
+        L.orc_full_pad_f64.argtypes = [_p, _p, _i, _i, _i, _i, _i, _i]
+        L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
+        L.orc_picker_f64.argtypes = [_p, _i, _p, _i, _i, _i, _p, _p, _i, _p, _p, _p]
         pick_args = [_p, _i, _d, _p, _p, _i, _i, _i, _p, _i, _d, _p, _p, _p, _p, _p]
         L.orc_pick_f64.argtypes = pick_args
         L.orc_pick_f64.restype = _d
@@ -187,6 +190,41 @@ def local_peaks(vol, localextent, threshold):
         cap = k
     lin = out[:k]
     return np.stack([lin % nx, (lin // nx) % ny, lin // (nx * ny)], axis=1).astype(np.int64)
+
+
+def full_pad(vol, newshape):
+    """d_FFTFullPad of a real (z, y, x) volume in FFT order to newshape (z, y, x)."""
+    v = _c(vol, np.float64)
+    oz, oy, ox = v.shape
+    nz, ny, nx = newshape
+    out = np.empty(newshape, np.float64)
+    lib().orc_full_pad_f64(_ptr(v), _ptr(out), ox, oy, oz, nx, ny, nz)
+    return out
+
+
+def norm_mean01std(a):
+    """d_Norm with T_NORM_MEAN01STD (population std, double accumulation)."""
+    v = _c(a, np.float64)
+    out = np.empty_like(v)
+    lib().orc_norm_mean01std_f64(_ptr(v), _ptr(out), v.size)
+    return out
+
+
+def picker(ref, image, ctf, angles):
+    """The Picker pipeline (Initialize, SetImage, PerformCorrelation) over an explicit
+    angle list. Returns dict(best, idx, top2) with best/top2 started at -1e30."""
+    r = _c(ref, np.float64)
+    nr = r.shape[0]
+    im = _c(image, np.float64)
+    nz, ny, nx = im.shape
+    c = _c(ctf, np.float64)
+    A = _c(angles, np.float64).reshape(-1, 3)
+    best = np.full(im.shape, -1e30)
+    top2 = np.full(im.shape, -1e30)
+    idx = np.zeros(im.shape, np.int32)
+    lib().orc_picker_f64(_ptr(r), nr, _ptr(im), nx, ny, nz, _ptr(c), _ptr(A), len(A), _ptr(best), _ptr(idx),
+                         _ptr(top2))
+    return dict(best=best, idx=idx, top2=top2)
 
 
 def pick(projft, p, oversample, expft, ctf, n, angles, maskradius, ctf_per_volume=False,

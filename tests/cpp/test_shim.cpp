@@ -113,8 +113,103 @@ static void pick_finds_shift() {
          "peak at (%d,%d,%d)", ax, ay, az);
   EXPECT(ang[arg] == 0.f, "orientation index %g", ang[arg]);
   std::printf("ok pick_finds_shift (peak %.3f at (%d,%d,%d), orientation %g)\n", best[arg], ax, ay, az, ang[arg]);
+
+  // the picked-particle list: the planted position is a local peak above half the maximum
+  int3* peaks[1] = {nullptr};
+  int npeaks[1] = {0};
+  d_LocalPeaks(d_best, peaks, npeaks, make_int3(n, n, n), 4, 0.5f * best[arg]);
+  bool found = false;
+  for (int k = 0; k < npeaks[0]; ++k)
+    found |= peaks[0][k].x == ax && peaks[0][k].y == ay && peaks[0][k].z == az;
+  EXPECT(npeaks[0] >= 1 && found, "%d peaks, planted position not among them", npeaks[0]);
+  std::printf("ok local_peaks (%d peaks above half the maximum)\n", npeaks[0]);
+  std::free(peaks[0]);
   hipFree(d_t); hipFree(d_pad); hipFree(d_shift); hipFree(d_proj); hipFree(d_tomo); hipFree(d_E);
   hipFree(d_ctf); hipFree(d_best); hipFree(d_angle);
+}
+
+// The Picker class: a copy of the reference, shifted by a known amount, is
+// found at that shift with the identity orientation (the first of the list).
+static void picker_finds_copy() {
+  const int nr = 16, n = 32;
+  std::vector<float> ref((size_t)nr * nr * nr);
+  for (int z = 0; z < nr; ++z)
+    for (int y = 0; y < nr; ++y)
+      for (int x = 0; x < nr; ++x) {
+        double dx = x - nr / 2 - 1.0, dy = y - nr / 2 + 2.0, dz = z - nr / 2;
+        ref[((size_t)z * nr + y) * nr + x] = (float)(std::exp(-(dx * dx / 3 + dy * dy / 6 + dz * dz / 2) / 2) +
+                                                     0.7 * std::exp(-((dx - 3) * (dx - 3) + dy * dy + dz * dz) / 2));
+      }
+  float* d_ref = dev<float>(ref.size());
+  hipMemcpy(d_ref, ref.data(), ref.size() * sizeof(float), hipMemcpyHostToDevice);
+  // image: the reference padded to n around its centre, moved by (sx, sy, sz), plus noise
+  std::vector<float> image((size_t)n * n * n, 0.f);
+  const int sx = 3, sy = -4, sz = 5;
+  std::mt19937 rng(9);
+  std::normal_distribution<float> noise(0.f, 0.02f);
+  for (auto& v : image) v = noise(rng);
+  for (int z = 0; z < nr; ++z)
+    for (int y = 0; y < nr; ++y)
+      for (int x = 0; x < nr; ++x) {
+        int X = (x - nr / 2 + sx + n) % n, Y = (y - nr / 2 + sy + n) % n, Z = (z - nr / 2 + sz + n) % n;
+        image[((size_t)Z * n + Y) * n + X] += ref[((size_t)z * nr + y) * nr + x];
+      }
+  float* d_image = dev<float>(image.size());
+  hipMemcpy(d_image, image.data(), image.size() * sizeof(float), hipMemcpyHostToDevice);
+  std::vector<float> ctf((size_t)(nr / 2 + 1) * nr * nr, 1.f);
+  float* d_ctf = dev<float>(ctf.size());
+  hipMemcpy(d_ctf, ctf.data(), ctf.size() * sizeof(float), hipMemcpyHostToDevice);
+  std::vector<float> best0(image.size(), -1e30f);
+  float* d_best = dev<float>(image.size());
+  hipMemcpy(d_best, best0.data(), best0.size() * sizeof(float), hipMemcpyHostToDevice);
+  tfloat3* d_angle = dev<tfloat3>(image.size());
+  hipMemset(d_angle, 0, image.size() * sizeof(tfloat3));
+
+  Picker picker;
+  picker.Initialize(d_ref, make_int3(nr, nr, nr), nullptr, make_int3(n, n, n));
+  picker.SetImage(d_image, d_ctf);
+  picker.PerformCorrelation(90.f * 3.14159265f / 180.f, d_best, d_angle);
+  std::vector<float> best(image.size());
+  std::vector<tfloat3> ang(image.size());
+  hipMemcpy(best.data(), d_best, best.size() * sizeof(float), hipMemcpyDeviceToHost);
+  hipMemcpy(ang.data(), d_angle, ang.size() * sizeof(tfloat3), hipMemcpyDeviceToHost);
+  size_t arg = 0;
+  for (size_t i = 1; i < best.size(); ++i)
+    if (best[i] > best[arg]) arg = i;
+  int ax = (int)(arg % n), ay = (int)((arg / n) % n), az = (int)(arg / ((size_t)n * n));
+  EXPECT(ax == (sx + n) % n && ay == (sy + n) % n && az == (sz + n) % n, "Picker peak at (%d,%d,%d)", ax, ay, az);
+  EXPECT(ang[arg].x == 0.f && ang[arg].y == 0.f && ang[arg].z == 0.f, "Picker angle (%g,%g,%g)", ang[arg].x,
+         ang[arg].y, ang[arg].z);
+  std::printf("ok picker_finds_copy (peak %.1f at (%d,%d,%d))\n", best[arg], ax, ay, az);
+  hipFree(d_ref); hipFree(d_image); hipFree(d_ctf); hipFree(d_best); hipFree(d_angle);
+}
+
+// MRC: a device volume written with d_WriteMRC reads back with ReadMRC, and
+// MixedToDeviceTfloat brings it to the device unchanged.
+static void mrc_round_trip() {
+  const int3 dims = make_int3(12, 10, 8);
+  const size_t n = 12 * 10 * 8;
+  std::vector<float> h(n);
+  for (size_t i = 0; i < n; ++i) h[i] = (float)std::sin(0.37 * i) * 3.f;
+  float* d = dev<float>(n);
+  hipMemcpy(d, h.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  const std::string path = "gtomx_shim_test.mrc";
+  d_WriteMRC(d, dims, path);
+  HeaderMRC hdr = ReadMRCHeader(path);
+  EXPECT(hdr.dimensions.x == 12 && hdr.dimensions.y == 10 && hdr.dimensions.z == 8 && hdr.mode == MRC_FLOAT,
+         "header %d %d %d mode %d", hdr.dimensions.x, hdr.dimensions.y, hdr.dimensions.z, (int)hdr.mode);
+  void* raw = nullptr;
+  ReadMRC(path, &raw);
+  float* d2 = dev<float>(n);
+  MixedToDeviceTfloat(raw, d2, hdr.mode, n);
+  std::vector<float> back(n);
+  hipMemcpy(back.data(), d2, n * sizeof(float), hipMemcpyDeviceToHost);
+  for (size_t i = 0; i < n; ++i) EXPECT(back[i] == h[i], "voxel %zu: %g vs %g", i, back[i], h[i]);
+  EXPECT(hdr.minvalue <= -2.9f && hdr.maxvalue >= 2.9f, "min/max %g %g", hdr.minvalue, hdr.maxvalue);
+  hipHostFree(raw);
+  std::remove(path.c_str());
+  hipFree(d); hipFree(d2);
+  std::printf("ok mrc_round_trip\n");
 }
 
 static void errors_throw() {
@@ -131,6 +226,8 @@ static void errors_throw() {
 int main() {
   fft_roundtrip();
   pick_finds_shift();
+  picker_finds_copy();
+  mrc_round_trip();
   errors_throw();
   std::printf("all passed\n");
   return 0;
