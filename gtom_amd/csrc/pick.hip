@@ -51,8 +51,10 @@ template <int N> struct GLayout {
 // lane (their strided accesses are slower, so they start first), and the
 // remaining FT*N workgroups take TX full columns of one row each. Otherwise
 // every row has FT tiles and lanes past x = n/2 idle.
-template <int N> struct ColMap {
-  static constexpr int TX = ColShape<N>::TX;
+template <int N, int TXW = ColShape<N>::TX> struct ColMap {
+  static constexpr int TX = TXW;
+  static_assert(TXW == ColShape<N>::TX || (GLayout<N>::SPLIT && (N / 2) % TXW == 0),
+                "a tile width other than the layout's needs the split layout");
   static constexpr bool SPLIT = GLayout<N>::SPLIT;
   static constexpr int FT = SPLIT ? (N / 2) / TX : (N / 2 + TX) / TX;
   static constexpr int NQ = SPLIT ? N / TX : 0;
@@ -295,12 +297,20 @@ __global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float
 // ---------------------------------------------------------------------------
 // pass 2
 // ---------------------------------------------------------------------------
+// Column tile width of pass 2 (its only tie to pass 1 is the batch buffer's
+// layout): 32 columns (256-byte row segments) where the split layout allows,
+// which streams measurably better than 16.
+template <int N> struct Pass2Tile {
+  static constexpr int TX = (GLayout<N>::SPLIT && (N / 2) % 32 == 0) ? 32 : ColShape<N>::TX;
+};
+
 template <int N>
-__global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
+__global__ __launch_bounds__(Pass2Tile<N>::TX * ColShape<N>::R2) void k_pick_y(
     float2* __restrict__ G, const float* __restrict__ partial, int ntiles, const float* __restrict__ corr,
     float* __restrict__ scale, const float2* __restrict__ tw) {
   using L = GLayout<N>;
-  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
+  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = Pass2Tile<N>::TX;
+  using CM = ColMap<N, TX>;
   constexpr int NH = N / 2 + 1;
   constexpr size_t H = (size_t)NH * N * N;
   constexpr int NT = TX * R2;
@@ -311,9 +321,9 @@ __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick_y(
 
   const int t = threadIdx.x, c = t % TX, j = t / TX;
   int ix, z;
-  ColMap<N>::map(blockIdx.x, c, ix, z);
+  CM::map(blockIdx.x, c, ix, z);
   const int o = blockIdx.y;
-  const bool nyq_wg = (int)blockIdx.x < ColMap<N>::NQ;
+  const bool nyq_wg = (int)blockIdx.x < CM::NQ;
   // Column (x, z) along y: main rows (stride PITCH) or the compact Nyquist column (stride 1).
   const int base = nyq_wg ? (int)L::MAIN + z * N : z * N * L::PITCH + ix;
   const int ystep = nyq_wg ? 1 : L::PITCH;
@@ -603,8 +613,8 @@ int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float
   hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
   GX_LAUNCHED("k_pick_herm_norm");
   GX_TRY(timed(pl, 1, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick_y<N>, dim3(ColMap<N>::NWG, nb), dim3(TX * R2), 0, s, pl->G, pl->partial, pl->ntiles,
-                       pl->corr, pl->scale, tw);
+    hipLaunchKernelGGL(k_pick_y<N>, dim3(ColMap<N, Pass2Tile<N>::TX>::NWG, nb), dim3(Pass2Tile<N>::TX * R2), 0, s,
+                       pl->G, pl->partial, pl->ntiles, pl->corr, pl->scale, tw);
     GX_LAUNCHED("k_pick_y");
     return GTOMX_OK;
   }));
