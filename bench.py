@@ -92,8 +92,8 @@ def measured_traffic(kernel, n, batch):
         js = json.load(f)
     if js.get("block") != n or js.get("batch") != batch:
         return None
-    rec = js["kernels"].get(f"gx::{kernel}<{n}>")
-    return int(rec["traffic_bytes"]) if rec else None
+    recs = [v for k, v in js["kernels"].items() if k.startswith(f"gx::{kernel}<{n}")]
+    return int(recs[0]["traffic_bytes"]) if recs else None
 
 
 def cpu_baseline(w, target_seconds):
