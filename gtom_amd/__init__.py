@@ -282,26 +282,32 @@ def prepare_projector(template, p, stream=None):
     return d_FFTR2C(d_RemapFull2FullFFT(padded, stream), 3, stream)
 
 
-def d_rlnProject(volume_ft, p, n, matrices=None, angles=None, oversample=1.0, rmax=None, stream=None):
-    """Project a (p, p, p/2+1) half spectrum to `batch` (n, n, n/2+1) half spectra."""
+def d_rlnProject(volume_ft, p, n, matrices=None, angles=None, oversample=1.0, rmax=None, stream=None, ndims=3):
+    """Project a (p, p, p/2+1) half spectrum to `batch` (n, n, n/2+1) half spectra, or with
+    ndims=2 to (n, n/2+1) central sections."""
     import torch
     if matrices is None:
         matrices = torch.from_numpy(projector_matrices(angles, oversample)).to(volume_ft.device)
     matrices = matrices.reshape(-1, 9).contiguous().float()
     batch = matrices.shape[0]
-    out = torch.empty((batch, n, n, n // 2 + 1), dtype=torch.complex64, device=volume_ft.device)
-    check(lib().gtomx_project(_ptr(volume_ft), int3(p, p, p), _ptr(out), int3(n, n, n),
+    nz = n if ndims == 3 else 1
+    shape = (batch, n, n, n // 2 + 1) if ndims == 3 else (batch, n, n // 2 + 1)
+    out = torch.empty(shape, dtype=torch.complex64, device=volume_ft.device)
+    check(lib().gtomx_project(_ptr(volume_ft), int3(p, p, p), _ptr(out), int3(n, n, nz),
                               n // 2 if rmax is None else rmax, _ptr(matrices), batch, _stream(stream)),
           "gtomx_project")
     return out
 
 
-def ctf_3d(params, n, device="cuda", stream=None):
+def ctf_3d(params, n, device="cuda", stream=None, ndims=3):
+    """Radially symmetric CTFs on (n, n, n/2+1) half spectra, or (n, n/2+1) with ndims=2."""
     import torch
     params = list(params) if isinstance(params, (list, tuple)) else [params]
     arr = (CtfParams * len(params))(*params)
-    out = torch.empty((len(params), n, n, n // 2 + 1), dtype=torch.float32, device=device)
-    check(lib().gtomx_ctf_3d(arr, len(params), int3(n, n, n), _ptr(out), _stream(stream)), "gtomx_ctf_3d")
+    nz = n if ndims == 3 else 1
+    shape = (len(params), n, n, n // 2 + 1) if ndims == 3 else (len(params), n, n // 2 + 1)
+    out = torch.empty(shape, dtype=torch.float32, device=device)
+    check(lib().gtomx_ctf_3d(arr, len(params), int3(n, n, nz), _ptr(out), _stream(stream)), "gtomx_ctf_3d")
     return out
 
 
@@ -347,10 +353,12 @@ def d_LocalStd(vol, radius, with_mean=False, stream=None):
 class PickPlan:
     """Owns a gtomx_pick_plan (scratch for d_PickSubTomograms on n^3 volumes)."""
 
-    def __init__(self, p, n, batch=0):
-        self.p, self.n = p, n
+    def __init__(self, p, n, batch=0, ndims=3):
+        """ndims 3: n^3 volumes; ndims 2: n x n images (the reference's 2D branch)."""
+        self.p, self.n, self.ndims = p, n, ndims
+        self.nz = n if ndims == 3 else 1
         h = _P()
-        check(lib().gtomx_pick_plan_create(ctypes.byref(h), int3(p, p, p), int3(n, n, n), batch),
+        check(lib().gtomx_pick_plan_create(ctypes.byref(h), int3(p, p, p), int3(n, n, self.nz), batch),
               "gtomx_pick_plan_create")
         self.handle = h
         b, nbytes = _U(0), _SZ(0)
@@ -516,21 +524,23 @@ def _angles_arr(angles):
 
 def d_PickSubTomograms(plan, projector_ft, oversample, exp_ft, ctf, angles, maskradius,
                        ctf_volume_stride=0, best=None, bestidx=None, bestangle=None, stream=None):
-    """Exhaustive template match; exp_ft is (nvol, n, n, n/2+1) complex64.
+    """Exhaustive template match; exp_ft is (nvol, n, n, n/2+1) complex64 for volumes
+    or (nvol, n, n/2+1) for images (a 2D plan).
 
     Returns (best, bestidx[, bestangle]) with best normalised by the local std.
     """
     import torch
-    n, p = plan.n, plan.p
-    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    n, p, nz = plan.n, plan.p, plan.nz
+    nvol = exp_ft.numel() // (nz * n * (n // 2 + 1))
     dev = exp_ft.device
+    shape = (nvol, n, n, n) if nz > 1 else (nvol, n, n)
     if best is None:
-        best = torch.empty((nvol, n, n, n), dtype=torch.float32, device=dev)
+        best = torch.empty(shape, dtype=torch.float32, device=dev)
     if bestidx is None:
-        bestidx = torch.empty((nvol, n, n, n), dtype=torch.int32, device=dev)
+        bestidx = torch.empty(shape, dtype=torch.int32, device=dev)
     a = _angles_arr(angles)
     check(lib().gtomx_pick_subtomograms(plan.handle, _ptr(projector_ft), oversample, int3(p, p, p), _ptr(exp_ft),
-                                        _ptr(ctf), ctf_volume_stride, int3(n, n, n), nvol, a.ctypes.data,
+                                        _ptr(ctf), ctf_volume_stride, int3(n, n, nz), nvol, a.ctypes.data,
                                         a.shape[0], maskradius, _ptr(best), _ptr(bestidx), _ptr(bestangle),
                                         _stream(stream)), "gtomx_pick_subtomograms")
     return best, bestidx
@@ -538,22 +548,22 @@ def d_PickSubTomograms(plan, projector_ft, oversample, exp_ft, ctf, angles, mask
 
 def pick_begin(plan, nvol, best, bestidx, stream=None):
     n = plan.n
-    check(lib().gtomx_pick_begin(plan.handle, int3(n, n, n), nvol, _ptr(best), _ptr(bestidx), _stream(stream)),
+    check(lib().gtomx_pick_begin(plan.handle, int3(n, n, plan.nz), nvol, _ptr(best), _ptr(bestidx), _stream(stream)),
           "gtomx_pick_begin")
 
 
 def pick_update(plan, projector_ft, oversample, exp_ft, ctf, angles, angle_offset, best, bestidx,
                 ctf_volume_stride=0, stream=None):
     n, p = plan.n, plan.p
-    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    nvol = exp_ft.numel() // (plan.nz * n * (n // 2 + 1))
     a = _angles_arr(angles)
     check(lib().gtomx_pick_update(plan.handle, _ptr(projector_ft), oversample, int3(p, p, p), _ptr(exp_ft),
-                                  _ptr(ctf), ctf_volume_stride, int3(n, n, n), nvol, a.ctypes.data, a.shape[0],
+                                  _ptr(ctf), ctf_volume_stride, int3(n, n, plan.nz), nvol, a.ctypes.data, a.shape[0],
                                   angle_offset, _ptr(best), _ptr(bestidx), _stream(stream)), "gtomx_pick_update")
 
 
 def pick_finish(plan, exp_ft, maskradius, best, bestidx, bestangle=None, stream=None):
     n = plan.n
-    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
-    check(lib().gtomx_pick_finish(plan.handle, _ptr(exp_ft), int3(n, n, n), nvol, maskradius, _ptr(best),
+    nvol = exp_ft.numel() // (plan.nz * n * (n // 2 + 1))
+    check(lib().gtomx_pick_finish(plan.handle, _ptr(exp_ft), int3(n, n, plan.nz), nvol, maskradius, _ptr(best),
                                   _ptr(bestidx), _ptr(bestangle), _stream(stream)), "gtomx_pick_finish")

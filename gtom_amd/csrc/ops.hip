@@ -179,11 +179,12 @@ __global__ void k_local_std_combine(const float* c1, const float* c2, const doub
   }
 }
 
-// Standalone projector (Project3DArraytoNDKernel<3>, src/Relion/Project.cu:89-177)
-__global__ void k_project(Projector P, float2* __restrict__ out, int n, int rmax2, const float* __restrict__ mats,
-                          unsigned batch) {
+// Standalone projector (Project3DArraytoNDKernel<3>, src/Relion/Project.cu:89-177);
+// nz = 1 gives the 2D central sections of Project3DArraytoNDKernel<2> (z = 0).
+__global__ void k_project(Projector P, float2* __restrict__ out, int n, int nz, int rmax2,
+                          const float* __restrict__ mats, unsigned batch) {
   const int nh = n / 2 + 1;
-  const size_t H = (size_t)nh * n * n;
+  const size_t H = (size_t)nh * n * nz;
   const unsigned b = blockIdx.y;
   float M[9];
 #pragma unroll
@@ -192,7 +193,7 @@ __global__ void k_project(Projector P, float2* __restrict__ out, int n, int rmax
   const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
   for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < H; id += (size_t)gridDim.x * blockDim.x) {
     int ix = (int)(id % nh), iy = (int)((id / nh) % n), iz = (int)(id / ((size_t)nh * n));
-    int x = ix, y = signed_freq(iy, n), z = signed_freq(iz, n);
+    int x = ix, y = signed_freq(iy, n), z = nz > 1 ? signed_freq(iz, n) : 0;
     if (x * x + y * y + z * z > rmax2) {
       o[id] = make_float2(0.f, 0.f);
       continue;
@@ -206,14 +207,14 @@ __global__ void k_project(Projector P, float2* __restrict__ out, int n, int rmax
 struct CtfLean {
   double ny, defocus, K1, K2, K3, phaseshift, B, scale;
 };
-__global__ void k_ctf3(const CtfLean* __restrict__ params, int n, float* __restrict__ out) {
+__global__ void k_ctf3(const CtfLean* __restrict__ params, int n, int nz, float* __restrict__ out) {
   const int nh = n / 2 + 1;
-  const size_t H = (size_t)nh * n * n;
+  const size_t H = (size_t)nh * n * nz;
   const CtfLean p = params[blockIdx.y];
   float* o = out + blockIdx.y * H;
   for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < H; id += (size_t)gridDim.x * blockDim.x) {
     int ix = (int)(id % nh), iy = (int)((id / nh) % n), iz = (int)(id / ((size_t)nh * n));
-    int y = ((iy + n / 2) % n) - n / 2, z = ((iz + n / 2) % n) - n / 2;
+    int y = ((iy + n / 2) % n) - n / 2, z = nz > 1 ? ((iz + n / 2) % n) - n / 2 : 0;
     double r = sqrt((double)ix * ix + (double)y * y + (double)z * z);
     double k = r * p.ny, k2 = k * k, k4 = k2 * k2;
     double gamma = p.K1 * p.defocus * k2 + p.K2 * k4 - p.phaseshift - p.K3;
@@ -317,11 +318,13 @@ int local_std_apply(const LocalStdScratch& sc, const float* map, gtomx_int3 d, f
 }
 
 int project_launch(const float2* vol, int p, float2* out, int n, unsigned rmax, const float* d_mats,
-                   unsigned batch, hipStream_t s) {
+                   unsigned batch, hipStream_t s, int nz) {
+  if (nz < 0) nz = n;
   Projector P{vol, p, p / 2 + 1, (p / 2 + 1) * p * p * 8};
-  size_t H = (size_t)(n / 2 + 1) * n * n;
+  size_t H = (size_t)(n / 2 + 1) * n * nz;
   unsigned gx_ = (unsigned)std::min<size_t>((H + TPB - 1) / TPB, 4096);
-  hipLaunchKernelGGL(k_project, dim3(gx_, batch), dim3(TPB), 0, s, P, out, n, (int)(rmax * rmax), d_mats, batch);
+  hipLaunchKernelGGL(k_project, dim3(gx_, batch), dim3(TPB), 0, s, P, out, n, nz, (int)(rmax * rmax), d_mats,
+                     batch);
   GX_LAUNCHED("k_project");
   return GTOMX_OK;
 }
@@ -370,11 +373,13 @@ int gtomx_project(const gtomx_complex* d_volumeft, gtomx_int3 dimsvolume, gtomx_
   if (dimsvolume.x != dimsvolume.y || dimsvolume.x != dimsvolume.z || dimsvolume.x < 2 || dimsvolume.x % 2 ||
       dimsvolume.x > 512)
     return fail(GTOMX_ERR_UNSUPPORTED, "projector volume must be an even cube of side <= 512");
-  if (dimsproj.x != dimsproj.y || dimsproj.x != dimsproj.z || dimsproj.x < 2 || dimsproj.x % 2 || dimsproj.x > 1024)
-    return fail(GTOMX_ERR_UNSUPPORTED, "projections must be 3D even cubes of side <= 1024 (2D is not built)");
+  const bool is2d = dimsproj.z == 1;
+  if (dimsproj.x != dimsproj.y || (!is2d && dimsproj.x != dimsproj.z) || dimsproj.x < 2 || dimsproj.x % 2 ||
+      dimsproj.x > 1024)
+    return fail(GTOMX_ERR_UNSUPPORTED, "projections must be even squares or cubes of side <= 1024");
   rmax = std::min<unsigned>(rmax, (unsigned)dimsproj.x / 2);  // src/Relion/Project.cu:32
   return project_launch(reinterpret_cast<const float2*>(d_volumeft), dimsvolume.x, reinterpret_cast<float2*>(d_proj),
-                        dimsproj.x, rmax, d_matrices, batch, as_stream(stream));
+                        dimsproj.x, rmax, d_matrices, batch, as_stream(stream), is2d ? 1 : dimsproj.x);
 }
 
 int gtomx_project_angles(const gtomx_complex* d_volumeft, gtomx_int3 dimsvolume, gtomx_complex* d_proj,
@@ -395,8 +400,8 @@ int gtomx_project_angles(const gtomx_complex* d_volumeft, gtomx_int3 dimsvolume,
 
 int gtomx_ctf_3d(const gtomx_ctf_params* h_params, unsigned batch, gtomx_int3 dims, float* d_ctf, void* stream) {
   if (!h_params || !d_ctf || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer or zero batch");
-  if (dims.x != dims.y || dims.x != dims.z || dims.x < 2 || dims.x % 2)
-    return fail(GTOMX_ERR_UNSUPPORTED, "the 3D CTF needs an even cube");
+  if (dims.x != dims.y || (dims.z != 1 && dims.x != dims.z) || dims.x < 2 || dims.x % 2)
+    return fail(GTOMX_ERR_UNSUPPORTED, "the CTF needs an even square or cube");
   std::vector<CtfLean> lean(batch);
   for (unsigned b = 0; b < batch; ++b) {
     const gtomx_ctf_params& q = h_params[b];
@@ -419,7 +424,7 @@ int gtomx_ctf_3d(const gtomx_ctf_params* h_params, unsigned batch, gtomx_int3 di
   GX_HIP(hipMemcpyAsync(dp.p, lean.data(), sizeof(CtfLean) * batch, hipMemcpyHostToDevice, s));
   size_t H = elements_ft(dims);
   unsigned gx_ = (unsigned)std::min<size_t>((H + TPB - 1) / TPB, 4096);
-  hipLaunchKernelGGL(k_ctf3, dim3(gx_, batch), dim3(TPB), 0, s, dp.as<CtfLean>(), dims.x, d_ctf);
+  hipLaunchKernelGGL(k_ctf3, dim3(gx_, batch), dim3(TPB), 0, s, dp.as<CtfLean>(), dims.x, dims.z, d_ctf);
   GX_LAUNCHED("k_ctf3");
   GX_HIP(hipStreamSynchronize(s));
   return GTOMX_OK;

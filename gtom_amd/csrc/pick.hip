@@ -44,6 +44,19 @@ template <int N> struct GLayout {
   static constexpr int PITCH = SPLIT ? N / 2 : (N / 2 + 1 + TX - 1) / TX * TX;
   static constexpr size_t MAIN = (size_t)PITCH * N * N;
   static constexpr size_t STRIDE = MAIN + (SPLIT ? (size_t)N * N : 0);
+  static constexpr int ROWS = N * N;  // rows (z, y) of the x transform
+};
+
+// The 2D branch (n x n images): per orientation, n rows (y) of all n/2+1
+// columns, padded to whole column tiles.
+template <int N> struct GLayout2 {
+  static constexpr int TX = ColShape<N>::TX;
+  static constexpr bool SPLIT = false;
+  static constexpr int FT = (N / 2 + 1 + TX - 1) / TX;  // column tiles
+  static constexpr int PITCH = FT * TX;
+  static constexpr size_t MAIN = (size_t)PITCH * N;
+  static constexpr size_t STRIDE = MAIN;
+  static constexpr int ROWS = N;
 };
 
 // Workgroup layout of the column passes (1 and 2). In the split layout the
@@ -372,18 +385,32 @@ __global__ __launch_bounds__(Pass2Tile<N>::TX * ColShape<N>::R2) void k_pick_y(
 // ---------------------------------------------------------------------------
 // pass 3
 // ---------------------------------------------------------------------------
-template <int N>
+// L is the batch buffer's layout (GLayout for volumes, GLayout2 for images).
+// With FRESH the running maximum starts at -1e30 / 0 instead of the values in
+// best/bidx: orientation group blockIdx.y (og orientations from og*blockIdx.y)
+// then writes its own maximum to best/bidx + blockIdx.y * ROWS * N, and
+// k_pick_merge combines the groups in order.
+template <int N, class L = GLayout<N>, bool FRESH = false>
 __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G, int nb, int angle0,
                                                     const float* __restrict__ scale, float* __restrict__ best,
-                                                    int32_t* __restrict__ bidx, const float2* __restrict__ tw) {
-  using L = GLayout<N>;
+                                                    int32_t* __restrict__ bidx, const float2* __restrict__ tw,
+                                                    int og = 0) {
   constexpr int NC = N / 2;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
   constexpr int GS = R2 * (R1 + 1);
   __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
-  const int row = blockIdx.x * RW + r;  // row = z*N + y
-  const bool valid = row < N * N;
+  const int row = blockIdx.x * RW + r;  // row = z*N + y (volumes) or y (images)
+  const bool valid = row < L::ROWS;
+  if (FRESH) {
+    const int o0 = og * (int)blockIdx.y;
+    G += (size_t)o0 * L::STRIDE;
+    scale += o0;
+    angle0 += o0;
+    nb = min(og, nb - o0);
+    best += (size_t)blockIdx.y * L::ROWS * N;
+    bidx += (size_t)blockIdx.y * L::ROWS * N;
+  }
 
   float bv[2 * R1];
   int32_t bi[2 * R1];
@@ -392,8 +419,8 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
 #pragma unroll
   for (int s = 0; s < R1; ++s) {
     const int m = out_index<R1, R2>(j, s);
-    float2 b = valid ? bp[m] : make_float2(0.f, 0.f);
-    int2 a = valid ? ip[m] : make_int2(0, 0);
+    float2 b = valid && !FRESH ? bp[m] : make_float2(-1e30f, -1e30f);
+    int2 a = valid && !FRESH ? ip[m] : make_int2(0, 0);
     bv[2 * s] = b.x;
     bv[2 * s + 1] = b.y;
     bi[2 * s] = a.x;
@@ -474,6 +501,127 @@ __global__ __launch_bounds__(256) void k_centre_projector(const float2* __restri
 
 inline int centred_bytes(int p) { return (p / 2 + 2) * (p + 3) * (p + 3) * 8; }
 
+// Combines the per-group maxima of the 2D pass 3 in increasing group order with
+// strict '>', so the first maximum over all orientations wins.
+__global__ __launch_bounds__(256) void k_pick_merge(const float* __restrict__ gbest, const int32_t* __restrict__ gidx,
+                                                   int groups, size_t v, float* __restrict__ best,
+                                                   int32_t* __restrict__ bidx) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < v; i += (size_t)gridDim.x * 256) {
+    float b = best[i];
+    int32_t k = bidx[i];
+    for (int g = 0; g < groups; ++g) {
+      const float c = gbest[(size_t)g * v + i];
+      if (c > b) {
+        b = c;
+        k = gidx[(size_t)g * v + i];
+      }
+    }
+    best[i] = b;
+    bidx[i] = k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The 2D branch, pass 1: one workgroup per (column tile, orientation) projects
+// the central section M (x, y, 0) (Project3DArraytoNDKernel<2>) at its TX
+// columns and all y, applies the CTF, forms E * conj(R) with the DC zeroed and
+// inverse-transforms along y; per-workgroup sums of |R|^2 go to partial.
+// ---------------------------------------------------------------------------
+template <int N, class PJ>
+__global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_pick2d_project_y(
+    PJ P, const float* __restrict__ mats, const float2* __restrict__ E, const float* __restrict__ ctf,
+    float2* __restrict__ G, float* __restrict__ partial, const float2* __restrict__ tw) {
+  using L = GLayout2<N>;
+  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
+  constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
+  constexpr int NT = TX * R2;
+  __shared__ float2 xch[R2 > 1 ? TX * N : 1];
+  __shared__ float red[(NT + 63) / 64];
+  __shared__ float2 twl[N];
+  const int t = threadIdx.x, c = t % TX, j = t / TX;
+  for (int i = t; i < N; i += NT) twl[i] = twiddle<-1, N>(tw, i);
+  const int tile = blockIdx.x, o = blockIdx.y;
+  const int ix = tile * TX + c;
+  const bool valid = ix < NH;
+  float M[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+  const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
+  float acc = 0.f;
+  float2 v[R1];
+#pragma unroll
+  for (int m = 0; m < R1; ++m) {
+    const int iy = j + R2 * m, y = signed_freq(iy, N);
+    const bool ins = valid && ix * ix + y * y <= RMAX2;
+    const float2 e = valid ? E[iy * NH + ix] : make_float2(0.f, 0.f);
+    const float cf = valid ? (ctf ? ctf[iy * NH + ix] : 1.f) : 0.f;
+    const float2 r = cscale(sample(rs, P, M[0] * ix + M[1] * y, M[3] * ix + M[4] * y, M[6] * ix + M[7] * y, ins),
+                            cf);
+    acc += r.x * r.x + r.y * r.y;
+    v[m] = cmulc(e, r);
+  }
+  if (ix == 0 && j == 0) v[0] = make_float2(0.f, 0.f);  // DC
+  __syncthreads();
+  fft_two_step<N, R1, R2, 1>(v, j, twl, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; },
+                             TwiddleLDS{});
+  float2* g = G + (size_t)o * L::STRIDE;
+#pragma unroll
+  for (int s = 0; s < R1; ++s) g[(size_t)out_index<R1, R2>(j, s) * L::PITCH + ix] = v[s];
+  const float tot = block_sum(acc, red);
+  if (t == 0) partial[(size_t)o * L::FT + tile] = tot;
+}
+
+// The 2D unit-RMS correction: |Herm R|^2 - |R|^2 on the lines x = 0 and
+// x = n/2, Herm R(0, y) = (R(0, y) + conj R(0, -y))/2, DC removed.
+template <int N>
+__global__ __launch_bounds__(256) void k_pick2d_herm_norm(Projector P, const float* __restrict__ mats,
+                                                          const float* __restrict__ ctf, float* __restrict__ corr) {
+  constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
+  __shared__ float red[4];
+  const int o = blockIdx.x;
+  float M[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+  const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
+  float acc = 0.f;
+  for (int b = threadIdx.x; b < 2 * N; b += 256) {
+    const int ix = b < N ? 0 : HALF;
+    const int iy = b % N, iyp = (N - iy) % N;
+    const int y = signed_freq(iy, N), yp = signed_freq(iyp, N);
+    const bool in = ix * ix + y * y <= RMAX2;
+    const float c = ctf ? ctf[iy * NH + ix] : 1.f;
+    const float cp = ctf ? ctf[iyp * NH + ix] : 1.f;
+    const float2 R = cscale(project_at(rs, P.p, P.ph, M[0] * ix + M[1] * y, M[3] * ix + M[4] * y,
+                                       M[6] * ix + M[7] * y, in), c);
+    const float2 Rp = cscale(project_at(rs, P.p, P.ph, M[0] * ix + M[1] * yp, M[3] * ix + M[4] * yp,
+                                        M[6] * ix + M[7] * yp, in), cp);
+    float2 h = make_float2(0.5f * (R.x + Rp.x), 0.5f * (R.y - Rp.y));
+    if (b == 0) h = make_float2(0.f, 0.f);
+    acc += (h.x * h.x + h.y * h.y) - (R.x * R.x + R.y * R.y);
+  }
+  const float tot = block_sum(acc, red);
+  if (threadIdx.x == 0) corr[o] = tot;
+}
+
+// scale[o] = 1/max(sqrt((sum of partial sums + corrections)/H), 1e-20) in double
+// (src/ImageManipulation/Norm.cu:476-509).
+__global__ __launch_bounds__(256) void k_pick_scale(const float* __restrict__ partial, int nparts,
+                                                   const float* __restrict__ corr, int ncorr, double h,
+                                                   float* __restrict__ scale) {
+  __shared__ double red[256];
+  const int o = blockIdx.x, t = threadIdx.x;
+  double sum = 0;
+  for (int i = t; i < nparts; i += 256) sum += (double)partial[(size_t)o * nparts + i];
+  for (int i = t; i < ncorr; i += 256) sum += (double)corr[(size_t)o * ncorr + i];
+  red[t] = sum;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) scale[o] = (float)(1.0 / fmax(sqrt(red[0] / h), 1e-20));
+}
+
 }  // namespace gx
 
 // ---------------------------------------------------------------------------
@@ -481,6 +629,7 @@ inline int centred_bytes(int p) { return (p / 2 + 2) * (p + 3) * (p + 3) * 8; }
 // ---------------------------------------------------------------------------
 struct gtomx_pick_plan {
   int n = 0, p = 0;
+  int nz = 0;  // n for n^3 volumes, 1 for n x n images (the 2D branch)
   unsigned B = 0;
   int device = 0;
   size_t H = 0, V = 0;
@@ -501,6 +650,9 @@ struct gtomx_pick_plan {
   gx::LocalStdScratch ls{};
   float* ebuf = nullptr;
   float2* cproj = nullptr;  // centred projector layout
+  float* gbest = nullptr;   // 2D: per-orientation-group maxima (groups x V)
+  int32_t* gidx = nullptr;
+  int og = 0;               // 2D: orientations per group
   size_t device_bytes = 0;
   // kernel timing
   bool timing = false;
@@ -526,6 +678,8 @@ void plan_release(gtomx_pick_plan* pl) {
   dev_free(pl->ls_mem);
   dev_free(pl->ebuf);
   dev_free(pl->cproj);
+  dev_free(pl->gbest);
+  dev_free(pl->gidx);
   if (pl->h_mats) (void)hipHostFree(pl->h_mats);
   if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
   for (hipEvent_t e : pl->ev_pool) (void)hipEventDestroy(e);
@@ -537,6 +691,28 @@ size_t g_stride_for(int n) {
 #define GX_CASE(N) \
   case N:          \
     return GLayout<N>::STRIDE;
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
+  }
+  return 0;
+}
+
+size_t g2_stride_for(int n) {
+  switch (n) {
+#define GX_CASE(N) \
+  case N:          \
+    return GLayout2<N>::STRIDE;
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
+  }
+  return 0;
+}
+
+int g2_tiles_for(int n) {
+  switch (n) {
+#define GX_CASE(N) \
+  case N:          \
+    return GLayout2<N>::FT;
     GX_FOR_SIZES(GX_CASE)
 #undef GX_CASE
   }
@@ -644,9 +820,58 @@ int run_batch_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projector&
   return fail(GTOMX_ERR_UNSUPPORTED, "volume side not supported");
 }
 
+// The 2D branch: pass 1 (projection + y transform), the line correction and
+// the scale, then pass 3 per orientation group and the in-order merge.
+template <int N, class PJ>
+int run_batch2d(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float* mats, int nb, const float2* E,
+                const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
+  using L = GLayout2<N>;
+  constexpr int TX = ColShape<N>::TX, R2 = ColShape<N>::R2;
+  constexpr int RW = 256 / RowShape<N / 2>::R2;
+  GX_TRY(timed(pl, 0, s, [&]() -> int {
+    hipLaunchKernelGGL((k_pick2d_project_y<N, PJ>), dim3(L::FT, nb), dim3(TX * R2), 0, s, PS, mats, E, ctf, pl->G,
+                       pl->partial, tw);
+    GX_LAUNCHED("k_pick2d_project_y");
+    return GTOMX_OK;
+  }));
+  hipLaunchKernelGGL(k_pick2d_herm_norm<N>, dim3(nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
+  GX_LAUNCHED("k_pick2d_herm_norm");
+  GX_TRY(timed(pl, 1, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick_scale, dim3(nb), dim3(256), 0, s, pl->partial, L::FT, pl->corr, 1, (double)pl->H,
+                       pl->scale);
+    GX_LAUNCHED("k_pick_scale");
+    return GTOMX_OK;
+  }));
+  const int groups = (nb + pl->og - 1) / pl->og;
+  GX_TRY(timed(pl, 2, s, [&]() -> int {
+    hipLaunchKernelGGL((k_pick_x_max<N, L, true>), dim3((N + RW - 1) / RW, groups), dim3(256), 0, s, pl->G, nb,
+                       angle0, pl->scale, pl->gbest, pl->gidx, tw, pl->og);
+    GX_LAUNCHED("k_pick_x_max");
+    const unsigned grid = (unsigned)std::min<size_t>((pl->V + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_pick_merge, dim3(grid), dim3(256), 0, s, pl->gbest, pl->gidx, groups, pl->V, best, idx);
+    GX_LAUNCHED("k_pick_merge");
+    return GTOMX_OK;
+  }));
+  return GTOMX_OK;
+}
+
+int run_batch2d_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projector& P, const float* mats, int nb,
+                  const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
+                  hipStream_t s) {
+  switch (pl->n) {
+#define GX_CASE(N)                                                                         \
+  case N:                                                                                  \
+    return C ? run_batch2d<N>(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, tw, s)       \
+             : run_batch2d<N>(pl, P, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
+  }
+  return fail(GTOMX_ERR_UNSUPPORTED, "image side not supported");
+}
+
 int check_plan_dims(const gtomx_pick_plan* pl, gtomx_int3 dimsvolume, unsigned nvolumes) {
   if (!pl) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
-  if (dimsvolume.x != pl->n || dimsvolume.y != pl->n || dimsvolume.z != pl->n)
+  if (dimsvolume.x != pl->n || dimsvolume.y != pl->n || dimsvolume.z != pl->nz)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "dimsvolume does not match the plan");
   if (nvolumes == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "nvolumes must be positive");
   int dev = 0;
@@ -699,19 +924,24 @@ extern "C" {
 int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, unsigned batch) {
   if (!out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan pointer is NULL");
   *out = nullptr;
-  if (dv.x != dv.y || dv.x != dv.z) return fail(GTOMX_ERR_UNSUPPORTED, "volumes must be cubes");
+  const bool is2d = dv.z == 1;
+  if (dv.x != dv.y || (!is2d && dv.x != dv.z))
+    return fail(GTOMX_ERR_UNSUPPORTED, "volumes must be cubes (or images squares)");
   if (!fft_len_ok(dv.x)) return fail(GTOMX_ERR_UNSUPPORTED, "volume side must be 2^a or 3*2^a in [8, 256]");
   if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 512)
     return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 512");
   if (batch > 256) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 256");
   auto* pl = new gtomx_pick_plan();
   pl->n = dv.x;
+  pl->nz = dv.z;
   pl->p = dp.x;
-  pl->B = batch ? batch : (dv.x >= 128 ? 32u : 16u);
+  // 2D: the reference's 240 orientations per pass (src/Correlation/SubTomograms.cu:35-36)
+  pl->B = batch ? batch : (is2d ? 240u : (dv.x >= 128 ? 32u : 16u));
+  pl->og = dv.x >= 128 ? 2 : 1;
   pl->H = elements_ft(dv);
   pl->V = elements(dv);
-  const std::vector<int> tiles = tiles_for(pl->n);
-  pl->ntiles = (int)tiles.size();
+  const std::vector<int> tiles = is2d ? std::vector<int>{0} : tiles_for(pl->n);
+  pl->ntiles = is2d ? g2_tiles_for(pl->n) : (int)tiles.size();
   int s = GTOMX_OK;
   hipError_t he = hipGetDevice(&pl->device);
   if (he != hipSuccess) {
@@ -725,7 +955,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
     if (s == GTOMX_OK) pl->device_bytes += bytes;
   };
   // The batch buffer also serves as the C2R workspace in pick_finish (H elements).
-  alloc(&d, std::max((size_t)pl->B * g_stride_for(pl->n), pl->H) * sizeof(float2));
+  alloc(&d, std::max((size_t)pl->B * (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n)), pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->B * pl->ntiles * sizeof(float));
@@ -746,13 +976,23 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   d = nullptr;
   alloc(&d, (size_t)centred_bytes(pl->p));
   pl->cproj = static_cast<float2*>(d);
+  if (is2d) {
+    const size_t groups = (pl->B + pl->og - 1) / pl->og;
+    d = nullptr;
+    alloc(&d, groups * pl->V * sizeof(float));
+    pl->gbest = static_cast<float*>(d);
+    d = nullptr;
+    alloc(&d, groups * pl->V * sizeof(int32_t));
+    pl->gidx = static_cast<int32_t*>(d);
+  }
   if (s != GTOMX_OK) {
     std::string msg = gtomx_last_error();
     plan_release(pl);
     return fail(s, msg);
   }
   pl->ls = local_std_carve(pl->ls_mem, dv);
-  he = hipMemcpy(pl->tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice);
+  he = hipMemcpy(pl->tiles, tiles.data(), std::min(tiles.size(), (size_t)pl->ntiles) * sizeof(int),
+                 hipMemcpyHostToDevice);
   int cus = 0;
   if (he == hipSuccess) he = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
   if (he != hipSuccess) {
@@ -868,8 +1108,12 @@ int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
     int32_t* iv = d_idx + (size_t)v * plan->V;
     for (unsigned b0 = 0; b0 < nangles; b0 += plan->B) {
       int nb = (int)std::min<unsigned>(plan->B, nangles - b0);
-      GX_TRY(run_batch_n(plan, centred ? &C : nullptr, P, plan->mats + 9 * (size_t)b0, nb, Ev, cv, bv, iv,
-                         (int)(angle_offset + b0), tw, s));
+      if (plan->nz == 1)
+        GX_TRY(run_batch2d_n(plan, centred ? &C : nullptr, P, plan->mats + 9 * (size_t)b0, nb, Ev, cv, bv, iv,
+                             (int)(angle_offset + b0), tw, s));
+      else
+        GX_TRY(run_batch_n(plan, centred ? &C : nullptr, P, plan->mats + 9 * (size_t)b0, nb, Ev, cv, bv, iv,
+                           (int)(angle_offset + b0), tw, s));
     }
   }
   return GTOMX_OK;

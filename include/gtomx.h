@@ -112,7 +112,8 @@ int gtomx_remap_full2fullfft(const float* d_input, float* d_output, gtomx_int3 d
  * src/Relion/Project.cu:25-43, kernel src/Relion/Project.cu:89-177: samples
  * the (p/2+1, p, p) projector half spectrum at M_b * k for every half-spectrum
  * bin k of a dimsproj volume with |k| <= rmax, for `batch` row-major 3x3
- * matrices (device memory, 9 floats each). 3D projections only. */
+ * matrices (device memory, 9 floats each). dimsproj = (n, n, 1) gives 2D
+ * central sections (k = (x, y, 0), Project3DArraytoNDKernel<2>). */
 int gtomx_project(const gtomx_complex* d_volumeft, gtomx_int3 dimsvolume, gtomx_complex* d_proj,
                   gtomx_int3 dimsproj, unsigned rmax, const float* d_matrices, unsigned batch,
                   void* stream);
@@ -141,8 +142,8 @@ typedef struct {
   float scale;       /* (1) */
 } gtomx_ctf_params;
 
-/* 3D CTF on the half spectrum of a cube of side n (dims.x = dims.y = dims.z),
- * one array of (n/2+1)*n*n floats per parameter set: d_GetCTF<false,false>
+/* Radially symmetric CTF on the half spectrum of a cube of side n (or of an
+ * n x n image, dims.z = 1), one array of (n/2+1)*n*dims.z floats per parameter set: d_GetCTF<false,false>
  * (include/CTF.cuh:195-223) with CTFParamsLean (include/CTF.cuh:100-134) and
  * the 3D frequency mapping of src/CTF/Wiener.cu:99-110. Evaluated in double. */
 int gtomx_ctf_3d(const gtomx_ctf_params* h_params, unsigned batch, gtomx_int3 dims, float* d_ctf,
@@ -322,10 +323,12 @@ int gtomx_mrc_to_device_f32(const void* h_data, int mode, size_t elements, float
 typedef struct gtomx_pick_plan gtomx_pick_plan;
 
 /* Scratch and configuration for d_PickSubTomograms on cubes of side n =
- * dimsvolume.x against a projector of side p = dimsprojector.x. The plan
- * owns every device buffer the search needs (the reference allocates them
- * per call, src/Correlation/SubTomograms.cu:44-56). batch is the number of
- * orientations processed per pass (0 selects the default). */
+ * dimsvolume.x, or on n x n images (dimsvolume.z = 1: the reference's 2D
+ * branch, central sections of the projector), against a projector of side
+ * p = dimsprojector.x. The plan owns every device buffer the search needs
+ * (the reference allocates them per call, src/Correlation/SubTomograms.cu:44-56).
+ * batch is the number of orientations processed per pass (0 selects the
+ * default: 32 or 16 for volumes, 240 for images as in the reference). */
 int gtomx_pick_plan_create(gtomx_pick_plan** plan, gtomx_int3 dimsprojector,
                            gtomx_int3 dimsvolume, unsigned batch);
 int gtomx_pick_plan_destroy(gtomx_pick_plan* plan);

@@ -192,13 +192,16 @@ void c2r3(const cpx<T>* in, T* out, int nx, int ny, int nz) {
 // an address outside the array reads 0 (the reference's behaviour there is
 // undefined), which changes nothing for in-range reads.
 // ---------------------------------------------------------------------------
+// nz = n gives the 3D projection (Project3DArraytoNDKernel<3>), nz = 1 the 2D
+// central section (Project3DArraytoNDKernel<2>: z = 0, sphere test x^2 + y^2).
 template <class T>
 void project1(const cpx<T>* vol, int p, cpx<T>* out, int n, const double* M,
-              int rmax) {
+              int rmax, int nz = -1) {
+  if (nz < 0) nz = n;
   int ph = p / 2 + 1, nh = n / 2 + 1;
   long long total = (long long)ph * p * p;
   int rmax2 = rmax * rmax;
-  long long H = (long long)nh * n * n;
+  long long H = (long long)nh * n * nz;
   auto at = [&](long long xx, long long yy, long long zz) -> cpx<T> {
     long long i = (zz * p + yy) * ph + xx;
     if (i < 0 || i >= total) return cpx<T>(0, 0);
@@ -393,15 +396,18 @@ void ctf3(T* out, int n, double pixelsize_m, double voltage_V, double Cs_m,
 // out = |sigma| > 1e-15 ? best / sigma : 0 (src/Generics/Arithmetics.cu:478-493).
 // The oracle also keeps the second-best CC per voxel.
 // ---------------------------------------------------------------------------
+// nz = n: volumes of n^3; nz = 1: images of n^2 (the reference's ndims == 2 branch,
+// src/Correlation/SubTomograms.cu:33-36, same sequence with 2D transforms).
 template <class T>
 double pick(const T* projft, int p, double oversample, const T* expft,
             const T* ctf, int ctf_per_volume, int n, int nvol,
             const double* angles, int nangles, double maskradius, T* best_out,
-            int32_t* idx_out, T* top1_out, T* top2_out, T* sigma_out) {
+            int32_t* idx_out, T* top1_out, T* top2_out, T* sigma_out, int nz = -1) {
+  if (nz < 0) nz = n;
   const cpx<T>* P = reinterpret_cast<const cpx<T>*>(projft);
   const cpx<T>* E = reinterpret_cast<const cpx<T>*>(expft);
   int nh = n / 2 + 1;
-  long long V = (long long)n * n * n, H = (long long)nh * n * n;
+  long long V = (long long)n * n * nz, H = (long long)nh * n * nz;
   std::vector<cpx<T>> R(H), Y(H), Q(H);
   std::vector<T> r(V), C(V);
   std::vector<T> top1(V * nvol, (T)-1e30), top2(V * nvol, (T)-1e30);
@@ -417,9 +423,9 @@ double pick(const T* projft, int p, double oversample, const T* expft,
     for (int o = 0; o < nangles; ++o) {
       double M[9];
       projector_matrix(angles[3 * o], angles[3 * o + 1], angles[3 * o + 2], oversample, M);
-      project1(P, p, R.data(), n, M, n / 2);
+      project1(P, p, R.data(), n, M, n / 2, nz);
       for (long long i = 0; i < H; ++i) R[i] *= c[i];
-      c2r3(R.data(), r.data(), n, n, n);
+      c2r3(R.data(), r.data(), n, n, nz);
       double s1 = 0, s2 = 0;
       for (long long i = 0; i < V; ++i) {
         double val = r[i];
@@ -430,7 +436,7 @@ double pick(const T* projft, int p, double oversample, const T* expft,
       double sd = std::sqrt((double)V * s2 - s1 * s1) / (double)V;
       T tmean = (T)mean, tinv = (T)(sd > 0 ? 1.0 / sd : 0.0);
       for (long long i = 0; i < V; ++i) r[i] = (r[i] - tmean) * tinv;
-      r2c3(r.data(), Y.data(), n, n, n);
+      r2c3(r.data(), Y.data(), n, n, nz);
       double ss = 0;
       for (long long i = 0; i < H; ++i) ss += (double)std::norm(Y[i]);
       double rms = std::sqrt(ss / (double)H);
@@ -439,7 +445,7 @@ double pick(const T* projft, int p, double oversample, const T* expft,
         Y[i] *= inv;
         Q[i] = Ev[i] * std::conj(Y[i]);
       }
-      c2r3(Q.data(), C.data(), n, n, n);
+      c2r3(Q.data(), C.data(), n, n, nz);
       for (long long i = 0; i < V; ++i) {
         T cc = C[i];
         if (cc > t1[i]) {
@@ -454,8 +460,8 @@ double pick(const T* projft, int p, double oversample, const T* expft,
     loop_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // Local-std normalisation (SubTomograms.cu:103-118)
     std::vector<T> e(V), sig(V);
-    c2r3(Ev, e.data(), n, n, n);
-    local_std(e.data(), n, n, n, maskradius, sig.data(), (T*)nullptr);
+    c2r3(Ev, e.data(), n, n, nz);
+    local_std(e.data(), n, n, nz, maskradius, sig.data(), (T*)nullptr);
     for (long long i = 0; i < V; ++i) {
       T s = sig[i];
       T out = std::fabs(s) > (T)1e-15 ? t1[i] / s : (T)0;
@@ -618,6 +624,11 @@ void orc_project_f64(const double* vol, int p, double* out, int n, const double*
   project1(reinterpret_cast<const cpx<double>*>(vol), p, reinterpret_cast<cpx<double>*>(out), n, M, rmax);
 }
 
+// 2D central sections ((n/2+1) x n output), Project3DArraytoNDKernel<2>.
+void orc_project2d_f64(const double* vol, int p, double* out, int n, const double* M, int rmax) {
+  project1(reinterpret_cast<const cpx<double>*>(vol), p, reinterpret_cast<cpx<double>*>(out), n, M, rmax, 1);
+}
+
 // GetEqualAngularSpacing (src/Helpers/AngularSpacing.cu:46-67), restated with
 // the reference's float32 loop counters. Writes up to maxcount (rot, tilt, psi)
 // triples and returns the total count.
@@ -737,6 +748,15 @@ double orc_pick_f64(const double* projft, int p, double oversample, const double
                     int32_t* idx_out, double* top1_out, double* top2_out, double* sigma_out) {
   return pick<double>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
                       nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+// The same search on n x n images (nvol images, CTF of (n/2+1)*n per image or shared).
+double orc_pick2d_f64(const double* projft, int p, double oversample, const double* expft,
+                      const double* ctf, int ctf_per_volume, int n, int nvol,
+                      const double* angles, int nangles, double maskradius, double* best_out,
+                      int32_t* idx_out, double* top1_out, double* top2_out, double* sigma_out) {
+  return pick<double>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
+                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out, 1);
 }
 
 double orc_pick_f32(const float* projft, int p, double oversample, const float* expft,

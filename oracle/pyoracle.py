@@ -51,6 +51,9 @@ def lib():
         L.orc_pick_f64.restype = _d
         L.orc_pick_f32.argtypes = pick_args
         L.orc_pick_f32.restype = _d
+        L.orc_pick2d_f64.argtypes = pick_args
+        L.orc_pick2d_f64.restype = _d
+        L.orc_project2d_f64.argtypes = [_p, _i, _p, _i, _p, _i]
         _lib = L
     return _lib
 
@@ -122,6 +125,16 @@ def project(vol_ft, p, n, M, rmax=None):
     out = np.empty((n, n, n // 2 + 1), np.complex128)
     Mm = _c(M, np.float64).reshape(9)
     lib().orc_project_f64(_ptr(v), p, _ptr(out), n, _ptr(Mm), n // 2 if rmax is None else int(rmax))
+    return out
+
+
+def project2d(vol_ft, p, n, M, rmax=None):
+    """2D central section: the (p/2+1, p, p) half spectrum sampled at M (x, y, 0) onto (n, n/2+1)."""
+    v = _c(vol_ft, np.complex128)
+    assert v.shape == (p, p, p // 2 + 1)
+    out = np.empty((n, n // 2 + 1), np.complex128)
+    Mm = _c(M, np.float64).reshape(9)
+    lib().orc_project2d_f64(_ptr(v), p, _ptr(out), n, _ptr(Mm), n // 2 if rmax is None else int(rmax))
     return out
 
 
@@ -228,23 +241,27 @@ def picker(ref, image, ctf, angles):
 
 
 def pick(projft, p, oversample, expft, ctf, n, angles, maskradius, ctf_per_volume=False,
-         dtype=np.float64):
-    """Run the whole path. Returns dict(best, idx, top1, top2, sigma, loop_seconds)."""
+         dtype=np.float64, ndims=3):
+    """Run the whole path on n^3 volumes (ndims 3) or n^2 images (ndims 2, float64 only).
+    Returns dict(best, idx, top1, top2, sigma, loop_seconds)."""
     cdt = np.complex128 if dtype == np.float64 else np.complex64
     P = _c(projft, cdt)
     E = _c(expft, cdt)
-    nvol = E.shape[0] if E.ndim == 4 else 1
+    nvol = E.shape[0] if E.ndim == ndims + 1 else 1
     C = _c(ctf, dtype)
     A = _c(angles, np.float64).reshape(-1, 3)
-    V = n ** 3
-    best = np.empty((nvol, n, n, n), dtype)
-    idx = np.empty((nvol, n, n, n), np.int32)
-    top1 = np.empty((nvol, n, n, n), dtype)
-    top2 = np.empty((nvol, n, n, n), dtype)
-    sigma = np.empty((nvol, n, n, n), dtype)
-    fn = lib().orc_pick_f64 if dtype == np.float64 else lib().orc_pick_f32
+    shape = (nvol,) + (n,) * ndims
+    best = np.empty(shape, dtype)
+    idx = np.empty(shape, np.int32)
+    top1 = np.empty(shape, dtype)
+    top2 = np.empty(shape, dtype)
+    sigma = np.empty(shape, dtype)
+    if ndims == 2:
+        assert dtype == np.float64
+        fn = lib().orc_pick2d_f64
+    else:
+        fn = lib().orc_pick_f64 if dtype == np.float64 else lib().orc_pick_f32
     secs = fn(_ptr(P), p, oversample, _ptr(E), _ptr(C), int(bool(ctf_per_volume)), n, nvol,
               _ptr(A), A.shape[0], maskradius, _ptr(best), _ptr(idx), _ptr(top1), _ptr(top2),
               _ptr(sigma))
-    del V
     return dict(best=best, idx=idx, top1=top1, top2=top2, sigma=sigma, loop_seconds=secs)

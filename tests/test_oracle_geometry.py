@@ -158,3 +158,29 @@ def test_project_reads_next_row_with_zero_weight(orc):
     P = _projector(orc, rng, p)
     out = orc.project(P, p, n, np.eye(3))
     assert out[0, 0, n // 2] == P[0, 0, p // 2]
+
+
+def test_project2d_is_the_central_plane_of_the_3d_projection(orc):
+    """Project3DArraytoNDKernel<2> samples M (x, y, 0) with the test x^2 + y^2 <= rmax^2,
+    i.e. exactly the z = 0 plane of the 3D projection."""
+    rng = np.random.default_rng(31)
+    p, n = 24, 16
+    F = np.fft.rfftn(rng.standard_normal((p, p, p)))
+    for k in range(4):
+        M = orc.projector_matrix(*rng.uniform(0, 2 * np.pi, 3), p / n)
+        assert np.array_equal(orc.project2d(F, p, n, M), orc.project(F, p, n, M)[0])
+
+
+def test_project2d_identity_is_the_transform_of_the_z_sum(orc):
+    """Central-section theorem on the grid (p = n, identity): the 2D projection is the 2D
+    transform of the volume summed along z, inside the disc."""
+    rng = np.random.default_rng(32)
+    n = 16
+    vol = rng.standard_normal((n, n, n))
+    got = orc.project2d(np.fft.rfftn(vol), n, n, np.eye(3))
+    ref = np.fft.rfft2(vol.sum(axis=0))
+    ky = np.fft.fftfreq(n, 1 / n)[:, None]
+    kx = np.arange(n // 2 + 1)[None, :]
+    inside = kx ** 2 + ky ** 2 <= (n // 2) ** 2
+    assert np.allclose(got[inside], ref[inside], atol=1e-9 * np.abs(ref).max())
+    assert np.all(got[~inside] == 0)

@@ -197,3 +197,33 @@ def test_float32_twin_agrees_with_float64(orc):
     r32 = orc.pick(P.astype(np.complex64), p, 1.0, E[None].astype(np.complex64), ctf.astype(np.float32), n, ang,
                    4.0, dtype=np.float32)
     assert np.abs(r32["best"] - r64["best"]).max() < 1e-4 * np.abs(r64["best"]).max()
+
+
+def test_pick2d_planted_projection_found():
+    """The 2D branch: an image holding the central section of the template at orientation k,
+    shifted, is matched at that shift with index k."""
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(41)
+    n, box = 32, 16
+    p = 2 * box
+    g = np.arange(box) - box // 2
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    t = np.zeros((box, box, box))
+    for _ in range(6):
+        c = rng.uniform(-4, 4, 3)
+        t += np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / 4.0)
+    padded = np.zeros((p, p, p))
+    padded[p // 2 - box // 2:p // 2 + box // 2, p // 2 - box // 2:p // 2 + box // 2,
+           p // 2 - box // 2:p // 2 + box // 2] = t
+    F = np.fft.rfftn(np.fft.ifftshift(padded))
+    tau = 2 * np.pi
+    angles = po.equal_angular_spacing((0, tau), (0, np.pi), (0, tau), np.deg2rad(60.0))[:20].astype(np.float64)
+    k = 13
+    R = po.project2d(F, p, n, po.projector_matrix(*angles[k], p / n))
+    img = np.roll(np.fft.irfft2(R, s=(n, n)), (5, -7), axis=(0, 1))
+    img = img / img.std() + 0.05 * rng.standard_normal(img.shape)
+    E = np.fft.rfft2(img)
+    out = po.pick(F, p, p / n, E[None], np.ones((n, n // 2 + 1)), n, angles, 6.0, ndims=2)
+    pos = np.unravel_index(np.argmax(out["top1"][0]), (n, n))
+    assert pos == (5, n - 7)
+    assert out["idx"][0][pos] == k
