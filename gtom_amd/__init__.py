@@ -96,6 +96,7 @@ _SIGS = {
     "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
     "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
     "gtomx_fft_full_pad": [_P, _P, int3, int3, _U, _P],
+    "gtomx_pick_subtomograms_diff2": [_P, _F, int3, _P, _P, int3, _U, int3, _P, _U, _P, _P, _P],
     "gtomx_mrc_header_default": [_P],
     "gtomx_mrc_read_header": [ctypes.c_char_p, _P],
     "gtomx_mrc_read": [ctypes.c_char_p, _I, _P, _SZ, _P],
@@ -544,6 +545,24 @@ def d_PickSubTomograms(plan, projector_ft, oversample, exp_ft, ctf, angles, mask
                                         a.shape[0], maskradius, _ptr(best), _ptr(bestidx), _ptr(bestangle),
                                         _stream(stream)), "gtomx_pick_subtomograms")
     return best, bestidx
+
+
+def d_PickSubTomogramsDiff2(projector_ft, p, oversample, exp_ft, ctf, relevant, angles, stream=None):
+    """d_PickSubTomogramsDiff2 (src/Correlation/SubTomograms.cu:165-313): exp_ft and ctf are
+    (nvol, n, n, n/2+1); relevant = (rz, ry, rx). Returns (best score, orientation index
+    as float), each (nvol, rz, ry, rx)."""
+    import torch
+    n = exp_ft.shape[-2]
+    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    rz, ry, rx = relevant
+    best = torch.empty((nvol, rz, ry, rx), dtype=torch.float32, device=exp_ft.device)
+    ang = torch.empty_like(best)
+    a = _angles_arr(angles)
+    check(lib().gtomx_pick_subtomograms_diff2(_ptr(projector_ft), oversample, int3(p, p, p), _ptr(exp_ft), _ptr(ctf),
+                                              int3(n, n, n), nvol, int3(rx, ry, rz), a.ctypes.data, a.shape[0],
+                                              _ptr(best), _ptr(ang), _stream(stream)),
+          "gtomx_pick_subtomograms_diff2")
+    return best, ang
 
 
 def pick_begin(plan, nvol, best, bestidx, stream=None):

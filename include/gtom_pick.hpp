@@ -140,6 +140,21 @@ inline void d_LocalStd(tfloat* d_map, int3 dimsmap, tfloat* d_fouriermask, tfloa
   detail::check(gtomx_local_std(d_map, detail::i3(dimsmap), localradius, d_std, d_mean, nullptr), "d_LocalStd");
 }
 
+// d_PickSubTomogramsDiff2, include/Correlation.cuh:111-123, with the
+// device-array projector in place of the texture pair.
+inline void d_PickSubTomogramsDiff2(tcomplex* d_projectordata, tfloat projectoroversample, int3 dimsprojector,
+                                    tcomplex* d_experimentalft, tfloat* d_ctf, int3 dimsvolume, uint nvolumes,
+                                    int3 dimsrelevant, tfloat3* h_angles, uint nangles, tfloat* d_bestdiff2,
+                                    float* d_bestangle) {
+  detail::check(gtomx_pick_subtomograms_diff2(reinterpret_cast<const gtomx_complex*>(d_projectordata),
+                                              projectoroversample, detail::i3(dimsprojector),
+                                              reinterpret_cast<const gtomx_complex*>(d_experimentalft), d_ctf,
+                                              detail::i3(dimsvolume), nvolumes, detail::i3(dimsrelevant),
+                                              reinterpret_cast<const gtomx_float3*>(h_angles), nangles, d_bestdiff2,
+                                              d_bestangle, nullptr),
+                "d_PickSubTomogramsDiff2");
+}
+
 // d_LocalPeaks, include/Correlation.cuh:81 / src/Correlation/LocalPeaks.cu:24-62:
 // h_peaks[b] is malloc'ed (free() it) when volume b has peaks, as in the
 // reference; h_peaksnum[b] receives the count.

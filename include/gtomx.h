@@ -191,6 +191,25 @@ int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, floa
                     float* d_mean, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* The squared-difference scorer                                             */
+/* ------------------------------------------------------------------------ */
+
+/* d_PickSubTomogramsDiff2, include/Correlation.cuh:111-123 /
+ * src/Correlation/SubTomograms.cu:165-313 (device-array projector instead of
+ * the texture pair): for each of nvolumes cubes (E and CTF of (n/2+1)*n*n
+ * each, one CTF per volume), every voxel of dimsrelevant (the shift
+ * d = index - dimsrelevant/2 around the volume origin) and every orientation,
+ * score = sum over the half spectrum of |R ctf e^{-2 pi i k.d/n} - E|^2;
+ * d_bestdiff2 receives the smallest score (started at 1e30, strict '<') and
+ * d_bestangle its orientation index as float (Elements(dimsrelevant) per
+ * volume). Synchronous (its scratch is freed on return). */
+int gtomx_pick_subtomograms_diff2(const gtomx_complex* d_projectorft, float projectoroversample,
+                                  gtomx_int3 dimsprojector, const gtomx_complex* d_experimentalft,
+                                  const float* d_ctf, gtomx_int3 dimsvolume, unsigned nvolumes,
+                                  gtomx_int3 dimsrelevant, const gtomx_float3* h_angles, unsigned nangles,
+                                  float* d_bestdiff2, float* d_bestangle, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* The Picker class (whole-volume matcher)                                   */
 /* ------------------------------------------------------------------------ */
 

@@ -562,6 +562,57 @@ void picker(const T* ref, int nr, const T* image, int nx, int ny, int nz, const 
   }
 }
 
+// d_PickSubTomogramsDiff2 (src/Correlation/SubTomograms.cu:165-209, kernel
+// PickSubTomogramsDiff2Kernel :211-313), literally in double: for every voxel
+// (ix, iy, iz) of dimsrelevant the shift
+//   s = -((ix - rx/2)/nx, (iy - ry/2)/ny, (iz - rz/2)/nz) * 2 pi   (rx/2 etc. integer)
+// and for every orientation a the score
+//   sum over all half-spectrum bins k of |R_a(k) ctf(k) exp(i k.s) - E(k)|^2,
+// with k = (x, signed y, signed z) (:248-250); best = the smallest score with
+// strict '<' in increasing orientation (:285-299, :303-311), started at 1e30.
+// One cubic volume per call (the reference loops volumes with its own CTF).
+void diff2(const cpx<double>* P, int p, double oversample, const cpx<double>* E, const double* ctf, int n,
+           int rx, int ry, int rz, const double* angles, int nangles, double* best, int32_t* idx,
+           double* second) {
+  const int nh = n / 2 + 1;
+  const long long H = (long long)nh * n * n;
+  const long long Vr = (long long)rx * ry * rz;
+  std::vector<cpx<double>> R(H);
+  for (long long i = 0; i < Vr; ++i) {
+    best[i] = 1e30;
+    idx[i] = 0;
+    if (second) second[i] = 1e30;
+  }
+  for (int a = 0; a < nangles; ++a) {
+    double M[9];
+    projector_matrix(angles[3 * a], angles[3 * a + 1], angles[3 * a + 2], oversample, M);
+    project1(P, p, R.data(), n, M, n / 2);
+    for (long long i = 0; i < H; ++i) R[i] *= ctf[i];
+#pragma omp parallel for schedule(static)
+    for (long long v = 0; v < Vr; ++v) {
+      const int ix = (int)(v % rx), iy = (int)((v / rx) % ry), iz = (int)(v / ((long long)rx * ry));
+      const double two_pi = 6.283185307179586476925286766559;
+      const double sx = -((double)ix - rx / 2) / n * two_pi, sy = -((double)iy - ry / 2) / n * two_pi,
+                   sz = -((double)iz - rz / 2) / n * two_pi;
+      double score = 0;
+      for (long long k = 0; k < H; ++k) {
+        const int kx = (int)(k % nh), jy = (int)((k / nh) % n), jz = (int)(k / ((long long)nh * n));
+        const double y = jy > n / 2 ? jy - n : jy, z = jz > n / 2 ? jz - n : jz;
+        const double ph = kx * sx + y * sy + z * sz;
+        const cpx<double> d = R[k] * cpx<double>(std::cos(ph), std::sin(ph)) - E[k];
+        score += std::norm(d);
+      }
+      if (score < best[v]) {
+        if (second) second[v] = best[v];
+        best[v] = score;
+        idx[v] = a;
+      } else if (second && score < second[v]) {
+        second[v] = score;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -728,6 +779,13 @@ long long orc_local_peaks_f32(const float* in, int nx, int ny, int nz, int local
         }
       }
   return count;
+}
+
+void orc_diff2_f64(const double* projft, int p, double oversample, const double* expft, const double* ctf, int n,
+                   int rx, int ry, int rz, const double* angles, int nangles, double* best, int32_t* idx,
+                   double* second) {
+  diff2(reinterpret_cast<const cpx<double>*>(projft), p, oversample, reinterpret_cast<const cpx<double>*>(expft), ctf,
+        n, rx, ry, rz, angles, nangles, best, idx, second);
 }
 
 void orc_full_pad_f64(const double* in, double* out, int ox, int oy, int oz, int nx, int ny, int nz) {

@@ -43,6 +43,7 @@ def lib():
         L.orc_update_max_argmax_f64.argtypes = [_p, _ll, _i, _i, _p, _p]
         L.orc_local_peaks_f32.argtypes = [_p, _i, _i, _i, _i, _f, _p, _ll]
         L.orc_local_peaks_f32.restype = _ll
+        L.orc_diff2_f64.argtypes = [_p, _i, _d, _p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _p]
         L.orc_full_pad_f64.argtypes = [_p, _p, _i, _i, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
         L.orc_picker_f64.argtypes = [_p, _i, _p, _i, _i, _i, _p, _p, _i, _p, _p, _p]
@@ -203,6 +204,22 @@ def local_peaks(vol, localextent, threshold):
         cap = k
     lin = out[:k]
     return np.stack([lin % nx, (lin // nx) % ny, lin // (nx * ny)], axis=1).astype(np.int64)
+
+
+def diff2(projft, p, oversample, expft, ctf, n, relevant, angles, with_second=False):
+    """d_PickSubTomogramsDiff2 on one n^3 volume; relevant = (rz, ry, rx).
+    Returns (best score, orientation index[, second-best score]), each of shape relevant."""
+    P = _c(projft, np.complex128)
+    E = _c(expft, np.complex128)
+    C = _c(ctf, np.float64)
+    A = _c(angles, np.float64).reshape(-1, 3)
+    rz, ry, rx = relevant
+    best = np.empty(relevant, np.float64)
+    idx = np.empty(relevant, np.int32)
+    second = np.empty(relevant, np.float64)
+    lib().orc_diff2_f64(_ptr(P), p, oversample, _ptr(E), _ptr(C), n, rx, ry, rz, _ptr(A), len(A), _ptr(best),
+                        _ptr(idx), _ptr(second))
+    return (best, idx, second) if with_second else (best, idx)
 
 
 def full_pad(vol, newshape):

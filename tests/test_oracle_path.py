@@ -227,3 +227,58 @@ def test_pick2d_planted_projection_found():
     pos = np.unravel_index(np.argmax(out["top1"][0]), (n, n))
     assert pos == (5, n - 7)
     assert out["idx"][0][pos] == k
+
+
+def _diff2_setup(seed=51, n=16, p=16, nang=6):
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(seed)
+    g = np.arange(p) - p // 2
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    t = sum(np.exp(-(((x - c[0]) / 1.5) ** 2 + ((y - c[1]) / 2.0) ** 2 + ((z - c[2]) / 1.2) ** 2) / 2)
+            for c in rng.uniform(-3, 3, (4, 3)))
+    P = np.fft.rfftn(np.fft.ifftshift(t))
+    tau = 2 * np.pi
+    angles = po.equal_angular_spacing((0, tau), (0, np.pi), (0, tau), np.deg2rad(90.0))[:nang].astype(np.float64)
+    ctf = rng.uniform(0.3, 1.0, (n, n, n // 2 + 1))
+    return po, rng, P, angles, ctf
+
+
+def test_diff2_matches_fft_formulation():
+    """sum|R c e^{ik.s} - E|^2 = sum|Rc|^2 + sum|E|^2 - 2 Re X(d), X = forward DFT of
+    Rc conj(E) on the half grid (zero for kx > n/2) evaluated at d = index - r/2."""
+    po, rng, P, angles, ctf = _diff2_setup()
+    n, p = 16, 16
+    E = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    rel = (3, 4, 5)  # (rz, ry, rx)
+    best, idx = po.diff2(P, p, p / n, E, ctf, n, rel, angles)
+    scores = []
+    for a in range(len(angles)):
+        R = po.project(P, p, n, po.projector_matrix(*angles[a], p / n)) * ctf
+        full = np.zeros((n, n, n), np.complex128)
+        full[:, :, :n // 2 + 1] = R * np.conj(E)
+        X = np.fft.fftn(full)
+        dz, dy, dx = [np.arange(r) - r // 2 for r in rel]
+        Xd = X[np.ix_(dz % n, dy % n, dx % n)]
+        scores.append(np.sum(np.abs(R) ** 2) + np.sum(np.abs(E) ** 2) - 2 * Xd.real)
+    scores = np.stack(scores)
+    assert np.allclose(best, scores.min(axis=0), rtol=1e-9, atol=0)
+    assert np.array_equal(idx, scores.argmin(axis=0))
+
+
+def test_diff2_planted_shift_scores_zero():
+    """E built as R_k ctf exp(i k.s0) for orientation k and the shift of one relevant voxel:
+    that voxel scores 0 with index k."""
+    po, rng, P, angles, ctf = _diff2_setup(seed=52)
+    n, p, k = 16, 16, 4
+    rel = (4, 4, 4)
+    vz, vy, vx = 1, 3, 0  # voxel index in the relevant box
+    s = -2 * np.pi * (np.array([vx, vy, vz]) - np.array([2, 2, 2])) / n
+    R = po.project(P, p, n, po.projector_matrix(*angles[k], p / n)) * ctf
+    kz = np.fft.fftfreq(n, 1 / n)[:, None, None]
+    ky = np.fft.fftfreq(n, 1 / n)[None, :, None]
+    kx = np.arange(n // 2 + 1)[None, None, :]
+    E = R * np.exp(1j * (kx * s[0] + ky * s[1] + kz * s[2]))
+    best, idx = po.diff2(P, p, p / n, E, ctf, n, rel, angles)
+    assert best[vz, vy, vx] < 1e-18 * np.sum(np.abs(E) ** 2)
+    assert idx[vz, vy, vx] == k
+    assert np.argmin(best) == np.ravel_multi_index((vz, vy, vx), rel)
