@@ -96,6 +96,8 @@ _SIGS = {
     "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
     "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
     "gtomx_fft_full_pad": [_P, _P, int3, int3, _U, _P],
+    "gtomx_ctf_simulate": [_P, _P, _P, _P, _U, _I, _I, _U, _P],
+    "gtomx_ctf_2d": [_P, _U, int3, _P, _P],
     "gtomx_pick_subtomograms_diff2": [_P, _F, int3, _P, _P, int3, _U, int3, _P, _U, _P, _P, _P],
     "gtomx_mrc_header_default": [_P],
     "gtomx_mrc_read_header": [ctypes.c_char_p, _P],
@@ -309,6 +311,46 @@ def ctf_3d(params, n, device="cuda", stream=None, ndims=3):
     shape = (len(params), n, n, n // 2 + 1) if ndims == 3 else (len(params), n, n // 2 + 1)
     out = torch.empty(shape, dtype=torch.float32, device=device)
     check(lib().gtomx_ctf_3d(arr, len(params), int3(n, n, nz), _ptr(out), _stream(stream)), "gtomx_ctf_3d")
+    return out
+
+
+class CtfParamsFull(ctypes.Structure):
+    """gtomx_ctf_params_full: all of CTFParams (include/CTF.cuh:11-43), SI units."""
+    _fields_ = [(f, ctypes.c_float) for f in ("pixelsize", "pixeldelta", "pixelangle", "Cs", "voltage", "defocus",
+                                              "astigmatismangle", "defocusdelta", "amplitude", "Bfactor",
+                                              "Bfactordelta", "Bfactorangle", "scale", "phaseshift")]
+
+    @classmethod
+    def make(cls, **kw):
+        d = dict(pixelsize=1e-10, pixeldelta=0.0, pixelangle=0.0, Cs=2e-3, voltage=300e3, defocus=-3e-6,
+                 astigmatismangle=0.0, defocusdelta=0.0, amplitude=0.07, Bfactor=0.0, Bfactordelta=0.0,
+                 Bfactorangle=0.0, scale=1.0, phaseshift=0.0)
+        d.update(kw)
+        return cls(**d)
+
+
+def d_CTFSimulate(params, addresses, gammacorrection=None, amplitudesquared=False, ignorefirstpeak=False,
+                  stream=None):
+    """d_CTFSimulate (src/CTF/Simulate.cu:22-47): addresses is an (n, 2) float32 CUDA tensor of
+    (k [cycles/pixel], angle); returns (batch, n)."""
+    import torch
+    params = list(params) if isinstance(params, (list, tuple)) else [params]
+    arr = (CtfParamsFull * len(params))(*params)
+    n = addresses.shape[0]
+    out = torch.empty((len(params), n), dtype=torch.float32, device=addresses.device)
+    check(lib().gtomx_ctf_simulate(arr, _ptr(addresses), _ptr(gammacorrection), _ptr(out), n,
+                                   int(amplitudesquared), int(ignorefirstpeak), len(params), _stream(stream)),
+          "gtomx_ctf_simulate")
+    return out
+
+
+def ctf_2d(params, n, device="cuda", stream=None):
+    """Astigmatic CTFs on (n, n/2+1) half spectra, one per parameter set."""
+    import torch
+    params = list(params) if isinstance(params, (list, tuple)) else [params]
+    arr = (CtfParamsFull * len(params))(*params)
+    out = torch.empty((len(params), n, n // 2 + 1), dtype=torch.float32, device=device)
+    check(lib().gtomx_ctf_2d(arr, len(params), int3(n, n, 1), _ptr(out), _stream(stream)), "gtomx_ctf_2d")
     return out
 
 

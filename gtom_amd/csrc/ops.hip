@@ -224,6 +224,86 @@ __global__ void k_ctf3(const CtfLean* __restrict__ params, int n, int nz, float*
   }
 }
 
+// CTFParams in the units d_GetCTF works in (CTFParamsLean, include/CTF.cuh:100-134).
+struct CtfFullLean {
+  double px, pxdelta, pxangle, defocus, dfdelta, dfangle, K1, K2, K3, phaseshift, B, Bdelta, Bangle, scale;
+};
+
+// d_GetCTF (include/CTF.cuh:195-223) at k [cycles/pixel] and angle, evaluated in double.
+__device__ double ctf_full(const CtfFullLean& p, double k, double angle, double gammacorr, bool ampsq,
+                           bool ignorefirst) {
+  k /= p.px + p.pxdelta * cos(2.0 * (angle - p.pxangle));
+  const double k2 = k * k, k4 = k2 * k2;
+  const double deltaf = p.defocus + p.dfdelta * cos(2.0 * (angle - p.dfangle));
+  const double gamma = p.K1 * deltaf * k2 + p.K2 * k4 - p.phaseshift - p.K3 + gammacorr;
+  double v = (ignorefirst && fabs(gamma) < 1.57079632679489661923) ? 1.0 : -sin(gamma);
+  if (p.B != 0 || p.Bdelta != 0) {
+    double b = p.B;
+    if (p.Bdelta != 0) b += p.Bdelta * cos(2.0 * (angle - p.Bangle));
+    v *= exp(b * k2);
+  }
+  if (ampsq) v = fabs(v);
+  return v * p.scale;
+}
+
+// CTFSimulateKernel (src/CTF/Simulate.cu:117-139): addresses (k, angle).
+__global__ void k_ctf_simulate(const CtfFullLean* __restrict__ params, const float2* __restrict__ addr,
+                               const float* __restrict__ gcorr, float* __restrict__ out, unsigned n, int ampsq,
+                               int ignorefirst) {
+  const CtfFullLean p = params[blockIdx.y];
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float2 a = addr[i];
+    out[(size_t)blockIdx.y * n + i] =
+        (float)ctf_full(p, a.x, a.y, gcorr ? gcorr[i] : 0.0, ampsq != 0, ignorefirst != 0);
+  }
+}
+
+// Half-spectrum CTF images: bin (x, y) at k = |(x, y)|/n cycles per pixel, angle atan2(y, x),
+// y signed in [-n/2, n/2).
+__global__ void k_ctf_2d(const CtfFullLean* __restrict__ params, int n, float* __restrict__ out) {
+  const int nh = n / 2 + 1;
+  const CtfFullLean p = params[blockIdx.y];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nh * n; i += gridDim.x * blockDim.x) {
+    const int x = i % nh, iy = i / nh;
+    const int y = ((iy + n / 2) % n) - n / 2;  // the Nyquist row as -n/2, as k_ctf3 does
+    out[(size_t)blockIdx.y * nh * n + i] =
+        (float)ctf_full(p, sqrt((double)x * x + (double)y * y) / n, atan2((double)y, (double)x), 0.0, false, false);
+  }
+}
+
+CtfFullLean lean_of(const gtomx_ctf_params_full& q) {
+  const double lambda = 12.2643247 / std::sqrt((double)q.voltage * (1.0 + (double)q.voltage * 0.978466e-6));
+  CtfFullLean l;
+  l.px = (double)q.pixelsize * 1e10;
+  l.pxdelta = (double)q.pixeldelta * 0.5e10;
+  l.pxangle = q.pixelangle;
+  l.defocus = (double)q.defocus * 1e10;
+  l.dfdelta = (double)q.defocusdelta * 0.5e10;
+  l.dfangle = q.astigmatismangle;
+  l.K1 = 3.14159265358979323846 * lambda;
+  l.K2 = 1.57079632679489661923 * ((double)q.Cs * 1e10) * lambda * lambda * lambda;
+  l.K3 = std::atan((double)q.amplitude / std::sqrt(1.0 - (double)q.amplitude * q.amplitude));
+  l.phaseshift = q.phaseshift;
+  l.B = (double)q.Bfactor * 0.25e20;
+  l.Bdelta = (double)q.Bfactordelta * 0.25e20;
+  l.Bangle = q.Bfactorangle;
+  l.scale = q.scale;
+  return l;
+}
+
+int upload_full(const gtomx_ctf_params_full* h, unsigned batch, DevBuf& dp, hipStream_t s) {
+  std::vector<CtfFullLean> lean(batch);
+  for (unsigned b = 0; b < batch; ++b) {
+    const gtomx_ctf_params_full& q = h[b];
+    if (!(q.pixelsize > 0) || !(q.voltage > 0) || !(q.amplitude > -1 && q.amplitude < 1))
+      return fail(GTOMX_ERR_INVALID_ARGUMENT, "CTF parameters out of range");
+    lean[b] = lean_of(q);
+  }
+  GX_TRY(dp.alloc(sizeof(CtfFullLean) * batch));
+  GX_HIP(hipMemcpyAsync(dp.p, lean.data(), sizeof(CtfFullLean) * batch, hipMemcpyHostToDevice, s));
+  return GTOMX_OK;
+}
+
 }  // namespace
 
 int fill_f32(float* d, size_t n, float v, hipStream_t s) {
@@ -426,6 +506,38 @@ int gtomx_ctf_3d(const gtomx_ctf_params* h_params, unsigned batch, gtomx_int3 di
   unsigned gx_ = (unsigned)std::min<size_t>((H + TPB - 1) / TPB, 4096);
   hipLaunchKernelGGL(k_ctf3, dim3(gx_, batch), dim3(TPB), 0, s, dp.as<CtfLean>(), dims.x, dims.z, d_ctf);
   GX_LAUNCHED("k_ctf3");
+  GX_HIP(hipStreamSynchronize(s));
+  return GTOMX_OK;
+}
+
+int gtomx_ctf_simulate(const gtomx_ctf_params_full* h_params, const float* d_addresses, const float* d_gammacorrection,
+                       float* d_output, unsigned n, int amplitudesquared, int ignorefirstpeak, unsigned batch,
+                       void* stream) {
+  if (!h_params || !d_addresses || !d_output || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer or zero batch");
+  if (n == 0) return GTOMX_OK;
+  hipStream_t s = as_stream(stream);
+  DevBuf dp;
+  GX_TRY(upload_full(h_params, batch, dp, s));
+  const unsigned g = (unsigned)std::min<size_t>((n + TPB - 1) / TPB, 4096);
+  hipLaunchKernelGGL(k_ctf_simulate, dim3(g, batch), dim3(TPB), 0, s, dp.as<CtfFullLean>(),
+                     reinterpret_cast<const float2*>(d_addresses), d_gammacorrection, d_output, n, amplitudesquared,
+                     ignorefirstpeak);
+  GX_LAUNCHED("k_ctf_simulate");
+  GX_HIP(hipStreamSynchronize(s));
+  return GTOMX_OK;
+}
+
+int gtomx_ctf_2d(const gtomx_ctf_params_full* h_params, unsigned batch, gtomx_int3 dims, float* d_ctf, void* stream) {
+  if (!h_params || !d_ctf || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer or zero batch");
+  if (dims.x != dims.y || dims.z != 1 || dims.x < 2 || dims.x % 2)
+    return fail(GTOMX_ERR_UNSUPPORTED, "2D CTFs need an even square");
+  hipStream_t s = as_stream(stream);
+  DevBuf dp;
+  GX_TRY(upload_full(h_params, batch, dp, s));
+  const size_t H = elements_ft(dims);
+  const unsigned g = (unsigned)std::min<size_t>((H + TPB - 1) / TPB, 4096);
+  hipLaunchKernelGGL(k_ctf_2d, dim3(g, batch), dim3(TPB), 0, s, dp.as<CtfFullLean>(), dims.x, d_ctf);
+  GX_LAUNCHED("k_ctf_2d");
   GX_HIP(hipStreamSynchronize(s));
   return GTOMX_OK;
 }

@@ -285,23 +285,28 @@ __global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float
   for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
   const __amdgpu_buffer_rsrc_t rs = projector_rsrc(P);
   float acc = 0.f;
-  for (int b = blockIdx.x * 256 + threadIdx.x; b < 2 * N * N; b += gridDim.x * 256) {
-    const int ix = b < N * N ? 0 : HALF;
-    const int r = b % (N * N), iy = r % N, iz = r / N;
+  // the x = 0 plane: bins of the disc y^2 + z^2 <= (n/2)^2 (R is 0 elsewhere), paired with -k
+  for (int b = blockIdx.x * 256 + threadIdx.x; b < N * N; b += gridDim.x * 256) {
+    const int iy = b % N, iz = b / N;
     const int y = signed_freq(iy, N), z = signed_freq(iz, N);
+    if (y * y + z * z > RMAX2) continue;
     const int iyp = (N - iy) % N, izp = (N - iz) % N;
     const int yp = signed_freq(iyp, N), zp = signed_freq(izp, N);
-    const bool in = ix * ix + y * y + z * z <= RMAX2;  // the same radius for k and -k
-    const float c = ctf ? ctf[((size_t)iz * N + iy) * NH + ix] : 1.f;
-    const float cp = ctf ? ctf[((size_t)izp * N + iyp) * NH + ix] : 1.f;
-    const float2 R = cscale(project_at(rs, P.p, P.ph, M[0] * ix + M[1] * y + M[2] * z,
-                                       M[3] * ix + M[4] * y + M[5] * z, M[6] * ix + M[7] * y + M[8] * z, in), c);
-    const float2 Rp = cscale(project_at(rs, P.p, P.ph, M[0] * ix + M[1] * yp + M[2] * zp,
-                                        M[3] * ix + M[4] * yp + M[5] * zp, M[6] * ix + M[7] * yp + M[8] * zp, in),
-                             cp);
+    const float c = ctf ? ctf[((size_t)iz * N + iy) * NH] : 1.f;
+    const float cp = ctf ? ctf[((size_t)izp * N + iyp) * NH] : 1.f;
+    const float2 R = cscale(project_at(rs, P.p, P.ph, M[1] * y + M[2] * z, M[4] * y + M[5] * z, M[7] * y + M[8] * z),
+                            c);
+    const float2 Rp = cscale(
+        project_at(rs, P.p, P.ph, M[1] * yp + M[2] * zp, M[4] * yp + M[5] * zp, M[7] * yp + M[8] * zp), cp);
     float2 h = make_float2(0.5f * (R.x + Rp.x), 0.5f * (R.y - Rp.y));
     if (b == 0) h = make_float2(0.f, 0.f);  // DC
     acc += (h.x * h.x + h.y * h.y) - (R.x * R.x + R.y * R.y);
+  }
+  // the x = n/2 plane meets the sphere only at (n/2, 0, 0), its own mirror: Herm R = Re R
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const float c = ctf ? ctf[HALF] : 1.f;
+    const float2 R = cscale(project_at(rs, P.p, P.ph, M[0] * HALF, M[3] * HALF, M[6] * HALF), c);
+    acc -= R.y * R.y;
   }
   const float tot = block_sum(acc, red);
   if (threadIdx.x == 0) corr[(size_t)o * gridDim.x + blockIdx.x] = tot;

@@ -301,6 +301,28 @@ inline void d_NormMean01Std(tfloat* d_input, tfloat* d_output, size_t elements, 
   detail::check(gtomx_norm_mean01std(d_input, d_output, elements, (unsigned)batch, nullptr), "d_Norm");
 }
 
+// CTFParams, include/CTF.cuh:11-43: same fields, order and defaults (SI units,
+// angles in radians), laid out as gtomx_ctf_params_full.
+struct CTFParams {
+  tfloat pixelsize = 1e-10f, pixeldelta = 0, pixelangle = 0;
+  tfloat Cs = 2e-3f, voltage = 300e3f;
+  tfloat defocus = -3e-6f, astigmatismangle = 0, defocusdelta = 0;
+  tfloat amplitude = 0.07f;
+  tfloat Bfactor = 0, Bfactordelta = 0, Bfactorangle = 0;
+  tfloat scale = 1.0f, phaseshift = 0;
+};
+static_assert(sizeof(CTFParams) == sizeof(gtomx_ctf_params_full), "CTFParams layout");
+
+// d_CTFSimulate, include/CTF.cuh:317: batch CTFs at n (k, angle) addresses,
+// d_gammacorrection optional (nullptr), output batch x n.
+inline void d_CTFSimulate(CTFParams* h_params, float2* d_addresses, float* d_gammacorrection, tfloat* d_output,
+                          uint n, bool amplitudesquared = false, bool ignorefirstpeak = false, int batch = 1) {
+  detail::check(gtomx_ctf_simulate(reinterpret_cast<const gtomx_ctf_params_full*>(h_params),
+                                   reinterpret_cast<const float*>(d_addresses), d_gammacorrection, d_output, n,
+                                   amplitudesquared ? 1 : 0, ignorefirstpeak ? 1 : 0, (unsigned)batch, nullptr),
+                "d_CTFSimulate");
+}
+
 // class Picker, include/Correlation.cuh:136-164. The reference's members
 // (cuFFT plans, textures, scratch) live inside the library object; the public
 // dimensions are kept. 3D only.

@@ -149,6 +149,32 @@ typedef struct {
 int gtomx_ctf_3d(const gtomx_ctf_params* h_params, unsigned batch, gtomx_int3 dims, float* d_ctf,
                  void* stream);
 
+/* CTFParams, include/CTF.cuh:11-43, all fields (SI units, angles in radians). */
+typedef struct {
+  float pixelsize, pixeldelta, pixelangle;
+  float Cs, voltage;
+  float defocus, astigmatismangle, defocusdelta;
+  float amplitude;
+  float Bfactor, Bfactordelta, Bfactorangle;
+  float scale, phaseshift;
+} gtomx_ctf_params_full;
+
+/* d_CTFSimulate, include/CTF.cuh:317 / src/CTF/Simulate.cu:22-47 (kernel :117-139):
+ * the CTF of each of `batch` parameter sets at n addresses (d_addresses: n
+ * (k, angle) float pairs, k in cycles per pixel) with astigmatic defocus
+ * defocus + defocusdelta/2 cos 2(angle - astigmatismangle), anisotropic pixel
+ * size and B-factor, optional per-address gamma correction (may be NULL);
+ * output batch x n floats. Evaluated in double. Synchronous. */
+int gtomx_ctf_simulate(const gtomx_ctf_params_full* h_params, const float* d_addresses,
+                       const float* d_gammacorrection, float* d_output, unsigned n, int amplitudesquared,
+                       int ignorefirstpeak, unsigned batch, void* stream);
+
+/* The same on the (n/2+1) x n half spectrum of n x n images (dims.z = 1):
+ * bin (x, y) at k = |(x, y)|/n, angle = atan2(y, x) (y in [-n/2, n/2)), one array per parameter
+ * set: per-micrograph astigmatic CTFs for the 2D branch (no reference
+ * counterpart beyond d_CTFSimulate itself). Synchronous. */
+int gtomx_ctf_2d(const gtomx_ctf_params_full* h_params, unsigned batch, gtomx_int3 dims, float* d_ctf, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Element-wise pieces of the path (drop-ins for the unfused reference)      */
 /* ------------------------------------------------------------------------ */

@@ -613,6 +613,41 @@ void diff2(const cpx<double>* P, int p, double oversample, const cpx<double>* E,
   }
 }
 
+// CTFParams (include/CTF.cuh:11-43) in SI units, as the reference stores them.
+struct CtfFull {
+  double pixelsize, pixeldelta, pixelangle, Cs, voltage, defocus, astigmatismangle, defocusdelta, amplitude,
+      Bfactor, Bfactordelta, Bfactorangle, scale, phaseshift;
+};
+
+// CTFSimulateKernel (src/CTF/Simulate.cu:117-139) with d_GetCTF
+// (include/CTF.cuh:195-223) and CTFParamsLean's unit conversions
+// (include/CTF.cuh:100-134), in double: address (k, angle) with k in cycles per
+// pixel; pixel size pixelsize + pixeldelta/2 cos 2(angle - pixelangle) [A];
+// defocus + defocusdelta/2 cos 2(angle - astigmatismangle) [A];
+// gamma = K1 deltaf k^2 + K2 k^4 - phaseshift - K3 + gammacorrection;
+// -sin(gamma) (1 when ignorefirstpeak and |gamma| < pi/2), times
+// exp((B + dB cos 2(angle - Bangle)) k^2) with B in A^2/4, |.| when
+// amplitudesquared, times scale.
+double ctf_at(const CtfFull& q, double k, double angle, double gammacorr, bool ampsq, bool ignorefirst) {
+  const double pi = 3.14159265358979323846;
+  const double lambda = 12.2643247 / std::sqrt(q.voltage * (1.0 + q.voltage * 0.978466e-6));
+  const double px = q.pixelsize * 1e10 + q.pixeldelta * 0.5e10 * std::cos(2.0 * (angle - q.pixelangle));
+  k /= px;
+  const double K1 = pi * lambda, K2 = pi / 2 * (q.Cs * 1e10) * lambda * lambda * lambda;
+  const double K3 = std::atan(q.amplitude / std::sqrt(1.0 - q.amplitude * q.amplitude));
+  const double k2 = k * k, k4 = k2 * k2;
+  const double deltaf = q.defocus * 1e10 + q.defocusdelta * 0.5e10 * std::cos(2.0 * (angle - q.astigmatismangle));
+  const double gamma = K1 * deltaf * k2 + K2 * k4 - q.phaseshift - K3 + gammacorr;
+  double v = (ignorefirst && std::fabs(gamma) < pi / 2) ? 1.0 : -std::sin(gamma);
+  if (q.Bfactor != 0 || q.Bfactordelta != 0) {
+    double b = q.Bfactor * 0.25e20;
+    if (q.Bfactordelta != 0) b += q.Bfactordelta * 0.25e20 * std::cos(2.0 * (angle - q.Bfactorangle));
+    v *= std::exp(b * k2);
+  }
+  if (ampsq) v = std::fabs(v);
+  return v * q.scale;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -786,6 +821,19 @@ void orc_diff2_f64(const double* projft, int p, double oversample, const double*
                    double* second) {
   diff2(reinterpret_cast<const cpx<double>*>(projft), p, oversample, reinterpret_cast<const cpx<double>*>(expft), ctf,
         n, rx, ry, rz, angles, nangles, best, idx, second);
+}
+
+// d_CTFSimulate at n addresses (k [cycles/pixel], angle) per parameter set;
+// params: 14 doubles per set in CtfFull order.
+void orc_ctf_simulate_f64(const double* params, const double* addresses, const double* gammacorr, double* out,
+                          int n, int ampsq, int ignorefirst, int batch) {
+  for (int b = 0; b < batch; ++b) {
+    CtfFull q;
+    std::memcpy(&q, params + 14 * b, sizeof(q));
+    for (int i = 0; i < n; ++i)
+      out[(long long)b * n + i] = ctf_at(q, addresses[2 * i], addresses[2 * i + 1], gammacorr ? gammacorr[i] : 0.0,
+                                         ampsq != 0, ignorefirst != 0);
+  }
 }
 
 void orc_full_pad_f64(const double* in, double* out, int ox, int oy, int oz, int nx, int ny, int nz) {

@@ -44,6 +44,7 @@ def lib():
         L.orc_local_peaks_f32.argtypes = [_p, _i, _i, _i, _i, _f, _p, _ll]
         L.orc_local_peaks_f32.restype = _ll
         L.orc_diff2_f64.argtypes = [_p, _i, _d, _p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _p]
+        L.orc_ctf_simulate_f64.argtypes = [_p, _p, _p, _p, _i, _i, _i, _i]
         L.orc_full_pad_f64.argtypes = [_p, _p, _i, _i, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
         L.orc_picker_f64.argtypes = [_p, _i, _p, _i, _i, _i, _p, _p, _i, _p, _p, _p]
@@ -220,6 +221,33 @@ def diff2(projft, p, oversample, expft, ctf, n, relevant, angles, with_second=Fa
     lib().orc_diff2_f64(_ptr(P), p, oversample, _ptr(E), _ptr(C), n, rx, ry, rz, _ptr(A), len(A), _ptr(best),
                         _ptr(idx), _ptr(second))
     return (best, idx, second) if with_second else (best, idx)
+
+
+CTF_FIELDS = ("pixelsize", "pixeldelta", "pixelangle", "Cs", "voltage", "defocus", "astigmatismangle",
+              "defocusdelta", "amplitude", "Bfactor", "Bfactordelta", "Bfactorangle", "scale", "phaseshift")
+CTF_DEFAULTS = dict(pixelsize=1e-10, pixeldelta=0.0, pixelangle=0.0, Cs=2e-3, voltage=300e3, defocus=-3e-6,
+                    astigmatismangle=0.0, defocusdelta=0.0, amplitude=0.07, Bfactor=0.0, Bfactordelta=0.0,
+                    Bfactorangle=0.0, scale=1.0, phaseshift=0.0)
+
+
+def ctf_simulate(params, addresses, gammacorr=None, ampsq=False, ignorefirst=False):
+    """d_CTFSimulate: params is a list of dicts (CTFParams fields, SI units), addresses
+    (n, 2) of (k [cycles/pixel], angle). Returns (batch, n) float64."""
+    P = np.array([[dict(CTF_DEFAULTS, **p)[f] for f in CTF_FIELDS] for p in params], np.float64)
+    A = _c(addresses, np.float64).reshape(-1, 2)
+    out = np.empty((len(P), len(A)), np.float64)
+    g = None if gammacorr is None else _c(gammacorr, np.float64)
+    lib().orc_ctf_simulate_f64(_ptr(P), _ptr(A), None if g is None else _ptr(g), _ptr(out), len(A), int(ampsq),
+                               int(ignorefirst), len(P))
+    return out
+
+
+def ctf_addresses_2d(n):
+    """(k, angle) of the (n, n/2+1) half-spectrum bins: k = |(x, y)|/n, angle = atan2(y, x)."""
+    y = np.fft.fftfreq(n, 1 / n)[:, None]
+    x = np.arange(n // 2 + 1)[None, :]
+    yy, xx = np.broadcast_arrays(y, x)
+    return np.stack([np.hypot(xx, yy) / n, np.arctan2(yy, xx)], axis=-1).reshape(-1, 2)
 
 
 def full_pad(vol, newshape):

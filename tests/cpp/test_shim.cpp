@@ -212,6 +212,29 @@ static void mrc_round_trip() {
   std::printf("ok mrc_round_trip\n");
 }
 
+// d_CTFSimulate: CTF(0) equals the amplitude contrast, and along the
+// astigmatism angle an astigmatic CTF equals the symmetric one at
+// defocus + defocusdelta/2 (two parameter sets, one call, shared addresses).
+static void ctf_simulate_known_answers() {
+  CTFParams p[2];
+  p[0].defocusdelta = 0.4e-6f;
+  p[0].astigmatismangle = 0.3f;
+  p[1].defocus = p[0].defocus + 0.2e-6f;
+  const int n = 6;
+  std::vector<float2> addr(n);
+  for (int i = 0; i < n; ++i) addr[i] = make_float2(0.08f * i, 0.3f);
+  float2* d_addr = dev<float2>(n);
+  float* d_out = dev<float>(2 * n);
+  hipMemcpy(d_addr, addr.data(), n * sizeof(float2), hipMemcpyHostToDevice);
+  d_CTFSimulate(p, d_addr, nullptr, d_out, n, false, false, 2);
+  std::vector<float> out(2 * n);
+  hipMemcpy(out.data(), d_out, out.size() * sizeof(float), hipMemcpyDeviceToHost);
+  EXPECT(std::fabs(out[0] - 0.07f) < 1e-6f && std::fabs(out[n] - 0.07f) < 1e-6f, "CTF(0) = %g, %g", out[0], out[n]);
+  for (int i = 0; i < n; ++i) EXPECT(std::fabs(out[i] - out[n + i]) < 1e-4f, "k %d: %g vs %g", i, out[i], out[n + i]);
+  std::printf("ok ctf_simulate_known_answers\n");
+  hipFree(d_addr); hipFree(d_out);
+}
+
 static void errors_throw() {
   bool threw = false;
   try {
@@ -228,6 +251,7 @@ int main() {
   pick_finds_shift();
   picker_finds_copy();
   mrc_round_trip();
+  ctf_simulate_known_answers();
   errors_throw();
   std::printf("all passed\n");
   return 0;

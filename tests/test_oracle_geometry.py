@@ -184,3 +184,36 @@ def test_project2d_identity_is_the_transform_of_the_z_sum(orc):
     inside = kx ** 2 + ky ** 2 <= (n // 2) ** 2
     assert np.allclose(got[inside], ref[inside], atol=1e-9 * np.abs(ref).max())
     assert np.all(got[~inside] == 0)
+
+
+def test_ctf_simulate_symmetric_case_is_the_3d_ctf_plane(orc):
+    """Without astigmatism, on the 2D half-spectrum addresses, d_CTFSimulate equals the
+    z = 0 plane of the radially symmetric 3D CTF (already pinned by its closed form)."""
+    n, px = 32, 2e-10
+    got = orc.ctf_simulate([dict(pixelsize=px, defocus=-2e-6, Cs=2.7e-3)], orc.ctf_addresses_2d(n))[0]
+    ref = orc.ctf3(n, px, 300e3, 2.7e-3, -2e-6, 0.07)[0]
+    assert np.allclose(got.reshape(n, n // 2 + 1), ref, atol=1e-12)
+
+
+def test_ctf_simulate_astigmatism_axes(orc):
+    """Along the astigmatism angle the defocus is defocus + delta/2, at 90 degrees to it
+    defocus - delta/2 (CTFParamsLean halves defocusdelta)."""
+    k = np.linspace(0, 0.5, 50)
+    ang, d, dd = 0.3, -2.5e-6, 0.4e-6
+    astig = dict(pixelsize=1.5e-10, defocus=d, defocusdelta=dd, astigmatismangle=ang)
+    along = orc.ctf_simulate([astig], np.stack([k, np.full_like(k, ang)], 1))[0]
+    across = orc.ctf_simulate([astig], np.stack([k, np.full_like(k, ang + np.pi / 2)], 1))[0]
+    sym_hi = orc.ctf_simulate([dict(pixelsize=1.5e-10, defocus=d + dd / 2)], np.stack([k, np.zeros_like(k)], 1))[0]
+    sym_lo = orc.ctf_simulate([dict(pixelsize=1.5e-10, defocus=d - dd / 2)], np.stack([k, np.zeros_like(k)], 1))[0]
+    assert np.allclose(along, sym_hi, atol=1e-12) and np.allclose(across, sym_lo, atol=1e-12)
+
+
+def test_ctf_simulate_flags_and_bfactor(orc):
+    k = np.linspace(0, 0.5, 64)
+    a = np.stack([k, np.zeros_like(k)], 1)
+    base = orc.ctf_simulate([dict(pixelsize=2e-10)], a)[0]
+    assert np.allclose(orc.ctf_simulate([dict(pixelsize=2e-10)], a, ampsq=True)[0], np.abs(base))
+    b = orc.ctf_simulate([dict(pixelsize=2e-10, Bfactor=-50e-20)], a)[0]
+    assert np.allclose(b, base * np.exp(-50 * 0.25 * (k / 2.0) ** 2))
+    first = orc.ctf_simulate([dict(pixelsize=2e-10)], a, ignorefirst=True)[0]
+    assert first[0] == 1.0 and np.allclose(first[-1], base[-1])
