@@ -96,6 +96,9 @@ _SIGS = {
     "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
     "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
     "gtomx_fft_full_pad": [_P, _P, int3, int3, _U, _P],
+    "gtomx_peak": [_P, _P, _P, int3, _I, _U, _P],
+    "gtomx_peak_one_2d": [_P, _P, _P, int3, int3, _I, _U, _P],
+    "gtomx_subpixel_max": [_P, _P, int3, _I, _P],
     "gtomx_ctf_simulate": [_P, _P, _P, _P, _U, _I, _I, _U, _P],
     "gtomx_ctf_2d": [_P, _U, int3, _P, _P],
     "gtomx_pick_subtomograms_diff2": [_P, _F, int3, _P, _P, int3, _U, int3, _P, _U, _P, _P, _P],
@@ -381,6 +384,53 @@ def d_LocalPeaks(vol, localextent, threshold, stream=None):
             break
         cap = need
     return [peaks[b, :int(counts[b])].astype(np.int64) for b in range(batch)]
+
+
+PEAK_INTEGER, PEAK_SUBCOARSE, PEAK_SUBFINE = 1, 2, 3
+_PEAK_MODES = {"integer": 1, "subcoarse": 2, "subfine": 3}
+
+
+def d_Peak(maps, mode, ndims=3, stream=None):
+    """d_Peak (src/Correlation/Peak.cu:26-216) on a float32 CUDA tensor holding
+    maps of shape (..., z, y, x) (ndims=3) or (..., y, x) (ndims=2). mode is
+    "integer", "subcoarse", "subfine" or 1/2/3. Returns (positions (batch, 3)
+    as (x, y, z), values (batch,)) as float32 CUDA tensors."""
+    import torch
+    shape = maps.shape[-ndims:]
+    nx, ny = shape[-1], (shape[-2] if ndims >= 2 else 1)
+    nz = shape[-3] if ndims == 3 else 1
+    batch = max(1, maps.numel() // (nx * ny * nz))
+    pos = torch.empty((batch, 3), dtype=torch.float32, device=maps.device)
+    val = torch.empty(batch, dtype=torch.float32, device=maps.device)
+    check(lib().gtomx_peak(_ptr(maps), _ptr(pos), _ptr(val), int3(nx, ny, nz), int(_PEAK_MODES.get(mode, mode)),
+                           batch, _stream(stream)), "gtomx_peak")
+    return pos, val
+
+
+def d_PeakOne2D(images, region, subtractcenter, stream=None):
+    """d_PeakOne2D (src/Correlation/Peak.cu:218-225) on a float32 CUDA tensor of
+    shape (..., y, x); region = (rx, ry). Returns (positions (batch, 3), values)."""
+    import torch
+    ny, nx = images.shape[-2:]
+    batch = max(1, images.numel() // (nx * ny))
+    pos = torch.empty((batch, 3), dtype=torch.float32, device=images.device)
+    val = torch.empty(batch, dtype=torch.float32, device=images.device)
+    check(lib().gtomx_peak_one_2d(_ptr(images), _ptr(pos), _ptr(val), int3(nx, ny, 1),
+                                  int3(int(region[0]), int(region[1]), 1), int(bool(subtractcenter)), batch,
+                                  _stream(stream)), "gtomx_peak_one_2d")
+    return pos, val
+
+
+def d_SubpixelMax(vol, out, subpixsteps, stream=None):
+    """d_SubpixelMax (src/Correlation/LocalPeaks.cu:64-91): `out` (float32 CUDA
+    tensor of the shape of `vol`, (z, y, x) or (y, x)) is raised in place to the
+    running maximum over the sub-sample shifts at the voxels the reference
+    covers."""
+    nx, ny = vol.shape[-1], vol.shape[-2]
+    nz = vol.shape[-3] if vol.dim() == 3 else 1
+    check(lib().gtomx_subpixel_max(_ptr(vol), _ptr(out), int3(nx, ny, nz), int(subpixsteps), _stream(stream)),
+          "gtomx_subpixel_max")
+    return out
 
 
 def d_LocalStd(vol, radius, with_mean=False, stream=None):

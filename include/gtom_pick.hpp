@@ -186,6 +186,33 @@ inline void d_LocalPeaks(tfloat* d_input, int3** h_peaks, int* h_peaksnum, int3 
   }
 }
 
+// T_PEAK_MODE, include/Correlation.cuh:44-52.
+enum T_PEAK_MODE { T_PEAK_INTEGER = 1, T_PEAK_SUBCOARSE = 2, T_PEAK_SUBFINE = 3 };
+
+// d_Peak, include/Correlation.cuh:65. The optional cuFFT plans of the
+// reference have no counterpart; positions are tfloat3 (x, y, z) per map.
+inline void d_Peak(tfloat* d_input, tfloat3* d_positions, tfloat* d_values, int3 dims, T_PEAK_MODE mode,
+                   int batch = 1) {
+  detail::check(gtomx_peak(d_input, reinterpret_cast<gtomx_float3*>(d_positions), d_values, detail::i3(dims),
+                           (int)mode, (unsigned)batch, nullptr),
+                "d_Peak");
+}
+
+// d_PeakOne2D, include/Correlation.cuh:67.
+inline void d_PeakOne2D(tfloat* d_input, float3* d_positions, tfloat* d_values, int2 dims, int2 dimsregion,
+                        bool subtractcenter, int batch = 1) {
+  detail::check(gtomx_peak_one_2d(d_input, reinterpret_cast<gtomx_float3*>(d_positions), d_values,
+                                  gtomx_int3{dims.x, dims.y, 1}, gtomx_int3{dimsregion.x, dimsregion.y, 1},
+                                  subtractcenter ? 1 : 0, (unsigned)batch, nullptr),
+                "d_PeakOne2D");
+  detail::sync();
+}
+
+// d_SubpixelMax, include/Correlation.cuh:83: d_output must hold the starting values.
+inline void d_SubpixelMax(tfloat* d_input, tfloat* d_output, int3 dims, int subpixsteps) {
+  detail::check(gtomx_subpixel_max(d_input, d_output, detail::i3(dims), subpixsteps, nullptr), "d_SubpixelMax");
+}
+
 // GetEqualAngularSpacing, src/Helpers/AngularSpacing.cu:46-67.
 inline std::vector<float3> GetEqualAngularSpacing(float2 phirange, float2 thetarange, float2 psirange,
                                                   float increment) {

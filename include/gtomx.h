@@ -305,6 +305,37 @@ int gtomx_local_peaks(const float* d_input, gtomx_int3 dims, int localextent, fl
                       unsigned batch, gtomx_int3* h_peaks, size_t capacity, size_t* h_counts,
                       void* stream);
 
+/* T_PEAK_MODE, include/Correlation.cuh:44-52. */
+enum { GTOMX_PEAK_INTEGER = 1, GTOMX_PEAK_SUBCOARSE = 2, GTOMX_PEAK_SUBFINE = 3 };
+
+/* d_Peak, include/Correlation.cuh:65 / src/Correlation/Peak.cu:26-216: the
+ * maximum of each of `batch` maps of dims (the first in raster order on ties)
+ * as (x, y, z) and value, optionally refined: SUBCOARSE moves each coordinate
+ * to the sinc-interpolated maximum (1/512 steps) on the line through the
+ * maximum and evaluates the sinc interpolation there; SUBFINE up-scales the
+ * 5^3 (3D) or 9^2 (2D) region around it in Fourier space by 63 (3D) or 105
+ * (2D) and moves to that maximum. The cuFFT plan arguments of the reference
+ * have no counterpart. dims with y = 1 and z > 1 are rejected. Synchronous. */
+int gtomx_peak(const float* d_input, gtomx_float3* d_positions, float* d_values, gtomx_int3 dims, int mode,
+               unsigned batch, void* stream);
+
+/* d_PeakOne2D, include/Correlation.cuh:67 / src/Correlation/Peak.cu:218-225
+ * (kernel :380-454): per image (dims.z = 1), the maximum of the centred
+ * dimsregion plus the value-weighted centroid of its 3x3 neighbourhood, minus
+ * (dims.x/2, dims.y/2) when subtractcenter is set; z of each position is 0.
+ * Asynchronous on `stream`. */
+int gtomx_peak_one_2d(const float* d_input, gtomx_float3* d_positions, float* d_values, gtomx_int3 dims,
+                      gtomx_int3 dimsregion, int subtractcenter, unsigned batch, void* stream);
+
+/* d_SubpixelMax, include/Correlation.cuh:83 / src/Correlation/LocalPeaks.cu:64-91
+ * (kernel :145-178): for every shift s = (i + 1/2)/subpixsteps - 1/2 per
+ * dimension, d_output = max(d_output, sum over the 13^ndims window of input *
+ * sinc(s - offset)), at the voxels the reference's launch covers (x from 6 to
+ * min(nx - 9, 128 ceil((nx - 15)/128) + 5), y from 6 to ny - 10, z from 6 to
+ * nz - 10 in 3D); nothing else is written, so d_output must hold the starting
+ * values. 2D and 3D maps, subpixsteps in [1, 64]. Synchronous. */
+int gtomx_subpixel_max(const float* d_input, float* d_output, gtomx_int3 dims, int subpixsteps, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* MRC files (the on-disk format of tomograms, templates and output maps)    */
 /* ------------------------------------------------------------------------ */

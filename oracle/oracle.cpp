@@ -648,6 +648,295 @@ double ctf_at(const CtfFull& q, double k, double angle, double gammacorr, bool a
   return v * q.scale;
 }
 
+// ---------------------------------------------------------------------------
+// Sub-voxel peak location: d_Peak (src/Correlation/Peak.cu:26-216), d_PeakOne2D
+// (:218-225, PeakOne2DKernel :380-454) and d_SubpixelMax
+// (src/Correlation/LocalPeaks.cu:64-91, SubpixelMaxKernel :145-178), per map,
+// in double from the float map.
+// ---------------------------------------------------------------------------
+// DimensionCount (include/Prerequisites.cuh): 3 if nz > 1, 2 if ny > 1, else 1.
+inline int dimension_count(int ny, int nz) { return nz > 1 ? 3 : (ny > 1 ? 2 : 1); }
+inline long long wrap(long long i, long long n) { const long long r = i % n; return r < 0 ? r + n : r; }
+// sinc of include/DeviceFunctions.cuh:11-25 with the given zero threshold
+// (1e-4 for the float overload, 1e-8 for the double one), evaluated in double.
+inline double sinc_t(double x, double zero) {
+  const double pi = 3.14159265358979323846;
+  return std::fabs(x) <= zero ? 1.0 : std::sin(x * pi) / (x * pi);
+}
+
+struct PeakResult { double pos[3]; double value; };
+
+// T_PEAK_INTEGER (:28-53): the maximum and its voxel. d_Max breaks ties in the
+// order of its block reduction (src/Generics/MinMax.cu:382-425); the first
+// maximum in raster order is this restatement's rule.
+PeakResult peak_integer(const float* in, int nx, int ny, int nz) {
+  const long long V = (long long)nx * ny * nz;
+  long long arg = 0;
+  for (long long i = 1; i < V; ++i)
+    if (in[i] > in[arg]) arg = i;
+  PeakResult r;
+  r.pos[0] = (double)(arg % nx);
+  r.pos[1] = (double)((arg / nx) % ny);
+  r.pos[2] = (double)(arg / ((long long)nx * ny));
+  r.value = in[arg];
+  return r;
+}
+
+// SincMax1DKernel<n> (:230-320): y(x) = sum over |k| <= n/2 - 1 of
+// sinc(k - x) s[n/2 + k] at x = id/512 - 1, id = 0..1023, terms added from the
+// outermost pair inwards (-k before +k), the centre last, in double. 256
+// threads: thread t visits id = t + 256 j in order and keeps a float pair
+// (x, y), replaced when (double)best_y < y; the tree then keeps the lower slot
+// unless the upper one's float y is strictly larger, so among equal float
+// maxima the thread with the smallest bit-reversed index wins. Returns x.
+double sinc_max_1d(const double* s, int n) {
+  float bx[256], by[256];
+  for (int t = 0; t < 256; ++t) {
+    bx[t] = 0.f;
+    by[t] = -1e30f;
+    for (int id = t; id < 1024; id += 256) {
+      const double x = (double)id / 512.0 - 1.0;
+      double y = 0;
+      for (int k = 9; k >= 1; --k) {
+        if (!(n > 2 * k)) continue;
+        y += sinc_t(-(double)k - x, 1e-8) * s[n / 2 - k];
+        y += sinc_t((double)k - x, 1e-8) * s[n / 2 + k];
+      }
+      y += sinc_t(x, 1e-8) * s[n / 2];
+      if ((double)by[t] < y) {
+        bx[t] = (float)x;
+        by[t] = (float)y;
+      }
+    }
+  }
+  for (int st = 128; st >= 1; st /= 2)
+    for (int t = 0; t < st; ++t)
+      if (by[t + st] > by[t]) {
+        by[t] = by[t + st];
+        bx[t] = bx[t + st];
+      }
+  return bx[0];
+}
+
+// T_PEAK_SUBCOARSE (:54-158): n = min(3D ? 10 : 20, nx rounded up to even)
+// samples on the line through the integer peak along each dimension (cyclic,
+// ExtractKernel src/Generics/Extraction.cu:139-153), starting at peak - n/2;
+// each line gives the shift in [-1, 1) of its sinc maximum; the position is
+// peak + shift. The value is the sinc interpolation at that position over the
+// n^ndims region starting at peak - n/2 (SincEvalMaxKernel :322-378, float
+// sinc of the sample-to-position distance per dimension).
+PeakResult peak_subcoarse(const float* in, int nx, int ny, int nz) {
+  PeakResult r = peak_integer(in, nx, ny, nz);
+  const int nd = dimension_count(ny, nz);
+  const int n = std::min(nd == 3 ? 10 : 20, (nx + 1) / 2 * 2);
+  const long long dims[3] = {nx, ny, nz};
+  const long long ip[3] = {(long long)r.pos[0], (long long)r.pos[1], (long long)r.pos[2]};
+  auto at = [&](long long x, long long y, long long z) {
+    return (double)in[(wrap(z, dims[2]) * dims[1] + wrap(y, dims[1])) * dims[0] + wrap(x, dims[0])];
+  };
+  double shift[3] = {0, 0, 0};
+  std::vector<double> line(n);
+  for (int d = 0; d < nd; ++d) {
+    for (int i = 0; i < n; ++i) {
+      long long q[3] = {ip[0], ip[1], ip[2]};
+      q[d] += i - n / 2;
+      line[i] = at(q[0], q[1], q[2]);
+    }
+    shift[d] = sinc_max_1d(line.data(), n);
+  }
+  double e[3] = {0, 0, 0};
+  for (int d = 0; d < nd; ++d) {
+    r.pos[d] = (double)ip[d] + shift[d];
+    e[d] = (double)(n / 2) + shift[d];
+  }
+  const int ry = nd > 1 ? n : 1, rz = nd > 2 ? n : 1;
+  double sum = 0;
+  for (int z = 0; z < rz; ++z)
+    for (int y = 0; y < ry; ++y)
+      for (int x = 0; x < n; ++x) {
+        double w = sinc_t(x - e[0], 1e-4);
+        if (nd > 1) w *= sinc_t(y - e[1], 1e-4);
+        if (nd > 2) w *= sinc_t(z - e[2], 1e-4);
+        sum += at(ip[0] - n / 2 + x, ip[1] - ry / 2 + y, ip[2] - rz / 2 + z) * w;
+      }
+  r.value = sum;
+  return r;
+}
+
+// The (new index, old index) pairs of one axis that FFTPadEvenKernel
+// (src/FFT/FFTResize.cu:162-184) copies: new index i has centred coordinate
+// nr = (i + nn/2) % nn, old centred coordinate or = nr + (no - nn)/2, kept
+// when 0 <= or < no and read at (or + (no+1)/2) % no.
+std::vector<std::pair<int, int>> pad_pairs(int no, int nn) {
+  std::vector<std::pair<int, int>> v;
+  for (int i = 0; i < nn; ++i) {
+    const int nr = (i + nn / 2) % nn, orr = nr + (no - nn) / 2;
+    if (orr >= 0 && orr < no) v.push_back({i, (orr + (no + 1) / 2) % no});
+  }
+  return v;
+}
+
+// T_PEAK_SUBFINE (:159-210): samples = 3D ? 5 : 9 (at most each dimension),
+// subdivisions = 3D ? 63 : 105; the samples^ndims region centred on the integer
+// peak (d_Extract src/Generics/Extraction.cu:25-35, cyclic) is scaled by
+// d_Scale T_INTERP_FOURIER (src/Transformation/Scale.cu:109-139): R2C, zero
+// pad of the half spectrum (FFTPadEvenKernel), C2R_u, times 1/N_old. The
+// maximum of the scaled region (first in raster order on ties) moves the peak
+// by (index - samples/2 * subdivisions) / subdivisions per dimension; the value
+// is the larger of the integer maximum and the scaled one. The C2R is
+// evaluated directly at every output point from the few non-zero bins.
+PeakResult peak_subfine(const float* in, int nx, int ny, int nz) {
+  PeakResult r = peak_integer(in, nx, ny, nz);
+  const int nd = dimension_count(ny, nz);
+  const long long dims[3] = {nx, ny, nz};
+  int S = nd < 3 ? 9 : 5;
+  for (int d = 0; d < nd; ++d) S = std::min<long long>(S, dims[d]);
+  const int sub = nd < 3 ? 105 : 63, centre = S / 2 * sub;
+  const int ro[3] = {S, (int)std::min<long long>(ny, S), (int)std::min<long long>(nz, S)};
+  const int rn[3] = {S * sub, ny == 1 ? 1 : S * sub, nz == 1 ? 1 : S * sub};
+  std::vector<double> reg((size_t)ro[0] * ro[1] * ro[2]);
+  long long org[3];
+  for (int d = 0; d < 3; ++d) org[d] = wrap((long long)r.pos[d] - ro[d] / 2, dims[d]);
+  for (int z = 0; z < ro[2]; ++z)
+    for (int y = 0; y < ro[1]; ++y)
+      for (int x = 0; x < ro[0]; ++x)
+        reg[((size_t)z * ro[1] + y) * ro[0] + x] =
+            in[(wrap(org[2] + z, nz) * ny + wrap(org[1] + y, ny)) * nx + wrap(org[0] + x, nx)];
+  const int hx = ro[0] / 2 + 1;
+  std::vector<cpx<double>> spec((size_t)hx * ro[1] * ro[2]);
+  r2c3<double>(reg.data(), spec.data(), ro[0], ro[1], ro[2]);
+  const int kxn = (ro[0] + 1) / 2;  // x bins the pad keeps
+  const auto py = pad_pairs(ro[1], rn[1]), pz = pad_pairs(ro[2], rn[2]);
+  const double tau = 6.283185307179586476925286766559;
+  auto phase = [&](long long k, long long j, long long n) { return std::polar(1.0, tau * (double)((k * j) % n) / n); };
+  // G(kx, y', z') = sum over the kept (y, z) bins of S e^{2 pi i (ky y'/Ny' + kz z'/Nz')}
+  std::vector<cpx<double>> ey(py.size() * rn[1]), ez(pz.size() * rn[2]);
+  for (size_t b = 0; b < py.size(); ++b)
+    for (int yy = 0; yy < rn[1]; ++yy) ey[b * rn[1] + yy] = phase(py[b].first, yy, rn[1]);
+  for (size_t a = 0; a < pz.size(); ++a)
+    for (int zz = 0; zz < rn[2]; ++zz) ez[a * rn[2] + zz] = phase(pz[a].first, zz, rn[2]);
+  std::vector<cpx<double>> G((size_t)kxn * rn[1] * rn[2]);
+  for (int zz = 0; zz < rn[2]; ++zz)
+    for (int yy = 0; yy < rn[1]; ++yy)
+      for (int kx = 0; kx < kxn; ++kx) {
+        cpx<double> acc = 0;
+        for (size_t a = 0; a < pz.size(); ++a)
+          for (size_t b = 0; b < py.size(); ++b)
+            acc += spec[((size_t)pz[a].second * ro[1] + py[b].second) * hx + kx] * ez[a * rn[2] + zz] *
+                   ey[b * rn[1] + yy];
+        G[((size_t)zz * rn[1] + yy) * kxn + kx] = acc;
+      }
+  std::vector<cpx<double>> tw((size_t)kxn * rn[0]);
+  for (int kx = 0; kx < kxn; ++kx)
+    for (int xx = 0; xx < rn[0]; ++xx) tw[(size_t)kx * rn[0] + xx] = (kx == 0 ? 1.0 : 2.0) * phase(kx, xx, rn[0]);
+  const double norm = 1.0 / ((double)ro[0] * ro[1] * ro[2]);
+  double best = -HUGE_VAL;
+  long long arg = 0;
+  for (long long row = 0; row < (long long)rn[1] * rn[2]; ++row)
+    for (int xx = 0; xx < rn[0]; ++xx) {
+      double v = 0;
+      for (int kx = 0; kx < kxn; ++kx) v += (G[(size_t)row * kxn + kx] * tw[(size_t)kx * rn[0] + xx]).real();
+      v *= norm;
+      if (v > best) {
+        best = v;
+        arg = row * rn[0] + xx;
+      }
+    }
+  const long long plane = (long long)S * S * sub * sub, rowlen = (long long)S * sub;
+  const long long iz = arg / plane, iy = (arg - iz * plane) / rowlen, ix = arg - iz * plane - iy * rowlen;
+  r.value = std::max(r.value, best);
+  r.pos[0] += (double)(ix - centre) / sub;
+  if (ny > 1) r.pos[1] += (double)(iy - centre) / sub;
+  if (nz > 1) r.pos[2] += (double)(iz - centre) / sub;
+  return r;
+}
+
+// PeakOne2DKernel (:380-454): the maximum of the centred region (rx, ry) of an
+// nx x ny image. min(128, rx*ry) threads: thread t visits id = t + T j with
+// strict '>' from (-1e20, (0, 0)); thread 0 then takes the others in order with
+// strict '>'. The position moves by the value-weighted centroid of the 3x3
+// neighbourhood (clipped at the borders) and loses (nx/2, ny/2) on request.
+void peak_one_2d(const float* in, int nx, int ny, int rx, int ry, bool subtractcenter, double* pos, double* value) {
+  const int T = std::min(128, rx * ry), ox = (nx - rx) / 2, oy = (ny - ry) / 2;
+  float mv = -1e20f;
+  int mx = 0, my = 0;
+  for (int t = 0; t < T; ++t) {
+    float tv = -1e20f;
+    int tx = 0, ty = 0;
+    for (int id = t; id < rx * ry; id += T) {
+      const int y = id / rx + oy, x = id % rx + ox;
+      const float v = in[(size_t)y * nx + x];
+      if (v > tv) { tv = v; tx = x; ty = y; }
+    }
+    if (t == 0 || tv > mv) { mv = tv; mx = tx; my = ty; }
+  }
+  double cx = 0, cy = 0, sw = 0;
+  for (int y = -1; y <= 1; ++y) {
+    if (my + y < 0 || my + y >= ny) continue;
+    for (int x = -1; x <= 1; ++x) {
+      if (mx + x < 0 || mx + x >= nx) continue;
+      const double v = in[(size_t)(my + y) * nx + mx + x];
+      cx += v * x;
+      cy += v * y;
+      sw += v;
+    }
+  }
+  if (sw != 0) { cx /= sw; cy /= sw; }
+  pos[0] = mx + cx - (subtractcenter ? nx / 2 : 0);
+  pos[1] = my + cy - (subtractcenter ? ny / 2 : 0);
+  pos[2] = 0;
+  *value = mv;
+}
+
+// d_SubpixelMax: for every sub-sample shift s = (i + 1/2)/steps - 1/2 per
+// dimension (float arithmetic as at :70-74; s.z = 0 in 2D), out = max(out,
+// sum over the 13^ndims window of in * sinc(s - offset) per dimension) at the
+// voxels the launch covers: x in [6, min(nx - 9, 128 ceil((nx - 15)/128) + 5)],
+// y in [6, ny - 10], z in [6, nz - 10] (3D; z = 0 in 2D). Nothing is written when
+// a grid dimension (nx - 15, ny - 15, nz - 15) is not positive. 1D is rejected by
+// the reference (throw) and here returns without writing.
+void subpixel_max(const float* in, double* out, int nx, int ny, int nz, int steps) {
+  const int nd = dimension_count(ny, nz), rad = 6;
+  if (nd < 2 || steps < 1) return;
+  const int gx = (nx - 15 + 127) / 128, gy = ny - 15, gz = nd > 2 ? nz - 15 : 1;
+  if (gx <= 0 || gy <= 0 || gz <= 0) return;
+  const int xhi = std::min(nx - rad - 3, gx * 128 - 1 + rad), yhi = std::min(ny - rad - 3, gy - 1 + rad);
+  const int zlo = nd > 2 ? rad : 0, zhi = nd > 2 ? std::min(nz - rad - 3, gz - 1 + rad) : 0;
+  volatile float step = 1.0f / steps;
+  std::vector<double> wx(13), wy(13), wz(13);
+  auto weights = [&](int i, std::vector<double>& w) {
+    volatile float a = (float)i * step;
+    volatile float b = a - 0.5f;
+    volatile float c = b + step / 2;
+    for (int k = -rad; k <= rad; ++k) {
+      volatile float d = c - (float)k;
+      w[k + rad] = sinc_t(d, 1e-4);
+    }
+  };
+  for (int sz = 0; sz < (nd == 3 ? steps : 1); ++sz) {
+    if (nd == 3) weights(sz, wz);
+    for (int sy = 0; sy < steps; ++sy) {
+      weights(sy, wy);
+      for (int sx = 0; sx < steps; ++sx) {
+        weights(sx, wx);
+        for (int cz = zlo; cz <= zhi; ++cz)
+          for (int cy = rad; cy <= yhi; ++cy)
+            for (int cx = rad; cx <= xhi; ++cx) {
+              double sum = 0;
+              for (int z = (nd == 3 ? -rad : 0); z <= (nd == 3 ? rad : 0); ++z) {
+                const double fz = nd == 3 ? wz[z + rad] : 1.0;
+                for (int y = -rad; y <= rad; ++y)
+                  for (int x = -rad; x <= rad; ++x)
+                    sum += in[((size_t)(cz + z) * ny + cy + y) * nx + cx + x] * wx[x + rad] * wy[y + rad] * fz;
+              }
+              double& o = out[((size_t)cz * ny + cy) * nx + cx];
+              o = std::max(o, sum);
+            }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -871,6 +1160,30 @@ double orc_pick_f32(const float* projft, int p, double oversample, const float* 
                     int32_t* idx_out, float* top1_out, float* top2_out, float* sigma_out) {
   return pick<float>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+// d_Peak on `batch` float maps of (nx, ny, nz): mode 1 integer, 2 sub-coarse,
+// 3 sub-fine. pos receives (x, y, z) per map, value one double per map.
+void orc_peak_f64(const float* in, int nx, int ny, int nz, int mode, int batch, double* pos, double* value) {
+  const long long V = (long long)nx * ny * nz;
+  for (int b = 0; b < batch; ++b) {
+    const float* m = in + (size_t)b * V;
+    PeakResult r = mode == 1 ? peak_integer(m, nx, ny, nz)
+                 : mode == 2 ? peak_subcoarse(m, nx, ny, nz)
+                             : peak_subfine(m, nx, ny, nz);
+    for (int d = 0; d < 3; ++d) pos[3 * b + d] = r.pos[d];
+    value[b] = r.value;
+  }
+}
+
+void orc_peak_one_2d_f64(const float* in, int nx, int ny, int rx, int ry, int subtractcenter, int batch,
+                         double* pos, double* value) {
+  for (int b = 0; b < batch; ++b)
+    peak_one_2d(in + (size_t)b * nx * ny, nx, ny, rx, ry, subtractcenter != 0, pos + 3 * b, value + b);
+}
+
+void orc_subpixel_max_f64(const float* in, double* out, int nx, int ny, int nz, int steps) {
+  subpixel_max(in, out, nx, ny, nz, steps);
 }
 
 }  // extern "C"

@@ -46,6 +46,9 @@ def lib():
         L.orc_diff2_f64.argtypes = [_p, _i, _d, _p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _p]
         L.orc_ctf_simulate_f64.argtypes = [_p, _p, _p, _p, _i, _i, _i, _i]
         L.orc_full_pad_f64.argtypes = [_p, _p, _i, _i, _i, _i, _i, _i]
+        L.orc_peak_f64.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
+        L.orc_peak_one_2d_f64.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p, _p]
+        L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
         L.orc_picker_f64.argtypes = [_p, _i, _p, _i, _i, _i, _p, _p, _i, _p, _p, _p]
         pick_args = [_p, _i, _d, _p, _p, _i, _i, _i, _p, _i, _d, _p, _p, _p, _p, _p]
@@ -248,6 +251,51 @@ def ctf_addresses_2d(n):
     x = np.arange(n // 2 + 1)[None, :]
     yy, xx = np.broadcast_arrays(y, x)
     return np.stack([np.hypot(xx, yy) / n, np.arctan2(yy, xx)], axis=-1).reshape(-1, 2)
+
+
+PEAK_MODES = {"integer": 1, "subcoarse": 2, "subfine": 3}
+
+
+def peak(maps, mode, dims=None):
+    """d_Peak on float32 maps: maps is (z, y, x) or (batch, z, y, x) (or (y, x) /
+    (batch, y, x) with dims=2). Returns (positions (batch, 3) as (x, y, z), values (batch,))."""
+    m = _c(maps, np.float32)
+    nd = dims or (m.ndim if m.ndim <= 3 else 3)
+    if m.ndim == nd:
+        m = m[None]
+    shape = m.shape[1:]
+    nx = shape[-1]
+    ny = shape[-2] if nd >= 2 else 1
+    nz = shape[-3] if nd >= 3 else 1
+    b = m.shape[0]
+    pos = np.empty((b, 3), np.float64)
+    val = np.empty(b, np.float64)
+    lib().orc_peak_f64(_ptr(m), nx, ny, nz, PEAK_MODES.get(mode, mode), b, _ptr(pos), _ptr(val))
+    return pos, val
+
+
+def peak_one_2d(images, region, subtractcenter):
+    """d_PeakOne2D on (batch, ny, nx) float32 images, region (rx, ry)."""
+    m = _c(images, np.float32)
+    if m.ndim == 2:
+        m = m[None]
+    b, ny, nx = m.shape
+    pos = np.empty((b, 3), np.float64)
+    val = np.empty(b, np.float64)
+    lib().orc_peak_one_2d_f64(_ptr(m), nx, ny, int(region[0]), int(region[1]), int(subtractcenter), b, _ptr(pos),
+                              _ptr(val))
+    return pos, val
+
+
+def subpixel_max(vol, out, steps):
+    """d_SubpixelMax: vol (z, y, x) or (y, x) float32; out (same shape, float64) is updated in place."""
+    v = _c(vol, np.float32)
+    assert out.dtype == np.float64 and out.flags.c_contiguous and out.shape == v.shape
+    nx = v.shape[-1]
+    ny = v.shape[-2]
+    nz = v.shape[-3] if v.ndim == 3 else 1
+    lib().orc_subpixel_max_f64(_ptr(v), _ptr(out), nx, ny, nz, int(steps))
+    return out
 
 
 def full_pad(vol, newshape):

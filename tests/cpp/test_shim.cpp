@@ -123,6 +123,23 @@ static void pick_finds_shift() {
     found |= peaks[0][k].x == ax && peaks[0][k].y == ay && peaks[0][k].z == az;
   EXPECT(npeaks[0] >= 1 && found, "%d peaks, planted position not among them", npeaks[0]);
   std::printf("ok local_peaks (%d peaks above half the maximum)\n", npeaks[0]);
+
+  // the same maximum through d_Peak: integer and sub-voxel positions agree to a voxel
+  tfloat3* d_pos = dev<tfloat3>(1);
+  float* d_val = dev<float>(1);
+  for (T_PEAK_MODE mode : {T_PEAK_INTEGER, T_PEAK_SUBCOARSE, T_PEAK_SUBFINE}) {
+    d_Peak(d_best, d_pos, d_val, make_int3(n, n, n), mode);
+    tfloat3 hp;
+    float hv;
+    hipMemcpy(&hp, d_pos, sizeof(hp), hipMemcpyDeviceToHost);
+    hipMemcpy(&hv, d_val, sizeof(hv), hipMemcpyDeviceToHost);
+    EXPECT(std::fabs(hp.x - ax) <= 1.f && std::fabs(hp.y - ay) <= 1.f && std::fabs(hp.z - az) <= 1.f,
+           "d_Peak mode %d at (%g,%g,%g)", (int)mode, hp.x, hp.y, hp.z);
+    EXPECT(mode != T_PEAK_INTEGER || (hp.x == ax && hp.y == ay && hp.z == az && hv == best[arg]),
+           "integer peak (%g,%g,%g) %g", hp.x, hp.y, hp.z, hv);
+  }
+  std::printf("ok d_Peak (integer, sub-coarse, sub-fine)\n");
+  hipFree(d_pos); hipFree(d_val);
   std::free(peaks[0]);
   hipFree(d_t); hipFree(d_pad); hipFree(d_shift); hipFree(d_proj); hipFree(d_tomo); hipFree(d_E);
   hipFree(d_ctf); hipFree(d_best); hipFree(d_angle);
