@@ -101,6 +101,7 @@ _SIGS = {
     "gtomx_subpixel_max": [_P, _P, int3, _I, _P],
     "gtomx_ctf_simulate": [_P, _P, _P, _P, _U, _I, _I, _U, _P],
     "gtomx_ctf_2d": [_P, _U, int3, _P, _P],
+    "gtomx_ctf_tilt_params_grid": [_P, int3, int3, _P, _U, _P],
     "gtomx_pick_subtomograms_diff2": [_P, _F, int3, _P, _P, int3, _U, int3, _P, _U, _P, _P, _P],
     "gtomx_mrc_header_default": [_P],
     "gtomx_mrc_read_header": [ctypes.c_char_p, _P],
@@ -330,6 +331,28 @@ class CtfParamsFull(ctypes.Structure):
                  Bfactorangle=0.0, scale=1.0, phaseshift=0.0)
         d.update(kw)
         return cls(**d)
+
+
+class CtfTiltParams(ctypes.Structure):
+    """gtomx_ctf_tilt_params: CTFTiltParams (include/CTF.cuh:136-158), radians."""
+    _fields_ = [("imageangle", ctypes.c_float), ("stageangle", ctypes.c_float * 2),
+                ("specimenangle", ctypes.c_float * 2), ("centerparams", CtfParamsFull)]
+
+    @classmethod
+    def make(cls, centerparams, imageangle=0.0, stageangle=(0.0, 0.0), specimenangle=(0.0, 0.0)):
+        return cls(float(imageangle), (ctypes.c_float * 2)(*stageangle), (ctypes.c_float * 2)(*specimenangle),
+                   centerparams)
+
+
+def ctf_tilt_params_grid(tilt, dimsimage, dimsregion, positions):
+    """CTFTiltParams::GetParamsGrid2D: a CtfParamsFull per region origin (x, y)."""
+    n = len(positions)
+    pos = (int3 * max(1, n))(*[int3(int(p[0]), int(p[1]), 0) for p in positions])
+    out = (CtfParamsFull * max(1, n))()
+    check(lib().gtomx_ctf_tilt_params_grid(ctypes.byref(tilt), int3(int(dimsimage[0]), int(dimsimage[1]), 1),
+                                           int3(int(dimsregion[0]), int(dimsregion[1]), 1), pos, n, out),
+          "gtomx_ctf_tilt_params_grid")
+    return [out[i] for i in range(n)]
 
 
 def d_CTFSimulate(params, addresses, gammacorrection=None, amplitudesquared=False, ignorefirstpeak=False,

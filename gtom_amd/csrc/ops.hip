@@ -527,6 +527,40 @@ int gtomx_ctf_simulate(const gtomx_ctf_params_full* h_params, const float* d_add
   return GTOMX_OK;
 }
 
+int gtomx_ctf_tilt_params_grid(const gtomx_ctf_tilt_params* tilt, gtomx_int3 dimsimage, gtomx_int3 dimsregion,
+                               const gtomx_int3* h_positions, unsigned npoints, gtomx_ctf_params_full* h_params) {
+  if (!tilt || (npoints > 0 && (!h_positions || !h_params))) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
+  // Rotation about the in-plane axis (cos a, sin a, 0) by r, the reference's
+  // Matrix3RotationCustom as written (glm column-major, m[col][row]).
+  auto rotation = [](double a, double r, double m[3][3]) {
+    const double ax = std::cos(a), ay = std::sin(a), c = std::cos(r), c1 = 1.0 - c, s = std::sin(r);
+    m[0][0] = c + ax + ax * c1;
+    m[0][1] = ay * ax * c1;
+    m[0][2] = -ay * s;
+    m[1][0] = ax * ay * c1;
+    m[1][1] = c + ay * ay * c1;
+    m[1][2] = ax * s;
+    m[2][0] = ay * s;
+    m[2][1] = -ax * s;
+    m[2][2] = c;
+  };
+  double st[3][3], sp[3][3], normal[3];
+  rotation(tilt->stageangle[0], tilt->stageangle[1], st);
+  rotation(tilt->specimenangle[0], tilt->specimenangle[1], sp);
+  for (int r = 0; r < 3; ++r) normal[r] = st[0][r] * sp[2][0] + st[1][r] * sp[2][1] + st[2][r] * sp[2][2];
+  if (normal[2] == 0.0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "specimen plane is vertical");
+  const double ci = std::cos(tilt->imageangle), si = std::sin(tilt->imageangle);
+  const double px = tilt->centerparams.pixelsize;
+  for (unsigned i = 0; i < npoints; ++i) {
+    const double fx = (h_positions[i].x + dimsregion.x / 2.0 - dimsimage.x / 2.0) * px;
+    const double fy = (h_positions[i].y + dimsregion.y / 2.0 - dimsimage.y / 2.0) * px;
+    const double gx = ci * fx - si * fy, gy = si * fx + ci * fy;
+    h_params[i] = tilt->centerparams;
+    h_params[i].defocus = (float)(tilt->centerparams.defocus + (-gx * normal[0] - gy * normal[1]) / normal[2]);
+  }
+  return ok();
+}
+
 int gtomx_ctf_2d(const gtomx_ctf_params_full* h_params, unsigned batch, gtomx_int3 dims, float* d_ctf, void* stream) {
   if (!h_params || !d_ctf || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer or zero batch");
   if (dims.x != dims.y || dims.z != 1 || dims.x < 2 || dims.x % 2)

@@ -18,6 +18,7 @@
 
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -349,6 +350,45 @@ inline void d_CTFSimulate(CTFParams* h_params, float2* d_addresses, float* d_gam
                                    amplitudesquared ? 1 : 0, ignorefirstpeak ? 1 : 0, (unsigned)batch, nullptr),
                 "d_CTFSimulate");
 }
+
+// CTFTiltParams, include/CTF.cuh:136-194: the plane geometry runs in the
+// library (double precision, the reference's rotation matrix as written).
+class CTFTiltParams {
+ public:
+  tfloat imageangle;
+  float2 stageangle;
+  float2 specimenangle;
+  CTFParams centerparams;
+
+  CTFTiltParams(tfloat3 _angles, CTFParams _centerparams)
+      : imageangle(_angles.z), stageangle(make_float2(_angles.x, _angles.y)), specimenangle(make_float2(0, 0)),
+        centerparams(_centerparams) {}
+  CTFTiltParams(tfloat _imageangle, float2 _stageangle, float2 _specimenangle, CTFParams _centerparams)
+      : imageangle(_imageangle), stageangle(_stageangle), specimenangle(_specimenangle),
+        centerparams(_centerparams) {}
+
+  void GetParamsGrid2D(int2 dimsimage, int2 dimsregion, int3* h_positions, uint npoints, CTFParams* h_params) {
+    gtomx_ctf_tilt_params t;
+    t.imageangle = imageangle;
+    t.stageangle[0] = stageangle.x;
+    t.stageangle[1] = stageangle.y;
+    t.specimenangle[0] = specimenangle.x;
+    t.specimenangle[1] = specimenangle.y;
+    static_assert(sizeof(CTFParams) == sizeof(gtomx_ctf_params_full), "CTFParams layout");
+    std::memcpy(&t.centerparams, &centerparams, sizeof(CTFParams));
+    detail::check(gtomx_ctf_tilt_params_grid(&t, gtomx_int3{dimsimage.x, dimsimage.y, 1},
+                                             gtomx_int3{dimsregion.x, dimsregion.y, 1},
+                                             reinterpret_cast<const gtomx_int3*>(h_positions), npoints,
+                                             reinterpret_cast<gtomx_ctf_params_full*>(h_params)),
+                  "CTFTiltParams::GetParamsGrid2D");
+  }
+
+  void GetZGrid2D(int2 dimsimage, int2 dimsregion, int3* h_positions, uint npoints, float* h_zvalues) {
+    std::vector<CTFParams> p(npoints);
+    GetParamsGrid2D(dimsimage, dimsregion, h_positions, npoints, p.data());
+    for (uint n = 0; n < npoints; ++n) h_zvalues[n] = p[n].defocus;
+  }
+};
 
 // class Picker, include/Correlation.cuh:136-164. The reference's members
 // (cuFFT plans, textures, scratch) live inside the library object; the public

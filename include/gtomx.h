@@ -169,6 +169,27 @@ int gtomx_ctf_simulate(const gtomx_ctf_params_full* h_params, const float* d_add
                        const float* d_gammacorrection, float* d_output, unsigned n, int amplitudesquared,
                        int ignorefirstpeak, unsigned batch, void* stream);
 
+/* CTFTiltParams, include/CTF.cuh:136-158: the specimen plane tilted by the
+ * stage and the specimen, each an (in-plane axis angle, rotation about it)
+ * pair; imageangle rotates image coordinates. Radians. */
+typedef struct {
+  float imageangle;
+  float stageangle[2];
+  float specimenangle[2];
+  gtomx_ctf_params_full centerparams;
+} gtomx_ctf_tilt_params;
+
+/* CTFTiltParams::GetParamsGrid2D, include/CTF.cuh:182-194 (GetZGrid2D
+ * :160-180): for npoints regions of dimsregion at origins h_positions (x, y) of
+ * an image of dimsimage, centerparams with the defocus of the tilted plane
+ * under the region centre. The plane normal is built from the reference's
+ * rotation matrix as written (its element [0][0] is c + a.x + a.x(1 - c),
+ * src/Helpers/Euler.cu:319; 1 + a.x for an untilted stage), which changes the
+ * plane whenever the specimen tilt gives the normal an x component.
+ * Host computation in double; no device work. */
+int gtomx_ctf_tilt_params_grid(const gtomx_ctf_tilt_params* tilt, gtomx_int3 dimsimage, gtomx_int3 dimsregion,
+                               const gtomx_int3* h_positions, unsigned npoints, gtomx_ctf_params_full* h_params);
+
 /* The same on the (n/2+1) x n half spectrum of n x n images (dims.z = 1):
  * bin (x, y) at k = |(x, y)|/n, angle = atan2(y, x) (y in [-n/2, n/2)), one array per parameter
  * set: per-micrograph astigmatic CTFs for the 2D branch (no reference

@@ -648,6 +648,43 @@ double ctf_at(const CtfFull& q, double k, double angle, double gammacorr, bool a
   return v * q.scale;
 }
 
+// Tilted specimen plane: CTFTiltParams::GetZGrid2D (include/CTF.cuh:160-180).
+// Matrix3RotationCustom (src/Helpers/Euler.cu:312-331) as written there, glm
+// column-major (m[col][row]); its element [0][0] is c + a.x + a.x (1 - c), not
+// the Rodrigues c + a.x^2 (1 - c), and is kept. Matrix3RotationInPlaneAxis
+// (:333-336) uses the axis (cos a, sin a, 0); Matrix2Rotation (:343-349) the
+// columns (c, s), (-s, c).
+void rot_custom(double ax, double ay, double az, double angle, double m[3][3]) {
+  const double c = std::cos(angle), c1 = 1.0 - c, s = std::sin(angle);
+  m[0][0] = c + ax + ax * c1;
+  m[0][1] = ay * ax * c1 + az * s;
+  m[0][2] = az * ax * c1 - ay * s;
+  m[1][0] = ax * ay * c1 - az * s;
+  m[1][1] = c + ay * ay * c1;
+  m[1][2] = az * ay * c1 + ax * s;
+  m[2][0] = ax * az * c1 + ay * s;
+  m[2][1] = ay * az * c1 - ax * s;
+  m[2][2] = c + az * az * c1;
+}
+
+// angles = (imageangle, stage axis, stage rotation, specimen axis, specimen
+// rotation); z[n] = defocus + the plane's height under the region centre.
+void tilt_zgrid(const double* angles, double defocus, double pixelsize, int imx, int imy, int rx, int ry,
+                const int* pos, int n, double* z) {
+  double A[3][3], B[3][3];
+  rot_custom(std::cos(angles[1]), std::sin(angles[1]), 0.0, angles[2], A);
+  rot_custom(std::cos(angles[3]), std::sin(angles[3]), 0.0, angles[4], B);
+  double nrm[3];
+  for (int r = 0; r < 3; ++r) nrm[r] = A[0][r] * B[2][0] + A[1][r] * B[2][1] + A[2][r] * B[2][2];
+  const double c = std::cos(angles[0]), s = std::sin(angles[0]);
+  for (int i = 0; i < n; ++i) {
+    const double fx = (pos[3 * i] + rx / 2.0 - imx / 2.0) * pixelsize;
+    const double fy = (pos[3 * i + 1] + ry / 2.0 - imy / 2.0) * pixelsize;
+    const double gx = c * fx - s * fy, gy = s * fx + c * fy;
+    z[i] = defocus + (-gx * nrm[0] - gy * nrm[1]) / nrm[2];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Sub-voxel peak location: d_Peak (src/Correlation/Peak.cu:26-216), d_PeakOne2D
 // (:218-225, PeakOne2DKernel :380-454) and d_SubpixelMax
@@ -1160,6 +1197,11 @@ double orc_pick_f32(const float* projft, int p, double oversample, const float* 
                     int32_t* idx_out, float* top1_out, float* top2_out, float* sigma_out) {
   return pick<float>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+void orc_ctf_tilt_zgrid_f64(const double* angles, double defocus, double pixelsize, int imx, int imy, int rx,
+                            int ry, const int* pos, int n, double* z) {
+  tilt_zgrid(angles, defocus, pixelsize, imx, imy, rx, ry, pos, n, z);
 }
 
 // d_Peak on `batch` float maps of (nx, ny, nz): mode 1 integer, 2 sub-coarse,

@@ -47,6 +47,7 @@ def lib():
         L.orc_ctf_simulate_f64.argtypes = [_p, _p, _p, _p, _i, _i, _i, _i]
         L.orc_full_pad_f64.argtypes = [_p, _p, _i, _i, _i, _i, _i, _i]
         L.orc_peak_f64.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
+        L.orc_ctf_tilt_zgrid_f64.argtypes = [_p, _d, _d, _i, _i, _i, _i, _p, _i, _p]
         L.orc_peak_one_2d_f64.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
@@ -243,6 +244,17 @@ def ctf_simulate(params, addresses, gammacorr=None, ampsq=False, ignorefirst=Fal
     lib().orc_ctf_simulate_f64(_ptr(P), _ptr(A), None if g is None else _ptr(g), _ptr(out), len(A), int(ampsq),
                                int(ignorefirst), len(P))
     return out
+
+
+def ctf_tilt_zgrid(imageangle, stageangle, specimenangle, defocus, pixelsize, dimsimage, dimsregion, positions):
+    """CTFTiltParams::GetZGrid2D: defocus under each region centre; positions (n, 2 or 3) of (x, y[, z])."""
+    a = np.array([imageangle, stageangle[0], stageangle[1], specimenangle[0], specimenangle[1]], np.float64)
+    p = np.zeros((len(positions), 3), np.int32)
+    p[:, :np.asarray(positions).shape[1]] = positions
+    z = np.empty(len(p), np.float64)
+    lib().orc_ctf_tilt_zgrid_f64(_ptr(a), float(defocus), float(pixelsize), int(dimsimage[0]), int(dimsimage[1]),
+                                 int(dimsregion[0]), int(dimsregion[1]), _ptr(p), len(p), _ptr(z))
+    return z
 
 
 def ctf_addresses_2d(n):

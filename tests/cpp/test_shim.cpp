@@ -249,6 +249,15 @@ static void ctf_simulate_known_answers() {
   EXPECT(std::fabs(out[0] - 0.07f) < 1e-6f && std::fabs(out[n] - 0.07f) < 1e-6f, "CTF(0) = %g, %g", out[0], out[n]);
   for (int i = 0; i < n; ++i) EXPECT(std::fabs(out[i] - out[n + i]) < 1e-4f, "k %d: %g vs %g", i, out[i], out[n + i]);
   std::printf("ok ctf_simulate_known_answers\n");
+
+  // a stage tilt of 0.5 rad about x: defocus changes by tan(0.5) * y offset
+  CTFTiltParams tilt(make_float3(0.f, 0.5f, 0.f), CTFParams());
+  int3 origins[2] = {make_int3(448, 448, 0), make_int3(448, 0, 0)};
+  float z[2];
+  tilt.GetZGrid2D(make_int2(1024, 1024), make_int2(128, 128), origins, 2, z);
+  EXPECT(std::fabs(z[0] - p[0].defocus) < 1e-12f, "centre defocus %g", z[0]);
+  EXPECT(std::fabs(z[1] - (p[0].defocus - 448e-10f * std::tan(0.5f))) < 1e-12f, "tilted defocus %g", z[1]);
+  std::printf("ok ctf_tilt_grid\n");
   hipFree(d_addr); hipFree(d_out);
 }
 

@@ -36,3 +36,21 @@ def test_ctf_2d_matches_oracle(gx, orc, cuda):
     got = to_np(gx.ctf_2d([gx.CtfParamsFull.make(**c) for c in CASES], n, device=cuda))
     ref = orc.ctf_simulate(_f32(CASES), orc.ctf_addresses_2d(n)).reshape(len(CASES), n, n // 2 + 1)
     assert np.max(np.abs(got - ref)) <= 2e-6
+
+
+def test_tilted_region_ctfs_match_oracle(gx, orc, cuda):
+    """Per-region CTFs of a tilted specimen: GetParamsGrid2D on the host, then the
+    2D half-spectrum CTF of each region on the GPU, against the oracle's
+    d_CTFSimulate at the same addresses with the oracle's own tilted defoci."""
+    centre = gx.CtfParamsFull.make(pixelsize=2e-10, defocus=-3e-6, defocusdelta=0.2e-6, astigmatismangle=0.7)
+    tilt = gx.CtfTiltParams.make(centre, imageangle=0.3, stageangle=(0.2, 0.6))
+    pos = [(0, 0), (896, 0), (448, 448), (0, 896), (896, 896)]
+    params = gx.ctf_tilt_params_grid(tilt, (1024, 1024), (128, 128), pos)
+    got = to_np(gx.ctf_2d(params, 128, device=cuda))
+    f32 = lambda v: float(np.float32(v))
+    z = orc.ctf_tilt_zgrid(f32(0.3), (f32(0.2), f32(0.6)), (0.0, 0.0), f32(centre.defocus), f32(centre.pixelsize),
+                           (1024, 1024), (128, 128), pos)
+    assert np.ptp(z) > 1e-8  # the plane's tilt separates the regions' defoci
+    fields = {f: f32(getattr(centre, f)) for f, _ in gx.CtfParamsFull._fields_}
+    ref = orc.ctf_simulate([dict(fields, defocus=f32(zi)) for zi in z], orc.ctf_addresses_2d(128))
+    assert np.abs(got.reshape(len(pos), -1) - ref).max() <= 2e-5

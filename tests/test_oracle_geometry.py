@@ -217,3 +217,50 @@ def test_ctf_simulate_flags_and_bfactor(orc):
     assert np.allclose(b, base * np.exp(-50 * 0.25 * (k / 2.0) ** 2))
     first = orc.ctf_simulate([dict(pixelsize=2e-10)], a, ignorefirst=True)[0]
     assert first[0] == 1.0 and np.allclose(first[-1], base[-1])
+
+
+def _tilt_positions(imx, imy, rx, ry):
+    return [(x, y) for y in range(0, imy - ry + 1, (imy - ry) // 3) for x in range(0, imx - rx + 1, (imx - rx) // 4)]
+
+
+def test_ctf_tilt_zgrid_untilted_is_constant(orc):
+    pos = _tilt_positions(1024, 768, 128, 128)
+    z = orc.ctf_tilt_zgrid(0.7, (0.3, 0.0), (1.1, 0.0), -2.5e-6, 1.5e-10, (1024, 768), (128, 128), pos)
+    assert np.allclose(z, -2.5e-6, rtol=0, atol=1e-18)
+
+
+@pytest.mark.parametrize("axis,theta,imageangle", [(0.0, 0.4, 0.0), (np.pi / 2, -0.3, 0.0), (0.6, 0.25, 0.9)])
+def test_ctf_tilt_zgrid_single_tilt_is_the_rotated_plane(orc, axis, theta, imageangle):
+    """One tilt: the height of the plane whose normal is scipy's rotation of e_z
+    about the in-plane axis, at the image-rotated region centre."""
+    from scipy.spatial.transform import Rotation
+    imx, imy, rx, ry, px, d0 = 1024, 768, 128, 96, 2e-10, -3e-6
+    pos = _tilt_positions(imx, imy, rx, ry)
+    n = Rotation.from_rotvec(theta * np.array([np.cos(axis), np.sin(axis), 0.0])).apply([0, 0, 1.0])
+    c, s = np.cos(imageangle), np.sin(imageangle)
+    want = []
+    for x, y in pos:
+        fx, fy = (x + rx / 2 - imx / 2) * px, (y + ry / 2 - imy / 2) * px
+        gx, gy = c * fx - s * fy, s * fx + c * fy
+        want.append(d0 - (gx * n[0] + gy * n[1]) / n[2])
+    z = orc.ctf_tilt_zgrid(imageangle, (axis, theta), (0.0, 0.0), d0, px, (imx, imy), (rx, ry), pos)
+    assert np.allclose(z, want, rtol=0, atol=1e-20)
+
+
+@pytest.mark.parametrize("sa,st,pa,pt", [(0.4, 0.35, 1.3, -0.2), (0.0, 0.0, np.pi / 2, -0.3), (0.9, 0.0, 0.2, 0.4)])
+def test_ctf_tilt_zgrid_specimen_tilt_keeps_the_reference_matrix(orc, sa, st, pa, pt):
+    """A specimen tilt: the stage matrix's [0][0] element is c + a_x + a_x(1 - c)
+    (src/Helpers/Euler.cu:319) instead of Rodrigues' c + a_x^2 (1 - c) (even for
+    an untilted stage, where it is 1 + a_x), so the normal's x component exceeds
+    scipy's composition by (a_x + a_x(1-c)(1 - a_x)) v_x, v = the specimen-rotated e_z."""
+    from scipy.spatial.transform import Rotation
+    ax = np.array([np.cos(sa), np.sin(sa), 0.0])
+    v = Rotation.from_rotvec(pt * np.array([np.cos(pa), np.sin(pa), 0.0])).apply([0, 0, 1.0])
+    n = Rotation.from_rotvec(st * ax).apply(v)
+    c = np.cos(st)
+    n[0] += (ax[0] + ax[0] * (1 - c) * (1 - ax[0])) * v[0]
+    imx, imy, rx, ry, px, d0 = 512, 512, 64, 64, 3e-10, -2e-6
+    pos = _tilt_positions(imx, imy, rx, ry)
+    want = [d0 - ((x + rx / 2 - imx / 2) * px * n[0] + (y + ry / 2 - imy / 2) * px * n[1]) / n[2] for x, y in pos]
+    z = orc.ctf_tilt_zgrid(0.0, (sa, st), (pa, pt), d0, px, (imx, imy), (rx, ry), pos)
+    assert np.allclose(z, want, rtol=0, atol=1e-20)
