@@ -102,6 +102,8 @@ _SIGS = {
     "gtomx_ctf_simulate": [_P, _P, _P, _P, _U, _I, _I, _U, _P],
     "gtomx_ctf_2d": [_P, _U, int3, _P, _P],
     "gtomx_ctf_tilt_params_grid": [_P, int3, int3, _P, _U, _P],
+    "gtomx_rotate3d_ft": [_P, _P, int3, _P, _U, _I, _P],
+    "gtomx_rotate3d_ft_real": [_P, _P, int3, _P, _U, _I, _P],
     "gtomx_pick_subtomograms_diff2": [_P, _F, int3, _P, _P, int3, _U, int3, _P, _U, _P, _P, _P],
     "gtomx_mrc_header_default": [_P],
     "gtomx_mrc_read_header": [ctypes.c_char_p, _P],
@@ -303,6 +305,21 @@ def d_rlnProject(volume_ft, p, n, matrices=None, angles=None, oversample=1.0, rm
     check(lib().gtomx_project(_ptr(volume_ft), int3(p, p, p), _ptr(out), int3(n, n, nz),
                               n // 2 if rmax is None else rmax, _ptr(matrices), batch, _stream(stream)),
           "gtomx_project")
+    return out
+
+
+def d_Rotate3DFT(volume, angles, outputzerocentered=True, stream=None):
+    """d_Rotate3DFT, linear (src/Transformation/Rotation.cu:263-374): a (n, n, n/2+1)
+    zero-centred half spectrum (complex64 or float32 CUDA tensor) rotated by each
+    (rot, tilt, psi) of `angles`; returns (nangles, n, n, n/2+1) of the same dtype."""
+    import numpy as np
+    import torch
+    a = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(-1, 3))
+    n = volume.shape[0]
+    out = torch.empty((len(a),) + tuple(volume.shape), dtype=volume.dtype, device=volume.device)
+    fn = lib().gtomx_rotate3d_ft if volume.is_complex() else lib().gtomx_rotate3d_ft_real
+    check(fn(_ptr(volume), _ptr(out), int3(n, n, n), ctypes.c_void_p(a.ctypes.data), len(a), int(bool(outputzerocentered)),
+             _stream(stream)), "gtomx_rotate3d_ft")
     return out
 
 

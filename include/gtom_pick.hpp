@@ -187,6 +187,29 @@ inline void d_LocalPeaks(tfloat* d_input, int3** h_peaks, int* h_peaksnum, int3 
   }
 }
 
+// T_INTERP_MODE, include/Prerequisites.cuh:239-245.
+enum T_INTERP_MODE { T_INTERP_LINEAR = 1, T_INTERP_CUBIC = 2, T_INTERP_FOURIER = 3, T_INTERP_SINC = 4 };
+
+// d_Rotate3DFT, include/Transformation.cuh:54 (complex) and the real-valued
+// overload (src/Transformation/Rotation.cu:325): linear interpolation only.
+inline void d_Rotate3DFT(tcomplex* d_volume, tcomplex* d_output, int3 dims, tfloat3* h_angles, int nangles,
+                         T_INTERP_MODE mode, bool outputzerocentered) {
+  if (mode != T_INTERP_LINEAR) throw std::runtime_error("d_Rotate3DFT: only T_INTERP_LINEAR is built");
+  detail::check(gtomx_rotate3d_ft(reinterpret_cast<const gtomx_complex*>(d_volume),
+                                  reinterpret_cast<gtomx_complex*>(d_output), detail::i3(dims),
+                                  reinterpret_cast<const gtomx_float3*>(h_angles), (unsigned)nangles,
+                                  outputzerocentered ? 1 : 0, nullptr),
+                "d_Rotate3DFT");
+}
+inline void d_Rotate3DFT(tfloat* d_volume, tfloat* d_output, int3 dims, tfloat3* h_angles, int nangles,
+                         T_INTERP_MODE mode, bool outputzerocentered) {
+  if (mode != T_INTERP_LINEAR) throw std::runtime_error("d_Rotate3DFT: only T_INTERP_LINEAR is built");
+  detail::check(gtomx_rotate3d_ft_real(d_volume, d_output, detail::i3(dims),
+                                       reinterpret_cast<const gtomx_float3*>(h_angles), (unsigned)nangles,
+                                       outputzerocentered ? 1 : 0, nullptr),
+                "d_Rotate3DFT");
+}
+
 // T_PEAK_MODE, include/Correlation.cuh:44-52.
 enum T_PEAK_MODE { T_PEAK_INTEGER = 1, T_PEAK_SUBCOARSE = 2, T_PEAK_SUBFINE = 3 };
 

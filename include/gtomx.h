@@ -310,6 +310,25 @@ int gtomx_picker_correlate(gtomx_picker* picker, float anglestep, float* d_bestc
 int gtomx_picker_correlate_angles(gtomx_picker* picker, const gtomx_float3* h_angles, unsigned nangles,
                                   float* d_bestccf, gtomx_float3* d_bestangle, void* stream);
 
+/* d_Rotate3DFT with T_INTERP_LINEAR, include/Transformation.cuh:54 /
+ * src/Transformation/Rotation.cu:263-374 (Rotate3DFTKernel :541-654): each of
+ * nangles rotations of the half spectrum of a cube of side n. Input in the
+ * zero-centred half layout: element (tx, ty, tz) of (n/2+1, n, n) holds the
+ * frequency (tx - n/2, ty - n/2, tz - n/2). Output bin (idx, idy, idz), in the
+ * same layout when outputzerocentered, else in FFT order read at the negated
+ * centred frequency as the reference does, is the input interpolated
+ * trilinearly at transpose(Euler(angle)) applied to its centred frequency,
+ * 0 where that lies at radius >= n/2, taken from the Hermitian mate (conjugate)
+ * where its x is positive. The reference's hardware filter quantises weights
+ * to 1/256; here they are exact float and texel indices are clamped. Output:
+ * nangles * (n/2+1)*n*n values. Cubic interpolation is not built. Synchronous. */
+int gtomx_rotate3d_ft(const gtomx_complex* d_volume, gtomx_complex* d_output, gtomx_int3 dims,
+                      const gtomx_float3* h_angles, unsigned nangles, int outputzerocentered, void* stream);
+/* The same on real-valued half spectra (src/Transformation/Rotation.cu:325-374,
+ * kernel :606-654): no conjugation. */
+int gtomx_rotate3d_ft_real(const float* d_volume, float* d_output, gtomx_int3 dims, const gtomx_float3* h_angles,
+                           unsigned nangles, int outputzerocentered, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Peak extraction (the consumer of the search output)                       */
 /* ------------------------------------------------------------------------ */

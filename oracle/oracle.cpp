@@ -648,6 +648,52 @@ double ctf_at(const CtfFull& q, double k, double angle, double gammacorr, bool a
   return v * q.scale;
 }
 
+// d_Rotate3DFT, T_INTERP_LINEAR (src/Transformation/Rotation.cu:263-374,
+// Rotate3DFTKernel :541-654), one orientation; M = transpose(Euler) row-major.
+// in: centred half layout (n/2+1, n, n), element (tx, ty, tz) at frequency
+// (tx - n/2, ty - n/2, tz - n/2); ncomp = 2 for interleaved complex, 1 for real.
+// Trilinear interpolation with exact weights and clamped texel indices (the
+// reference's texture: 8-bit weights; wrap mode on unnormalised coordinates
+// acts as clamp).
+void rotate3d_ft(const double* in, double* out, int n, const double* M, bool zerocentred, int ncomp) {
+  const int nh = n / 2 + 1;
+  auto clampi = [](long v, long hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  for (int idz = 0; idz < n; ++idz)
+    for (int idy = 0; idy < n; ++idy)
+      for (int idx = 0; idx < nh; ++idx) {
+        int x = idx, y = idy, z = idz;
+        if (!zerocentred) {
+          x = n / 2 - idx;
+          y = n - 1 - ((idy + n / 2 - 1) % n);
+          z = n - 1 - ((idz + n / 2 - 1) % n);
+        }
+        const double cx = x - n / 2, cy = y - n / 2, cz = z - n / 2;
+        double p[3];
+        for (int r = 0; r < 3; ++r) p[r] = M[3 * r] * cx + M[3 * r + 1] * cy + M[3 * r + 2] * cz;
+        double* o = out + (((size_t)idz * n + idy) * nh + idx) * ncomp;
+        if (p[0] * p[0] + p[1] * p[1] + p[2] * p[2] >= (double)(n * n / 4)) {
+          for (int c = 0; c < ncomp; ++c) o[c] = 0;
+          continue;
+        }
+        const bool neg = p[0] > 1e-6;
+        double t[3];
+        for (int r = 0; r < 3; ++r) t[r] = (neg ? -p[r] : p[r]) + n / 2;
+        const long f[3] = {(long)std::floor(t[0]), (long)std::floor(t[1]), (long)std::floor(t[2])};
+        const double a[3] = {t[0] - f[0], t[1] - f[1], t[2] - f[2]};
+        const long hi[3] = {nh - 1, n - 1, n - 1};
+        for (int c = 0; c < ncomp; ++c) {
+          double v = 0;
+          for (int k = 0; k < 8; ++k) {
+            const int bx = k & 1, by = (k >> 1) & 1, bz = k >> 2;
+            const double w = (bx ? a[0] : 1 - a[0]) * (by ? a[1] : 1 - a[1]) * (bz ? a[2] : 1 - a[2]);
+            const long ix = clampi(f[0] + bx, hi[0]), iy = clampi(f[1] + by, hi[1]), iz = clampi(f[2] + bz, hi[2]);
+            v += w * in[(((size_t)iz * n + iy) * nh + ix) * ncomp + c];
+          }
+          o[c] = (c == 1 && neg) ? -v : v;
+        }
+      }
+}
+
 // Tilted specimen plane: CTFTiltParams::GetZGrid2D (include/CTF.cuh:160-180).
 // Matrix3RotationCustom (src/Helpers/Euler.cu:312-331) as written there, glm
 // column-major (m[col][row]); its element [0][0] is c + a.x + a.x (1 - c), not
@@ -1197,6 +1243,10 @@ double orc_pick_f32(const float* projft, int p, double oversample, const float* 
                     int32_t* idx_out, float* top1_out, float* top2_out, float* sigma_out) {
   return pick<float>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+void orc_rotate3d_ft_f64(const double* in, double* out, int n, const double* M, int zerocentred, int ncomp) {
+  rotate3d_ft(in, out, n, M, zerocentred != 0, ncomp);
 }
 
 void orc_ctf_tilt_zgrid_f64(const double* angles, double defocus, double pixelsize, int imx, int imy, int rx,

@@ -48,6 +48,7 @@ def lib():
         L.orc_full_pad_f64.argtypes = [_p, _p, _i, _i, _i, _i, _i, _i]
         L.orc_peak_f64.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
         L.orc_ctf_tilt_zgrid_f64.argtypes = [_p, _d, _d, _i, _i, _i, _i, _p, _i, _p]
+        L.orc_rotate3d_ft_f64.argtypes = [_p, _p, _i, _p, _i, _i]
         L.orc_peak_one_2d_f64.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
@@ -255,6 +256,18 @@ def ctf_tilt_zgrid(imageangle, stageangle, specimenangle, defocus, pixelsize, di
     lib().orc_ctf_tilt_zgrid_f64(_ptr(a), float(defocus), float(pixelsize), int(dimsimage[0]), int(dimsimage[1]),
                                  int(dimsregion[0]), int(dimsregion[1]), _ptr(p), len(p), _ptr(z))
     return z
+
+
+def rotate3d_ft(vol, M, zerocentred=True):
+    """d_Rotate3DFT (linear) of one centred half spectrum (n, n, n/2+1), complex or
+    real, by the row-major matrix M = transpose(Euler)."""
+    v = np.asarray(vol)
+    cplx = np.iscomplexobj(v)
+    src = np.ascontiguousarray(v, np.complex128).view(np.float64) if cplx else _c(v, np.float64)
+    n = v.shape[0]
+    out = np.empty_like(src)
+    lib().orc_rotate3d_ft_f64(_ptr(src), _ptr(out), n, _ptr(_c(M, np.float64)), int(zerocentred), 2 if cplx else 1)
+    return out.view(np.complex128) if cplx else out
 
 
 def ctf_addresses_2d(n):
