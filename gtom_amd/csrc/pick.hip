@@ -132,6 +132,15 @@ template <int N, class PJ> struct Pass1Occupancy {
   static constexpr int MIN_BLOCKS = (N == 256 && std::is_same<PJ, CentredProjector>::value) ? 3 : 1;
 };
 
+// Visiting order of a thread's samples m (z = j + R2 m in FFT order): 0, R1-1,
+// 1, R1-2, ... is increasing |z|, so each group of samples is a shell of |z| and
+// whole groups fall outside the sphere for the outer columns.
+template <int R1> __device__ constexpr int zorder(int k) { return k % 2 == 0 ? k / 2 : R1 - 1 - k / 2; }
+
+// Waves per pass-1 workgroup; each stores its own partial sum of |R|^2 per
+// (orientation, tile), so that no barrier is spent on a block reduction.
+template <int N> constexpr int pass1_waves() { return (ColShape<N>::TX * ColShape<N>::R2 + 63) / 64; }
+
 template <int N, class PJ>
 __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2, (Pass1Occupancy<N, PJ>::MIN_BLOCKS)) void
 k_pick_project_z(
@@ -142,10 +151,10 @@ k_pick_project_z(
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
   constexpr int NT = TX * R2;
-  constexpr int GRP = 4;  // projector samples per group of in-flight gathers
+  constexpr int GRP = 2;  // projector samples per group of in-flight gathers
   static_assert(R1 % GRP == 0, "R1 must be a multiple of the sample group");
+  static_assert(R1 % 2 == 0, "zorder is a permutation of 0..R1-1 for even R1");
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
-  __shared__ float red[(NT + 63) / 64];
   // The N twiddles of the z transform, staged once per workgroup in LDS (the
   // first barrier before their use is the one ahead of the first transform).
   __shared__ float2 twl[N];
@@ -197,24 +206,24 @@ k_pick_project_z(
       const float q0x = M[0] * ix + M[1] * y, q0y = M[3] * ix + M[4] * y, q0z = M[6] * ix + M[7] * y;
       float acc = 0.f;
       float2 v[R1];
-      // Samples in groups of GRP: the group's gathers are in flight together, the
-      // scheduling barrier keeps the next group's loads from being hoisted (which
-      // would exhaust the registers), and a group whose bins all lie outside the
-      // sphere in this wave is skipped.
+      // Samples in groups of GRP, in increasing |z|: the group's gathers are in
+      // flight together, the scheduling barrier keeps the next group's loads from
+      // being hoisted (which would exhaust the registers), and a group whose bins
+      // all lie outside the sphere in this wave is skipped.
 #pragma unroll
       for (int g = 0; g < R1 / GRP; ++g) {
         bool ins[GRP];
         bool any = false;
 #pragma unroll
         for (int u = 0; u < GRP; ++u) {
-          const int z = signed_freq(j + R2 * (g * GRP + u), N);
+          const int z = signed_freq(j + R2 * zorder<R1>(g * GRP + u), N);
           ins[u] = valid && ix * ix + y * y + z * z <= RMAX2;
           any |= ins[u];
         }
         if (__any(any)) {
 #pragma unroll
           for (int u = 0; u < GRP; ++u) {
-            const int m = g * GRP + u;
+            const int m = zorder<R1>(g * GRP + u);
             const float z = (float)signed_freq(j + R2 * m, N);
             const float2 r = cscale(
                 sample(rs, P, fmaf(M[2], z, q0x), fmaf(M[5], z, q0y), fmaf(M[8], z, q0z), ins[u]), cf[m]);
@@ -223,7 +232,7 @@ k_pick_project_z(
           }
         } else {
 #pragma unroll
-          for (int u = 0; u < GRP; ++u) v[g * GRP + u] = make_float2(0.f, 0.f);
+          for (int u = 0; u < GRP; ++u) v[zorder<R1>(g * GRP + u)] = make_float2(0.f, 0.f);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -240,8 +249,9 @@ k_pick_project_z(
         const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
         __builtin_amdgcn_raw_buffer_store_b64(vs, rG, gbase8, (u + R1 * k2) * gstep * 8, CPOL_NT);
       }
-      const float tot = block_sum(acc, red);
-      if (t == 0) partial[(size_t)o * ntiles + ti] = tot;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if ((t & 63) == 0) partial[((size_t)o * ntiles + ti) * pass1_waves<N>() + (t >> 6)] = acc;
     }
   }
 }
@@ -372,7 +382,8 @@ __global__ __launch_bounds__(Pass2Tile<N>::TX * ColShape<N>::R2) void k_pick_y(
     // unit-RMS scale over the H half-spectrum bins (src/ImageManipulation/Norm.cu:476-509)
     __syncthreads();  // the exchange area is free again
     double sum = 0;
-    for (int i = t; i < ntiles; i += NT) sum += (double)partial[(size_t)o * ntiles + i];
+    const int nparts = ntiles * pass1_waves<N>();
+    for (int i = t; i < nparts; i += NT) sum += (double)partial[(size_t)o * nparts + i];
     for (int i = t; i < HERM_BLOCKS; i += NT) sum += (double)corr[(size_t)o * HERM_BLOCKS + i];
     dred[t] = sum;
     __syncthreads();
@@ -963,7 +974,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   alloc(&d, std::max((size_t)pl->B * (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n)), pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
-  alloc(&d, (size_t)pl->B * pl->ntiles * sizeof(float));
+  alloc(&d, (size_t)pl->B * pl->ntiles * 4 * sizeof(float));  // up to 4 pass-1 waves per tile
   pl->partial = static_cast<float*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->ntiles * sizeof(int));
