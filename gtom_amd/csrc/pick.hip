@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
                                                     int og = 0) {
   constexpr int NC = N / 2;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
-  constexpr int GS = R2 * (R1 + 1);
+  constexpr int GS = R2 * (R1 + 2);  // two float2 of padding per thread slot (fewer bank conflicts than one)
   __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
   const int row = blockIdx.x * RW + r;  // row = z*N + y (volumes) or y (images)
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
       v[m] = cadd(cadd(a0, bc), mul_pj<1>(cmul(twp[m], csub(a0, bc))));
     }
     __syncthreads();
-    fft_two_step<NC, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[r * GS + jj * (R1 + 1) + k1]; },
+    fft_two_step<NC, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[r * GS + jj * (R1 + 2) + k1]; },
                                 TwiddleRegs<R1>{twh});
     const int32_t oi = angle0 + o;
 #pragma unroll
