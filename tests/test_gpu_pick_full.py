@@ -1,0 +1,61 @@
+"""d_PickSubTomograms at the benchmark's block size: 256^3 blocks searched with
+a 48^3 template through its 96^3 projector at oversample 0.375 (the centred
+sampler, one-wave-group sphere skipping and per-wave partial sums that only
+n = 256 uses), against the FP64 oracle on 40 orientations in two batches,
+and by a size-independent property over more orientations: a copy planted at
+a search orientation peaks at its centre with that orientation's index.
+
+Tolerance as for every search test: max-CC within 1e-4 normwise, argmax
+identical wherever the oracle's top-2 margin exceeds 1e-4 * max|top1|."""
+import numpy as np
+import pytest
+
+from tests._util import argmax_agreement, rel_linf, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+def test_full_size_block_matches_oracle(gx, orc, cuda):
+    import torch
+    from gtom_amd import synth
+    w = synth.cfg2(seed=5, shape=(256, 256, 256), ncopies=6)
+    n, p, s = w["n"], w["p"], w["oversample"]
+    angles = w["angles"][::225][:40]  # spread over the 9000 directions x psi; a full batch of 32 and 8
+    ctf_np = synth.ctf3_numpy(n, pixelsize=w["pixelsize"], defocus=-w["defoci"][0]).astype(np.float32)
+    projft = gx.prepare_projector(torch.from_numpy(w["template"]).to(cuda), p)
+    exp_ft = gx.d_FFTR2C(torch.from_numpy(w["blocks"]).to(cuda))
+    ctf = torch.from_numpy(ctf_np).to(cuda)
+    plan = gx.PickPlan(p, n)  # the benchmark's batch of 32
+    best, idx = gx.d_PickSubTomograms(plan, projft, s, exp_ft, ctf, angles, w["maskradius"])
+    torch.cuda.synchronize()
+    plan.close()
+    ref = orc.pick(to_np(projft).astype(np.complex128), p, s, to_np(exp_ft).astype(np.complex128),
+                   ctf_np.astype(np.float64), n, angles.astype(np.float64), w["maskradius"])
+    assert rel_linf(to_np(best), ref["best"]) <= 1e-4
+    agree, cnt = argmax_agreement(to_np(idx), ref["idx"], ref["top1"], ref["top2"], 1e-4)
+    assert cnt > 0 and agree == 1.0, (agree, cnt)
+
+
+def test_full_size_planted_copy_found(gx, cuda):
+    import torch
+    from gtom_amd import synth
+    rng = np.random.default_rng(21)
+    n, box = 256, 48
+    p = 2 * box
+    angles = gx.fibonacci_angles(500, 18)[::141][:64]  # 64 orientations, two batches of 32
+    target = 45
+    template = synth.blob_template(box, rng, radius=14.0)
+    pos = (131, 77, 190)
+    sig = synth.plant((n, n, n), template, [pos], [tuple(float(a) for a in angles[target])], 1.0)
+    tomo = (sig + 0.05 * rng.standard_normal((n, n, n))).astype(np.float32)
+    projft = gx.prepare_projector(torch.from_numpy(template).to(cuda), p)
+    exp_ft = gx.d_FFTR2C(torch.from_numpy(tomo[None]).to(cuda))
+    ctf = torch.ones((n, n, n // 2 + 1), device=cuda)
+    plan = gx.PickPlan(p, n)
+    best, idx = gx.d_PickSubTomograms(plan, projft, p / n, exp_ft, ctf, angles, 24.0)
+    torch.cuda.synchronize()
+    plan.close()
+    b = to_np(best)[0]
+    z, y, x = np.unravel_index(np.argmax(b), b.shape)
+    assert (x, y, z) == pos
+    assert to_np(idx)[0, z, y, x] == target
