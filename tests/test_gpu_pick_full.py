@@ -1,7 +1,6 @@
 """d_PickSubTomograms at the benchmark's block size: 256^3 blocks searched with
 a 48^3 template through its 96^3 projector at oversample 0.375 (the centred
-sampler, one-wave-group sphere skipping and per-wave partial sums that only
-n = 256 uses), against the FP64 oracle on 40 orientations in two batches,
+sampler and the occupancy bound that only n = 256 uses), against the FP64 oracle on 40 orientations in two batches,
 and by a size-independent property over more orientations: a copy planted at
 a search orientation peaks at its centre with that orientation's index.
 
