@@ -113,6 +113,8 @@ _SIGS = {
     "gtomx_mrc_write_device_f32": [_P, int3, ctypes.c_char_p, _P],
     "gtomx_mrc_to_device_f32": [_P, _I, _SZ, _P, _P],
     "gtomx_norm_mean01std": [_P, _P, _SZ, _U, _P],
+    "gtomx_norm_monolithic": [_P, _P, _SZ, _U, _P],
+    "gtomx_norm_ft_monolithic": [_P, _P, _SZ, _U, _P],
     "gtomx_picker_create": [ctypes.POINTER(_P), _P, int3, _P, int3, _U, _P],
     "gtomx_picker_destroy": [_P],
     "gtomx_picker_set_image": [_P, _P, _P, _P],
@@ -470,6 +472,26 @@ def d_SubpixelMax(vol, out, subpixsteps, stream=None):
     nz = vol.shape[-3] if vol.dim() == 3 else 1
     check(lib().gtomx_subpixel_max(_ptr(vol), _ptr(out), int3(nx, ny, nz), int(subpixsteps), _stream(stream)),
           "gtomx_subpixel_max")
+    return out
+
+
+def d_NormMonolithic(x, batch=1, stream=None):
+    """d_NormMonolithic (src/ImageManipulation/Norm.cu:76-91) of `batch` equal parts of a
+    float32 CUDA tensor; returns a new tensor."""
+    import torch
+    out = torch.empty_like(x)
+    check(lib().gtomx_norm_monolithic(_ptr(x), _ptr(out), x.numel() // batch, batch, _stream(stream)),
+          "gtomx_norm_monolithic")
+    return out
+
+
+def d_NormFTMonolithic(x, batch=1, stream=None):
+    """d_NormFTMonolithic (src/ImageManipulation/Norm.cu:147-154) of `batch` equal parts of a
+    complex64 CUDA tensor; returns a new tensor."""
+    import torch
+    out = torch.empty_like(x)
+    check(lib().gtomx_norm_ft_monolithic(_ptr(x), _ptr(out), x.numel() // batch, batch, _stream(stream)),
+          "gtomx_norm_ft_monolithic")
     return out
 
 

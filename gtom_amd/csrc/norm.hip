@@ -1,0 +1,137 @@
+// Per-volume normalisation of a batch:
+//  * d_NormMonolithic (src/ImageManipulation/Norm.cu:76-91, NormMeanStdDevMonoKernel
+//    :242-284): y = (x - mean) / std with sums of x and x^2 in double, std =
+//    sqrt(N sum x^2 - (sum x)^2) / N, and y = 0 where that is not positive; the
+//    mode argument of the reference is not read there, so there is none here;
+//  * d_NormFTMonolithic (:147-154, NormFTMonoKernel :476-509): a complex spectrum
+//    divided by max(sqrt(sum |x|^2 / N), 1e-20).
+// The reference reduces each volume in one workgroup; here NB workgroups per
+// volume write double partial sums, one workgroup per volume finishes them, and
+// a wide grid applies the result. Mean and scale are rounded to float before
+// they are applied, as the reference stores them.
+#include <algorithm>
+#include <cmath>
+
+#include "runtime.hpp"
+
+namespace gx {
+namespace {
+
+constexpr int NB = 256;  // reduction workgroups per volume
+
+__device__ double2 block_sum2(double2 v, double2* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_down(v.x, o);
+    v.y += __shfl_down(v.y, o);
+  }
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
+  __syncthreads();
+  double2 s = make_double2(0, 0);
+  if (threadIdx.x == 0)
+    for (int w = 0; w < 4; ++w) {
+      s.x += red[w].x;
+      s.y += red[w].y;
+    }
+  __syncthreads();
+  return s;
+}
+
+// part[b][p] = (sum x, sum x^2) for real data, (sum |x|^2, 0) for complex.
+template <bool COMPLEX>
+__global__ __launch_bounds__(256) void k_mono_partial(const float* __restrict__ x, size_t n,
+                                                      double2* __restrict__ part) {
+  __shared__ double2 red[4];
+  const size_t b = blockIdx.y;
+  double2 acc = make_double2(0, 0);
+  if constexpr (COMPLEX) {
+    const float2* xb = reinterpret_cast<const float2*>(x) + b * n;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+      const float2 v = xb[i];
+      acc.x += (double)v.x * v.x + (double)v.y * v.y;
+    }
+  } else {
+    const float* xb = x + b * n;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+      const double v = xb[i];
+      acc.x += v;
+      acc.y += v * v;
+    }
+  }
+  const double2 t = block_sum2(acc, red);
+  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = t;
+}
+
+// coef[b] = (float mean, float 1/std) for real data, (0, float 1/rms) for complex.
+template <bool COMPLEX>
+__global__ __launch_bounds__(256) void k_mono_finish(const double2* __restrict__ part, int nparts, size_t n,
+                                                     float2* __restrict__ coef) {
+  __shared__ double2 red[4];
+  const size_t b = blockIdx.x;
+  double2 acc = make_double2(0, 0);
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    acc.x += part[b * nparts + i].x;
+    acc.y += part[b * nparts + i].y;
+  }
+  const double2 t = block_sum2(acc, red);
+  if (threadIdx.x != 0) return;
+  const double N = (double)n;
+  if constexpr (COMPLEX) {
+    const float rms = fmaxf((float)sqrt(t.x / N), 1e-20f);
+    coef[b] = make_float2(0.f, 1.0f / rms);
+  } else {
+    const double sd = sqrt(N * t.y - t.x * t.x) / N;
+    coef[b] = make_float2((float)(t.x / N), (float)(sd > 0.0 ? 1.0 / sd : 0.0));
+  }
+}
+
+template <bool COMPLEX>
+__global__ __launch_bounds__(256) void k_mono_apply(const float* __restrict__ x, float* __restrict__ y, size_t n,
+                                                    const float2* __restrict__ coef) {
+  const size_t b = blockIdx.y;
+  const float2 c = coef[b];
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    if constexpr (COMPLEX) {
+      const float2 v = reinterpret_cast<const float2*>(x)[b * n + i];
+      reinterpret_cast<float2*>(y)[b * n + i] = make_float2(v.x * c.y, v.y * c.y);
+    } else {
+      y[b * n + i] = (x[b * n + i] - c.x) * c.y;
+    }
+  }
+}
+
+template <bool COMPLEX> int mono(const float* in, float* out, size_t n, unsigned batch, void* stream) {
+  if (!in || !out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
+  if (n == 0 || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "elements and batch must be positive");
+  if (batch > 65535) return fail(GTOMX_ERR_UNSUPPORTED, "batch above 65535");
+  hipStream_t s = as_stream(stream);
+  const int nb = (int)std::min<size_t>(NB, (n + 255) / 256);
+  DevBuf part, coef;
+  GX_TRY(part.alloc((size_t)nb * batch * sizeof(double2)));
+  GX_TRY(coef.alloc((size_t)batch * sizeof(float2)));
+  hipLaunchKernelGGL(k_mono_partial<COMPLEX>, dim3(nb, batch), dim3(256), 0, s, in, n, part.as<double2>());
+  GX_LAUNCHED("k_mono_partial");
+  hipLaunchKernelGGL(k_mono_finish<COMPLEX>, dim3(batch), dim3(256), 0, s, part.as<double2>(), nb, n,
+                     coef.as<float2>());
+  GX_LAUNCHED("k_mono_finish");
+  const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_mono_apply<COMPLEX>, dim3(grid, batch), dim3(256), 0, s, in, out, n, coef.as<float2>());
+  GX_LAUNCHED("k_mono_apply");
+  GX_HIP(hipStreamSynchronize(s));  // the scratch is freed on return
+  return ok();
+}
+
+}  // namespace
+}  // namespace gx
+
+using namespace gx;
+
+extern "C" int gtomx_norm_monolithic(const float* d_input, float* d_output, size_t elements, unsigned batch,
+                                     void* stream) {
+  return mono<false>(d_input, d_output, elements, batch, stream);
+}
+
+extern "C" int gtomx_norm_ft_monolithic(const gtomx_complex* d_input, gtomx_complex* d_output, size_t elements,
+                                        unsigned batch, void* stream) {
+  return mono<true>(reinterpret_cast<const float*>(d_input), reinterpret_cast<float*>(d_output), elements, batch,
+                    stream);
+}

@@ -346,6 +346,27 @@ inline void d_FFTFullPad(tfloat* d_input, tfloat* d_output, int3 olddims, int3 n
   detail::sync();
 }
 
+// T_NORM_MODE, include/ImageManipulation.cuh:37-47.
+enum T_NORM_MODE {
+  T_NORM_NONE = 0, T_NORM_MEAN01STD = 1, T_NORM_PHASE = 2, T_NORM_STD1 = 3, T_NORM_STD2 = 4, T_NORM_STD3 = 5,
+  T_NORM_OSCAR = 6, T_NORM_CUSTOM = 7
+};
+
+// d_NormMonolithic, include/ImageManipulation.cuh:49: mean0/std1 per item
+// whatever the mode, as the reference's kernel does.
+inline void d_NormMonolithic(tfloat* d_input, tfloat* d_output, size_t elements, T_NORM_MODE mode, int batch) {
+  (void)mode;
+  detail::check(gtomx_norm_monolithic(d_input, d_output, elements, (unsigned)batch, nullptr), "d_NormMonolithic");
+}
+
+// d_NormFTMonolithic, include/ImageManipulation.cuh:55.
+inline void d_NormFTMonolithic(tcomplex* d_input, tcomplex* d_output, size_t elements, int batch) {
+  detail::check(gtomx_norm_ft_monolithic(reinterpret_cast<const gtomx_complex*>(d_input),
+                                         reinterpret_cast<gtomx_complex*>(d_output), elements, (unsigned)batch,
+                                         nullptr),
+                "d_NormFTMonolithic");
+}
+
 // d_Norm with T_NORM_MEAN01STD and no mask, include/ImageManipulation.cuh:48
 // (other modes and masks are not built).
 inline void d_NormMean01Std(tfloat* d_input, tfloat* d_output, size_t elements, int batch = 1) {

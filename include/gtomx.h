@@ -268,6 +268,21 @@ int gtomx_pick_subtomograms_diff2(const gtomx_complex* d_projectorft, float proj
 int gtomx_fft_full_pad(const float* d_input, float* d_output, gtomx_int3 olddims, gtomx_int3 newdims,
                        unsigned batch, void* stream);
 
+/* d_NormMonolithic, include/ImageManipulation.cuh:49 / src/ImageManipulation/Norm.cu:76-91
+ * (kernel :242-284): each of `batch` arrays of `elements` floats becomes
+ * (x - mean) / std with sums of x and x^2 in double, std = sqrt(N sum x^2 -
+ * (sum x)^2) / N, and 0 where std is not positive; mean and 1/std are rounded to
+ * float before use. (The reference's mode argument is not read by it.)
+ * Synchronises the stream (its scratch is freed on return). */
+int gtomx_norm_monolithic(const float* d_input, float* d_output, size_t elements, unsigned batch, void* stream);
+
+/* d_NormFTMonolithic, include/ImageManipulation.cuh:55 / src/ImageManipulation/Norm.cu:147-154
+ * (kernel :476-509): each of `batch` complex arrays divided by
+ * max(sqrt(sum |x|^2 / elements), 1e-20) (sum in double, the root rounded to
+ * float). Synchronises the stream. */
+int gtomx_norm_ft_monolithic(const gtomx_complex* d_input, gtomx_complex* d_output, size_t elements,
+                             unsigned batch, void* stream);
+
 /* d_Norm with T_NORM_MEAN01STD and no mask, include/ImageManipulation.cuh:48 /
  * src/ImageManipulation/Norm.cu:36-70 (statistics of d_Dev, src/Generics/Dev.cu:13-72):
  * each of `batch` arrays of `elements` floats becomes (x - mean)/std with the

@@ -648,6 +648,29 @@ double ctf_at(const CtfFull& q, double k, double angle, double gammacorr, bool a
   return v * q.scale;
 }
 
+// d_NormMonolithic (src/ImageManipulation/Norm.cu:76-91, NormMeanStdDevMonoKernel
+// :242-284): (x - mean) / std, std = sqrt(N sum x^2 - (sum x)^2) / N in double, 0
+// where std is not positive. In double throughout (the reference rounds mean
+// and 1/std to float).
+void norm_monolithic(const float* in, double* out, long long n) {
+  double s1 = 0, s2 = 0;
+  for (long long i = 0; i < n; ++i) {
+    s1 += in[i];
+    s2 += (double)in[i] * in[i];
+  }
+  const double mean = s1 / n, sd = std::sqrt(n * s2 - s1 * s1) / n;
+  const double inv = sd > 0 ? 1.0 / sd : 0.0;
+  for (long long i = 0; i < n; ++i) out[i] = (in[i] - mean) * inv;
+}
+
+// d_NormFTMonolithic (:147-154, NormFTMonoKernel :476-509): x / max(sqrt(sum |x|^2 / N), 1e-20).
+void norm_ft_monolithic(const float* in, double* out, long long n) {
+  double s = 0;
+  for (long long i = 0; i < 2 * n; ++i) s += (double)in[i] * in[i];
+  const double inv = 1.0 / std::max(std::sqrt(s / n), 1e-20);
+  for (long long i = 0; i < 2 * n; ++i) out[i] = in[i] * inv;
+}
+
 // d_Rotate3DFT, T_INTERP_LINEAR (src/Transformation/Rotation.cu:263-374,
 // Rotate3DFTKernel :541-654), one orientation; M = transpose(Euler) row-major.
 // in: centred half layout (n/2+1, n, n), element (tx, ty, tz) at frequency
@@ -1243,6 +1266,15 @@ double orc_pick_f32(const float* projft, int p, double oversample, const float* 
                     int32_t* idx_out, float* top1_out, float* top2_out, float* sigma_out) {
   return pick<float>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+void orc_norm_monolithic_f64(const float* in, double* out, long long n, int batch, int complex_values) {
+  for (int b = 0; b < batch; ++b) {
+    if (complex_values)
+      norm_ft_monolithic(in + (size_t)b * 2 * n, out + (size_t)b * 2 * n, n);
+    else
+      norm_monolithic(in + (size_t)b * n, out + (size_t)b * n, n);
+  }
 }
 
 void orc_rotate3d_ft_f64(const double* in, double* out, int n, const double* M, int zerocentred, int ncomp) {

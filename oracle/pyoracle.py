@@ -49,6 +49,7 @@ def lib():
         L.orc_peak_f64.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
         L.orc_ctf_tilt_zgrid_f64.argtypes = [_p, _d, _d, _i, _i, _i, _i, _p, _i, _p]
         L.orc_rotate3d_ft_f64.argtypes = [_p, _p, _i, _p, _i, _i]
+        L.orc_norm_monolithic_f64.argtypes = [_p, _p, _ll, _i, _i]
         L.orc_peak_one_2d_f64.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
@@ -331,6 +332,26 @@ def full_pad(vol, newshape):
     out = np.empty(newshape, np.float64)
     lib().orc_full_pad_f64(_ptr(v), _ptr(out), ox, oy, oz, nx, ny, nz)
     return out
+
+
+def norm_monolithic(a):
+    """d_NormMonolithic over the last axis group: a is (batch, ...) float32; each item normalised."""
+    v = _c(a, np.float32)
+    b = v.shape[0]
+    n = v.size // b
+    out = np.empty(v.shape, np.float64)
+    lib().orc_norm_monolithic_f64(_ptr(v), _ptr(out), n, b, 0)
+    return out
+
+
+def norm_ft_monolithic(a):
+    """d_NormFTMonolithic: a is (batch, ...) complex; each item divided by its RMS magnitude."""
+    v = np.ascontiguousarray(a, np.complex64).view(np.float32)
+    b = v.shape[0]
+    n = v.size // (2 * b)
+    out = np.empty(v.shape, np.float64)
+    lib().orc_norm_monolithic_f64(_ptr(v), _ptr(out), n, b, 1)
+    return out.view(np.complex128)
 
 
 def norm_mean01std(a):

@@ -1,0 +1,37 @@
+"""d_NormMonolithic and d_NormFTMonolithic on the GPU against the FP64 oracle:
+within 2e-6 of the output scale (float data, float mean and scale), with the
+zero output for a constant item bit-exact."""
+import numpy as np
+import pytest
+
+from tests._util import to_np
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape", [(3, 100), (2, 64, 64, 64), (1, 256, 256, 256)])
+def test_norm_monolithic_matches_oracle(gx, orc, cuda, shape):
+    import torch
+    rng = np.random.default_rng(len(shape))
+    x = (rng.standard_normal(shape) * 3.0 + 1.0).astype(np.float32)
+    got = to_np(gx.d_NormMonolithic(torch.from_numpy(x).to(cuda), batch=shape[0]))
+    ref = orc.norm_monolithic(x)
+    assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+def test_norm_monolithic_constant_item_is_zero(gx, cuda):
+    import torch
+    x = torch.full((2, 5000), 1.5, device=cuda)
+    x[1] = torch.arange(5000, device=cuda, dtype=torch.float32)
+    y = gx.d_NormMonolithic(x, batch=2)
+    assert bool((y[0] == 0).all()) and float(y[1].std(unbiased=False)) == pytest.approx(1.0, rel=1e-5)
+
+
+@pytest.mark.parametrize("shape", [(4, 33), (2, 256, 256, 129)])
+def test_norm_ft_monolithic_matches_oracle(gx, orc, cuda, shape):
+    import torch
+    rng = np.random.default_rng(shape[-1])
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64) * 7.0
+    got = to_np(gx.d_NormFTMonolithic(torch.from_numpy(x).to(cuda), batch=shape[0]))
+    ref = orc.norm_ft_monolithic(x)
+    assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
