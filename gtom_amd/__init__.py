@@ -96,6 +96,7 @@ _SIGS = {
     "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
     "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
     "gtomx_fft_full_pad": [_P, _P, int3, int3, _U, _P],
+    "gtomx_fft_resize": [_I, _I, _P, _P, int3, int3, _U, _P],
     "gtomx_peak": [_P, _P, _P, int3, _I, _U, _P],
     "gtomx_peak_one_2d": [_P, _P, _P, int3, int3, _I, _U, _P],
     "gtomx_subpixel_max": [_P, _P, int3, _I, _P],
@@ -615,6 +616,27 @@ def d_FFTFullPad(x, newshape, stream=None):
     out = torch.empty(tuple(x.shape[:-3]) + (nz, ny, nx), dtype=torch.float32, device=x.device)
     check(lib().gtomx_fft_full_pad(_ptr(x), _ptr(out), int3(ox, oy, oz), int3(nx, ny, nz), batch, _stream(stream)),
           "gtomx_fft_full_pad")
+    return out
+
+
+_FFT_RESIZE = {"crop": 1, "pad": 2, "full_crop": 3, "full_pad": 4}
+
+
+def d_FFTResize(op, x, olddims, newdims, stream=None):
+    """d_FFTCrop ("crop"), d_FFTPad ("pad") on half spectra (..., z, y, x/2+1), or
+    d_FFTFullCrop ("full_crop"), d_FFTFullPad ("full_pad") on full arrays (..., z, y, x),
+    float32 or complex64 CUDA tensors; dims are (x, y, z) (src/FFT/FFTResize.cu)."""
+    import torch
+    k = _FFT_RESIZE[op]
+    half = k <= 2
+    ox, oy, oz = olddims
+    nx, ny, nz = newdims
+    per = (ox // 2 + 1 if half else ox) * oy * oz
+    batch = max(1, x.numel() // per)
+    shape = (nz, ny, nx // 2 + 1 if half else nx)
+    out = torch.empty((batch,) + shape, dtype=x.dtype, device=x.device)
+    check(lib().gtomx_fft_resize(k, int(x.is_complex()), _ptr(x), _ptr(out), int3(ox, oy, oz), int3(nx, ny, nz),
+                                 batch, _stream(stream)), "gtomx_fft_resize")
     return out
 
 

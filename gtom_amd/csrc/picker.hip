@@ -25,41 +25,6 @@ namespace gx {
 namespace {
 
 // ---------------------------------------------------------------------------
-// d_FFTFullPad (real): output x has centred coordinate (x + new/2) % new,
-// shifted into the input's centred coordinates by
-// (old - new - ((old & (1 - (new & 1))) % 2)) / 2 (the kernel's
-// `old & 1 - (new & 1)` is old & (1 - (new & 1)) by C precedence), read at
-// (c + (old+1)/2) % old when inside [0, old), else 0.
-// ---------------------------------------------------------------------------
-__host__ __device__ inline int pad_shift(int o, int n) { return (o - n - ((o & (1 - (n & 1))) % 2)) / 2; }
-
-__global__ __launch_bounds__(256) void k_full_pad(const float* __restrict__ in, float* __restrict__ out, int3 od,
-                                                  int3 nd, int3 sh, size_t vnew, size_t vold, unsigned batch) {
-  const size_t total = vnew * batch;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-    const size_t b = i / vnew, r = i % vnew;
-    const int x = (int)(r % nd.x), y = (int)((r / nd.x) % nd.y), z = (int)(r / ((size_t)nd.x * nd.y));
-    const int rx = (x + nd.x / 2) % nd.x + sh.x, ry = (y + nd.y / 2) % nd.y + sh.y, rz = (z + nd.z / 2) % nd.z + sh.z;
-    float v = 0.f;
-    if (rx >= 0 && rx < od.x && ry >= 0 && ry < od.y && rz >= 0 && rz < od.z) {
-      const int ix = (rx + (od.x + 1) / 2) % od.x, iy = (ry + (od.y + 1) / 2) % od.y, iz = (rz + (od.z + 1) / 2) % od.z;
-      v = in[b * vold + ((size_t)iz * od.y + iy) * od.x + ix];
-    }
-    out[i] = v;
-  }
-}
-
-int full_pad(const float* in, float* out, gtomx_int3 od, gtomx_int3 nd, unsigned batch, hipStream_t s) {
-  const size_t vnew = elements(nd), vold = elements(od);
-  const int3 sh = make_int3(pad_shift(od.x, nd.x), pad_shift(od.y, nd.y), pad_shift(od.z, nd.z));
-  const unsigned grid = (unsigned)std::min<size_t>((vnew * batch + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_full_pad, dim3(grid), dim3(256), 0, s, in, out, make_int3(od.x, od.y, od.z),
-                     make_int3(nd.x, nd.y, nd.z), sh, vnew, vold, batch);
-  GX_LAUNCHED("k_full_pad");
-  return GTOMX_OK;
-}
-
-// ---------------------------------------------------------------------------
 // mean0/std1 per volume: two passes over the data for the mean and the
 // population variance (as d_Dev), each reduced in double by NB blocks per
 // volume and finished by one block per volume; then the normalisation.
@@ -224,24 +189,6 @@ using namespace gx;
 
 extern "C" {
 
-int gtomx_fft_full_pad(const float* d_input, float* d_output, gtomx_int3 olddims, gtomx_int3 newdims,
-                       unsigned batch, void* stream) {
-  if (!d_input || !d_output) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
-  if (olddims.x < 1 || olddims.y < 1 || olddims.z < 1 || newdims.x < 1 || newdims.y < 1 || newdims.z < 1)
-    return fail(GTOMX_ERR_INVALID_ARGUMENT, "dims must be positive");
-  if (batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be positive");
-  hipStream_t s = as_stream(stream);
-  if (d_input != d_output) return full_pad(d_input, d_output, olddims, newdims, batch, s);
-  // in place, as the reference allows: pad into a temporary, then copy back
-  DevBuf tmp;
-  const size_t bytes = elements(newdims) * batch * sizeof(float);
-  GX_TRY(tmp.alloc(bytes));
-  GX_TRY(full_pad(d_input, static_cast<float*>(tmp.p), olddims, newdims, batch, s));
-  GX_HIP(hipMemcpyAsync(d_output, tmp.p, bytes, hipMemcpyDeviceToDevice, s));
-  GX_HIP(hipStreamSynchronize(s));
-  return GTOMX_OK;
-}
-
 int gtomx_norm_mean01std(const float* d_input, float* d_output, size_t elements_, unsigned batch, void* stream) {
   if (!d_input || !d_output) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (elements_ == 0 || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "elements and batch must be positive");
@@ -366,7 +313,7 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
                                             reinterpret_cast<gtomx_complex*>(pk->proj), pk->Hr, nb, stream));
     GX_TRY(fft_c2r(pk->proj, pk->proj, pk->realref, dr, nb, 1.0f / (float)pk->Vr, s));
     // FullPad, mean0/std1, R2C   (:225-227)
-    GX_TRY(full_pad(pk->realref, pk->padded, dr, pk->di, nb, s));
+    GX_TRY(fft_resize(FFT_FULL_PAD, false, pk->realref, pk->padded, dr, pk->di, nb, s));
     GX_TRY(norm_mean01std(pk->padded, pk->padded, pk->V, nb, pk->nscr, s));
     GX_TRY(fft_r2c(pk->padded, pk->pft, pk->di, nb, s));
     // C = C2R_u(imageft * conj(P))   (:228-229)

@@ -95,6 +95,12 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 dims, unsigne
 // In-place inverse along z then y of half spectra; used by fft_c2r.
 int fft_c2r_inplace_yz(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
 
+// Fourier-space resizing (resize.hip): op is one of the GTOMX_FFT_* values.
+constexpr int FFT_CROP = GTOMX_FFT_CROP, FFT_PAD = GTOMX_FFT_PAD, FFT_FULL_CROP = GTOMX_FFT_FULL_CROP,
+              FFT_FULL_PAD = GTOMX_FFT_FULL_PAD;
+int fft_resize(int op, bool complex_values, const void* in, void* out, gtomx_int3 od, gtomx_int3 nd, unsigned batch,
+               hipStream_t s);
+
 // Element-wise helpers (ops.hip).
 int fill_f32(float* d, size_t n, float v, hipStream_t s);
 int fill_i32(int32_t* d, size_t n, int32_t v, hipStream_t s);

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "gtomx.h"
@@ -344,6 +345,30 @@ inline void d_FFTFullPad(tfloat* d_input, tfloat* d_output, int3 olddims, int3 n
                                    nullptr),
                 "d_FFTFullPad");
   detail::sync();
+}
+
+// d_FFTCrop / d_FFTPad (half spectra) and d_FFTFullCrop / d_FFTFullPad (full
+// arrays), include/FFT.cuh:294-297, for tfloat and tcomplex.
+namespace detail {
+template <class T> inline void fft_resize(int op, T* in, T* out, int3 od, int3 nd, int batch, const char* name) {
+  static_assert(std::is_same<T, tfloat>::value || std::is_same<T, tcomplex>::value, "tfloat or tcomplex");
+  check(gtomx_fft_resize(op, std::is_same<T, tcomplex>::value ? 1 : 0, in, out, i3(od), i3(nd), (unsigned)batch,
+                         nullptr),
+        name);
+  sync();
+}
+}  // namespace detail
+template <class T> inline void d_FFTCrop(T* d_input, T* d_output, int3 olddims, int3 newdims, int batch = 1) {
+  detail::fft_resize(GTOMX_FFT_CROP, d_input, d_output, olddims, newdims, batch, "d_FFTCrop");
+}
+template <class T> inline void d_FFTPad(T* d_input, T* d_output, int3 olddims, int3 newdims, int batch = 1) {
+  detail::fft_resize(GTOMX_FFT_PAD, d_input, d_output, olddims, newdims, batch, "d_FFTPad");
+}
+template <class T> inline void d_FFTFullCrop(T* d_input, T* d_output, int3 olddims, int3 newdims, int batch = 1) {
+  detail::fft_resize(GTOMX_FFT_FULL_CROP, d_input, d_output, olddims, newdims, batch, "d_FFTFullCrop");
+}
+inline void d_FFTFullPad(tcomplex* d_input, tcomplex* d_output, int3 olddims, int3 newdims, int batch = 1) {
+  detail::fft_resize(GTOMX_FFT_FULL_PAD, d_input, d_output, olddims, newdims, batch, "d_FFTFullPad");
 }
 
 // T_NORM_MODE, include/ImageManipulation.cuh:37-47.

@@ -260,6 +260,20 @@ int gtomx_pick_subtomograms_diff2(const gtomx_complex* d_projectorft, float proj
 /* The Picker class (whole-volume matcher)                                   */
 /* ------------------------------------------------------------------------ */
 
+/* Fourier-space resizing, include/FFT.cuh:294-297 (src/FFT/FFTResize.cu):
+ * GTOMX_FFT_CROP = d_FFTCrop and GTOMX_FFT_PAD = d_FFTPad on half spectra
+ * ((n/2+1) x n x n layout), GTOMX_FFT_FULL_CROP = d_FFTFullCrop and
+ * GTOMX_FFT_FULL_PAD = d_FFTFullPad on full arrays in FFT order. */
+enum { GTOMX_FFT_CROP = 1, GTOMX_FFT_PAD = 2, GTOMX_FFT_FULL_CROP = 3, GTOMX_FFT_FULL_PAD = 4 };
+
+/* One of the four resizes on `batch` real (complex_values = 0) or interleaved
+ * complex arrays, index maps as the reference's kernels (FFTCropKernel :121-138,
+ * FFTPadEvenKernel :162-205, FFTFullCropKernel :140-160, FFTFullPadEvenKernel
+ * :212-238); crops need newdims <= olddims. In place (d_input == d_output,
+ * buffer sized for the larger dims) is allowed and then synchronises the stream. */
+int gtomx_fft_resize(int op, int complex_values, const void* d_input, void* d_output, gtomx_int3 olddims,
+                     gtomx_int3 newdims, unsigned batch, void* stream);
+
 /* d_FFTFullPad on real volumes, include/FFT.cuh:297 / src/FFT/FFTResize.cu:92-110
  * (kernel :212-238): pads `batch` volumes in FFT order (origin at index 0)
  * from olddims to newdims around the origin, zero elsewhere. In place

@@ -648,6 +648,41 @@ double ctf_at(const CtfFull& q, double k, double angle, double gammacorr, bool a
   return v * q.scale;
 }
 
+// Fourier-space resizing (src/FFT/FFTResize.cu), one array of ncomp doubles per
+// element: op 1 FFTCropKernel (:121-138), 2 FFTPadEvenKernel (:162-205), 3
+// FFTFullCropKernel (:140-160), 4 FFTFullPadEvenKernel (:212-238). Half
+// spectra (ops 1, 2) have (n/2+1) x n x n elements, full arrays n^3.
+void fft_resize(int op, const double* in, double* out, const int* od, const int* nd, int ncomp) {
+  const bool half = op <= 2;
+  const int oxs = half ? od[0] / 2 + 1 : od[0], nxs = half ? nd[0] / 2 + 1 : nd[0];
+  for (int z = 0; z < nd[2]; ++z)
+    for (int y = 0; y < nd[1]; ++y)
+      for (int x = 0; x < nxs; ++x) {
+        const int p[3] = {x, y, z};
+        int q[3];
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+          const int o = od[a], n = nd[a], i = p[a];
+          if (op == 1) {
+            q[a] = (a == 0) ? i : (i < n / 2 + 1 ? i : i - n + o);
+          } else if (op == 3) {
+            q[a] = i >= n / 2 ? i + o - n : i;
+          } else if (op == 2 && a == 0) {
+            ok = ok && i < (o + 1) / 2;
+            q[a] = i;
+          } else {
+            const int shift = op == 2 ? (o - n) / 2 : (o - n - ((o & (1 - (n & 1))) % 2)) / 2;
+            const int c = (i + n / 2) % n + shift;
+            ok = ok && c >= 0 && c < o;
+            q[a] = (c + (o + 1) / 2) % o;
+          }
+        }
+        double* dst = out + (((size_t)z * nd[1] + y) * nxs + x) * ncomp;
+        for (int k = 0; k < ncomp; ++k)
+          dst[k] = ok ? in[(((size_t)q[2] * od[1] + q[1]) * oxs + q[0]) * ncomp + k] : 0.0;
+      }
+}
+
 // d_NormMonolithic (src/ImageManipulation/Norm.cu:76-91, NormMeanStdDevMonoKernel
 // :242-284): (x - mean) / std, std = sqrt(N sum x^2 - (sum x)^2) / N in double, 0
 // where std is not positive. In double throughout (the reference rounds mean
@@ -1266,6 +1301,10 @@ double orc_pick_f32(const float* projft, int p, double oversample, const float* 
                     int32_t* idx_out, float* top1_out, float* top2_out, float* sigma_out) {
   return pick<float>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+void orc_fft_resize_f64(int op, const double* in, double* out, const int* od, const int* nd, int ncomp) {
+  fft_resize(op, in, out, od, nd, ncomp);
 }
 
 void orc_norm_monolithic_f64(const float* in, double* out, long long n, int batch, int complex_values) {

@@ -50,6 +50,7 @@ def lib():
         L.orc_ctf_tilt_zgrid_f64.argtypes = [_p, _d, _d, _i, _i, _i, _i, _p, _i, _p]
         L.orc_rotate3d_ft_f64.argtypes = [_p, _p, _i, _p, _i, _i]
         L.orc_norm_monolithic_f64.argtypes = [_p, _p, _ll, _i, _i]
+        L.orc_fft_resize_f64.argtypes = [_i, _p, _p, _p, _p, _i]
         L.orc_peak_one_2d_f64.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
@@ -332,6 +333,24 @@ def full_pad(vol, newshape):
     out = np.empty(newshape, np.float64)
     lib().orc_full_pad_f64(_ptr(v), _ptr(out), ox, oy, oz, nx, ny, nz)
     return out
+
+
+FFT_RESIZE = {"crop": 1, "pad": 2, "full_crop": 3, "full_pad": 4}
+
+
+def fft_resize(op, a, olddims, newdims):
+    """d_FFTCrop/d_FFTPad (half spectra, a of shape (z, y, x/2+1)) or d_FFTFullCrop/
+    d_FFTFullPad (full arrays (z, y, x)); dims are (x, y, z). Real or complex."""
+    cplx = np.iscomplexobj(a)
+    src = np.ascontiguousarray(a, np.complex128).view(np.float64) if cplx else _c(a, np.float64)
+    k = FFT_RESIZE[op]
+    nxs = newdims[0] // 2 + 1 if k <= 2 else newdims[0]
+    shape = (newdims[2], newdims[1], nxs)
+    out = np.empty(shape + ((2,) if cplx else ()), np.float64)
+    od = np.array(olddims, np.int32)
+    nd = np.array(newdims, np.int32)
+    lib().orc_fft_resize_f64(k, _ptr(src), _ptr(out), _ptr(od), _ptr(nd), 2 if cplx else 1)
+    return out.view(np.complex128).reshape(shape) if cplx else out
 
 
 def norm_monolithic(a):
