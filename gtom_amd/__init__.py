@@ -115,6 +115,9 @@ _SIGS = {
     "gtomx_mrc_to_device_f32": [_P, _I, _SZ, _P, _P],
     "gtomx_norm_mean01std": [_P, _P, _SZ, _U, _P],
     "gtomx_norm_monolithic": [_P, _P, _SZ, _U, _P],
+    "gtomx_sum_f32": [_P, _P, _SZ, _U, _P],
+    "gtomx_sum_f64": [_P, _P, _SZ, _U, _P],
+    "gtomx_sum_i32": [_P, _P, _SZ, _U, _P],
     "gtomx_norm_ft_monolithic": [_P, _P, _SZ, _U, _P],
     "gtomx_picker_create": [ctypes.POINTER(_P), _P, int3, _P, int3, _U, _P],
     "gtomx_picker_destroy": [_P],
@@ -473,6 +476,19 @@ def d_SubpixelMax(vol, out, subpixsteps, stream=None):
     nz = vol.shape[-3] if vol.dim() == 3 else 1
     check(lib().gtomx_subpixel_max(_ptr(vol), _ptr(out), int3(nx, ny, nz), int(subpixsteps), _stream(stream)),
           "gtomx_subpixel_max")
+    return out
+
+
+def d_Sum(x, batch=1, stream=None):
+    """d_Sum (src/Generics/Sum.cu:99-139) of `batch` equal parts of a float32, float64
+    or int32 CUDA tensor; returns a tensor of `batch` sums of the same dtype."""
+    import torch
+    fn = {torch.float32: lib().gtomx_sum_f32, torch.float64: lib().gtomx_sum_f64,
+          torch.int32: lib().gtomx_sum_i32}.get(x.dtype)
+    if fn is None:
+        raise TypeError("d_Sum takes float32, float64 or int32")
+    out = torch.zeros(batch, dtype=x.dtype, device=x.device)
+    check(fn(_ptr(x), _ptr(out), x.numel() // batch, batch, _stream(stream)), "gtomx_sum")
     return out
 
 

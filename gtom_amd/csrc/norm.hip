@@ -1,4 +1,7 @@
-// Per-volume normalisation of a batch:
+// Per-item reductions and normalisation of a batch:
+//  * d_Sum (src/Generics/Sum.cu:99-139) for float, double and int: float and
+//    double accumulated in double, int in 64 bits and truncated to 32 (the
+//    reference's int sum wraps the same way);
 //  * d_NormMonolithic (src/ImageManipulation/Norm.cu:76-91, NormMeanStdDevMonoKernel
 //    :242-284): y = (x - mean) / std with sums of x and x^2 in double, std =
 //    sqrt(N sum x^2 - (sum x)^2) / N, and y = 0 where that is not positive; the
@@ -11,6 +14,7 @@
 // they are applied, as the reference stores them.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 
 #include "runtime.hpp"
 
@@ -99,6 +103,53 @@ __global__ __launch_bounds__(256) void k_mono_apply(const float* __restrict__ x,
   }
 }
 
+template <class T> struct SumAcc { using type = double; };
+template <> struct SumAcc<int32_t> { using type = long long; };
+
+// part[b][p]: the sum of this workgroup's share of item b.
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_partial(const T* __restrict__ x, size_t n,
+                                                     typename SumAcc<T>::type* __restrict__ part) {
+  using A = typename SumAcc<T>::type;
+  __shared__ A red[4];
+  const size_t b = blockIdx.y;
+  A acc = 0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) acc += (A)x[b * n + i];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_sum_finish(const typename SumAcc<T>::type* __restrict__ part, int nparts,
+                                                   T* __restrict__ out) {
+  using A = typename SumAcc<T>::type;
+  const size_t b = blockIdx.x;
+  A acc = 0;
+  for (int i = threadIdx.x; i < nparts; i += 64) acc += part[b * nparts + i];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  if (threadIdx.x == 0) out[b] = (T)acc;
+}
+
+template <class T> int sum(const T* in, T* out, size_t n, unsigned batch, void* stream) {
+  using A = typename SumAcc<T>::type;
+  if (!in || !out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
+  if (batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be positive");
+  if (batch > 65535) return fail(GTOMX_ERR_UNSUPPORTED, "batch above 65535");
+  if (n == 0) return ok();  // the reference returns without writing
+  hipStream_t s = as_stream(stream);
+  const int nb = (int)std::min<size_t>(NB, (n + 255) / 256);
+  DevBuf part;
+  GX_TRY(part.alloc((size_t)nb * batch * sizeof(A)));
+  hipLaunchKernelGGL(k_sum_partial<T>, dim3(nb, batch), dim3(256), 0, s, in, n, part.as<A>());
+  GX_LAUNCHED("k_sum_partial");
+  hipLaunchKernelGGL(k_sum_finish<T>, dim3(batch), dim3(64), 0, s, part.as<A>(), nb, out);
+  GX_LAUNCHED("k_sum_finish");
+  GX_HIP(hipStreamSynchronize(s));  // the scratch is freed on return
+  return ok();
+}
+
 template <bool COMPLEX> int mono(const float* in, float* out, size_t n, unsigned batch, void* stream) {
   if (!in || !out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (n == 0 || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "elements and batch must be positive");
@@ -134,4 +185,14 @@ extern "C" int gtomx_norm_ft_monolithic(const gtomx_complex* d_input, gtomx_comp
                                         unsigned batch, void* stream) {
   return mono<true>(reinterpret_cast<const float*>(d_input), reinterpret_cast<float*>(d_output), elements, batch,
                     stream);
+}
+
+extern "C" int gtomx_sum_f32(const float* d_input, float* d_output, size_t n, unsigned batch, void* stream) {
+  return sum<float>(d_input, d_output, n, batch, stream);
+}
+extern "C" int gtomx_sum_f64(const double* d_input, double* d_output, size_t n, unsigned batch, void* stream) {
+  return sum<double>(d_input, d_output, n, batch, stream);
+}
+extern "C" int gtomx_sum_i32(const int32_t* d_input, int32_t* d_output, size_t n, unsigned batch, void* stream) {
+  return sum<int32_t>(d_input, d_output, n, batch, stream);
 }

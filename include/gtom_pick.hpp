@@ -371,6 +371,18 @@ inline void d_FFTFullPad(tcomplex* d_input, tcomplex* d_output, int3 olddims, in
   detail::fft_resize(GTOMX_FFT_FULL_PAD, d_input, d_output, olddims, newdims, batch, "d_FFTFullPad");
 }
 
+// d_Sum, include/Generics.cuh:585, for float, double and int.
+inline int sum_dispatch(const float* in, float* out, size_t n, unsigned b) { return gtomx_sum_f32(in, out, n, b, nullptr); }
+inline int sum_dispatch(const double* in, double* out, size_t n, unsigned b) {
+  return gtomx_sum_f64(in, out, n, b, nullptr);
+}
+inline int sum_dispatch(const int* in, int* out, size_t n, unsigned b) {
+  return gtomx_sum_i32(reinterpret_cast<const int32_t*>(in), reinterpret_cast<int32_t*>(out), n, b, nullptr);
+}
+template <class T> inline void d_Sum(T* d_input, T* d_output, size_t n, int batch = 1) {
+  detail::check(sum_dispatch(d_input, d_output, n, (unsigned)batch), "d_Sum");
+}
+
 // T_NORM_MODE, include/ImageManipulation.cuh:37-47.
 enum T_NORM_MODE {
   T_NORM_NONE = 0, T_NORM_MEAN01STD = 1, T_NORM_PHASE = 2, T_NORM_STD1 = 3, T_NORM_STD2 = 4, T_NORM_STD3 = 5,

@@ -282,6 +282,14 @@ int gtomx_fft_resize(int op, int complex_values, const void* d_input, void* d_ou
 int gtomx_fft_full_pad(const float* d_input, float* d_output, gtomx_int3 olddims, gtomx_int3 newdims,
                        unsigned batch, void* stream);
 
+/* d_Sum, include/Generics.cuh:585 / src/Generics/Sum.cu:99-139: the sum of each
+ * of `batch` arrays of n elements into d_output[b]; float and double summed in
+ * double, int in 64 bits and truncated to 32 (as the reference's int sum wraps).
+ * n = 0 writes nothing. Synchronises the stream. */
+int gtomx_sum_f32(const float* d_input, float* d_output, size_t n, unsigned batch, void* stream);
+int gtomx_sum_f64(const double* d_input, double* d_output, size_t n, unsigned batch, void* stream);
+int gtomx_sum_i32(const int32_t* d_input, int32_t* d_output, size_t n, unsigned batch, void* stream);
+
 /* d_NormMonolithic, include/ImageManipulation.cuh:49 / src/ImageManipulation/Norm.cu:76-91
  * (kernel :242-284): each of `batch` arrays of `elements` floats becomes
  * (x - mean) / std with sums of x and x^2 in double, std = sqrt(N sum x^2 -
