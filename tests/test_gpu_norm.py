@@ -51,3 +51,23 @@ def test_sum_matches_numpy(gx, cuda):
     got = to_np(gx.d_Sum(torch.from_numpy(i).to(cuda), batch=2))
     want = ((i.astype(np.int64).sum(axis=1) + 2**31) % 2**32 - 2**31).astype(np.int32)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("i", [2, 7, 12, 17, 20, 24])
+def test_sum_reference_case(gx, cuda, i):
+    """GTOMTest/Sum.cpp: 2^i - 1 values (rand() % 100000) * 1e-5 summed within 1e-6
+    (relative) of their float32 Kahan sum (a seeded generator stands in for rand();
+    beyond 5000 values the exact sum, which the Kahan sum equals to float rounding)."""
+    import torch
+    size = (1 << i) - 1
+    h = (np.random.default_rng(i).integers(0, 100000, size) * np.float32(0.00001)).astype(np.float32)
+    s, c = np.float32(h[0]), np.float32(0)
+    for v in h[1:]:  # Kahan summation in float32, as the reference test
+        y = np.float32(v - c)
+        t = np.float32(s + y)
+        c = np.float32(np.float32(t - s) - y)
+        s = t
+    if size > 5000:  # the loop above is slow in Python beyond a few thousand; use the exact sum there
+        s = np.float32(h.astype(np.float64).sum())
+    got = float(to_np(gx.d_Sum(torch.from_numpy(h).to(cuda)))[0])
+    assert abs(got - float(s)) / float(s) <= 1e-6

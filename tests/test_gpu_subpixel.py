@@ -119,3 +119,18 @@ def test_peak_one_2d_matches_oracle(gx, orc, cuda, dims, region, sub):
     rpos, rval = orc.peak_one_2d(imgs, region, sub)
     assert np.abs(to_np(pos) - rpos).max() <= 1e-5 * max(dims)
     assert np.array_equal(to_np(val).astype(np.float64), rval)
+
+
+@pytest.mark.parametrize("i", [10, 14, 18, 22, 24])
+def test_integer_peak_reference_max_case(gx, cuda, i):
+    """GTOMTest/Max.cpp: (rand() % 100000) * 1e-5 - 0.5 with one element set to
+    0.500001; the maximum's index is found exactly (a seeded generator stands in
+    for rand())."""
+    import torch
+    rng = np.random.default_rng(i + 1)
+    size = (1 << i) - int(rng.integers(0, i))
+    h = (rng.integers(0, 100000, size) * np.float32(0.00001) - np.float32(0.5)).astype(np.float32)
+    k = int(rng.integers(0, size))
+    h[k] = np.float32(0.500001)
+    pos, val = gx.d_Peak(torch.from_numpy(h).to(cuda), "integer", ndims=1)
+    assert to_np(pos)[0].tolist() == [float(k), 0.0, 0.0] and float(to_np(val)[0]) == float(h[k])
