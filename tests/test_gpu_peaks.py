@@ -4,6 +4,8 @@ element for element and in order."""
 import numpy as np
 import pytest
 
+from tests._util import to_np
+
 pytestmark = pytest.mark.gpu
 
 
@@ -64,3 +66,24 @@ def test_search_output_peaks_at_planted_copies(gx, orc, cuda):
     got = gx.d_LocalPeaks(best, 4, float(best.max().item()) * 0.5)[0]
     ref = orc.local_peaks(best.cpu().numpy()[0], 4, float(best.max().item()) * 0.5)
     assert np.array_equal(got, ref)
+
+
+def test_reference_scenario_two_upscaled_deltas(gx, orc, cuda):
+    """GTOMTest/LocalPeaks.cpp (which asserts nothing): deltas at (1, 1) and (3, 6) of
+    an 8 x 8 image, Fourier-upscaled to 1769 x 1769 (d_Scale: R2C, d_FFTPad, C2R / 64;
+    the 1769-point inverse here with numpy), mean0/std1, local peaks with extent 100
+    and threshold 4.6: exactly the two deltas, within 2 voxels of their upscaled
+    positions, the same list as the oracle."""
+    import torch
+    img = np.zeros((8, 8))
+    img[1, 1] = img[6, 3] = 1.0
+    half = np.fft.rfftn(img).astype(np.complex64)
+    padded = to_np(gx.d_FFTResize("pad", torch.from_numpy(half[None]).to(cuda), (8, 8, 1), (1769, 1769, 1)))[0, 0]
+    up = (np.fft.irfftn(padded.astype(np.complex128), s=(1769, 1769), axes=(0, 1)) * (1769 * 1769) / 64.0)
+    up = up.astype(np.float32)
+    normed = gx.d_NormMean01Std(torch.from_numpy(up).to(cuda))
+    got = gx.d_LocalPeaks(normed.reshape(1, 1769, 1769), 100, 4.6)[0]
+    ref = orc.local_peaks(orc.norm_mean01std(up).astype(np.float32).reshape(1, 1769, 1769), 100, 4.6)
+    assert np.array_equal(got, ref)
+    want = np.array([[1 * 1769 / 8, 1 * 1769 / 8], [3 * 1769 / 8, 6 * 1769 / 8]])
+    assert len(got) == 2 and np.abs(got[:, :2] - want).max() <= 2
