@@ -8,8 +8,9 @@
 //     Q = E_v * conj(R) with DC = 0            (src/Correlation/SubTomograms.cu:127-136)
 //     and the partial sums of |R|^2; inverse DFT along z, written to the
 //     batch buffer G[o].
-//  k_pick_herm_norm: the correction of those sums on the x = 0 and x = n/2
-//     planes, so that together they are sum |R'|^2 for R' = Herm(R) with DC
+//  k_pick_herm_norm: the correction of those sums on the x = 0 plane (the
+//     x = n/2 plane meets the sphere only at its centre bin), so that together
+//     they are sum |R'|^2 for R' = Herm(R) with DC
 //     zeroed: the Fourier form of C2R -> mean0/std1 -> R2C -> /RMS
 //     (src/Correlation/SubTomograms.cu:66-70, src/ImageManipulation/Norm.cu:242-284, 476-509).
 //  pass 2 (k_pick_y): inverse DFT along y in place; one workgroup per
