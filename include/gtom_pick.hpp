@@ -211,6 +211,17 @@ inline void d_Rotate3DFT(tfloat* d_volume, tfloat* d_output, int3 dims, tfloat3*
                 "d_Rotate3DFT");
 }
 
+// struct Peak, include/Correlation.cuh:127-134: a picked particle, filled by
+// the caller from the search output (position from d_LocalPeaks / d_Peak,
+// angles from the orientation index, fom from the score map).
+struct Peak {
+  tfloat3 position;
+  uint ref;
+  tfloat3 angles;
+  tfloat fom;
+  tfloat relativefom;
+};
+
 // T_PEAK_MODE, include/Correlation.cuh:44-52.
 enum T_PEAK_MODE { T_PEAK_INTEGER = 1, T_PEAK_SUBCOARSE = 2, T_PEAK_SUBFINE = 3 };
 
