@@ -97,6 +97,7 @@ _SIGS = {
     "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
     "gtomx_fft_full_pad": [_P, _P, int3, int3, _U, _P],
     "gtomx_fft_resize": [_I, _I, _P, _P, int3, int3, _U, _P],
+    "gtomx_sphere_mask_ex": [_P, _P, int3, _F, _F, _P, _I, _I, _U, _P],
     "gtomx_peak": [_P, _P, _P, int3, _I, _U, _P],
     "gtomx_peak_one_2d": [_P, _P, _P, int3, int3, _I, _U, _P],
     "gtomx_subpixel_max": [_P, _P, int3, _I, _P],
@@ -653,6 +654,23 @@ def d_FFTResize(op, x, olddims, newdims, stream=None):
     out = torch.empty((batch,) + shape, dtype=x.dtype, device=x.device)
     check(lib().gtomx_fft_resize(k, int(x.is_complex()), _ptr(x), _ptr(out), int3(ox, oy, oz), int3(nx, ny, nz),
                                  batch, _stream(stream)), "gtomx_fft_resize")
+    return out
+
+
+def d_SphereMask(x, radius, sigma, center=None, decentered=False, ndims=3, stream=None):
+    """d_SphereMask (src/Masking/SphereMask.cu:18-106) of float32 or complex64 CUDA
+    tensors (..., z, y, x) (ndims=3) or (..., y, x); center (x, y, z) or None; returns a
+    new tensor."""
+    import torch
+    shape = x.shape[-ndims:]
+    nx, ny = shape[-1], shape[-2]
+    nz = shape[-3] if ndims == 3 else 1
+    batch = max(1, x.numel() // (nx * ny * nz))
+    out = torch.empty_like(x)
+    c = float3(*[float(v) for v in center]) if center is not None else None
+    check(lib().gtomx_sphere_mask_ex(_ptr(x), _ptr(out), int3(nx, ny, nz), float(radius), float(sigma),
+                                     ctypes.byref(c) if c is not None else None, int(bool(decentered)),
+                                     int(x.is_complex()), batch, _stream(stream)), "gtomx_sphere_mask_ex")
     return out
 
 

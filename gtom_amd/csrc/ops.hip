@@ -122,6 +122,31 @@ __global__ void k_sphere_mask(const float* in, float* out, int3 d, double radius
   }
 }
 
+// SphereMaskKernel in full (src/Masking/SphereMask.cu:49-106): per axis the
+// distance is the FFT-order signed index (i < n/2+1 ? i : i - n) when decentered,
+// else (int)(i - c) with the float centre c (C truncation of the difference);
+// z contributes only when n.z > 1. Real or interleaved complex data.
+template <bool COMPLEX>
+__global__ void k_sphere_mask_ex(const float* in, float* out, int3 d, double radius, double sigma, float3 c,
+                                 int decentered, unsigned batch) {
+  const size_t total = (size_t)d.x * d.y * d.z;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total * batch; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i % total;
+    const int x = (int)(r % d.x), y = (int)((r / d.x) % d.y), z = (int)(r / ((size_t)d.x * d.y));
+    auto axis = [&](int v, int n, float cv) { return decentered ? (v < n / 2 + 1 ? v : v - n) : (int)((float)v - cv); };
+    const long long xx = axis(x, d.x, c.x), yy = axis(y, d.y, c.y), zz = d.z > 1 ? axis(z, d.z, c.z) : 0;
+    const double len = sqrt((double)(xx * xx + yy * yy + zz * zz));
+    double m = 1.0;
+    if (!(len < radius)) m = sigma > 0 ? fmax(0.0, (cos(fmin(1.0, (len - radius) / sigma) * 3.14159265358979323846) + 1.0) * 0.5) : 0.0;
+    if constexpr (COMPLEX) {
+      out[2 * i] = in[2 * i] * (float)m;
+      out[2 * i + 1] = in[2 * i + 1] * (float)m;
+    } else {
+      out[i] = in[i] * (float)m;
+    }
+  }
+}
+
 // The soft sphere already moved to the origin: value at index i is the mask at
 // the position r with ifftshift(r) = i (d_SphereMask then d_RemapFull2FullFFT,
 // src/Generics/LocalStd.cu:24-26).
@@ -624,6 +649,27 @@ int gtomx_sphere_mask(const float* d_input, float* d_output, gtomx_int3 dims, fl
   hipLaunchKernelGGL(k_sphere_mask, dim3(grid_for(n)), dim3(TPB), 0, as_stream(stream), d_input, d_output,
                      make_int3(dims.x, dims.y, dims.z), (double)radius, (double)sigma, batch);
   GX_LAUNCHED("k_sphere_mask");
+  return GTOMX_OK;
+}
+
+int gtomx_sphere_mask_ex(const void* d_input, void* d_output, gtomx_int3 dims, float radius, float sigma,
+                         const gtomx_float3* center, int decentered, int complex_values, unsigned batch,
+                         void* stream) {
+  GX_TRY(check_vol(dims));
+  if (!d_input || !d_output || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer or zero batch");
+  const float3 c = center ? make_float3(center->x, center->y, center->z)
+                          : make_float3((float)(dims.x / 2), (float)(dims.y / 2), (float)(dims.z / 2));
+  const size_t n = elements(dims) * batch;
+  const int3 d = make_int3(dims.x, dims.y, dims.z);
+  if (complex_values)
+    hipLaunchKernelGGL(k_sphere_mask_ex<true>, dim3(grid_for(n)), dim3(TPB), 0, as_stream(stream),
+                       static_cast<const float*>(d_input), static_cast<float*>(d_output), d, (double)radius,
+                       (double)sigma, c, decentered ? 1 : 0, batch);
+  else
+    hipLaunchKernelGGL(k_sphere_mask_ex<false>, dim3(grid_for(n)), dim3(TPB), 0, as_stream(stream),
+                       static_cast<const float*>(d_input), static_cast<float*>(d_output), d, (double)radius,
+                       (double)sigma, c, decentered ? 1 : 0, batch);
+  GX_LAUNCHED("k_sphere_mask_ex");
   return GTOMX_OK;
 }
 

@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -124,13 +125,16 @@ inline void d_DivideSafeByVector(tfloat* d_input, tfloat* d_divisors, tfloat* d_
   detail::sync();
 }
 
-// d_SphereMask (real, centred), include/Masking.cuh:20; radius NULL uses min(dims)/2.
-inline void d_SphereMask(tfloat* d_input, tfloat* d_output, int3 size, tfloat* radius, tfloat sigma,
-                         tfloat3* center, bool decentered, int batch = 1) {
-  if (center != nullptr || decentered)
-    throw std::runtime_error("d_SphereMask: only the centred, non-decentered form is built");
+// d_SphereMask, include/Masking.cuh:20, for tfloat and tcomplex: radius NULL
+// uses min(dims)/2 (z only in 3D), center NULL the volume centre (dims/2).
+template <class T>
+inline void d_SphereMask(T* d_input, T* d_output, int3 size, tfloat* radius, tfloat sigma, tfloat3* center,
+                         bool decentered, int batch = 1) {
+  static_assert(std::is_same<T, tfloat>::value || std::is_same<T, tcomplex>::value, "tfloat or tcomplex");
   tfloat r = radius ? *radius : (tfloat)(std::min(std::min(size.x, size.y), size.z > 1 ? size.z : size.x) / 2);
-  detail::check(gtomx_sphere_mask(d_input, d_output, detail::i3(size), r, sigma, (unsigned)batch, nullptr),
+  detail::check(gtomx_sphere_mask_ex(d_input, d_output, detail::i3(size), r, sigma,
+                                     reinterpret_cast<const gtomx_float3*>(center), decentered ? 1 : 0,
+                                     std::is_same<T, tcomplex>::value ? 1 : 0, (unsigned)batch, nullptr),
                 "d_SphereMask");
   detail::sync();
 }

@@ -230,6 +230,16 @@ int gtomx_divide_safe_by_vector(const float* d_input, const float* d_divisors, f
 int gtomx_sphere_mask(const float* d_input, float* d_output, gtomx_int3 dims, float radius,
                       float sigma, unsigned batch, void* stream);
 
+/* d_SphereMask in full, include/Masking.cuh:20 / src/Masking/SphereMask.cu:18-33
+ * (kernel :49-106): center NULL = (dims/2), a float centre is subtracted from
+ * each integer index and the difference truncated to int as in the kernel;
+ * decentered measures FFT-order distances from the origin (index i < n/2+1 ?
+ * i : i - n) and ignores center; z contributes only when dims.z > 1. Real
+ * (complex_values = 0) or interleaved complex data, both parts scaled. */
+int gtomx_sphere_mask_ex(const void* d_input, void* d_output, gtomx_int3 dims, float radius, float sigma,
+                         const gtomx_float3* center, int decentered, int complex_values, unsigned batch,
+                         void* stream);
+
 /* d_LocalStd without a Fourier mask, include/Generics.cuh:782 /
  * src/Generics/LocalStd.cu:10-118: local standard deviation (and, if d_mean
  * is not NULL, mean) of d_map under a soft sphere of localradius. The

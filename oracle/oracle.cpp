@@ -648,6 +648,25 @@ double ctf_at(const CtfFull& q, double k, double angle, double gammacorr, bool a
   return v * q.scale;
 }
 
+// SphereMaskKernel in full (src/Masking/SphereMask.cu:49-106): mask values of a
+// (nx, ny, nz) volume with float centre c (the difference i - c truncated to int)
+// or, decentered, FFT-order distances from the origin; z only when nz > 1.
+void sphere_mask_ex(double* out, int nx, int ny, int nz, double radius, double sigma, const float* c,
+                    bool decentered) {
+  const double pi = 3.14159265358979323846;
+  auto axis = [&](int v, int n, float cv) { return decentered ? (v < n / 2 + 1 ? v : v - n) : (int)((float)v - cv); };
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const long long xx = axis(x, nx, c[0]), yy = axis(y, ny, c[1]), zz = nz > 1 ? axis(z, nz, c[2]) : 0;
+        const double len = std::sqrt((double)(xx * xx + yy * yy + zz * zz));
+        double m = 1.0;
+        if (!(len < radius))
+          m = sigma > 0 ? std::max(0.0, (std::cos(std::min(1.0, (len - radius) / sigma) * pi) + 1.0) * 0.5) : 0.0;
+        out[((size_t)z * ny + y) * nx + x] = m;
+      }
+}
+
 // Fourier-space resizing (src/FFT/FFTResize.cu), one array of ncomp doubles per
 // element: op 1 FFTCropKernel (:121-138), 2 FFTPadEvenKernel (:162-205), 3
 // FFTFullCropKernel (:140-160), 4 FFTFullPadEvenKernel (:212-238). Half
@@ -1301,6 +1320,11 @@ double orc_pick_f32(const float* projft, int p, double oversample, const float* 
                     int32_t* idx_out, float* top1_out, float* top2_out, float* sigma_out) {
   return pick<float>(projft, p, oversample, expft, ctf, ctf_per_volume, n, nvol, angles,
                      nangles, maskradius, best_out, idx_out, top1_out, top2_out, sigma_out);
+}
+
+void orc_sphere_mask_ex_f64(double* out, int nx, int ny, int nz, double radius, double sigma, const float* c,
+                            int decentered) {
+  sphere_mask_ex(out, nx, ny, nz, radius, sigma, c, decentered != 0);
 }
 
 void orc_fft_resize_f64(int op, const double* in, double* out, const int* od, const int* nd, int ncomp) {

@@ -51,6 +51,7 @@ def lib():
         L.orc_rotate3d_ft_f64.argtypes = [_p, _p, _i, _p, _i, _i]
         L.orc_norm_monolithic_f64.argtypes = [_p, _p, _ll, _i, _i]
         L.orc_fft_resize_f64.argtypes = [_i, _p, _p, _p, _p, _i]
+        L.orc_sphere_mask_ex_f64.argtypes = [_p, _i, _i, _i, _d, _d, _p, _i]
         L.orc_peak_one_2d_f64.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
@@ -333,6 +334,15 @@ def full_pad(vol, newshape):
     out = np.empty(newshape, np.float64)
     lib().orc_full_pad_f64(_ptr(v), _ptr(out), ox, oy, oz, nx, ny, nz)
     return out
+
+
+def sphere_mask_ex(shape, radius, sigma, center=None, decentered=False):
+    """d_SphereMask values for a (z, y, x) or (y, x) shape; center (x, y, z) or None = dims/2."""
+    nz, ny, nx = (1,) + tuple(shape) if len(shape) == 2 else tuple(shape)
+    c = np.array(center if center is not None else (nx // 2, ny // 2, nz // 2), np.float32)
+    out = np.empty((nz, ny, nx), np.float64)
+    lib().orc_sphere_mask_ex_f64(_ptr(out), nx, ny, nz, float(radius), float(sigma), _ptr(c), int(decentered))
+    return out.reshape(shape)
 
 
 FFT_RESIZE = {"crop": 1, "pad": 2, "full_crop": 3, "full_pad": 4}

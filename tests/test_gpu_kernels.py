@@ -130,3 +130,17 @@ def test_pad_remap_match_oracle(gx, orc, cuda):
     rm = to_np(gx.d_RemapFull2FullFFT(torch.from_numpy(ref).to(cuda)))
     assert np.array_equal(rm, orc.remap_full2fullfft(ref.astype(np.float64)).astype(np.float32))
     assert np.array_equal(rm, np.fft.ifftshift(ref))
+
+
+@pytest.mark.parametrize("shape,radius,sigma,center,dec,cplx", [
+    ((16, 18, 20), 6.0, 2.0, None, False, False), ((16, 18, 20), 5.5, 0.0, (7.5, 9.25, 6.75), False, True),
+    ((16, 18, 20), 7.0, 3.0, None, True, True), ((1, 24, 30), 9.0, 1.5, (10.5, 12.0, 0.0), False, False)])
+def test_sphere_mask_ex_matches_oracle(gx, orc, cuda, shape, radius, sigma, center, dec, cplx):
+    import torch
+    rng = np.random.default_rng(shape[2])
+    x = rng.standard_normal((2,) + shape).astype(np.float32)
+    if cplx:
+        x = (x + 1j * rng.standard_normal(x.shape)).astype(np.complex64)
+    got = to_np(gx.d_SphereMask(torch.from_numpy(x).to(cuda), radius, sigma, center, dec))
+    m = orc.sphere_mask_ex(shape, radius, sigma, center, dec)
+    assert np.abs(got - x * m[None]).max() <= 1e-6 * np.abs(x).max()

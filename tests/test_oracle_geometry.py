@@ -264,3 +264,36 @@ def test_ctf_tilt_zgrid_specimen_tilt_keeps_the_reference_matrix(orc, sa, st, pa
     want = [d0 - ((x + rx / 2 - imx / 2) * px * n[0] + (y + ry / 2 - imy / 2) * px * n[1]) / n[2] for x, y in pos]
     z = orc.ctf_tilt_zgrid(0.0, (sa, st), (pa, pt), d0, px, (imx, imy), (rx, ry), pos)
     assert np.allclose(z, want, rtol=0, atol=1e-20)
+
+
+def _numpy_sphere(shape, radius, sigma, center=None, decentered=False):
+    nz, ny, nx = shape
+    c = center if center is not None else (nx // 2, ny // 2, nz // 2)
+    def d(n, cv):
+        i = np.arange(n)
+        return np.where(i < n // 2 + 1, i, i - n) if decentered else np.trunc(i - np.float32(cv)).astype(np.int64)
+    zz = d(nz, c[2]) if nz > 1 else np.zeros(nz, np.int64)
+    r = np.sqrt((d(nx, c[0])[None, None, :] ** 2 + d(ny, c[1])[None, :, None] ** 2
+                 + zz[:, None, None] ** 2).astype(np.float64))
+    soft = np.maximum(0, (np.cos(np.minimum(1, (r - radius) / sigma) * np.pi) + 1) / 2) if sigma > 0 else 0 * r
+    return np.where(r < radius, 1.0, soft)
+
+
+@pytest.mark.parametrize("shape,radius,sigma,center,dec", [
+    ((16, 18, 20), 6.0, 2.0, None, False), ((16, 18, 20), 5.5, 0.0, (7.5, 9.25, 6.75), False),
+    ((16, 18, 20), 7.0, 3.0, None, True), ((1, 24, 30), 9.0, 1.5, (10.5, 12.0, 0.0), False)])
+def test_sphere_mask_ex_matches_numpy(orc, shape, radius, sigma, center, dec):
+    got = orc.sphere_mask_ex(shape, radius, sigma, center, dec)
+    assert np.allclose(got, _numpy_sphere(shape, radius, sigma, center, dec), atol=1e-12)
+
+
+def test_sphere_mask_ex_known_answers(orc):
+    shape = (12, 14, 16)
+    # an integer centre moved by (2, -3, 1) rolls a mask that stays clear of the faces
+    small = orc.sphere_mask_ex(shape, 2.5, 1.0)
+    moved = orc.sphere_mask_ex(shape, 2.5, 1.0, center=(16 // 2 + 2, 14 // 2 - 3, 12 // 2 + 1))
+    assert np.array_equal(moved, np.roll(small, (1, -3, 2), axis=(0, 1, 2)))
+    centred = orc.sphere_mask_ex(shape, 4.5, 2.0)
+    # decentered = the centred mask moved to the origin (ifftshift) for even sizes
+    dec = orc.sphere_mask_ex(shape, 4.5, 2.0, decentered=True)
+    assert np.array_equal(dec, np.fft.ifftshift(centred))
