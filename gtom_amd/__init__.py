@@ -93,7 +93,7 @@ _SIGS = {
     "gtomx_update_max_argmax": [_P, _SZ, _U, _I, _P, _P, _P],
     "gtomx_divide_safe_by_vector": [_P, _P, _P, _SZ, _U, _P],
     "gtomx_sphere_mask": [_P, _P, int3, _F, _F, _U, _P],
-    "gtomx_local_std": [_P, int3, _F, _P, _P, _P],
+    "gtomx_local_std": [_P, int3, _P, _F, _P, _P, _P],
     "gtomx_local_peaks": [_P, int3, _I, _F, _U, _P, _SZ, _P, _P],
     "gtomx_fft_full_pad": [_P, _P, int3, int3, _U, _P],
     "gtomx_fft_resize": [_I, _I, _P, _P, int3, int3, _U, _P],
@@ -513,12 +513,18 @@ def d_NormFTMonolithic(x, batch=1, stream=None):
     return out
 
 
-def d_LocalStd(vol, radius, with_mean=False, stream=None):
+def d_LocalStd(vol, radius, with_mean=False, fouriermask=None, stream=None):
+    """d_LocalStd (src/Generics/LocalStd.cu:10-118) of a float32 CUDA volume (nz, ny, nx):
+    sigma, and the mean if asked, under a soft sphere of `radius`; `fouriermask`
+    is None or a float32 CUDA half spectrum (nz, ny, nx//2+1) in FFT order."""
     import torch
     nz, ny, nx = vol.shape[-3:]
+    if fouriermask is not None and tuple(fouriermask.shape[-3:]) != (nz, ny, nx // 2 + 1):
+        raise ValueError("fouriermask must be the half spectrum (nz, ny, nx//2+1)")
     sd = torch.empty_like(vol)
     mean = torch.empty_like(vol) if with_mean else None
-    check(lib().gtomx_local_std(_ptr(vol), int3(nx, ny, nz), radius, _ptr(sd), _ptr(mean), _stream(stream)),
+    check(lib().gtomx_local_std(_ptr(vol), int3(nx, ny, nz), _ptr(fouriermask), radius, _ptr(sd), _ptr(mean),
+                                _stream(stream)),
           "gtomx_local_std")
     return (sd, mean) if with_mean else sd
 

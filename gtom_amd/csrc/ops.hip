@@ -186,6 +186,18 @@ __global__ void k_sum_stage2(const double* partial, int np, double* result) {
   if (threadIdx.x == 0) *result = red[0];
 }
 
+// The mask spectrum times a real half-spectrum Fourier mask (d_ComplexMultiplyByVector,
+// src/Generics/LocalStd.cu:38-41), and the mask sum that the reference takes as
+// the sum of the renormalised C2R of that product (:43-52): that sum is the
+// real part of the product's DC bin, the old sum times fouriermask[0].
+__global__ void k_mask_fourier(float2* mft, const float* fm, size_t n, double* masksum) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float f = fm[i];
+    mft[i] = make_float2(mft[i].x * f, mft[i].y * f);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *masksum *= (double)fm[0];
+}
+
 // sigma = sqrt(max(0, M*conv2 - conv1^2)) / M with conv = C2R_u(...)/N, in double
 // (src/Generics/LocalStd.cu:98-109); optionally fused with DivideSafeByVector
 // of the best correlation (src/Correlation/SubTomograms.cu:117).
@@ -399,11 +411,18 @@ LocalStdScratch local_std_carve(void* base, gtomx_int3 d) {
   return sc;
 }
 
-// Mask FT and mask sum for a radius (once per call).
-int local_std_prepare(const LocalStdScratch& sc, gtomx_int3 d, float radius, hipStream_t s) {
+// Mask FT and mask sum for a radius (once per call), optionally under a Fourier mask.
+int local_std_prepare(const LocalStdScratch& sc, gtomx_int3 d, float radius, hipStream_t s,
+                      const float* fouriermask) {
   GX_TRY(sphere_mask_ifftshift(sc.rbuf, d, radius, 1.0f, s));
   GX_TRY(sum_f64(sc.rbuf, elements(d), sc.dsum, s));
   GX_TRY(fft_r2c(sc.rbuf, sc.maskft, d, 1, s));
+  if (fouriermask) {
+    const size_t H = elements_ft(d);
+    hipLaunchKernelGGL(k_mask_fourier, dim3(grid_for(H)), dim3(TPB), 0, s, sc.maskft, fouriermask, H,
+                       sc.dsum + 256);
+    GX_LAUNCHED("k_mask_fourier");
+  }
   return GTOMX_OK;
 }
 
@@ -673,8 +692,8 @@ int gtomx_sphere_mask_ex(const void* d_input, void* d_output, gtomx_int3 dims, f
   return GTOMX_OK;
 }
 
-int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, float* d_std, float* d_mean,
-                    void* stream) {
+int gtomx_local_std(const float* d_map, gtomx_int3 dims, const float* d_fouriermask, float localradius,
+                    float* d_std, float* d_mean, void* stream) {
   if (!d_map || !d_std) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   int nd = dims.z > 1 ? 3 : (dims.y > 1 ? 2 : 1);
   GX_TRY(check_fft_dims(nd, dims));
@@ -682,7 +701,7 @@ int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, floa
   GX_TRY(sb.alloc(local_std_scratch_bytes(dims)));
   LocalStdScratch sc = local_std_carve(sb.p, dims);
   hipStream_t s = as_stream(stream);
-  GX_TRY(local_std_prepare(sc, dims, localradius, s));
+  GX_TRY(local_std_prepare(sc, dims, localradius, s, d_fouriermask));
   GX_TRY(local_std_apply(sc, d_map, dims, d_std, d_mean, nullptr, nullptr, nullptr, s));
   GX_HIP(hipStreamSynchronize(s));
   return GTOMX_OK;

@@ -127,7 +127,9 @@ struct LocalStdScratch {
 };
 size_t local_std_scratch_bytes(gtomx_int3 d);
 LocalStdScratch local_std_carve(void* base, gtomx_int3 d);
-int local_std_prepare(const LocalStdScratch& sc, gtomx_int3 d, float radius, hipStream_t s);
+// fouriermask: NULL, or a real half spectrum (FFT order) that scales the mask spectrum.
+int local_std_prepare(const LocalStdScratch& sc, gtomx_int3 d, float radius, hipStream_t s,
+                      const float* fouriermask = nullptr);
 int local_std_apply(const LocalStdScratch& sc, const float* map, gtomx_int3 d, float* std_out, float* mean_out,
                     float* best, const int32_t* idx, float* angle_out, hipStream_t s);
 

@@ -139,11 +139,11 @@ inline void d_SphereMask(T* d_input, T* d_output, int3 size, tfloat* radius, tfl
   detail::sync();
 }
 
-// d_LocalStd without a Fourier mask, include/Generics.cuh:782.
+// d_LocalStd, include/Generics.cuh:782; d_fouriermask NULL or a real half spectrum.
 inline void d_LocalStd(tfloat* d_map, int3 dimsmap, tfloat* d_fouriermask, tfloat localradius, tfloat* d_std,
                        tfloat* d_mean = nullptr) {
-  if (d_fouriermask != nullptr) throw std::runtime_error("d_LocalStd: a Fourier mask is not supported");
-  detail::check(gtomx_local_std(d_map, detail::i3(dimsmap), localradius, d_std, d_mean, nullptr), "d_LocalStd");
+  detail::check(gtomx_local_std(d_map, detail::i3(dimsmap), d_fouriermask, localradius, d_std, d_mean, nullptr),
+                "d_LocalStd");
 }
 
 // d_PickSubTomogramsDiff2, include/Correlation.cuh:111-123, with the

@@ -240,12 +240,14 @@ int gtomx_sphere_mask_ex(const void* d_input, void* d_output, gtomx_int3 dims, f
                          const gtomx_float3* center, int decentered, int complex_values, unsigned batch,
                          void* stream);
 
-/* d_LocalStd without a Fourier mask, include/Generics.cuh:782 /
- * src/Generics/LocalStd.cu:10-118: local standard deviation (and, if d_mean
- * is not NULL, mean) of d_map under a soft sphere of localradius. The
- * variance is combined in double. Allocates and frees its own scratch. */
-int gtomx_local_std(const float* d_map, gtomx_int3 dims, float localradius, float* d_std,
-                    float* d_mean, void* stream);
+/* d_LocalStd, include/Generics.cuh:782 / src/Generics/LocalStd.cu:10-118:
+ * local standard deviation (and, if d_mean is not NULL, mean) of d_map under a
+ * soft sphere of localradius. d_fouriermask is NULL or a real half spectrum
+ * (FFT order, dims.x/2+1 fastest) that multiplies the sphere's spectrum; the
+ * mask sum is then that of the filtered sphere. The variance is combined in
+ * double. Allocates and frees its own scratch. */
+int gtomx_local_std(const float* d_map, gtomx_int3 dims, const float* d_fouriermask, float localradius,
+                    float* d_std, float* d_mean, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* The squared-difference scorer                                             */

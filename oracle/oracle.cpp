@@ -307,13 +307,13 @@ void remap_full2fullfft(const T* in, T* out, int nx, int ny, int nz) {
       }
 }
 
-// d_LocalStd without a Fourier mask (src/Generics/LocalStd.cu:10-118):
+// d_LocalStd (src/Generics/LocalStd.cu:10-118), optionally under a Fourier mask:
 // sigma = sqrt(max(0, M*conv(map^2) - conv(map)^2)) / M, with conv = C2R_n of
 // the product with the conjugate FT of the ifftshifted soft sphere and
 // M = sum of the mask. Optional local mean = conv(map) / M.
 template <class T>
 void local_std(const T* map, int nx, int ny, int nz, double radius, T* std_out,
-               T* mean_out) {
+               T* mean_out, const T* fouriermask = nullptr) {
   long long V = (long long)nx * ny * nz;
   int nh = nx / 2 + 1;
   long long H = (long long)nh * ny * nz;
@@ -328,6 +328,16 @@ void local_std(const T* map, int nx, int ny, int nz, double radius, T* std_out,
   for (long long i = 0; i < V; ++i) msum += maskr[i];
   std::vector<cpx<T>> mft(H), aft(H), bft(H);
   r2c3(maskr.data(), mft.data(), nx, ny, nz);
+  if (fouriermask) {
+    // Filtered sphere (LocalStd.cu:38-52): spectrum times the mask, sum of the
+    // renormalised inverse transform.
+    for (long long i = 0; i < H; ++i) mft[i] *= fouriermask[i];
+    std::vector<cpx<T>> tmp(mft);
+    std::vector<T> conv(V);
+    c2r3(tmp.data(), conv.data(), nx, ny, nz);
+    msum = 0;
+    for (long long i = 0; i < V; ++i) msum += conv[i] / (T)V;
+  }
   std::vector<T> sq(V);
   for (long long i = 0; i < V; ++i) sq[i] = map[i] * map[i];
   r2c3(map, aft.data(), nx, ny, nz);
@@ -1206,8 +1216,8 @@ void orc_remap_full2fullfft_f64(const double* in, double* out, int nx, int ny, i
 }
 
 void orc_local_std_f64(const double* map, int nx, int ny, int nz, double radius,
-                       double* std_out, double* mean_out) {
-  local_std(map, nx, ny, nz, radius, std_out, mean_out);
+                       double* std_out, double* mean_out, const double* fouriermask) {
+  local_std(map, nx, ny, nz, radius, std_out, mean_out, fouriermask);
 }
 
 // Running max/argmax over a batch of maps (UpdateCorrelationKernel,

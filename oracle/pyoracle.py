@@ -39,7 +39,7 @@ def lib():
         L.orc_ctf3_f64.argtypes = [_p, _i, _d, _d, _d, _d, _d, _d, _d, _d]
         L.orc_sphere_mask_f64.argtypes = [_p, _i, _i, _i, _d, _d]
         L.orc_remap_full2fullfft_f64.argtypes = [_p, _p, _i, _i, _i]
-        L.orc_local_std_f64.argtypes = [_p, _i, _i, _i, _d, _p, _p]
+        L.orc_local_std_f64.argtypes = [_p, _i, _i, _i, _d, _p, _p, _p]
         L.orc_update_max_argmax_f64.argtypes = [_p, _ll, _i, _i, _p, _p]
         L.orc_local_peaks_f32.argtypes = [_p, _i, _i, _i, _i, _f, _p, _ll]
         L.orc_local_peaks_f32.restype = _ll
@@ -182,12 +182,16 @@ def remap_full2fullfft(vol):
     return out
 
 
-def local_std(vol, radius, with_mean=False):
+def local_std(vol, radius, with_mean=False, fouriermask=None):
     v = _c(vol, np.float64)
     nz, ny, nx = v.shape
+    fm = None if fouriermask is None else _c(fouriermask, np.float64)
+    if fm is not None:
+        assert fm.shape == (nz, ny, nx // 2 + 1)
     sd = np.empty_like(v)
     mean = np.empty_like(v) if with_mean else None
-    lib().orc_local_std_f64(_ptr(v), nx, ny, nz, radius, _ptr(sd), _ptr(mean) if with_mean else None)
+    lib().orc_local_std_f64(_ptr(v), nx, ny, nz, radius, _ptr(sd), _ptr(mean) if with_mean else None,
+                            None if fm is None else _ptr(fm))
     return (sd, mean) if with_mean else sd
 
 

@@ -103,6 +103,56 @@ def test_local_std_matches_brute_force_window(orc):
     assert np.abs(sd - ref_sd).max() < 1e-10
 
 
+def _aniso_lowpass(n, scale=1.0, widths=(0.02, 0.05, 0.11)):
+    """A real, even half-spectrum filter (FFT order, x fastest) with a different
+    width on each axis (x, y, z), so that a transposed layout changes the result."""
+    f = np.fft.fftfreq(n)
+    fz, fy, fx = np.meshgrid(f, f, np.fft.rfftfreq(n), indexing="ij")
+    return scale * np.exp(-(fx ** 2 / widths[0] + fy ** 2 / widths[1] + fz ** 2 / widths[2]))
+
+
+def test_local_std_fourier_mask_matches_brute_force_window(orc):
+    """With a Fourier mask the window is the sphere filtered by it, k =
+    irfftn(rfftn(ifftshift(sphere)) * fm), weighting offsets in FFT order; mean
+    and sigma are the k-weighted moments with M = sum k."""
+    rng = np.random.default_rng(18)
+    n, rad = 12, 3.0
+    m = rng.standard_normal((n, n, n)) * 2 + 0.5
+    fm = _aniso_lowpass(n, 0.8)
+    sd, mean = orc.local_std(m, rad, with_mean=True, fouriermask=fm)
+    W = orc.sphere_mask((n, n, n), rad, 1.0)
+    k = np.fft.irfftn(np.fft.rfftn(np.fft.ifftshift(W)) * fm, s=(n, n, n), axes=(0, 1, 2))
+    M = k.sum()
+    assert math.isclose(M, 0.8 * W.sum(), rel_tol=1e-12)  # the DC bin scales the sum
+    s1 = np.zeros_like(m)
+    s2 = np.zeros_like(m)
+    for dz in range(n):
+        for dy in range(n):
+            for dx in range(n):
+                sh = np.roll(m, (-dz, -dy, -dx), axis=(0, 1, 2))  # sh[x] = m[x + d]
+                s1 += k[dz, dy, dx] * sh
+                s2 += k[dz, dy, dx] * sh * sh
+    ref_mean = s1 / M
+    ref_sd = np.sqrt(np.maximum(s2 / M - ref_mean ** 2, 0))
+    assert np.abs(mean - ref_mean).max() < 1e-10
+    assert np.abs(sd - ref_sd).max() < 1e-10
+    # the axis widths matter: x and z swapped give another window
+    sd_t = orc.local_std(m, rad, fouriermask=_aniso_lowpass(n, 0.8, (0.11, 0.05, 0.02)))
+    assert np.abs(sd_t - sd).max() > 1e-3
+
+
+def test_local_std_fourier_mask_neutral_and_scale_invariant(orc):
+    rng = np.random.default_rng(19)
+    n, rad = 16, 4.0
+    m = rng.standard_normal((n, n, n)) + 1.0
+    sd0, mean0 = orc.local_std(m, rad, with_mean=True)
+    sd1, mean1 = orc.local_std(m, rad, with_mean=True, fouriermask=np.ones((n, n, n // 2 + 1)))
+    assert np.abs(sd1 - sd0).max() < 1e-12 and np.abs(mean1 - mean0).max() < 1e-12
+    # a constant filter scales both moments and the mask sum alike
+    sdc, meanc = orc.local_std(m, rad, with_mean=True, fouriermask=np.full((n, n, n // 2 + 1), 3.5))
+    assert np.abs(sdc - sd0).max() < 1e-12 and np.abs(meanc - mean0).max() < 1e-12
+
+
 def test_sphere_mask_edge(orc):
     n, rad = 32, 6.0
     W = orc.sphere_mask((n, n, n), rad, 1.0)
