@@ -125,6 +125,12 @@ _SIGS = {
     "gtomx_picker_set_image": [_P, _P, _P, _P],
     "gtomx_picker_correlate": [_P, _F, _P, _P, _P],
     "gtomx_picker_correlate_angles": [_P, _P, _U, _P, _P, _P],
+    "gtomx_prob_picker_create": [ctypes.POINTER(_P), _P, int3, _U, _P, _I, _I, int3, _U, _P],
+    "gtomx_prob_picker_destroy": [_P],
+    "gtomx_prob_picker_set_image": [_P, _P, _P, _P],
+    "gtomx_prob_picker_correlate": [_P, _U, _F, _P, _P, _P, _P],
+    "gtomx_prob_picker_count_angles": [_I, _F, ctypes.POINTER(_I)],
+    "gtomx_rotate2d_ft": [_P, _P, int3, _P, _F, _I, _U, _P],
     "gtomx_pick_plan_create": [ctypes.POINTER(_P), int3, int3, _U],
     "gtomx_pick_plan_destroy": [_P],
     "gtomx_pick_plan_info": [_P, ctypes.POINTER(_U), ctypes.POINTER(_SZ)],
@@ -327,6 +333,23 @@ def d_Rotate3DFT(volume, angles, outputzerocentered=True, stream=None):
     fn = lib().gtomx_rotate3d_ft if volume.is_complex() else lib().gtomx_rotate3d_ft_real
     check(fn(_ptr(volume), _ptr(out), int3(n, n, n), ctypes.c_void_p(a.ctypes.data), len(a), int(bool(outputzerocentered)),
              _stream(stream)), "gtomx_rotate3d_ft")
+    return out
+
+
+def d_Rotate2DFT(image, angles, maxfreq, outputzerocentered=True, stream=None):
+    """d_Rotate2DFT, linear (src/Transformation/Rotation.cu:200-256): (batch, n, n/2+1)
+    zero-centred complex64 half spectra, item b rotated by angles[b] (radians);
+    returns a new tensor of the same shape."""
+    import torch
+    a = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(-1))
+    x = image if image.dim() == 3 else image.unsqueeze(0)
+    if x.shape[0] != len(a):
+        raise ValueError("one angle per image")
+    ny, nh = x.shape[-2:]
+    out = torch.empty_like(x)
+    check(lib().gtomx_rotate2d_ft(_ptr(x), _ptr(out), int3(2 * (nh - 1), ny, 1), ctypes.c_void_p(a.ctypes.data),
+                                  float(maxfreq), int(bool(outputzerocentered)), len(a), _stream(stream)),
+          "gtomx_rotate2d_ft")
     return out
 
 
@@ -732,6 +755,54 @@ class Picker:
         else:
             check(lib().gtomx_picker_correlate(self.handle, float(anglestep), _ptr(best), _ptr(bestangle),
                                                _stream(stream)), "gtomx_picker_correlate")
+
+
+class ProbabilisticPicker:
+    """The ProbabilisticPicker class (include/Correlation.cuh:166-207) with circular
+    masks: refs and masks are (nrefs, z, y, x) or (nrefs, y, x) float32 CUDA
+    tensors; imageshape (z, y, x) or (y, x)."""
+
+    def __init__(self, refs, refmasks, imageshape, lowpassfreq, doctf=False, ismaskcircular=True, stream=None):
+        nd = len(imageshape)
+        rshape = tuple(refs.shape[-nd:])
+        nrefs = refs.numel() // int(np.prod(rshape))
+        dr = int3(rshape[-1], rshape[-2], rshape[0] if nd == 3 else 1)
+        di = int3(imageshape[-1], imageshape[-2], imageshape[0] if nd == 3 else 1)
+        h = _P()
+        check(lib().gtomx_prob_picker_create(ctypes.byref(h), _ptr(refs), dr, nrefs, _ptr(refmasks),
+                                             int(bool(ismaskcircular)), int(bool(doctf)), di, int(lowpassfreq),
+                                             _stream(stream)), "gtomx_prob_picker_create")
+        self.handle = h
+        self.ndims = nd
+        self.imageshape = tuple(imageshape)
+
+    def close(self):
+        if self.handle:
+            lib().gtomx_prob_picker_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_image(self, image, ctf=None, stream=None):
+        check(lib().gtomx_prob_picker_set_image(self.handle, _ptr(image), _ptr(ctf), _stream(stream)),
+              "gtomx_prob_picker_set_image")
+
+    def count_angles(self, anglestep):
+        c = _I()
+        check(lib().gtomx_prob_picker_count_angles(self.ndims, float(anglestep), ctypes.byref(c)),
+              "gtomx_prob_picker_count_angles")
+        return c.value
+
+    def perform_correlation(self, n, anglestep, bestccf, bestangle, bestref, stream=None):
+        """Updates bestccf (image shape, float32), bestangle (image shape + (3,),
+        float32) and bestref (image shape, int32) in place for reference n."""
+        check(lib().gtomx_prob_picker_correlate(self.handle, int(n), float(anglestep), _ptr(bestccf),
+                                                _ptr(bestangle), _ptr(bestref), _stream(stream)),
+              "gtomx_prob_picker_correlate")
 
 
 def _angles_arr(angles):

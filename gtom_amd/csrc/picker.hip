@@ -84,6 +84,8 @@ __global__ __launch_bounds__(256) void k_norm_apply(const float* __restrict__ x,
   }
 }
 
+}  // namespace
+
 // scratch: NB * batch + 2 * batch doubles
 int norm_mean01std(const float* in, float* out, size_t n, unsigned batch, double* scratch, hipStream_t s) {
   double* part = scratch;
@@ -100,6 +102,8 @@ int norm_mean01std(const float* in, float* out, size_t n, unsigned batch, double
   return GTOMX_OK;
 }
 size_t norm_scratch_doubles(unsigned batch) { return (size_t)NB * batch + 2 * (size_t)batch; }
+
+namespace {
 
 // P[b][i] = E[i] * conj(P[b][i])  (d_ComplexMultiplyByConjVector(d_imageft, P, P),
 // src/Correlation/Picker.cu:228, src/Generics/Arithmetics.cu:334-344)
@@ -134,6 +138,14 @@ __global__ __launch_bounds__(256) void k_update_max_angle(const float* __restric
 }
 
 }  // namespace
+
+int image_times_conj(const float2* E, float2* P, size_t h, unsigned batch, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min<size_t>((h * batch + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_image_times_conj, dim3(grid), dim3(256), 0, s, E, P, h, batch);
+  GX_LAUNCHED("k_image_times_conj");
+  return GTOMX_OK;
+}
+
 }  // namespace gx
 
 // ---------------------------------------------------------------------------

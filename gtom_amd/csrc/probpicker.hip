@@ -1,0 +1,599 @@
+// ProbabilisticPicker (include/Correlation.cuh:166-207,
+// src/Correlation/ProbabilisticPicker.cu:29-455) for circular masks, 2D and 3D.
+//
+// Initialize (:33-152), per reference n, on the device:
+//   R_n = ifftshift(pad(ref_n, dimsimage, 0)), C_n = FFTCrop(R2C(R_n), 2*lowpass),
+//   C_n *= [|k| < lowpass] (d_Bandpass, :136), then d_RemapHalfFFT2Half (:138).
+//   That remap (src/FFT/FFTRemap.cu, RemapHalfFFT2HalfKernel) reads row
+//   signed(i) + n/2 for output row i, which is n (one past the array) for the
+//   zero-frequency row and plane of the centred output; those reads land in the
+//   next plane's DC row or past the cropped spectrum, where the reference's buffer
+//   still holds R_n (the crop was written over its head). Both are restated here:
+//   element j >= ElementsFFT(2*lowpass) is the float pair (R_n[2j], R_n[2j+1]).
+//   Masks: M_n = ifftshift(pad(mask_n)), IM_n = R2C(ifftshift(pad(1 - mask_n))).
+// SetImage (:205-224): I = R2C(mean0std1(image)) and I2 = R2C(mean0std1(image)^2),
+//   both band-limited to |k| < lowpass; the CTF cropped to 2*lowpass.
+// PerformCorrelation (:226-389), orientations B at a time:
+//   rot = Rotate{3D,2D}FT(C_n, angle) in FFT order, * ctf_low if doctf,
+//   r = C2R(FFTPad(rot, dimsimage)) / N * M_n, corr = C2R_u(I * conj(R2C(r))),
+//   sums over the FFTFullCrop of r and M_n to dimsref (in the crop's order):
+//   samples = sum m, mean = sum r / samples, mean2 = sum r^2 / samples,
+//   ratio = exp((sum_i 5 r_i^2 + 2 r_i G_i) / 5 / (2 samples)) with G_i the sum
+//   of the five gaussrand() draws of element i (src/Helpers/Misc.cpp:14-31, the
+//   C runtime's rand() after srand(123), drawn on the host in the reference's
+//   order while the device works on the previous batch);
+//   solvent mean/std from I, I2 and IM_n once per call (CalcSolventStatistics,
+//   :423-441, ProbSolventStatisticsKernel :443-454);
+//   score (ProbPickingUpdateKernel :391-421) in double, kept where it beats the
+//   stored float, orientations in increasing order with strict '>'.
+// Nothing goes back to the host per orientation: the reference's per-orientation
+// device-to-host copies and host sums become one reduction kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <mutex>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace gx {
+namespace {
+
+constexpr int NBS = 64;  // reduction workgroups per orientation for the crop sums
+
+// Per-orientation scalars of the score, as the reference passes them.
+struct ProbCoef {
+  double ratio;       // expectedpratio
+  float normfactor;   // 1 / (N * samples), float arithmetic
+  float smr, smr2;    // summaskedref, summaskedref2 rounded to float
+  float solventnorm;  // 1 / (N * (Er - samples)), float arithmetic
+};
+
+__device__ __forceinline__ int signed_freq(int i, int n) { return i < n / 2 + 1 ? i : i - n; }
+
+// d_Bandpass(low 0, high r, smooth 0) on FFT-order half spectra: keeps |k| < r.
+__global__ __launch_bounds__(256) void k_bandpass(float2* __restrict__ x, int3 d, int r, size_t per, unsigned batch) {
+  const int nh = d.x / 2 + 1;
+  const long long r2 = (long long)r * r;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < per * batch; i += (size_t)gridDim.x * 256) {
+    const size_t e = i % per;
+    const int kx = (int)(e % nh), ky = signed_freq((int)((e / nh) % d.y), d.y),
+              kz = signed_freq((int)(e / ((size_t)nh * d.y)), d.z);
+    const long long q = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+    if (!(q < r2)) x[i] = make_float2(0.f, 0.f);
+  }
+}
+
+// RemapHalfFFT2HalfKernel as the reference runs it on its buffer: C (FFT-order
+// crop, `per` complex values) followed by the tail of R viewed as float pairs.
+__global__ __launch_bounds__(256) void k_remap_half_quirk(const float2* __restrict__ C, const float* __restrict__ R,
+                                                          float2* __restrict__ out, int n, int nz, size_t per,
+                                                          size_t vimage, unsigned batch) {
+  const int nh = n / 2 + 1;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < per * batch; i += (size_t)gridDim.x * 256) {
+    const size_t b = i / per, e = i % per;
+    const int x = (int)(e % nh), y = (int)((e / nh) % n), z = (int)(e / ((size_t)nh * n));
+    const int rz = (z < nz / 2 + 1 ? z : z - n) + nz / 2;
+    const int ry = (y < n / 2 + 1 ? y : y - n) + n / 2;
+    const size_t j = ((size_t)rz * n + ry) * nh + x;
+    out[i] = j < per ? C[b * per + j] : reinterpret_cast<const float2*>(R + b * vimage)[j];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_one_minus(const float* __restrict__ in, float* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) out[i] = 1.0f - in[i];
+}
+
+__global__ __launch_bounds__(256) void k_times_real(float2* __restrict__ x, const float* __restrict__ v, size_t per,
+                                                    unsigned batch) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < per * batch; i += (size_t)gridDim.x * 256) {
+    const float f = v[i % per];
+    const float2 a = x[i];
+    x[i] = make_float2(a.x * f, a.y * f);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_times_mask(float* __restrict__ x, const float* __restrict__ m, size_t per,
+                                                    unsigned batch) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < per * batch; i += (size_t)gridDim.x * 256)
+    x[i] *= m[i % per];
+}
+
+// out = a * conj(b) (d_ComplexMultiplyByConjVector, src/Generics/Arithmetics.cu:334-344)
+__global__ __launch_bounds__(256) void k_mul_conj_into(const float2* __restrict__ a, const float2* __restrict__ b,
+                                                       float2* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const float2 x = a[i], y = b[i];
+    out[i] = make_float2(x.x * y.x + x.y * y.y, x.y * y.x - x.x * y.y);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_square_into(const float* __restrict__ in, float* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) out[i] = in[i] * in[i];
+}
+
+// Sums over the FFTFullCrop (src/FFT/FFTResize.cu:140-160) of r and the mask to
+// dimsref, element i of the crop paired with noise G[b][i]:
+// part[b][blk] = (sum m, sum r, sum r^2, sum r*G).
+__global__ __launch_bounds__(256) void k_prob_sums(const float* __restrict__ r, const float* __restrict__ mask,
+                                                   const float* __restrict__ G, int3 di, int3 dr, size_t vimage,
+                                                   size_t er, double4* __restrict__ part) {
+  __shared__ double4 red[4];
+  const unsigned b = blockIdx.y;
+  double sm = 0, sr = 0, srr = 0, srg = 0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < er; i += (size_t)gridDim.x * 256) {
+    const int x = (int)(i % dr.x), y = (int)((i / dr.x) % dr.y), z = (int)(i / ((size_t)dr.x * dr.y));
+    const int sx = x >= dr.x / 2 ? x + di.x - dr.x : x, sy = y >= dr.y / 2 ? y + di.y - dr.y : y,
+              sz = z >= dr.z / 2 ? z + di.z - dr.z : z;
+    const size_t src = ((size_t)sz * di.y + sy) * di.x + sx;
+    const double rv = r[(size_t)b * vimage + src];
+    sm += mask[src];
+    sr += rv;
+    srr += rv * rv;
+    srg += rv * (double)G[(size_t)b * er + i];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sm += __shfl_down(sm, o);
+    sr += __shfl_down(sr, o);
+    srr += __shfl_down(srr, o);
+    srg += __shfl_down(srg, o);
+  }
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = make_double4(sm, sr, srr, srg);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double4 t = red[0];
+    for (int w = 1; w < 4; ++w) {
+      t.x += red[w].x;
+      t.y += red[w].y;
+      t.z += red[w].z;
+      t.w += red[w].w;
+    }
+    part[(size_t)b * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// The host block of PerformCorrelation (:306-340) and the launch arguments
+// (:366-379) from the sums, one thread per orientation.
+__global__ void k_prob_coef(const double4* __restrict__ part, int nparts, unsigned batch, size_t vimage, size_t er,
+                            ProbCoef* __restrict__ coef) {
+  const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  double sm = 0, sr = 0, srr = 0, srg = 0;
+  for (int p = 0; p < nparts; ++p) {
+    const double4 t = part[(size_t)b * nparts + p];
+    sm += t.x;
+    sr += t.y;
+    srr += t.z;
+    srg += t.w;
+  }
+  ProbCoef c;
+  const double pr = (5.0 * srr + 2.0 * srg) / 5.0;
+  c.ratio = exp(pr / (2.0 * sm));
+  c.smr = (float)(sr / sm);
+  c.smr2 = (float)(srr / sm);
+  const float masksum = (float)sm;                   // h_masksum[n] is a tfloat
+  const float invmasksum = (float)((double)er - sm);  // h_invmasksum[n] likewise
+  c.normfactor = 1.0f / ((float)vimage * masksum);
+  c.solventnorm = 1.0f / ((float)vimage * invmasksum);
+  coef[b] = c;
+}
+
+// ProbSolventStatisticsKernel: mean = m * f, std = sqrt(max(0, s * f - mean^2)), float.
+__global__ __launch_bounds__(256) void k_solvent_stats(float* __restrict__ mean, float* __restrict__ sd, size_t n,
+                                                       const ProbCoef* __restrict__ coef) {
+  const float f = coef[0].solventnorm;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const float m = mean[i] * f;
+    mean[i] = m;
+    const float v = sd[i] * f - m * m;
+    sd[i] = sqrtf(v < 0.f ? 0.f : v);
+  }
+}
+
+// ProbPickingUpdateKernel over B maps in orientation order.
+__global__ __launch_bounds__(256) void k_prob_update(const float* __restrict__ corr, const float* __restrict__ smean,
+                                                     const float* __restrict__ sstd, const ProbCoef* __restrict__ coef,
+                                                     const float3* __restrict__ angles, unsigned batch, size_t n,
+                                                     int refid, float* __restrict__ bestccf,
+                                                     float3* __restrict__ bestangle, int* __restrict__ bestref) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    float best = bestccf[i];
+    int k = -1;
+    const double m = smean[i];
+    const double sd = fmax(1e-7, (double)sstd[i]);
+    for (unsigned b = 0; b < batch; ++b) {
+      const ProbCoef c = coef[b];
+      double d = -2.0 * (double)c.normfactor * (double)corr[(size_t)b * n + i];
+      d += 2.0 * m * (double)c.smr;
+      d /= sd;
+      d += (double)c.smr2;
+      d = exp(-d / 2.0);
+      d = (d - 1.0) / (c.ratio - 1.0);
+      if (d > (double)best) {
+        best = (float)d;
+        k = (int)b;
+      }
+    }
+    if (k >= 0) {
+      bestccf[i] = best;
+      bestangle[i] = angles[k];
+      bestref[i] = refid;
+    }
+  }
+}
+
+inline unsigned grid_of(size_t n, size_t cap = 16384) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, cap)); }
+
+// gaussrand (src/Helpers/Misc.cpp:14-31): Box-Muller pairs from the C runtime's
+// rand(), in float as the reference's tfloat statics and float arguments make
+// it; the state lives for the process and is shared by every picker, as there.
+struct GaussRand {
+  float U = 0, V = 0;
+  int phase = 0;
+  float next() {
+    float Z;
+    if (phase == 0) {
+      U = (float)((std::rand() + 1.) / (RAND_MAX + 2.));
+      V = (float)(std::rand() / (RAND_MAX + 1.));
+      Z = std::sqrt(-2 * std::log(U)) * std::sin(2 * 3.1415926535897932384626433832795f * V);
+    } else {
+      Z = std::sqrt(-2 * std::log(U)) * std::cos(2 * 3.1415926535897932384626433832795f * V);
+    }
+    phase = 1 - phase;
+    return Z;
+  }
+};
+GaussRand g_gauss;
+std::mutex g_gauss_mutex;
+
+}  // namespace
+}  // namespace gx
+
+struct gtomx_prob_picker {
+  gtomx_int3 dr{}, di{}, dl{};
+  int ndims = 3;
+  unsigned nrefs = 0;
+  bool circular = true, doctf = false;
+  int device = 0;
+  unsigned B = 0;
+  size_t Er = 0, Vi = 0, Hi = 0, Hl = 0;
+  float2* refs_low = nullptr;    // nrefs x Hl, zero-centred half spectra
+  float* masks = nullptr;        // nrefs x Vi, FFT order
+  float2* invmasksft = nullptr;  // nrefs x Hi
+  float2* imageft = nullptr;
+  float2* image2ft = nullptr;
+  float* ctf_low = nullptr;      // Hl
+  float2* rot = nullptr;         // B x Hl
+  float2* padft = nullptr;       // B x Hi: padded spectra, C2R workspace, then R2C of r
+  float* real = nullptr;         // B x Vi: r, then the correlation maps
+  float* smean = nullptr;        // Vi
+  float* sstd = nullptr;         // Vi
+  float* noise = nullptr;        // B x Er
+  double4* part = nullptr;       // B x NBS
+  gx::ProbCoef* coef = nullptr;  // B
+  float* mats = nullptr;         // B x 9 (3D) or B x 2 (2D)
+  float3* dang = nullptr;        // B
+  double* nscr = nullptr;
+  float* h_noise[2] = {nullptr, nullptr};  // pinned, B x Er each
+  float* h_stage[2] = {nullptr, nullptr};  // pinned, B x 12 floats each: matrices then angles
+  hipEvent_t uploaded[2] = {nullptr, nullptr};
+  bool have_image = false;
+};
+
+namespace gx {
+namespace {
+
+void prob_release(gtomx_prob_picker* pp) {
+  if (!pp) return;
+  for (void* p : {(void*)pp->refs_low, (void*)pp->masks, (void*)pp->invmasksft, (void*)pp->imageft,
+                  (void*)pp->image2ft, (void*)pp->ctf_low, (void*)pp->rot, (void*)pp->padft, (void*)pp->real,
+                  (void*)pp->smean, (void*)pp->sstd, (void*)pp->noise, (void*)pp->part, (void*)pp->coef,
+                  (void*)pp->mats, (void*)pp->dang, (void*)pp->nscr})
+    dev_free(p);
+  for (int k = 0; k < 2; ++k) {
+    if (pp->h_noise[k]) (void)hipHostFree(pp->h_noise[k]);
+    if (pp->h_stage[k]) (void)hipHostFree(pp->h_stage[k]);
+    if (pp->uploaded[k]) (void)hipEventDestroy(pp->uploaded[k]);
+  }
+  delete pp;
+}
+
+int prob_check(const gtomx_prob_picker* pp) {
+  if (!pp) return fail(GTOMX_ERR_INVALID_ARGUMENT, "picker is NULL");
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  if (dev != pp->device) return fail(GTOMX_ERR_INVALID_ARGUMENT, "picker was created on another device");
+  return GTOMX_OK;
+}
+
+// Highest complex index the reference's half remap reads for a 2*lowpass square
+// or cube (the zero-frequency plane reads plane n, the zero-frequency row row n).
+size_t remap_quirk_max_index(int n, int nz) {
+  const size_t nh = (size_t)(n / 2 + 1);
+  const int rzmax = nz > 1 ? n : 0;
+  return ((size_t)rzmax * n + n) * nh + (nh - 1);
+}
+
+// The orientation list of PerformCorrelation (:234-244).
+int prob_angles(int ndims, float anglestep, std::vector<gtomx_float3>& out) {
+  const float PIf = 3.1415926535897932384626433832795f, PI2f = 6.283185307179586476925286766559f;
+  if (!(anglestep > 0.0f)) return fail(GTOMX_ERR_INVALID_ARGUMENT, "anglestep must be positive");
+  if (ndims == 3) {
+    int count = 0;
+    GX_TRY(gtomx_equal_angular_spacing(0, PI2f, 0, PIf, 0, PI2f, anglestep, nullptr, 0, &count));
+    out.resize((size_t)count);
+    GX_TRY(gtomx_equal_angular_spacing(0, PI2f, 0, PIf, 0, PI2f, anglestep, out.data(), count, &count));
+  } else {
+    const int steps = (int)std::ceil(PI2f / anglestep);
+    const float step = PI2f / (float)steps;
+    out.resize((size_t)steps);
+    for (int a = 0; a < steps; ++a) out[(size_t)a] = gtomx_float3{0.f, 0.f, (float)a * step};
+  }
+  return GTOMX_OK;
+}
+
+}  // namespace
+}  // namespace gx
+
+using namespace gx;
+
+extern "C" {
+
+int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx_int3 dimsref, unsigned nrefs,
+                             const float* d_refmasks, int ismaskcircular, int doctf, gtomx_int3 dimsimage,
+                             unsigned lowpassfreq, void* stream) {
+  if (!out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (!d_refs || !d_refmasks) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
+  if (nrefs == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "nrefs must be positive");
+  if (!ismaskcircular)
+    return fail(GTOMX_ERR_UNSUPPORTED,
+                "non-circular masks are not supported: the reference rotates them with cubic B-splines about the "
+                "centre while the reference volume they multiply sits at the origin");
+  const int nd = dimsimage.z > 1 ? 3 : 2;
+  if ((dimsref.z > 1 ? 3 : 2) != nd) return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference and image dimensionality differ");
+  GX_TRY(check_fft_dims(nd, dimsimage));
+  if (dimsref.x < 1 || dimsref.y < 1 || dimsref.z < 1 || dimsref.x > dimsimage.x || dimsref.y > dimsimage.y ||
+      dimsref.z > dimsimage.z)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference dims must be positive and fit the image");
+  const int n = 2 * (int)lowpassfreq;
+  if (lowpassfreq == 0 || n > dimsimage.x || n > dimsimage.y || (nd == 3 && n > dimsimage.z))
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "2*lowpassfreq must be positive and fit the image");
+  const gtomx_int3 dl{n, n, nd == 3 ? n : 1};
+  const size_t Vi = elements(dimsimage), Hl = elements_ft(dl);
+  if (2 * remap_quirk_max_index(n, dl.z) + 2 > Vi)
+    return fail(GTOMX_ERR_UNSUPPORTED,
+                "the image is too small next to 2*lowpassfreq: the reference's half-spectrum remap would read "
+                "memory it never wrote");
+  if (nrefs > 65535) return fail(GTOMX_ERR_UNSUPPORTED, "more than 65535 references");
+
+  auto* pp = new gtomx_prob_picker();
+  pp->dr = dimsref;
+  pp->di = dimsimage;
+  pp->dl = dl;
+  pp->ndims = nd;
+  pp->nrefs = nrefs;
+  pp->circular = true;
+  pp->doctf = doctf != 0;
+  pp->Er = elements(dimsref);
+  pp->Vi = Vi;
+  pp->Hi = elements_ft(dimsimage);
+  pp->Hl = Hl;
+  hipStream_t s = as_stream(stream);
+  int st = GTOMX_OK;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (st == GTOMX_OK) st = dev_alloc(p, bytes);
+  };
+  st = hipGetDevice(&pp->device) == hipSuccess ? GTOMX_OK : fail(GTOMX_ERR_DEVICE, "hipGetDevice");
+  // orientations per batch: about 2 GiB of per-orientation buffers, at most 32
+  const size_t per = Hl * 8 + pp->Hi * 8 + Vi * 4 + pp->Er * 4;
+  pp->B = (unsigned)std::max<size_t>(1, std::min<size_t>(32, ((size_t)2 << 30) / per));
+  const unsigned B = pp->B;
+  alloc((void**)&pp->refs_low, (size_t)nrefs * Hl * sizeof(float2));
+  alloc((void**)&pp->masks, (size_t)nrefs * Vi * sizeof(float));
+  alloc((void**)&pp->invmasksft, (size_t)nrefs * pp->Hi * sizeof(float2));
+  alloc((void**)&pp->imageft, pp->Hi * sizeof(float2));
+  alloc((void**)&pp->image2ft, pp->Hi * sizeof(float2));
+  alloc((void**)&pp->ctf_low, Hl * sizeof(float));
+  alloc((void**)&pp->rot, (size_t)B * Hl * sizeof(float2));
+  alloc((void**)&pp->padft, (size_t)B * pp->Hi * sizeof(float2));
+  alloc((void**)&pp->real, (size_t)B * Vi * sizeof(float));
+  alloc((void**)&pp->smean, Vi * sizeof(float));
+  alloc((void**)&pp->sstd, Vi * sizeof(float));
+  alloc((void**)&pp->noise, (size_t)B * pp->Er * sizeof(float));
+  alloc((void**)&pp->part, (size_t)B * NBS * sizeof(double4));
+  alloc((void**)&pp->coef, (size_t)B * sizeof(ProbCoef));
+  alloc((void**)&pp->mats, (size_t)B * 9 * sizeof(float));
+  alloc((void**)&pp->dang, (size_t)B * sizeof(float3));
+  alloc((void**)&pp->nscr, norm_scratch_doubles(1) * sizeof(double));
+  for (int k = 0; k < 2 && st == GTOMX_OK; ++k) {
+    if (hipHostMalloc((void**)&pp->h_noise[k], (size_t)B * pp->Er * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&pp->h_stage[k], (size_t)B * 12 * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&pp->uploaded[k], hipEventDisableTiming) != hipSuccess)
+      st = fail(GTOMX_ERR_OUT_OF_MEMORY, "pinned staging buffers");
+  }
+  if (st != GTOMX_OK) {
+    prob_release(pp);
+    return st;
+  }
+
+  // Reference spectra and masks, all references at once. Workspace: the padded
+  // references R (nrefs x Vi) in `real`-sized chunks would not fit every case, so
+  // one temporary of nrefs x (Vi floats + Hi complex) is taken here.
+  DevBuf tR, tF, tC;
+  auto fin = [&](int status) {
+    if (status != GTOMX_OK) prob_release(pp);
+    return status;
+  };
+  if ((st = tR.alloc((size_t)nrefs * Vi * sizeof(float))) != GTOMX_OK) return fin(st);
+  if ((st = tF.alloc((size_t)nrefs * pp->Hi * sizeof(float2))) != GTOMX_OK) return fin(st);
+  if ((st = tC.alloc((size_t)nrefs * std::max(Hl * sizeof(float2), pp->Er * sizeof(float)))) != GTOMX_OK)
+    return fin(st);
+  float* R = tR.as<float>();
+  float2* F = tF.as<float2>();
+  // R = ifftshift(pad(ref)); F = R2C(R); C = crop(F) band-limited; quirky remap.
+  if ((st = gtomx_pad(d_refs, reinterpret_cast<float*>(F), dimsref, dimsimage, 0.0f, nrefs, stream)) != GTOMX_OK)
+    return fin(st);
+  if ((st = gtomx_remap_full2fullfft(reinterpret_cast<float*>(F), R, dimsimage, nrefs, stream)) != GTOMX_OK)
+    return fin(st);
+  if ((st = fft_r2c(R, F, dimsimage, nrefs, s)) != GTOMX_OK) return fin(st);
+  float2* C = tC.as<float2>();
+  if ((st = fft_resize(FFT_CROP, true, F, C, dimsimage, dl, nrefs, s)) != GTOMX_OK) return fin(st);
+  hipLaunchKernelGGL(k_bandpass, dim3(grid_of(Hl * nrefs)), dim3(256), 0, s, C, make_int3(dl.x, dl.y, dl.z),
+                     (int)lowpassfreq, Hl, nrefs);
+  if (hipGetLastError() != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "k_bandpass"));
+  hipLaunchKernelGGL(k_remap_half_quirk, dim3(grid_of(Hl * nrefs)), dim3(256), 0, s, C, R, pp->refs_low, n, dl.z,
+                     Hl, Vi, nrefs);
+  if (hipGetLastError() != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "k_remap_half_quirk"));
+  // M = ifftshift(pad(mask)); IM = R2C(ifftshift(pad(1 - mask)))
+  if ((st = gtomx_pad(d_refmasks, R, dimsref, dimsimage, 0.0f, nrefs, stream)) != GTOMX_OK) return fin(st);
+  if ((st = gtomx_remap_full2fullfft(R, pp->masks, dimsimage, nrefs, stream)) != GTOMX_OK) return fin(st);
+  float* om = tC.as<float>();
+  hipLaunchKernelGGL(k_one_minus, dim3(grid_of(pp->Er * nrefs)), dim3(256), 0, s, d_refmasks, om, pp->Er * nrefs);
+  if (hipGetLastError() != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "k_one_minus"));
+  if ((st = gtomx_pad(om, reinterpret_cast<float*>(F), dimsref, dimsimage, 0.0f, nrefs, stream)) != GTOMX_OK)
+    return fin(st);
+  if ((st = gtomx_remap_full2fullfft(reinterpret_cast<float*>(F), R, dimsimage, nrefs, stream)) != GTOMX_OK)
+    return fin(st);
+  if ((st = fft_r2c(R, pp->invmasksft, dimsimage, nrefs, s)) != GTOMX_OK) return fin(st);
+  if (hipStreamSynchronize(s) != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "hipStreamSynchronize"));
+  *out = pp;
+  return GTOMX_OK;
+}
+
+int gtomx_prob_picker_destroy(gtomx_prob_picker* picker) {
+  prob_release(picker);
+  return GTOMX_OK;
+}
+
+int gtomx_prob_picker_set_image(gtomx_prob_picker* pp, const float* d_image, const float* d_ctf, void* stream) {
+  GX_TRY(prob_check(pp));
+  if (!d_image) return fail(GTOMX_ERR_INVALID_ARGUMENT, "d_image is NULL");
+  if (pp->doctf && !d_ctf) return fail(GTOMX_ERR_INVALID_ARGUMENT, "d_ctf is NULL and the picker applies a CTF");
+  hipStream_t s = as_stream(stream);
+  const int lp = pp->dl.x / 2;
+  const int3 di = make_int3(pp->di.x, pp->di.y, pp->di.z);
+  // the mean0/std1 image goes to the first map of `real`, its square to a temporary
+  DevBuf sq;
+  GX_TRY(sq.alloc(pp->Vi * sizeof(float)));
+  GX_TRY(norm_mean01std(d_image, pp->real, pp->Vi, 1, pp->nscr, s));
+  GX_TRY(fft_r2c(pp->real, pp->imageft, pp->di, 1, s));
+  hipLaunchKernelGGL(k_bandpass, dim3(grid_of(pp->Hi)), dim3(256), 0, s, pp->imageft, di, lp, pp->Hi, 1u);
+  GX_LAUNCHED("k_bandpass");
+  hipLaunchKernelGGL(k_square_into, dim3(grid_of(pp->Vi)), dim3(256), 0, s, pp->real, sq.as<float>(), pp->Vi);
+  GX_LAUNCHED("k_square_into");
+  GX_TRY(fft_r2c(sq.as<float>(), pp->image2ft, pp->di, 1, s));
+  hipLaunchKernelGGL(k_bandpass, dim3(grid_of(pp->Hi)), dim3(256), 0, s, pp->image2ft, di, lp, pp->Hi, 1u);
+  GX_LAUNCHED("k_bandpass");
+  if (d_ctf) GX_TRY(fft_resize(FFT_CROP, false, d_ctf, pp->ctf_low, pp->di, pp->dl, 1, s));
+  GX_HIP(hipStreamSynchronize(s));
+  pp->have_image = true;
+  return GTOMX_OK;
+}
+
+int gtomx_prob_picker_count_angles(int ndims, float anglestep, int* count) {
+  if (!count) return fail(GTOMX_ERR_INVALID_ARGUMENT, "count is NULL");
+  if (ndims != 2 && ndims != 3) return fail(GTOMX_ERR_INVALID_ARGUMENT, "ndims must be 2 or 3");
+  std::vector<gtomx_float3> a;
+  GX_TRY(prob_angles(ndims, anglestep, a));
+  *count = (int)a.size();
+  return GTOMX_OK;
+}
+
+int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglestep, float* d_bestccf,
+                                gtomx_float3* d_bestangle, int* d_bestref, void* stream) {
+  GX_TRY(prob_check(pp));
+  if (!pp->have_image) return fail(GTOMX_ERR_INVALID_ARGUMENT, "set_image was not called");
+  if (n >= pp->nrefs) return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference index out of range");
+  if (!d_bestccf || !d_bestangle || !d_bestref) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL output");
+  std::vector<gtomx_float3> angles;
+  GX_TRY(prob_angles(pp->ndims, anglestep, angles));
+  hipStream_t s = as_stream(stream);
+  const unsigned B = pp->B;
+  const size_t Er = pp->Er, Vi = pp->Vi, Hi = pp->Hi, Hl = pp->Hl;
+  const int3 di = make_int3(pp->di.x, pp->di.y, pp->di.z), dr = make_int3(pp->dr.x, pp->dr.y, pp->dr.z);
+  const float2* refft = pp->refs_low + (size_t)n * Hl;
+  const float* mask = pp->masks + (size_t)n * Vi;
+  const float2* invmaskft = pp->invmasksft + (size_t)n * Hi;
+  const float maxfreq = (float)(pp->dl.x / 2);
+  std::lock_guard<std::mutex> lock(g_gauss_mutex);
+  std::srand(123);
+  bool used[2] = {false, false};
+  for (size_t a0 = 0, k = 0; a0 < angles.size(); a0 += B, ++k) {
+    const unsigned nb = (unsigned)std::min<size_t>(B, angles.size() - a0);
+    const int slot = (int)(k % 2);
+    if (used[slot]) GX_HIP(hipEventSynchronize(pp->uploaded[slot]));
+    // noise of these orientations, in the reference's draw order
+    float* hn = pp->h_noise[slot];
+    for (unsigned b = 0; b < nb; ++b)
+      for (size_t i = 0; i < Er; ++i) {
+        double g = 0;
+        for (int r = 0; r < 5; ++r) g += (double)g_gauss.next();
+        hn[(size_t)b * Er + i] = (float)g;
+      }
+    float* hs = pp->h_stage[slot];
+    if (pp->ndims == 3) {
+      GX_TRY(gtomx_projector_matrices(angles.data() + a0, nb, 1.0f, hs));
+    } else {
+      for (unsigned b = 0; b < nb; ++b) rotate2d_coefficients(angles[a0 + b].z, reinterpret_cast<float2*>(hs) + b);
+    }
+    float* ha = hs + (size_t)B * 9;
+    for (unsigned b = 0; b < nb; ++b) {
+      ha[3 * b] = angles[a0 + b].x;
+      ha[3 * b + 1] = angles[a0 + b].y;
+      ha[3 * b + 2] = angles[a0 + b].z;
+    }
+    GX_HIP(hipMemcpyAsync(pp->noise, hn, (size_t)nb * Er * sizeof(float), hipMemcpyHostToDevice, s));
+    const size_t nmat = (size_t)nb * (pp->ndims == 3 ? 9 : 2);
+    GX_HIP(hipMemcpyAsync(pp->mats, hs, nmat * sizeof(float), hipMemcpyHostToDevice, s));
+    GX_HIP(hipMemcpyAsync(pp->dang, ha, (size_t)nb * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+    GX_HIP(hipEventRecord(pp->uploaded[slot], s));
+    used[slot] = true;
+
+    // rotate, CTF, pad, r = C2R / N, r *= mask
+    if (pp->ndims == 3)
+      GX_TRY(rotate3d_ft_batch(refft, pp->rot, pp->dl.x, pp->mats, nb, false, s));
+    else
+      GX_TRY(rotate2d_ft_batch(refft, 0, pp->rot, pp->dl.x, pp->dl.y, reinterpret_cast<const float2*>(pp->mats), nb,
+                               maxfreq, false, s));
+    if (pp->doctf) {
+      hipLaunchKernelGGL(k_times_real, dim3(grid_of(Hl * nb)), dim3(256), 0, s, pp->rot, pp->ctf_low, Hl, nb);
+      GX_LAUNCHED("k_times_real");
+    }
+    GX_TRY(fft_resize(FFT_PAD, true, pp->rot, pp->padft, pp->dl, pp->di, nb, s));
+    GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f / (float)Vi, s));
+    hipLaunchKernelGGL(k_times_mask, dim3(grid_of(Vi * nb)), dim3(256), 0, s, pp->real, mask, Vi, nb);
+    GX_LAUNCHED("k_times_mask");
+    // statistics of the cropped, masked reference
+    const int nparts = (int)std::min<size_t>(NBS, (Er + 255) / 256);
+    hipLaunchKernelGGL(k_prob_sums, dim3(nparts, nb), dim3(256), 0, s, pp->real, mask, pp->noise, di, dr, Vi, Er,
+                       pp->part);
+    GX_LAUNCHED("k_prob_sums");
+    hipLaunchKernelGGL(k_prob_coef, dim3((nb + 63) / 64), dim3(64), 0, s, pp->part, nparts, nb, Vi, Er, pp->coef);
+    GX_LAUNCHED("k_prob_coef");
+    // correlation with the image
+    GX_TRY(fft_r2c(pp->real, pp->padft, pp->di, nb, s));
+    GX_TRY(image_times_conj(pp->imageft, pp->padft, Hi, nb, s));
+    GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f, s));
+    if (a0 == 0) {
+      // solvent statistics with the first orientation's mask sum (the mask is
+      // circular, so every orientation has the same); padft[0] is free again.
+      float2* tmp = pp->padft;
+      hipLaunchKernelGGL(k_mul_conj_into, dim3(grid_of(Hi)), dim3(256), 0, s, pp->imageft, invmaskft, tmp, Hi);
+      GX_LAUNCHED("k_mul_conj_into");
+      GX_TRY(fft_c2r(tmp, tmp, pp->smean, pp->di, 1, 1.0f, s));
+      hipLaunchKernelGGL(k_mul_conj_into, dim3(grid_of(Hi)), dim3(256), 0, s, pp->image2ft, invmaskft, tmp, Hi);
+      GX_LAUNCHED("k_mul_conj_into");
+      GX_TRY(fft_c2r(tmp, tmp, pp->sstd, pp->di, 1, 1.0f, s));
+      hipLaunchKernelGGL(k_solvent_stats, dim3(grid_of(Vi)), dim3(256), 0, s, pp->smean, pp->sstd, Vi, pp->coef);
+      GX_LAUNCHED("k_solvent_stats");
+    }
+    hipLaunchKernelGGL(k_prob_update, dim3(grid_of(Vi)), dim3(256), 0, s, pp->real, pp->smean, pp->sstd, pp->coef,
+                       pp->dang, nb, Vi, (int)n, d_bestccf, reinterpret_cast<float3*>(d_bestangle), d_bestref);
+    GX_LAUNCHED("k_prob_update");
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  return GTOMX_OK;
+}
+
+}  // extern "C"

@@ -101,6 +101,24 @@ constexpr int FFT_CROP = GTOMX_FFT_CROP, FFT_PAD = GTOMX_FFT_PAD, FFT_FULL_CROP 
 int fft_resize(int op, bool complex_values, const void* in, void* out, gtomx_int3 od, gtomx_int3 nd, unsigned batch,
                hipStream_t s);
 
+// Fourier-space rotation (rotate.hip). rotate3d_ft_batch: one zero-centred half
+// spectrum of an n^3 cube rotated by `batch` matrices (9 floats each, from
+// gtomx_projector_matrices at scale 1). rotate2d_ft_batch: item b of the input
+// (stride in_stride complex values, 0 for one shared image) rotated with the
+// coefficients cs[b] from rotate2d_coefficients(angle_b).
+int rotate3d_ft_batch(const float2* in, float2* out, int n, const float* d_mats, unsigned batch, bool zerocentred,
+                      hipStream_t s);
+int rotate2d_ft_batch(const float2* in, size_t in_stride, float2* out, int nx, int ny, const float2* d_cs,
+                      unsigned batch, float maxfreq, bool zerocentred, hipStream_t s);
+void rotate2d_coefficients(float angle, float2* cs);
+
+// d_Norm with T_NORM_MEAN01STD per item (picker.hip); scratch holds
+// norm_scratch_doubles(batch) doubles.
+int norm_mean01std(const float* in, float* out, size_t n, unsigned batch, double* scratch, hipStream_t s);
+size_t norm_scratch_doubles(unsigned batch);
+// P[b][i] = E[i] * conj(P[b][i]) for b < batch (picker.hip).
+int image_times_conj(const float2* E, float2* P, size_t h, unsigned batch, hipStream_t s);
+
 // Element-wise helpers (ops.hip).
 int fill_f32(float* d, size_t n, float v, hipStream_t s);
 int fill_i32(int32_t* d, size_t n, int32_t v, hipStream_t s);

@@ -215,6 +215,17 @@ inline void d_Rotate3DFT(tfloat* d_volume, tfloat* d_output, int3 dims, tfloat3*
                 "d_Rotate3DFT");
 }
 
+// d_Rotate2DFT, include/Transformation.cuh:52: `batch` images, image b rotated
+// by angles[b] (host array, radians); linear interpolation only.
+inline void d_Rotate2DFT(tcomplex* d_input, tcomplex* d_output, int3 dims, tfloat* angles, tfloat maxfreq,
+                         T_INTERP_MODE mode, bool isoutputzerocentered, int batch = 1) {
+  if (mode != T_INTERP_LINEAR) throw std::runtime_error("d_Rotate2DFT: only T_INTERP_LINEAR is built");
+  detail::check(gtomx_rotate2d_ft(reinterpret_cast<const gtomx_complex*>(d_input),
+                                  reinterpret_cast<gtomx_complex*>(d_output), detail::i3(dims), angles, maxfreq,
+                                  isoutputzerocentered ? 1 : 0, (unsigned)batch, nullptr),
+                "d_Rotate2DFT");
+}
+
 // struct Peak, include/Correlation.cuh:127-134: a picked particle, filled by
 // the caller from the search output (position from d_LocalPeaks / d_Peak,
 // angles from the orientation index, fom from the score map).
@@ -525,6 +536,56 @@ class Picker {
 
  private:
   gtomx_picker* handle_ = nullptr;
+};
+
+// class ProbabilisticPicker, include/Correlation.cuh:166-207, circular masks
+// only. The reference keeps its low-pass references and masks in pinned host
+// memory and uploads one per PerformCorrelation; here they stay on the device.
+// PickImage is declared but never defined in the reference and is absent here.
+class ProbabilisticPicker {
+ public:
+  int3 dimsref{}, dimsimage{}, dimslowpass{};
+  uint ndims = 0;
+  uint nrefs = 0;
+  bool ismaskcircular = true;
+  bool doctf = false;
+
+  ProbabilisticPicker() = default;
+  ProbabilisticPicker(const ProbabilisticPicker&) = delete;
+  ProbabilisticPicker& operator=(const ProbabilisticPicker&) = delete;
+  ~ProbabilisticPicker() { gtomx_prob_picker_destroy(handle_); }
+
+  void Initialize(tfloat* _d_refs, int3 _dimsref, uint _nrefs, tfloat* _d_refmasks, bool _ismaskcircular, bool _doctf,
+                  int3 _dimsimage, uint _lowpassfreq) {
+    gtomx_prob_picker_destroy(handle_);
+    handle_ = nullptr;
+    dimsref = _dimsref;
+    nrefs = _nrefs;
+    ndims = (uint)detail::ndims(_dimsref);
+    dimsimage = _dimsimage;
+    const int lp2 = (int)_lowpassfreq * 2;
+    dimslowpass = ndims == 3 ? make_int3(lp2, lp2, lp2) : make_int3(lp2, lp2, 1);
+    ismaskcircular = _ismaskcircular;
+    doctf = _doctf;
+    detail::check(gtomx_prob_picker_create(&handle_, _d_refs, detail::i3(_dimsref), _nrefs, _d_refmasks,
+                                           _ismaskcircular ? 1 : 0, _doctf ? 1 : 0, detail::i3(_dimsimage),
+                                           _lowpassfreq, nullptr),
+                  "ProbabilisticPicker::Initialize");
+    detail::sync();
+  }
+  void SetImage(tfloat* _d_image, tfloat* _d_ctf) {
+    detail::check(gtomx_prob_picker_set_image(handle_, _d_image, _d_ctf, nullptr), "ProbabilisticPicker::SetImage");
+    detail::sync();
+  }
+  void PerformCorrelation(uint n, tfloat anglestep, tfloat* d_bestccf, tfloat3* d_bestangle, int* d_bestref) {
+    detail::check(gtomx_prob_picker_correlate(handle_, n, anglestep, d_bestccf,
+                                              reinterpret_cast<gtomx_float3*>(d_bestangle), d_bestref, nullptr),
+                  "ProbabilisticPicker::PerformCorrelation");
+    detail::sync();
+  }
+
+ private:
+  gtomx_prob_picker* handle_ = nullptr;
 };
 
 }  // namespace gtom

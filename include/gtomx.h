@@ -359,6 +359,45 @@ int gtomx_picker_correlate(gtomx_picker* picker, float anglestep, float* d_bestc
 int gtomx_picker_correlate_angles(gtomx_picker* picker, const gtomx_float3* h_angles, unsigned nangles,
                                   float* d_bestccf, gtomx_float3* d_bestangle, void* stream);
 
+typedef struct gtomx_prob_picker gtomx_prob_picker;
+
+/* ProbabilisticPicker::Initialize, include/Correlation.cuh:202 /
+ * src/Correlation/ProbabilisticPicker.cu:33-152: nrefs centred references
+ * (d_refs) and masks (d_refmasks) of dimsref for images of dimsimage (2D when
+ * dims.z == 1), band-limited to |k| < lowpassfreq and rotated at 2*lowpassfreq.
+ * Only circular masks (ismaskcircular != 0) are supported. The low-pass spectra
+ * are stored as the reference stores them, including what its half remap
+ * (src/FFT/FFTRemap.cu, RemapHalfFFT2HalfKernel) reads past the cropped spectrum
+ * for the zero-frequency row and plane; dimsimage must be large enough that
+ * those reads stay inside the padded reference (GTOMX_ERR_UNSUPPORTED
+ * otherwise). Everything stays in device memory. */
+int gtomx_prob_picker_create(gtomx_prob_picker** picker, const float* d_refs, gtomx_int3 dimsref, unsigned nrefs,
+                             const float* d_refmasks, int ismaskcircular, int doctf, gtomx_int3 dimsimage,
+                             unsigned lowpassfreq, void* stream);
+int gtomx_prob_picker_destroy(gtomx_prob_picker* picker);
+
+/* ProbabilisticPicker::SetImage, include/Correlation.cuh:204 /
+ * src/Correlation/ProbabilisticPicker.cu:205-224: the image normalised to mean 0
+ * / std 1 and its square, transformed and band-limited; d_ctf (half spectrum
+ * of dimsimage) cropped to the low-pass size. d_ctf may be NULL when the picker
+ * was created without a CTF. The caller's image is left unchanged. */
+int gtomx_prob_picker_set_image(gtomx_prob_picker* picker, const float* d_image, const float* d_ctf, void* stream);
+
+/* ProbabilisticPicker::PerformCorrelation, include/Correlation.cuh:205 /
+ * src/Correlation/ProbabilisticPicker.cu:226-389, for reference n: for every
+ * orientation of GetEqualAngularSpacing((0, 2pi), (0, pi), (0, 2pi), anglestep)
+ * (3D) or of ceil(2pi/anglestep) in-plane steps (2D), the probability-ratio
+ * score of ProbPickingUpdateKernel (:391-421); where it exceeds d_bestccf,
+ * d_bestccf, d_bestangle and d_bestref (= n) are replaced. Calls srand(123) and
+ * draws the reference's gaussrand() noise from the C runtime's rand(), whose
+ * state, like gaussrand's own, is shared with the rest of the process. The
+ * caller initialises the three outputs (dimsimage elements each). */
+int gtomx_prob_picker_correlate(gtomx_prob_picker* picker, unsigned n, float anglestep, float* d_bestccf,
+                                gtomx_float3* d_bestangle, int* d_bestref, void* stream);
+/* Number of orientations PerformCorrelation visits for ndims (2 or 3) and
+ * anglestep (no reference counterpart; the list is the reference's). */
+int gtomx_prob_picker_count_angles(int ndims, float anglestep, int* count);
+
 /* d_Rotate3DFT with T_INTERP_LINEAR, include/Transformation.cuh:54 /
  * src/Transformation/Rotation.cu:263-374 (Rotate3DFTKernel :541-654): each of
  * nangles rotations of the half spectrum of a cube of side n. Input in the
@@ -377,6 +416,17 @@ int gtomx_rotate3d_ft(const gtomx_complex* d_volume, gtomx_complex* d_output, gt
  * kernel :606-654): no conjugation. */
 int gtomx_rotate3d_ft_real(const float* d_volume, float* d_output, gtomx_int3 dims, const gtomx_float3* h_angles,
                            unsigned nangles, int outputzerocentered, void* stream);
+
+/* d_Rotate2DFT with T_INTERP_LINEAR, include/Transformation.cuh:52 /
+ * src/Transformation/Rotation.cu:200-256 (kernel :485-539): `batch` zero-centred
+ * half spectra of dims.x x dims.y images (dims.z == 1), item b rotated by
+ * h_angles[b] (radians): output bin (idx, idy) takes the input at R(-angle)
+ * (idx - dims.x/2, idy - dims.y/2), 0 beyond maxfreq, read at the Hermitian mate
+ * (imaginary part negated) when its x > 1e-5; written in place or, when
+ * outputzerocentered is 0, at the FFT-order bin (dims.x/2 - idx, dims.y-1 -
+ * ((idy + dims.y/2 - 1) % dims.y)). Exact bilinear weights, clamped indices. */
+int gtomx_rotate2d_ft(const gtomx_complex* d_input, gtomx_complex* d_output, gtomx_int3 dims, const float* h_angles,
+                      float maxfreq, int outputzerocentered, unsigned batch, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Peak extraction (the consumer of the search output)                       */

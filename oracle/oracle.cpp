@@ -19,6 +19,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #ifdef _OPENMP
@@ -757,12 +758,19 @@ void rotate3d_ft(const double* in, double* out, int n, const double* M, bool zer
         const double cx = x - n / 2, cy = y - n / 2, cz = z - n / 2;
         double p[3];
         for (int r = 0; r < 3; ++r) p[r] = M[3 * r] * cx + M[3 * r + 1] * cy + M[3 * r + 2] * cz;
+        // the radius cut and the flip are decided in float, as the reference
+        // does, with the device's fixed evaluation order; the value is
+        // interpolated at the double position
+        float pf[3];
+        for (int r = 0; r < 3; ++r)
+          pf[r] = std::fma((float)M[3 * r], (float)cx, std::fma((float)M[3 * r + 1], (float)cy,
+                                                                (float)M[3 * r + 2] * (float)cz));
         double* o = out + (((size_t)idz * n + idy) * nh + idx) * ncomp;
-        if (p[0] * p[0] + p[1] * p[1] + p[2] * p[2] >= (double)(n * n / 4)) {
+        if (std::fma(pf[0], pf[0], std::fma(pf[1], pf[1], pf[2] * pf[2])) >= (float)(n * n / 4)) {
           for (int c = 0; c < ncomp; ++c) o[c] = 0;
           continue;
         }
-        const bool neg = p[0] > 1e-6;
+        const bool neg = pf[0] > 1e-6f;
         double t[3];
         for (int r = 0; r < 3; ++r) t[r] = (neg ? -p[r] : p[r]) + n / 2;
         const long f[3] = {(long)std::floor(t[0]), (long)std::floor(t[1]), (long)std::floor(t[2])};
@@ -1109,6 +1117,279 @@ void subpixel_max(const float* in, double* out, int nx, int ny, int nz, int step
 
 }  // namespace
 
+extern "C" int orc_equal_angular_spacing(float phimin, float phimax, float thetamin, float thetamax, float psimin,
+                                         float psimax, float increment, float* out, int maxcount);
+
+namespace {
+
+// gaussrand (src/Helpers/Misc.cpp:14-31): Box-Muller pairs drawn from the C
+// runtime's rand(); U, V and the result are tfloat (float) and the C++ float
+// overloads of log, sqrt, sin and cos apply to them. The state persists for the
+// process, as the reference's function-scope statics do.
+struct OrcGaussRand {
+  float U = 0, V = 0;
+  int phase = 0;
+  float next() {
+    float Z;
+    if (phase == 0) {
+      U = (float)((std::rand() + 1.) / (RAND_MAX + 2.));
+      V = (float)(std::rand() / (RAND_MAX + 1.));
+      Z = std::sqrt(-2 * std::log(U)) * std::sin(2 * 3.1415926535897932384626433832795f * V);
+    } else {
+      Z = std::sqrt(-2 * std::log(U)) * std::cos(2 * 3.1415926535897932384626433832795f * V);
+    }
+    phase = 1 - phase;
+    return Z;
+  }
+};
+OrcGaussRand g_orc_gauss;
+
+// d_Rotate2DFT with linear interpolation (src/Transformation/Rotation.cu:235-256,
+// Rotate2DFTKernel :485-539) on a zero-centred complex half layout
+// ((nx/2+1) x ny): the position comes from the output thread's own (idx, idy),
+// pos = R(-angle) (idx - nx/2, idy - ny/2) with the matrix entries rounded to
+// float as glm::mat2 stores them (Matrix2Rotation, src/Helpers/Euler.cu:343-349);
+// 0 when |pos| > maxfreq; the Hermitian mate (imaginary part negated) when
+// pos.x > 1e-5; bilinear at texel pos + n/2 with indices clamped; written at
+// (idx, idy) or at the FFT-order bin (nx/2 - idx, ny-1 - ((idy + ny/2 - 1) % ny)).
+void rotate2d_ft(const double* in, double* out, int nx, int ny, double angle, double maxfreq, bool zerocentred) {
+  const int nh = nx / 2 + 1;
+  const double c = (float)std::cos(-angle), s = (float)std::sin(-angle);
+  auto clampi = [](long v, long hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  for (int idy = 0; idy < ny; ++idy)
+    for (int idx = 0; idx < nh; ++idx) {
+      int x = idx, y = idy;
+      if (!zerocentred) {
+        x = nx / 2 - idx;
+        y = ny - 1 - ((idy + ny / 2 - 1) % ny);
+      }
+      const double vx = idx - nx / 2, vy = idy - ny / 2;
+      double px = c * vx - s * vy, py = s * vx + c * vy;
+      // the radius cut and the flip in float (the reference's glm::vec2), in
+      // the device's fixed evaluation order; the value at the double position
+      const float qx = std::fma((float)c, (float)vx, -((float)s * (float)vy));
+      const float qy = std::fma((float)s, (float)vx, (float)c * (float)vy);
+      double* o = out + ((size_t)y * nh + x) * 2;
+      if (std::sqrt(std::fma(qx, qx, qy * qy)) > (float)maxfreq) {
+        o[0] = o[1] = 0;
+        continue;
+      }
+      const bool neg = qx > 1e-5f;
+      if (neg) {
+        px = -px;
+        py = -py;
+      }
+      const double tx = px + nx / 2, ty = py + ny / 2;
+      const long fx = (long)std::floor(tx), fy = (long)std::floor(ty);
+      const double ax = tx - fx, ay = ty - fy;
+      for (int k = 0; k < 2; ++k) {
+        double v = 0;
+        for (int q = 0; q < 4; ++q) {
+          const int bx = q & 1, by = q >> 1;
+          const double w = (bx ? ax : 1 - ax) * (by ? ay : 1 - ay);
+          v += w * in[((size_t)clampi(fy + by, ny - 1) * nh + clampi(fx + bx, nh - 1)) * 2 + k];
+        }
+        o[k] = (k == 1 && neg) ? -v : v;
+      }
+    }
+}
+
+// ProbabilisticPicker (src/Correlation/ProbabilisticPicker.cu) with a circular
+// mask, for one reference (index refid), in the reference's order:
+//  Initialize (:33-152): R = ifftshift(pad(ref, dimsimage, 0)) (:128-129),
+//   C = FFTCrop(R2C(R), 2*lowpass) (:131-132), C *= [|k| < lowpass] (d_Bandpass
+//   :136 with a hard sphere, src/ImageManipulation/Bandpass.cu:37-87),
+//   Hc = d_RemapHalfFFT2Half(C) as RemapHalfFFT2HalfKernel (src/FFT/FFTRemap.cu)
+//   reads it: row signed(i) + n/2 (n for the zero-frequency row and plane),
+//   complex index j >= ElementsFFT(2*lowpass) reading the float pair
+//   (R[2j], R[2j+1]) that the reference's buffer still holds there;
+//   M = ifftshift(pad(mask)) (:104-106), IM = R2C(ifftshift(pad(1 - mask))) (:95-100);
+//  SetImage (:205-224): image to mean 0 / std 1 (d_Norm), I = R2C(image) and
+//   I2 = R2C(image^2) band-limited, ctf cropped to 2*lowpass;
+//  PerformCorrelation (:226-389): srand(123); per orientation rot =
+//   Rotate{3D,2D}FT(Hc) in FFT order, *ctf, r = C2R(FFTPad(rot)) / N * M,
+//   corr = C2R_u(I * conj(R2C(r))); over the FFTFullCrop of r and M to dimsref,
+//   the host block (:306-340) with five gaussrand() draws per element;
+//   solvent mean/std at the first orientation (CalcSolventStatistics :423-454);
+//   the score of ProbPickingUpdateKernel (:391-421), kept where it exceeds best.
+// top2 (optional) receives the second-highest score over the orientations.
+void prob_picker(const double* ref, const double* mask, const int* dr, const int* di, int lowpass, bool doctf,
+                 const double* image, const double* ctf, float anglestep, int refid, double* bestccf,
+                 double* bestangle, int* bestref, double* top2) {
+  const bool three = di[2] > 1;
+  const int n = 2 * lowpass;
+  const int dl[3] = {n, n, three ? n : 1};
+  const size_t Vi = (size_t)di[0] * di[1] * di[2], Hi = (size_t)(di[0] / 2 + 1) * di[1] * di[2];
+  const size_t Hl = (size_t)(n / 2 + 1) * n * dl[2], Er = (size_t)dr[0] * dr[1] * dr[2];
+  auto pad = [&](const double* in) {
+    std::vector<double> out(Vi, 0.0);
+    const int off[3] = {dr[0] / 2 - di[0] / 2, dr[1] / 2 - di[1] / 2, dr[2] / 2 - di[2] / 2};
+    for (int z = 0; z < di[2]; ++z)
+      for (int y = 0; y < di[1]; ++y)
+        for (int x = 0; x < di[0]; ++x) {
+          const int ix = x + off[0], iy = y + off[1], iz = z + off[2];
+          if (ix >= 0 && ix < dr[0] && iy >= 0 && iy < dr[1] && iz >= 0 && iz < dr[2])
+            out[((size_t)z * di[1] + y) * di[0] + x] = in[((size_t)iz * dr[1] + iy) * dr[0] + ix];
+        }
+    return out;
+  };
+  auto shifted = [&](const std::vector<double>& a) {
+    std::vector<double> out(Vi);
+    remap_full2fullfft(a.data(), out.data(), di[0], di[1], di[2]);
+    return out;
+  };
+  auto r2c = [&](const std::vector<double>& r) {
+    std::vector<cpx<double>> f(Hi);
+    r2c3(r.data(), f.data(), di[0], di[1], di[2]);
+    return f;
+  };
+  auto c2r = [&](std::vector<cpx<double>> f) {
+    std::vector<double> r(Vi);
+    c2r3(f.data(), r.data(), di[0], di[1], di[2]);
+    return r;
+  };
+  auto sfreq = [](int i, int len) { return i < len / 2 + 1 ? i : i - len; };
+  auto bandpass = [&](cpx<double>* x, const int* d) {
+    const int nh = d[0] / 2 + 1;
+    for (int z = 0; z < d[2]; ++z)
+      for (int y = 0; y < d[1]; ++y)
+        for (int xx = 0; xx < nh; ++xx) {
+          const long long kz = sfreq(z, d[2]), ky = sfreq(y, d[1]);
+          if (!((long long)xx * xx + ky * ky + kz * kz < (long long)lowpass * lowpass))
+            x[((size_t)z * d[1] + y) * nh + xx] = 0;
+        }
+  };
+
+  // Initialize
+  const std::vector<double> R = shifted(pad(ref));
+  std::vector<cpx<double>> F = r2c(R);
+  std::vector<cpx<double>> C(Hl);
+  fft_resize(1, reinterpret_cast<const double*>(F.data()), reinterpret_cast<double*>(C.data()), di, dl, 2);
+  bandpass(C.data(), dl);
+  std::vector<double> Hc(2 * Hl);
+  {
+    const int nh = n / 2 + 1, nz = dl[2];
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < n; ++y)
+        for (int x = 0; x < nh; ++x) {
+          const int rz = (z < nz / 2 + 1 ? z : z - n) + nz / 2, ry = (y < n / 2 + 1 ? y : y - n) + n / 2;
+          const size_t j = ((size_t)rz * n + ry) * nh + x, o = ((size_t)z * n + y) * nh + x;
+          if (j < Hl) {
+            Hc[2 * o] = C[j].real();
+            Hc[2 * o + 1] = C[j].imag();
+          } else {
+            Hc[2 * o] = R[2 * j];
+            Hc[2 * o + 1] = R[2 * j + 1];
+          }
+        }
+  }
+  const std::vector<double> M = shifted(pad(mask));
+  std::vector<double> om(Er);
+  for (size_t i = 0; i < Er; ++i) om[i] = 1.0 - mask[i];
+  const std::vector<cpx<double>> IM = r2c(shifted(pad(om.data())));
+
+  // SetImage
+  std::vector<double> imn(Vi), sq(Vi);
+  norm_mean01std(image, imn.data(), (long long)Vi);
+  std::vector<cpx<double>> I = r2c(imn);
+  bandpass(I.data(), di);
+  for (size_t i = 0; i < Vi; ++i) sq[i] = imn[i] * imn[i];
+  std::vector<cpx<double>> I2 = r2c(sq);
+  bandpass(I2.data(), di);
+  std::vector<double> ctfl(Hl, 1.0);
+  if (doctf) fft_resize(1, ctf, ctfl.data(), di, dl, 1);
+
+  // PerformCorrelation
+  std::srand(123);
+  std::vector<float> ang;
+  if (three) {
+    const float PIf = 3.1415926535897932384626433832795f, PI2f = 6.283185307179586476925286766559f;
+    const int cnt = orc_equal_angular_spacing(0, PI2f, 0, PIf, 0, PI2f, anglestep, nullptr, 0);
+    ang.resize(3 * (size_t)cnt);
+    orc_equal_angular_spacing(0, PI2f, 0, PIf, 0, PI2f, anglestep, ang.data(), cnt);
+  } else {
+    const float PI2f = 6.283185307179586476925286766559f;
+    const int steps = (int)std::ceil(PI2f / anglestep);
+    const float step = PI2f / (float)steps;
+    for (int a = 0; a < steps; ++a) {
+      ang.push_back(0.f);
+      ang.push_back(0.f);
+      ang.push_back((float)a * step);
+    }
+  }
+  std::vector<double> smean(Vi), sstd(Vi);
+  std::vector<double> rot(2 * Hl);
+  std::vector<cpx<double>> P(Hi);
+  for (size_t a = 0; a < ang.size() / 3; ++a) {
+    if (three) {
+      double Mr[9];
+      projector_matrix(ang[3 * a], ang[3 * a + 1], ang[3 * a + 2], 1.0, Mr);
+      rotate3d_ft(Hc.data(), rot.data(), n, Mr, false, 2);
+    } else {
+      rotate2d_ft(Hc.data(), rot.data(), n, n, ang[3 * a + 2], (double)(float)(n / 2), false);
+    }
+    if (doctf)
+      for (size_t i = 0; i < Hl; ++i) {
+        rot[2 * i] *= ctfl[i];
+        rot[2 * i + 1] *= ctfl[i];
+      }
+    fft_resize(2, rot.data(), reinterpret_cast<double*>(P.data()), dl, di, 2);
+    std::vector<double> r = c2r(P);
+    for (size_t i = 0; i < Vi; ++i) r[i] = r[i] / (double)Vi * M[i];
+    std::vector<cpx<double>> B1 = r2c(r);
+    for (size_t i = 0; i < Hi; ++i) B1[i] = I[i] * std::conj(B1[i]);
+    const std::vector<double> corr = c2r(B1);
+    std::vector<double> refc(Er), maskc(Er);
+    fft_resize(3, r.data(), refc.data(), di, dr, 1);
+    fft_resize(3, M.data(), maskc.data(), di, dr, 1);
+    double smr = 0, smr2 = 0, samples = 0, epr[5] = {0, 0, 0, 0, 0};
+    for (size_t i = 0; i < Er; ++i) {
+      samples += maskc[i];
+      const double rs = refc[i], rs2 = rs * rs;
+      for (int k = 0; k < 5; ++k) epr[k] += rs2 + 2.0 * rs * (double)g_orc_gauss.next();
+      smr += rs;
+      smr2 += rs2;
+    }
+    smr /= samples;
+    smr2 /= samples;
+    double ep = 0;
+    for (int k = 0; k < 5; ++k) ep += epr[k];
+    ep /= 5.0;
+    ep = std::exp(ep / (2.0 * samples));
+    if (a == 0) {
+      const double f = 1.0 / ((double)Vi * ((double)Er - samples));
+      std::vector<cpx<double>> t(Hi);
+      for (size_t i = 0; i < Hi; ++i) t[i] = I[i] * std::conj(IM[i]);
+      smean = c2r(t);
+      for (size_t i = 0; i < Hi; ++i) t[i] = I2[i] * std::conj(IM[i]);
+      sstd = c2r(t);
+      for (size_t i = 0; i < Vi; ++i) {
+        smean[i] *= f;
+        sstd[i] = std::sqrt(std::max(0.0, sstd[i] * f - smean[i] * smean[i]));
+      }
+    }
+    const double nf = 1.0 / ((double)Vi * samples);
+    for (size_t i = 0; i < Vi; ++i) {
+      double d = -2.0 * nf * corr[i];
+      d += 2.0 * smean[i] * smr;
+      d /= std::max(1e-7, sstd[i]);
+      d += smr2;
+      d = std::exp(-d / 2.0);
+      d = (d - 1.0) / (ep - 1.0);
+      if (d > bestccf[i]) {
+        if (top2) top2[i] = std::max(top2[i], bestccf[i]);
+        bestccf[i] = d;
+        for (int k = 0; k < 3; ++k) bestangle[3 * i + k] = ang[3 * a + k];
+        bestref[i] = refid;
+      } else if (top2) {
+        top2[i] = std::max(top2[i], d);
+      }
+    }
+  }
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI for ctypes. Complex arrays are interleaved (re, im).
 // ---------------------------------------------------------------------------
@@ -1381,6 +1662,25 @@ void orc_peak_one_2d_f64(const float* in, int nx, int ny, int rx, int ry, int su
 
 void orc_subpixel_max_f64(const float* in, double* out, int nx, int ny, int nz, int steps) {
   subpixel_max(in, out, nx, ny, nz, steps);
+}
+
+void orc_rotate2d_ft_f64(const double* in, double* out, int nx, int ny, double angle, double maxfreq,
+                         int zerocentred) {
+  rotate2d_ft(in, out, nx, ny, angle, maxfreq, zerocentred != 0);
+}
+
+void orc_prob_picker_f64(const double* ref, const double* mask, const int* dimsref, const int* dimsimage,
+                         int lowpass, int doctf, const double* image, const double* ctf, float anglestep, int refid,
+                         double* bestccf, double* bestangle, int* bestref, double* top2) {
+  prob_picker(ref, mask, dimsref, dimsimage, lowpass, doctf != 0, image, ctf, anglestep, refid, bestccf, bestangle,
+              bestref, top2);
+}
+
+// The oracle's own gaussrand stream: srand(seed), reset of its phase, n draws.
+void orc_gaussrand_f32(unsigned seed, int reset, float* out, int n) {
+  std::srand(seed);
+  if (reset) g_orc_gauss = OrcGaussRand();
+  for (int i = 0; i < n; ++i) out[i] = g_orc_gauss.next();
 }
 
 }  // extern "C"

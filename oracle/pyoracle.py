@@ -64,6 +64,9 @@ def lib():
         L.orc_pick2d_f64.argtypes = pick_args
         L.orc_pick2d_f64.restype = _d
         L.orc_project2d_f64.argtypes = [_p, _i, _p, _i, _p, _i]
+        L.orc_rotate2d_ft_f64.argtypes = [_p, _p, _i, _i, _d, _d, _i]
+        L.orc_prob_picker_f64.argtypes = [_p, _p, _p, _p, _i, _i, _p, _p, _f, _i, _p, _p, _p, _p]
+        L.orc_gaussrand_f32.argtypes = [ctypes.c_uint, _i, _p, _i]
         _lib = L
     return _lib
 
@@ -437,3 +440,41 @@ def pick(projft, p, oversample, expft, ctf, n, angles, maskradius, ctf_per_volum
               _ptr(A), A.shape[0], maskradius, _ptr(best), _ptr(idx), _ptr(top1), _ptr(top2),
               _ptr(sigma))
     return dict(best=best, idx=idx, top1=top1, top2=top2, sigma=sigma, loop_seconds=secs)
+
+
+def rotate2d_ft(img, angle, maxfreq, zerocentred=True):
+    """d_Rotate2DFT (linear) of one zero-centred complex half spectrum (ny, nx/2+1)."""
+    v = np.ascontiguousarray(img, np.complex128)
+    ny, nh = v.shape
+    out = np.empty_like(v)
+    lib().orc_rotate2d_ft_f64(_ptr(v.view(np.float64)), _ptr(out.view(np.float64)), 2 * (nh - 1), ny, float(angle),
+                              float(maxfreq), int(zerocentred))
+    return out
+
+
+def gaussrand(n, seed=123, reset=True):
+    """n draws of the oracle's gaussrand after srand(seed); reset restarts its pair phase."""
+    out = np.empty(n, np.float32)
+    lib().orc_gaussrand_f32(seed, int(reset), _ptr(out), n)
+    return out
+
+
+def prob_picker(ref, mask, image, lowpass, anglestep, ctf=None, refid=0, best=None):
+    """ProbabilisticPicker with a circular mask, one reference: returns (best,
+    bestangle, bestref, top2), arrays of the image shape (bestangle + (3,)).
+    Starts from best (default -1e30 everywhere)."""
+    r = _c(ref, np.float64)
+    m = _c(mask, np.float64)
+    im = _c(image, np.float64)
+    nd = im.ndim
+    dr = np.array(list(r.shape[::-1]) + ([1] if nd == 2 else []), np.int32)
+    di = np.array(list(im.shape[::-1]) + ([1] if nd == 2 else []), np.int32)
+    b = np.full(im.shape, -1e30) if best is None else _c(best, np.float64).copy()
+    ang = np.zeros(im.shape + (3,))
+    bref = np.zeros(im.shape, np.int32)
+    top2 = np.full(im.shape, -np.inf)
+    c = None if ctf is None else _c(ctf, np.float64)
+    lib().orc_prob_picker_f64(_ptr(r), _ptr(m), _ptr(dr), _ptr(di), int(lowpass), int(ctf is not None), _ptr(im),
+                              None if c is None else _ptr(c), float(anglestep), int(refid), _ptr(b), _ptr(ang),
+                              _ptr(bref), _ptr(top2))
+    return b, ang, bref, top2

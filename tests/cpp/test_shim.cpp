@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
+#include <stdexcept>
 #include <vector>
 
 #include "gtom_pick.hpp"
@@ -201,6 +203,87 @@ static void picker_finds_copy() {
   hipFree(d_ref); hipFree(d_image); hipFree(d_ctf); hipFree(d_best); hipFree(d_angle);
 }
 
+// ProbabilisticPicker: the class passes its arguments through unchanged, so two
+// references correlated through it equal the C entry points called directly
+// (bit for bit: rand() is reseeded by every call).
+static void prob_picker_matches_c_abi() {
+  const int n = 48, nr = 12, nrefs = 2, lp = 6;
+  const size_t V = (size_t)n * n, E = (size_t)nr * nr;
+  std::mt19937 rng(31);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  std::vector<float> refs(E * nrefs), masks(E * nrefs), image(V);
+  for (auto& v : refs) v = nd(rng);
+  for (int r = 0; r < nrefs; ++r)
+    for (int y = 0; y < nr; ++y)
+      for (int x = 0; x < nr; ++x)
+        masks[r * E + y * nr + x] = std::hypot(x - nr / 2.0, y - nr / 2.0) < 5.0 ? 1.f : 0.f;
+  for (auto& v : image) v = nd(rng);
+  float* d_refs = dev<float>(refs.size());
+  float* d_masks = dev<float>(masks.size());
+  float* d_image = dev<float>(V);
+  hipMemcpy(d_refs, refs.data(), refs.size() * sizeof(float), hipMemcpyHostToDevice);
+  hipMemcpy(d_masks, masks.data(), masks.size() * sizeof(float), hipMemcpyHostToDevice);
+  hipMemcpy(d_image, image.data(), V * sizeof(float), hipMemcpyHostToDevice);
+  std::vector<float> init(V, -1e30f);
+  float* best[2];
+  tfloat3* ang[2];
+  int* ref[2];
+  for (int k = 0; k < 2; ++k) {
+    best[k] = dev<float>(V);
+    ang[k] = dev<tfloat3>(V);
+    ref[k] = dev<int>(V);
+    hipMemcpy(best[k], init.data(), V * sizeof(float), hipMemcpyHostToDevice);
+    hipMemset(ang[k], 0, V * sizeof(tfloat3));
+    hipMemset(ref[k], 0xff, V * sizeof(int));
+  }
+  {
+    ProbabilisticPicker pp;
+    pp.Initialize(d_refs, make_int3(nr, nr, 1), nrefs, d_masks, true, false, make_int3(n, n, 1), lp);
+    EXPECT(pp.dimslowpass.x == 2 * lp && pp.dimslowpass.z == 1 && pp.ndims == 2, "ProbabilisticPicker dims");
+    pp.SetImage(d_image, nullptr);
+    for (uint r = 0; r < (uint)nrefs; ++r) pp.PerformCorrelation(r, 0.6f, best[0], ang[0], ref[0]);
+  }
+  gtomx_prob_picker* h = nullptr;
+  EXPECT(gtomx_prob_picker_create(&h, d_refs, gtomx_int3{nr, nr, 1}, nrefs, d_masks, 1, 0, gtomx_int3{n, n, 1}, lp,
+                                  nullptr) == GTOMX_OK, "gtomx_prob_picker_create: %s", gtomx_last_error());
+  EXPECT(gtomx_prob_picker_set_image(h, d_image, nullptr, nullptr) == GTOMX_OK, "set_image");
+  for (unsigned r = 0; r < (unsigned)nrefs; ++r)
+    EXPECT(gtomx_prob_picker_correlate(h, r, 0.6f, best[1], reinterpret_cast<gtomx_float3*>(ang[1]), ref[1],
+                                       nullptr) == GTOMX_OK, "correlate");
+  gtomx_prob_picker_destroy(h);
+  std::vector<float> b0(V), b1(V);
+  std::vector<int> r0(V), r1(V);
+  std::vector<tfloat3> a0(V), a1(V);
+  hipMemcpy(b0.data(), best[0], V * sizeof(float), hipMemcpyDeviceToHost);
+  hipMemcpy(b1.data(), best[1], V * sizeof(float), hipMemcpyDeviceToHost);
+  hipMemcpy(r0.data(), ref[0], V * sizeof(int), hipMemcpyDeviceToHost);
+  hipMemcpy(r1.data(), ref[1], V * sizeof(int), hipMemcpyDeviceToHost);
+  hipMemcpy(a0.data(), ang[0], V * sizeof(tfloat3), hipMemcpyDeviceToHost);
+  hipMemcpy(a1.data(), ang[1], V * sizeof(tfloat3), hipMemcpyDeviceToHost);
+  EXPECT(std::memcmp(b0.data(), b1.data(), V * sizeof(float)) == 0, "ProbabilisticPicker scores differ");
+  EXPECT(r0 == r1, "ProbabilisticPicker reference indices differ");
+  EXPECT(std::memcmp(a0.data(), a1.data(), V * sizeof(tfloat3)) == 0, "ProbabilisticPicker angles differ");
+  int seen[2] = {0, 0};
+  for (int v : r0) {
+    EXPECT(v == 0 || v == 1, "reference index %d", v);
+    seen[v]++;
+  }
+  EXPECT(seen[0] > 0 && seen[1] > 0, "both references win somewhere (%d, %d)", seen[0], seen[1]);
+  bool caught = false;
+  try {
+    ProbabilisticPicker bad;
+    bad.Initialize(d_refs, make_int3(nr, nr, 1), 1, d_masks, false, false, make_int3(n, n, 1), lp);
+  } catch (const std::runtime_error&) {
+    caught = true;
+  }
+  EXPECT(caught, "a non-circular mask throws");
+  std::printf("ok ProbabilisticPicker (class = C ABI; references won %d / %d)\n", seen[0], seen[1]);
+  for (int k = 0; k < 2; ++k) {
+    hipFree(best[k]); hipFree(ang[k]); hipFree(ref[k]);
+  }
+  hipFree(d_refs); hipFree(d_masks); hipFree(d_image);
+}
+
 // MRC: a device volume written with d_WriteMRC reads back with ReadMRC, and
 // MixedToDeviceTfloat brings it to the device unchanged.
 static void mrc_round_trip() {
@@ -276,6 +359,7 @@ int main() {
   fft_roundtrip();
   pick_finds_shift();
   picker_finds_copy();
+  prob_picker_matches_c_abi();
   mrc_round_trip();
   ctf_simulate_known_answers();
   errors_throw();

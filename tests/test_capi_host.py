@@ -100,3 +100,39 @@ def test_validation_errors_without_device_work(gx):
     ctf = gx.CtfParams.make(pixelsize=0.0)
     assert L.gtomx_ctf_3d(ctypes.byref(ctf), 1, i3(16, 16, 16), ctypes.c_void_p(16), None) == \
         gx.ERR_INVALID_ARGUMENT
+
+
+def test_prob_picker_host_paths(gx, orc):
+    """The orientation count is the reference's list; creation rejects what it
+    cannot run before touching the device."""
+    L = gx.lib()
+    i3 = gx.int3
+    c = ctypes.c_int()
+    for step in (0.5, 0.3, 1.0):
+        assert L.gtomx_prob_picker_count_angles(2, step, ctypes.byref(c)) == gx.OK
+        assert c.value == int(np.ceil(np.float32(2 * np.pi) / np.float32(step)))
+    for step in (1.2, 0.5):
+        assert L.gtomx_prob_picker_count_angles(3, step, ctypes.byref(c)) == gx.OK
+        assert c.value == len(orc.equal_angular_spacing((0, 2 * np.pi), (0, np.pi), (0, 2 * np.pi), step))
+    assert L.gtomx_prob_picker_count_angles(3, 1.2, ctypes.byref(c)) == gx.OK and c.value == 50
+    assert L.gtomx_prob_picker_count_angles(4, 1.0, ctypes.byref(c)) == gx.ERR_INVALID_ARGUMENT
+    assert L.gtomx_prob_picker_count_angles(2, 0.0, ctypes.byref(c)) == gx.ERR_INVALID_ARGUMENT
+    h = ctypes.c_void_p()
+    fake = ctypes.c_void_p(256)
+    # non-circular masks
+    assert L.gtomx_prob_picker_create(ctypes.byref(h), fake, i3(16, 16, 1), 1, fake, 0, 0, i3(64, 64, 1), 8, None) \
+        == gx.ERR_UNSUPPORTED
+    assert "circular" in gx.last_error()
+    # an image too small next to the low-pass box: the remap would read unwritten memory
+    assert L.gtomx_prob_picker_create(ctypes.byref(h), fake, i3(8, 8, 1), 1, fake, 1, 0, i3(16, 16, 1), 8, None) \
+        == gx.ERR_UNSUPPORTED
+    # low-pass box larger than the image, reference larger than the image, mixed dimensionality
+    assert L.gtomx_prob_picker_create(ctypes.byref(h), fake, i3(8, 8, 1), 1, fake, 1, 0, i3(32, 32, 1), 20, None) \
+        == gx.ERR_INVALID_ARGUMENT
+    assert L.gtomx_prob_picker_create(ctypes.byref(h), fake, i3(40, 40, 1), 1, fake, 1, 0, i3(32, 32, 1), 4, None) \
+        == gx.ERR_INVALID_ARGUMENT
+    assert L.gtomx_prob_picker_create(ctypes.byref(h), fake, i3(8, 8, 8), 1, fake, 1, 0, i3(32, 32, 1), 4, None) \
+        == gx.ERR_INVALID_ARGUMENT
+    assert L.gtomx_prob_picker_create(ctypes.byref(h), None, i3(8, 8, 1), 1, fake, 1, 0, i3(32, 32, 1), 4, None) \
+        == gx.ERR_INVALID_ARGUMENT
+    assert L.gtomx_prob_picker_destroy(None) == gx.OK
