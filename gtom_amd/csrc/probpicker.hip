@@ -19,9 +19,12 @@
 //   sums over the FFTFullCrop of r and M_n to dimsref (in the crop's order):
 //   samples = sum m, mean = sum r / samples, mean2 = sum r^2 / samples,
 //   ratio = exp((sum_i 5 r_i^2 + 2 r_i G_i) / 5 / (2 samples)) with G_i the sum
-//   of the five gaussrand() draws of element i (src/Helpers/Misc.cpp:14-31, the
-//   C runtime's rand() after srand(123), drawn on the host in the reference's
-//   order while the device works on the previous batch);
+//   of the five gaussrand() draws of element i (src/Helpers/Misc.cpp:14-31) in
+//   the reference's order. The C runtime's rand() integers after srand(123) are
+//   produced on the host (glibc's additive generator, restated and checked
+//   against rand() itself; rand() is called directly where they differ) and
+//   uploaded per batch while the device works on the previous one; the
+//   Box-Muller transform runs on the device inside the reduction;
 //   solvent mean/std from I, I2 and IM_n once per call (CalcSolventStatistics,
 //   :423-441, ProbSolventStatisticsKernel :443-454);
 //   score (ProbPickingUpdateKernel :391-421) in double, kept where it beats the
@@ -112,11 +115,40 @@ __global__ __launch_bounds__(256) void k_square_into(const float* __restrict__ i
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) out[i] = in[i] * in[i];
 }
 
+// gaussrand draws of one call: draw g (from the call's first) has t = g + p0
+// (p0 the pair phase the call starts in); pair q = t / 2 gives sin (t even) or
+// cos (t odd) of the Box-Muller pair (U, V). Pair q < p0 is the one a previous
+// call left half used (its U, V passed in); pair q >= p0 takes the rand()
+// integers raw[q - qlo], U = (a + 1) / (RAND_MAX + 2), V = b / (RAND_MAX + 1),
+// rounded to float, Z = sqrt(-2 log U) * {sin, cos}(2 pi V) in float.
+struct Draws {
+  const int2* raw;
+  long long qlo;
+  int p0;
+  float Us, Vs;
+  double rmax;
+};
+
+__device__ __forceinline__ float gauss_draw(const Draws& d, long long g) {
+  const long long t = g + d.p0;
+  const long long q = t >> 1;
+  float U = d.Us, V = d.Vs;
+  if (q >= d.p0) {
+    const int2 w = d.raw[q - d.qlo];
+    U = (float)(((double)w.x + 1.) / (d.rmax + 2.));
+    V = (float)((double)w.y / (d.rmax + 1.));
+  }
+  const float m = sqrtf(-2 * logf(U));
+  const float a = 2 * 3.1415926535897932384626433832795f * V;
+  return (t & 1) ? m * cosf(a) : m * sinf(a);
+}
+
 // Sums over the FFTFullCrop (src/FFT/FFTResize.cu:140-160) of r and the mask to
-// dimsref, element i of the crop paired with noise G[b][i]:
+// dimsref, element i of the crop of orientation b weighted by the sum of its
+// five draws (draw index (g0 + b * er + i) * 5 + k, g0 = first orientation * er):
 // part[b][blk] = (sum m, sum r, sum r^2, sum r*G).
 __global__ __launch_bounds__(256) void k_prob_sums(const float* __restrict__ r, const float* __restrict__ mask,
-                                                   const float* __restrict__ G, int3 di, int3 dr, size_t vimage,
+                                                   Draws draws, long long g0, int3 di, int3 dr, size_t vimage,
                                                    size_t er, double4* __restrict__ part) {
   __shared__ double4 red[4];
   const unsigned b = blockIdx.y;
@@ -127,10 +159,13 @@ __global__ __launch_bounds__(256) void k_prob_sums(const float* __restrict__ r, 
               sz = z >= dr.z / 2 ? z + di.z - dr.z : z;
     const size_t src = ((size_t)sz * di.y + sy) * di.x + sx;
     const double rv = r[(size_t)b * vimage + src];
+    const long long gbase = (g0 + (long long)b * (long long)er + (long long)i) * 5;
+    double G = 0;
+    for (int k = 0; k < 5; ++k) G += (double)gauss_draw(draws, gbase + k);
     sm += mask[src];
     sr += rv;
     srr += rv * rv;
-    srg += rv * (double)G[(size_t)b * er + i];
+    srg += rv * G;
   }
   for (int o = 32; o > 0; o >>= 1) {
     sm += __shfl_down(sm, o);
@@ -224,27 +259,60 @@ __global__ __launch_bounds__(256) void k_prob_update(const float* __restrict__ c
 
 inline unsigned grid_of(size_t n, size_t cap = 16384) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, cap)); }
 
-// gaussrand (src/Helpers/Misc.cpp:14-31): Box-Muller pairs from the C runtime's
-// rand(), in float as the reference's tfloat statics and float arguments make
-// it; the state lives for the process and is shared by every picker, as there.
-struct GaussRand {
+// The pair state of gaussrand (src/Helpers/Misc.cpp:14-31): its function-scope
+// statics U, V (tfloat) and phase live for the process and are shared by every
+// picker, as there.
+struct GaussState {
   float U = 0, V = 0;
   int phase = 0;
-  float next() {
-    float Z;
-    if (phase == 0) {
-      U = (float)((std::rand() + 1.) / (RAND_MAX + 2.));
-      V = (float)(std::rand() / (RAND_MAX + 1.));
-      Z = std::sqrt(-2 * std::log(U)) * std::sin(2 * 3.1415926535897932384626433832795f * V);
-    } else {
-      Z = std::sqrt(-2 * std::log(U)) * std::cos(2 * 3.1415926535897932384626433832795f * V);
+};
+GaussState g_gauss;
+std::mutex g_gauss_mutex;
+
+// glibc's rand(): the additive generator of random_r with a 31-word state
+// seeded by srandom_r (state[0] = seed, state[i] = 16807 state[i-1] mod
+// 2^31 - 1 by Schrage's method, 310 outputs discarded), each output the sum
+// of the words 31 and 3 back, shifted right by one.
+struct AdditiveRand {
+  uint32_t s[31];
+  int f = 3, r = 0;
+  void seed(unsigned seed) {
+    int32_t word = seed == 0 ? 1 : (int32_t)seed;
+    s[0] = (uint32_t)word;
+    for (int i = 1; i < 31; ++i) {
+      const long long hi = word / 127773, lo = word % 127773;
+      long long w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      word = (int32_t)w;
+      s[i] = (uint32_t)word;
     }
-    phase = 1 - phase;
-    return Z;
+    f = 3;
+    r = 0;
+    for (int i = 0; i < 310; ++i) next();
+  }
+  int next() {
+    s[f] += s[r];
+    const int out = (int)(s[f] >> 1);
+    if (++f == 31) f = 0;
+    if (++r == 31) r = 0;
+    return out;
   }
 };
-GaussRand g_gauss;
-std::mutex g_gauss_mutex;
+
+// Whether the C runtime's rand() is that generator: compared over 1000
+// outputs after srand(123) (the only seed used), once per process.
+bool runtime_rand_is_additive() {
+  static int known = -1;
+  if (known < 0) {
+    AdditiveRand g;
+    g.seed(123);
+    std::srand(123);
+    bool same = RAND_MAX == 2147483647;
+    for (int i = 0; i < 1000 && same; ++i) same = std::rand() == g.next();
+    known = same ? 1 : 0;
+  }
+  return known == 1;
+}
 
 }  // namespace
 }  // namespace gx
@@ -268,13 +336,14 @@ struct gtomx_prob_picker {
   float* real = nullptr;         // B x Vi: r, then the correlation maps
   float* smean = nullptr;        // Vi
   float* sstd = nullptr;         // Vi
-  float* noise = nullptr;        // B x Er
+  int2* raw = nullptr;           // rand() pairs of one batch
   double4* part = nullptr;       // B x NBS
   gx::ProbCoef* coef = nullptr;  // B
   float* mats = nullptr;         // B x 9 (3D) or B x 2 (2D)
   float3* dang = nullptr;        // B
   double* nscr = nullptr;
-  float* h_noise[2] = {nullptr, nullptr};  // pinned, B x Er each
+  int2* h_raw[2] = {nullptr, nullptr};     // pinned rand() pairs of one batch each
+  size_t raw_cap = 0;                      // pairs per batch buffer
   float* h_stage[2] = {nullptr, nullptr};  // pinned, B x 12 floats each: matrices then angles
   hipEvent_t uploaded[2] = {nullptr, nullptr};
   bool have_image = false;
@@ -287,11 +356,11 @@ void prob_release(gtomx_prob_picker* pp) {
   if (!pp) return;
   for (void* p : {(void*)pp->refs_low, (void*)pp->masks, (void*)pp->invmasksft, (void*)pp->imageft,
                   (void*)pp->image2ft, (void*)pp->ctf_low, (void*)pp->rot, (void*)pp->padft, (void*)pp->real,
-                  (void*)pp->smean, (void*)pp->sstd, (void*)pp->noise, (void*)pp->part, (void*)pp->coef,
+                  (void*)pp->smean, (void*)pp->sstd, (void*)pp->raw, (void*)pp->part, (void*)pp->coef,
                   (void*)pp->mats, (void*)pp->dang, (void*)pp->nscr})
     dev_free(p);
   for (int k = 0; k < 2; ++k) {
-    if (pp->h_noise[k]) (void)hipHostFree(pp->h_noise[k]);
+    if (pp->h_raw[k]) (void)hipHostFree(pp->h_raw[k]);
     if (pp->h_stage[k]) (void)hipHostFree(pp->h_stage[k]);
     if (pp->uploaded[k]) (void)hipEventDestroy(pp->uploaded[k]);
   }
@@ -386,7 +455,7 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   };
   st = hipGetDevice(&pp->device) == hipSuccess ? GTOMX_OK : fail(GTOMX_ERR_DEVICE, "hipGetDevice");
   // orientations per batch: about 2 GiB of per-orientation buffers, at most 32
-  const size_t per = Hl * 8 + pp->Hi * 8 + Vi * 4 + pp->Er * 4;
+  const size_t per = Hl * 8 + pp->Hi * 8 + Vi * 4 + pp->Er * 20;
   pp->B = (unsigned)std::max<size_t>(1, std::min<size_t>(32, ((size_t)2 << 30) / per));
   const unsigned B = pp->B;
   alloc((void**)&pp->refs_low, (size_t)nrefs * Hl * sizeof(float2));
@@ -400,14 +469,15 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   alloc((void**)&pp->real, (size_t)B * Vi * sizeof(float));
   alloc((void**)&pp->smean, Vi * sizeof(float));
   alloc((void**)&pp->sstd, Vi * sizeof(float));
-  alloc((void**)&pp->noise, (size_t)B * pp->Er * sizeof(float));
+  pp->raw_cap = ((size_t)B * pp->Er * 5 + 3) / 2 + 1;  // pairs touched by B orientations' draws
+  alloc((void**)&pp->raw, pp->raw_cap * sizeof(int2));
   alloc((void**)&pp->part, (size_t)B * NBS * sizeof(double4));
   alloc((void**)&pp->coef, (size_t)B * sizeof(ProbCoef));
   alloc((void**)&pp->mats, (size_t)B * 9 * sizeof(float));
   alloc((void**)&pp->dang, (size_t)B * sizeof(float3));
   alloc((void**)&pp->nscr, norm_scratch_doubles(1) * sizeof(double));
   for (int k = 0; k < 2 && st == GTOMX_OK; ++k) {
-    if (hipHostMalloc((void**)&pp->h_noise[k], (size_t)B * pp->Er * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc((void**)&pp->h_raw[k], pp->raw_cap * sizeof(int2), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&pp->h_stage[k], (size_t)B * 12 * sizeof(float), hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&pp->uploaded[k], hipEventDisableTiming) != hipSuccess)
       st = fail(GTOMX_ERR_OUT_OF_MEMORY, "pinned staging buffers");
@@ -517,20 +587,39 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
   const float2* invmaskft = pp->invmasksft + (size_t)n * Hi;
   const float maxfreq = (float)(pp->dl.x / 2);
   std::lock_guard<std::mutex> lock(g_gauss_mutex);
+  // rand() integers in draw order: the restated generator where it is the C
+  // runtime's, else rand() itself (reseeded as the reference does, :229)
+  const bool additive = runtime_rand_is_additive();
+  AdditiveRand gen;
+  gen.seed(123);
   std::srand(123);
+  auto next_int = [&]() { return additive ? gen.next() : std::rand(); };
+  const int p0 = g_gauss.phase;
+  const float Us = g_gauss.U, Vs = g_gauss.V;
+  long long pairs_made = 0;  // rand() pairs drawn so far (pair q uses pair index q - p0)
+  int2 last_pair = make_int2(0, 0);
   bool used[2] = {false, false};
   for (size_t a0 = 0, k = 0; a0 < angles.size(); a0 += B, ++k) {
     const unsigned nb = (unsigned)std::min<size_t>(B, angles.size() - a0);
     const int slot = (int)(k % 2);
     if (used[slot]) GX_HIP(hipEventSynchronize(pp->uploaded[slot]));
-    // noise of these orientations, in the reference's draw order
-    float* hn = pp->h_noise[slot];
-    for (unsigned b = 0; b < nb; ++b)
-      for (size_t i = 0; i < Er; ++i) {
-        double g = 0;
-        for (int r = 0; r < 5; ++r) g += (double)g_gauss.next();
-        hn[(size_t)b * Er + i] = (float)g;
+    // the pairs these orientations' draws touch: q in [qlo, qhi]
+    const long long gs = (long long)a0 * (long long)Er * 5, ge = (long long)(a0 + nb) * (long long)Er * 5;
+    const long long qlo = (gs + p0) / 2, qhi = (ge - 1 + p0) / 2;
+    int2* hr = pp->h_raw[slot];
+    for (long long q = qlo; q <= qhi; ++q) {
+      if (q < p0) {
+        hr[q - qlo] = make_int2(0, 0);  // the half-used pair of the previous call; U, V passed instead
+        continue;
       }
+      if (q - p0 >= pairs_made) {
+        const int a = next_int();
+        const int b2 = next_int();
+        last_pair = make_int2(a, b2);
+        ++pairs_made;
+      }
+      hr[q - qlo] = last_pair;  // a pair straddling two batches is repeated
+    }
     float* hs = pp->h_stage[slot];
     if (pp->ndims == 3) {
       GX_TRY(gtomx_projector_matrices(angles.data() + a0, nb, 1.0f, hs));
@@ -543,7 +632,7 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
       ha[3 * b + 1] = angles[a0 + b].y;
       ha[3 * b + 2] = angles[a0 + b].z;
     }
-    GX_HIP(hipMemcpyAsync(pp->noise, hn, (size_t)nb * Er * sizeof(float), hipMemcpyHostToDevice, s));
+    GX_HIP(hipMemcpyAsync(pp->raw, hr, (size_t)(qhi - qlo + 1) * sizeof(int2), hipMemcpyHostToDevice, s));
     const size_t nmat = (size_t)nb * (pp->ndims == 3 ? 9 : 2);
     GX_HIP(hipMemcpyAsync(pp->mats, hs, nmat * sizeof(float), hipMemcpyHostToDevice, s));
     GX_HIP(hipMemcpyAsync(pp->dang, ha, (size_t)nb * 3 * sizeof(float), hipMemcpyHostToDevice, s));
@@ -566,8 +655,9 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     GX_LAUNCHED("k_times_mask");
     // statistics of the cropped, masked reference
     const int nparts = (int)std::min<size_t>(NBS, (Er + 255) / 256);
-    hipLaunchKernelGGL(k_prob_sums, dim3(nparts, nb), dim3(256), 0, s, pp->real, mask, pp->noise, di, dr, Vi, Er,
-                       pp->part);
+    const Draws draws{pp->raw, qlo, p0, Us, Vs, (double)RAND_MAX};
+    hipLaunchKernelGGL(k_prob_sums, dim3(nparts, nb), dim3(256), 0, s, pp->real, mask, draws,
+                       (long long)a0 * (long long)Er, di, dr, Vi, Er, pp->part);
     GX_LAUNCHED("k_prob_sums");
     hipLaunchKernelGGL(k_prob_coef, dim3((nb + 63) / 64), dim3(64), 0, s, pp->part, nparts, nb, Vi, Er, pp->coef);
     GX_LAUNCHED("k_prob_coef");
@@ -593,6 +683,16 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     GX_LAUNCHED("k_prob_update");
   }
   GX_HIP(hipStreamSynchronize(s));
+  // the pair state left for the next call, as gaussrand's statics
+  const long long tend = (long long)angles.size() * (long long)Er * 5 + p0;
+  if (tend % 2 == 1) {
+    const long long q = (tend - 1) / 2;
+    if (q >= p0) {
+      g_gauss.U = (float)(((double)last_pair.x + 1.) / (RAND_MAX + 2.));
+      g_gauss.V = (float)((double)last_pair.y / (RAND_MAX + 1.));
+    }
+  }
+  g_gauss.phase = (int)(tend % 2);
   return GTOMX_OK;
 }
 

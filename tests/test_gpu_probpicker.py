@@ -128,3 +128,29 @@ def test_larger_images_match_oracle(gx, orc, cuda, nd):
     assert sel.mean() > 0.9
     assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
     assert np.all(to_np(r) == 0)
+
+
+def test_gaussrand_phase_carries_between_calls(gx, orc, cuda):
+    """5 * 225 * 13 draws per call is odd: the second call starts with the cos
+    half of the previous call's last pair, as gaussrand's statics do. The two
+    calls leave the phase even again."""
+    rng = np.random.default_rng(95)
+    di, dr, lp, step = (48, 48), (15, 15), 6, 0.5
+    refs = rng.standard_normal((2,) + dr).astype(np.float32)
+    masks = np.stack([_disc(dr, 6.0)] * 2)
+    image = rng.standard_normal(di).astype(np.float32)
+    b, a, r, nang = _run_gpu(gx, cuda, refs, masks, image, lp, step, None, 0)
+    assert (5 * refs[0].size * nang) % 2 == 1
+    b, a, r, _ = _run_gpu(gx, cuda, refs, masks, image, lp, step, None, 1, b, a, r)
+    orc.gaussrand(0, reset=True)
+    b0, _, _, _ = orc.prob_picker(refs[0].astype(np.float64), masks[0].astype(np.float64), image.astype(np.float64),
+                                  lp, step, refid=0)
+    b1, _, _, _ = orc.prob_picker(refs[1].astype(np.float64), masks[1].astype(np.float64), image.astype(np.float64),
+                                  lp, step, refid=1)
+    want = np.maximum(b0, b1)
+    assert rel_linf(to_np(b).astype(np.float64), want) <= 1e-4
+    # the second call with a fresh phase gives another answer
+    orc.gaussrand(0, reset=True)
+    b1f, _, _, _ = orc.prob_picker(refs[1].astype(np.float64), masks[1].astype(np.float64), image.astype(np.float64),
+                                   lp, step, refid=1)
+    assert rel_linf(b1f, b1) > 1e-3
