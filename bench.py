@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--row", choices=["prob_picker"], default=None,
+                    help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
 
 
@@ -119,6 +121,64 @@ def cpu_baseline(w, target_seconds):
                       f"orientation loop {r['loop_seconds']:.2f} s"}
 
 
+def row_prob_picker(args):
+    """ProbabilisticPicker, 3D: a 128^3 volume (mean 0 / std 1 noise with three
+    planted copies), references of 32^3 with a circular mask of radius 14,
+    low-pass 24 (rotation at 48^3), anglestep 0.35 rad (1,819 orientations of
+    GetEqualAngularSpacing). One step is PerformCorrelation for one reference:
+    all orientations, host rand() integers included (the call is synchronous)."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(args.seed)
+    di, dr, lp, step = (128, 128, 128), (32, 32, 32), 24, 0.35
+    g = np.meshgrid(*[np.arange(d) - d // 2 for d in dr], indexing="ij")
+    mask = (np.sqrt(sum(a.astype(float) ** 2 for a in g)) < 14).astype(np.float32)
+    refs = (rng.standard_normal((2,) + dr).astype(np.float32) * mask)
+    image = rng.standard_normal(di).astype(np.float32)
+    for c in ((40, 50, 60), (90, 30, 70), (64, 100, 20)):
+        image[tuple(slice(x - 16, x + 16) for x in c)] += refs[0][::-1, ::-1, ::-1]
+    dev = torch.device("cuda:0")
+    pp = gx.ProbabilisticPicker(torch.from_numpy(refs).to(dev), torch.from_numpy(np.stack([mask] * 2)).to(dev), di,
+                                lp)
+    pp.set_image(torch.from_numpy(image).to(dev))
+    nang = pp.count_angles(step)
+    best = torch.full(di, -1e30, device=dev)
+    ang = torch.zeros(di + (3,), device=dev)
+    ref = torch.zeros(di, dtype=torch.int32, device=dev)
+    for i in range(args.warmup):
+        pp.perform_correlation(i % 2, step, best, ang, ref)
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        pp.perform_correlation(i % 2, step, best, ang, ref)
+        log(f"step {i + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    pp.close()
+    vox = int(np.prod(di))
+    value = vox * nang / sec
+    out = {"row": "prob_picker", "metric": "image voxel-orientations/s, ProbabilisticPicker 3D", "value": value,
+           "unit": "voxel-orientations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": sec * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "ProbabilisticPicker.PerformCorrelation", "image": list(di), "ref": list(dr),
+                      "lowpassfreq": lp, "anglestep": step, "orientations": nang}}
+    if not args.no_cpu_baseline:
+        cstep = 1.2  # a bounded sample: the 50 orientations of anglestep 1.2 rad
+        ncpu = len(po.equal_angular_spacing((0, 2 * np.pi), (0, np.pi), (0, 2 * np.pi), cstep))
+        po.gaussrand(0, reset=True)
+        t0 = time.perf_counter()
+        po.prob_picker(refs[0].astype(np.float64), mask.astype(np.float64), image.astype(np.float64), lp, cstep)
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": vox * ncpu / ct, "unit": "voxel-orientations/s", "cores": po.num_threads(),
+                               "kind": "port",
+                               "sample": f"same volume and reference, the {ncpu} orientations of anglestep {cstep}, "
+                                         f"FP64 oracle restatement, {ct:.1f} s including its set-up"}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -128,6 +188,8 @@ def main():
     if args.copy_probe:
         copy_probe()
         return 0
+    if args.row == "prob_picker":
+        return row_prob_picker(args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth

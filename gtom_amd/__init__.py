@@ -130,6 +130,7 @@ _SIGS = {
     "gtomx_prob_picker_set_image": [_P, _P, _P, _P],
     "gtomx_prob_picker_correlate": [_P, _U, _F, _P, _P, _P, _P],
     "gtomx_prob_picker_count_angles": [_I, _F, ctypes.POINTER(_I)],
+    "gtomx_debug_rand_outputs": [_U, ctypes.c_longlong, _I, _P],
     "gtomx_rotate2d_ft": [_P, _P, int3, _P, _F, _I, _U, _P],
     "gtomx_pick_plan_create": [ctypes.POINTER(_P), int3, int3, _U],
     "gtomx_pick_plan_destroy": [_P],

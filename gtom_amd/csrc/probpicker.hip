@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "runtime.hpp"
@@ -270,32 +271,71 @@ GaussState g_gauss;
 std::mutex g_gauss_mutex;
 
 // glibc's rand(): the additive generator of random_r with a 31-word state
-// seeded by srandom_r (state[0] = seed, state[i] = 16807 state[i-1] mod
-// 2^31 - 1 by Schrage's method, 310 outputs discarded), each output the sum
-// of the words 31 and 3 back, shifted right by one.
-struct AdditiveRand {
-  uint32_t s[31];
-  int f = 3, r = 0;
-  void seed(unsigned seed) {
+// seeded by srandom_r (word 0 = seed, word i = 16807 * word i-1 mod 2^31 - 1 by
+// Schrage's method) and 310 outputs discarded. With r_i its words (r_0..r_30 from
+// the seeding, r_31..r_33 = r_0..r_2, r_i = r_(i-31) + r_(i-3) from i = 34), output
+// k is r_(k+344) >> 1, and every r_(3+e) is the combination of the words r_3..r_33
+// whose coefficients are those of x^e mod (x^31 - x^28 - 1) over Z/2^32, so any
+// stretch of the sequence can be produced on its own.
+struct AdditiveJump {
+  uint32_t base[31];  // r_3 .. r_33
+  explicit AdditiveJump(unsigned seed) {
+    uint32_t r[34];
     int32_t word = seed == 0 ? 1 : (int32_t)seed;
-    s[0] = (uint32_t)word;
+    r[0] = (uint32_t)word;
     for (int i = 1; i < 31; ++i) {
       const long long hi = word / 127773, lo = word % 127773;
       long long w = 16807 * lo - 2836 * hi;
       if (w < 0) w += 2147483647;
       word = (int32_t)w;
-      s[i] = (uint32_t)word;
+      r[i] = (uint32_t)word;
     }
-    f = 3;
-    r = 0;
-    for (int i = 0; i < 310; ++i) next();
+    for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+    for (int i = 0; i < 31; ++i) base[i] = r[3 + i];
   }
-  int next() {
-    s[f] += s[r];
-    const int out = (int)(s[f] >> 1);
-    if (++f == 31) f = 0;
-    if (++r == 31) r = 0;
-    return out;
+  static void mulmod(const uint32_t* a, const uint32_t* b, uint32_t* out) {
+    uint32_t c[61] = {0};
+    for (int i = 0; i < 31; ++i)
+      for (int j = 0; j < 31; ++j) c[i + j] += a[i] * b[j];
+    for (int d = 60; d >= 31; --d) {  // x^31 = x^28 + 1
+      c[d - 3] += c[d];
+      c[d - 31] += c[d];
+    }
+    for (int i = 0; i < 31; ++i) out[i] = c[i];
+  }
+  // Fills n outputs starting at output index k (k >= 0).
+  void outputs(long long k, long long n, int* out) const {
+    const long long m = k + 344 - 31;  // the first word of the window before output k
+    uint32_t res[31] = {1}, sq[31] = {0};
+    sq[1] = 1;  // x
+    for (long long e = m - 3; e > 0; e >>= 1) {
+      uint32_t t[31];
+      if (e & 1) {
+        mulmod(res, sq, t);
+        std::copy(t, t + 31, res);
+      }
+      mulmod(sq, sq, t);
+      std::copy(t, t + 31, sq);
+    }
+    uint32_t w[31];  // r_m .. r_(m+30)
+    uint32_t q[31];
+    std::copy(res, res + 31, q);
+    for (int j = 0; j < 31; ++j) {
+      uint32_t v = 0;
+      for (int i = 0; i < 31; ++i) v += q[i] * base[i];
+      w[j] = v;
+      const uint32_t top = q[30];  // q <- x q mod P
+      for (int i = 30; i > 0; --i) q[i] = q[i - 1];
+      q[0] = top;
+      q[28] += top;
+    }
+    int f = 0;  // w is a ring: the word 31 back sits at f, the word 3 back at f + 28
+    for (long long i = 0; i < n; ++i) {
+      const uint32_t v = w[f] + w[(f + 28) % 31];
+      w[f] = v;
+      out[i] = (int)(v >> 1);
+      if (++f == 31) f = 0;
+    }
   }
 };
 
@@ -304,11 +344,11 @@ struct AdditiveRand {
 bool runtime_rand_is_additive() {
   static int known = -1;
   if (known < 0) {
-    AdditiveRand g;
-    g.seed(123);
+    std::vector<int> mine(1000);
+    AdditiveJump(123).outputs(0, 1000, mine.data());
     std::srand(123);
     bool same = RAND_MAX == 2147483647;
-    for (int i = 0; i < 1000 && same; ++i) same = std::rand() == g.next();
+    for (int i = 0; i < 1000 && same; ++i) same = std::rand() == mine[(size_t)i];
     known = same ? 1 : 0;
   }
   return known == 1;
@@ -561,6 +601,11 @@ int gtomx_prob_picker_set_image(gtomx_prob_picker* pp, const float* d_image, con
   return GTOMX_OK;
 }
 
+void gtomx_debug_rand_outputs(unsigned seed, long long first, int n, int* out) {
+  if (!out || n <= 0 || first < 0) return;
+  AdditiveJump(seed).outputs(first, n, out);
+}
+
 int gtomx_prob_picker_count_angles(int ndims, float anglestep, int* count) {
   if (!count) return fail(GTOMX_ERR_INVALID_ARGUMENT, "count is NULL");
   if (ndims != 2 && ndims != 3) return fail(GTOMX_ERR_INVALID_ARGUMENT, "ndims must be 2 or 3");
@@ -590,10 +635,10 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
   // rand() integers in draw order: the restated generator where it is the C
   // runtime's, else rand() itself (reseeded as the reference does, :229)
   const bool additive = runtime_rand_is_additive();
-  AdditiveRand gen;
-  gen.seed(123);
+  const AdditiveJump jump(123);
+  const int fill_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::srand(123);
-  auto next_int = [&]() { return additive ? gen.next() : std::rand(); };
+  auto next_int = [&]() { return std::rand(); };
   const int p0 = g_gauss.phase;
   const float Us = g_gauss.U, Vs = g_gauss.V;
   long long pairs_made = 0;  // rand() pairs drawn so far (pair q uses pair index q - p0)
@@ -607,18 +652,31 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     const long long gs = (long long)a0 * (long long)Er * 5, ge = (long long)(a0 + nb) * (long long)Er * 5;
     const long long qlo = (gs + p0) / 2, qhi = (ge - 1 + p0) / 2;
     int2* hr = pp->h_raw[slot];
-    for (long long q = qlo; q <= qhi; ++q) {
-      if (q < p0) {
-        hr[q - qlo] = make_int2(0, 0);  // the half-used pair of the previous call; U, V passed instead
-        continue;
+    if (qlo < p0) hr[0] = make_int2(0, 0);  // the half-used pair of the previous call; U, V passed instead
+    const long long q0 = std::max<long long>(qlo, p0);
+    if (additive && qhi >= q0) {
+      // pair q holds outputs 2(q - p0) and 2(q - p0) + 1: threads fill slices from their own positions
+      const long long npairs = qhi - q0 + 1;
+      const int nt = (int)std::max<long long>(1, std::min<long long>(fill_threads, npairs / 4096));
+      std::vector<std::thread> pool;
+      for (int t = 0; t < nt; ++t) {
+        const long long a = q0 + npairs * t / nt, e = q0 + npairs * (t + 1) / nt;
+        pool.emplace_back([&, a, e]() {
+          jump.outputs(2 * (a - p0), 2 * (e - a), reinterpret_cast<int*>(hr + (a - qlo)));
+        });
       }
-      if (q - p0 >= pairs_made) {
-        const int a = next_int();
-        const int b2 = next_int();
-        last_pair = make_int2(a, b2);
-        ++pairs_made;
+      for (auto& th : pool) th.join();
+      last_pair = hr[qhi - qlo];
+    } else {
+      for (long long q = q0; q <= qhi; ++q) {
+        if (q - p0 >= pairs_made) {
+          const int a = next_int();
+          const int b2 = next_int();
+          last_pair = make_int2(a, b2);
+          ++pairs_made;
+        }
+        hr[q - qlo] = last_pair;  // a pair straddling two batches is repeated
       }
-      hr[q - qlo] = last_pair;  // a pair straddling two batches is repeated
     }
     float* hs = pp->h_stage[slot];
     if (pp->ndims == 3) {

@@ -394,6 +394,11 @@ int gtomx_prob_picker_set_image(gtomx_prob_picker* picker, const float* d_image,
  * caller initialises the three outputs (dimsimage elements each). */
 int gtomx_prob_picker_correlate(gtomx_prob_picker* picker, unsigned n, float anglestep, float* d_bestccf,
                                 gtomx_float3* d_bestangle, int* d_bestref, void* stream);
+/* Test hook (no reference counterpart): outputs first .. first+n-1 of glibc's
+ * rand() after srand(seed), from the library's jump-ahead restatement of its
+ * additive generator, which PerformCorrelation uses once it has matched the
+ * C runtime's rand(). */
+void gtomx_debug_rand_outputs(unsigned seed, long long first, int n, int* out);
 /* Number of orientations PerformCorrelation visits for ndims (2 or 3) and
  * anglestep (no reference counterpart; the list is the reference's). */
 int gtomx_prob_picker_count_angles(int ndims, float anglestep, int* count);

@@ -136,3 +136,19 @@ def test_prob_picker_host_paths(gx, orc):
     assert L.gtomx_prob_picker_create(ctypes.byref(h), None, i3(8, 8, 1), 1, fake, 1, 0, i3(32, 32, 1), 4, None) \
         == gx.ERR_INVALID_ARGUMENT
     assert L.gtomx_prob_picker_destroy(None) == gx.OK
+
+
+@pytest.mark.parametrize("seed", [123, 1, 987654])
+def test_rand_jump_ahead_equals_libc_rand(gx, seed):
+    """The library's jump-ahead restatement of glibc's rand(), read at several
+    offsets, equals the C runtime's own sequence after srand(seed)."""
+    import ctypes.util
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.srand(seed)
+    total = 140000
+    want = np.array([libc.rand() for _ in range(total)], np.int32)
+    L = gx.lib()
+    for first, n in ((0, 1000), (1, 31), (344, 500), (131071, 2000), (total - 700, 700)):
+        out = np.empty(n, np.int32)
+        L.gtomx_debug_rand_outputs(seed, first, n, out.ctypes.data_as(ctypes.c_void_p))
+        assert np.array_equal(out, want[first:first + n]), (seed, first)
