@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--row", choices=["prob_picker"], default=None,
+    ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks"], default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
 
@@ -121,6 +121,179 @@ def cpu_baseline(w, target_seconds):
                       f"orientation loop {r['loop_seconds']:.2f} s"}
 
 
+def _blob(box, rng, n=10):
+    g = np.arange(box) - box // 2
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    t = np.zeros((box, box, box))
+    for _ in range(n):
+        c = rng.uniform(-box / 4, box / 4, 3)
+        s = rng.uniform(1.0, box / 10, 3)
+        t += np.exp(-(((x - c[0]) / s[0]) ** 2 + ((y - c[1]) / s[1]) ** 2 + ((z - c[2]) / s[2]) ** 2) / 2)
+    return t.astype(np.float32)
+
+
+def _timed_steps(args, step):
+    import torch
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        log(f"step {i + 1}/{args.steps} done")
+    return (time.perf_counter() - t0) / args.steps
+
+
+def _row_line(args, row, metric, unit, work, sec, config, cpu):
+    out = {"row": row, "metric": metric, "value": work / sec, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic", "config": config}
+    if cpu is not None and not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+        cw, ct, sample = cpu()
+        out["cpu_baseline"] = {"value": cw / ct, "unit": unit, "cores": po.num_threads(), "kind": "port",
+                               "sample": sample}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def row_picker(args):
+    """The Picker class, 3D: a 128^3 image, a 32^3 reference, anglestep 0.5 rad
+    (648 orientations). One step is PerformCorrelation over all of them."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(args.seed)
+    nr, di, step = 32, (128, 128, 128), 0.5
+    ref = _blob(nr, rng)
+    image = rng.standard_normal(di).astype(np.float32)
+    dev = torch.device("cuda:0")
+    ctf = np.ones((nr, nr, nr // 2 + 1), np.float32)
+    pk = gx.Picker(torch.from_numpy(ref).to(dev), di)
+    dimg = torch.from_numpy(image.copy()).to(dev)
+    dctf = torch.from_numpy(ctf).to(dev)
+    pk.set_image(dimg, dctf)
+    best = torch.full(di, -1e30, device=dev)
+    ang = torch.zeros(di + (3,), device=dev)
+    angles = po.equal_angular_spacing((0, 2 * np.pi), (0, np.pi), (0, 2 * np.pi), step)
+    sec = _timed_steps(args, lambda: pk.perform_correlation(best, ang, anglestep=step))
+    pk.close()
+    vox = int(np.prod(di))
+
+    def cpu():
+        k = 40
+        t0 = time.perf_counter()
+        po.picker(ref.astype(np.float64), image.astype(np.float64), ctf.astype(np.float64), angles[:k])
+        ct = time.perf_counter() - t0
+        return vox * k, ct, f"same image and reference, the first {k} orientations, FP64 oracle, {ct:.1f} s"
+    return _row_line(args, "picker", "image voxel-orientations/s, Picker 3D", "voxel-orientations/s",
+                     vox * len(angles), sec, {"workload": "Picker.PerformCorrelation", "image": list(di),
+                                              "ref": [nr] * 3, "anglestep": step, "orientations": len(angles)}, cpu)
+
+
+def row_pick2d(args):
+    """The 2D branch of d_PickSubTomograms: 16 images of 256^2, a 64^3 template
+    (projector 128^3, oversample 0.5), 2,000 orientations (250 Fibonacci
+    directions x 8 in-plane steps), one CTF, local-std radius 32."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(args.seed)
+    n, box, nimg = 256, 64, 16
+    p = 2 * box
+    dev = torch.device("cuda:0")
+    angles = gx.fibonacci_angles(250, 8)
+    projft = gx.prepare_projector(torch.from_numpy(_blob(box, rng)).to(dev), p)
+    imgs = rng.standard_normal((nimg, n, n)).astype(np.float32)
+    E = gx.d_FFTR2C(torch.from_numpy(imgs).to(dev), ndims=2)
+    ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=5e-10, Cs=2.7e-3, voltage=300e3, defocus=-2e-6, amplitude=0.07)],
+                    n, device=dev, ndims=2)
+    plan = gx.PickPlan(p, n, 0, ndims=2)
+    best = torch.empty((nimg, n, n), device=dev)
+    idx = torch.empty((nimg, n, n), dtype=torch.int32, device=dev)
+    sec = _timed_steps(args, lambda: gx.d_PickSubTomograms(plan, projft, p / n, E, ctf, angles, n / 8, best=best,
+                                                            bestidx=idx))
+    plan.close()
+    work = nimg * n * n * len(angles)
+
+    def cpu():
+        k = 200
+        P = projft.cpu().numpy()
+        E0 = E[:1].cpu().numpy()
+        C = ctf[0].cpu().numpy()
+        r = po.pick(P, p, p / n, E0, C, n, angles[:k].astype(np.float64), n / 8, ndims=2)
+        return n * n * k, r["loop_seconds"], (f"1 image, the first {k} orientations, FP64 oracle, orientation loop "
+                                              f"{r['loop_seconds']:.1f} s")
+    return _row_line(args, "pick2d", "image pixel-orientations/s, d_PickSubTomograms 2D branch",
+                     "pixel-orientations/s", work, sec,
+                     {"workload": "d_PickSubTomograms (2D)", "images": nimg, "image": [n, n], "template": [box] * 3,
+                      "orientations": len(angles)}, cpu)
+
+
+def row_diff2(args):
+    """d_PickSubTomogramsDiff2: 8 volumes of 64^3, a 32^3 template (projector 64^3,
+    oversample 1), relevant shifts 12^3, 500 orientations (50 x 10)."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(args.seed)
+    n, box, nvol, rel = 64, 32, 8, (12, 12, 12)
+    p = 2 * box
+    dev = torch.device("cuda:0")
+    angles = gx.fibonacci_angles(50, 10)
+    projft = gx.prepare_projector(torch.from_numpy(_blob(box, rng)).to(dev), p)
+    E = gx.d_FFTR2C(torch.from_numpy(rng.standard_normal((nvol, n, n, n)).astype(np.float32)).to(dev))
+    ctf = torch.from_numpy(rng.uniform(-1, 1, (nvol, n, n, n // 2 + 1)).astype(np.float32)).to(dev)
+    sec = _timed_steps(args, lambda: gx.d_PickSubTomogramsDiff2(projft, p, p / n, E, ctf, rel, angles))
+    shifts = int(np.prod(rel))
+    work = nvol * shifts * len(angles)
+
+    def cpu():
+        k = 4
+        P = projft.cpu().numpy().astype(np.complex128)
+        E0 = E[0].cpu().numpy().astype(np.complex128)
+        C0 = ctf[0].cpu().numpy().astype(np.float64)
+        t0 = time.perf_counter()
+        po.diff2(P, p, p / n, E0, C0, n, rel, angles[:k].astype(np.float64))
+        ct = time.perf_counter() - t0
+        return shifts * k, ct, f"1 volume, the first {k} orientations, FP64 oracle, {ct:.1f} s"
+    return _row_line(args, "diff2", "shift-orientations/s, d_PickSubTomogramsDiff2", "shift-orientations/s", work,
+                     sec, {"workload": "d_PickSubTomogramsDiff2", "volumes": nvol, "volume": [n] * 3,
+                           "relevant": list(rel), "orientations": len(angles)}, cpu)
+
+
+def row_local_peaks(args):
+    """d_LocalPeaks on a 256^3 smooth random map (noise low-passed to about 4
+    voxels, normalised), local extent 8, threshold 3."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(args.seed)
+    n, extent, thr = 256, 8, 3.0
+    f = np.fft.rfftn(rng.standard_normal((n, n, n)).astype(np.float32))
+    k = np.sqrt(np.fft.fftfreq(n)[:, None, None] ** 2 + np.fft.fftfreq(n)[None, :, None] ** 2 +
+                np.fft.rfftfreq(n)[None, None, :] ** 2)
+    vol = np.fft.irfftn(f * np.exp(-(k * 16) ** 2), s=(n, n, n)).astype(np.float32)
+    vol = ((vol - vol.mean()) / vol.std()).astype(np.float32)
+    dev = torch.device("cuda:0")
+    dv = torch.from_numpy(vol).to(dev)
+    found = []
+    sec = _timed_steps(args, lambda: found.append(len(gx.d_LocalPeaks(dv, extent, thr)[0])))
+    vox = n ** 3
+
+    def cpu():
+        t0 = time.perf_counter()
+        po.local_peaks(vol, extent, thr)
+        ct = time.perf_counter() - t0
+        return vox, ct, f"the same map, FP32 oracle, {ct:.1f} s"
+    return _row_line(args, "local_peaks", "voxels/s, d_LocalPeaks", "voxels/s", vox, sec,
+                     {"workload": "d_LocalPeaks", "volume": [n] * 3, "localextent": extent, "threshold": thr,
+                      "peaks": found[-1]}, cpu)
+
+
 def row_prob_picker(args):
     """ProbabilisticPicker, 3D: a 128^3 volume (mean 0 / std 1 noise with three
     planted copies), references of 32^3 with a circular mask of radius 14,
@@ -188,8 +361,9 @@ def main():
     if args.copy_probe:
         copy_probe()
         return 0
-    if args.row == "prob_picker":
-        return row_prob_picker(args)
+    if args.row is not None:
+        return {"prob_picker": row_prob_picker, "picker": row_picker, "pick2d": row_pick2d, "diff2": row_diff2,
+                "local_peaks": row_local_peaks}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth

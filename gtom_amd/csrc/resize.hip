@@ -10,7 +10,8 @@
 //  * d_FFTFullPad (:92-110, FFTFullPadEvenKernel :212-238): full arrays, as
 //    d_FFTPad on every axis with the shift (old - new - ((old & (1 - (new &
 //    1))) % 2)) / 2 (the kernel's `old & 1 - (new & 1)` parses that way).
-// One kernel walks the output and reads the mapped input element or writes 0.
+// One kernel walks the output row by row and reads the mapped input element or
+// writes 0.
 // In place (input == output, buffer sized for the larger dims) goes through a
 // temporary, as the reference's wrappers do.
 #include <algorithm>
@@ -40,20 +41,30 @@ template <int OP> __device__ __forceinline__ int src_axis(int i, int o, int n, b
   }
 }
 
+// One wave per output row (4 rows per workgroup, blockIdx.y the item): the
+// row's source row is decided once, then the wave writes the row with
+// coalesced stores. Row arithmetic is 32-bit (n <= 256 per axis).
 template <class T, int OP>
 __global__ __launch_bounds__(256) void k_fft_resize(const T* __restrict__ in, T* __restrict__ out, int3 od, int3 nd,
-                                                    size_t vnew, size_t vold, unsigned batch) {
+                                                    size_t vnew, size_t vold) {
   constexpr bool HALF = OP == FFT_CROP || OP == FFT_PAD;
   const int nxs = HALF ? nd.x / 2 + 1 : nd.x, oxs = HALF ? od.x / 2 + 1 : od.x;
-  const size_t total = vnew * batch;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-    const size_t b = i / vnew, r = i % vnew;
-    const int x = (int)(r % nxs), y = (int)((r / nxs) % nd.y), z = (int)(r / ((size_t)nxs * nd.y));
-    const int sx = src_axis<OP>(x, od.x, nd.x, HALF), sy = src_axis<OP>(y, od.y, nd.y, false),
-              sz = src_axis<OP>(z, od.z, nd.z, false);
-    T v{};
-    if (sx >= 0 && sy >= 0 && sz >= 0) v = in[b * vold + ((size_t)sz * od.y + sy) * oxs + sx];
-    out[i] = v;
+  const int lane = threadIdx.x & 63;
+  const unsigned rows = (unsigned)nd.y * (unsigned)nd.z;
+  const unsigned row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int y = (int)(row % (unsigned)nd.y), z = (int)(row / (unsigned)nd.y);
+  const size_t b = blockIdx.y;
+  const int sy = src_axis<OP>(y, od.y, nd.y, false), sz = src_axis<OP>(z, od.z, nd.z, false);
+  T* o = out + b * vnew + (size_t)row * nxs;
+  if (sy < 0 || sz < 0) {
+    for (int x = lane; x < nxs; x += 64) o[x] = T{};
+    return;
+  }
+  const T* src = in + b * vold + ((size_t)sz * od.y + sy) * oxs;
+  for (int x = lane; x < nxs; x += 64) {
+    const int sx = src_axis<OP>(x, od.x, nd.x, HALF);
+    o[x] = sx >= 0 ? src[sx] : T{};
   }
 }
 
@@ -61,11 +72,14 @@ template <class T, int OP> int launch(const void* in, void* out, gtomx_int3 od, 
                                       hipStream_t s) {
   constexpr bool HALF = OP == FFT_CROP || OP == FFT_PAD;
   const size_t vnew = HALF ? elements_ft(nd) : elements(nd), vold = HALF ? elements_ft(od) : elements(od);
-  const unsigned grid = (unsigned)std::min<size_t>((vnew * batch + 255) / 256, 65536);
-  hipLaunchKernelGGL((k_fft_resize<T, OP>), dim3(grid), dim3(256), 0, s, static_cast<const T*>(in),
-                     static_cast<T*>(out), make_int3(od.x, od.y, od.z), make_int3(nd.x, nd.y, nd.z), vnew, vold,
-                     batch);
-  GX_LAUNCHED("k_fft_resize");
+  const unsigned rows = (unsigned)nd.y * (unsigned)nd.z;
+  for (unsigned b0 = 0; b0 < batch; b0 += 65535) {
+    const unsigned nb = std::min(batch - b0, 65535u);
+    hipLaunchKernelGGL((k_fft_resize<T, OP>), dim3((rows + 3) / 4, nb), dim3(256), 0, s,
+                       static_cast<const T*>(in) + (size_t)b0 * vold, static_cast<T*>(out) + (size_t)b0 * vnew,
+                       make_int3(od.x, od.y, od.z), make_int3(nd.x, nd.y, nd.z), vnew, vold);
+    GX_LAUNCHED("k_fft_resize");
+  }
   return GTOMX_OK;
 }
 
@@ -103,7 +117,8 @@ int resize_entry(int op, bool complex_values, const void* d_input, void* d_outpu
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "a crop needs new dims no larger than the old ones");
   const bool half = op == FFT_CROP || op == FFT_PAD;
   const size_t vnew = half ? elements_ft(nd) : elements(nd);
-  if (vnew * batch >= ((size_t)1 << 40)) return fail(GTOMX_ERR_UNSUPPORTED, "output too large");
+  if (vnew * batch >= ((size_t)1 << 40) || (size_t)nd.y * nd.z >= ((size_t)1 << 31))
+    return fail(GTOMX_ERR_UNSUPPORTED, "output too large");
   hipStream_t s = as_stream(stream);
   if (d_input != d_output) return fft_resize(op, complex_values, d_input, d_output, od, nd, batch, s);
   DevBuf tmp;
