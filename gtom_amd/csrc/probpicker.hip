@@ -1,5 +1,5 @@
 // ProbabilisticPicker (include/Correlation.cuh:166-207,
-// src/Correlation/ProbabilisticPicker.cu:29-455) for circular masks, 2D and 3D.
+// src/Correlation/ProbabilisticPicker.cu:29-455), 2D and 3D.
 //
 // Initialize (:33-152), per reference n, on the device:
 //   R_n = ifftshift(pad(ref_n, dimsimage, 0)), C_n = FFTCrop(R2C(R_n), 2*lowpass),
@@ -10,7 +10,11 @@
 //   next plane's DC row or past the cropped spectrum, where the reference's buffer
 //   still holds R_n (the crop was written over its head). Both are restated here:
 //   element j >= ElementsFFT(2*lowpass) is the float pair (R_n[2j], R_n[2j+1]).
-//   Masks: M_n = ifftshift(pad(mask_n)), IM_n = R2C(ifftshift(pad(1 - mask_n))).
+//   Masks: M_n = ifftshift(pad(mask_n)), IM_n = R2C(ifftshift(pad(1 - mask_n)));
+//   a non-circular mask is kept as the cubic B-spline coefficients of pad(mask_n)
+//   (d_CubicBSplinePrefilter3D/2D, :104-110) and rotated into FFT order with each
+//   orientation (d_Rotate3D / d_Rotate2D, T_INTERP_CUBIC, :273-279), whose crop
+//   then gives that orientation's inverse mask and solvent statistics (:342-356).
 // SetImage (:205-224): I = R2C(mean0std1(image)) and I2 = R2C(mean0std1(image)^2),
 //   both band-limited to |k| < lowpass; the CTF cropped to 2*lowpass.
 // PerformCorrelation (:226-389), orientations B at a time:
@@ -97,10 +101,11 @@ __global__ __launch_bounds__(256) void k_times_real(float2* __restrict__ x, cons
   }
 }
 
+// x[b][i] *= m[b * mstride + i] (mstride 0: one mask for every item)
 __global__ __launch_bounds__(256) void k_times_mask(float* __restrict__ x, const float* __restrict__ m, size_t per,
-                                                    unsigned batch) {
+                                                    unsigned batch, size_t mstride) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < per * batch; i += (size_t)gridDim.x * 256)
-    x[i] *= m[i % per];
+    x[i] *= m[(i / per) * mstride + i % per];
 }
 
 // out = a * conj(b) (d_ComplexMultiplyByConjVector, src/Generics/Arithmetics.cu:334-344)
@@ -108,6 +113,124 @@ __global__ __launch_bounds__(256) void k_mul_conj_into(const float2* __restrict_
                                                        float2* __restrict__ out, size_t n) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     const float2 x = a[i], y = b[i];
+    out[i] = make_float2(x.x * y.x + x.y * y.y, x.y * y.x - x.x * y.y);
+  }
+}
+
+// ConvertToInterpolationCoefficients (src/CubicSplines/internal/cubicPrefilter_kernel.cu):
+// the cubic B-spline prefilter of one line in place, pole z = sqrt(3) - 2, gain
+// (1 - z)(1 - 1/z), causal start c0 + sum_{n < min(12, len)} z^(n+1) c_n, anticausal
+// start z/(z - 1) c_(len-1), in float. Lines along axis 0 (x), 1 (y) or 2 (z) of
+// `batch` volumes (nx, ny, nz), one thread per line (SamplesToCoefficients3D{X,Y,Z}
+// and the 2D forms, src/CubicSplines/cubicPrefilter3D.cu, cubicPrefilter2D.cu).
+__global__ __launch_bounds__(256) void k_bspline_lines(float* __restrict__ v, int3 d, int axis, unsigned batch) {
+  const size_t V = (size_t)d.x * d.y * d.z;
+  const int len = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+  const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)d.x : (size_t)d.x * d.y);
+  const size_t per = V / len;
+  const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (line >= per * batch) return;
+  const size_t b = line / per, l = line % per;
+  size_t start;
+  if (axis == 0) start = l * d.x;
+  else if (axis == 1) start = (l / d.x) * ((size_t)d.x * d.y) + l % d.x;
+  else start = l;
+  float* c = v + b * V + start;
+  const float z = sqrtf(3.0f) - 2.0f;
+  const float lambda = (1.0f - z) * (1.0f - 1.0f / z);
+  const int horizon = len < 12 ? len : 12;
+  float zn = z, sum = c[0];
+  for (int n = 0; n < horizon; ++n) {
+    sum += zn * c[(size_t)n * stride];
+    zn *= z;
+  }
+  float prev = lambda * sum;
+  c[0] = prev;
+  for (int n = 1; n < len; ++n) {
+    prev = lambda * c[(size_t)n * stride] + z * prev;
+    c[(size_t)n * stride] = prev;
+  }
+  prev = (z / (z - 1.0f)) * prev;
+  c[(size_t)(len - 1) * stride] = prev;
+  for (int n = len - 2; n >= 0; --n) {
+    prev = z * (prev - c[(size_t)n * stride]);
+    c[(size_t)n * stride] = prev;
+  }
+}
+
+// bspline() (include/CubicInterp.cuh:119-130)
+__device__ __forceinline__ float bspline(float t) {
+  t = fabsf(t);
+  const float a = 2.0f - t;
+  if (t < 1.0f) return 2.0f / 3.0f - 0.5f * t * t * a;
+  if (t < 2.0f) return a * a * a / 6.0f;
+  return 0.0f;
+}
+
+// d_Rotate3D / d_Rotate2D with T_INTERP_CUBIC into FFT order (Rotate3DKernel,
+// Rotate2DKernel<true, false>, src/Transformation/Rotation.cu:386-418, 450-480):
+// output voxel i takes x = (i + n/2) % n per axis, p = M (x - n/2) + n/2 + 0.5
+// (3D: M = transpose(Euler), 2D: M = R(-angle) from its cos/sin pair), and the
+// cubic B-spline of the prefiltered coefficients at p - 0.5 over the 4^ndims
+// taps, indices clamped, summed z, y, x as cubicTex3DSimple (include/CubicInterp.cuh:287-312).
+__global__ __launch_bounds__(256) void k_rotate_cubic(const float* __restrict__ coef, float* __restrict__ out,
+                                                      int3 d, const float* __restrict__ mats, int ndims) {
+  const size_t V = (size_t)d.x * d.y * d.z;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= V) return;
+  const int b = blockIdx.y;
+  const int ix = (int)(i % d.x), iy = (int)((i / d.x) % d.y), iz = (int)(i / ((size_t)d.x * d.y));
+  const float cx = (float)((ix + d.x / 2) % d.x - d.x / 2), cy = (float)((iy + d.y / 2) % d.y - d.y / 2),
+              cz = (float)((iz + d.z / 2) % d.z - d.z / 2);
+  float px, py, pz = 0.f;
+  if (ndims == 3) {
+    const float* M = mats + 9 * b;
+    px = M[0] * cx + M[1] * cy + M[2] * cz;
+    py = M[3] * cx + M[4] * cy + M[5] * cz;
+    pz = M[6] * cx + M[7] * cy + M[8] * cz;
+  } else {
+    const float c = mats[2 * b], sn = mats[2 * b + 1];
+    px = c * cx - sn * cy;
+    py = sn * cx + c * cy;
+  }
+  // p - 0.5 with p = M(x - n/2) + n/2 + 0.5
+  const float gx = px + (float)(d.x / 2), gy = py + (float)(d.y / 2), gz = pz + (float)(d.z / 2);
+  const float fx0 = floorf(gx), fy0 = floorf(gy), fz0 = floorf(gz);
+  const float fx = gx - fx0, fy = gy - fy0, fz = gz - fz0;
+  auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  float r = 0.f;
+  const int zlo = ndims == 3 ? -1 : 0, zhi = ndims == 3 ? 2 : 0;
+  for (int tz = zlo; tz <= zhi; ++tz) {
+    const float wz = ndims == 3 ? bspline((float)tz - fz) : 1.0f;
+    const int zz = ndims == 3 ? clampi((int)fz0 + tz, d.z - 1) : 0;
+    for (int ty = -1; ty <= 2; ++ty) {
+      const float wyz = bspline((float)ty - fy) * wz;
+      const int yy = clampi((int)fy0 + ty, d.y - 1);
+      const float* row = coef + ((size_t)zz * d.y + yy) * d.x;
+      for (int tx = -1; tx <= 2; ++tx) r += bspline((float)tx - fx) * wyz * row[clampi((int)fx0 + tx, d.x - 1)];
+    }
+  }
+  out[(size_t)b * V + i] = r;
+}
+
+// om[b][i] = 1 - m[b][src(i)] over the FFTFullCrop to dimsref (d_OneMinus of the
+// cropped rotated mask, src/Correlation/ProbabilisticPicker.cu:346-348).
+__global__ __launch_bounds__(256) void k_crop_one_minus(const float* __restrict__ m, float* __restrict__ om, int3 di,
+                                                        int3 dr, size_t vimage, size_t er) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= er) return;
+  const unsigned b = blockIdx.y;
+  const int x = (int)(i % dr.x), y = (int)((i / dr.x) % dr.y), z = (int)(i / ((size_t)dr.x * dr.y));
+  const int sx = x >= dr.x / 2 ? x + di.x - dr.x : x, sy = y >= dr.y / 2 ? y + di.y - dr.y : y,
+            sz = z >= dr.z / 2 ? z + di.z - dr.z : z;
+  om[(size_t)b * er + i] = 1.0f - m[(size_t)b * vimage + ((size_t)sz * di.y + sy) * di.x + sx];
+}
+
+// out[b][i] = a[i] * conj(bb[b][i])
+__global__ __launch_bounds__(256) void k_mul_conj_batch(const float2* __restrict__ a, const float2* __restrict__ bb,
+                                                        float2* __restrict__ out, size_t n, unsigned batch) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n * batch; i += (size_t)gridDim.x * 256) {
+    const float2 x = a[i % n], y = bb[i];
     out[i] = make_float2(x.x * y.x + x.y * y.y, x.y * y.x - x.x * y.y);
   }
 }
@@ -149,8 +272,8 @@ __device__ __forceinline__ float gauss_draw(const Draws& d, long long g) {
 // five draws (draw index (g0 + b * er + i) * 5 + k, g0 = first orientation * er):
 // part[b][blk] = (sum m, sum r, sum r^2, sum r*G).
 __global__ __launch_bounds__(256) void k_prob_sums(const float* __restrict__ r, const float* __restrict__ mask,
-                                                   Draws draws, long long g0, int3 di, int3 dr, size_t vimage,
-                                                   size_t er, double4* __restrict__ part) {
+                                                   size_t mstride, Draws draws, long long g0, int3 di, int3 dr,
+                                                   size_t vimage, size_t er, double4* __restrict__ part) {
   __shared__ double4 red[4];
   const unsigned b = blockIdx.y;
   double sm = 0, sr = 0, srr = 0, srg = 0;
@@ -163,7 +286,7 @@ __global__ __launch_bounds__(256) void k_prob_sums(const float* __restrict__ r, 
     const long long gbase = (g0 + (long long)b * (long long)er + (long long)i) * 5;
     double G = 0;
     for (int k = 0; k < 5; ++k) G += (double)gauss_draw(draws, gbase + k);
-    sm += mask[src];
+    sm += mask[(size_t)b * mstride + src];
     sr += rv;
     srr += rv * rv;
     srg += rv * G;
@@ -215,10 +338,11 @@ __global__ void k_prob_coef(const double4* __restrict__ part, int nparts, unsign
 }
 
 // ProbSolventStatisticsKernel: mean = m * f, std = sqrt(max(0, s * f - mean^2)), float.
+// (item b of `batch` uses orientation b's factor)
 __global__ __launch_bounds__(256) void k_solvent_stats(float* __restrict__ mean, float* __restrict__ sd, size_t n,
-                                                       const ProbCoef* __restrict__ coef) {
-  const float f = coef[0].solventnorm;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+                                                       const ProbCoef* __restrict__ coef, unsigned batch) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n * batch; i += (size_t)gridDim.x * 256) {
+    const float f = coef[i / n].solventnorm;
     const float m = mean[i] * f;
     mean[i] = m;
     const float v = sd[i] * f - m * m;
@@ -227,18 +351,20 @@ __global__ __launch_bounds__(256) void k_solvent_stats(float* __restrict__ mean,
 }
 
 // ProbPickingUpdateKernel over B maps in orientation order.
+// (solvent statistics of item b at b * sstride: 0 when one set serves every orientation)
 __global__ __launch_bounds__(256) void k_prob_update(const float* __restrict__ corr, const float* __restrict__ smean,
-                                                     const float* __restrict__ sstd, const ProbCoef* __restrict__ coef,
+                                                     const float* __restrict__ sstd, size_t sstride,
+                                                     const ProbCoef* __restrict__ coef,
                                                      const float3* __restrict__ angles, unsigned batch, size_t n,
                                                      int refid, float* __restrict__ bestccf,
                                                      float3* __restrict__ bestangle, int* __restrict__ bestref) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     float best = bestccf[i];
     int k = -1;
-    const double m = smean[i];
-    const double sd = fmax(1e-7, (double)sstd[i]);
     for (unsigned b = 0; b < batch; ++b) {
       const ProbCoef c = coef[b];
+      const double m = smean[b * sstride + i];
+      const double sd = fmax(1e-7, (double)sstd[b * sstride + i]);
       double d = -2.0 * (double)c.normfactor * (double)corr[(size_t)b * n + i];
       d += 2.0 * m * (double)c.smr;
       d /= sd;
@@ -366,16 +492,19 @@ struct gtomx_prob_picker {
   unsigned B = 0;
   size_t Er = 0, Vi = 0, Hi = 0, Hl = 0;
   float2* refs_low = nullptr;    // nrefs x Hl, zero-centred half spectra
-  float* masks = nullptr;        // nrefs x Vi, FFT order
-  float2* invmasksft = nullptr;  // nrefs x Hi
+  float* masks = nullptr;        // nrefs x Vi: FFT order (circular), or centred B-spline coefficients
+  float2* invmasksft = nullptr;  // nrefs x Hi (circular)
+  float* mrot = nullptr;         // B x Vi rotated masks in FFT order (non-circular)
+  float2* imft = nullptr;        // B x Hi inverse-mask spectra (non-circular)
+  float* om = nullptr;           // B x Er cropped inverse masks (non-circular)
   float2* imageft = nullptr;
   float2* image2ft = nullptr;
   float* ctf_low = nullptr;      // Hl
   float2* rot = nullptr;         // B x Hl
   float2* padft = nullptr;       // B x Hi: padded spectra, C2R workspace, then R2C of r
   float* real = nullptr;         // B x Vi: r, then the correlation maps
-  float* smean = nullptr;        // Vi
-  float* sstd = nullptr;         // Vi
+  float* smean = nullptr;        // Vi (circular) or B x Vi
+  float* sstd = nullptr;         // Vi (circular) or B x Vi
   int2* raw = nullptr;           // rand() pairs of one batch
   double4* part = nullptr;       // B x NBS
   gx::ProbCoef* coef = nullptr;  // B
@@ -397,7 +526,8 @@ void prob_release(gtomx_prob_picker* pp) {
   for (void* p : {(void*)pp->refs_low, (void*)pp->masks, (void*)pp->invmasksft, (void*)pp->imageft,
                   (void*)pp->image2ft, (void*)pp->ctf_low, (void*)pp->rot, (void*)pp->padft, (void*)pp->real,
                   (void*)pp->smean, (void*)pp->sstd, (void*)pp->raw, (void*)pp->part, (void*)pp->coef,
-                  (void*)pp->mats, (void*)pp->dang, (void*)pp->nscr})
+                  (void*)pp->mats, (void*)pp->dang, (void*)pp->nscr, (void*)pp->mrot, (void*)pp->imft,
+                  (void*)pp->om})
     dev_free(p);
   for (int k = 0; k < 2; ++k) {
     if (pp->h_raw[k]) (void)hipHostFree(pp->h_raw[k]);
@@ -455,10 +585,6 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   *out = nullptr;
   if (!d_refs || !d_refmasks) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (nrefs == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "nrefs must be positive");
-  if (!ismaskcircular)
-    return fail(GTOMX_ERR_UNSUPPORTED,
-                "non-circular masks are not supported: the reference rotates them with cubic B-splines about the "
-                "centre while the reference volume they multiply sits at the origin");
   const int nd = dimsimage.z > 1 ? 3 : 2;
   if ((dimsref.z > 1 ? 3 : 2) != nd) return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference and image dimensionality differ");
   GX_TRY(check_fft_dims(nd, dimsimage));
@@ -482,7 +608,7 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   pp->dl = dl;
   pp->ndims = nd;
   pp->nrefs = nrefs;
-  pp->circular = true;
+  pp->circular = ismaskcircular != 0;
   pp->doctf = doctf != 0;
   pp->Er = elements(dimsref);
   pp->Vi = Vi;
@@ -495,20 +621,27 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   };
   st = hipGetDevice(&pp->device) == hipSuccess ? GTOMX_OK : fail(GTOMX_ERR_DEVICE, "hipGetDevice");
   // orientations per batch: about 2 GiB of per-orientation buffers, at most 32
-  const size_t per = Hl * 8 + pp->Hi * 8 + Vi * 4 + pp->Er * 20;
+  const size_t per = Hl * 8 + pp->Hi * 8 + Vi * 4 + pp->Er * 20 +
+                     (pp->circular ? 0 : Vi * 12 + pp->Hi * 8 + pp->Er * 4);
   pp->B = (unsigned)std::max<size_t>(1, std::min<size_t>(32, ((size_t)2 << 30) / per));
   const unsigned B = pp->B;
   alloc((void**)&pp->refs_low, (size_t)nrefs * Hl * sizeof(float2));
   alloc((void**)&pp->masks, (size_t)nrefs * Vi * sizeof(float));
-  alloc((void**)&pp->invmasksft, (size_t)nrefs * pp->Hi * sizeof(float2));
+  if (pp->circular) {
+    alloc((void**)&pp->invmasksft, (size_t)nrefs * pp->Hi * sizeof(float2));
+  } else {
+    alloc((void**)&pp->mrot, (size_t)B * Vi * sizeof(float));
+    alloc((void**)&pp->imft, (size_t)B * pp->Hi * sizeof(float2));
+    alloc((void**)&pp->om, (size_t)B * pp->Er * sizeof(float));
+  }
   alloc((void**)&pp->imageft, pp->Hi * sizeof(float2));
   alloc((void**)&pp->image2ft, pp->Hi * sizeof(float2));
   alloc((void**)&pp->ctf_low, Hl * sizeof(float));
   alloc((void**)&pp->rot, (size_t)B * Hl * sizeof(float2));
   alloc((void**)&pp->padft, (size_t)B * pp->Hi * sizeof(float2));
   alloc((void**)&pp->real, (size_t)B * Vi * sizeof(float));
-  alloc((void**)&pp->smean, Vi * sizeof(float));
-  alloc((void**)&pp->sstd, Vi * sizeof(float));
+  alloc((void**)&pp->smean, (pp->circular ? 1 : (size_t)B) * Vi * sizeof(float));
+  alloc((void**)&pp->sstd, (pp->circular ? 1 : (size_t)B) * Vi * sizeof(float));
   pp->raw_cap = ((size_t)B * pp->Er * 5 + 3) / 2 + 1;  // pairs touched by B orientations' draws
   alloc((void**)&pp->raw, pp->raw_cap * sizeof(int2));
   alloc((void**)&pp->part, (size_t)B * NBS * sizeof(double4));
@@ -555,6 +688,20 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   hipLaunchKernelGGL(k_remap_half_quirk, dim3(grid_of(Hl * nrefs)), dim3(256), 0, s, C, R, pp->refs_low, n, dl.z,
                      Hl, Vi, nrefs);
   if (hipGetLastError() != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "k_remap_half_quirk"));
+  if (!pp->circular) {
+    // the padded masks, centred, as cubic B-spline coefficients (:102-110)
+    if ((st = gtomx_pad(d_refmasks, pp->masks, dimsref, dimsimage, 0.0f, nrefs, stream)) != GTOMX_OK) return fin(st);
+    const int3 d3 = make_int3(dimsimage.x, dimsimage.y, dimsimage.z);
+    for (int axis = 0; axis < nd; ++axis) {
+      const size_t lines = Vi / (axis == 0 ? dimsimage.x : (axis == 1 ? dimsimage.y : dimsimage.z)) * nrefs;
+      hipLaunchKernelGGL(k_bspline_lines, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, pp->masks, d3,
+                         axis, nrefs);
+      if (hipGetLastError() != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "k_bspline_lines"));
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "hipStreamSynchronize"));
+    *out = pp;
+    return GTOMX_OK;
+  }
   // M = ifftshift(pad(mask)); IM = R2C(ifftshift(pad(1 - mask)))
   if ((st = gtomx_pad(d_refmasks, R, dimsref, dimsimage, 0.0f, nrefs, stream)) != GTOMX_OK) return fin(st);
   if ((st = gtomx_remap_full2fullfft(R, pp->masks, dimsimage, nrefs, stream)) != GTOMX_OK) return fin(st);
@@ -629,7 +776,7 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
   const int3 di = make_int3(pp->di.x, pp->di.y, pp->di.z), dr = make_int3(pp->dr.x, pp->dr.y, pp->dr.z);
   const float2* refft = pp->refs_low + (size_t)n * Hl;
   const float* mask = pp->masks + (size_t)n * Vi;
-  const float2* invmaskft = pp->invmasksft + (size_t)n * Hi;
+  const float2* invmaskft = pp->circular ? pp->invmasksft + (size_t)n * Hi : nullptr;
   const float maxfreq = (float)(pp->dl.x / 2);
   std::lock_guard<std::mutex> lock(g_gauss_mutex);
   // rand() integers in draw order: the restated generator where it is the C
@@ -709,12 +856,23 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     }
     GX_TRY(fft_resize(FFT_PAD, true, pp->rot, pp->padft, pp->dl, pp->di, nb, s));
     GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f / (float)Vi, s));
-    hipLaunchKernelGGL(k_times_mask, dim3(grid_of(Vi * nb)), dim3(256), 0, s, pp->real, mask, Vi, nb);
+    // the mask of each orientation: the same one (circular), or the B-spline
+    // coefficients rotated into FFT order (d_Rotate3D / d_Rotate2D, T_INTERP_CUBIC, :273-279)
+    const float* mk = mask;
+    size_t mstride = 0;
+    if (!pp->circular) {
+      hipLaunchKernelGGL(k_rotate_cubic, dim3((unsigned)((Vi + 255) / 256), nb), dim3(256), 0, s, mask, pp->mrot,
+                         di, pp->mats, pp->ndims);
+      GX_LAUNCHED("k_rotate_cubic");
+      mk = pp->mrot;
+      mstride = Vi;
+    }
+    hipLaunchKernelGGL(k_times_mask, dim3(grid_of(Vi * nb)), dim3(256), 0, s, pp->real, mk, Vi, nb, mstride);
     GX_LAUNCHED("k_times_mask");
     // statistics of the cropped, masked reference
     const int nparts = (int)std::min<size_t>(NBS, (Er + 255) / 256);
     const Draws draws{pp->raw, qlo, p0, Us, Vs, (double)RAND_MAX};
-    hipLaunchKernelGGL(k_prob_sums, dim3(nparts, nb), dim3(256), 0, s, pp->real, mask, draws,
+    hipLaunchKernelGGL(k_prob_sums, dim3(nparts, nb), dim3(256), 0, s, pp->real, mk, mstride, draws,
                        (long long)a0 * (long long)Er, di, dr, Vi, Er, pp->part);
     GX_LAUNCHED("k_prob_sums");
     hipLaunchKernelGGL(k_prob_coef, dim3((nb + 63) / 64), dim3(64), 0, s, pp->part, nparts, nb, Vi, Er, pp->coef);
@@ -723,7 +881,7 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     GX_TRY(fft_r2c(pp->real, pp->padft, pp->di, nb, s));
     GX_TRY(image_times_conj(pp->imageft, pp->padft, Hi, nb, s));
     GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f, s));
-    if (a0 == 0) {
+    if (pp->circular && a0 == 0) {
       // solvent statistics with the first orientation's mask sum (the mask is
       // circular, so every orientation has the same); padft[0] is free again.
       float2* tmp = pp->padft;
@@ -733,11 +891,31 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
       hipLaunchKernelGGL(k_mul_conj_into, dim3(grid_of(Hi)), dim3(256), 0, s, pp->image2ft, invmaskft, tmp, Hi);
       GX_LAUNCHED("k_mul_conj_into");
       GX_TRY(fft_c2r(tmp, tmp, pp->sstd, pp->di, 1, 1.0f, s));
-      hipLaunchKernelGGL(k_solvent_stats, dim3(grid_of(Vi)), dim3(256), 0, s, pp->smean, pp->sstd, Vi, pp->coef);
+      hipLaunchKernelGGL(k_solvent_stats, dim3(grid_of(Vi)), dim3(256), 0, s, pp->smean, pp->sstd, Vi, pp->coef, 1u);
+      GX_LAUNCHED("k_solvent_stats");
+    } else if (!pp->circular) {
+      // every orientation's own inverse mask and solvent statistics (:342-356):
+      // 1 - crop(mask), full pad, R2C; then I and I2 against it, as above
+      hipLaunchKernelGGL(k_crop_one_minus, dim3((unsigned)((Er + 255) / 256), nb), dim3(256), 0, s, pp->mrot, pp->om,
+                         di, dr, Vi, Er);
+      GX_LAUNCHED("k_crop_one_minus");
+      GX_TRY(fft_resize(FFT_FULL_PAD, false, pp->om, pp->smean, pp->dr, pp->di, nb, s));
+      GX_TRY(fft_r2c(pp->smean, pp->imft, pp->di, nb, s));
+      hipLaunchKernelGGL(k_mul_conj_batch, dim3(grid_of(Hi * nb)), dim3(256), 0, s, pp->imageft, pp->imft, pp->padft,
+                         Hi, nb);
+      GX_LAUNCHED("k_mul_conj_batch");
+      GX_TRY(fft_c2r(pp->padft, pp->padft, pp->smean, pp->di, nb, 1.0f, s));
+      hipLaunchKernelGGL(k_mul_conj_batch, dim3(grid_of(Hi * nb)), dim3(256), 0, s, pp->image2ft, pp->imft,
+                         pp->padft, Hi, nb);
+      GX_LAUNCHED("k_mul_conj_batch");
+      GX_TRY(fft_c2r(pp->padft, pp->padft, pp->sstd, pp->di, nb, 1.0f, s));
+      hipLaunchKernelGGL(k_solvent_stats, dim3(grid_of(Vi * nb)), dim3(256), 0, s, pp->smean, pp->sstd, Vi, pp->coef,
+                         nb);
       GX_LAUNCHED("k_solvent_stats");
     }
-    hipLaunchKernelGGL(k_prob_update, dim3(grid_of(Vi)), dim3(256), 0, s, pp->real, pp->smean, pp->sstd, pp->coef,
-                       pp->dang, nb, Vi, (int)n, d_bestccf, reinterpret_cast<float3*>(d_bestangle), d_bestref);
+    hipLaunchKernelGGL(k_prob_update, dim3(grid_of(Vi)), dim3(256), 0, s, pp->real, pp->smean, pp->sstd,
+                       pp->circular ? (size_t)0 : Vi, pp->coef, pp->dang, nb, Vi, (int)n, d_bestccf,
+                       reinterpret_cast<float3*>(d_bestangle), d_bestref);
     GX_LAUNCHED("k_prob_update");
   }
   GX_HIP(hipStreamSynchronize(s));

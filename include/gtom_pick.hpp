@@ -538,8 +538,8 @@ class Picker {
   gtomx_picker* handle_ = nullptr;
 };
 
-// class ProbabilisticPicker, include/Correlation.cuh:166-207, circular masks
-// only. The reference keeps its low-pass references and masks in pinned host
+// class ProbabilisticPicker, include/Correlation.cuh:166-207, circular and
+// non-circular masks. The reference keeps its low-pass references and masks in pinned host
 // memory and uploads one per PerformCorrelation; here they stay on the device.
 // PickImage is declared but never defined in the reference and is absent here.
 class ProbabilisticPicker {

@@ -365,7 +365,9 @@ typedef struct gtomx_prob_picker gtomx_prob_picker;
  * src/Correlation/ProbabilisticPicker.cu:33-152: nrefs centred references
  * (d_refs) and masks (d_refmasks) of dimsref for images of dimsimage (2D when
  * dims.z == 1), band-limited to |k| < lowpassfreq and rotated at 2*lowpassfreq.
- * Only circular masks (ismaskcircular != 0) are supported. The low-pass spectra
+ * A non-circular mask (ismaskcircular == 0) is kept as cubic B-spline
+ * coefficients and rotated with each orientation (d_Rotate3D / d_Rotate2D,
+ * T_INTERP_CUBIC, exact weights), with its own solvent statistics. The low-pass spectra
  * are stored as the reference stores them, including what its half remap
  * (src/FFT/FFTRemap.cu, RemapHalfFFT2HalfKernel) reads past the cropped spectrum
  * for the zero-frequency row and plane; dimsimage must be large enough that

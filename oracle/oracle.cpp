@@ -1194,8 +1194,81 @@ void rotate2d_ft(const double* in, double* out, int nx, int ny, double angle, do
     }
 }
 
-// ProbabilisticPicker (src/Correlation/ProbabilisticPicker.cu) with a circular
-// mask, for one reference (index refid), in the reference's order:
+// Cubic B-spline prefilter of one line (ConvertToInterpolationCoefficients,
+// src/CubicSplines/internal/cubicPrefilter_kernel.cu): pole z = sqrt(3) - 2, gain
+// (1 - z)(1 - 1/z), causal start c0 + sum_{n < min(12, len)} z^(n+1) c_n (the
+// reference's truncated clamped-boundary sum), anticausal start z/(z - 1) c_(len-1).
+void bspline_line(double* c, int len, size_t stride) {
+  const double z = std::sqrt(3.0) - 2.0, lambda = (1.0 - z) * (1.0 - 1.0 / z);
+  const int horizon = std::min(12, len);
+  double zn = z, sum = c[0];
+  for (int n = 0; n < horizon; ++n) {
+    sum += zn * c[(size_t)n * stride];
+    zn *= z;
+  }
+  double prev = lambda * sum;
+  c[0] = prev;
+  for (int n = 1; n < len; ++n) c[(size_t)n * stride] = prev = lambda * c[(size_t)n * stride] + z * prev;
+  c[(size_t)(len - 1) * stride] = prev = (z / (z - 1.0)) * prev;
+  for (int n = len - 2; n >= 0; --n) c[(size_t)n * stride] = prev = z * (prev - c[(size_t)n * stride]);
+}
+
+// d_CubicBSplinePrefilter3D / 2D (src/CubicSplines/cubicPrefilter3D.cu:114-127,
+// cubicPrefilter2D.cu): lines along x, then y, then z.
+void bspline_prefilter(double* v, int nx, int ny, int nz) {
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y) bspline_line(v + ((size_t)z * ny + y) * nx, nx, 1);
+  for (int z = 0; z < nz; ++z)
+    for (int x = 0; x < nx; ++x) bspline_line(v + (size_t)z * ny * nx + x, ny, nx);
+  if (nz > 1)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) bspline_line(v + (size_t)y * nx + x, nz, (size_t)nx * ny);
+}
+
+double bspline_w(double t) {
+  t = std::fabs(t);
+  const double a = 2.0 - t;
+  if (t < 1.0) return 2.0 / 3.0 - 0.5 * t * t * a;
+  if (t < 2.0) return a * a * a / 6.0;
+  return 0.0;
+}
+
+// d_Rotate3D / d_Rotate2D with T_INTERP_CUBIC into FFT order
+// (Rotate3DKernel/Rotate2DKernel<true, false>, src/Transformation/Rotation.cu):
+// output voxel i reads x = (i + n/2) % n per axis at p = M (x - n/2) + n/2 + 0.5,
+// the cubic B-spline of coef at p - 0.5 over 4^ndims taps with clamped indices
+// (cubicTex3DSimple, include/CubicInterp.cuh:287-312). M: row-major 3x3 (3D,
+// transpose(Euler)), or the 2x2 R(-angle) of Matrix2Rotation in M[0], M[1],
+// M[3], M[4] (2D, nz = 1).
+void rotate_cubic(const double* coef, double* out, int nx, int ny, int nz, const double* M) {
+  const bool three = nz > 1;
+  auto cl = [](long v, long hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const double cx = (ix + nx / 2) % nx - nx / 2, cy = (iy + ny / 2) % ny - ny / 2,
+                     cz = three ? (iz + nz / 2) % nz - nz / 2 : 0;
+        const double gx = M[0] * cx + M[1] * cy + M[2] * cz + nx / 2;
+        const double gy = M[3] * cx + M[4] * cy + M[5] * cz + ny / 2;
+        const double gz = three ? M[6] * cx + M[7] * cy + M[8] * cz + nz / 2 : 0;
+        const long fx = (long)std::floor(gx), fy = (long)std::floor(gy), fz = (long)std::floor(gz);
+        double r = 0;
+        for (int tz = three ? -1 : 0; tz <= (three ? 2 : 0); ++tz) {
+          const double wz = three ? bspline_w(tz - (gz - fz)) : 1.0;
+          const long zz = three ? cl(fz + tz, nz - 1) : 0;
+          for (int ty = -1; ty <= 2; ++ty) {
+            const double wyz = bspline_w(ty - (gy - fy)) * wz;
+            const long yy = cl(fy + ty, ny - 1);
+            for (int tx = -1; tx <= 2; ++tx)
+              r += bspline_w(tx - (gx - fx)) * wyz * coef[((size_t)zz * ny + yy) * nx + cl(fx + tx, nx - 1)];
+          }
+        }
+        out[((size_t)iz * ny + iy) * nx + ix] = r;
+      }
+}
+
+// ProbabilisticPicker (src/Correlation/ProbabilisticPicker.cu), for one
+// reference (index refid), in the reference's order:
 //  Initialize (:33-152): R = ifftshift(pad(ref, dimsimage, 0)) (:128-129),
 //   C = FFTCrop(R2C(R), 2*lowpass) (:131-132), C *= [|k| < lowpass] (d_Bandpass
 //   :136 with a hard sphere, src/ImageManipulation/Bandpass.cu:37-87),
@@ -1213,9 +1286,13 @@ void rotate2d_ft(const double* in, double* out, int nx, int ny, double angle, do
 //   solvent mean/std at the first orientation (CalcSolventStatistics :423-454);
 //   the score of ProbPickingUpdateKernel (:391-421), kept where it exceeds best.
 // top2 (optional) receives the second-highest score over the orientations.
+// With a non-circular mask (circular = false): the padded mask, centred, as
+// cubic B-spline coefficients (:102-110); per orientation it is rotated into FFT
+// order (:273-279) and replaces M, and the inverse mask, its spectrum and the
+// solvent statistics are recomputed from its crop (:342-356).
 void prob_picker(const double* ref, const double* mask, const int* dr, const int* di, int lowpass, bool doctf,
                  const double* image, const double* ctf, float anglestep, int refid, double* bestccf,
-                 double* bestangle, int* bestref, double* top2) {
+                 double* bestangle, int* bestref, double* top2, bool circular) {
   const bool three = di[2] > 1;
   const int n = 2 * lowpass;
   const int dl[3] = {n, n, three ? n : 1};
@@ -1283,10 +1360,12 @@ void prob_picker(const double* ref, const double* mask, const int* dr, const int
           }
         }
   }
-  const std::vector<double> M = shifted(pad(mask));
+  std::vector<double> M = circular ? shifted(pad(mask)) : pad(mask);
+  if (!circular) bspline_prefilter(M.data(), di[0], di[1], di[2]);
   std::vector<double> om(Er);
   for (size_t i = 0; i < Er; ++i) om[i] = 1.0 - mask[i];
-  const std::vector<cpx<double>> IM = r2c(shifted(pad(om.data())));
+  std::vector<cpx<double>> IM = r2c(shifted(pad(om.data())));
+  std::vector<double> Mrot(Vi);
 
   // SetImage
   std::vector<double> imn(Vi), sq(Vi);
@@ -1321,12 +1400,22 @@ void prob_picker(const double* ref, const double* mask, const int* dr, const int
   std::vector<double> rot(2 * Hl);
   std::vector<cpx<double>> P(Hi);
   for (size_t a = 0; a < ang.size() / 3; ++a) {
+    double Mr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (three) {
-      double Mr[9];
       projector_matrix(ang[3 * a], ang[3 * a + 1], ang[3 * a + 2], 1.0, Mr);
       rotate3d_ft(Hc.data(), rot.data(), n, Mr, false, 2);
     } else {
       rotate2d_ft(Hc.data(), rot.data(), n, n, ang[3 * a + 2], (double)(float)(n / 2), false);
+      const double c = (float)std::cos(-(double)ang[3 * a + 2]), sn = (float)std::sin(-(double)ang[3 * a + 2]);
+      Mr[0] = c;
+      Mr[1] = -sn;
+      Mr[3] = sn;
+      Mr[4] = c;
+    }
+    const double* Mk = M.data();
+    if (!circular) {
+      rotate_cubic(M.data(), Mrot.data(), di[0], di[1], di[2], Mr);
+      Mk = Mrot.data();
     }
     if (doctf)
       for (size_t i = 0; i < Hl; ++i) {
@@ -1335,13 +1424,13 @@ void prob_picker(const double* ref, const double* mask, const int* dr, const int
       }
     fft_resize(2, rot.data(), reinterpret_cast<double*>(P.data()), dl, di, 2);
     std::vector<double> r = c2r(P);
-    for (size_t i = 0; i < Vi; ++i) r[i] = r[i] / (double)Vi * M[i];
+    for (size_t i = 0; i < Vi; ++i) r[i] = r[i] / (double)Vi * Mk[i];
     std::vector<cpx<double>> B1 = r2c(r);
     for (size_t i = 0; i < Hi; ++i) B1[i] = I[i] * std::conj(B1[i]);
     const std::vector<double> corr = c2r(B1);
     std::vector<double> refc(Er), maskc(Er);
     fft_resize(3, r.data(), refc.data(), di, dr, 1);
-    fft_resize(3, M.data(), maskc.data(), di, dr, 1);
+    fft_resize(3, Mk, maskc.data(), di, dr, 1);
     double smr = 0, smr2 = 0, samples = 0, epr[5] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < Er; ++i) {
       samples += maskc[i];
@@ -1356,7 +1445,13 @@ void prob_picker(const double* ref, const double* mask, const int* dr, const int
     for (int k = 0; k < 5; ++k) ep += epr[k];
     ep /= 5.0;
     ep = std::exp(ep / (2.0 * samples));
-    if (a == 0) {
+    if (!circular) {
+      std::vector<double> omc(Er), padded(Vi);
+      for (size_t i = 0; i < Er; ++i) omc[i] = 1.0 - maskc[i];
+      fft_resize(4, omc.data(), padded.data(), dr, di, 1);
+      IM = r2c(padded);
+    }
+    if (a == 0 || !circular) {
       const double f = 1.0 / ((double)Vi * ((double)Er - samples));
       std::vector<cpx<double>> t(Hi);
       for (size_t i = 0; i < Hi; ++i) t[i] = I[i] * std::conj(IM[i]);
@@ -1671,9 +1766,15 @@ void orc_rotate2d_ft_f64(const double* in, double* out, int nx, int ny, double a
 
 void orc_prob_picker_f64(const double* ref, const double* mask, const int* dimsref, const int* dimsimage,
                          int lowpass, int doctf, const double* image, const double* ctf, float anglestep, int refid,
-                         double* bestccf, double* bestangle, int* bestref, double* top2) {
+                         double* bestccf, double* bestangle, int* bestref, double* top2, int circular) {
   prob_picker(ref, mask, dimsref, dimsimage, lowpass, doctf != 0, image, ctf, anglestep, refid, bestccf, bestangle,
-              bestref, top2);
+              bestref, top2, circular != 0);
+}
+
+void orc_bspline_prefilter_f64(double* v, int nx, int ny, int nz) { bspline_prefilter(v, nx, ny, nz); }
+
+void orc_rotate_cubic_f64(const double* coef, double* out, int nx, int ny, int nz, const double* M) {
+  rotate_cubic(coef, out, nx, ny, nz, M);
 }
 
 // The oracle's own gaussrand stream: srand(seed), reset of its phase, n draws.

@@ -65,7 +65,9 @@ def lib():
         L.orc_pick2d_f64.restype = _d
         L.orc_project2d_f64.argtypes = [_p, _i, _p, _i, _p, _i]
         L.orc_rotate2d_ft_f64.argtypes = [_p, _p, _i, _i, _d, _d, _i]
-        L.orc_prob_picker_f64.argtypes = [_p, _p, _p, _p, _i, _i, _p, _p, _f, _i, _p, _p, _p, _p]
+        L.orc_prob_picker_f64.argtypes = [_p, _p, _p, _p, _i, _i, _p, _p, _f, _i, _p, _p, _p, _p, _i]
+        L.orc_bspline_prefilter_f64.argtypes = [_p, _i, _i, _i]
+        L.orc_rotate_cubic_f64.argtypes = [_p, _p, _i, _i, _i, _p]
         L.orc_gaussrand_f32.argtypes = [ctypes.c_uint, _i, _p, _i]
         _lib = L
     return _lib
@@ -459,10 +461,34 @@ def gaussrand(n, seed=123, reset=True):
     return out
 
 
-def prob_picker(ref, mask, image, lowpass, anglestep, ctf=None, refid=0, best=None):
-    """ProbabilisticPicker with a circular mask, one reference: returns (best,
-    bestangle, bestref, top2), arrays of the image shape (bestangle + (3,)).
-    Starts from best (default -1e30 everywhere)."""
+def bspline_prefilter(vol):
+    """d_CubicBSplinePrefilter3D / 2D of a copy of vol ((nz, ny, nx) or (ny, nx))."""
+    v = np.array(vol, np.float64, copy=True, order="C")
+    shp = v.shape if v.ndim == 3 else (1,) + v.shape
+    lib().orc_bspline_prefilter_f64(_ptr(v), shp[2], shp[1], shp[0])
+    return v
+
+
+def rotate_cubic(coef, M):
+    """d_Rotate3D / d_Rotate2D (T_INTERP_CUBIC, FFT-order output) of B-spline
+    coefficients; M is the 3x3 transpose(Euler) or the 2x2 R(-angle)."""
+    c = _c(coef, np.float64)
+    shp = c.shape if c.ndim == 3 else (1,) + c.shape
+    m = np.zeros(9)
+    Mi = np.asarray(M, np.float64)
+    if Mi.shape == (2, 2):
+        m[[0, 1, 3, 4]] = Mi.ravel()
+    else:
+        m[:] = Mi.ravel()
+    out = np.empty_like(c)
+    lib().orc_rotate_cubic_f64(_ptr(c), _ptr(out), shp[2], shp[1], shp[0], _ptr(m))
+    return out
+
+
+def prob_picker(ref, mask, image, lowpass, anglestep, ctf=None, refid=0, best=None, circular=True):
+    """ProbabilisticPicker, one reference: returns (best, bestangle, bestref,
+    top2), arrays of the image shape (bestangle + (3,)). Starts from best
+    (default -1e30 everywhere). circular=False rotates the mask per orientation."""
     r = _c(ref, np.float64)
     m = _c(mask, np.float64)
     im = _c(image, np.float64)
@@ -476,5 +502,5 @@ def prob_picker(ref, mask, image, lowpass, anglestep, ctf=None, refid=0, best=No
     c = None if ctf is None else _c(ctf, np.float64)
     lib().orc_prob_picker_f64(_ptr(r), _ptr(m), _ptr(dr), _ptr(di), int(lowpass), int(ctf is not None), _ptr(im),
                               None if c is None else _ptr(c), float(anglestep), int(refid), _ptr(b), _ptr(ang),
-                              _ptr(bref), _ptr(top2))
+                              _ptr(bref), _ptr(top2), int(bool(circular)))
     return b, ang, bref, top2

@@ -119,10 +119,6 @@ def test_prob_picker_host_paths(gx, orc):
     assert L.gtomx_prob_picker_count_angles(2, 0.0, ctypes.byref(c)) == gx.ERR_INVALID_ARGUMENT
     h = ctypes.c_void_p()
     fake = ctypes.c_void_p(256)
-    # non-circular masks
-    assert L.gtomx_prob_picker_create(ctypes.byref(h), fake, i3(16, 16, 1), 1, fake, 0, 0, i3(64, 64, 1), 8, None) \
-        == gx.ERR_UNSUPPORTED
-    assert "circular" in gx.last_error()
     # an image too small next to the low-pass box: the remap would read unwritten memory
     assert L.gtomx_prob_picker_create(ctypes.byref(h), fake, i3(8, 8, 1), 1, fake, 1, 0, i3(16, 16, 1), 8, None) \
         == gx.ERR_UNSUPPORTED

@@ -154,3 +154,37 @@ def test_gaussrand_phase_carries_between_calls(gx, orc, cuda):
     b1f, _, _, _ = orc.prob_picker(refs[1].astype(np.float64), masks[1].astype(np.float64), image.astype(np.float64),
                                    lp, step, refid=1)
     assert rel_linf(b1f, b1) > 1e-3
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_noncircular_mask_matches_oracle(gx, orc, cuda, nd):
+    """A non-circular (ellipsoidal) mask, rotated per orientation by cubic
+    B-splines into FFT order, with per-orientation solvent statistics."""
+    rng = np.random.default_rng(110 + nd)
+    if nd == 2:
+        di, dr, lp, step = (64, 64), (16, 16), 8, 0.5  # 13 orientations
+    else:
+        di, dr, lp, step = (32, 32, 32), (12, 12, 12), 6, 1.2  # 50 orientations
+    g = np.meshgrid(*[np.arange(s) - s // 2 for s in dr], indexing="ij")
+    radii = [3.0 + 1.5 * i for i in range(nd)]
+    mask = (sum((a / r) ** 2 for a, r in zip(g, radii)) < 1).astype(np.float32)
+    refs = rng.standard_normal((1,) + dr).astype(np.float32)
+    image = rng.standard_normal(di).astype(np.float32)
+    import torch
+    pp = gx.ProbabilisticPicker(torch.from_numpy(refs).to(cuda), torch.from_numpy(mask[None]).to(cuda), di, lp,
+                                ismaskcircular=False)
+    pp.set_image(torch.from_numpy(image).to(cuda))
+    b = torch.full(di, -1e30, device=cuda)
+    a = torch.zeros(di + (3,), device=cuda)
+    r = torch.full(di, -1, dtype=torch.int32, device=cuda)
+    pp.perform_correlation(0, step, b, a, r)
+    torch.cuda.synchronize()
+    pp.close()
+    orc.gaussrand(0, reset=True)
+    ob, oa, _, top2 = orc.prob_picker(refs[0].astype(np.float64), mask.astype(np.float64), image.astype(np.float64),
+                                      lp, step, refid=0, circular=False)
+    assert rel_linf(to_np(b).astype(np.float64), ob) <= 1e-4
+    sel = (ob - top2) > 1e-4 * np.abs(ob).max()
+    assert sel.mean() > 0.9
+    assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
+    assert np.all(to_np(r) == 0)

@@ -264,3 +264,84 @@ def test_picker_zero_frequency_reads_matter():
     plain = _np_prob_identity(ref, mask, image, lp, G, plain_shift=True)
     assert np.allclose(best, quirk, rtol=1e-12, atol=1e-12 * np.abs(quirk).max())
     assert np.abs(best - plain).max() > 1e-2 * np.abs(plain).max()
+
+
+# ---------------------------------------------------------------- non-circular masks
+@pytest.mark.parametrize("shape", [(40, 36), (24, 30), (20, 18, 22), (32, 30, 34)])
+def test_bspline_prefilter_interior_equals_scipy(shape):
+    """Away from the ends (where the reference's truncated clamped start sum sits)
+    the coefficients equal scipy's exact cubic spline prefilter, and evaluating
+    them at the grid points gives the samples back."""
+    from scipy.ndimage import spline_filter
+    rng = np.random.default_rng(len(shape))
+    v = rng.standard_normal(shape)
+    c = po.bspline_prefilter(v)
+    want = spline_filter(v, order=3, mode="mirror")
+    inner = tuple(slice(14, s - 14) for s in shape)
+    if all(s > 28 for s in shape):
+        assert np.allclose(c[inner], want[inner], atol=1e-7)
+    # interpolation at the grid points: (c[i-1] + 4 c[i] + c[i+1]) / 6 along every axis
+    back = c.copy()
+    for ax in range(len(shape)):
+        back = (np.roll(back, 1, ax) + 4 * back + np.roll(back, -1, ax)) / 6
+    inner2 = tuple(slice(2, s - 2) for s in shape)
+    assert np.allclose(back[inner2], v[inner2], atol=1e-6)
+    # a constant stays that constant up to the truncated start sum
+    k = po.bspline_prefilter(np.full(shape, 2.5))
+    assert np.allclose(k, 2.5, atol=2.5e-6)
+
+
+def test_rotate_cubic_identity_is_the_fft_shifted_samples():
+    rng = np.random.default_rng(3)
+    v = np.zeros((16, 16, 16))
+    v[4:12, 4:12, 4:12] = rng.standard_normal((8, 8, 8))
+    out = po.rotate_cubic(po.bspline_prefilter(v), np.eye(3))
+    assert np.allclose(out, np.fft.ifftshift(v), atol=1e-10)  # content within the first 12 samples: exact
+    v2 = np.zeros((20, 20))
+    v2[5:15, 5:15] = rng.standard_normal((10, 10))
+    out2 = po.rotate_cubic(po.bspline_prefilter(v2), np.eye(2))
+    # content past the 12-sample start sum: its truncation leaves ~z^13 ~ 4e-8 per sample
+    assert np.allclose(out2, np.fft.ifftshift(v2), atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_rotate_cubic_matches_scipy_spline_evaluation(seed):
+    """Output voxel i holds the cubic spline of the coefficients at
+    M (x - n/2) + n/2, x = (i + n/2) % n (scipy's order-3 map_coordinates of
+    the same coefficients without prefiltering), away from the clamped faces."""
+    from scipy.ndimage import map_coordinates
+    rng = np.random.default_rng(20 + seed)
+    n = 24
+    coef = rng.standard_normal((n, n, n))
+    M = po.projector_matrix(*rng.uniform(-np.pi, np.pi, 3), 1.0)
+    out = po.rotate_cubic(coef, M)
+    x = (np.arange(n) + n // 2) % n - n // 2
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    c = np.stack([X, Y, Z], -1).astype(float)
+    p = c @ M.T + n // 2  # (x, y, z) grid positions
+    sel = np.all((p > 2.5) & (p < n - 3.5), axis=-1)
+    want = map_coordinates(coef, [p[sel][:, 2], p[sel][:, 1], p[sel][:, 0]], order=3, prefilter=False, mode="nearest")
+    assert sel.sum() > 1000
+    assert np.allclose(out[sel], want, atol=1e-12)
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_noncircular_mask_at_identity_equals_the_circular_path(nd):
+    """At the identity orientation the rotated mask is the FFT-shifted padded
+    mask, so the per-orientation inverse mask and solvent statistics equal the
+    circular path's, for any mask (an ellipsoid here)."""
+    rng = np.random.default_rng(50 + nd)
+    if nd == 2:
+        di, dr, lp, step = (32, 32), (12, 12), 6, 7.0
+    else:
+        di, dr, lp, step = (16, 16, 16), (8, 8, 8), 4, 4.0
+    g = np.meshgrid(*[np.arange(s) - s // 2 for s in dr], indexing="ij")
+    radii = [2.5 + i for i in range(nd)]
+    mask = (sum((a / r) ** 2 for a, r in zip(g, radii)) < 1).astype(float)
+    ref = rng.standard_normal(dr)
+    image = rng.standard_normal(di)
+    po.gaussrand(0, reset=True)
+    bc, _, _, _ = po.prob_picker(ref, mask, image, lp, step, circular=True)
+    po.gaussrand(0, reset=True)
+    bn, _, _, _ = po.prob_picker(ref, mask, image, lp, step, circular=False)
+    assert np.allclose(bn, bc, rtol=1e-9, atol=1e-9 * np.abs(bc).max())
