@@ -104,8 +104,8 @@ _SIGS = {
     "gtomx_ctf_simulate": [_P, _P, _P, _P, _U, _I, _I, _U, _P],
     "gtomx_ctf_2d": [_P, _U, int3, _P, _P],
     "gtomx_ctf_tilt_params_grid": [_P, int3, int3, _P, _U, _P],
-    "gtomx_rotate3d_ft": [_P, _P, int3, _P, _U, _I, _P],
-    "gtomx_rotate3d_ft_real": [_P, _P, int3, _P, _U, _I, _P],
+    "gtomx_rotate3d_ft": [_P, _P, int3, _P, _U, _I, _I, _P],
+    "gtomx_rotate3d_ft_real": [_P, _P, int3, _P, _U, _I, _I, _P],
     "gtomx_pick_subtomograms_diff2": [_P, _F, int3, _P, _P, int3, _U, int3, _P, _U, _P, _P, _P],
     "gtomx_mrc_header_default": [_P],
     "gtomx_mrc_read_header": [ctypes.c_char_p, _P],
@@ -131,7 +131,7 @@ _SIGS = {
     "gtomx_prob_picker_correlate": [_P, _U, _F, _P, _P, _P, _P],
     "gtomx_prob_picker_count_angles": [_I, _F, ctypes.POINTER(_I)],
     "gtomx_debug_rand_outputs": [_U, ctypes.c_longlong, _I, _P],
-    "gtomx_rotate2d_ft": [_P, _P, int3, _P, _F, _I, _U, _P],
+    "gtomx_rotate2d_ft": [_P, _P, int3, _P, _F, _I, _I, _U, _P],
     "gtomx_pick_plan_create": [ctypes.POINTER(_P), int3, int3, _U],
     "gtomx_pick_plan_destroy": [_P],
     "gtomx_pick_plan_info": [_P, ctypes.POINTER(_U), ctypes.POINTER(_SZ)],
@@ -322,22 +322,26 @@ def d_rlnProject(volume_ft, p, n, matrices=None, angles=None, oversample=1.0, rm
     return out
 
 
-def d_Rotate3DFT(volume, angles, outputzerocentered=True, stream=None):
-    """d_Rotate3DFT, linear (src/Transformation/Rotation.cu:263-374): a (n, n, n/2+1)
+INTERP_LINEAR, INTERP_CUBIC = 1, 2
+
+
+def d_Rotate3DFT(volume, angles, outputzerocentered=True, stream=None, mode=INTERP_LINEAR):
+    """d_Rotate3DFT (src/Transformation/Rotation.cu:263-374): a (n, n, n/2+1)
     zero-centred half spectrum (complex64 or float32 CUDA tensor) rotated by each
-    (rot, tilt, psi) of `angles`; returns (nangles, n, n, n/2+1) of the same dtype."""
+    (rot, tilt, psi) of `angles`, linear or cubic; returns (nangles, n, n, n/2+1)
+    of the same dtype."""
     import numpy as np
     import torch
     a = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(-1, 3))
     n = volume.shape[0]
     out = torch.empty((len(a),) + tuple(volume.shape), dtype=volume.dtype, device=volume.device)
     fn = lib().gtomx_rotate3d_ft if volume.is_complex() else lib().gtomx_rotate3d_ft_real
-    check(fn(_ptr(volume), _ptr(out), int3(n, n, n), ctypes.c_void_p(a.ctypes.data), len(a), int(bool(outputzerocentered)),
-             _stream(stream)), "gtomx_rotate3d_ft")
+    check(fn(_ptr(volume), _ptr(out), int3(n, n, n), ctypes.c_void_p(a.ctypes.data), len(a), int(mode),
+             int(bool(outputzerocentered)), _stream(stream)), "gtomx_rotate3d_ft")
     return out
 
 
-def d_Rotate2DFT(image, angles, maxfreq, outputzerocentered=True, stream=None):
+def d_Rotate2DFT(image, angles, maxfreq, outputzerocentered=True, stream=None, mode=INTERP_LINEAR):
     """d_Rotate2DFT, linear (src/Transformation/Rotation.cu:200-256): (batch, n, n/2+1)
     zero-centred complex64 half spectra, item b rotated by angles[b] (radians);
     returns a new tensor of the same shape."""
@@ -349,7 +353,7 @@ def d_Rotate2DFT(image, angles, maxfreq, outputzerocentered=True, stream=None):
     ny, nh = x.shape[-2:]
     out = torch.empty_like(x)
     check(lib().gtomx_rotate2d_ft(_ptr(x), _ptr(out), int3(2 * (nh - 1), ny, 1), ctypes.c_void_p(a.ctypes.data),
-                                  float(maxfreq), int(bool(outputzerocentered)), len(a), _stream(stream)),
+                                  float(maxfreq), int(mode), int(bool(outputzerocentered)), len(a), _stream(stream)),
           "gtomx_rotate2d_ft")
     return out
 

@@ -42,6 +42,7 @@
 #include <thread>
 #include <vector>
 
+#include "dev_bspline.hpp"
 #include "runtime.hpp"
 
 namespace gx {
@@ -115,56 +116,6 @@ __global__ __launch_bounds__(256) void k_mul_conj_into(const float2* __restrict_
     const float2 x = a[i], y = b[i];
     out[i] = make_float2(x.x * y.x + x.y * y.y, x.y * y.x - x.x * y.y);
   }
-}
-
-// ConvertToInterpolationCoefficients (src/CubicSplines/internal/cubicPrefilter_kernel.cu):
-// the cubic B-spline prefilter of one line in place, pole z = sqrt(3) - 2, gain
-// (1 - z)(1 - 1/z), causal start c0 + sum_{n < min(12, len)} z^(n+1) c_n, anticausal
-// start z/(z - 1) c_(len-1), in float. Lines along axis 0 (x), 1 (y) or 2 (z) of
-// `batch` volumes (nx, ny, nz), one thread per line (SamplesToCoefficients3D{X,Y,Z}
-// and the 2D forms, src/CubicSplines/cubicPrefilter3D.cu, cubicPrefilter2D.cu).
-__global__ __launch_bounds__(256) void k_bspline_lines(float* __restrict__ v, int3 d, int axis, unsigned batch) {
-  const size_t V = (size_t)d.x * d.y * d.z;
-  const int len = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
-  const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)d.x : (size_t)d.x * d.y);
-  const size_t per = V / len;
-  const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (line >= per * batch) return;
-  const size_t b = line / per, l = line % per;
-  size_t start;
-  if (axis == 0) start = l * d.x;
-  else if (axis == 1) start = (l / d.x) * ((size_t)d.x * d.y) + l % d.x;
-  else start = l;
-  float* c = v + b * V + start;
-  const float z = sqrtf(3.0f) - 2.0f;
-  const float lambda = (1.0f - z) * (1.0f - 1.0f / z);
-  const int horizon = len < 12 ? len : 12;
-  float zn = z, sum = c[0];
-  for (int n = 0; n < horizon; ++n) {
-    sum += zn * c[(size_t)n * stride];
-    zn *= z;
-  }
-  float prev = lambda * sum;
-  c[0] = prev;
-  for (int n = 1; n < len; ++n) {
-    prev = lambda * c[(size_t)n * stride] + z * prev;
-    c[(size_t)n * stride] = prev;
-  }
-  prev = (z / (z - 1.0f)) * prev;
-  c[(size_t)(len - 1) * stride] = prev;
-  for (int n = len - 2; n >= 0; --n) {
-    prev = z * (prev - c[(size_t)n * stride]);
-    c[(size_t)n * stride] = prev;
-  }
-}
-
-// bspline() (include/CubicInterp.cuh:119-130)
-__device__ __forceinline__ float bspline(float t) {
-  t = fabsf(t);
-  const float a = 2.0f - t;
-  if (t < 1.0f) return 2.0f / 3.0f - 0.5f * t * t * a;
-  if (t < 2.0f) return a * a * a / 6.0f;
-  return 0.0f;
 }
 
 // d_Rotate3D / d_Rotate2D with T_INTERP_CUBIC into FFT order (Rotate3DKernel,
@@ -691,13 +642,7 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   if (!pp->circular) {
     // the padded masks, centred, as cubic B-spline coefficients (:102-110)
     if ((st = gtomx_pad(d_refmasks, pp->masks, dimsref, dimsimage, 0.0f, nrefs, stream)) != GTOMX_OK) return fin(st);
-    const int3 d3 = make_int3(dimsimage.x, dimsimage.y, dimsimage.z);
-    for (int axis = 0; axis < nd; ++axis) {
-      const size_t lines = Vi / (axis == 0 ? dimsimage.x : (axis == 1 ? dimsimage.y : dimsimage.z)) * nrefs;
-      hipLaunchKernelGGL(k_bspline_lines, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, pp->masks, d3,
-                         axis, nrefs);
-      if (hipGetLastError() != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "k_bspline_lines"));
-    }
+    if ((st = bspline_prefilter(pp->masks, dimsimage, nd, nrefs, 1, s)) != GTOMX_OK) return fin(st);
     if (hipStreamSynchronize(s) != hipSuccess) return fin(fail(GTOMX_ERR_DEVICE, "hipStreamSynchronize"));
     *out = pp;
     return GTOMX_OK;

@@ -25,12 +25,88 @@
 #include <cmath>
 #include <vector>
 
+#include "dev_bspline.hpp"
 #include "runtime.hpp"
 
 namespace gx {
 namespace {
 
-template <bool COMPLEX>
+// ConvertToInterpolationCoefficients (src/CubicSplines/internal/cubicPrefilter_kernel.cu):
+// the cubic B-spline prefilter of one line in place, pole z = sqrt(3) - 2, gain
+// (1 - z)(1 - 1/z), causal start c0 + sum_{n < min(12, len)} z^(n+1) c_n, anticausal
+// start z/(z - 1) c_(len-1), in float; one thread per (line, channel) along
+// `axis` (SamplesToCoefficients3D{X,Y,Z} and the 2D forms).
+__global__ __launch_bounds__(256) void k_bspline_lines(float* __restrict__ v, int3 d, int axis, unsigned batch,
+                                                       int ch) {
+  const size_t V = (size_t)d.x * d.y * d.z;
+  const int len = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+  const size_t stride = (axis == 0 ? 1 : (axis == 1 ? (size_t)d.x : (size_t)d.x * d.y)) * ch;
+  const size_t per = V / len;
+  const size_t id = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (id >= per * batch * ch) return;
+  const int c0 = (int)(id % ch);
+  const size_t line = id / ch;
+  const size_t b = line / per, l = line % per;
+  size_t start;
+  if (axis == 0) start = l * d.x;
+  else if (axis == 1) start = (l / d.x) * ((size_t)d.x * d.y) + l % d.x;
+  else start = l;
+  float* c = v + (b * V + start) * ch + c0;
+  const float z = sqrtf(3.0f) - 2.0f;
+  const float lambda = (1.0f - z) * (1.0f - 1.0f / z);
+  const int horizon = len < 12 ? len : 12;
+  float zn = z, sum = c[0];
+  for (int n = 0; n < horizon; ++n) {
+    sum += zn * c[(size_t)n * stride];
+    zn *= z;
+  }
+  float prev = lambda * sum;
+  c[0] = prev;
+  for (int n = 1; n < len; ++n) {
+    prev = lambda * c[(size_t)n * stride] + z * prev;
+    c[(size_t)n * stride] = prev;
+  }
+  prev = (z / (z - 1.0f)) * prev;
+  c[(size_t)(len - 1) * stride] = prev;
+  for (int n = len - 2; n >= 0; --n) {
+    prev = z * (prev - c[(size_t)n * stride]);
+    c[(size_t)n * stride] = prev;
+  }
+}
+
+// Cubic B-spline of coefficients src (dims nxs x ny x nz, V values per voxel)
+// at texel coordinate (tx, ty, tz) = p + n/2: taps floor(t) - 1 .. floor(t) + 2
+// per axis, clamped (cubicTex3D/2D, include/CubicInterp.cuh, with exact weights).
+template <class V>
+__device__ __forceinline__ V cubic_at(const V* __restrict__ src, int nxs, int ny, int nz, float tx, float ty,
+                                      float tz) {
+  auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  const float fx0 = floorf(tx), fy0 = floorf(ty), fz0 = floorf(tz);
+  const float ax = tx - fx0, ay = ty - fy0, az = tz - fz0;
+  V r{};
+  const bool three = nz > 1;
+  for (int kz = three ? -1 : 0; kz <= (three ? 2 : 0); ++kz) {
+    const float wz = three ? bspline((float)kz - az) : 1.0f;
+    const int zz = three ? clampi((int)fz0 + kz, nz - 1) : 0;
+    for (int ky = -1; ky <= 2; ++ky) {
+      const float wyz = bspline((float)ky - ay) * wz;
+      const V* row = src + ((size_t)zz * ny + clampi((int)fy0 + ky, ny - 1)) * nxs;
+      for (int kx = -1; kx <= 2; ++kx) {
+        const float w = bspline((float)kx - ax) * wyz;
+        const V t = row[clampi((int)fx0 + kx, nxs - 1)];
+        if constexpr (std::is_same<V, float2>::value) {
+          r.x = fmaf(w, t.x, r.x);
+          r.y = fmaf(w, t.y, r.y);
+        } else {
+          r = fmaf(w, t, r);
+        }
+      }
+    }
+  }
+  return r;
+}
+
+template <bool COMPLEX, bool CUBIC = false>
 __global__ __launch_bounds__(256) void k_rotate3d_ft(const float* __restrict__ in, float* __restrict__ out, int n,
                                                      const float* __restrict__ mats, int zerocentred) {
   using V = typename std::conditional<COMPLEX, float2, float>::type;
@@ -70,6 +146,14 @@ __global__ __launch_bounds__(256) void k_rotate3d_ft(const float* __restrict__ i
   px += c;
   py += c;
   pz += c;
+  if constexpr (CUBIC) {
+    V v = cubic_at<V>(src, nh, n, n, px, py, pz);
+    if constexpr (COMPLEX) {
+      if (neg) v.y = -v.y;
+    }
+    *o = v;
+    return;
+  }
   const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
   const float ax = px - fx0, ay = py - fy0, az = pz - fz0;
   auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
@@ -101,6 +185,7 @@ __global__ __launch_bounds__(256) void k_rotate3d_ft(const float* __restrict__ i
 // (nx/2 - idx, ny-1 - ((idy + ny/2 - 1) % ny)). mats holds (c, s) per item with
 // c = cos(-angle), s = sin(-angle) rounded to float, as glm::mat2(c, s, -s, c).
 // in_stride 0 rotates one input by every angle.
+template <bool CUBIC = false>
 __global__ __launch_bounds__(256) void k_rotate2d_ft(const float2* __restrict__ in, size_t in_stride,
                                                      float2* __restrict__ out, int nx, int ny,
                                                      const float2* __restrict__ cs, float maxfreq, int zerocentred) {
@@ -132,6 +217,12 @@ __global__ __launch_bounds__(256) void k_rotate2d_ft(const float2* __restrict__ 
   }
   px += (float)(nx / 2);
   py += (float)(ny / 2);
+  if constexpr (CUBIC) {
+    float2 v = cubic_at<float2>(in + (size_t)b * in_stride, nh, ny, 1, px, py, 0.f);
+    if (neg) v.y = -v.y;
+    *o = v;
+    return;
+  }
   const float fx0 = floorf(px), fy0 = floorf(py);
   const float ax = px - fx0, ay = py - fy0;
   auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
@@ -160,8 +251,8 @@ int rotate3d_ft_batch(const float2* in, float2* out, int n, const float* d_mats,
 int rotate2d_ft_batch(const float2* in, size_t in_stride, float2* out, int nx, int ny, const float2* d_cs,
                       unsigned batch, float maxfreq, bool zerocentred, hipStream_t s) {
   const int per = (nx / 2 + 1) * ny;
-  hipLaunchKernelGGL(k_rotate2d_ft, dim3((unsigned)((per + 255) / 256), batch), dim3(256), 0, s, in, in_stride, out,
-                     nx, ny, d_cs, maxfreq, zerocentred ? 1 : 0);
+  hipLaunchKernelGGL(k_rotate2d_ft<false>, dim3((unsigned)((per + 255) / 256), batch), dim3(256), 0, s, in, in_stride,
+                     out, nx, ny, d_cs, maxfreq, zerocentred ? 1 : 0);
   GX_LAUNCHED("k_rotate2d_ft");
   return GTOMX_OK;
 }
@@ -171,11 +262,32 @@ void rotate2d_coefficients(float angle, float2* cs) {
   *cs = make_float2((float)c, (float)sn);
 }
 
+int bspline_prefilter(float* v, gtomx_int3 d, int ndims, unsigned batch, int channels, hipStream_t s) {
+  const size_t V = elements(d);
+  for (int axis = 0; axis < ndims; ++axis) {
+    const size_t len = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+    const size_t n = V / len * batch * channels;
+    hipLaunchKernelGGL(k_bspline_lines, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v,
+                       make_int3(d.x, d.y, d.z), axis, batch, channels);
+    GX_LAUNCHED("k_bspline_lines");
+  }
+  return GTOMX_OK;
+}
+
 namespace {
 
+int check_mode(int mode) {
+  if (mode != GTOMX_INTERP_LINEAR && mode != GTOMX_INTERP_CUBIC)
+    return fail(GTOMX_ERR_UNSUPPORTED, "interpolation is GTOMX_INTERP_LINEAR or GTOMX_INTERP_CUBIC");
+  return GTOMX_OK;
+}
+
+// For cubic interpolation the input is copied and prefiltered (the reference
+// prefilters its own Re and Im copies, src/Transformation/Rotation.cu:274-279).
 int rotate_launch(const float* in, float* out, gtomx_int3 dims, const gtomx_float3* h_angles, unsigned nangles,
-                  int zerocentred, bool complex_values, void* stream) {
+                  int mode, int zerocentred, bool complex_values, void* stream) {
   if (!in || !out || (nangles > 0 && !h_angles)) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
+  GX_TRY(check_mode(mode));
   if (dims.x < 2 || dims.y != dims.x || dims.z != dims.x)
     return fail(GTOMX_ERR_UNSUPPORTED, "cubes of side >= 2 only (the reference offsets every axis by dims.x/2)");
   if (nangles == 0) return ok();
@@ -185,22 +297,38 @@ int rotate_launch(const float* in, float* out, gtomx_int3 dims, const gtomx_floa
   std::vector<float> h((size_t)nangles * 9);
   GX_TRY(gtomx_projector_matrices(h_angles, nangles, 1.0f, h.data()));
   hipStream_t s = as_stream(stream);
-  DevBuf dm;
+  DevBuf dm, coef;
   GX_TRY(dm.alloc(h.size() * sizeof(float)));
   GX_HIP(hipMemcpyAsync(dm.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  const bool cubic = mode == GTOMX_INTERP_CUBIC;
+  const int ch = complex_values ? 2 : 1;
+  const float* src = in;
+  if (cubic) {
+    GX_TRY(coef.alloc((size_t)per * ch * sizeof(float)));
+    GX_HIP(hipMemcpyAsync(coef.p, in, (size_t)per * ch * sizeof(float), hipMemcpyDeviceToDevice, s));
+    GX_TRY(bspline_prefilter(coef.as<float>(), gtomx_int3{dims.x / 2 + 1, dims.y, dims.z}, 3, 1, ch, s));
+    src = coef.as<float>();
+  }
   const dim3 grid((unsigned)((per + 255) / 256), nangles);
-  if (complex_values)
-    hipLaunchKernelGGL((k_rotate3d_ft<true>), grid, dim3(256), 0, s, in, out, dims.x, dm.as<float>(), zerocentred);
+  if (complex_values && cubic)
+    hipLaunchKernelGGL((k_rotate3d_ft<true, true>), grid, dim3(256), 0, s, src, out, dims.x, dm.as<float>(),
+                       zerocentred);
+  else if (complex_values)
+    hipLaunchKernelGGL((k_rotate3d_ft<true>), grid, dim3(256), 0, s, src, out, dims.x, dm.as<float>(), zerocentred);
+  else if (cubic)
+    hipLaunchKernelGGL((k_rotate3d_ft<false, true>), grid, dim3(256), 0, s, src, out, dims.x, dm.as<float>(),
+                       zerocentred);
   else
-    hipLaunchKernelGGL((k_rotate3d_ft<false>), grid, dim3(256), 0, s, in, out, dims.x, dm.as<float>(), zerocentred);
+    hipLaunchKernelGGL((k_rotate3d_ft<false>), grid, dim3(256), 0, s, src, out, dims.x, dm.as<float>(), zerocentred);
   GX_LAUNCHED("k_rotate3d_ft");
   GX_HIP(hipStreamSynchronize(s));
   return ok();
 }
 
 int rotate2d_entry(const gtomx_complex* d_input, gtomx_complex* d_output, gtomx_int3 dims, const float* h_angles,
-                   float maxfreq, int outputzerocentered, unsigned batch, void* stream) {
+                   float maxfreq, int mode, int outputzerocentered, unsigned batch, void* stream) {
   if (!d_input || !d_output || (batch > 0 && !h_angles)) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
+  GX_TRY(check_mode(mode));
   if (dims.x < 2 || dims.y < 2 || dims.z != 1) return fail(GTOMX_ERR_UNSUPPORTED, "2D images (dims.z == 1) of side >= 2");
   if (batch == 0) return ok();
   if (batch > 65535) return fail(GTOMX_ERR_UNSUPPORTED, "more than 65535 images per call");
@@ -208,12 +336,25 @@ int rotate2d_entry(const gtomx_complex* d_input, gtomx_complex* d_output, gtomx_
   std::vector<float2> h(batch);
   for (unsigned b = 0; b < batch; ++b) rotate2d_coefficients(h_angles[b], &h[b]);
   hipStream_t s = as_stream(stream);
-  DevBuf dm;
+  DevBuf dm, coef;
   GX_TRY(dm.alloc(h.size() * sizeof(float2)));
   GX_HIP(hipMemcpyAsync(dm.p, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice, s));
   const size_t per = elements_ft(dims);
-  GX_TRY(rotate2d_ft_batch(reinterpret_cast<const float2*>(d_input), per, reinterpret_cast<float2*>(d_output), dims.x,
-                           dims.y, dm.as<float2>(), batch, maxfreq, outputzerocentered != 0, s));
+  const float2* src = reinterpret_cast<const float2*>(d_input);
+  if (mode == GTOMX_INTERP_CUBIC) {
+    GX_TRY(coef.alloc(per * batch * sizeof(float2)));
+    GX_HIP(hipMemcpyAsync(coef.p, d_input, per * batch * sizeof(float2), hipMemcpyDeviceToDevice, s));
+    GX_TRY(bspline_prefilter(coef.as<float>(), gtomx_int3{dims.x / 2 + 1, dims.y, 1}, 2, batch, 2, s));
+    src = coef.as<float2>();
+    const int nh = dims.x / 2 + 1;
+    hipLaunchKernelGGL(k_rotate2d_ft<true>, dim3((unsigned)((nh * dims.y + 255) / 256), batch), dim3(256), 0, s, src,
+                       per, reinterpret_cast<float2*>(d_output), dims.x, dims.y, dm.as<float2>(), maxfreq,
+                       outputzerocentered != 0 ? 1 : 0);
+    GX_LAUNCHED("k_rotate2d_ft");
+  } else {
+    GX_TRY(rotate2d_ft_batch(src, per, reinterpret_cast<float2*>(d_output), dims.x, dims.y, dm.as<float2>(), batch,
+                             maxfreq, outputzerocentered != 0, s));
+  }
   GX_HIP(hipStreamSynchronize(s));
   return ok();
 }
@@ -224,20 +365,20 @@ int rotate2d_entry(const gtomx_complex* d_input, gtomx_complex* d_output, gtomx_
 using namespace gx;
 
 extern "C" int gtomx_rotate2d_ft(const gtomx_complex* d_input, gtomx_complex* d_output, gtomx_int3 dims,
-                                 const float* h_angles, float maxfreq, int outputzerocentered, unsigned batch,
-                                 void* stream) {
-  return rotate2d_entry(d_input, d_output, dims, h_angles, maxfreq, outputzerocentered, batch, stream);
+                                 const float* h_angles, float maxfreq, int mode, int outputzerocentered,
+                                 unsigned batch, void* stream) {
+  return rotate2d_entry(d_input, d_output, dims, h_angles, maxfreq, mode, outputzerocentered, batch, stream);
 }
 
 extern "C" int gtomx_rotate3d_ft(const gtomx_complex* d_volume, gtomx_complex* d_output, gtomx_int3 dims,
-                                 const gtomx_float3* h_angles, unsigned nangles, int outputzerocentered,
+                                 const gtomx_float3* h_angles, unsigned nangles, int mode, int outputzerocentered,
                                  void* stream) {
   return rotate_launch(reinterpret_cast<const float*>(d_volume), reinterpret_cast<float*>(d_output), dims, h_angles,
-                       nangles, outputzerocentered ? 1 : 0, true, stream);
+                       nangles, mode, outputzerocentered ? 1 : 0, true, stream);
 }
 
 extern "C" int gtomx_rotate3d_ft_real(const float* d_volume, float* d_output, gtomx_int3 dims,
-                                      const gtomx_float3* h_angles, unsigned nangles, int outputzerocentered,
-                                      void* stream) {
-  return rotate_launch(d_volume, d_output, dims, h_angles, nangles, outputzerocentered ? 1 : 0, false, stream);
+                                      const gtomx_float3* h_angles, unsigned nangles, int mode,
+                                      int outputzerocentered, void* stream) {
+  return rotate_launch(d_volume, d_output, dims, h_angles, nangles, mode, outputzerocentered ? 1 : 0, false, stream);
 }

@@ -112,6 +112,11 @@ int rotate2d_ft_batch(const float2* in, size_t in_stride, float2* out, int nx, i
                       unsigned batch, float maxfreq, bool zerocentred, hipStream_t s);
 void rotate2d_coefficients(float angle, float2* cs);
 
+// d_CubicBSplinePrefilter3D / 2D (rotate.hip) in place on `batch` arrays of
+// dims, ndims axes (x, then y, then z), `channels` interleaved components (1
+// real, 2 complex), each line by ConvertToInterpolationCoefficients.
+int bspline_prefilter(float* v, gtomx_int3 dims, int ndims, unsigned batch, int channels, hipStream_t s);
+
 // d_Norm with T_NORM_MEAN01STD per item (picker.hip); scratch holds
 // norm_scratch_doubles(batch) doubles.
 int norm_mean01std(const float* in, float* out, size_t n, unsigned batch, double* scratch, hipStream_t s);

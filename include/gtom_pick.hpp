@@ -196,33 +196,30 @@ inline void d_LocalPeaks(tfloat* d_input, int3** h_peaks, int* h_peaksnum, int3 
 enum T_INTERP_MODE { T_INTERP_LINEAR = 1, T_INTERP_CUBIC = 2, T_INTERP_FOURIER = 3, T_INTERP_SINC = 4 };
 
 // d_Rotate3DFT, include/Transformation.cuh:54 (complex) and the real-valued
-// overload (src/Transformation/Rotation.cu:325): linear interpolation only.
+// overload (src/Transformation/Rotation.cu:325): T_INTERP_LINEAR or T_INTERP_CUBIC.
 inline void d_Rotate3DFT(tcomplex* d_volume, tcomplex* d_output, int3 dims, tfloat3* h_angles, int nangles,
                          T_INTERP_MODE mode, bool outputzerocentered) {
-  if (mode != T_INTERP_LINEAR) throw std::runtime_error("d_Rotate3DFT: only T_INTERP_LINEAR is built");
   detail::check(gtomx_rotate3d_ft(reinterpret_cast<const gtomx_complex*>(d_volume),
                                   reinterpret_cast<gtomx_complex*>(d_output), detail::i3(dims),
-                                  reinterpret_cast<const gtomx_float3*>(h_angles), (unsigned)nangles,
+                                  reinterpret_cast<const gtomx_float3*>(h_angles), (unsigned)nangles, (int)mode,
                                   outputzerocentered ? 1 : 0, nullptr),
                 "d_Rotate3DFT");
 }
 inline void d_Rotate3DFT(tfloat* d_volume, tfloat* d_output, int3 dims, tfloat3* h_angles, int nangles,
                          T_INTERP_MODE mode, bool outputzerocentered) {
-  if (mode != T_INTERP_LINEAR) throw std::runtime_error("d_Rotate3DFT: only T_INTERP_LINEAR is built");
   detail::check(gtomx_rotate3d_ft_real(d_volume, d_output, detail::i3(dims),
-                                       reinterpret_cast<const gtomx_float3*>(h_angles), (unsigned)nangles,
+                                       reinterpret_cast<const gtomx_float3*>(h_angles), (unsigned)nangles, (int)mode,
                                        outputzerocentered ? 1 : 0, nullptr),
                 "d_Rotate3DFT");
 }
 
 // d_Rotate2DFT, include/Transformation.cuh:52: `batch` images, image b rotated
-// by angles[b] (host array, radians); linear interpolation only.
+// by angles[b] (host array, radians); T_INTERP_LINEAR or T_INTERP_CUBIC.
 inline void d_Rotate2DFT(tcomplex* d_input, tcomplex* d_output, int3 dims, tfloat* angles, tfloat maxfreq,
                          T_INTERP_MODE mode, bool isoutputzerocentered, int batch = 1) {
-  if (mode != T_INTERP_LINEAR) throw std::runtime_error("d_Rotate2DFT: only T_INTERP_LINEAR is built");
   detail::check(gtomx_rotate2d_ft(reinterpret_cast<const gtomx_complex*>(d_input),
                                   reinterpret_cast<gtomx_complex*>(d_output), detail::i3(dims), angles, maxfreq,
-                                  isoutputzerocentered ? 1 : 0, (unsigned)batch, nullptr),
+                                  (int)mode, isoutputzerocentered ? 1 : 0, (unsigned)batch, nullptr),
                 "d_Rotate2DFT");
 }
 

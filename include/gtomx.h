@@ -405,35 +405,42 @@ void gtomx_debug_rand_outputs(unsigned seed, long long first, int n, int* out);
  * anglestep (no reference counterpart; the list is the reference's). */
 int gtomx_prob_picker_count_angles(int ndims, float anglestep, int* count);
 
-/* d_Rotate3DFT with T_INTERP_LINEAR, include/Transformation.cuh:54 /
- * src/Transformation/Rotation.cu:263-374 (Rotate3DFTKernel :541-654): each of
- * nangles rotations of the half spectrum of a cube of side n. Input in the
- * zero-centred half layout: element (tx, ty, tz) of (n/2+1, n, n) holds the
- * frequency (tx - n/2, ty - n/2, tz - n/2). Output bin (idx, idy, idz), in the
- * same layout when outputzerocentered, else in FFT order read at the negated
- * centred frequency as the reference does, is the input interpolated
- * trilinearly at transpose(Euler(angle)) applied to its centred frequency,
- * 0 where that lies at radius >= n/2, taken from the Hermitian mate (conjugate)
- * where its x is positive. The reference's hardware filter quantises weights
- * to 1/256; here they are exact float and texel indices are clamped. Output:
- * nangles * (n/2+1)*n*n values. Cubic interpolation is not built. Synchronous. */
+/* Interpolation modes of the rotations, the values of T_INTERP_MODE
+ * (include/Prerequisites.cuh:239-245). */
+enum { GTOMX_INTERP_LINEAR = 1, GTOMX_INTERP_CUBIC = 2 };
+
+/* d_Rotate3DFT, include/Transformation.cuh:54 / src/Transformation/Rotation.cu:263-374
+ * (Rotate3DFTKernel :541-654): each of nangles rotations of the half spectrum of
+ * a cube of side n. Input in the zero-centred half layout: element (tx, ty, tz)
+ * of (n/2+1, n, n) holds the frequency (tx - n/2, ty - n/2, tz - n/2). Output
+ * bin (idx, idy, idz), in the same layout when outputzerocentered, else in FFT
+ * order read at the negated centred frequency as the reference does, is the
+ * input interpolated at transpose(Euler(angle)) applied to its centred
+ * frequency, 0 where that lies at radius >= n/2, taken from the Hermitian mate
+ * (conjugate) where its x is positive. mode GTOMX_INTERP_LINEAR: trilinear;
+ * GTOMX_INTERP_CUBIC: the cubic B-spline of the prefiltered input
+ * (d_CubicBSplinePrefilter3D on a copy, cubicTex3D). The reference's texture
+ * unit quantises the weights of its fetches to 1/256; here they are exact float,
+ * and texel indices are clamped. Output: nangles * (n/2+1)*n*n values. Synchronous. */
 int gtomx_rotate3d_ft(const gtomx_complex* d_volume, gtomx_complex* d_output, gtomx_int3 dims,
-                      const gtomx_float3* h_angles, unsigned nangles, int outputzerocentered, void* stream);
+                      const gtomx_float3* h_angles, unsigned nangles, int mode, int outputzerocentered,
+                      void* stream);
 /* The same on real-valued half spectra (src/Transformation/Rotation.cu:325-374,
  * kernel :606-654): no conjugation. */
 int gtomx_rotate3d_ft_real(const float* d_volume, float* d_output, gtomx_int3 dims, const gtomx_float3* h_angles,
-                           unsigned nangles, int outputzerocentered, void* stream);
+                           unsigned nangles, int mode, int outputzerocentered, void* stream);
 
-/* d_Rotate2DFT with T_INTERP_LINEAR, include/Transformation.cuh:52 /
- * src/Transformation/Rotation.cu:200-256 (kernel :485-539): `batch` zero-centred
- * half spectra of dims.x x dims.y images (dims.z == 1), item b rotated by
- * h_angles[b] (radians): output bin (idx, idy) takes the input at R(-angle)
- * (idx - dims.x/2, idy - dims.y/2), 0 beyond maxfreq, read at the Hermitian mate
- * (imaginary part negated) when its x > 1e-5; written in place or, when
- * outputzerocentered is 0, at the FFT-order bin (dims.x/2 - idx, dims.y-1 -
- * ((idy + dims.y/2 - 1) % dims.y)). Exact bilinear weights, clamped indices. */
+/* d_Rotate2DFT, include/Transformation.cuh:52 / src/Transformation/Rotation.cu:200-256
+ * (kernel :485-539): `batch` zero-centred half spectra of dims.x x dims.y images
+ * (dims.z == 1), item b rotated by h_angles[b] (radians): output bin (idx, idy)
+ * takes the input at R(-angle) (idx - dims.x/2, idy - dims.y/2), 0 beyond
+ * maxfreq, read at the Hermitian mate (imaginary part negated) when its x >
+ * 1e-5; written in place or, when outputzerocentered is 0, at the FFT-order bin
+ * (dims.x/2 - idx, dims.y-1 - ((idy + dims.y/2 - 1) % dims.y)). mode as for
+ * gtomx_rotate3d_ft (bilinear, or the cubic B-spline of the prefiltered input);
+ * exact weights, clamped indices. */
 int gtomx_rotate2d_ft(const gtomx_complex* d_input, gtomx_complex* d_output, gtomx_int3 dims, const float* h_angles,
-                      float maxfreq, int outputzerocentered, unsigned batch, void* stream);
+                      float maxfreq, int mode, int outputzerocentered, unsigned batch, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Peak extraction (the consumer of the search output)                       */

@@ -742,9 +742,82 @@ void norm_ft_monolithic(const float* in, double* out, long long n) {
 // (tx - n/2, ty - n/2, tz - n/2); ncomp = 2 for interleaved complex, 1 for real.
 // Trilinear interpolation with exact weights and clamped texel indices (the
 // reference's texture: 8-bit weights; wrap mode on unnormalised coordinates
+// Cubic B-spline prefilter of one line (ConvertToInterpolationCoefficients,
+// src/CubicSplines/internal/cubicPrefilter_kernel.cu): pole z = sqrt(3) - 2, gain
+// (1 - z)(1 - 1/z), causal start c0 + sum_{n < min(12, len)} z^(n+1) c_n (the
+// reference's truncated clamped-boundary sum), anticausal start z/(z - 1) c_(len-1).
+void bspline_line(double* c, int len, size_t stride) {
+  const double z = std::sqrt(3.0) - 2.0, lambda = (1.0 - z) * (1.0 - 1.0 / z);
+  const int horizon = std::min(12, len);
+  double zn = z, sum = c[0];
+  for (int n = 0; n < horizon; ++n) {
+    sum += zn * c[(size_t)n * stride];
+    zn *= z;
+  }
+  double prev = lambda * sum;
+  c[0] = prev;
+  for (int n = 1; n < len; ++n) c[(size_t)n * stride] = prev = lambda * c[(size_t)n * stride] + z * prev;
+  c[(size_t)(len - 1) * stride] = prev = (z / (z - 1.0)) * prev;
+  for (int n = len - 2; n >= 0; --n) c[(size_t)n * stride] = prev = z * (prev - c[(size_t)n * stride]);
+}
+
+// d_CubicBSplinePrefilter3D / 2D (src/CubicSplines/cubicPrefilter3D.cu:114-127,
+// cubicPrefilter2D.cu): lines along x, then y, then z.
+void bspline_prefilter(double* v, int nx, int ny, int nz) {
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y) bspline_line(v + ((size_t)z * ny + y) * nx, nx, 1);
+  for (int z = 0; z < nz; ++z)
+    for (int x = 0; x < nx; ++x) bspline_line(v + (size_t)z * ny * nx + x, ny, nx);
+  if (nz > 1)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) bspline_line(v + (size_t)y * nx + x, nz, (size_t)nx * ny);
+}
+
+double bspline_w(double t) {
+  t = std::fabs(t);
+  const double a = 2.0 - t;
+  if (t < 1.0) return 2.0 / 3.0 - 0.5 * t * t * a;
+  if (t < 2.0) return a * a * a / 6.0;
+  return 0.0;
+}
+
+// Cubic B-spline of coef (nx x ny x nz, nz = 1 for images) at texel
+// coordinate t: taps floor(t) - 1 .. floor(t) + 2 per axis, clamped.
+double cubic_at(const double* coef, int nx, int ny, int nz, double tx, double ty, double tz) {
+  auto cl = [](long v, long hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  const long fx = (long)std::floor(tx), fy = (long)std::floor(ty), fz = (long)std::floor(tz);
+  const bool three = nz > 1;
+  double r = 0;
+  for (int kz = three ? -1 : 0; kz <= (three ? 2 : 0); ++kz) {
+    const double wz = three ? bspline_w(kz - (tz - fz)) : 1.0;
+    const long zz = three ? cl(fz + kz, nz - 1) : 0;
+    for (int ky = -1; ky <= 2; ++ky) {
+      const double wyz = bspline_w(ky - (ty - fy)) * wz;
+      const long yy = cl(fy + ky, ny - 1);
+      for (int kx = -1; kx <= 2; ++kx) r += bspline_w(kx - (tx - fx)) * wyz * coef[((size_t)zz * ny + yy) * nx + cl(fx + kx, nx - 1)];
+    }
+  }
+  return r;
+}
+
+// Per-component cubic B-spline coefficients of an interleaved array of ncomp
+// components (d_CubicBSplinePrefilter3D on the reference's Re and Im copies).
+std::vector<std::vector<double>> spline_components(const double* in, int nx, int ny, int nz, int ncomp) {
+  const size_t V = (size_t)nx * ny * nz;
+  std::vector<std::vector<double>> out((size_t)ncomp, std::vector<double>(V));
+  for (int c = 0; c < ncomp; ++c) {
+    for (size_t i = 0; i < V; ++i) out[(size_t)c][i] = in[i * ncomp + c];
+    bspline_prefilter(out[(size_t)c].data(), nx, ny, nz);
+  }
+  return out;
+}
+
 // acts as clamp).
-void rotate3d_ft(const double* in, double* out, int n, const double* M, bool zerocentred, int ncomp) {
+void rotate3d_ft(const double* in, double* out, int n, const double* M, bool zerocentred, int ncomp,
+                 bool cubic = false) {
   const int nh = n / 2 + 1;
+  std::vector<std::vector<double>> coef;
+  if (cubic) coef = spline_components(in, nh, n, n, ncomp);
   auto clampi = [](long v, long hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
   for (int idz = 0; idz < n; ++idz)
     for (int idy = 0; idy < n; ++idy)
@@ -773,6 +846,13 @@ void rotate3d_ft(const double* in, double* out, int n, const double* M, bool zer
         const bool neg = pf[0] > 1e-6f;
         double t[3];
         for (int r = 0; r < 3; ++r) t[r] = (neg ? -p[r] : p[r]) + n / 2;
+        if (cubic) {
+          for (int c = 0; c < ncomp; ++c) {
+            const double v = cubic_at(coef[(size_t)c].data(), nh, n, n, t[0], t[1], t[2]);
+            o[c] = (c == 1 && neg) ? -v : v;
+          }
+          continue;
+        }
         const long f[3] = {(long)std::floor(t[0]), (long)std::floor(t[1]), (long)std::floor(t[2])};
         const double a[3] = {t[0] - f[0], t[1] - f[1], t[2] - f[2]};
         const long hi[3] = {nh - 1, n - 1, n - 1};
@@ -1152,8 +1232,11 @@ OrcGaussRand g_orc_gauss;
 // 0 when |pos| > maxfreq; the Hermitian mate (imaginary part negated) when
 // pos.x > 1e-5; bilinear at texel pos + n/2 with indices clamped; written at
 // (idx, idy) or at the FFT-order bin (nx/2 - idx, ny-1 - ((idy + ny/2 - 1) % ny)).
-void rotate2d_ft(const double* in, double* out, int nx, int ny, double angle, double maxfreq, bool zerocentred) {
+void rotate2d_ft(const double* in, double* out, int nx, int ny, double angle, double maxfreq, bool zerocentred,
+                 bool cubic = false) {
   const int nh = nx / 2 + 1;
+  std::vector<std::vector<double>> coef;
+  if (cubic) coef = spline_components(in, nh, ny, 1, 2);
   const double c = (float)std::cos(-angle), s = (float)std::sin(-angle);
   auto clampi = [](long v, long hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
   for (int idy = 0; idy < ny; ++idy)
@@ -1180,6 +1263,13 @@ void rotate2d_ft(const double* in, double* out, int nx, int ny, double angle, do
         py = -py;
       }
       const double tx = px + nx / 2, ty = py + ny / 2;
+      if (cubic) {
+        for (int k = 0; k < 2; ++k) {
+          const double v = cubic_at(coef[(size_t)k].data(), nh, ny, 1, tx, ty, 0.0);
+          o[k] = (k == 1 && neg) ? -v : v;
+        }
+        continue;
+      }
       const long fx = (long)std::floor(tx), fy = (long)std::floor(ty);
       const double ax = tx - fx, ay = ty - fy;
       for (int k = 0; k < 2; ++k) {
@@ -1192,45 +1282,6 @@ void rotate2d_ft(const double* in, double* out, int nx, int ny, double angle, do
         o[k] = (k == 1 && neg) ? -v : v;
       }
     }
-}
-
-// Cubic B-spline prefilter of one line (ConvertToInterpolationCoefficients,
-// src/CubicSplines/internal/cubicPrefilter_kernel.cu): pole z = sqrt(3) - 2, gain
-// (1 - z)(1 - 1/z), causal start c0 + sum_{n < min(12, len)} z^(n+1) c_n (the
-// reference's truncated clamped-boundary sum), anticausal start z/(z - 1) c_(len-1).
-void bspline_line(double* c, int len, size_t stride) {
-  const double z = std::sqrt(3.0) - 2.0, lambda = (1.0 - z) * (1.0 - 1.0 / z);
-  const int horizon = std::min(12, len);
-  double zn = z, sum = c[0];
-  for (int n = 0; n < horizon; ++n) {
-    sum += zn * c[(size_t)n * stride];
-    zn *= z;
-  }
-  double prev = lambda * sum;
-  c[0] = prev;
-  for (int n = 1; n < len; ++n) c[(size_t)n * stride] = prev = lambda * c[(size_t)n * stride] + z * prev;
-  c[(size_t)(len - 1) * stride] = prev = (z / (z - 1.0)) * prev;
-  for (int n = len - 2; n >= 0; --n) c[(size_t)n * stride] = prev = z * (prev - c[(size_t)n * stride]);
-}
-
-// d_CubicBSplinePrefilter3D / 2D (src/CubicSplines/cubicPrefilter3D.cu:114-127,
-// cubicPrefilter2D.cu): lines along x, then y, then z.
-void bspline_prefilter(double* v, int nx, int ny, int nz) {
-  for (int z = 0; z < nz; ++z)
-    for (int y = 0; y < ny; ++y) bspline_line(v + ((size_t)z * ny + y) * nx, nx, 1);
-  for (int z = 0; z < nz; ++z)
-    for (int x = 0; x < nx; ++x) bspline_line(v + (size_t)z * ny * nx + x, ny, nx);
-  if (nz > 1)
-    for (int y = 0; y < ny; ++y)
-      for (int x = 0; x < nx; ++x) bspline_line(v + (size_t)y * nx + x, nz, (size_t)nx * ny);
-}
-
-double bspline_w(double t) {
-  t = std::fabs(t);
-  const double a = 2.0 - t;
-  if (t < 1.0) return 2.0 / 3.0 - 0.5 * t * t * a;
-  if (t < 2.0) return a * a * a / 6.0;
-  return 0.0;
 }
 
 // d_Rotate3D / d_Rotate2D with T_INTERP_CUBIC into FFT order
@@ -1726,8 +1777,9 @@ void orc_norm_monolithic_f64(const float* in, double* out, long long n, int batc
   }
 }
 
-void orc_rotate3d_ft_f64(const double* in, double* out, int n, const double* M, int zerocentred, int ncomp) {
-  rotate3d_ft(in, out, n, M, zerocentred != 0, ncomp);
+void orc_rotate3d_ft_f64(const double* in, double* out, int n, const double* M, int zerocentred, int ncomp,
+                         int cubic) {
+  rotate3d_ft(in, out, n, M, zerocentred != 0, ncomp, cubic != 0);
 }
 
 void orc_ctf_tilt_zgrid_f64(const double* angles, double defocus, double pixelsize, int imx, int imy, int rx,
@@ -1760,8 +1812,8 @@ void orc_subpixel_max_f64(const float* in, double* out, int nx, int ny, int nz, 
 }
 
 void orc_rotate2d_ft_f64(const double* in, double* out, int nx, int ny, double angle, double maxfreq,
-                         int zerocentred) {
-  rotate2d_ft(in, out, nx, ny, angle, maxfreq, zerocentred != 0);
+                         int zerocentred, int cubic) {
+  rotate2d_ft(in, out, nx, ny, angle, maxfreq, zerocentred != 0, cubic != 0);
 }
 
 void orc_prob_picker_f64(const double* ref, const double* mask, const int* dimsref, const int* dimsimage,

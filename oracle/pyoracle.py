@@ -48,7 +48,7 @@ def lib():
         L.orc_full_pad_f64.argtypes = [_p, _p, _i, _i, _i, _i, _i, _i]
         L.orc_peak_f64.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
         L.orc_ctf_tilt_zgrid_f64.argtypes = [_p, _d, _d, _i, _i, _i, _i, _p, _i, _p]
-        L.orc_rotate3d_ft_f64.argtypes = [_p, _p, _i, _p, _i, _i]
+        L.orc_rotate3d_ft_f64.argtypes = [_p, _p, _i, _p, _i, _i, _i]
         L.orc_norm_monolithic_f64.argtypes = [_p, _p, _ll, _i, _i]
         L.orc_fft_resize_f64.argtypes = [_i, _p, _p, _p, _p, _i]
         L.orc_sphere_mask_ex_f64.argtypes = [_p, _i, _i, _i, _d, _d, _p, _i]
@@ -64,7 +64,7 @@ def lib():
         L.orc_pick2d_f64.argtypes = pick_args
         L.orc_pick2d_f64.restype = _d
         L.orc_project2d_f64.argtypes = [_p, _i, _p, _i, _p, _i]
-        L.orc_rotate2d_ft_f64.argtypes = [_p, _p, _i, _i, _d, _d, _i]
+        L.orc_rotate2d_ft_f64.argtypes = [_p, _p, _i, _i, _d, _d, _i, _i]
         L.orc_prob_picker_f64.argtypes = [_p, _p, _p, _p, _i, _i, _p, _p, _f, _i, _p, _p, _p, _p, _i]
         L.orc_bspline_prefilter_f64.argtypes = [_p, _i, _i, _i]
         L.orc_rotate_cubic_f64.argtypes = [_p, _p, _i, _i, _i, _p]
@@ -270,7 +270,7 @@ def ctf_tilt_zgrid(imageangle, stageangle, specimenangle, defocus, pixelsize, di
     return z
 
 
-def rotate3d_ft(vol, M, zerocentred=True):
+def rotate3d_ft(vol, M, zerocentred=True, cubic=False):
     """d_Rotate3DFT (linear) of one centred half spectrum (n, n, n/2+1), complex or
     real, by the row-major matrix M = transpose(Euler)."""
     v = np.asarray(vol)
@@ -278,7 +278,8 @@ def rotate3d_ft(vol, M, zerocentred=True):
     src = np.ascontiguousarray(v, np.complex128).view(np.float64) if cplx else _c(v, np.float64)
     n = v.shape[0]
     out = np.empty_like(src)
-    lib().orc_rotate3d_ft_f64(_ptr(src), _ptr(out), n, _ptr(_c(M, np.float64)), int(zerocentred), 2 if cplx else 1)
+    lib().orc_rotate3d_ft_f64(_ptr(src), _ptr(out), n, _ptr(_c(M, np.float64)), int(zerocentred), 2 if cplx else 1,
+                              int(bool(cubic)))
     return out.view(np.complex128) if cplx else out
 
 
@@ -444,13 +445,13 @@ def pick(projft, p, oversample, expft, ctf, n, angles, maskradius, ctf_per_volum
     return dict(best=best, idx=idx, top1=top1, top2=top2, sigma=sigma, loop_seconds=secs)
 
 
-def rotate2d_ft(img, angle, maxfreq, zerocentred=True):
+def rotate2d_ft(img, angle, maxfreq, zerocentred=True, cubic=False):
     """d_Rotate2DFT (linear) of one zero-centred complex half spectrum (ny, nx/2+1)."""
     v = np.ascontiguousarray(img, np.complex128)
     ny, nh = v.shape
     out = np.empty_like(v)
     lib().orc_rotate2d_ft_f64(_ptr(v.view(np.float64)), _ptr(out.view(np.float64)), 2 * (nh - 1), ny, float(angle),
-                              float(maxfreq), int(zerocentred))
+                              float(maxfreq), int(zerocentred), int(bool(cubic)))
     return out
 
 

@@ -98,3 +98,50 @@ def test_real_valued_spectrum_has_no_conjugation():
     i_re = po.rotate3d_ft(np.ascontiguousarray(half.real), M)
     assert np.allclose(i_re, c.real, atol=1e-12)  # linear in the real part
     assert np.all(r >= -1e-12)  # interpolating a non-negative field, no sign flip
+
+
+# ---------------------------------------------------------------- cubic interpolation
+def test_cubic_identity_reproduces_the_input_inside():
+    """Cubic B-spline interpolation of the prefiltered spectrum at the grid points
+    gives the samples back, away from the array faces (where the prefilter's
+    truncated start sum and the clamped taps sit)."""
+    rng = np.random.default_rng(30)
+    half = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    out = po.rotate3d_ft(half, np.eye(3), cubic=True)
+    c = _centred_coords(N)
+    t = c + N // 2
+    inside = ((c ** 2).sum(-1) < N * N / 4) & np.all(t >= 2, -1) & (t[..., 1] <= N - 3) & (t[..., 2] <= N - 3) & \
+        (t[..., 0] <= N // 2 - 2)
+    assert inside.sum() > 300
+    assert np.allclose(out[inside], half[inside], atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_cubic_rotation_equals_scipy_spline_of_the_coefficients(seed):
+    """Bins whose rotated position needs no flip hold scipy's order-3 spline of the
+    prefiltered Re and Im at that texel (the prefilter is pinned separately)."""
+    from scipy.ndimage import map_coordinates
+    rng = np.random.default_rng(40 + seed)
+    half = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    M = po.projector_matrix(*rng.uniform(-np.pi, np.pi, 3), 1.0)
+    out = po.rotate3d_ft(half, M, cubic=True)
+    cre, cim = po.bspline_prefilter(half.real), po.bspline_prefilter(half.imag)
+    c = _centred_coords(N)
+    p = c @ M.T
+    t = p + N // 2
+    sel = ((c ** 2).sum(-1) < N * N / 4) & (p[..., 0] < -1e-3) & np.all(t > 2.5, -1) & (t[..., 0] < N // 2 - 2.5) & \
+        (t[..., 1] < N - 3.5) & (t[..., 2] < N - 3.5)
+    q = [t[sel][:, 2], t[sel][:, 1], t[sel][:, 0]]
+    want = map_coordinates(cre, q, order=3, prefilter=False) + 1j * map_coordinates(cim, q, order=3, prefilter=False)
+    assert sel.sum() > 100
+    assert np.allclose(out[sel], want, atol=1e-12)
+
+
+def test_cubic_2d_identity_reproduces_the_input_inside():
+    rng = np.random.default_rng(31)
+    n = 24
+    half = rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))
+    out = po.rotate2d_ft(half, 0.0, n / 2, cubic=True)
+    y, x = np.meshgrid(np.arange(n), np.arange(n // 2 + 1), indexing="ij")
+    inside = (np.hypot(x - n // 2, y - n // 2) <= n / 2) & (x >= 2) & (x <= n // 2 - 2) & (y >= 2) & (y <= n - 3)
+    assert np.allclose(out[inside], half[inside], atol=1e-6)
