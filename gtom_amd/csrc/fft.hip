@@ -17,11 +17,16 @@ namespace gx {
 // columns (contiguous in memory) x R2 threads per column. Plane p selects the
 // base offset (p / P1) * S1 + (p % P1) * S0.
 // ---------------------------------------------------------------------------
+// Tile width of the generic column transforms: 32 columns (256-byte row
+// segments) at n = 128 and 256, which stream better along the long strides of
+// the y and z passes; the search's own column passes keep ColShape's width.
+template <int N> constexpr int fft_col_tx() { return (N == 128 || N == 256) ? 32 : ColShape<N>::TX; }
+
 template <int N, int DIR>
-__global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2) void k_cols(
+__global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
     float2* __restrict__ data, int width, long long axis_stride, long long nplanes, long long P1,
     long long S0, long long S1, const float2* __restrict__ tw) {
-  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
+  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = fft_col_tx<N>();
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
   const int t = threadIdx.x, c = t % TX, j = t / TX;
   const int col = blockIdx.x * TX + c;
@@ -137,7 +142,7 @@ int launch_cols_n(int n, float2* data, int width, long long axis_stride, long lo
   switch (n) {
 #define GX_CASE(N) \
   case N:          \
-    return go(k_cols<N, DIR>, ColShape<N>::TX, ColShape<N>::R2);
+    return go(k_cols<N, DIR>, fft_col_tx<N>(), ColShape<N>::R2);
     GX_FOR_SIZES(GX_CASE)
 #undef GX_CASE
     default: return fail(GTOMX_ERR_UNSUPPORTED, "column FFT length not supported");
