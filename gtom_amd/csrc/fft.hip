@@ -1,12 +1,16 @@
-// Batched R2C / C2R transforms of 3D, 2D and 1D volumes (axes 2^a or 3*2^a,
-// 8..256) for gfx950. Replace the cuFFT plans wrapped by d_FFTR2C and
-// d_IFFTC2R (src/FFT/FFT.cu:7-42, src/FFT/IFFT.cu:8-56).
+// Batched R2C / C2R transforms of 3D, 2D and 1D volumes (axes 2^a up to 4096
+// or 3*2^a up to 3072, from 8) for gfx950. Replace the cuFFT plans wrapped by
+// d_FFTR2C and d_IFFTC2R (src/FFT/FFT.cu:7-42, src/FFT/IFFT.cu:8-56). Axes up
+// to 256 (rows up to 256 reals) take one pass each; longer ones two.
 //
 // Layout: real (nx, ny, nz) x-fastest; half spectrum (nx/2+1, ny, nz).
 // R2C = rows (real -> half, via an nx/2-point complex FFT) then y then z.
 // C2R = z then y (in place on a copy) then rows (half -> real). The row C2R
 // uses only the real parts of the x = 0 and x = nx/2 bins, which makes the
 // result the real part of the Hermitian-completed inverse.
+#include <cstdint>
+#include <type_traits>
+
 #include "dev_fft.hpp"
 #include "runtime.hpp"
 
@@ -51,6 +55,9 @@ __global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
 // Z[k] = (X[k] + conj X[N/2-k]) + i w^k (X[k] - conj X[N/2-k]), w = e^{2 pi i/N},
 // z = IDFT_{N/2}(Z), out[2m] = Re z[m], out[2m+1] = Im z[m].
 // ---------------------------------------------------------------------------
+// The workgroup's RW rows are one contiguous block in and out, so both go
+// through LDS (the exchange area) with coalesced accesses; barriers separate
+// the area's uses.
 template <int N>
 __global__ __launch_bounds__(256) void k_rows_c2r(const float2* __restrict__ in, float* __restrict__ out,
                                                   long long nrows, float scale,
@@ -58,17 +65,21 @@ __global__ __launch_bounds__(256) void k_rows_c2r(const float2* __restrict__ in,
   constexpr int NC = N / 2, NH = N / 2 + 1;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
   constexpr int GS = R2 * (R1 + 1);  // exchange floats2 per row (padded)
-  __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
+  constexpr int AREA = RW * (GS > NH ? GS : NH);
+  __shared__ alignas(16) float2 xch[AREA];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
-  const long long row = (long long)blockIdx.x * RW + r;
-  const bool valid = row < nrows;
-  const float2* X = in + row * NH;
+  const long long row0 = (long long)blockIdx.x * RW;
+  const int nr = (int)(nrows - row0 < RW ? nrows - row0 : RW);
+  const float2* src = in + row0 * NH;
+  for (int i = t; i < nr * NH; i += 256) xch[i] = src[i];
+  __syncthreads();
+  const bool valid = r < nr;
   float2 v[R1];
 #pragma unroll
   for (int m = 0; m < R1; ++m) {
     const int k = j + R2 * m;
-    float2 a = valid ? X[k] : make_float2(0.f, 0.f);
-    float2 b = valid ? X[NC - k] : make_float2(0.f, 0.f);
+    float2 a = valid ? xch[r * NH + k] : make_float2(0.f, 0.f);
+    float2 b = valid ? xch[r * NH + NC - k] : make_float2(0.f, 0.f);
     if (k == 0) {
       a.y = 0.f;
       b.y = 0.f;
@@ -76,11 +87,20 @@ __global__ __launch_bounds__(256) void k_rows_c2r(const float2* __restrict__ in,
     const float2 bc = cconj(b);
     v[m] = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle<1, N>(tw, k), csub(a, bc))));
   }
+  __syncthreads();  // the rows are in registers: the area becomes the exchange
   fft_two_step<NC, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[r * GS + jj * (R1 + 1) + k1]; });
-  if (valid) {
-    float2* o = reinterpret_cast<float2*>(out + row * N);
+  __syncthreads();  // exchange reads done: the area becomes the output rows
 #pragma unroll
-    for (int s = 0; s < R1; ++s) o[out_index<R1, R2>(j, s)] = cscale(v[s], scale);
+  for (int s = 0; s < R1; ++s) xch[r * NC + out_index<R1, R2>(j, s)] = cscale(v[s], scale);
+  __syncthreads();
+  float* dst = out + row0 * N;
+  if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    const float4* b4 = reinterpret_cast<const float4*>(xch);
+    for (int i = t; i < nr * N / 4; i += 256) d4[i] = b4[i];
+  } else {
+    float2* d2 = reinterpret_cast<float2*>(dst);
+    for (int i = t; i < nr * NC; i += 256) d2[i] = xch[i];
   }
 }
 
@@ -123,6 +143,118 @@ __global__ __launch_bounds__(256) void k_rows_r2c(const float* __restrict__ in, 
       X[NC] = make_float2(z0.x - z0.y, 0.f);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Lengths above 256 (row half lengths above 128): the four-step decomposition
+// L = L1 * L2. With n = n2 + L2 n1 and k = k1 + L1 k2,
+//   X[k1 + L1 k2] = sum_n2 W_L2^{n2 k2} W_L^{n2 k1} sum_n1 x[n2 + L2 n1] W_L1^{n1 k1}.
+// Pass 1 runs an L1-point transform for each n2, multiplies by W_L^{n2 k1} and
+// stores at n2 + L2 k1 (the positions it read) in a scratch copy. Pass 2 runs
+// an L2-point transform for each k1 over the entries L2 k1 + n2 and stores at
+// k1 + L1 k2 of the destination: natural order without a transpose pass. Both
+// passes are batches of strided lines, addressed by LineMap.
+// ---------------------------------------------------------------------------
+struct LineMap {
+  long long P1, S0s, S1s, S0d, S1d;  // plane q: base = (q / P1) * S1 + (q % P1) * S0
+  long long Cs, Cd;                  // column c (adjacent threads) offset multipliers
+  long long Gs, Gd;                  // sub-line g offset multipliers, g in [0, G)
+  long long Ms, Md;                  // element strides in and out
+  long long nplanes;                 // planes * G
+  int width, G;
+  int L, TG, TC;                     // output times W_L^{(g TG + c TC) k} when L > 0
+  float scale;
+};
+
+// exp(DIR 2 pi i e / L) from the twiddle table, L = 2^a <= 4096 or 3 * 2^a <= 3072.
+template <int DIR> __device__ __forceinline__ float2 twiddle_rt(const float2* __restrict__ tw, int L, int e) {
+  const float2 w = (L & (L - 1)) == 0 ? tw[(e & (L - 1)) * (TW_LEN / L)] : tw[TW3_OFF + (e % L) * (TW3_LEN / L)];
+  return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+template <int N, int DIR>
+__global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_lines(const float2* __restrict__ src,
+                                                                              float2* __restrict__ dst, LineMap p,
+                                                                              const float2* __restrict__ tw) {
+  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = fft_col_tx<N>();
+  __shared__ float2 xch[R2 > 1 ? TX * N : 1];
+  const int t = threadIdx.x, c = t % TX, j = t / TX;
+  const int col = blockIdx.x * TX + c;
+  const bool valid = col < p.width;
+  for (long long pl = blockIdx.y; pl < p.nplanes; pl += gridDim.y) {
+    const long long q = pl / p.G;
+    const int g = (int)(pl % p.G);
+    const long long sb = (q / p.P1) * p.S1s + (q % p.P1) * p.S0s + col * p.Cs + g * p.Gs;
+    const long long db = (q / p.P1) * p.S1d + (q % p.P1) * p.S0d + col * p.Cd + g * p.Gd;
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m)
+      v[m] = valid ? src[sb + (long long)(j + R2 * m) * p.Ms] : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft_two_step<N, R1, R2, DIR>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
+    if (valid) {
+      const int e = g * p.TG + col * p.TC;
+#pragma unroll
+      for (int s = 0; s < R1; ++s) {
+        const int k = out_index<R1, R2>(j, s);
+        float2 o = v[s];
+        if (p.L > 0) o = cmul(o, twiddle_rt<DIR>(tw, p.L, e * k));
+        dst[db + (long long)k * p.Md] = cscale(o, p.scale);
+      }
+    }
+  }
+}
+
+// Rows of length N = 2 NC above 256, C2R side: Z[k] = (X[k] + conj X[NC-k]) +
+// i w^k (X[k] - conj X[NC-k]), w = e^{2 pi i/N}, in place on rows of pitch
+// `pitch`; one thread per pair (k, NC - k), k in [0, NC/2]. The x = 0 and
+// x = N/2 bins contribute their real parts only.
+__global__ __launch_bounds__(256) void k_half_to_z(float2* __restrict__ data, long long nrows, int NC,
+                                                   long long pitch, const float2* __restrict__ tw) {
+  const int np = NC / 2 + 1;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= nrows * np) return;
+  float2* row = data + (idx / np) * pitch;
+  const int k = (int)(idx % np), N = 2 * NC;
+  float2 a = row[k], b = row[NC - k];
+  if (k == 0) {
+    a.y = 0.f;
+    b.y = 0.f;
+  }
+  const float2 ac = cconj(a), bc = cconj(b);
+  const float2 zk = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle_rt<1>(tw, N, k), csub(a, bc))));
+  if (k == 0) {
+    row[0] = zk;
+    return;
+  }
+  const float2 zn = cadd(cadd(b, ac), mul_pj<1>(cmul(twiddle_rt<1>(tw, N, NC - k), csub(b, ac))));
+  row[k] = zk;
+  row[NC - k] = zn;
+}
+
+// Rows of length N = 2 NC above 256, R2C side: from Z = DFT_NC(x[2m] + i x[2m+1])
+// in entries [0, NC) of rows of pitch `pitch`, in place,
+// X[k] = ((Z[k] + conj Z[NC-k]) - i w^k (Z[k] - conj Z[NC-k])) / 2, w = e^{-2 pi i/N},
+// and X[NC] = Re Z[0] - Im Z[0]; one thread per pair (k, NC - k).
+__global__ __launch_bounds__(256) void k_z_to_half(float2* __restrict__ data, long long nrows, int NC,
+                                                   long long pitch, const float2* __restrict__ tw) {
+  const int np = NC / 2 + 1;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= nrows * np) return;
+  float2* row = data + (idx / np) * pitch;
+  const int k = (int)(idx % np), N = 2 * NC;
+  const float2 zk = row[k], zn = row[(NC - k) % NC];
+  const float2 zkc = cconj(zk), znc = cconj(zn);
+  const float2 xk = cscale(csub(cadd(zk, znc), mul_pj<1>(cmul(twiddle_rt<-1>(tw, N, k), csub(zk, znc)))), 0.5f);
+  if (k == 0) {
+    row[0] = xk;
+    row[NC] = make_float2(zk.x - zk.y, 0.f);
+    return;
+  }
+  const float2 xn =
+      cscale(csub(cadd(zn, zkc), mul_pj<1>(cmul(twiddle_rt<-1>(tw, N, NC - k), csub(zn, zkc)))), 0.5f);
+  row[k] = xk;
+  row[NC - k] = xn;
 }
 
 // ---------------------------------------------------------------------------
@@ -188,29 +320,152 @@ int launch_rows_r2c(int n, const float* in, float2* out, long long nrows, const 
 
 }  // namespace
 
-int fft_r2c(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStream_t s) {
-  const float2* tw = nullptr;
-  GX_TRY(twiddles(&tw));
-  const long long nh = d.x / 2 + 1;
-  GX_TRY(launch_rows_r2c(d.x, in, out, (long long)d.y * d.z * batch, tw, s));
-  if (d.y > 1)
-    GX_TRY(launch_cols_n<-1>(d.y, out, (int)nh, nh, (long long)d.z * batch, 1LL << 62, nh * d.y, 0, tw, s));
-  if (d.z > 1)
-    GX_TRY(launch_cols_n<-1>(d.z, out, (int)nh, nh * d.y, (long long)d.y * batch, d.y, nh,
-                             nh * d.y * d.z, tw, s));
+// Factorisation L = L1 * L2 of the lengths above the one-pass kernels.
+bool split_len(int L, int* L1, int* L2) {
+  switch (L) {
+    case 256: *L1 = 16, *L2 = 16; return true;
+    case 384: *L1 = 16, *L2 = 24; return true;
+    case 512: *L1 = 16, *L2 = 32; return true;
+    case 768: *L1 = 32, *L2 = 24; return true;
+    case 1024: *L1 = 32, *L2 = 32; return true;
+    case 1536: *L1 = 48, *L2 = 32; return true;
+    case 2048: *L1 = 64, *L2 = 32; return true;
+    case 3072: *L1 = 64, *L2 = 48; return true;
+    case 4096: *L1 = 64, *L2 = 64; return true;
+    default: return false;
+  }
+}
+
+template <int DIR>
+int launch_lines(int n, const float2* src, float2* dst, const LineMap& p, const float2* tw, hipStream_t s) {
+  auto go = [&](auto kernel, int TX, int R2) -> int {
+    dim3 grid((p.width + TX - 1) / TX, (unsigned)std::min<long long>(p.nplanes, 65535));
+    hipLaunchKernelGGL(kernel, grid, dim3(TX * R2), 0, s, src, dst, p, tw);
+    GX_LAUNCHED("k_lines");
+    return GTOMX_OK;
+  };
+  switch (n) {
+    case 16: return go(k_lines<16, DIR>, fft_col_tx<16>(), ColShape<16>::R2);
+    case 24: return go(k_lines<24, DIR>, fft_col_tx<24>(), ColShape<24>::R2);
+    case 32: return go(k_lines<32, DIR>, fft_col_tx<32>(), ColShape<32>::R2);
+    case 48: return go(k_lines<48, DIR>, fft_col_tx<48>(), ColShape<48>::R2);
+    case 64: return go(k_lines<64, DIR>, fft_col_tx<64>(), ColShape<64>::R2);
+    default: return fail(GTOMX_ERR_INTERNAL, "four-step factor not instantiated");
+  }
+}
+
+// One strided axis of length L in place; above 256 through `scratch`, which
+// mirrors data's extent.
+template <int DIR>
+int cols_axis(int L, float2* data, float2* scratch, int width, long long as, long long nplanes, long long P1,
+              long long S0, long long S1, const float2* tw, hipStream_t s) {
+  if (L <= 256) return launch_cols_n<DIR>(L, data, width, as, nplanes, P1, S0, S1, tw, s);
+  int L1, L2;
+  if (!split_len(L, &L1, &L2) || !scratch) return fail(GTOMX_ERR_INTERNAL, "axis length not split");
+  LineMap a{};
+  a.P1 = P1, a.S0s = a.S0d = S0, a.S1s = a.S1d = S1, a.Cs = a.Cd = 1;
+  a.G = L2, a.Gs = a.Gd = as, a.Ms = a.Md = L2 * as, a.nplanes = nplanes * L2, a.width = width;
+  a.L = L, a.TG = 1, a.TC = 0, a.scale = 1.f;
+  GX_TRY(launch_lines<DIR>(L1, data, scratch, a, tw, s));
+  LineMap b = a;
+  b.G = L1, b.Gs = L2 * as, b.Gd = as, b.Ms = as, b.Md = L1 * as, b.nplanes = nplanes * L1, b.L = 0;
+  return launch_lines<DIR>(L2, scratch, data, b, tw, s);
+}
+
+// Row R2C of length N > 256: rows of N reals (NC = N/2 complex) into rows of
+// NC + 1 bins, through `scratch` (nrows * NC complex).
+int rows_r2c_big(int N, const float* in, float2* out, float2* scratch, long long nrows, const float2* tw,
+                 hipStream_t s) {
+  const int NC = N / 2;
+  int L1, L2;
+  if (!split_len(NC, &L1, &L2)) return fail(GTOMX_ERR_INTERNAL, "row length not split");
+  LineMap a{};
+  a.P1 = 1LL << 62, a.S0s = a.S0d = NC, a.Cs = a.Cd = 1, a.G = 1, a.Ms = a.Md = L2, a.nplanes = nrows;
+  a.width = L2, a.L = NC, a.TG = 0, a.TC = 1, a.scale = 1.f;
+  GX_TRY(launch_lines<-1>(L1, reinterpret_cast<const float2*>(in), scratch, a, tw, s));
+  LineMap b = a;
+  b.S0d = NC + 1, b.Cs = L2, b.Cd = 1, b.Ms = 1, b.Md = L1, b.width = L1, b.L = 0;
+  GX_TRY(launch_lines<-1>(L2, scratch, out, b, tw, s));
+  const long long n = nrows * (NC / 2 + 1);
+  hipLaunchKernelGGL(k_z_to_half, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, nrows, NC,
+                     (long long)(NC + 1), tw);
+  GX_LAUNCHED("k_z_to_half");
   return GTOMX_OK;
 }
 
-int fft_c2r_inplace_yz(float2* data, gtomx_int3 d, unsigned batch, hipStream_t s) {
+// Row C2R of length N > 256: rows of NC + 1 bins (modified in place) into
+// rows of N reals, scaled, through `scratch` (nrows * NC complex).
+int rows_c2r_big(int N, float2* work, float* out, float2* scratch, long long nrows, float scale,
+                 const float2* tw, hipStream_t s) {
+  const int NC = N / 2;
+  int L1, L2;
+  if (!split_len(NC, &L1, &L2)) return fail(GTOMX_ERR_INTERNAL, "row length not split");
+  const long long n = nrows * (NC / 2 + 1);
+  hipLaunchKernelGGL(k_half_to_z, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, work, nrows, NC,
+                     (long long)(NC + 1), tw);
+  GX_LAUNCHED("k_half_to_z");
+  LineMap a{};
+  a.P1 = 1LL << 62, a.S0s = NC + 1, a.S0d = NC, a.Cs = a.Cd = 1, a.G = 1, a.Ms = a.Md = L2, a.nplanes = nrows;
+  a.width = L2, a.L = NC, a.TG = 0, a.TC = 1, a.scale = 1.f;
+  GX_TRY(launch_lines<1>(L1, work, scratch, a, tw, s));
+  LineMap b = a;
+  b.S0s = NC, b.S0d = NC, b.Cs = L2, b.Cd = 1, b.Ms = 1, b.Md = L1, b.width = L1, b.L = 0, b.scale = scale;
+  return launch_lines<1>(L2, scratch, reinterpret_cast<float2*>(out), b, tw, s);
+}
+
+bool needs_scratch(gtomx_int3 d) { return d.x > 256 || d.y > 256 || d.z > 256; }
+
+// Stream-ordered scratch for the two-pass axes: taken from the stream's pool
+// and given back on the same stream, so no synchronisation is needed.
+struct StreamScratch {
+  float2* p = nullptr;
+  hipStream_t s = nullptr;
+  int take(size_t n, hipStream_t st) {
+    s = st;
+    void* q = nullptr;
+    hipError_t e = hipMallocAsync(&q, n * sizeof(float2), st);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(FFT scratch)");
+    p = static_cast<float2*>(q);
+    return GTOMX_OK;
+  }
+  ~StreamScratch() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+};
+
+int yz_axes(float2* data, float2* scr, gtomx_int3 d, unsigned batch, int dir, const float2* tw, hipStream_t s) {
+  const long long nh = d.x / 2 + 1;
+  auto zax = [&](auto DIRC) {
+    return cols_axis<decltype(DIRC)::value>(d.z, data, scr, (int)nh, nh * d.y, (long long)d.y * batch, d.y, nh,
+                                            nh * d.y * d.z, tw, s);
+  };
+  auto yax = [&](auto DIRC) {
+    return cols_axis<decltype(DIRC)::value>(d.y, data, scr, (int)nh, nh, (long long)d.z * batch, 1LL << 62,
+                                            nh * d.y, 0, tw, s);
+  };
+  using F = std::integral_constant<int, -1>;
+  using B = std::integral_constant<int, 1>;
+  if (dir < 0) {
+    if (d.y > 1) GX_TRY(yax(F{}));
+    if (d.z > 1) GX_TRY(zax(F{}));
+  } else {
+    if (d.z > 1) GX_TRY(zax(B{}));
+    if (d.y > 1) GX_TRY(yax(B{}));
+  }
+  return GTOMX_OK;
+}
+
+int fft_r2c(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStream_t s) {
   const float2* tw = nullptr;
   GX_TRY(twiddles(&tw));
-  const long long nh = d.x / 2 + 1;
-  if (d.z > 1)
-    GX_TRY(launch_cols_n<1>(d.z, data, (int)nh, nh * d.y, (long long)d.y * batch, d.y, nh,
-                            nh * d.y * d.z, tw, s));
-  if (d.y > 1)
-    GX_TRY(launch_cols_n<1>(d.y, data, (int)nh, nh, (long long)d.z * batch, 1LL << 62, nh * d.y, 0, tw, s));
-  return GTOMX_OK;
+  const long long rows = (long long)d.y * d.z * batch;
+  StreamScratch scr;
+  if (needs_scratch(d)) GX_TRY(scr.take(elements_ft(d) * batch, s));
+  if (d.x > 256)
+    GX_TRY(rows_r2c_big(d.x, in, out, scr.p, rows, tw, s));
+  else
+    GX_TRY(launch_rows_r2c(d.x, in, out, rows, tw, s));
+  return yz_axes(out, scr.p, d, batch, -1, tw, s);
 }
 
 int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned batch, float scale,
@@ -219,9 +474,12 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned b
   GX_TRY(twiddles(&tw));
   const size_t H = elements_ft(d) * batch;
   if (work != in) GX_HIP(hipMemcpyAsync(work, in, H * sizeof(float2), hipMemcpyDeviceToDevice, s));
-  GX_TRY(fft_c2r_inplace_yz(work, d, batch, s));
-  GX_TRY(launch_rows_c2r(d.x, work, out, (long long)d.y * d.z * batch, scale, tw, s));
-  return GTOMX_OK;
+  StreamScratch scr;
+  if (needs_scratch(d)) GX_TRY(scr.take(H, s));
+  GX_TRY(yz_axes(work, scr.p, d, batch, 1, tw, s));
+  const long long rows = (long long)d.y * d.z * batch;
+  if (d.x > 256) return rows_c2r_big(d.x, work, out, scr.p, rows, scale, tw, s);
+  return launch_rows_c2r(d.x, work, out, rows, scale, tw, s);
 }
 
 }  // namespace gx

@@ -96,9 +96,12 @@ int check_fft_dims(int ndims, gtomx_int3 d) {
   int want = 3 - (d.z == 1) - (d.z == 1 && d.y == 1);
   if (ndims != want)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "ndims does not match dims (trailing extents must be 1)");
-  if (!fft_len_ok(d.x)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.x must be 2^a or 3*2^a in [8, 256]");
-  if (d.y != 1 && !fft_len_ok(d.y)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.y must be 1, 2^a or 3*2^a in [8, 256]");
-  if (d.z != 1 && !fft_len_ok(d.z)) return fail(GTOMX_ERR_UNSUPPORTED, "dims.z must be 1, 2^a or 3*2^a in [8, 256]");
+  if (!fft_len_ok_any(d.x))
+    return fail(GTOMX_ERR_UNSUPPORTED, "dims.x must be 2^a in [8, 4096] or 3*2^a in [12, 3072]");
+  if (d.y != 1 && !fft_len_ok_any(d.y))
+    return fail(GTOMX_ERR_UNSUPPORTED, "dims.y must be 1, 2^a in [8, 4096] or 3*2^a in [12, 3072]");
+  if (d.z != 1 && !fft_len_ok_any(d.z))
+    return fail(GTOMX_ERR_UNSUPPORTED, "dims.z must be 1, 2^a in [8, 4096] or 3*2^a in [12, 3072]");
   return GTOMX_OK;
 }
 

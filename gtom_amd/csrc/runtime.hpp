@@ -53,9 +53,15 @@ int twiddles(const float2** table);
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-// Axis extents accepted by the FFT kernels: 2^a or 3 * 2^a in [8, 256].
+// Axis extents of the one-pass FFT kernels (and of the fused search kernels
+// built on them): 2^a or 3 * 2^a in [8, 256].
 inline bool fft_len_ok(int n) {
   return n >= 8 && n <= 256 && (is_pow2(n) || (n % 3 == 0 && is_pow2(n / 3)));
+}
+// Axis extents of the generic transforms: the above, plus 2^a up to 4096 and
+// 3 * 2^a up to 3072 through the two-pass (four-step) decomposition.
+inline bool fft_len_ok_any(int n) {
+  return fft_len_ok(n) || (n > 256 && ((is_pow2(n) && n <= 4096) || (n % 3 == 0 && is_pow2(n / 3) && n <= 3072)));
 }
 // Expands X(n) for every supported transform length.
 #define GX_FOR_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
@@ -92,8 +98,8 @@ struct DevBuf {
 int fft_r2c(const float* in, float2* out, gtomx_int3 dims, unsigned batch, hipStream_t s);
 int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 dims, unsigned batch,
             float scale, hipStream_t s);
-// In-place inverse along z then y of half spectra; used by fft_c2r.
-int fft_c2r_inplace_yz(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
+// Axes above 256 go through two passes with a scratch copy of the data, taken
+// from the stream's memory pool (hipMallocAsync) and returned on the stream.
 
 // Fourier-space resizing (resize.hip): op is one of the GTOMX_FFT_* values.
 constexpr int FFT_CROP = GTOMX_FFT_CROP, FFT_PAD = GTOMX_FFT_PAD, FFT_FULL_CROP = GTOMX_FFT_FULL_CROP,

@@ -15,8 +15,11 @@
  *    waiting for it unless documented otherwise.
  *  - Every function returns a gtomx_status; no C++ exception crosses the ABI.
  *    gtomx_last_error() gives a human-readable message for the calling thread.
- *  - Supported transform sizes: every axis 2^a or 3*2^a in [8, 256]
- *    (an axis of extent 1 is allowed for 2D and 1D data).
+ *  - Supported transform sizes: every axis 2^a in [8, 4096] or 3*2^a in
+ *    [12, 3072] (an axis of extent 1 is allowed for 2D and 1D data). The
+ *    fused search (gtomx_pick_*) takes sides up to 256; the generic
+ *    transforms and the pickers built on them take the full range, axes above
+ *    256 in two passes with stream-ordered scratch (hipMallocAsync).
  *
  * The reference interface each function replaces is cited as file:line of
  * the reference tree (dtegunov/gtom).

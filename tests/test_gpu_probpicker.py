@@ -188,3 +188,24 @@ def test_noncircular_mask_matches_oracle(gx, orc, cuda, nd):
     assert sel.mean() > 0.9
     assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
     assert np.all(to_np(r) == 0)
+
+
+@pytest.mark.parametrize("di", [(1024, 1024), (1536, 1536)])
+def test_micrograph_sizes_match_oracle(gx, orc, cuda, di):
+    """Micrograph-sized images, whose transforms run in two passes per axis
+    (1024 = 32 x 32 and 1536 = 48 x 32 along y; rows of 512 and 768 complex)."""
+    rng = np.random.default_rng(di[0])
+    dr, lp, step = (32, 32), 16, 0.5  # 13 orientations
+    refs = rng.standard_normal((1,) + dr).astype(np.float32)
+    masks = _disc(dr, 14.0)[None]
+    image = rng.standard_normal(di).astype(np.float32)
+    b, a, r, nang = _run_gpu(gx, cuda, refs, masks, image, lp, step, None, 0)
+    assert nang == 13
+    orc.gaussrand(0, reset=True)
+    ob, oa, _, top2 = orc.prob_picker(refs[0].astype(np.float64), masks[0].astype(np.float64),
+                                      image.astype(np.float64), lp, step, refid=0)
+    assert rel_linf(to_np(b).astype(np.float64), ob) <= 1e-4
+    sel = (ob - top2) > 1e-4 * np.abs(ob).max()
+    assert sel.mean() > 0.9
+    assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
+    assert np.all(to_np(r) == 0)
