@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks"], default=None,
+    ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
+                                      "prob_picker_micrograph"], default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
 
@@ -352,6 +353,81 @@ def row_prob_picker(args):
     return 0
 
 
+def row_fft_micrograph(args):
+    """The generic transforms at micrograph size: 8 images of 4096^2, R2C then
+    C2R (renormalised) per step; both axes go through the two-pass path."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    n, nb = 4096, 8
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    x = torch.randn((nb, n, n), device=dev, generator=g)
+    X = torch.empty((nb, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+    y = torch.empty_like(x)
+    L = gx.lib()
+
+    def step():
+        gx.check(L.gtomx_fft_r2c(gx._ptr(x), gx._ptr(X), 2, gx.int3(n, n, 1), nb, None), "gtomx_fft_r2c")
+        gx.check(L.gtomx_ifft_c2r(gx._ptr(X), gx._ptr(y), 2, gx.int3(n, n, 1), nb, 1, None), "gtomx_ifft_c2r")
+    sec = _timed_steps(args, step)
+    pts = n * n * nb
+
+    def cpu():
+        xs = np.random.default_rng(args.seed).standard_normal((1, 1, n, n))
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0:  # a bounded sample: whole images for about 10 s
+            po.c2r(po.r2c(xs), n)
+            k += 1
+        ct = time.perf_counter() - t0
+        return n * n * k, ct, f"{k} 4096^2 images one at a time, R2C + C2R with the FP64 oracle's own FFT, {ct:.1f} s"
+    return _row_line(args, "fft_micrograph", "transformed points/s (R2C + C2R), 4096^2 images", "points/s",
+                     pts, sec, {"workload": "d_FFTR2C + d_IFFTC2R", "image": [n, n], "batch": nb}, cpu)
+
+
+def row_prob_picker_micrograph(args):
+    """ProbabilisticPicker, 2D, at micrograph size: one 4096^2 micrograph
+    (noise with planted copies), a 64^2 reference with a circular mask of
+    radius 28, low-pass 32, anglestep 0.05 rad (126 in-plane orientations).
+    One step is PerformCorrelation for one reference."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(args.seed)
+    di, dr, lp, step = (4096, 4096), (64, 64), 32, 0.05
+    g = np.meshgrid(*[np.arange(d) - d // 2 for d in dr], indexing="ij")
+    mask = (np.sqrt(sum(a.astype(float) ** 2 for a in g)) < 28).astype(np.float32)
+    refs = rng.standard_normal((1,) + dr).astype(np.float32) * mask
+    image = rng.standard_normal(di).astype(np.float32)
+    for _ in range(200):
+        cy, cx = rng.integers(32, 4096 - 32, 2)
+        image[cy - 32:cy + 32, cx - 32:cx + 32] += 0.5 * refs[0]
+    dev = torch.device("cuda:0")
+    pp = gx.ProbabilisticPicker(torch.from_numpy(refs).to(dev), torch.from_numpy(mask[None]).to(dev), di, lp)
+    pp.set_image(torch.from_numpy(image).to(dev))
+    nang = pp.count_angles(step)
+    best = torch.full(di, -1e30, device=dev)
+    ang = torch.zeros(di + (3,), device=dev)
+    ref = torch.zeros(di, dtype=torch.int32, device=dev)
+    sec = _timed_steps(args, lambda: pp.perform_correlation(0, step, best, ang, ref))
+    cstep = 0.4  # the CPU sample: the orientations of anglestep 0.4 rad
+    ncpu = pp.count_angles(cstep)
+    pp.close()
+    px = int(np.prod(di))
+
+    def cpu():
+        po.gaussrand(0, reset=True)
+        t0 = time.perf_counter()
+        po.prob_picker(refs[0].astype(np.float64), mask.astype(np.float64), image.astype(np.float64), lp, cstep)
+        ct = time.perf_counter() - t0
+        return px * ncpu, ct, (f"same micrograph and reference, the {ncpu} orientations of anglestep {cstep}, FP64 "
+                               f"oracle restatement, {ct:.1f} s including its set-up")
+    return _row_line(args, "prob_picker_micrograph", "micrograph pixel-orientations/s, ProbabilisticPicker 2D",
+                     "pixel-orientations/s", px * nang, sec,
+                     {"workload": "ProbabilisticPicker.PerformCorrelation", "image": list(di), "ref": list(dr),
+                      "lowpassfreq": lp, "anglestep": step, "orientations": nang}, cpu)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -363,7 +439,8 @@ def main():
         return 0
     if args.row is not None:
         return {"prob_picker": row_prob_picker, "picker": row_picker, "pick2d": row_pick2d, "diff2": row_diff2,
-                "local_peaks": row_local_peaks}[args.row](args)
+                "local_peaks": row_local_peaks, "fft_micrograph": row_fft_micrograph,
+                "prob_picker_micrograph": row_prob_picker_micrograph}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth

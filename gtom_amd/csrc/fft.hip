@@ -172,11 +172,11 @@ template <int DIR> __device__ __forceinline__ float2 twiddle_rt(const float2* __
   return DIR > 0 ? make_float2(w.x, -w.y) : w;
 }
 
-template <int N, int DIR>
-__global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_lines(const float2* __restrict__ src,
-                                                                              float2* __restrict__ dst, LineMap p,
-                                                                              const float2* __restrict__ tw) {
-  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = fft_col_tx<N>();
+template <int N, int DIR, int TX>
+__global__ __launch_bounds__(TX * ColShape<N>::R2) void k_lines(const float2* __restrict__ src,
+                                                                float2* __restrict__ dst, LineMap p,
+                                                                const float2* __restrict__ tw) {
+  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2;
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
   const int t = threadIdx.x, c = t % TX, j = t / TX;
   const int col = blockIdx.x * TX + c;
@@ -336,20 +336,30 @@ bool split_len(int L, int* L1, int* L2) {
   }
 }
 
-template <int DIR>
-int launch_lines(int n, const float2* src, float2* dst, const LineMap& p, const float2* tw, hipStream_t s) {
-  auto go = [&](auto kernel, int TX, int R2) -> int {
+// Column tile of a pass: up to 64 adjacent lines (512-byte segments) within
+// 256 threads, narrower when the pass has fewer lines.
+template <int N, int DIR>
+int launch_lines_n(const float2* src, float2* dst, const LineMap& p, const float2* tw, hipStream_t s) {
+  constexpr int R2 = ColShape<N>::R2;
+  auto go = [&](auto kernel, int TX) -> int {
     dim3 grid((p.width + TX - 1) / TX, (unsigned)std::min<long long>(p.nplanes, 65535));
     hipLaunchKernelGGL(kernel, grid, dim3(TX * R2), 0, s, src, dst, p, tw);
     GX_LAUNCHED("k_lines");
     return GTOMX_OK;
   };
+  if (p.width >= 64 && 64 * R2 <= 256) return go(k_lines<N, DIR, 64>, 64);
+  if (p.width >= 32 && 32 * R2 <= 256) return go(k_lines<N, DIR, 32>, 32);
+  return go(k_lines<N, DIR, 16>, 16);
+}
+
+template <int DIR>
+int launch_lines(int n, const float2* src, float2* dst, const LineMap& p, const float2* tw, hipStream_t s) {
   switch (n) {
-    case 16: return go(k_lines<16, DIR>, fft_col_tx<16>(), ColShape<16>::R2);
-    case 24: return go(k_lines<24, DIR>, fft_col_tx<24>(), ColShape<24>::R2);
-    case 32: return go(k_lines<32, DIR>, fft_col_tx<32>(), ColShape<32>::R2);
-    case 48: return go(k_lines<48, DIR>, fft_col_tx<48>(), ColShape<48>::R2);
-    case 64: return go(k_lines<64, DIR>, fft_col_tx<64>(), ColShape<64>::R2);
+    case 16: return launch_lines_n<16, DIR>(src, dst, p, tw, s);
+    case 24: return launch_lines_n<24, DIR>(src, dst, p, tw, s);
+    case 32: return launch_lines_n<32, DIR>(src, dst, p, tw, s);
+    case 48: return launch_lines_n<48, DIR>(src, dst, p, tw, s);
+    case 64: return launch_lines_n<64, DIR>(src, dst, p, tw, s);
     default: return fail(GTOMX_ERR_INTERNAL, "four-step factor not instantiated");
   }
 }
