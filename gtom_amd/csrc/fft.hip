@@ -146,14 +146,15 @@ __global__ __launch_bounds__(256) void k_rows_r2c(const float* __restrict__ in, 
 }
 
 // ---------------------------------------------------------------------------
-// Lengths above 256 (row half lengths above 128): the four-step decomposition
-// L = L1 * L2. With n = n2 + L2 n1 and k = k1 + L1 k2,
+// Strided axes above 256: the four-step decomposition L = L1 * L2. With n = n2 + L2 n1 and k = k1 + L1 k2,
 //   X[k1 + L1 k2] = sum_n2 W_L2^{n2 k2} W_L^{n2 k1} sum_n1 x[n2 + L2 n1] W_L1^{n1 k1}.
 // Pass 1 runs an L1-point transform for each n2, multiplies by W_L^{n2 k1} and
 // stores at n2 + L2 k1 (the positions it read) in a scratch copy. Pass 2 runs
 // an L2-point transform for each k1 over the entries L2 k1 + n2 and stores at
-// k1 + L1 k2 of the destination: natural order without a transpose pass. Both
-// passes are batches of strided lines, addressed by LineMap.
+// k1 + L1 k2 of the destination: natural order without a transpose pass (any
+// order of whole lines along a strided axis is coalesced). Both passes are
+// batches of strided lines, addressed by LineMap. Rows above 256 reals run the
+// same decomposition inside LDS (k_rows_*_long below).
 // ---------------------------------------------------------------------------
 struct LineMap {
   long long P1, S0s, S1s, S0d, S1d;  // plane q: base = (q / P1) * S1 + (q % P1) * S0
@@ -205,56 +206,130 @@ __global__ __launch_bounds__(TX * ColShape<N>::R2) void k_lines(const float2* __
   }
 }
 
-// Rows of length N = 2 NC above 256, C2R side: Z[k] = (X[k] + conj X[NC-k]) +
-// i w^k (X[k] - conj X[NC-k]), w = e^{2 pi i/N}, in place on rows of pitch
-// `pitch`; one thread per pair (k, NC - k), k in [0, NC/2]. The x = 0 and
-// x = N/2 bins contribute their real parts only.
-__global__ __launch_bounds__(256) void k_half_to_z(float2* __restrict__ data, long long nrows, int NC,
-                                                   long long pitch, const float2* __restrict__ tw) {
-  const int np = NC / 2 + 1;
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= nrows * np) return;
-  float2* row = data + (idx / np) * pitch;
-  const int k = (int)(idx % np), N = 2 * NC;
-  float2 a = row[k], b = row[NC - k];
-  if (k == 0) {
-    a.y = 0.f;
-    b.y = 0.f;
+// ---------------------------------------------------------------------------
+// Rows of N = 2 NC reals with NC = L1 * L2 in [256, 2048]: the whole four-step
+// in LDS, one HBM round trip. RWG rows per workgroup of 256 threads. Entry i
+// of a row sits at i + 4 (i / L2) in LDS (4 float2 of padding per group of L2,
+// which spreads both passes' strided reads over the banks). Pass A transforms
+// the L2 lines {n2 + L2 n1} in place, times W_NC^{n2 k1}; pass B transforms
+// the L1 contiguous groups in place, leaving Z[k1 + L1 k2] at entry L2 k1 + k2.
+// Lines are dealt to threads in rounds of 256 items (R2 threads per line);
+// threads past the last line compute on zeros, exchange through a sink and
+// store nothing, so every thread meets every barrier.
+// ---------------------------------------------------------------------------
+template <int L1, int L2> struct LongRow {
+  static constexpr int NC = L1 * L2, N = 2 * NC, NH = NC + 1;
+  static constexpr int PITCH = NC + 4 * L1 + 1;
+  __device__ static constexpr int phys(int i) { return i + 4 * (i / L2); }
+  __device__ static constexpr int zpos(int k) { return phys(L2 * (k % L1) + k / L1); }  // Z[k] after pass B
+};
+
+// One pass over all lines of the workgroup's rows: line ln of row r holds
+// entries pos(ln, i), i in [0, LN). The two-step exchange of a line goes
+// through the line's own entries (thread j writes exchange slots (k1, j) to
+// the entries j + R2 k1 it read itself), so a pass needs no other LDS.
+template <int LN, int NL, int RWG, int DIR, int PITCH, class Pos, class Out>
+__device__ __forceinline__ void long_row_pass(float2* buf, float2* sink, const float2* __restrict__ tw, Pos pos,
+                                              Out store) {
+  constexpr int R1 = ColShape<LN>::R1, R2 = ColShape<LN>::R2, ITEMS = RWG * NL * R2,
+                ROUNDS = (ITEMS + 255) / 256;
+  const int t = threadIdx.x, j = t % R2;
+  for (int rd = 0; rd < ROUNDS; ++rd) {
+    const int item = rd * 256 + t;
+    const bool active = item < ITEMS;
+    const int lg = active ? item / R2 : 0, r = lg / NL, ln = lg % NL;
+    float2* row = buf + r * PITCH;
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) v[m] = active ? row[pos(ln, j + R2 * m)] : make_float2(0.f, 0.f);
+    fft_two_step<LN, R1, R2, DIR>(v, j, tw,
+                                  [&](int k1, int jj) { return active ? &row[pos(ln, k1 * R2 + jj)] : sink; });
+    if constexpr (R2 > 1) __syncthreads();  // the line's exchange reads are done before its entries change
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < R1; ++s) {
+        const int k = out_index<R1, R2>(j, s);
+        row[pos(ln, k)] = store(ln, k, v[s]);
+      }
+    }
   }
-  const float2 ac = cconj(a), bc = cconj(b);
-  const float2 zk = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle_rt<1>(tw, N, k), csub(a, bc))));
-  if (k == 0) {
-    row[0] = zk;
-    return;
-  }
-  const float2 zn = cadd(cadd(b, ac), mul_pj<1>(cmul(twiddle_rt<1>(tw, N, NC - k), csub(b, ac))));
-  row[k] = zk;
-  row[NC - k] = zn;
 }
 
-// Rows of length N = 2 NC above 256, R2C side: from Z = DFT_NC(x[2m] + i x[2m+1])
-// in entries [0, NC) of rows of pitch `pitch`, in place,
-// X[k] = ((Z[k] + conj Z[NC-k]) - i w^k (Z[k] - conj Z[NC-k])) / 2, w = e^{-2 pi i/N},
-// and X[NC] = Re Z[0] - Im Z[0]; one thread per pair (k, NC - k).
-__global__ __launch_bounds__(256) void k_z_to_half(float2* __restrict__ data, long long nrows, int NC,
-                                                   long long pitch, const float2* __restrict__ tw) {
-  const int np = NC / 2 + 1;
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= nrows * np) return;
-  float2* row = data + (idx / np) * pitch;
-  const int k = (int)(idx % np), N = 2 * NC;
-  const float2 zk = row[k], zn = row[(NC - k) % NC];
-  const float2 zkc = cconj(zk), znc = cconj(zn);
-  const float2 xk = cscale(csub(cadd(zk, znc), mul_pj<1>(cmul(twiddle_rt<-1>(tw, N, k), csub(zk, znc)))), 0.5f);
-  if (k == 0) {
-    row[0] = xk;
-    row[NC] = make_float2(zk.x - zk.y, 0.f);
-    return;
+template <int L1, int L2, int RWG, int DIR>
+__device__ __forceinline__ void long_row_passes(float2* buf, float2* sink, const float2* __restrict__ tw) {
+  using LR = LongRow<L1, L2>;
+  // pass A: the L2 lines {n2 + L2 n1} of L1 points, times W_NC^{n2 k1}
+  long_row_pass<L1, L2, RWG, DIR, LR::PITCH>(
+      buf, sink, tw, [](int n2, int i) { return LR::phys(n2 + L2 * i); },
+      [&](int n2, int k1, float2 x) { return cmul(x, twiddle_rt<DIR>(tw, LR::NC, n2 * k1)); });
+  __syncthreads();
+  // pass B: the L1 contiguous groups of L2 points
+  long_row_pass<L2, L1, RWG, DIR, LR::PITCH>(
+      buf, sink, tw, [](int k1, int i) { return LR::phys(L2 * k1 + i); }, [](int, int, float2 x) { return x; });
+  __syncthreads();
+}
+
+template <int L1, int L2, int RWG>
+__global__ __launch_bounds__(256) void k_rows_r2c_long(const float* __restrict__ in, float2* __restrict__ out,
+                                                       long long nrows, const float2* __restrict__ tw) {
+  using LR = LongRow<L1, L2>;
+  constexpr int NC = LR::NC, NH = LR::NH;
+  __shared__ float2 buf[RWG * LR::PITCH];
+  __shared__ float2 sink[1];  // exchange target of threads without a line
+  const int t = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * RWG;
+  const int nr = (int)(nrows - row0 < RWG ? nrows - row0 : RWG);
+  const float2* src = reinterpret_cast<const float2*>(in) + row0 * NC;
+  for (int i = t; i < nr * NC; i += 256) buf[(i / NC) * LR::PITCH + LR::phys(i % NC)] = src[i];
+  __syncthreads();
+  long_row_passes<L1, L2, RWG, -1>(buf, sink, tw);
+  float2* dst = out + row0 * NH;
+  for (int i = t; i < nr * NH; i += 256) {
+    const int r = i / NH, k = i % NH;
+    const float2* row = buf + r * LR::PITCH;
+    float2 x;
+    if (k == NC) {
+      const float2 z0 = row[LR::zpos(0)];
+      x = make_float2(z0.x - z0.y, 0.f);
+    } else {
+      const float2 zk = row[LR::zpos(k)], zn = cconj(row[LR::zpos((NC - k) % NC)]);
+      x = cscale(csub(cadd(zk, zn), mul_pj<1>(cmul(twiddle_rt<-1>(tw, LR::N, k), csub(zk, zn)))), 0.5f);
+    }
+    dst[i] = x;
   }
-  const float2 xn =
-      cscale(csub(cadd(zn, zkc), mul_pj<1>(cmul(twiddle_rt<-1>(tw, N, NC - k), csub(zn, zkc)))), 0.5f);
-  row[k] = xk;
-  row[NC - k] = xn;
+}
+
+template <int L1, int L2, int RWG>
+__global__ __launch_bounds__(256) void k_rows_c2r_long(const float2* __restrict__ in, float* __restrict__ out,
+                                                       long long nrows, float scale,
+                                                       const float2* __restrict__ tw) {
+  using LR = LongRow<L1, L2>;
+  constexpr int NC = LR::NC, NH = LR::NH, NP = NC / 2 + 1;
+  __shared__ float2 buf[RWG * LR::PITCH];
+  __shared__ float2 sink[1];  // exchange target of threads without a line
+  const int t = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * RWG;
+  const int nr = (int)(nrows - row0 < RWG ? nrows - row0 : RWG);
+  const float2* src = in + row0 * NH;
+  for (int i = t; i < nr * NH; i += 256) buf[(i / NH) * LR::PITCH + LR::phys(i % NH)] = src[i];
+  __syncthreads();
+  for (int i = t; i < nr * NP; i += 256) {  // half spectrum -> Z, one thread per pair (k, NC - k)
+    const int r = i / NP, k = i % NP;
+    float2* row = buf + r * LR::PITCH;
+    float2 a = row[LR::phys(k)], b = row[LR::phys(NC - k)];
+    if (k == 0) {
+      a.y = 0.f;
+      b.y = 0.f;
+    }
+    const float2 ac = cconj(a), bc = cconj(b);
+    row[LR::phys(k)] = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle_rt<1>(tw, LR::N, k), csub(a, bc))));
+    if (k > 0)
+      row[LR::phys(NC - k)] = cadd(cadd(b, ac), mul_pj<1>(cmul(twiddle_rt<1>(tw, LR::N, NC - k), csub(b, ac))));
+  }
+  __syncthreads();
+  long_row_passes<L1, L2, RWG, 1>(buf, sink, tw);
+  float2* dst = reinterpret_cast<float2*>(out) + row0 * NC;
+  for (int i = t; i < nr * NC; i += 256) dst[i] = cscale(buf[(i / NC) * LR::PITCH + LR::zpos(i % NC)], scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,48 +457,38 @@ int cols_axis(int L, float2* data, float2* scratch, int width, long long as, lon
   return launch_lines<DIR>(L2, scratch, data, b, tw, s);
 }
 
-// Row R2C of length N > 256: rows of N reals (NC = N/2 complex) into rows of
-// NC + 1 bins, through `scratch` (nrows * NC complex).
-int rows_r2c_big(int N, const float* in, float2* out, float2* scratch, long long nrows, const float2* tw,
-                 hipStream_t s) {
-  const int NC = N / 2;
-  int L1, L2;
-  if (!split_len(NC, &L1, &L2)) return fail(GTOMX_ERR_INTERNAL, "row length not split");
-  LineMap a{};
-  a.P1 = 1LL << 62, a.S0s = a.S0d = NC, a.Cs = a.Cd = 1, a.G = 1, a.Ms = a.Md = L2, a.nplanes = nrows;
-  a.width = L2, a.L = NC, a.TG = 0, a.TC = 1, a.scale = 1.f;
-  GX_TRY(launch_lines<-1>(L1, reinterpret_cast<const float2*>(in), scratch, a, tw, s));
-  LineMap b = a;
-  b.S0d = NC + 1, b.Cs = L2, b.Cd = 1, b.Ms = 1, b.Md = L1, b.width = L1, b.L = 0;
-  GX_TRY(launch_lines<-1>(L2, scratch, out, b, tw, s));
-  const long long n = nrows * (NC / 2 + 1);
-  hipLaunchKernelGGL(k_z_to_half, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, nrows, NC,
-                     (long long)(NC + 1), tw);
-  GX_LAUNCHED("k_z_to_half");
-  return GTOMX_OK;
+// Long rows in one pass (NC = N/2 in [256, 2048]).
+template <bool R2C>
+int launch_rows_long(int n, const void* in, void* out, long long nrows, float scale, const float2* tw,
+                     hipStream_t s) {
+  auto go = [&](auto r2c_kernel, auto c2r_kernel, int rwg) -> int {
+    dim3 grid((unsigned)((nrows + rwg - 1) / rwg));
+    if constexpr (R2C) {
+      hipLaunchKernelGGL(r2c_kernel, grid, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float2*>(out),
+                         nrows, tw);
+      GX_LAUNCHED("k_rows_r2c_long");
+    } else {
+      hipLaunchKernelGGL(c2r_kernel, grid, dim3(256), 0, s, static_cast<const float2*>(in), static_cast<float*>(out),
+                         nrows, scale, tw);
+      GX_LAUNCHED("k_rows_c2r_long");
+    }
+    return GTOMX_OK;
+  };
+  switch (n / 2) {
+#define GX_LONG(NC, A, B, W) \
+  case NC: return go(k_rows_r2c_long<A, B, W>, k_rows_c2r_long<A, B, W>, W);
+    GX_LONG(256, 16, 16, 8)
+    GX_LONG(384, 16, 24, 8)
+    GX_LONG(512, 16, 32, 4)
+    GX_LONG(768, 32, 24, 2)
+    GX_LONG(1024, 32, 32, 2)
+    GX_LONG(1536, 48, 32, 1)
+    GX_LONG(2048, 64, 32, 1)
+#undef GX_LONG
+    default: return fail(GTOMX_ERR_INTERNAL, "long row length not instantiated");
+  }
 }
 
-// Row C2R of length N > 256: rows of NC + 1 bins (modified in place) into
-// rows of N reals, scaled, through `scratch` (nrows * NC complex).
-int rows_c2r_big(int N, float2* work, float* out, float2* scratch, long long nrows, float scale,
-                 const float2* tw, hipStream_t s) {
-  const int NC = N / 2;
-  int L1, L2;
-  if (!split_len(NC, &L1, &L2)) return fail(GTOMX_ERR_INTERNAL, "row length not split");
-  const long long n = nrows * (NC / 2 + 1);
-  hipLaunchKernelGGL(k_half_to_z, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, work, nrows, NC,
-                     (long long)(NC + 1), tw);
-  GX_LAUNCHED("k_half_to_z");
-  LineMap a{};
-  a.P1 = 1LL << 62, a.S0s = NC + 1, a.S0d = NC, a.Cs = a.Cd = 1, a.G = 1, a.Ms = a.Md = L2, a.nplanes = nrows;
-  a.width = L2, a.L = NC, a.TG = 0, a.TC = 1, a.scale = 1.f;
-  GX_TRY(launch_lines<1>(L1, work, scratch, a, tw, s));
-  LineMap b = a;
-  b.S0s = NC, b.S0d = NC, b.Cs = L2, b.Cd = 1, b.Ms = 1, b.Md = L1, b.width = L1, b.L = 0, b.scale = scale;
-  return launch_lines<1>(L2, scratch, reinterpret_cast<float2*>(out), b, tw, s);
-}
-
-bool needs_scratch(gtomx_int3 d) { return d.x > 256 || d.y > 256 || d.z > 256; }
 
 // Stream-ordered scratch for the two-pass axes: taken from the stream's pool
 // and given back on the same stream, so no synchronisation is needed.
@@ -470,9 +535,9 @@ int fft_r2c(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStrea
   GX_TRY(twiddles(&tw));
   const long long rows = (long long)d.y * d.z * batch;
   StreamScratch scr;
-  if (needs_scratch(d)) GX_TRY(scr.take(elements_ft(d) * batch, s));
+  if (d.y > 256 || d.z > 256) GX_TRY(scr.take(elements_ft(d) * batch, s));
   if (d.x > 256)
-    GX_TRY(rows_r2c_big(d.x, in, out, scr.p, rows, tw, s));
+    GX_TRY(launch_rows_long<true>(d.x, in, out, rows, 1.f, tw, s));
   else
     GX_TRY(launch_rows_r2c(d.x, in, out, rows, tw, s));
   return yz_axes(out, scr.p, d, batch, -1, tw, s);
@@ -485,10 +550,10 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned b
   const size_t H = elements_ft(d) * batch;
   if (work != in) GX_HIP(hipMemcpyAsync(work, in, H * sizeof(float2), hipMemcpyDeviceToDevice, s));
   StreamScratch scr;
-  if (needs_scratch(d)) GX_TRY(scr.take(H, s));
+  if (d.y > 256 || d.z > 256) GX_TRY(scr.take(H, s));
   GX_TRY(yz_axes(work, scr.p, d, batch, 1, tw, s));
   const long long rows = (long long)d.y * d.z * batch;
-  if (d.x > 256) return rows_c2r_big(d.x, work, out, scr.p, rows, scale, tw, s);
+  if (d.x > 256) return launch_rows_long<false>(d.x, work, out, rows, scale, tw, s);
   return launch_rows_c2r(d.x, work, out, rows, scale, tw, s);
 }
 
