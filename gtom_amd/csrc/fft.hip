@@ -173,11 +173,20 @@ template <int DIR> __device__ __forceinline__ float2 twiddle_rt(const float2* __
   return DIR > 0 ? make_float2(w.x, -w.y) : w;
 }
 
+// Register split of the passes' line lengths: the column shapes, except 64 =
+// 16 x 4, which lets a 256-thread workgroup span 64 lines (512-byte segments).
+template <int N> struct LineShape {
+  static constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2;
+};
+template <> struct LineShape<64> {
+  static constexpr int R1 = 16, R2 = 4;
+};
+
 template <int N, int DIR, int TX>
-__global__ __launch_bounds__(TX * ColShape<N>::R2) void k_lines(const float2* __restrict__ src,
-                                                                float2* __restrict__ dst, LineMap p,
-                                                                const float2* __restrict__ tw) {
-  constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2;
+__global__ __launch_bounds__(TX * LineShape<N>::R2) void k_lines(const float2* __restrict__ src,
+                                                                 float2* __restrict__ dst, LineMap p,
+                                                                 const float2* __restrict__ tw) {
+  constexpr int R1 = LineShape<N>::R1, R2 = LineShape<N>::R2;
   __shared__ float2 xch[R2 > 1 ? TX * N : 1];
   const int t = threadIdx.x, c = t % TX, j = t / TX;
   const int col = blockIdx.x * TX + c;
@@ -185,12 +194,14 @@ __global__ __launch_bounds__(TX * ColShape<N>::R2) void k_lines(const float2* __
   for (long long pl = blockIdx.y; pl < p.nplanes; pl += gridDim.y) {
     const long long q = pl / p.G;
     const int g = (int)(pl % p.G);
-    const long long sb = (q / p.P1) * p.S1s + (q % p.P1) * p.S0s + col * p.Cs + g * p.Gs;
-    const long long db = (q / p.P1) * p.S1d + (q % p.P1) * p.S0d + col * p.Cd + g * p.Gd;
+    // 64-bit line bases; offsets along a line are 32-bit (the host keeps every
+    // buffer below 2^31 elements)
+    const float2* sl = src + ((q / p.P1) * p.S1s + (q % p.P1) * p.S0s + col * p.Cs + g * p.Gs);
+    float2* dl = dst + ((q / p.P1) * p.S1d + (q % p.P1) * p.S0d + col * p.Cd + g * p.Gd);
+    const int ms = (int)p.Ms, md = (int)p.Md;
     float2 v[R1];
 #pragma unroll
-    for (int m = 0; m < R1; ++m)
-      v[m] = valid ? src[sb + (long long)(j + R2 * m) * p.Ms] : make_float2(0.f, 0.f);
+    for (int m = 0; m < R1; ++m) v[m] = valid ? sl[(j + R2 * m) * ms] : make_float2(0.f, 0.f);
     __syncthreads();
     fft_two_step<N, R1, R2, DIR>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
     if (valid) {
@@ -200,7 +211,7 @@ __global__ __launch_bounds__(TX * ColShape<N>::R2) void k_lines(const float2* __
         const int k = out_index<R1, R2>(j, s);
         float2 o = v[s];
         if (p.L > 0) o = cmul(o, twiddle_rt<DIR>(tw, p.L, e * k));
-        dst[db + (long long)k * p.Md] = cscale(o, p.scale);
+        dl[k * md] = cscale(o, p.scale);
       }
     }
   }
@@ -415,7 +426,7 @@ bool split_len(int L, int* L1, int* L2) {
 // 256 threads, narrower when the pass has fewer lines.
 template <int N, int DIR>
 int launch_lines_n(const float2* src, float2* dst, const LineMap& p, const float2* tw, hipStream_t s) {
-  constexpr int R2 = ColShape<N>::R2;
+  constexpr int R2 = LineShape<N>::R2;
   auto go = [&](auto kernel, int TX) -> int {
     dim3 grid((p.width + TX - 1) / TX, (unsigned)std::min<long long>(p.nplanes, 65535));
     hipLaunchKernelGGL(kernel, grid, dim3(TX * R2), 0, s, src, dst, p, tw);
@@ -447,6 +458,7 @@ int cols_axis(int L, float2* data, float2* scratch, int width, long long as, lon
   if (L <= 256) return launch_cols_n<DIR>(L, data, width, as, nplanes, P1, S0, S1, tw, s);
   int L1, L2;
   if (!split_len(L, &L1, &L2) || !scratch) return fail(GTOMX_ERR_INTERNAL, "axis length not split");
+  if ((long long)L * as >= (1LL << 31)) return fail(GTOMX_ERR_UNSUPPORTED, "axis span of 2^31 elements or more");
   LineMap a{};
   a.P1 = P1, a.S0s = a.S0d = S0, a.S1s = a.S1d = S1, a.Cs = a.Cd = 1;
   a.G = L2, a.Gs = a.Gd = as, a.Ms = a.Md = L2 * as, a.nplanes = nplanes * L2, a.width = width;

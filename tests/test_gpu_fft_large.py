@@ -90,3 +90,16 @@ def test_fft_batched_3072_planes_match_numpy(gx, cuda):
     X = gx.d_FFTR2C(torch.from_numpy(x).to(cuda), ndims=2)
     ref = np.fft.rfft2(x.astype(np.float64))
     assert rel_linf(to_np(X), ref) < TOL
+
+
+@pytest.mark.parametrize("shape,r", [((1, 1024, 1024), 24.0), ((1, 1024, 2048), 40.0)])
+def test_local_std_micrograph_matches_oracle(gx, orc, cuda, shape, r):
+    """d_LocalStd on micrograph-sized images (its transforms take the two-pass
+    and LDS-row paths), against the FP64 oracle at 1e-5 normwise."""
+    import torch
+    rng = np.random.default_rng(shape[2] + int(r))
+    m = (rng.standard_normal(shape) * 3 + 1).astype(np.float32)
+    sd, mean = gx.d_LocalStd(torch.from_numpy(m).to(cuda), r, with_mean=True)
+    rsd, rmean = orc.local_std(m.astype(np.float64), r, with_mean=True)
+    assert rel_linf(to_np(sd), rsd) < 1e-5
+    assert rel_linf(to_np(mean), rmean) < 1e-5
