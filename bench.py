@@ -43,7 +43,7 @@ def parse():
                     help="only run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
-                                      "prob_picker_micrograph"], default=None,
+                                      "prob_picker_micrograph", "pick2d_micrograph"], default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
 
@@ -229,6 +229,46 @@ def row_pick2d(args):
         return n * n * k, r["loop_seconds"], (f"1 image, the first {k} orientations, FP64 oracle, orientation loop "
                                               f"{r['loop_seconds']:.1f} s")
     return _row_line(args, "pick2d", "image pixel-orientations/s, d_PickSubTomograms 2D branch",
+                     "pixel-orientations/s", work, sec,
+                     {"workload": "d_PickSubTomograms (2D)", "images": nimg, "image": [n, n], "template": [box] * 3,
+                      "orientations": len(angles)}, cpu)
+
+
+def row_pick2d_micrograph(args):
+    """The 2D branch of d_PickSubTomograms at micrograph size: one 4096^2
+    micrograph, a 64^3 template (projector 128^3, oversample 1/32), 480
+    orientations (60 Fibonacci directions x 8 in-plane steps), one CTF at
+    1 A/px, local-std radius 512."""
+    import torch
+    import gtom_amd as gx
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(args.seed)
+    n, box, nimg = 4096, 64, 1
+    p = 2 * box
+    dev = torch.device("cuda:0")
+    angles = gx.fibonacci_angles(60, 8)
+    projft = gx.prepare_projector(torch.from_numpy(_blob(box, rng)).to(dev), p)
+    imgs = rng.standard_normal((nimg, n, n)).astype(np.float32)
+    E = gx.d_FFTR2C(torch.from_numpy(imgs).to(dev), ndims=2)
+    ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=1e-10, Cs=2.7e-3, voltage=300e3, defocus=-2e-6, amplitude=0.07)],
+                    n, device=dev, ndims=2)
+    plan = gx.PickPlan(p, n, 0, ndims=2)
+    best = torch.empty((nimg, n, n), device=dev)
+    idx = torch.empty((nimg, n, n), dtype=torch.int32, device=dev)
+    sec = _timed_steps(args, lambda: gx.d_PickSubTomograms(plan, projft, p / n, E, ctf, angles, n / 8, best=best,
+                                                            bestidx=idx))
+    plan.close()
+    work = nimg * n * n * len(angles)
+
+    def cpu():
+        k = 16
+        P = projft.cpu().numpy()
+        E0 = E[:1].cpu().numpy()
+        C = ctf[0].cpu().numpy()
+        r = po.pick(P, p, p / n, E0, C, n, angles[:k].astype(np.float64), n / 8, ndims=2)
+        return n * n * k, r["loop_seconds"], (f"the micrograph, the first {k} orientations, FP64 oracle, "
+                                              f"orientation loop {r['loop_seconds']:.1f} s")
+    return _row_line(args, "pick2d_micrograph", "micrograph pixel-orientations/s, d_PickSubTomograms 2D branch",
                      "pixel-orientations/s", work, sec,
                      {"workload": "d_PickSubTomograms (2D)", "images": nimg, "image": [n, n], "template": [box] * 3,
                       "orientations": len(angles)}, cpu)
@@ -440,7 +480,8 @@ def main():
     if args.row is not None:
         return {"prob_picker": row_prob_picker, "picker": row_picker, "pick2d": row_pick2d, "diff2": row_diff2,
                 "local_peaks": row_local_peaks, "fft_micrograph": row_fft_micrograph,
-                "prob_picker_micrograph": row_prob_picker_micrograph}[args.row](args)
+                "prob_picker_micrograph": row_prob_picker_micrograph,
+                "pick2d_micrograph": row_pick2d_micrograph}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth

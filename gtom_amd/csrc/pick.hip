@@ -639,6 +639,58 @@ __global__ __launch_bounds__(256) void k_pick_scale(const float* __restrict__ pa
   if (t == 0) scale[o] = (float)(1.0 / fmax(sqrt(red[0] / h), 1e-20));
 }
 
+// ---------------------------------------------------------------------------
+// The 2D branch for images above 256 (micrographs up to 4096^2), on the
+// generic transforms: pass 1 forms E * conj(CTF * R) for every bin of the
+// half spectrum (DC zeroed) with per-workgroup sums of |R|^2, the unit-RMS
+// scale comes from k_pick2d_herm_norm and k_pick_scale as in the fused form,
+// the batch goes through the generic C2R, and k_pick2d_update_any folds the
+// scaled maps into best / index in orientation order (strict >, first index
+// on ties).
+// ---------------------------------------------------------------------------
+template <class PJ>
+__global__ __launch_bounds__(256) void k_pick2d_project_any(PJ P, const float* __restrict__ mats,
+                                                            const float2* __restrict__ E,
+                                                            const float* __restrict__ ctf, float2* __restrict__ G,
+                                                            float* __restrict__ partial, int n) {
+  __shared__ float red[4];
+  const int NH = n / 2 + 1, HALF = n / 2;
+  const int H = NH * n;
+  const int o = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < H;
+  const int ix = valid ? i % NH : 0, iy = valid ? i / NH : 0, y = signed_freq(iy, n);
+  float M[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+  const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
+  const bool ins = valid && ix * ix + y * y <= HALF * HALF;
+  const float2 e = valid ? E[i] : make_float2(0.f, 0.f);
+  const float cf = valid ? (ctf ? ctf[i] : 1.f) : 0.f;
+  const float2 r = cscale(sample(rs, P, M[0] * ix + M[1] * y, M[3] * ix + M[4] * y, M[6] * ix + M[7] * y, ins), cf);
+  const float2 v = i == 0 ? make_float2(0.f, 0.f) : cmulc(e, r);
+  if (valid) G[(size_t)o * H + i] = v;
+  const float tot = block_sum(r.x * r.x + r.y * r.y, red);
+  if (threadIdx.x == 0) partial[(size_t)o * gridDim.x + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_pick2d_update_any(const float* __restrict__ maps, int nb, size_t V,
+                                                           const float* __restrict__ scale, int angle0,
+                                                           float* __restrict__ best, int32_t* __restrict__ bidx) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < V; i += (size_t)gridDim.x * 256) {
+    float b = best[i];
+    int32_t k = bidx[i];
+    for (int o = 0; o < nb; ++o) {
+      const float c = maps[(size_t)o * V + i] * scale[o];
+      if (c > b) {
+        b = c;
+        k = angle0 + o;
+      }
+    }
+    best[i] = b;
+    bidx[i] = k;
+  }
+}
+
 }  // namespace gx
 
 // ---------------------------------------------------------------------------
@@ -670,6 +722,8 @@ struct gtomx_pick_plan {
   float* gbest = nullptr;   // 2D: per-orientation-group maxima (groups x V)
   int32_t* gidx = nullptr;
   int og = 0;               // 2D: orientations per group
+  float* maps = nullptr;    // 2D above 256: B real correlation maps
+  int nparts = 0;           // 2D above 256: pass-1 workgroups per orientation
   size_t device_bytes = 0;
   // kernel timing
   bool timing = false;
@@ -697,6 +751,7 @@ void plan_release(gtomx_pick_plan* pl) {
   dev_free(pl->cproj);
   dev_free(pl->gbest);
   dev_free(pl->gidx);
+  dev_free(pl->maps);
   if (pl->h_mats) (void)hipHostFree(pl->h_mats);
   if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
   for (hipEvent_t e : pl->ev_pool) (void)hipEventDestroy(e);
@@ -872,9 +927,55 @@ int run_batch2d(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const flo
   return GTOMX_OK;
 }
 
+// The 2D branch above 256: pass 1 over every bin, the same scale, the generic
+// C2R into the plan's map buffer and the in-order update.
+int herm_norm_any(int n, const Projector& P, const float* mats, const float* ctf, float* corr, int nb,
+                  hipStream_t s) {
+  switch (n) {
+#define GX_CASE(N)                                                                                            \
+  case N:                                                                                                     \
+    hipLaunchKernelGGL(k_pick2d_herm_norm<N>, dim3(nb), dim3(256), 0, s, P, mats, ctf, corr);                 \
+    GX_LAUNCHED("k_pick2d_herm_norm");                                                                        \
+    return GTOMX_OK;
+    GX_CASE(384) GX_CASE(512) GX_CASE(768) GX_CASE(1024) GX_CASE(1536) GX_CASE(2048) GX_CASE(3072) GX_CASE(4096)
+#undef GX_CASE
+  }
+  return fail(GTOMX_ERR_UNSUPPORTED, "image side not supported");
+}
+
+template <class PJ>
+int run_batch2d_any(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float* mats, int nb,
+                    const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, hipStream_t s) {
+  const int n = pl->n;
+  GX_TRY(timed(pl, 0, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick2d_project_any<PJ>, dim3(pl->nparts, nb), dim3(256), 0, s, PS, mats, E, ctf, pl->G,
+                       pl->partial, n);
+    GX_LAUNCHED("k_pick2d_project_any");
+    return GTOMX_OK;
+  }));
+  GX_TRY(herm_norm_any(n, P, mats, ctf, pl->corr, nb, s));
+  GX_TRY(timed(pl, 1, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick_scale, dim3(nb), dim3(256), 0, s, pl->partial, pl->nparts, pl->corr, 1, (double)pl->H,
+                       pl->scale);
+    GX_LAUNCHED("k_pick_scale");
+    return GTOMX_OK;
+  }));
+  return timed(pl, 2, s, [&]() -> int {
+    GX_TRY(fft_c2r(pl->G, pl->G, pl->maps, gtomx_int3{n, n, 1}, (unsigned)nb, 1.0f, s));
+    const unsigned grid = (unsigned)std::min<size_t>((pl->V + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_pick2d_update_any, dim3(grid), dim3(256), 0, s, pl->maps, nb, pl->V, pl->scale, angle0,
+                       best, idx);
+    GX_LAUNCHED("k_pick2d_update_any");
+    return GTOMX_OK;
+  });
+}
+
 int run_batch2d_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projector& P, const float* mats, int nb,
                   const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
                   hipStream_t s) {
+  if (pl->n > 256)
+    return C ? run_batch2d_any(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, s)
+             : run_batch2d_any(pl, P, P, mats, nb, E, ctf, best, idx, angle0, s);
   switch (pl->n) {
 #define GX_CASE(N)                                                                         \
   case N:                                                                                  \
@@ -944,7 +1045,10 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   const bool is2d = dv.z == 1;
   if (dv.x != dv.y || (!is2d && dv.x != dv.z))
     return fail(GTOMX_ERR_UNSUPPORTED, "volumes must be cubes (or images squares)");
-  if (!fft_len_ok(dv.x)) return fail(GTOMX_ERR_UNSUPPORTED, "volume side must be 2^a or 3*2^a in [8, 256]");
+  const bool big2d = is2d && dv.x > 256;  // micrographs: the generic-transform form of the 2D branch
+  if (big2d ? !fft_len_ok_any(dv.x) : !fft_len_ok(dv.x))
+    return fail(GTOMX_ERR_UNSUPPORTED,
+                "volume side must be 2^a or 3*2^a in [8, 256]; image side also 2^a up to 4096 or 3*2^a up to 3072");
   if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 512)
     return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 512");
   if (batch > 256) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 256");
@@ -953,12 +1057,15 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   pl->nz = dv.z;
   pl->p = dp.x;
   // 2D: the reference's 240 orientations per pass (src/Correlation/SubTomograms.cu:35-36)
-  pl->B = batch ? batch : (is2d ? 240u : (dv.x >= 128 ? 32u : 16u));
-  pl->og = dv.x >= 128 ? 2 : 1;
   pl->H = elements_ft(dv);
   pl->V = elements(dv);
+  // above 256 the default batch keeps the spectra near 1 GiB (16 orientations at 4096^2)
+  const unsigned big_b = (unsigned)std::max<size_t>(1, std::min<size_t>(240, (1ull << 30) / (pl->H * sizeof(float2))));
+  pl->B = batch ? batch : (is2d ? (big2d ? big_b : 240u) : (dv.x >= 128 ? 32u : 16u));
+  pl->og = dv.x >= 128 ? 2 : 1;
+  pl->nparts = big2d ? (int)((pl->H + 255) / 256) : 0;
   const std::vector<int> tiles = is2d ? std::vector<int>{0} : tiles_for(pl->n);
-  pl->ntiles = is2d ? g2_tiles_for(pl->n) : (int)tiles.size();
+  pl->ntiles = big2d ? 1 : (is2d ? g2_tiles_for(pl->n) : (int)tiles.size());
   int s = GTOMX_OK;
   hipError_t he = hipGetDevice(&pl->device);
   if (he != hipSuccess) {
@@ -972,10 +1079,12 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
     if (s == GTOMX_OK) pl->device_bytes += bytes;
   };
   // The batch buffer also serves as the C2R workspace in pick_finish (H elements).
-  alloc(&d, std::max((size_t)pl->B * (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n)), pl->H) * sizeof(float2));
+  const size_t gstride = big2d ? pl->H : (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n));
+  alloc(&d, std::max((size_t)pl->B * gstride, pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
-  alloc(&d, (size_t)pl->B * pl->ntiles * 4 * sizeof(float));  // up to 4 pass-1 waves per tile
+  // up to 4 pass-1 waves per tile; above 256, one sum per 256 bins
+  alloc(&d, (size_t)pl->B * (big2d ? (size_t)pl->nparts : (size_t)pl->ntiles * 4) * sizeof(float));
   pl->partial = static_cast<float*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->ntiles * sizeof(int));
@@ -993,7 +1102,11 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   d = nullptr;
   alloc(&d, (size_t)centred_bytes(pl->p));
   pl->cproj = static_cast<float2*>(d);
-  if (is2d) {
+  if (big2d) {
+    d = nullptr;
+    alloc(&d, (size_t)pl->B * pl->V * sizeof(float));
+    pl->maps = static_cast<float*>(d);
+  } else if (is2d) {
     const size_t groups = (pl->B + pl->og - 1) / pl->og;
     d = nullptr;
     alloc(&d, groups * pl->V * sizeof(float));

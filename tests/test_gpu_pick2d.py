@@ -87,6 +87,22 @@ def test_pick2d_batch_size_does_not_change_results(gx, orc, cuda):
     assert np.array_equal(i1, i2) and np.array_equal(b1, b2)
 
 
+@pytest.mark.parametrize("n,nang,ctf_per_image", [(512, 60, False), (768, 40, True), (1024, 40, False)])
+def test_pick2d_micrograph_sizes_match_oracle(gx, orc, cuda, n, nang, ctf_per_image):
+    """Images above 256 (the generic-transform form of the 2D branch: every
+    half-spectrum bin projected, the batch through the two-pass / LDS-row C2R,
+    maps folded into best / index in orientation order)."""
+    best, idx, ref = _run(gx, orc, cuda, n, 32, nimg=2 if ctf_per_image else 1, nang=nang, batch=0, seed=n,
+                          ctf_per_image=ctf_per_image)
+    _check(best, idx, ref)
+
+
+def test_pick2d_micrograph_batch_size_does_not_change_results(gx, orc, cuda):
+    b1, i1, _ = _run(gx, orc, cuda, 512, 32, nimg=1, nang=50, batch=0, seed=5)
+    b2, i2, _ = _run(gx, orc, cuda, 512, 32, nimg=1, nang=50, batch=7, seed=5)
+    assert np.array_equal(i1, i2) and np.array_equal(b1, b2)
+
+
 def test_project2d_matches_oracle(gx, orc, cuda):
     import torch
     box, n = 16, 48
