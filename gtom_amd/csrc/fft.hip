@@ -310,18 +310,14 @@ __global__ __launch_bounds__(256) void k_rows_r2c_long(const float* __restrict__
   }
 }
 
+// Loads nr rows of half spectra (pitch NH in `src`) into LDS and leaves their
+// inverse Z (z[m] = x[2m] + i x[2m+1], unscaled) at zpos(m) of each row.
 template <int L1, int L2, int RWG>
-__global__ __launch_bounds__(256) void k_rows_c2r_long(const float2* __restrict__ in, float* __restrict__ out,
-                                                       long long nrows, float scale,
-                                                       const float2* __restrict__ tw) {
+__device__ __forceinline__ void long_row_c2r_core(float2* buf, float2* sink, const float2* __restrict__ src, int nr,
+                                                  const float2* __restrict__ tw) {
   using LR = LongRow<L1, L2>;
   constexpr int NC = LR::NC, NH = LR::NH, NP = NC / 2 + 1;
-  __shared__ float2 buf[RWG * LR::PITCH];
-  __shared__ float2 sink[1];  // exchange target of threads without a line
   const int t = threadIdx.x;
-  const long long row0 = (long long)blockIdx.x * RWG;
-  const int nr = (int)(nrows - row0 < RWG ? nrows - row0 : RWG);
-  const float2* src = in + row0 * NH;
   for (int i = t; i < nr * NH; i += 256) buf[(i / NH) * LR::PITCH + LR::phys(i % NH)] = src[i];
   __syncthreads();
   for (int i = t; i < nr * NP; i += 256) {  // half spectrum -> Z, one thread per pair (k, NC - k)
@@ -339,8 +335,84 @@ __global__ __launch_bounds__(256) void k_rows_c2r_long(const float2* __restrict_
   }
   __syncthreads();
   long_row_passes<L1, L2, RWG, 1>(buf, sink, tw);
+}
+
+template <int L1, int L2, int RWG>
+__global__ __launch_bounds__(256) void k_rows_c2r_long(const float2* __restrict__ in, float* __restrict__ out,
+                                                       long long nrows, float scale,
+                                                       const float2* __restrict__ tw) {
+  using LR = LongRow<L1, L2>;
+  constexpr int NC = LR::NC, NH = LR::NH;
+  __shared__ float2 buf[RWG * LR::PITCH];
+  __shared__ float2 sink[1];  // exchange target of threads without a line
+  const int t = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * RWG;
+  const int nr = (int)(nrows - row0 < RWG ? nrows - row0 : RWG);
+  long_row_c2r_core<L1, L2, RWG>(buf, sink, in + row0 * NH, nr, tw);
   float2* dst = reinterpret_cast<float2*>(out) + row0 * NC;
   for (int i = t; i < nr * NC; i += 256) dst[i] = cscale(buf[(i / NC) * LR::PITCH + LR::zpos(i % NC)], scale);
+}
+
+// The search's row pass for images above 256: the row C2R of every
+// orientation of a batch (spectra H apart in G), each value times scale[o],
+// folded into best / index in orientation order (strict >, first index on
+// ties). The workgroup's rows keep their running maxima in registers.
+template <int L1, int L2, int RWG>
+__global__ __launch_bounds__(256) void k_pick_rows_max_long(const float2* __restrict__ G, int nb, long long H,
+                                                            int nrows, int angle0, const float* __restrict__ scale,
+                                                            float* __restrict__ best, int32_t* __restrict__ bidx,
+                                                            const float2* __restrict__ tw) {
+  using LR = LongRow<L1, L2>;
+  constexpr int NC = LR::NC, NH = LR::NH, Q = (RWG * NC + 255) / 256;
+  __shared__ float2 buf[RWG * LR::PITCH];
+  __shared__ float2 sink[1];
+  const int t = threadIdx.x;
+  const int row0 = blockIdx.x * RWG;
+  const int nr = nrows - row0 < RWG ? nrows - row0 : RWG;
+  float2* bp = reinterpret_cast<float2*>(best) + (size_t)row0 * NC;  // values 2m, 2m+1 of row r at r * NC + m
+  int2* ip = reinterpret_cast<int2*>(bidx) + (size_t)row0 * NC;
+  float bv[2 * Q];
+  int32_t bi[2 * Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int i = t + 256 * q;
+    const float2 b = i < nr * NC ? bp[i] : make_float2(-1e30f, -1e30f);
+    const int2 a = i < nr * NC ? ip[i] : make_int2(0, 0);
+    bv[2 * q] = b.x;
+    bv[2 * q + 1] = b.y;
+    bi[2 * q] = a.x;
+    bi[2 * q + 1] = a.y;
+  }
+  for (int o = 0; o < nb; ++o) {
+    const float sc = scale[o];
+    __syncthreads();  // the previous orientation's reads of buf are done
+    long_row_c2r_core<L1, L2, RWG>(buf, sink, G + (size_t)o * H + (size_t)row0 * NH, nr, tw);
+    const int32_t oi = angle0 + o;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = t + 256 * q;
+      if (i < nr * NC) {
+        const float2 z = buf[(i / NC) * LR::PITCH + LR::zpos(i % NC)];
+        const float c0 = z.x * sc, c1 = z.y * sc;
+        if (c0 > bv[2 * q]) {
+          bv[2 * q] = c0;
+          bi[2 * q] = oi;
+        }
+        if (c1 > bv[2 * q + 1]) {
+          bv[2 * q + 1] = c1;
+          bi[2 * q + 1] = oi;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int i = t + 256 * q;
+    if (i < nr * NC) {
+      bp[i] = make_float2(bv[2 * q], bv[2 * q + 1]);
+      ip[i] = make_int2(bi[2 * q], bi[2 * q + 1]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -567,6 +639,42 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned b
   const long long rows = (long long)d.y * d.z * batch;
   if (d.x > 256) return launch_rows_long<false>(d.x, work, out, rows, scale, tw, s);
   return launch_rows_c2r(d.x, work, out, rows, scale, tw, s);
+}
+
+
+// The search's row pass above 256 (k_pick_rows_max_long): nrows rows per
+// spectrum, nb spectra H apart.
+int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, int angle0, const float* scale,
+                       float* best, int32_t* idx, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  auto go = [&](auto kernel, int rwg) -> int {
+    hipLaunchKernelGGL(kernel, dim3((unsigned)((nrows + rwg - 1) / rwg)), dim3(256), 0, s, G, nb, H, nrows, angle0,
+                       scale, best, idx, tw);
+    GX_LAUNCHED("k_pick_rows_max_long");
+    return GTOMX_OK;
+  };
+  switch (n / 2) {
+#define GX_LONG(NC, A, B, W) \
+  case NC: return go(k_pick_rows_max_long<A, B, W>, W);
+    GX_LONG(256, 16, 16, 8)
+    GX_LONG(384, 16, 24, 8)
+    GX_LONG(512, 16, 32, 4)
+    GX_LONG(768, 32, 24, 2)
+    GX_LONG(1024, 32, 32, 2)
+    GX_LONG(1536, 48, 32, 1)
+    GX_LONG(2048, 64, 32, 1)
+#undef GX_LONG
+    default: return fail(GTOMX_ERR_UNSUPPORTED, "row length not supported by the search's long-row pass");
+  }
+}
+
+int fft_c2r_cols_inplace(float2* data, gtomx_int3 d, unsigned batch, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  StreamScratch scr;
+  if (d.y > 256 || d.z > 256) GX_TRY(scr.take(elements_ft(d) * batch, s));
+  return yz_axes(data, scr.p, d, batch, 1, tw, s);
 }
 
 }  // namespace gx

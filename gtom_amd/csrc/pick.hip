@@ -644,9 +644,9 @@ __global__ __launch_bounds__(256) void k_pick_scale(const float* __restrict__ pa
 // generic transforms: pass 1 forms E * conj(CTF * R) for every bin of the
 // half spectrum (DC zeroed) with per-workgroup sums of |R|^2, the unit-RMS
 // scale comes from k_pick2d_herm_norm and k_pick_scale as in the fused form,
-// the batch goes through the generic C2R, and k_pick2d_update_any folds the
-// scaled maps into best / index in orientation order (strict >, first index
-// on ties).
+// the batch goes through the generic column inverse, and the row inverse of
+// every orientation is folded into best / index in orientation order
+// (strict >, first index on ties) by one row kernel (pick_rows_max_long).
 // ---------------------------------------------------------------------------
 template <class PJ>
 __global__ __launch_bounds__(256) void k_pick2d_project_any(PJ P, const float* __restrict__ mats,
@@ -673,23 +673,6 @@ __global__ __launch_bounds__(256) void k_pick2d_project_any(PJ P, const float* _
   if (threadIdx.x == 0) partial[(size_t)o * gridDim.x + blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_pick2d_update_any(const float* __restrict__ maps, int nb, size_t V,
-                                                           const float* __restrict__ scale, int angle0,
-                                                           float* __restrict__ best, int32_t* __restrict__ bidx) {
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < V; i += (size_t)gridDim.x * 256) {
-    float b = best[i];
-    int32_t k = bidx[i];
-    for (int o = 0; o < nb; ++o) {
-      const float c = maps[(size_t)o * V + i] * scale[o];
-      if (c > b) {
-        b = c;
-        k = angle0 + o;
-      }
-    }
-    best[i] = b;
-    bidx[i] = k;
-  }
-}
 
 }  // namespace gx
 
@@ -722,7 +705,6 @@ struct gtomx_pick_plan {
   float* gbest = nullptr;   // 2D: per-orientation-group maxima (groups x V)
   int32_t* gidx = nullptr;
   int og = 0;               // 2D: orientations per group
-  float* maps = nullptr;    // 2D above 256: B real correlation maps
   int nparts = 0;           // 2D above 256: pass-1 workgroups per orientation
   size_t device_bytes = 0;
   // kernel timing
@@ -751,7 +733,6 @@ void plan_release(gtomx_pick_plan* pl) {
   dev_free(pl->cproj);
   dev_free(pl->gbest);
   dev_free(pl->gidx);
-  dev_free(pl->maps);
   if (pl->h_mats) (void)hipHostFree(pl->h_mats);
   if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
   for (hipEvent_t e : pl->ev_pool) (void)hipEventDestroy(e);
@@ -928,7 +909,7 @@ int run_batch2d(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const flo
 }
 
 // The 2D branch above 256: pass 1 over every bin, the same scale, the generic
-// C2R into the plan's map buffer and the in-order update.
+// column inverse and the fused row inverse + in-order update.
 int herm_norm_any(int n, const Projector& P, const float* mats, const float* ctf, float* corr, int nb,
                   hipStream_t s) {
   switch (n) {
@@ -961,12 +942,8 @@ int run_batch2d_any(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const
     return GTOMX_OK;
   }));
   return timed(pl, 2, s, [&]() -> int {
-    GX_TRY(fft_c2r(pl->G, pl->G, pl->maps, gtomx_int3{n, n, 1}, (unsigned)nb, 1.0f, s));
-    const unsigned grid = (unsigned)std::min<size_t>((pl->V + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_pick2d_update_any, dim3(grid), dim3(256), 0, s, pl->maps, nb, pl->V, pl->scale, angle0,
-                       best, idx);
-    GX_LAUNCHED("k_pick2d_update_any");
-    return GTOMX_OK;
+    GX_TRY(fft_c2r_cols_inplace(pl->G, gtomx_int3{n, n, 1}, (unsigned)nb, s));
+    return pick_rows_max_long(n, pl->G, nb, (long long)pl->H, n, angle0, pl->scale, best, idx, s);
   });
 }
 
@@ -1102,11 +1079,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   d = nullptr;
   alloc(&d, (size_t)centred_bytes(pl->p));
   pl->cproj = static_cast<float2*>(d);
-  if (big2d) {
-    d = nullptr;
-    alloc(&d, (size_t)pl->B * pl->V * sizeof(float));
-    pl->maps = static_cast<float*>(d);
-  } else if (is2d) {
+  if (is2d && !big2d) {
     const size_t groups = (pl->B + pl->og - 1) / pl->og;
     d = nullptr;
     alloc(&d, groups * pl->V * sizeof(float));

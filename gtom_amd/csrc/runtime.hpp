@@ -100,6 +100,13 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 dims, unsigne
             float scale, hipStream_t s);
 // Axes above 256 go through two passes with a scratch copy of the data, taken
 // from the stream's memory pool (hipMallocAsync) and returned on the stream.
+// The inverse along z then y only, in place (the rows stay half spectra).
+int fft_c2r_cols_inplace(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
+// For n x n images with n > 256: the row C2R of nb spectra (H apart in G, in
+// place after fft_c2r_cols_inplace), times scale[o], folded into best / idx in
+// orientation order from angle0 (strict >).
+int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, int angle0, const float* scale,
+                       float* best, int32_t* idx, hipStream_t s);
 
 // Fourier-space resizing (resize.hip): op is one of the GTOMX_FFT_* values.
 constexpr int FFT_CROP = GTOMX_FFT_CROP, FFT_PAD = GTOMX_FFT_PAD, FFT_FULL_CROP = GTOMX_FFT_FULL_CROP,
