@@ -17,9 +17,10 @@
  *    gtomx_last_error() gives a human-readable message for the calling thread.
  *  - Supported transform sizes: every axis 2^a in [8, 4096] or 3*2^a in
  *    [12, 3072] (an axis of extent 1 is allowed for 2D and 1D data). The
- *    fused search (gtomx_pick_*) takes sides up to 256; the generic
- *    transforms and the pickers built on them take the full range, axes above
- *    256 in two passes with stream-ordered scratch (hipMallocAsync).
+ *    search (gtomx_pick_*) takes volume sides up to 256 and image sides up
+ *    to 4096 (above 256 through the generic transforms); the transforms and
+ *    the pickers built on them take the full range, strided axes above 256 in
+ *    two passes with stream-ordered scratch (hipMallocAsync).
  *
  * The reference interface each function replaces is cited as file:line of
  * the reference tree (dtegunov/gtom).
