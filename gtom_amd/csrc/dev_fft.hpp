@@ -278,6 +278,21 @@ __device__ __forceinline__ void fft_two_step(float2 (&v)[R1], int j, const float
   }
 }
 
+// exp(DIR 2 pi i e / L) from the twiddle table, L = 2^a <= 4096 or 3 * 2^a <= 3072.
+template <int DIR> __device__ __forceinline__ float2 twiddle_rt(const float2* __restrict__ tw, int L, int e) {
+  const float2 w = (L & (L - 1)) == 0 ? tw[(e & (L - 1)) * (TW_LEN / L)] : tw[TW3_OFF + (e % L) * (TW3_LEN / L)];
+  return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+// Register split of the passes' line lengths: the column shapes, except 64 =
+// 16 x 4, which lets a 256-thread workgroup span 64 lines (512-byte segments).
+template <int N> struct LineShape {
+  static constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2;
+};
+template <> struct LineShape<64> {
+  static constexpr int R1 = 16, R2 = 4;
+};
+
 // Workgroup sum of one float per thread (all threads must call). Returns the
 // total in thread 0. red must hold one float per wave.
 __device__ __forceinline__ float block_sum(float v, float* red) {

@@ -167,21 +167,6 @@ struct LineMap {
   float scale;
 };
 
-// exp(DIR 2 pi i e / L) from the twiddle table, L = 2^a <= 4096 or 3 * 2^a <= 3072.
-template <int DIR> __device__ __forceinline__ float2 twiddle_rt(const float2* __restrict__ tw, int L, int e) {
-  const float2 w = (L & (L - 1)) == 0 ? tw[(e & (L - 1)) * (TW_LEN / L)] : tw[TW3_OFF + (e % L) * (TW3_LEN / L)];
-  return DIR > 0 ? make_float2(w.x, -w.y) : w;
-}
-
-// Register split of the passes' line lengths: the column shapes, except 64 =
-// 16 x 4, which lets a 256-thread workgroup span 64 lines (512-byte segments).
-template <int N> struct LineShape {
-  static constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2;
-};
-template <> struct LineShape<64> {
-  static constexpr int R1 = 16, R2 = 4;
-};
-
 template <int N, int DIR, int TX>
 __global__ __launch_bounds__(TX * LineShape<N>::R2) void k_lines(const float2* __restrict__ src,
                                                                  float2* __restrict__ dst, LineMap p,
@@ -667,6 +652,24 @@ int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, i
 #undef GX_LONG
     default: return fail(GTOMX_ERR_UNSUPPORTED, "row length not supported by the search's long-row pass");
   }
+}
+
+int fft_len_split(int L, int* L1, int* L2) {
+  if (L <= 256 || !split_len(L, L1, L2)) return fail(GTOMX_ERR_UNSUPPORTED, "length has no two-pass split");
+  return GTOMX_OK;
+}
+
+int image_cols_pass2_inverse(int n, const float2* scratch, float2* out, unsigned nb, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  int L1, L2;
+  GX_TRY(fft_len_split(n, &L1, &L2));
+  const long long nh = n / 2 + 1, as = nh;
+  LineMap b{};
+  b.P1 = 1LL << 62, b.S0s = b.S0d = nh * n, b.Cs = b.Cd = 1;
+  b.G = L1, b.Gs = L2 * as, b.Gd = as, b.Ms = as, b.Md = L1 * as, b.nplanes = (long long)nb * L1;
+  b.width = (int)nh, b.L = 0, b.scale = 1.f;
+  return launch_lines<1>(L2, scratch, out, b, tw, s);
 }
 
 int fft_c2r_cols_inplace(float2* data, gtomx_int3 d, unsigned batch, hipStream_t s) {

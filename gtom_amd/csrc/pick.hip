@@ -648,29 +648,53 @@ __global__ __launch_bounds__(256) void k_pick_scale(const float* __restrict__ pa
 // every orientation is folded into best / index in orientation order
 // (strict >, first index on ties) by one row kernel (pick_rows_max_long).
 // ---------------------------------------------------------------------------
-template <class PJ>
-__global__ __launch_bounds__(256) void k_pick2d_project_any(PJ P, const float* __restrict__ mats,
-                                                            const float2* __restrict__ E,
-                                                            const float* __restrict__ ctf, float2* __restrict__ G,
-                                                            float* __restrict__ partial, int n) {
-  __shared__ float red[4];
+// Pass 1 above 256, with n = L1 * L2: the projection fused into the first
+// pass of the inverse y transform. Workgroup (column tile, (o, n2)) projects
+// its TX columns at the rows iy = n2 + L2 n1 (E * conj(CTF R), DC zeroed),
+// transforms over n1, multiplies by W_n^{n2 k1} and stores at row n2 + L2 k1
+// of the scratch spectra (image_cols_pass2_inverse finishes the transform);
+// one sum of |R|^2 per workgroup.
+template <class PJ, int L1, int TX>
+__global__ __launch_bounds__(TX * LineShape<L1>::R2) void k_pick2d_project_cols(
+    PJ P, const float* __restrict__ mats, const float2* __restrict__ E, const float* __restrict__ ctf,
+    float2* __restrict__ S, float* __restrict__ partial, int n, int L2, const float2* __restrict__ tw) {
+  constexpr int R1 = LineShape<L1>::R1, R2 = LineShape<L1>::R2, NT = TX * R2;
+  __shared__ float2 xch[R2 > 1 ? TX * L1 : 1];
+  __shared__ float red[(NT + 63) / 64];
   const int NH = n / 2 + 1, HALF = n / 2;
-  const int H = NH * n;
-  const int o = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = i < H;
-  const int ix = valid ? i % NH : 0, iy = valid ? i / NH : 0, y = signed_freq(iy, n);
+  const int t = threadIdx.x, c = t % TX, j = t / TX;
+  const int o = blockIdx.y / L2, n2 = blockIdx.y % L2;
+  const int ix = blockIdx.x * TX + c;
+  const bool valid = ix < NH;
   float M[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
   const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
-  const bool ins = valid && ix * ix + y * y <= HALF * HALF;
-  const float2 e = valid ? E[i] : make_float2(0.f, 0.f);
-  const float cf = valid ? (ctf ? ctf[i] : 1.f) : 0.f;
-  const float2 r = cscale(sample(rs, P, M[0] * ix + M[1] * y, M[3] * ix + M[4] * y, M[6] * ix + M[7] * y, ins), cf);
-  const float2 v = i == 0 ? make_float2(0.f, 0.f) : cmulc(e, r);
-  if (valid) G[(size_t)o * H + i] = v;
-  const float tot = block_sum(r.x * r.x + r.y * r.y, red);
-  if (threadIdx.x == 0) partial[(size_t)o * gridDim.x + blockIdx.x] = tot;
+  float acc = 0.f;
+  float2 v[R1];
+#pragma unroll
+  for (int m = 0; m < R1; ++m) {
+    const int iy = n2 + L2 * (j + R2 * m), y = signed_freq(iy, n);
+    const bool ins = valid && ix * ix + y * y <= HALF * HALF;
+    const size_t b = (size_t)iy * NH + ix;
+    const float2 e = valid ? E[b] : make_float2(0.f, 0.f);
+    const float cf = valid ? (ctf ? ctf[b] : 1.f) : 0.f;
+    const float2 r =
+        cscale(sample(rs, P, M[0] * ix + M[1] * y, M[3] * ix + M[4] * y, M[6] * ix + M[7] * y, ins), cf);
+    acc += r.x * r.x + r.y * r.y;
+    v[m] = ix == 0 && iy == 0 ? make_float2(0.f, 0.f) : cmulc(e, r);
+  }
+  fft_two_step<L1, R1, R2, 1>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
+  if (valid) {
+    float2* So = S + (size_t)o * NH * n;
+#pragma unroll
+    for (int s = 0; s < R1; ++s) {
+      const int k1 = out_index<R1, R2>(j, s);
+      So[(size_t)(n2 + L2 * k1) * NH + ix] = cmul(v[s], twiddle_rt<1>(tw, n, n2 * k1));
+    }
+  }
+  const float tot = block_sum(acc, red);
+  if (t == 0) partial[((size_t)o * gridDim.x + blockIdx.x) * L2 + n2] = tot;
 }
 
 
@@ -706,6 +730,7 @@ struct gtomx_pick_plan {
   int32_t* gidx = nullptr;
   int og = 0;               // 2D: orientations per group
   int nparts = 0;           // 2D above 256: pass-1 workgroups per orientation
+  float2* scr = nullptr;    // 2D above 256: pass-1 output (B spectra)
   size_t device_bytes = 0;
   // kernel timing
   bool timing = false;
@@ -733,6 +758,7 @@ void plan_release(gtomx_pick_plan* pl) {
   dev_free(pl->cproj);
   dev_free(pl->gbest);
   dev_free(pl->gidx);
+  dev_free(pl->scr);
   if (pl->h_mats) (void)hipHostFree(pl->h_mats);
   if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
   for (hipEvent_t e : pl->ev_pool) (void)hipEventDestroy(e);
@@ -908,8 +934,9 @@ int run_batch2d(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const flo
   return GTOMX_OK;
 }
 
-// The 2D branch above 256: pass 1 over every bin, the same scale, the generic
-// column inverse and the fused row inverse + in-order update.
+// The 2D branch above 256: pass 1 fused with the first half of the y
+// inverse, the same scale, the second half, then the fused row inverse +
+// in-order update.
 int herm_norm_any(int n, const Projector& P, const float* mats, const float* ctf, float* corr, int nb,
                   hipStream_t s) {
   switch (n) {
@@ -926,23 +953,36 @@ int herm_norm_any(int n, const Projector& P, const float* mats, const float* ctf
 
 template <class PJ>
 int run_batch2d_any(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float* mats, int nb,
-                    const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, hipStream_t s) {
+                    const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
+                    hipStream_t s) {
   const int n = pl->n;
+  int L1, L2;
+  GX_TRY(fft_len_split(n, &L1, &L2));
+  const int tiles = (n / 2 + 1 + 63) / 64;
   GX_TRY(timed(pl, 0, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick2d_project_any<PJ>, dim3(pl->nparts, nb), dim3(256), 0, s, PS, mats, E, ctf, pl->G,
-                       pl->partial, n);
-    GX_LAUNCHED("k_pick2d_project_any");
-    return GTOMX_OK;
+    auto go = [&](auto kernel, int threads) -> int {
+      hipLaunchKernelGGL(kernel, dim3(tiles, nb * L2), dim3(threads), 0, s, PS, mats, E, ctf, pl->scr, pl->partial, n,
+                         L2, tw);
+      GX_LAUNCHED("k_pick2d_project_cols");
+      return GTOMX_OK;
+    };
+    switch (L1) {
+      case 16: return go(k_pick2d_project_cols<PJ, 16, 64>, 64 * LineShape<16>::R2);
+      case 32: return go(k_pick2d_project_cols<PJ, 32, 64>, 64 * LineShape<32>::R2);
+      case 48: return go(k_pick2d_project_cols<PJ, 48, 64>, 64 * LineShape<48>::R2);
+      case 64: return go(k_pick2d_project_cols<PJ, 64, 64>, 64 * LineShape<64>::R2);
+      default: return fail(GTOMX_ERR_INTERNAL, "image split not instantiated");
+    }
   }));
   GX_TRY(herm_norm_any(n, P, mats, ctf, pl->corr, nb, s));
   GX_TRY(timed(pl, 1, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick_scale, dim3(nb), dim3(256), 0, s, pl->partial, pl->nparts, pl->corr, 1, (double)pl->H,
+    hipLaunchKernelGGL(k_pick_scale, dim3(nb), dim3(256), 0, s, pl->partial, tiles * L2, pl->corr, 1, (double)pl->H,
                        pl->scale);
     GX_LAUNCHED("k_pick_scale");
     return GTOMX_OK;
   }));
   return timed(pl, 2, s, [&]() -> int {
-    GX_TRY(fft_c2r_cols_inplace(pl->G, gtomx_int3{n, n, 1}, (unsigned)nb, s));
+    GX_TRY(image_cols_pass2_inverse(n, pl->scr, pl->G, (unsigned)nb, s));
     return pick_rows_max_long(n, pl->G, nb, (long long)pl->H, n, angle0, pl->scale, best, idx, s);
   });
 }
@@ -951,8 +991,8 @@ int run_batch2d_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projecto
                   const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
                   hipStream_t s) {
   if (pl->n > 256)
-    return C ? run_batch2d_any(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, s)
-             : run_batch2d_any(pl, P, P, mats, nb, E, ctf, best, idx, angle0, s);
+    return C ? run_batch2d_any(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, tw, s)
+             : run_batch2d_any(pl, P, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
   switch (pl->n) {
 #define GX_CASE(N)                                                                         \
   case N:                                                                                  \
@@ -1040,7 +1080,12 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   const unsigned big_b = (unsigned)std::max<size_t>(1, std::min<size_t>(240, (1ull << 30) / (pl->H * sizeof(float2))));
   pl->B = batch ? batch : (is2d ? (big2d ? big_b : 240u) : (dv.x >= 128 ? 32u : 16u));
   pl->og = dv.x >= 128 ? 2 : 1;
-  pl->nparts = big2d ? (int)((pl->H + 255) / 256) : 0;
+  int L1 = 0, L2 = 0;
+  if (big2d && fft_len_split(dv.x, &L1, &L2) != GTOMX_OK) {
+    delete pl;
+    return fail(GTOMX_ERR_UNSUPPORTED, "image side has no two-pass split");
+  }
+  pl->nparts = big2d ? (dv.x / 2 + 1 + 63) / 64 * L2 : 0;
   const std::vector<int> tiles = is2d ? std::vector<int>{0} : tiles_for(pl->n);
   pl->ntiles = big2d ? 1 : (is2d ? g2_tiles_for(pl->n) : (int)tiles.size());
   int s = GTOMX_OK;
@@ -1060,7 +1105,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   alloc(&d, std::max((size_t)pl->B * gstride, pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
-  // up to 4 pass-1 waves per tile; above 256, one sum per 256 bins
+  // up to 4 pass-1 waves per tile; above 256, one sum per pass-1 workgroup
   alloc(&d, (size_t)pl->B * (big2d ? (size_t)pl->nparts : (size_t)pl->ntiles * 4) * sizeof(float));
   pl->partial = static_cast<float*>(d);
   d = nullptr;
@@ -1079,7 +1124,11 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   d = nullptr;
   alloc(&d, (size_t)centred_bytes(pl->p));
   pl->cproj = static_cast<float2*>(d);
-  if (is2d && !big2d) {
+  if (big2d) {
+    d = nullptr;
+    alloc(&d, (size_t)pl->B * pl->H * sizeof(float2));
+    pl->scr = static_cast<float2*>(d);
+  } else if (is2d) {
     const size_t groups = (pl->B + pl->og - 1) / pl->og;
     d = nullptr;
     alloc(&d, groups * pl->V * sizeof(float));

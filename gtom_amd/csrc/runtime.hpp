@@ -100,6 +100,12 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 dims, unsigne
             float scale, hipStream_t s);
 // Axes above 256 go through two passes with a scratch copy of the data, taken
 // from the stream's memory pool (hipMallocAsync) and returned on the stream.
+// L = L1 * L2 as the two-pass transforms split it (L above 256).
+int fft_len_split(int L, int* L1, int* L2);
+// Pass 2 of the inverse y transform of nb n x n half spectra (n above 256),
+// from the pass-1 layout in `scratch` (the column position n2 + L2 k1 holds
+// pass 1's output k1 of line n2, twiddled) into natural order in `out`.
+int image_cols_pass2_inverse(int n, const float2* scratch, float2* out, unsigned nb, hipStream_t s);
 // The inverse along z then y only, in place (the rows stay half spectra).
 int fft_c2r_cols_inplace(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
 // For n x n images with n > 256: the row C2R of nb spectra (H apart in G, in
