@@ -265,16 +265,30 @@ __device__ __forceinline__ void long_row_passes(float2* buf, float2* sink, const
   __syncthreads();
 }
 
+// First row of a workgroup: rows in order, or with band > 0 only the rows
+// [0, band) and [per - band, per) of every image of `per` rows (band a
+// multiple of RWG, so a workgroup's rows stay contiguous).
+template <int RWG>
+__device__ __forceinline__ long long long_row0(long long nrows, int per, int band, int* nr) {
+  const long long v = (long long)blockIdx.x * RWG;
+  *nr = (int)(nrows - v < RWG ? nrows - v : RWG);
+  if (band == 0) return v;
+  const long long b = v / (2 * band);
+  const int r = (int)(v % (2 * band));
+  return b * per + (r < band ? r : per - 2 * band + r);
+}
+
 template <int L1, int L2, int RWG>
 __global__ __launch_bounds__(256) void k_rows_r2c_long(const float* __restrict__ in, float2* __restrict__ out,
-                                                       long long nrows, const float2* __restrict__ tw) {
+                                                       long long nrows, int per, int band,
+                                                       const float2* __restrict__ tw) {
   using LR = LongRow<L1, L2>;
   constexpr int NC = LR::NC, NH = LR::NH;
   __shared__ float2 buf[RWG * LR::PITCH];
   __shared__ float2 sink[1];  // exchange target of threads without a line
   const int t = threadIdx.x;
-  const long long row0 = (long long)blockIdx.x * RWG;
-  const int nr = (int)(nrows - row0 < RWG ? nrows - row0 : RWG);
+  int nr;
+  const long long row0 = long_row0<RWG>(nrows, per, band, &nr);
   const float2* src = reinterpret_cast<const float2*>(in) + row0 * NC;
   for (int i = t; i < nr * NC; i += 256) buf[(i / NC) * LR::PITCH + LR::phys(i % NC)] = src[i];
   __syncthreads();
@@ -324,15 +338,15 @@ __device__ __forceinline__ void long_row_c2r_core(float2* buf, float2* sink, con
 
 template <int L1, int L2, int RWG>
 __global__ __launch_bounds__(256) void k_rows_c2r_long(const float2* __restrict__ in, float* __restrict__ out,
-                                                       long long nrows, float scale,
+                                                       long long nrows, int per, int band, float scale,
                                                        const float2* __restrict__ tw) {
   using LR = LongRow<L1, L2>;
   constexpr int NC = LR::NC, NH = LR::NH;
   __shared__ float2 buf[RWG * LR::PITCH];
   __shared__ float2 sink[1];  // exchange target of threads without a line
   const int t = threadIdx.x;
-  const long long row0 = (long long)blockIdx.x * RWG;
-  const int nr = (int)(nrows - row0 < RWG ? nrows - row0 : RWG);
+  int nr;
+  const long long row0 = long_row0<RWG>(nrows, per, band, &nr);
   long_row_c2r_core<L1, L2, RWG>(buf, sink, in + row0 * NH, nr, tw);
   float2* dst = reinterpret_cast<float2*>(out) + row0 * NC;
   for (int i = t; i < nr * NC; i += 256) dst[i] = cscale(buf[(i / NC) * LR::PITCH + LR::zpos(i % NC)], scale);
@@ -398,6 +412,16 @@ __global__ __launch_bounds__(256) void k_pick_rows_max_long(const float2* __rest
       ip[i] = make_int2(bi[2 * q], bi[2 * q + 1]);
     }
   }
+}
+
+// Zeroes rows [band, per - band) of every image (per rows of NH bins).
+__global__ __launch_bounds__(256) void k_zero_mid_rows(float2* __restrict__ data, int per, int band, long long NH,
+                                                       unsigned nb) {
+  const long long mid = (long long)(per - 2 * band) * NH;
+  const unsigned b = blockIdx.y;
+  float2* d = data + (size_t)b * per * NH + (size_t)band * NH;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < mid; i += (long long)gridDim.x * 256)
+    d[i] = make_float2(0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,18 +551,21 @@ int cols_axis(int L, float2* data, float2* scratch, int width, long long as, lon
 }
 
 // Long rows in one pass (NC = N/2 in [256, 2048]).
+// nrows rows in order, or (band > 0) the 2 band rows of nrows / (2 band)
+// images of `per` rows each.
 template <bool R2C>
 int launch_rows_long(int n, const void* in, void* out, long long nrows, float scale, const float2* tw,
-                     hipStream_t s) {
+                     hipStream_t s, int per = 0, int band = 0) {
   auto go = [&](auto r2c_kernel, auto c2r_kernel, int rwg) -> int {
+    if (band % rwg) return fail(GTOMX_ERR_INTERNAL, "row band not a multiple of the rows per workgroup");
     dim3 grid((unsigned)((nrows + rwg - 1) / rwg));
     if constexpr (R2C) {
       hipLaunchKernelGGL(r2c_kernel, grid, dim3(256), 0, s, static_cast<const float*>(in), static_cast<float2*>(out),
-                         nrows, tw);
+                         nrows, per, band, tw);
       GX_LAUNCHED("k_rows_r2c_long");
     } else {
       hipLaunchKernelGGL(c2r_kernel, grid, dim3(256), 0, s, static_cast<const float2*>(in), static_cast<float*>(out),
-                         nrows, scale, tw);
+                         nrows, per, band, scale, tw);
       GX_LAUNCHED("k_rows_c2r_long");
     }
     return GTOMX_OK;
@@ -670,6 +697,35 @@ int image_cols_pass2_inverse(int n, const float2* scratch, float2* out, unsigned
   b.G = L1, b.Gs = L2 * as, b.Gd = as, b.Ms = as, b.Md = L1 * as, b.nplanes = (long long)nb * L1;
   b.width = (int)nh, b.L = 0, b.scale = 1.f;
   return launch_lines<1>(L2, scratch, out, b, tw, s);
+}
+
+int image_c2r_banded(float2* work, float* out, int n, unsigned nb, int cols, int band, float scale, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  if (n <= 256 || band <= 0 || 2 * band > n || cols < 1 || cols > n / 2 + 1)
+    return fail(GTOMX_ERR_INTERNAL, "banded C2R outside its range");
+  const long long nh = n / 2 + 1;
+  StreamScratch scr;
+  GX_TRY(scr.take((size_t)nh * n * nb, s));
+  GX_TRY(cols_axis<1>(n, work, scr.p, cols, nh, (long long)nb, 1LL << 62, nh * n, 0, tw, s));
+  return launch_rows_long<false>(n, work, out, (long long)nb * 2 * band, scale, tw, s, n, band);
+}
+
+int image_r2c_banded(const float* in, float2* out, int n, unsigned nb, int band, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  if (n <= 256 || band <= 0 || 2 * band > n) return fail(GTOMX_ERR_INTERNAL, "banded R2C outside its range");
+  const long long nh = n / 2 + 1;
+  if (2 * band < n) {
+    const long long mid = (long long)(n - 2 * band) * nh;
+    hipLaunchKernelGGL(k_zero_mid_rows, dim3((unsigned)std::min<long long>((mid + 255) / 256, 2048), nb), dim3(256),
+                       0, s, out, n, band, nh, nb);
+    GX_LAUNCHED("k_zero_mid_rows");
+  }
+  GX_TRY(launch_rows_long<true>(n, in, out, (long long)nb * 2 * band, 1.f, tw, s, n, band));
+  StreamScratch scr;
+  GX_TRY(scr.take((size_t)nh * n * nb, s));
+  return cols_axis<-1>(n, out, scr.p, (int)nh, nh, (long long)nb, 1LL << 62, nh * n, 0, tw, s);
 }
 
 int fft_c2r_cols_inplace(float2* data, gtomx_int3 d, unsigned batch, hipStream_t s) {

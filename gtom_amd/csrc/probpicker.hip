@@ -610,6 +610,13 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
     prob_release(pp);
     return st;
   }
+  // The banded transforms of micrographs leave rows of `real` outside the
+  // mask's band as they were, to be multiplied by the mask's zeros: they must
+  // hold finite values from the start.
+  if (hipMemsetAsync(pp->real, 0, (size_t)B * Vi * sizeof(float), s) != hipSuccess) {
+    prob_release(pp);
+    return fail(GTOMX_ERR_DEVICE, "clearing the batch buffer");
+  }
 
   // Reference spectra and masks, all references at once. Workspace: the padded
   // references R (nrefs x Vi) in `real`-sized chunks would not fit every case, so
@@ -800,7 +807,17 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
       GX_LAUNCHED("k_times_real");
     }
     GX_TRY(fft_resize(FFT_PAD, true, pp->rot, pp->padft, pp->dl, pp->di, nb, s));
-    GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f / (float)Vi, s));
+    // Square micrographs with a circular mask: only the padded reference's
+    // low-pass columns are nonzero, and only the rows of the mask's band
+    // survive the mask, so the transforms skip the rest (the same values
+    // where they are used).
+    const int n2d = pp->di.x;
+    const bool banded = pp->ndims == 2 && pp->circular && n2d > 256 && pp->di.y == n2d;
+    const int band = std::min(n2d / 2, (pp->dr.y / 2 + 1 + 7) / 8 * 8);
+    if (banded)
+      GX_TRY(image_c2r_banded(pp->padft, pp->real, n2d, nb, pp->dl.x / 2 + 1, band, 1.0f / (float)Vi, s));
+    else
+      GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f / (float)Vi, s));
     // the mask of each orientation: the same one (circular), or the B-spline
     // coefficients rotated into FFT order (d_Rotate3D / d_Rotate2D, T_INTERP_CUBIC, :273-279)
     const float* mk = mask;
@@ -823,7 +840,10 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     hipLaunchKernelGGL(k_prob_coef, dim3((nb + 63) / 64), dim3(64), 0, s, pp->part, nparts, nb, Vi, Er, pp->coef);
     GX_LAUNCHED("k_prob_coef");
     // correlation with the image
-    GX_TRY(fft_r2c(pp->real, pp->padft, pp->di, nb, s));
+    if (banded)
+      GX_TRY(image_r2c_banded(pp->real, pp->padft, n2d, nb, band, s));
+    else
+      GX_TRY(fft_r2c(pp->real, pp->padft, pp->di, nb, s));
     GX_TRY(image_times_conj(pp->imageft, pp->padft, Hi, nb, s));
     GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f, s));
     if (pp->circular && a0 == 0) {

@@ -106,6 +106,15 @@ int fft_len_split(int L, int* L1, int* L2);
 // from the pass-1 layout in `scratch` (the column position n2 + L2 k1 holds
 // pass 1's output k1 of line n2, twiddled) into natural order in `out`.
 int image_cols_pass2_inverse(int n, const float2* scratch, float2* out, unsigned nb, hipStream_t s);
+// Transforms of nb n x n images (n above 256) whose data is known to matter
+// only in part, with results equal to the full transforms where they are used:
+// C2R with the y inverse over the first `cols` columns only (the rest must be
+// zero) and the row inverse only over the rows [0, band) and [n - band, n)
+// (other rows of `out` keep their contents); R2C of input that is zero
+// outside those rows (rows of `out` between them are zeroed). band is a
+// multiple of 8 and at most n / 2.
+int image_c2r_banded(float2* work, float* out, int n, unsigned nb, int cols, int band, float scale, hipStream_t s);
+int image_r2c_banded(const float* in, float2* out, int n, unsigned nb, int band, hipStream_t s);
 // The inverse along z then y only, in place (the rows stay half spectra).
 int fft_c2r_cols_inplace(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
 // For n x n images with n > 256: the row C2R of nb spectra (H apart in G, in
