@@ -43,7 +43,7 @@ def _images(orc, gx, cuda, n, box, nimg, angles, seed):
     return projft, P, np.stack(imgs).astype(np.float32)
 
 
-def _run(gx, orc, cuda, n, box, nimg, nang, batch, seed, ctf_per_image=False):
+def _run(gx, orc, cuda, n, box, nimg, nang, batch, seed, ctf_per_image=False, oversample=None):
     import torch
     angles = orc.equal_angular_spacing((0, TAU), (0, np.pi), (0, TAU), np.deg2rad(30.0))[:nang]
     projft, P, imgs = _images(orc, gx, cuda, n, box, nimg, angles, seed)
@@ -54,11 +54,12 @@ def _run(gx, orc, cuda, n, box, nimg, nang, batch, seed, ctf_per_image=False):
     ctf = gx.ctf_3d(params, n, device=cuda, ndims=2)
     stride = ctf[0].numel() if ctf_per_image else 0
     plan = gx.PickPlan(p, n, batch, ndims=2)
-    best, idx = gx.d_PickSubTomograms(plan, projft, p / n, E, ctf, angles, n / 8, ctf_volume_stride=stride)
+    ov = p / n if oversample is None else oversample
+    best, idx = gx.d_PickSubTomograms(plan, projft, ov, E, ctf, angles, n / 8, ctf_volume_stride=stride)
     torch.cuda.synchronize()
     plan.close()
     C = to_np(ctf).astype(np.float64)
-    ref = orc.pick(P, p, p / n, to_np(E).astype(np.complex128), C if ctf_per_image else C[0], n,
+    ref = orc.pick(P, p, ov, to_np(E).astype(np.complex128), C if ctf_per_image else C[0], n,
                    angles.astype(np.float64), n / 8, ctf_per_volume=ctf_per_image, ndims=2)
     return to_np(best), to_np(idx), ref
 
@@ -94,6 +95,13 @@ def test_pick2d_micrograph_sizes_match_oracle(gx, orc, cuda, n, nang, ctf_per_im
     maps folded into best / index in orientation order)."""
     best, idx, ref = _run(gx, orc, cuda, n, 32, nimg=2 if ctf_per_image else 1, nang=nang, batch=0, seed=n,
                           ctf_per_image=ctf_per_image)
+    _check(best, idx, ref)
+
+
+def test_pick2d_micrograph_uncentred_projector_matches_oracle(gx, orc, cuda):
+    """An oversample above p/n samples the projector beyond its centred copy
+    (the plain projector layout) in the fused projection pass."""
+    best, idx, ref = _run(gx, orc, cuda, 512, 32, nimg=1, nang=40, batch=0, seed=9, oversample=0.25)
     _check(best, idx, ref)
 
 
