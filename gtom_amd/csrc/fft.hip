@@ -604,15 +604,19 @@ struct StreamScratch {
   }
 };
 
-int yz_axes(float2* data, float2* scr, gtomx_int3 d, unsigned batch, int dir, const float2* tw, hipStream_t s) {
+// The y and z transforms of half spectra over their first `cols` columns
+// (cols = 0: all n/2 + 1).
+int yz_axes(float2* data, float2* scr, gtomx_int3 d, unsigned batch, int dir, const float2* tw, hipStream_t s,
+            int cols = 0) {
   const long long nh = d.x / 2 + 1;
+  const int w = cols > 0 ? cols : (int)nh;
   auto zax = [&](auto DIRC) {
-    return cols_axis<decltype(DIRC)::value>(d.z, data, scr, (int)nh, nh * d.y, (long long)d.y * batch, d.y, nh,
+    return cols_axis<decltype(DIRC)::value>(d.z, data, scr, w, nh * d.y, (long long)d.y * batch, d.y, nh,
                                             nh * d.y * d.z, tw, s);
   };
   auto yax = [&](auto DIRC) {
-    return cols_axis<decltype(DIRC)::value>(d.y, data, scr, (int)nh, nh, (long long)d.z * batch, 1LL << 62,
-                                            nh * d.y, 0, tw, s);
+    return cols_axis<decltype(DIRC)::value>(d.y, data, scr, w, nh, (long long)d.z * batch, 1LL << 62, nh * d.y, 0,
+                                            tw, s);
   };
   using F = std::integral_constant<int, -1>;
   using B = std::integral_constant<int, 1>;
@@ -639,15 +643,16 @@ int fft_r2c(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStrea
   return yz_axes(out, scr.p, d, batch, -1, tw, s);
 }
 
-int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned batch, float scale,
-            hipStream_t s) {
+int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned batch, float scale, hipStream_t s,
+            int cols) {
   const float2* tw = nullptr;
   GX_TRY(twiddles(&tw));
   const size_t H = elements_ft(d) * batch;
+  if (cols < 0 || cols > d.x / 2 + 1) return fail(GTOMX_ERR_INTERNAL, "column count outside the half spectrum");
   if (work != in) GX_HIP(hipMemcpyAsync(work, in, H * sizeof(float2), hipMemcpyDeviceToDevice, s));
   StreamScratch scr;
   if (d.y > 256 || d.z > 256) GX_TRY(scr.take(H, s));
-  GX_TRY(yz_axes(work, scr.p, d, batch, 1, tw, s));
+  GX_TRY(yz_axes(work, scr.p, d, batch, 1, tw, s, cols));
   const long long rows = (long long)d.y * d.z * batch;
   if (d.x > 256) return launch_rows_long<false>(d.x, work, out, rows, scale, tw, s);
   return launch_rows_c2r(d.x, work, out, rows, scale, tw, s);

@@ -816,8 +816,8 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     const int band = std::min(n2d / 2, (pp->dr.y / 2 + 1 + 7) / 8 * 8);
     if (banded)
       GX_TRY(image_c2r_banded(pp->padft, pp->real, n2d, nb, pp->dl.x / 2 + 1, band, 1.0f / (float)Vi, s));
-    else
-      GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f / (float)Vi, s));
+    else  // the padded spectrum is zero beyond the low-pass half width
+      GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f / (float)Vi, s, pp->dl.x / 2 + 1));
     // the mask of each orientation: the same one (circular), or the B-spline
     // coefficients rotated into FFT order (d_Rotate3D / d_Rotate2D, T_INTERP_CUBIC, :273-279)
     const float* mk = mask;

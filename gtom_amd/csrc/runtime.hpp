@@ -96,8 +96,10 @@ struct DevBuf {
 // elements_ft(dims) * batch complex values for c2r (the input is copied
 // there first so it is left unmodified); r2c needs no workspace.
 int fft_r2c(const float* in, float2* out, gtomx_int3 dims, unsigned batch, hipStream_t s);
+// cols > 0 declares the columns x >= cols of every half spectrum zero: the y
+// and z inverses skip them (the result is the same).
 int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 dims, unsigned batch,
-            float scale, hipStream_t s);
+            float scale, hipStream_t s, int cols = 0);
 // Axes above 256 go through two passes with a scratch copy of the data, taken
 // from the stream's memory pool (hipMallocAsync) and returned on the stream.
 // L = L1 * L2 as the two-pass transforms split it (L above 256).
