@@ -209,3 +209,23 @@ def test_micrograph_sizes_match_oracle(gx, orc, cuda, di):
     assert sel.mean() > 0.9
     assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
     assert np.all(to_np(r) == 0)
+
+
+def test_bench_size_3d_matches_oracle(gx, orc, cuda):
+    """The measured 3D configuration: a 128^3 volume, a masked 32^3 reference,
+    low-pass 24 (rotation at 48^3, the first C2R over its 25 nonzero columns),
+    here at anglestep 1.2 (50 orientations)."""
+    rng = np.random.default_rng(128)
+    di, dr, lp, step = (128, 128, 128), (32, 32, 32), 24, 1.2
+    mask = _disc(dr, 14.0)
+    refs = (rng.standard_normal((1,) + dr) * mask).astype(np.float32)
+    image = rng.standard_normal(di).astype(np.float32)
+    b, a, r, nang = _run_gpu(gx, cuda, refs, mask[None], image, lp, step, None, 0)
+    assert nang == 50
+    orc.gaussrand(0, reset=True)
+    ob, oa, _, top2 = orc.prob_picker(refs[0].astype(np.float64), mask.astype(np.float64), image.astype(np.float64),
+                                      lp, step, refid=0)
+    assert rel_linf(to_np(b).astype(np.float64), ob) <= 1e-4
+    sel = (ob - top2) > 1e-4 * np.abs(ob).max()
+    assert sel.mean() > 0.9
+    assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
