@@ -50,6 +50,25 @@ __global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
   }
 }
 
+// Row selection of the one-pass row kernels: by == 0 takes rows in order;
+// otherwise only rows y in [0, by) and [ny - by, ny) of each plane, and (bz >
+// 0) only planes z in [0, bz) and [nz - bz, nz) of each volume, counted in
+// that order. by must be a multiple of the rows per workgroup.
+struct RowBand {
+  int ny = 0, by = 0, nz = 1, bz = 0;
+};
+__device__ __forceinline__ long long band_row(long long v, const RowBand& b) {
+  if (b.by == 0) return v;
+  const long long py = 2LL * b.by, pz = b.bz ? 2LL * b.bz : b.nz;
+  const int yy = (int)(v % py);
+  const long long q = v / py;
+  const int zz = (int)(q % pz);
+  const long long img = q / pz;
+  const int y = yy < b.by ? yy : b.ny - (int)py + yy;
+  const int z = b.bz == 0 ? zz : (zz < b.bz ? zz : b.nz - (int)pz + zz);
+  return (img * b.nz + z) * b.ny + y;
+}
+
 // ---------------------------------------------------------------------------
 // Row C2R: each row holds N/2+1 complex bins and becomes N reals, scaled.
 // Z[k] = (X[k] + conj X[N/2-k]) + i w^k (X[k] - conj X[N/2-k]), w = e^{2 pi i/N},
@@ -61,15 +80,15 @@ __global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
 template <int N>
 __global__ __launch_bounds__(256) void k_rows_c2r(const float2* __restrict__ in, float* __restrict__ out,
                                                   long long nrows, float scale,
-                                                  const float2* __restrict__ tw) {
+                                                  const float2* __restrict__ tw, RowBand rb) {
   constexpr int NC = N / 2, NH = N / 2 + 1;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
   constexpr int GS = R2 * (R1 + 1);  // exchange floats2 per row (padded)
   constexpr int AREA = RW * (GS > NH ? GS : NH);
   __shared__ alignas(16) float2 xch[AREA];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
-  const long long row0 = (long long)blockIdx.x * RW;
-  const int nr = (int)(nrows - row0 < RW ? nrows - row0 : RW);
+  const long long v0 = (long long)blockIdx.x * RW, row0 = band_row(v0, rb);
+  const int nr = (int)(nrows - v0 < RW ? nrows - v0 : RW);
   const float2* src = in + row0 * NH;
   for (int i = t; i < nr * NH; i += 256) xch[i] = src[i];
   __syncthreads();
@@ -110,7 +129,7 @@ __global__ __launch_bounds__(256) void k_rows_c2r(const float2* __restrict__ in,
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(256) void k_rows_r2c(const float* __restrict__ in, float2* __restrict__ out,
-                                                  long long nrows, const float2* __restrict__ tw) {
+                                                  long long nrows, const float2* __restrict__ tw, RowBand rb) {
   constexpr int NC = N / 2, NH = N / 2 + 1;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
   constexpr int GS = R2 * (R1 + 1);
@@ -118,8 +137,8 @@ __global__ __launch_bounds__(256) void k_rows_r2c(const float* __restrict__ in, 
   __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
   __shared__ float2 zrow[RW * ZS];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
-  const long long row = (long long)blockIdx.x * RW + r;
-  const bool valid = row < nrows;
+  const bool valid = (long long)blockIdx.x * RW + r < nrows;
+  const long long row = band_row((long long)blockIdx.x * RW, rb) + r;
   const float2* x = reinterpret_cast<const float2*>(in + row * N);
   float2 v[R1];
 #pragma unroll
@@ -424,6 +443,18 @@ __global__ __launch_bounds__(256) void k_zero_mid_rows(float2* __restrict__ data
     d[i] = make_float2(0.f, 0.f);
 }
 
+// Zeroes the rows of half spectra outside a RowBand (rows of NH bins, nrows
+// in all: every plane of every volume).
+__global__ __launch_bounds__(256) void k_zero_outside_band(float2* __restrict__ data, long long nrows, int NH,
+                                                           RowBand b) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nrows * NH; i += (long long)gridDim.x * 256) {
+    const long long row = i / NH;
+    const int y = (int)(row % b.ny), z = (int)((row / b.ny) % b.nz);
+    const bool in = (y < b.by || y >= b.ny - b.by) && (b.bz == 0 || z < b.bz || z >= b.nz - b.bz);
+    if (!in) data[i] = make_float2(0.f, 0.f);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host dispatch
 // ---------------------------------------------------------------------------
@@ -451,10 +482,11 @@ int launch_cols_n(int n, float2* data, int width, long long axis_stride, long lo
 template <int N> int rows_per_block() { return 256 / RowShape<N / 2>::R2; }
 
 int launch_rows_c2r(int n, const float2* in, float* out, long long nrows, float scale, const float2* tw,
-                    hipStream_t s) {
+                    hipStream_t s, RowBand rb = RowBand{}) {
   auto go = [&](auto kernel, int rw) -> int {
+    if (rb.by % rw) return fail(GTOMX_ERR_INTERNAL, "row band not a multiple of the rows per workgroup");
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, in, out, nrows, scale, tw);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, in, out, nrows, scale, tw, rb);
     GX_LAUNCHED("k_rows_c2r");
     return GTOMX_OK;
   };
@@ -468,10 +500,12 @@ int launch_rows_c2r(int n, const float2* in, float* out, long long nrows, float 
   }
 }
 
-int launch_rows_r2c(int n, const float* in, float2* out, long long nrows, const float2* tw, hipStream_t s) {
+int launch_rows_r2c(int n, const float* in, float2* out, long long nrows, const float2* tw, hipStream_t s,
+                    RowBand rb = RowBand{}) {
   auto go = [&](auto kernel, int rw) -> int {
+    if (rb.by % rw) return fail(GTOMX_ERR_INTERNAL, "row band not a multiple of the rows per workgroup");
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, in, out, nrows, tw);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, in, out, nrows, tw, rb);
     GX_LAUNCHED("k_rows_r2c");
     return GTOMX_OK;
   };
@@ -731,6 +765,57 @@ int image_r2c_banded(const float* in, float2* out, int n, unsigned nb, int band,
   StreamScratch scr;
   GX_TRY(scr.take((size_t)nh * n * nb, s));
   return cols_axis<-1>(n, out, scr.p, (int)nh, nh, (long long)nb, 1LL << 62, nh * n, 0, tw, s);
+}
+
+int rows_per_workgroup(int n) {
+  switch (n) {
+#define GX_CASE(N) \
+  case N:          \
+    return rows_per_block<N>();
+    GX_FOR_SIZES(GX_CASE)
+#undef GX_CASE
+  }
+  return 0;
+}
+
+int volume_c2r_banded(float2* work, float* out, gtomx_int3 d, unsigned nb, int cols, int by, int bz, float scale,
+                      hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  if (d.x > 256 || d.y > 256 || d.z > 256 || by <= 0 || 2 * by > d.y || bz < 0 || 2 * bz > d.z)
+    return fail(GTOMX_ERR_INTERNAL, "banded C2R outside its range");
+  GX_TRY(yz_axes(work, nullptr, d, nb, 1, tw, s, cols));
+  RowBand rb;
+  rb.ny = d.y, rb.by = by, rb.nz = d.z, rb.bz = bz;
+  const long long rows = (long long)nb * (bz ? 2 * bz : d.z) * 2 * by;
+  return launch_rows_c2r(d.x, work, out, rows, scale, tw, s, rb);
+}
+
+int volume_r2c_banded(const float* in, float2* out, gtomx_int3 d, unsigned nb, int by, int bz, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  if (d.x > 256 || d.y > 256 || d.z > 256 || by <= 0 || 2 * by > d.y || bz < 0 || 2 * bz > d.z)
+    return fail(GTOMX_ERR_INTERNAL, "banded R2C outside its range");
+  const long long nh = d.x / 2 + 1, all = (long long)nb * d.z * d.y;
+  RowBand rb;
+  rb.ny = d.y, rb.by = by, rb.nz = d.z, rb.bz = bz;
+  hipLaunchKernelGGL(k_zero_outside_band, dim3((unsigned)std::min<long long>((all * nh + 255) / 256, 8192)),
+                     dim3(256), 0, s, out, all, (int)nh, rb);
+  GX_LAUNCHED("k_zero_outside_band");
+  GX_TRY(launch_rows_r2c(d.x, in, out, (long long)nb * (bz ? 2 * bz : d.z) * 2 * by, tw, s, rb));
+  if (d.y > 1) {  // planes outside the z band are zero: their y transform is skipped
+    if (bz > 0) {
+      const long long pl = nh * d.y;
+      GX_TRY(launch_cols_n<-1>(d.y, out, (int)nh, nh, (long long)nb * bz, bz, pl, pl * d.z, tw, s));
+      GX_TRY(launch_cols_n<-1>(d.y, out + (d.z - bz) * pl, (int)nh, nh, (long long)nb * bz, bz, pl, pl * d.z, tw,
+                               s));
+    } else {
+      GX_TRY(launch_cols_n<-1>(d.y, out, (int)nh, nh, (long long)d.z * nb, 1LL << 62, nh * d.y, 0, tw, s));
+    }
+  }
+  if (d.z > 1)
+    GX_TRY(launch_cols_n<-1>(d.z, out, (int)nh, nh * d.y, (long long)d.y * nb, d.y, nh, nh * d.y * d.z, tw, s));
+  return GTOMX_OK;
 }
 
 int fft_c2r_cols_inplace(float2* data, gtomx_int3 d, unsigned batch, hipStream_t s) {

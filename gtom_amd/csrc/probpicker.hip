@@ -814,8 +814,17 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     const int n2d = pp->di.x;
     const bool banded = pp->ndims == 2 && pp->circular && n2d > 256 && pp->di.y == n2d;
     const int band = std::min(n2d / 2, (pp->dr.y / 2 + 1 + 7) / 8 * 8);
+    // the same for volumes and images up to 256 per axis: bands in y (whole
+    // row blocks of the row kernels) and, for volumes, in z
+    const int rw = rows_per_workgroup(pp->di.x);
+    const int vby = rw > 0 ? (pp->dr.y / 2 + 1 + rw - 1) / rw * rw : 0;
+    const int vbz = pp->ndims == 3 && 2 * (pp->dr.z / 2 + 1) <= pp->di.z ? pp->dr.z / 2 + 1 : 0;
+    const bool vbanded = !banded && pp->circular && pp->di.x <= 256 && pp->di.y <= 256 && pp->di.z <= 256 &&
+                         vby > 0 && 2 * vby <= pp->di.y;
     if (banded)
       GX_TRY(image_c2r_banded(pp->padft, pp->real, n2d, nb, pp->dl.x / 2 + 1, band, 1.0f / (float)Vi, s));
+    else if (vbanded)
+      GX_TRY(volume_c2r_banded(pp->padft, pp->real, pp->di, nb, pp->dl.x / 2 + 1, vby, vbz, 1.0f / (float)Vi, s));
     else  // the padded spectrum is zero beyond the low-pass half width
       GX_TRY(fft_c2r(pp->padft, pp->padft, pp->real, pp->di, nb, 1.0f / (float)Vi, s, pp->dl.x / 2 + 1));
     // the mask of each orientation: the same one (circular), or the B-spline
@@ -842,6 +851,8 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     // correlation with the image
     if (banded)
       GX_TRY(image_r2c_banded(pp->real, pp->padft, n2d, nb, band, s));
+    else if (vbanded)
+      GX_TRY(volume_r2c_banded(pp->real, pp->padft, pp->di, nb, vby, vbz, s));
     else
       GX_TRY(fft_r2c(pp->real, pp->padft, pp->di, nb, s));
     GX_TRY(image_times_conj(pp->imageft, pp->padft, Hi, nb, s));

@@ -117,6 +117,13 @@ int image_cols_pass2_inverse(int n, const float2* scratch, float2* out, unsigned
 // multiple of 8 and at most n / 2.
 int image_c2r_banded(float2* work, float* out, int n, unsigned nb, int cols, int band, float scale, hipStream_t s);
 int image_r2c_banded(const float* in, float2* out, int n, unsigned nb, int band, hipStream_t s);
+// The same for volumes and images with every axis up to 256: rows y in
+// [0, by) and [ny - by, ny) of the planes z in [0, bz) and [nz - bz, nz)
+// (bz = 0: every plane); by a multiple of rows_per_workgroup(nx).
+int rows_per_workgroup(int nx);
+int volume_c2r_banded(float2* work, float* out, gtomx_int3 dims, unsigned nb, int cols, int by, int bz, float scale,
+                      hipStream_t s);
+int volume_r2c_banded(const float* in, float2* out, gtomx_int3 dims, unsigned nb, int by, int bz, hipStream_t s);
 // The inverse along z then y only, in place (the rows stay half spectra).
 int fft_c2r_cols_inplace(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
 // For n x n images with n > 256: the row C2R of nb spectra (H apart in G, in
