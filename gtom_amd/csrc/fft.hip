@@ -50,25 +50,6 @@ __global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
   }
 }
 
-// Row selection of the one-pass row kernels: by == 0 takes rows in order;
-// otherwise only rows y in [0, by) and [ny - by, ny) of each plane, and (bz >
-// 0) only planes z in [0, bz) and [nz - bz, nz) of each volume, counted in
-// that order. by must be a multiple of the rows per workgroup.
-struct RowBand {
-  int ny = 0, by = 0, nz = 1, bz = 0;
-};
-__device__ __forceinline__ long long band_row(long long v, const RowBand& b) {
-  if (b.by == 0) return v;
-  const long long py = 2LL * b.by, pz = b.bz ? 2LL * b.bz : b.nz;
-  const int yy = (int)(v % py);
-  const long long q = v / py;
-  const int zz = (int)(q % pz);
-  const long long img = q / pz;
-  const int y = yy < b.by ? yy : b.ny - (int)py + yy;
-  const int z = b.bz == 0 ? zz : (zz < b.bz ? zz : b.nz - (int)pz + zz);
-  return (img * b.nz + z) * b.ny + y;
-}
-
 // ---------------------------------------------------------------------------
 // Row C2R: each row holds N/2+1 complex bins and becomes N reals, scaled.
 // Z[k] = (X[k] + conj X[N/2-k]) + i w^k (X[k] - conj X[N/2-k]), w = e^{2 pi i/N},

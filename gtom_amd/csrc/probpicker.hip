@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "dev_bspline.hpp"
+#include "dev_fft.hpp"
 #include "runtime.hpp"
 
 namespace gx {
@@ -107,6 +108,18 @@ __global__ __launch_bounds__(256) void k_times_mask(float* __restrict__ x, const
                                                     unsigned batch, size_t mstride) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < per * batch; i += (size_t)gridDim.x * 256)
     x[i] *= m[(i / per) * mstride + i % per];
+}
+
+// The mask product on the rows of a RowBand only (vrows of them, nx values
+// each; one mask shared by all volumes): the banded transforms read no other
+// rows.
+__global__ __launch_bounds__(256) void k_times_mask_band(float* __restrict__ x, const float* __restrict__ m, int nx,
+                                                         long long vrows, RowBand rb, long long plane_rows) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < vrows * nx; i += (long long)gridDim.x * 256) {
+    const long long row = band_row(i / nx, rb);
+    const int c = (int)(i % nx);
+    x[row * nx + c] *= m[(row % plane_rows) * nx + c];
+  }
 }
 
 // out = a * conj(b) (d_ComplexMultiplyByConjVector, src/Generics/Arithmetics.cu:334-344)
@@ -838,8 +851,17 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
       mk = pp->mrot;
       mstride = Vi;
     }
-    hipLaunchKernelGGL(k_times_mask, dim3(grid_of(Vi * nb)), dim3(256), 0, s, pp->real, mk, Vi, nb, mstride);
-    GX_LAUNCHED("k_times_mask");
+    if (banded || vbanded) {  // circular masks: one mask, rows of the band only
+      RowBand rb;
+      rb.ny = pp->di.y, rb.by = banded ? band : vby, rb.nz = pp->di.z, rb.bz = banded ? 0 : vbz;
+      const long long vrows = (long long)nb * (rb.bz ? 2 * rb.bz : rb.nz) * 2 * rb.by;
+      hipLaunchKernelGGL(k_times_mask_band, dim3((unsigned)std::min<long long>((vrows * pp->di.x + 255) / 256, 8192)),
+                         dim3(256), 0, s, pp->real, mk, pp->di.x, vrows, rb, (long long)pp->di.y * pp->di.z);
+      GX_LAUNCHED("k_times_mask_band");
+    } else {
+      hipLaunchKernelGGL(k_times_mask, dim3(grid_of(Vi * nb)), dim3(256), 0, s, pp->real, mk, Vi, nb, mstride);
+      GX_LAUNCHED("k_times_mask");
+    }
     // statistics of the cropped, masked reference
     const int nparts = (int)std::min<size_t>(NBS, (Er + 255) / 256);
     const Draws draws{pp->raw, qlo, p0, Us, Vs, (double)RAND_MAX};
