@@ -165,6 +165,8 @@ struct LineMap {
   int width, G;
   int L, TG, TC;                     // output times W_L^{(g TG + c TC) k} when L > 0
   float scale;
+  int band = 0;                      // > 0: entries at axis positions g + G i outside [0, band) and
+                                     // [L - band, L) read as zero (the first pass of a banded input)
 };
 
 template <int N, int DIR, int TX>
@@ -186,7 +188,11 @@ __global__ __launch_bounds__(TX * LineShape<N>::R2) void k_lines(const float2* _
     const int ms = (int)p.Ms, md = (int)p.Md;
     float2 v[R1];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = valid ? sl[(j + R2 * m) * ms] : make_float2(0.f, 0.f);
+    for (int m = 0; m < R1; ++m) {
+      const int y = g + p.G * (j + R2 * m);
+      const bool in = p.band == 0 || y < p.band || y >= p.L - p.band;
+      v[m] = valid && in ? sl[(j + R2 * m) * ms] : make_float2(0.f, 0.f);
+    }
     __syncthreads();
     fft_two_step<N, R1, R2, DIR>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
     if (valid) {
@@ -414,16 +420,6 @@ __global__ __launch_bounds__(256) void k_pick_rows_max_long(const float2* __rest
   }
 }
 
-// Zeroes rows [band, per - band) of every image (per rows of NH bins).
-__global__ __launch_bounds__(256) void k_zero_mid_rows(float2* __restrict__ data, int per, int band, long long NH,
-                                                       unsigned nb) {
-  const long long mid = (long long)(per - 2 * band) * NH;
-  const unsigned b = blockIdx.y;
-  float2* d = data + (size_t)b * per * NH + (size_t)band * NH;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < mid; i += (long long)gridDim.x * 256)
-    d[i] = make_float2(0.f, 0.f);
-}
-
 // Zeroes the rows of half spectra outside a RowBand (rows of NH bins, nrows
 // in all: every plane of every volume).
 __global__ __launch_bounds__(256) void k_zero_outside_band(float2* __restrict__ data, long long nrows, int NH,
@@ -550,7 +546,7 @@ int launch_lines(int n, const float2* src, float2* dst, const LineMap& p, const 
 // mirrors data's extent.
 template <int DIR>
 int cols_axis(int L, float2* data, float2* scratch, int width, long long as, long long nplanes, long long P1,
-              long long S0, long long S1, const float2* tw, hipStream_t s) {
+              long long S0, long long S1, const float2* tw, hipStream_t s, int band = 0) {
   if (L <= 256) return launch_cols_n<DIR>(L, data, width, as, nplanes, P1, S0, S1, tw, s);
   int L1, L2;
   if (!split_len(L, &L1, &L2) || !scratch) return fail(GTOMX_ERR_INTERNAL, "axis length not split");
@@ -558,10 +554,10 @@ int cols_axis(int L, float2* data, float2* scratch, int width, long long as, lon
   LineMap a{};
   a.P1 = P1, a.S0s = a.S0d = S0, a.S1s = a.S1d = S1, a.Cs = a.Cd = 1;
   a.G = L2, a.Gs = a.Gd = as, a.Ms = a.Md = L2 * as, a.nplanes = nplanes * L2, a.width = width;
-  a.L = L, a.TG = 1, a.TC = 0, a.scale = 1.f;
+  a.L = L, a.TG = 1, a.TC = 0, a.scale = 1.f, a.band = band;
   GX_TRY(launch_lines<DIR>(L1, data, scratch, a, tw, s));
   LineMap b = a;
-  b.G = L1, b.Gs = L2 * as, b.Gd = as, b.Ms = as, b.Md = L1 * as, b.nplanes = nplanes * L1, b.L = 0;
+  b.G = L1, b.Gs = L2 * as, b.Gd = as, b.Ms = as, b.Md = L1 * as, b.nplanes = nplanes * L1, b.L = 0, b.band = 0;
   return launch_lines<DIR>(L2, scratch, data, b, tw, s);
 }
 
@@ -736,16 +732,12 @@ int image_r2c_banded(const float* in, float2* out, int n, unsigned nb, int band,
   GX_TRY(twiddles(&tw));
   if (n <= 256 || band <= 0 || 2 * band > n) return fail(GTOMX_ERR_INTERNAL, "banded R2C outside its range");
   const long long nh = n / 2 + 1;
-  if (2 * band < n) {
-    const long long mid = (long long)(n - 2 * band) * nh;
-    hipLaunchKernelGGL(k_zero_mid_rows, dim3((unsigned)std::min<long long>((mid + 255) / 256, 2048), nb), dim3(256),
-                       0, s, out, n, band, nh, nb);
-    GX_LAUNCHED("k_zero_mid_rows");
-  }
+  // the rows between the bands are never written: the y transform's first
+  // pass reads them as zero and its second pass writes every row
   GX_TRY(launch_rows_long<true>(n, in, out, (long long)nb * 2 * band, 1.f, tw, s, n, band));
   StreamScratch scr;
   GX_TRY(scr.take((size_t)nh * n * nb, s));
-  return cols_axis<-1>(n, out, scr.p, (int)nh, nh, (long long)nb, 1LL << 62, nh * n, 0, tw, s);
+  return cols_axis<-1>(n, out, scr.p, (int)nh, nh, (long long)nb, 1LL << 62, nh * n, 0, tw, s, band);
 }
 
 int rows_per_workgroup(int n) {

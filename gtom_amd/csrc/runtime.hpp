@@ -113,7 +113,7 @@ int image_cols_pass2_inverse(int n, const float2* scratch, float2* out, unsigned
 // C2R with the y inverse over the first `cols` columns only (the rest must be
 // zero) and the row inverse only over the rows [0, band) and [n - band, n)
 // (other rows of `out` keep their contents); R2C of input that is zero
-// outside those rows (rows of `out` between them are zeroed). band is a
+// outside those rows (its rows between the bands are not read). band is a
 // multiple of 8 and at most n / 2.
 int image_c2r_banded(float2* work, float* out, int n, unsigned nb, int cols, int band, float scale, hipStream_t s);
 int image_r2c_banded(const float* in, float2* out, int n, unsigned nb, int band, hipStream_t s);
