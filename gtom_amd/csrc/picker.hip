@@ -86,8 +86,8 @@ __global__ __launch_bounds__(256) void k_norm_apply(const float* __restrict__ x,
 
 }  // namespace
 
-// scratch: NB * batch + 2 * batch doubles
-int norm_mean01std(const float* in, float* out, size_t n, unsigned batch, double* scratch, hipStream_t s) {
+// The (mean, sd) of each item, in double, at scratch + NB * batch.
+static int norm_stats(const float* in, size_t n, unsigned batch, double* scratch, hipStream_t s) {
   double* part = scratch;
   double* stats = scratch + (size_t)NB * batch;
   for (int pass = 0; pass < 2; ++pass) {
@@ -96,6 +96,13 @@ int norm_mean01std(const float* in, float* out, size_t n, unsigned batch, double
     hipLaunchKernelGGL(k_stats_finish, dim3(batch), dim3(256), 0, s, part, NB, n, pass, stats);
     GX_LAUNCHED("k_stats_finish");
   }
+  return GTOMX_OK;
+}
+
+// scratch: NB * batch + 2 * batch doubles
+int norm_mean01std(const float* in, float* out, size_t n, unsigned batch, double* scratch, hipStream_t s) {
+  double* stats = scratch + (size_t)NB * batch;
+  GX_TRY(norm_stats(in, n, batch, scratch, s));
   const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_norm_apply, dim3(grid, batch), dim3(256), 0, s, in, out, n, stats);
   GX_LAUNCHED("k_norm_apply");
@@ -112,6 +119,29 @@ __global__ __launch_bounds__(256) void k_image_times_conj(const float2* __restri
   const size_t total = h * batch;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
     const float2 e = E[i % h], p = P[i];
+    P[i] = make_float2(e.x * p.x + e.y * p.y, e.y * p.x - e.x * p.y);
+  }
+}
+
+// The same product with P the R2C of the un-normalised padded reference: the
+// normalisation (x - mean) / sd of k_norm_apply applied in Fourier space,
+// P' = (P - mean * n at the DC bin) / sd (sd = 0: no division), since the
+// transform is linear. Lets the R2C skip the rows the padding leaves zero.
+__global__ __launch_bounds__(256) void k_image_times_conj_norm(const float2* __restrict__ E, float2* __restrict__ P,
+                                                               size_t h, unsigned batch,
+                                                               const double* __restrict__ stats, size_t n) {
+  const size_t total = h * batch;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    const size_t b = i / h;
+    const double mean = stats[2 * b], sd = stats[2 * b + 1];
+    const float2 e = E[i % h], q = P[i];
+    double px = q.x, py = q.y;
+    if (i % h == 0) px -= mean * (double)n;
+    if (sd != 0.0) {
+      px /= sd;
+      py /= sd;
+    }
+    const float2 p = make_float2((float)px, (float)py);
     P[i] = make_float2(e.x * p.x + e.y * p.y, e.y * p.x - e.x * p.y);
   }
 }
@@ -326,11 +356,28 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
     GX_TRY(fft_c2r(pk->proj, pk->proj, pk->realref, dr, nb, 1.0f / (float)pk->Vr, s));
     // FullPad, mean0/std1, R2C   (:225-227)
     GX_TRY(fft_resize(FFT_FULL_PAD, false, pk->realref, pk->padded, dr, pk->di, nb, s));
-    GX_TRY(norm_mean01std(pk->padded, pk->padded, pk->V, nb, pk->nscr, s));
-    GX_TRY(fft_r2c(pk->padded, pk->pft, pk->di, nb, s));
-    // C = C2R_u(imageft * conj(P))   (:228-229)
-    hipLaunchKernelGGL(k_image_times_conj, dim3(grid_i), dim3(256), 0, s, pk->imageft, pk->pft, pk->H, nb);
-    GX_LAUNCHED("k_image_times_conj");
+    // The padded reference is zero outside the z x y box of the reference
+    // (FFT order); where the row kernels allow, the R2C runs over that band
+    // of rows and the normalisation moves into the product (same values up
+    // to rounding).
+    const int rw = rows_per_workgroup(pk->di.x);
+    const int vby = rw > 0 ? (dr.y / 2 + 1 + rw - 1) / rw * rw : 0;
+    const int vbz = 2 * (dr.z / 2 + 1) <= pk->di.z ? dr.z / 2 + 1 : 0;
+    const bool vbanded = pk->di.x <= 256 && pk->di.y <= 256 && pk->di.z <= 256 && vby > 0 && 2 * vby <= pk->di.y;
+    if (vbanded) {
+      GX_TRY(norm_stats(pk->padded, pk->V, nb, pk->nscr, s));
+      GX_TRY(volume_r2c_banded(pk->padded, pk->pft, pk->di, nb, vby, vbz, s));
+      // C = C2R_u(imageft * conj(P))   (:228-229)
+      hipLaunchKernelGGL(k_image_times_conj_norm, dim3(grid_i), dim3(256), 0, s, pk->imageft, pk->pft, pk->H, nb,
+                         pk->nscr + (size_t)NB * nb, pk->V);
+      GX_LAUNCHED("k_image_times_conj_norm");
+    } else {
+      GX_TRY(norm_mean01std(pk->padded, pk->padded, pk->V, nb, pk->nscr, s));
+      GX_TRY(fft_r2c(pk->padded, pk->pft, pk->di, nb, s));
+      // C = C2R_u(imageft * conj(P))   (:228-229)
+      hipLaunchKernelGGL(k_image_times_conj, dim3(grid_i), dim3(256), 0, s, pk->imageft, pk->pft, pk->H, nb);
+      GX_LAUNCHED("k_image_times_conj");
+    }
     GX_TRY(fft_c2r(pk->pft, pk->pft, pk->padded, pk->di, nb, 1.0f, s));
     // best/angle where C > best   (:233-241)
     hipLaunchKernelGGL(k_update_max_angle, dim3(grid_v), dim3(256), 0, s, pk->padded, pk->V, nb, pk->dang,

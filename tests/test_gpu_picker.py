@@ -73,7 +73,7 @@ def _run_picker(gx, cuda, ref, image, ctf, angles=None, anglestep=None, batch=0)
     return to_np(best), to_np(ang), to_np(img)
 
 
-@pytest.mark.parametrize("shape,batch", [((32, 32, 32), 0), ((24, 48, 32), 5)])
+@pytest.mark.parametrize("shape,batch", [((32, 32, 32), 0), ((24, 48, 32), 5), ((128, 128, 128), 0)])
 def test_picker_matches_oracle(gx, orc, cuda, shape, batch):
     nr = 16
     ref = _blob(nr, 11)
