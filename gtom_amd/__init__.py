@@ -141,6 +141,21 @@ _SIGS = {
     "gtomx_pick_begin": [_P, int3, _U, _P, _P, _P],
     "gtomx_pick_update": [_P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P],
     "gtomx_pick_finish": [_P, _P, int3, _U, _F, _P, _P, _P, _P],
+    "gtomx_pick_update_tiles": [_P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P, _P],
+    "gtomx_pick_subtomograms_learned": [_P, _P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _U, _F, _F, _P,
+                                        _P, _P, _P, _P],
+    "gtomx_pnet_param_count": [],
+    "gtomx_pnet_create": [ctypes.POINTER(_P), _U],
+    "gtomx_pnet_destroy": [_P],
+    "gtomx_pnet_set_params": [_P, _P, _SZ, _P],
+    "gtomx_pnet_get_params": [_P, _P, _SZ, _P],
+    "gtomx_pnet_load_tiles": [_P, _P, _U, _P],
+    "gtomx_pnet_load_volume": [_P, _P, int3, _P],
+    "gtomx_pnet_forward": [_P, _U, _P, _P],
+    "gtomx_pnet_gradients": [_P, _P, _U, _P, _P, _P],
+    "gtomx_pnet_train_step": [_P, _P, _U, _F, _F, _P, _P],
+    "gtomx_pnet_propose": [_P, _U, _P, _P, _P, _P],
+    "gtomx_pnet_debug_gemm": [_P, _P, _P, _I, _I, _I, _P],
 }
 _RESTYPES = {"gtomx_version": ctypes.c_char_p, "gtomx_status_string": ctypes.c_char_p,
              "gtomx_last_error": ctypes.c_char_p, "gtomx_debug_fail_alloc_after": None}
@@ -878,3 +893,52 @@ def pick_finish(plan, exp_ft, maskradius, best, bestidx, bestangle=None, stream=
     nvol = exp_ft.numel() // (plan.nz * n * (n // 2 + 1))
     check(lib().gtomx_pick_finish(plan.handle, _ptr(exp_ft), int3(n, n, plan.nz), nvol, maskradius, _ptr(best),
                                   _ptr(bestidx), _ptr(bestangle), _stream(stream)), "gtomx_pick_finish")
+
+
+class LearnedStats(ctypes.Structure):
+    """gtomx_learned_stats."""
+    _fields_ = [("tiles", ctypes.c_uint), ("exhaustive", ctypes.c_uint), ("shortlisted", ctypes.c_uint),
+                ("skipped", ctypes.c_uint), ("groups_searched", ctypes.c_uint)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def pick_update_tiles(plan, projector_ft, oversample, exp_ft, ctf, angles, group, tilemasks, best, bestidx,
+                      ctf_volume_stride=0, stream=None):
+    """gtomx_pick_update_tiles: angles (ngroups*group, 3); tilemasks uint8 (nvol, ngroups, nt, nt) with
+    bit tx of [.., tz, ty] selecting the 32^3 tile (tx, ty, tz)."""
+    n, p = plan.n, plan.p
+    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    a = _angles_arr(angles)
+    ngroups = a.shape[0] // group
+    m = np.ascontiguousarray(tilemasks, np.uint8)
+    nt = n // 32
+    assert m.size == nvol * ngroups * nt * nt, (m.shape, nvol, ngroups, nt)
+    check(lib().gtomx_pick_update_tiles(plan.handle, _ptr(projector_ft), oversample, int3(p, p, p), _ptr(exp_ft),
+                                        _ptr(ctf), ctf_volume_stride, int3(n, n, n), nvol, a.ctypes.data, ngroups,
+                                        group, m.ctypes.data, _ptr(best), _ptr(bestidx), _stream(stream)),
+          "gtomx_pick_update_tiles")
+
+
+def d_PickSubTomogramsLearned(plan, net, projector_ft, oversample, exp_ft, ctf, angles, maskradius, npsi=18,
+                              topk=16, confidence=0.5, ctf_volume_stride=0, best=None, bestidx=None,
+                              bestangle=None, stream=None):
+    """The learned search (gtomx_pick_subtomograms_learned): returns (best, bestidx, stats dict)."""
+    import torch
+    n, p = plan.n, plan.p
+    nvol = exp_ft.numel() // (n * n * (n // 2 + 1))
+    dev = exp_ft.device
+    if best is None:
+        best = torch.empty((nvol, n, n, n), dtype=torch.float32, device=dev)
+    if bestidx is None:
+        bestidx = torch.empty((nvol, n, n, n), dtype=torch.int32, device=dev)
+    a = _angles_arr(angles)
+    st = LearnedStats()
+    check(lib().gtomx_pick_subtomograms_learned(plan.handle, net.handle, _ptr(projector_ft), oversample,
+                                                int3(p, p, p), _ptr(exp_ft), _ptr(ctf), ctf_volume_stride,
+                                                int3(n, n, n), nvol, a.ctypes.data, a.shape[0] // npsi, npsi, topk,
+                                                confidence, maskradius, _ptr(best), _ptr(bestidx), _ptr(bestangle),
+                                                ctypes.byref(st), _stream(stream)),
+          "gtomx_pick_subtomograms_learned")
+    return best, bestidx, st.as_dict()

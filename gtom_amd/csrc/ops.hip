@@ -186,6 +186,13 @@ __global__ void k_sum_stage2(const double* partial, int np, double* result) {
   if (threadIdx.x == 0) *result = red[0];
 }
 
+// out = in - sum/N, the mean taken in double.
+__global__ void k_minus_mean(const float* in, const double* sum, double invN, float* out, size_t n) {
+  const double mu = *sum * invN;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)((double)in[i] - mu);
+}
+
 // The mask spectrum times a real half-spectrum Fourier mask (d_ComplexMultiplyByVector,
 // src/Generics/LocalStd.cu:38-41), and the mask sum that the reference takes as
 // the sum of the renormalised C2R of that product (:43-52): that sum is the
@@ -201,16 +208,21 @@ __global__ void k_mask_fourier(float2* mft, const float* fm, size_t n, double* m
 // sigma = sqrt(max(0, M*conv2 - conv1^2)) / M with conv = C2R_u(...)/N, in double
 // (src/Generics/LocalStd.cu:98-109); optionally fused with DivideSafeByVector
 // of the best correlation (src/Correlation/SubTomograms.cu:117).
+// The convolutions are of the map minus its global mean mu (shift), which
+// leaves M*conv2 - conv1^2 unchanged for any mask and keeps it from cancelling
+// in float when the map's offset is large against its local spread; the mean
+// gets mu back.
 __global__ void k_local_std_combine(const float* c1, const float* c2, const double* masksum, size_t n,
-                                    double invN, float* std_out, float* mean_out, float* best,
+                                    double invN, const double* shift, float* std_out, float* mean_out, float* best,
                                     const int32_t* idx, float* angle_out) {
   const double M = *masksum;
+  const double mu = shift ? *shift : 0.0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     double a = (double)c1[i] * invN, b = (double)c2[i] * invN;
     double v = fmax(M * b - a * a, 0.0);
     float sd = (float)(sqrt(v) / M);
     if (std_out) std_out[i] = sd;
-    if (mean_out) mean_out[i] = (float)(a / M);
+    if (mean_out) mean_out[i] = (float)(a / M + mu);
     if (best) best[i] = fabsf(sd) > 1e-15f ? best[i] / sd : 0.f;
     if (angle_out) angle_out[i] = (float)idx[i];
   }
@@ -379,9 +391,10 @@ int sum_f64(const float* d, size_t n, double* scratch, hipStream_t s) {
   return GTOMX_OK;
 }
 int local_std_combine(const float* c1, const float* c2, const double* d_masksum, size_t n, float* std_out,
-                      float* mean_out, float* best, const int32_t* idx, float* angle_out, hipStream_t s) {
+                      float* mean_out, float* best, const int32_t* idx, float* angle_out, hipStream_t s,
+                      const double* d_shift) {
   hipLaunchKernelGGL(k_local_std_combine, dim3(grid_for(n)), dim3(TPB), 0, s, c1, c2, d_masksum, n,
-                     1.0 / (double)n, std_out, mean_out, best, idx, angle_out);
+                     1.0 / (double)n, d_shift, std_out, mean_out, best, idx, angle_out);
   GX_LAUNCHED("k_local_std_combine");
   return GTOMX_OK;
 }
@@ -392,7 +405,7 @@ int local_std_combine(const float* c1, const float* c2, const double* d_masksum,
 
 size_t local_std_scratch_bytes(gtomx_int3 d) {
   size_t H = elements_ft(d), V = elements(d);
-  return 3 * H * sizeof(float2) + 3 * V * sizeof(float) + 257 * sizeof(double) + 256;
+  return 3 * H * sizeof(float2) + 3 * V * sizeof(float) + LS_DOUBLES * sizeof(double) + 256;
 }
 
 LocalStdScratch local_std_carve(void* base, gtomx_int3 d) {
@@ -427,17 +440,22 @@ int local_std_prepare(const LocalStdScratch& sc, gtomx_int3 d, float radius, hip
 }
 
 // sigma (and optional mean) of one map; with best != nullptr the best map is divided in place.
+// The map's global mean (double sum) is subtracted before the convolutions.
 int local_std_apply(const LocalStdScratch& sc, const float* map, gtomx_int3 d, float* std_out, float* mean_out,
                     float* best, const int32_t* idx, float* angle_out, hipStream_t s) {
   size_t V = elements(d), H = elements_ft(d);
-  GX_TRY(fft_r2c(map, sc.aft, d, 1, s));
-  GX_TRY(square_f32(map, sc.rbuf, V, s));
+  double* mapsum = sc.dsum + LS_MAPSUM;
+  GX_TRY(sum_f64(map, V, mapsum - 256, s));
+  hipLaunchKernelGGL(k_minus_mean, dim3(grid_for(V)), dim3(TPB), 0, s, map, mapsum, 1.0 / (double)V, sc.rbuf, V);
+  GX_LAUNCHED("k_minus_mean");
+  GX_TRY(fft_r2c(sc.rbuf, sc.aft, d, 1, s));
+  GX_TRY(square_f32(sc.rbuf, sc.rbuf, V, s));
   GX_TRY(fft_r2c(sc.rbuf, sc.bft, d, 1, s));
   GX_TRY(mul_conj_inplace(sc.aft, sc.maskft, H, s));
   GX_TRY(mul_conj_inplace(sc.bft, sc.maskft, H, s));
   GX_TRY(fft_c2r(sc.aft, sc.aft, sc.c1, d, 1, 1.0f, s));
   GX_TRY(fft_c2r(sc.bft, sc.bft, sc.c2, d, 1, 1.0f, s));
-  GX_TRY(local_std_combine(sc.c1, sc.c2, sc.dsum + 256, V, std_out, mean_out, best, idx, angle_out, s));
+  GX_TRY(local_std_combine(sc.c1, sc.c2, sc.dsum + 256, V, std_out, mean_out, best, idx, angle_out, s, mapsum));
   return GTOMX_OK;
 }
 

@@ -333,10 +333,25 @@ template <int N> struct Pass2Tile {
   static constexpr int TX = (GLayout<N>::SPLIT && (N / 2) % 32 == 0) ? 32 : ColShape<N>::TX;
 };
 
+// Tile masks (the learned search): tmask[tz * nt + ty] holds one bit per
+// 32-voxel tile tx of the output row band (z in tile tz, y in tile ty), nt =
+// N / 32; nullptr searches everything. Pass 2 transforms only the planes z
+// whose tile layer has a tile set and writes only the rows y of set (tz, ty)
+// bands; pass 3 takes only those rows and updates only the voxels of set
+// tiles. The unit-RMS scale is always computed.
+template <int N> __device__ __forceinline__ bool layer_any(const uint8_t* tmask, int z) {
+  constexpr int NT = N / 32;
+  const uint8_t* m = tmask + (z / 32) * NT;
+  uint8_t a = 0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) a |= m[k];
+  return a != 0;
+}
+
 template <int N>
 __global__ __launch_bounds__(Pass2Tile<N>::TX * ColShape<N>::R2) void k_pick_y(
     float2* __restrict__ G, const float* __restrict__ partial, int ntiles, const float* __restrict__ corr,
-    float* __restrict__ scale, const float2* __restrict__ tw) {
+    float* __restrict__ scale, const float2* __restrict__ tw, const uint8_t* __restrict__ tmask) {
   using L = GLayout<N>;
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = Pass2Tile<N>::TX;
   using CM = ColMap<N, TX>;
@@ -357,6 +372,10 @@ __global__ __launch_bounds__(Pass2Tile<N>::TX * ColShape<N>::R2) void k_pick_y(
   const int base = nyq_wg ? (int)L::MAIN + z * N : z * N * L::PITCH + ix;
   const int ystep = nyq_wg ? 1 : L::PITCH;
   const __amdgpu_buffer_rsrc_t rG = make_rsrc(G + (size_t)o * L::STRIDE, (unsigned)(L::STRIDE * 8));
+  // a whole workgroup's plane is skipped when its tile layer is empty (the Nyquist
+  // workgroups hold several planes and always run)
+  const bool skip = tmask && !nyq_wg && !layer_any<N>(tmask, z);
+  if (!skip) {
   float2 v[R1];
 #pragma unroll
   for (int m = 0; m < R1; ++m) {
@@ -376,9 +395,12 @@ __global__ __launch_bounds__(Pass2Tile<N>::TX * ColShape<N>::R2) void k_pick_y(
 #pragma unroll
   for (int s = 0; s < R1; ++s) {
     const int u = s / R2, k2 = s % R2;  // output y = j*(R1/R2) + u + R1*k2
+    const int y = j * (R1 / R2) + u + R1 * k2;
+    if (tmask && !nyq_wg && tmask[(z / 32) * (N / 32) + y / 32] == 0) continue;
     const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
     __builtin_amdgcn_raw_buffer_store_b64(vs, rG, obase * 8, (u + R1 * k2) * ystep * 8, CPOL_NT);
   }
+  }  // !skip
   if (blockIdx.x == 0) {
     // unit-RMS scale over the H half-spectrum bins (src/ImageManipulation/Norm.cu:476-509)
     __syncthreads();  // the exchange area is free again
@@ -411,14 +433,22 @@ template <int N, class L = GLayout<N>, bool FRESH = false>
 __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G, int nb, int angle0,
                                                     const float* __restrict__ scale, float* __restrict__ best,
                                                     int32_t* __restrict__ bidx, const float2* __restrict__ tw,
-                                                    int og = 0) {
+                                                    int og = 0, const uint8_t* __restrict__ tmask = nullptr) {
   constexpr int NC = N / 2;
   constexpr int R1 = RowShape<NC>::R1, R2 = RowShape<NC>::R2, RW = 256 / R2;
   constexpr int GS = R2 * (R1 + 2);  // two float2 of padding per thread slot (fewer bank conflicts than one)
   __shared__ float2 xch[R2 > 1 ? RW * GS : 1];
   const int t = threadIdx.x, r = t / R2, j = t % R2;
   const int row = blockIdx.x * RW + r;  // row = z*N + y (volumes) or y (images)
-  const bool valid = row < L::ROWS;
+  // tile bits of this row (volumes with a mask): bit k covers x in [32k, 32k + 32)
+  unsigned xbits = 0xffffffffu;
+  if (!FRESH && tmask) {
+    const int rz = min(row, L::ROWS - 1) / N, ry = min(row, L::ROWS - 1) % N;
+    xbits = tmask[(rz / 32) * (N / 32) + ry / 32];
+    // the workgroup leaves when none of its rows is searched
+    if (__syncthreads_or(row < L::ROWS && xbits != 0) == 0) return;
+  }
+  const bool valid = row < L::ROWS && xbits != 0;
   if (FRESH) {
     const int o0 = og * (int)blockIdx.y;
     G += (size_t)o0 * L::STRIDE;
@@ -477,6 +507,7 @@ __global__ __launch_bounds__(256) void k_pick_x_max(const float2* __restrict__ G
 #pragma unroll
     for (int s = 0; s < R1; ++s) {
       const float c0 = v[s].x * sc, c1 = v[s].y * sc;
+      if (tmask && !FRESH && !((xbits >> (out_index<R1, R2>(j, s) >> 4)) & 1u)) continue;  // x = 2m, 2m + 1
       if (c0 > bv[2 * s]) {
         bv[2 * s] = c0;
         bi[2 * s] = oi;
@@ -732,6 +763,17 @@ struct gtomx_pick_plan {
   int nparts = 0;           // 2D above 256: pass-1 workgroups per orientation
   float2* scr = nullptr;    // 2D above 256: pass-1 output (B spectra)
   size_t device_bytes = 0;
+  // tile masks of the grouped (learned) search: device copy and pinned staging
+  uint8_t* tmasks = nullptr;
+  size_t tmasks_cap = 0;
+  uint8_t* h_tmasks = nullptr;
+  size_t h_tmasks_cap = 0;
+  // proposals of the learned search (per tile: top-k directions, top-1 probability, class)
+  int32_t* ldirs = nullptr;
+  float* ltop1 = nullptr;
+  int32_t* lcls = nullptr;
+  size_t lcap = 0;  // tiles x topk
+  gx::StreamFence fence;  // the plan's buffers are reused by calls on any stream
   // kernel timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -747,6 +789,7 @@ namespace {
 
 void plan_release(gtomx_pick_plan* pl) {
   if (!pl) return;
+  pl->fence.release();
   dev_free(pl->G);
   dev_free(pl->partial);
   dev_free(pl->tiles);
@@ -759,6 +802,11 @@ void plan_release(gtomx_pick_plan* pl) {
   dev_free(pl->gbest);
   dev_free(pl->gidx);
   dev_free(pl->scr);
+  dev_free(pl->tmasks);
+  dev_free(pl->ldirs);
+  dev_free(pl->ltop1);
+  dev_free(pl->lcls);
+  if (pl->h_tmasks) (void)hipHostFree(pl->h_tmasks);
   if (pl->h_mats) (void)hipHostFree(pl->h_mats);
   if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
   for (hipEvent_t e : pl->ev_pool) (void)hipEventDestroy(e);
@@ -855,7 +903,8 @@ template <class F> int timed(gtomx_pick_plan* pl, int stage, hipStream_t s, F la
 
 template <int N, class PJ>
 int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float* mats, int nb, const float2* E,
-              const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s) {
+              const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw, hipStream_t s,
+              const uint8_t* tmask) {
   constexpr int TX = ColShape<N>::TX, R2 = ColShape<N>::R2;
   constexpr int RW = 256 / RowShape<N / 2>::R2;
   GX_TRY(timed(pl, 0, s, [&]() -> int {
@@ -865,17 +914,18 @@ int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float
     GX_LAUNCHED("k_pick_project_z");
     return GTOMX_OK;
   }));
-  hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
-  GX_LAUNCHED("k_pick_herm_norm");
+  // stage 1 of the timings: the plane correction and pass 2
   GX_TRY(timed(pl, 1, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
+    GX_LAUNCHED("k_pick_herm_norm");
     hipLaunchKernelGGL(k_pick_y<N>, dim3(ColMap<N, Pass2Tile<N>::TX>::NWG, nb), dim3(Pass2Tile<N>::TX * R2), 0, s,
-                       pl->G, pl->partial, pl->ntiles, pl->corr, pl->scale, tw);
+                       pl->G, pl->partial, pl->ntiles, pl->corr, pl->scale, tw, tmask);
     GX_LAUNCHED("k_pick_y");
     return GTOMX_OK;
   }));
   GX_TRY(timed(pl, 2, s, [&]() -> int {
     hipLaunchKernelGGL(k_pick_x_max<N>, dim3((N * N + RW - 1) / RW), dim3(256), 0, s, pl->G, nb, angle0, pl->scale,
-                       best, idx, tw);
+                       best, idx, tw, 0, tmask);
     GX_LAUNCHED("k_pick_x_max");
     return GTOMX_OK;
   }));
@@ -887,12 +937,12 @@ int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float
 // rounding of q), and the reference's layout otherwise.
 int run_batch_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projector& P, const float* mats, int nb,
                 const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
-                hipStream_t s) {
+                hipStream_t s, const uint8_t* tmask = nullptr) {
   switch (pl->n) {
 #define GX_CASE(N)                                                                            \
   case N:                                                                                     \
-    return C ? run_batch<N>(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, tw, s)            \
-             : run_batch<N>(pl, P, P, mats, nb, E, ctf, best, idx, angle0, tw, s);
+    return C ? run_batch<N>(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, tw, s, tmask)     \
+             : run_batch<N>(pl, P, P, mats, nb, E, ctf, best, idx, angle0, tw, s, tmask);
     GX_FOR_SIZES(GX_CASE)
 #undef GX_CASE
   }
@@ -1049,6 +1099,131 @@ int upload_matrices(gtomx_pick_plan* pl, const gtomx_float3* h_angles, unsigned 
   return GTOMX_OK;
 }
 
+// Copies `bytes` of tile masks to the device through pinned staging; call after
+// upload_matrices in the same call (which waited for the staging's last copy).
+int upload_masks(gtomx_pick_plan* pl, const uint8_t* h, size_t bytes, hipStream_t s) {
+  if (pl->tmasks_cap < bytes) {
+    dev_free(pl->tmasks);
+    pl->tmasks = nullptr;
+    pl->device_bytes -= pl->tmasks_cap;
+    pl->tmasks_cap = 0;
+    void* d = nullptr;
+    GX_TRY(dev_alloc(&d, bytes));
+    pl->tmasks = static_cast<uint8_t*>(d);
+    pl->tmasks_cap = bytes;
+    pl->device_bytes += bytes;
+  }
+  if (pl->h_tmasks_cap < bytes) {
+    if (pl->h_tmasks) (void)hipHostFree(pl->h_tmasks);
+    pl->h_tmasks = nullptr;
+    pl->h_tmasks_cap = 0;
+    void* h2 = nullptr;
+    hipError_t e = hipHostMalloc(&h2, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(tile masks)");
+    pl->h_tmasks = static_cast<uint8_t*>(h2);
+    pl->h_tmasks_cap = bytes;
+  }
+  std::memcpy(pl->h_tmasks, h, bytes);
+  GX_HIP(hipMemcpyAsync(pl->tmasks, pl->h_tmasks, bytes, hipMemcpyHostToDevice, s));
+  GX_HIP(hipEventRecord(pl->mats_ev, s));
+  pl->mats_ev_recorded = true;
+  return GTOMX_OK;
+}
+
+// The projector the sampler reads: the centred copy when every sampled q stays
+// within |q| < p/2 + 1/2, else the reference's layout. Returns whether centred.
+bool setup_projector(gtomx_pick_plan* pl, const gtomx_complex* d_projectorft, float oversample, hipStream_t s,
+                     Projector* P, CentredProjector* C, int* status) {
+  *P = Projector{reinterpret_cast<const float2*>(d_projectorft), pl->p, pl->p / 2 + 1,
+                 (pl->p / 2 + 1) * pl->p * pl->p * 8};
+  *status = GTOMX_OK;
+  const bool centred = (double)oversample * (pl->n / 2) <= pl->p / 2 + 0.25;
+  if (centred) {
+    const int p = pl->p, PX = p / 2 + 2, PY = p + 3, OFF = p / 2 + 1;
+    hipLaunchKernelGGL(k_centre_projector, dim3((PX * PY * PY + 255) / 256), dim3(256), 0, s, P->data, p, pl->cproj);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) *status = hip_fail(e, "k_centre_projector");
+    *C = CentredProjector{pl->cproj, centred_bytes(p), PX * 8, PX * PY * 8, (float)((OFF * PY + OFF) * PX * 8)};
+  }
+  return centred;
+}
+
+int check_update_args(const gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, float oversample,
+                      gtomx_int3 dimsprojector, const gtomx_complex* d_expft, gtomx_int3 dimsvolume,
+                      unsigned nvolumes, const float* d_best, const int32_t* d_idx) {
+  GX_TRY(check_plan_dims(plan, dimsvolume, nvolumes));
+  if (dimsprojector.x != plan->p || dimsprojector.y != plan->p || dimsprojector.z != plan->p)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "dimsprojector does not match the plan");
+  if (!d_projectorft || !d_expft || !d_best || !d_idx)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL device buffer");
+  if (!(oversample > 0.f) || !std::isfinite(oversample))
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "projectoroversample must be positive");
+  return GTOMX_OK;
+}
+
+// The grouped search: group g (orientations g*group .. g*group + group - 1 of
+// h_angles, which are also their indices) runs as one batch over the tiles set
+// in its mask (volume v's masks at h_masks + (v*ngroups + g)*nt*nt); groups with
+// empty masks are skipped.
+int update_tiles(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, float oversample,
+                 const gtomx_complex* d_expft, const float* d_ctf, size_t ctf_volume_stride, unsigned nvolumes,
+                 const gtomx_float3* h_angles, unsigned ngroups, unsigned group, const uint8_t* h_masks,
+                 float* d_best, int32_t* d_idx, hipStream_t s, unsigned* groups_run) {
+  const int nt = plan->n / 32;
+  const size_t mbytes = (size_t)nt * nt;
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  GX_TRY(upload_matrices(plan, h_angles, ngroups * group, oversample, s));
+  GX_TRY(upload_masks(plan, h_masks, nvolumes * (size_t)ngroups * mbytes, s));
+  Projector P;
+  CentredProjector C{};
+  int st = GTOMX_OK;
+  const bool centred = setup_projector(plan, d_projectorft, oversample, s, &P, &C, &st);
+  GX_TRY(st);
+  unsigned run = 0;
+  for (unsigned v = 0; v < nvolumes; ++v) {
+    const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
+    const float* cv = d_ctf ? d_ctf + (size_t)v * ctf_volume_stride : nullptr;
+    float* bv = d_best + (size_t)v * plan->V;
+    int32_t* iv = d_idx + (size_t)v * plan->V;
+    for (unsigned g = 0; g < ngroups; ++g) {
+      const uint8_t* hm = h_masks + ((size_t)v * ngroups + g) * mbytes;
+      bool any = false;
+      for (size_t k = 0; k < mbytes && !any; ++k) any = hm[k] != 0;
+      if (!any) continue;
+      ++run;
+      GX_TRY(run_batch_n(plan, centred ? &C : nullptr, P, plan->mats + 9 * (size_t)g * group, (int)group, Ev, cv,
+                         bv, iv, (int)(g * group), tw, s, plan->tmasks + ((size_t)v * ngroups + g) * mbytes));
+    }
+  }
+  if (groups_run) *groups_run = run;
+  return GTOMX_OK;
+}
+
+int check_grouped(const gtomx_pick_plan* plan, unsigned ngroups, unsigned group) {
+  if (plan->nz == 1 || plan->n % 32 || plan->n / 32 > 8)
+    return fail(GTOMX_ERR_UNSUPPORTED, "tile masks need n^3 volumes with n a multiple of 32, at most 256");
+  if (group == 0 || group > plan->B) return fail(GTOMX_ERR_INVALID_ARGUMENT, "group must be in [1, plan batch]");
+  if ((unsigned long long)ngroups * group > 0x7fffffffULL)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "orientation index exceeds int32");
+  return GTOMX_OK;
+}
+
+// Voxels of tiles that were not searched: best = 0, index = -1 (and the float angle -1).
+__global__ __launch_bounds__(256) void k_fill_unsearched(const uint8_t* __restrict__ searched, int n, float* best,
+                                                         int32_t* idx, float* angle, size_t total) {
+  const int nt = n / 32;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t V = (size_t)n * n * n;
+    const size_t v = i / V, r = i % V;
+    const int x = (int)(r % n), y = (int)((r / n) % n), z = (int)(r / ((size_t)n * n));
+    if (searched[v * nt * nt * nt + ((size_t)(z / 32) * nt + y / 32) * nt + x / 32]) continue;
+    best[i] = 0.f;
+    idx[i] = -1;
+    if (angle) angle[i] = -1.f;
+  }
+}
+
 }  // namespace
 }  // namespace gx
 
@@ -1171,8 +1346,7 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
 
 int gtomx_pick_plan_destroy(gtomx_pick_plan* plan) {
   if (!plan) return GTOMX_OK;
-  // Buffers may still be in use by enqueued work.
-  (void)hipDeviceSynchronize();
+  // Buffers may still be in use by the last call's work (plan_release waits for it).
   plan_release(plan);
   return GTOMX_OK;
 }
@@ -1217,10 +1391,11 @@ int gtomx_pick_begin(gtomx_pick_plan* plan, gtomx_int3 dimsvolume, unsigned nvol
   GX_TRY(check_plan_dims(plan, dimsvolume, nvolumes));
   if (!d_best || !d_idx) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL output buffer");
   hipStream_t s = as_stream(stream);
+  GX_TRY(plan->fence.enter(s));
   // d_ValueFill best = -1e30, angle = 0 (src/Correlation/SubTomograms.cu:41-42)
   GX_TRY(fill_f32(d_best, plan->V * nvolumes, -1e30f, s));
   GX_TRY(fill_i32(d_idx, plan->V * nvolumes, 0, s));
-  return GTOMX_OK;
+  return plan->fence.leave(s);
 }
 
 int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, float oversample,
@@ -1242,17 +1417,13 @@ int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
   hipStream_t s = as_stream(stream);
   const float2* tw = nullptr;
   GX_TRY(twiddles(&tw));
+  GX_TRY(plan->fence.enter(s));
   GX_TRY(upload_matrices(plan, h_angles, nangles, oversample, s));
-  Projector P{reinterpret_cast<const float2*>(d_projectorft), plan->p, plan->p / 2 + 1,
-              (plan->p / 2 + 1) * plan->p * plan->p * 8};
+  Projector P;
   CentredProjector C{};
-  const bool centred = (double)oversample * (plan->n / 2) <= plan->p / 2 + 0.25;
-  if (centred) {
-    const int p = plan->p, PX = p / 2 + 2, PY = p + 3, OFF = p / 2 + 1;
-    hipLaunchKernelGGL(k_centre_projector, dim3((PX * PY * PY + 255) / 256), dim3(256), 0, s, P.data, p, plan->cproj);
-    GX_LAUNCHED("k_centre_projector");
-    C = CentredProjector{plan->cproj, centred_bytes(p), PX * 8, PX * PY * 8, (float)((OFF * PY + OFF) * PX * 8)};
-  }
+  int st = GTOMX_OK;
+  const bool centred = setup_projector(plan, d_projectorft, oversample, s, &P, &C, &st);
+  GX_TRY(st);
   for (unsigned v = 0; v < nvolumes; ++v) {
     const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
     const float* cv = d_ctf ? d_ctf + (size_t)v * ctf_volume_stride : nullptr;
@@ -1268,7 +1439,7 @@ int gtomx_pick_update(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
                            (int)(angle_offset + b0), tw, s));
     }
   }
-  return GTOMX_OK;
+  return plan->fence.leave(s);
 }
 
 int gtomx_pick_finish(gtomx_pick_plan* plan, const gtomx_complex* d_expft, gtomx_int3 dimsvolume,
@@ -1279,6 +1450,7 @@ int gtomx_pick_finish(gtomx_pick_plan* plan, const gtomx_complex* d_expft, gtomx
   if (!(maskradius >= 0.f) || !std::isfinite(maskradius))
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "maskradius must be finite and non-negative");
   hipStream_t s = as_stream(stream);
+  GX_TRY(plan->fence.enter(s));
   GX_TRY(local_std_prepare(plan->ls, dimsvolume, maskradius, s));
   for (unsigned v = 0; v < nvolumes; ++v) {
     const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
@@ -1288,7 +1460,7 @@ int gtomx_pick_finish(gtomx_pick_plan* plan, const gtomx_complex* d_expft, gtomx
                            d_idx + (size_t)v * plan->V, d_bestangle ? d_bestangle + (size_t)v * plan->V : nullptr,
                            s));
   }
-  return GTOMX_OK;
+  return plan->fence.leave(s);
 }
 
 int gtomx_pick_subtomograms(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, float oversample,
@@ -1302,6 +1474,112 @@ int gtomx_pick_subtomograms(gtomx_pick_plan* plan, const gtomx_complex* d_projec
                            dimsvolume, nvolumes, h_angles, nangles, 0, d_best, d_idx, stream));
   GX_TRY(gtomx_pick_finish(plan, d_expft, dimsvolume, nvolumes, maskradius, d_best, d_idx, d_bestangle, stream));
   return GTOMX_OK;
+}
+
+int gtomx_pick_update_tiles(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, float oversample,
+                            gtomx_int3 dimsprojector, const gtomx_complex* d_expft, const float* d_ctf,
+                            size_t ctf_volume_stride, gtomx_int3 dimsvolume, unsigned nvolumes,
+                            const gtomx_float3* h_angles, unsigned ngroups, unsigned group, const uint8_t* h_tilemasks,
+                            float* d_best, int32_t* d_idx, void* stream) {
+  GX_TRY(check_update_args(plan, d_projectorft, oversample, dimsprojector, d_expft, dimsvolume, nvolumes, d_best,
+                           d_idx));
+  GX_TRY(check_grouped(plan, ngroups, group));
+  if (ngroups == 0) return GTOMX_OK;
+  if (!h_angles || !h_tilemasks) return fail(GTOMX_ERR_INVALID_ARGUMENT, "h_angles or h_tilemasks is NULL");
+  hipStream_t s = as_stream(stream);
+  GX_TRY(plan->fence.enter(s));
+  GX_TRY(update_tiles(plan, d_projectorft, oversample, d_expft, d_ctf, ctf_volume_stride, nvolumes, h_angles, ngroups,
+                      group, h_tilemasks, d_best, d_idx, s, nullptr));
+  return plan->fence.leave(s);
+}
+
+int gtomx_pick_subtomograms_learned(gtomx_pick_plan* plan, gtomx_pnet* net, const gtomx_complex* d_projectorft,
+                                    float oversample, gtomx_int3 dimsprojector, const gtomx_complex* d_expft,
+                                    const float* d_ctf, size_t ctf_volume_stride, gtomx_int3 dimsvolume,
+                                    unsigned nvolumes, const gtomx_float3* h_angles, unsigned ndirections,
+                                    unsigned npsi, unsigned topk, float confidence, float maskradius, float* d_best,
+                                    int32_t* d_idx, float* d_bestangle, gtomx_learned_stats* h_stats, void* stream) {
+  GX_TRY(check_update_args(plan, d_projectorft, oversample, dimsprojector, d_expft, dimsvolume, nvolumes, d_best,
+                           d_idx));
+  GX_TRY(check_grouped(plan, ndirections, npsi));
+  if (!net) return fail(GTOMX_ERR_INVALID_ARGUMENT, "network is NULL");
+  if (ndirections != 500) return fail(GTOMX_ERR_UNSUPPORTED, "the network scores 500 directions");
+  if (!h_angles) return fail(GTOMX_ERR_INVALID_ARGUMENT, "h_angles is NULL");
+  if (topk == 0 || topk > ndirections) return fail(GTOMX_ERR_INVALID_ARGUMENT, "topk must be in [1, 500]");
+  if (!(confidence >= 0.f && confidence <= 1.f)) return fail(GTOMX_ERR_INVALID_ARGUMENT, "confidence in [0, 1]");
+  hipStream_t s = as_stream(stream);
+  const int n = plan->n, nt = n / 32;
+  const size_t ntl = (size_t)nt * nt * nt, mbytes = (size_t)nt * nt;
+  GX_TRY(plan->fence.enter(s));
+  if (plan->lcap < ntl * topk) {
+    dev_free(plan->ldirs);
+    dev_free(plan->ltop1);
+    dev_free(plan->lcls);
+    plan->ldirs = nullptr;
+    plan->ltop1 = nullptr;
+    plan->lcls = nullptr;
+    plan->lcap = 0;
+    void *a = nullptr, *b = nullptr, *c = nullptr;
+    GX_TRY(dev_alloc(&a, ntl * topk * sizeof(int32_t)));
+    plan->ldirs = static_cast<int32_t*>(a);
+    GX_TRY(dev_alloc(&b, ntl * sizeof(float)));
+    plan->ltop1 = static_cast<float*>(b);
+    GX_TRY(dev_alloc(&c, ntl * sizeof(int32_t)));
+    plan->lcls = static_cast<int32_t*>(c);
+    plan->lcap = ntl * topk;
+  }
+  std::vector<uint8_t> masks((size_t)nvolumes * ndirections * mbytes, 0), searched((size_t)nvolumes * ntl, 0);
+  std::vector<int32_t> dirs(ntl * topk), cls(ntl);
+  std::vector<float> top1(ntl);
+  gtomx_learned_stats st{};
+  for (unsigned v = 0; v < nvolumes; ++v) {
+    // the network reads the block in real space: C2R_u(E_v) (its standardisation removes the scale)
+    const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
+    GX_TRY(fft_c2r(Ev, plan->G, plan->ebuf, dimsvolume, 1, 1.0f, s));
+    GX_TRY(gtomx_pnet_load_volume(net, plan->ebuf, dimsvolume, stream));
+    GX_TRY(gtomx_pnet_forward(net, (unsigned)ntl, nullptr, stream));
+    GX_TRY(gtomx_pnet_propose(net, topk, plan->ldirs, plan->ltop1, plan->lcls, stream));
+    GX_HIP(hipMemcpyAsync(dirs.data(), plan->ldirs, ntl * topk * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(top1.data(), plan->ltop1, ntl * sizeof(float), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(cls.data(), plan->lcls, ntl * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    uint8_t* mv = masks.data() + (size_t)v * ndirections * mbytes;
+    for (size_t t = 0; t < ntl; ++t) {
+      const int tz = (int)(t / mbytes), ty = (int)((t / nt) % nt), tx = (int)(t % nt);
+      const size_t mi = (size_t)tz * nt + ty;
+      const uint8_t bit = (uint8_t)(1u << tx);
+      if (top1[t] < confidence) {  // not confident: every direction
+        for (unsigned g = 0; g < ndirections; ++g) mv[g * mbytes + mi] |= bit;
+        searched[v * ntl + t] = 1;
+        st.exhaustive++;
+      } else if (cls[t] == (int)ndirections) {  // confidently no peak: not searched
+        st.skipped++;
+      } else {
+        for (unsigned k = 0; k < topk; ++k) {
+          const int d = dirs[t * topk + k];
+          if (d >= 0 && d < (int)ndirections) mv[(size_t)d * mbytes + mi] |= bit;
+        }
+        searched[v * ntl + t] = 1;
+        st.shortlisted++;
+      }
+    }
+  }
+  st.tiles = (unsigned)(ntl * nvolumes);
+  GX_TRY(gtomx_pick_begin(plan, dimsvolume, nvolumes, d_best, d_idx, stream));
+  GX_TRY(update_tiles(plan, d_projectorft, oversample, d_expft, d_ctf, ctf_volume_stride, nvolumes, h_angles,
+                      ndirections, npsi, masks.data(), d_best, d_idx, s, &st.groups_searched));
+  GX_TRY(gtomx_pick_finish(plan, d_expft, dimsvolume, nvolumes, maskradius, d_best, d_idx, d_bestangle, stream));
+  // the searched flags follow the masks in the staging: one more staged copy
+  DevBuf sflag;
+  GX_TRY(sflag.alloc(searched.size()));
+  GX_HIP(hipMemcpyAsync(sflag.p, searched.data(), searched.size(), hipMemcpyHostToDevice, s));
+  const size_t total = (size_t)nvolumes * plan->V;
+  hipLaunchKernelGGL(k_fill_unsearched, dim3((unsigned)std::min<size_t>((total + 255) / 256, 8192)), dim3(256), 0, s,
+                     sflag.as<uint8_t>(), n, d_best, d_idx, d_bestangle, total);
+  GX_LAUNCHED("k_fill_unsearched");
+  GX_HIP(hipStreamSynchronize(s));  // the flags' host copy and scratch go away on return
+  if (h_stats) *h_stats = st;
+  return plan->fence.leave(s);
 }
 
 }  // extern "C"

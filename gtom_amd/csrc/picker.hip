@@ -201,6 +201,7 @@ struct gtomx_picker {
   hipEvent_t staged = nullptr;
   bool staged_recorded = false;
   bool have_image = false;
+  gx::StreamFence fence;  // buffers are reused by calls on any stream
 };
 
 namespace gx {
@@ -208,6 +209,7 @@ namespace {
 
 void picker_release(gtomx_picker* pk) {
   if (!pk) return;
+  pk->fence.release();
   for (void* p : {(void*)pk->refft, (void*)pk->imageft, (void*)pk->proj, (void*)pk->realref, (void*)pk->padded,
                   (void*)pk->pft, (void*)pk->nscr, (void*)pk->mats, (void*)pk->dang})
     dev_free(p);
@@ -294,6 +296,7 @@ int gtomx_picker_create(gtomx_picker** out, const float* d_ref, gtomx_int3 dimsr
     // realref serves as the shifted copy.
     st = gtomx_remap_full2fullfft(d_ref, pk->realref, dimsref, 1, stream);
     if (st == GTOMX_OK) st = fft_r2c(pk->realref, pk->refft, dimsref, 1, s);
+    if (st == GTOMX_OK) st = pk->fence.leave(s);
   }
   if (st != GTOMX_OK) {
     std::string msg = gtomx_last_error();
@@ -306,8 +309,7 @@ int gtomx_picker_create(gtomx_picker** out, const float* d_ref, gtomx_int3 dimsr
 
 int gtomx_picker_destroy(gtomx_picker* picker) {
   if (!picker) return GTOMX_OK;
-  (void)hipDeviceSynchronize();  // buffers may still be in use by enqueued work
-  picker_release(picker);
+  picker_release(picker);  // waits for the last call's work on the picker's buffers
   return GTOMX_OK;
 }
 
@@ -315,6 +317,7 @@ int gtomx_picker_set_image(gtomx_picker* pk, float* d_image, const float* d_ctf,
   GX_TRY(picker_check(pk));
   if (!d_image || !d_ctf) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   hipStream_t s = as_stream(stream);
+  GX_TRY(pk->fence.enter(s));
   pk->ctf = d_ctf;
   // image -> mean0/std1 in place (src/Correlation/Picker.cu:148), imageft = R2C(image) (:152),
   // then the image is left holding its square (:154), as in the reference.
@@ -322,7 +325,7 @@ int gtomx_picker_set_image(gtomx_picker* pk, float* d_image, const float* d_ctf,
   GX_TRY(fft_r2c(d_image, pk->imageft, pk->di, 1, s));
   GX_TRY(square_f32(d_image, d_image, pk->V, s));
   pk->have_image = true;
-  return GTOMX_OK;
+  return pk->fence.leave(s);
 }
 
 int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles, unsigned nangles,
@@ -335,6 +338,7 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
   const gtomx_int3 dr{pk->nr, pk->nr, pk->nr};
   const unsigned grid_i = (unsigned)std::min<size_t>((pk->H * pk->B + 255) / 256, 65536);
   const unsigned grid_v = (unsigned)std::min<size_t>((pk->V + 255) / 256, 65536);
+  GX_TRY(pk->fence.enter(s));
   for (unsigned b0 = 0; b0 < nangles; b0 += pk->B) {
     const unsigned nb = std::min(pk->B, nangles - b0);
     // stage the matrices transpose(Euler) (no scale) and the angles for this batch
@@ -384,7 +388,7 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
                        d_bestccf, reinterpret_cast<float3*>(d_bestangle));
     GX_LAUNCHED("k_update_max_angle");
   }
-  return GTOMX_OK;
+  return pk->fence.leave(s);
 }
 
 int gtomx_picker_correlate(gtomx_picker* pk, float anglestep, float* d_bestccf, gtomx_float3* d_bestangle,

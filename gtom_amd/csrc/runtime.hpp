@@ -72,6 +72,44 @@ int check_fft_dims(int ndims, gtomx_int3 dims);
 inline size_t elements(gtomx_int3 d) { return (size_t)d.x * d.y * d.z; }
 inline size_t elements_ft(gtomx_int3 d) { return (size_t)(d.x / 2 + 1) * d.y * d.z; }
 
+// Orders a stateful object's calls across streams. Every call that enqueues
+// work on the object's buffers begins with enter(s): when the previous call
+// ran on another stream, s waits for that call's work (an event, no host
+// synchronisation). leave(s) records the event after the call's work, and
+// drain() blocks the host until it has completed (before buffers are freed).
+struct StreamFence {
+  hipEvent_t ev = nullptr;
+  hipStream_t last = nullptr;
+  bool recorded = false;
+  int enter(hipStream_t s) {
+    if (recorded && s != last) {
+      hipError_t e = hipStreamWaitEvent(s, ev, 0);
+      if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+    }
+    return GTOMX_OK;
+  }
+  int leave(hipStream_t s) {
+    if (!ev) {
+      hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    }
+    hipError_t e = hipEventRecord(ev, s);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    last = s;
+    recorded = true;
+    return GTOMX_OK;
+  }
+  void drain() {
+    if (recorded) (void)hipEventSynchronize(ev);
+    recorded = false;
+  }
+  void release() {
+    drain();
+    if (ev) (void)hipEventDestroy(ev);
+    ev = nullptr;
+  }
+};
+
 // Scratch device buffer that frees itself.
 struct DevBuf {
   void* p = nullptr;
@@ -170,9 +208,10 @@ int mul_conj_inplace(float2* a, const float2* b, size_t n, hipStream_t s);
 int square_f32(const float* in, float* out, size_t n, hipStream_t s);
 // std = sqrt(max(0, M*c2/N - (c1/N)^2)) / M (double), optional mean = c1/(N*M);
 // with best != nullptr also best = |std| > 1e-15 ? best/std : 0 and the float index copy.
+// d_shift (or nullptr): a constant subtracted from the map before the convolutions, added back to the mean.
 int local_std_combine(const float* c1, const float* c2, const double* d_masksum, size_t n,
                       float* std_out, float* mean_out, float* best, const int32_t* idx,
-                      float* angle_out, hipStream_t s);
+                      float* angle_out, hipStream_t s, const double* d_shift = nullptr);
 
 // Local standard deviation chain (ops.hip). The scratch holds the mask FT,
 // two spectra, three real volumes and a few doubles.
@@ -183,8 +222,10 @@ struct LocalStdScratch {
   float* rbuf;
   float* c1;
   float* c2;
-  double* dsum;  // 257 doubles; the mask sum is dsum[256]
+  double* dsum;  // LS_DOUBLES doubles; the mask sum is dsum[256], the map sum dsum[LS_MAPSUM]
 };
+constexpr int LS_MAPSUM = 520;  // its 256 partials precede it
+constexpr int LS_DOUBLES = LS_MAPSUM + 1;
 size_t local_std_scratch_bytes(gtomx_int3 d);
 LocalStdScratch local_std_carve(void* base, gtomx_int3 d);
 // fouriermask: NULL, or a real half spectrum (FFT order) that scales the mask spectrum.

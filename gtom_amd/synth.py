@@ -123,9 +123,11 @@ def cfg1(seed=1, n=64, box=32, ncopies=3, snr=0.1, angle_step_deg=30.0, maskradi
                                 amplitude=0.07), amplitude=amp)
 
 
-def cfg2(seed=2, shape=(256, 512, 512), block=256, box=48, ncopies=40, snr=0.05, ndirs=500, npsi=18,
+def cfg2(seed=2, shape=(256, 256, 512), block=256, box=48, ncopies=40, snr=0.05, ndirs=500, npsi=18,
          maskradius=24.0, pixelsize=10e-10, margin=24):
-    """512x512x256 tomogram as two 256^3 blocks, 48^3 template, 500x18 Fibonacci orientations.
+    """Two 256^3 blocks side by side in x (a 512 x 256 x 256 tomogram), 48^3 template,
+    500x18 Fibonacci orientations; copies are kept `margin` voxels from every face of
+    their block, because the correlation is circular.
 
     Returns the blocks (nblocks, 256, 256, 256) float32 and per-block defoci.
     """
@@ -138,8 +140,8 @@ def cfg2(seed=2, shape=(256, 512, 512), block=256, box=48, ncopies=40, snr=0.05,
     for i in range(ncopies):
         b = i % nblocks
         x = int(rng.integers(b * block + margin, (b + 1) * block - margin))
-        y = int(rng.integers(margin, ny - margin)) % block if ny > block else int(rng.integers(margin, ny - margin))
-        z = int(rng.integers(margin, nz - margin))
+        y = int(rng.integers(margin, block - margin))
+        z = int(rng.integers(margin, block - margin))
         positions.append((x, y, z))
         orientations.append(random_orientation(rng))
     amp = snr_amplitude(template, snr)
@@ -155,3 +157,63 @@ def cfg2(seed=2, shape=(256, 512, 512), block=256, box=48, ncopies=40, snr=0.05,
     return dict(n=block, p=p, oversample=p / block, template=template, blocks=blocks,
                 angles=fibonacci_angles(ndirs, npsi), positions=positions, orientations=orientations,
                 maskradius=maskradius, defoci=defoci, pixelsize=pixelsize, amplitude=amp)
+
+
+# ---------------------------------------------------------------------------
+# 256^3 blocks generated on the GPU (training and evaluation of the proposal
+# network need thousands of them)
+# ---------------------------------------------------------------------------
+def proposal_template(box=48):
+    """The fixed 48^3 random-blob template of the learned-proposal workloads (the
+    template cfg2(seed=2) draws first)."""
+    return blob_template(box, np.random.default_rng(2), radius=min(14.0, box / 2 - 6))
+
+
+def rotate_trilinear(template, mats):
+    """Rotated copies t(A^T (x - c) + c) of a (box, box, box) CUDA template for each
+    3x3 matrix in `mats` (k, 3, 3), by trilinear interpolation, zero outside."""
+    import torch
+    import torch.nn.functional as F
+    box = template.shape[0]
+    c = box // 2
+    dev = template.device
+    g = torch.arange(box, device=dev, dtype=torch.float32) - c
+    Z, Y, X = torch.meshgrid(g, g, g, indexing="ij")
+    pts = torch.stack([X, Y, Z], -1).reshape(-1, 3)  # (x, y, z)
+    A = torch.as_tensor(np.asarray(mats), dtype=torch.float32, device=dev)
+    src = pts[None] @ A + c  # row (A^T p)^T = p^T A
+    grid = (src / (box - 1) * 2 - 1).reshape(len(A), box, box, box, 3)
+    inp = template[None, None].expand(len(A), 1, box, box, box)
+    return F.grid_sample(inp, grid, mode="bilinear", padding_mode="zeros", align_corners=True)[:, 0]
+
+
+def tomogram_block_gpu(seed, template, amplitude, n=256, ncopies=20, margin=24, pixelsize=10e-10,
+                       defocus_range=(2e-6, 5e-6), device="cuda"):
+    """One n^3 block: Gaussian noise (sigma 1) plus `ncopies` copies of the template
+    at integer positions `margin` from the faces and uniform random orientations,
+    the signal scaled by `amplitude` and filtered by the block's CTF (defocus
+    uniform in defocus_range, 10 A/px, 300 kV, Cs 2.7 mm, amplitude contrast 0.07).
+    Positions, orientations and defocus come from numpy's generator at `seed`, the
+    noise from torch's CUDA generator at `seed`. Returns (block, positions (x, y, z),
+    orientations (rot, tilt, psi), defocus in metres)."""
+    import torch
+    from . import CtfParams, ctf_3d, d_FFTR2C, d_IFFTC2R
+    rng = np.random.default_rng(seed)
+    positions = rng.integers(margin, n - margin, (ncopies, 3))
+    orientations = [random_orientation(rng) for _ in range(ncopies)]
+    defocus = float(rng.uniform(*defocus_range))
+    box = template.shape[0]
+    h = box // 2
+    copies = rotate_trilinear(template, [euler_matrix(*a) for a in orientations]) * amplitude
+    sig = torch.zeros((n, n, n), dtype=torch.float32, device=device)
+    for k, (x, y, z) in enumerate(positions):
+        sig[z - h:z - h + box, y - h:y - h + box, x - h:x - h + box] += copies[k]
+    ctf = ctf_3d([CtfParams.make(pixelsize=pixelsize, Cs=2.7e-3, voltage=300e3, defocus=-defocus, amplitude=0.07)],
+                 n, device=device)
+    spec = d_FFTR2C(sig[None])
+    spec *= ctf
+    sig = d_IFFTC2R(spec, n)[0]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    sig += torch.randn((n, n, n), generator=gen, device=device)
+    return sig, positions, orientations, defocus

@@ -570,8 +570,8 @@ int gtomx_pick_plan_info(const gtomx_pick_plan* plan, unsigned* batch, size_t* d
 
 /* Kernel timing: when enabled, every launch of the search kernels is bracketed
  * by HIP events on the plan's stream; gtomx_pick_plan_timings synchronises and
- * returns, per stage k in {0: project+z, 1: y, 2: x+max}, the summed
- * milliseconds and launch count since the last reset. */
+ * returns, per stage k in {0: project+z, 1: plane correction + y, 2: x+max},
+ * the summed milliseconds and launch count since the last reset. */
 int gtomx_pick_plan_enable_timing(gtomx_pick_plan* plan, int enable);
 int gtomx_pick_plan_timings(gtomx_pick_plan* plan, double* ms3, unsigned long long* launches3,
                             int reset);
@@ -618,6 +618,92 @@ int gtomx_pick_finish(gtomx_pick_plan* plan, const gtomx_complex* d_experimental
                       gtomx_int3 dimsvolume, unsigned nvolumes, float maskradius,
                       float* d_bestcorrelation, int32_t* d_bestindex, float* d_bestangle,
                       void* stream);
+
+/* Grouped update over 32^3 tiles (no reference counterpart; the learned search
+ * below is built on it). h_angles holds ngroups * group orientations; group g
+ * (orientations g*group .. g*group + group - 1, whose indices are their
+ * positions in h_angles) is searched as one batch, and only on the tiles set in
+ * its mask: h_tilemasks holds, per volume and group, (n/32)^2 bytes, byte
+ * tz*(n/32) + ty with bit tx for the tile of voxels [32tx, 32tx+32) x
+ * [32ty, ...) x [32tz, ...). Voxels of unset tiles keep their values; groups
+ * with empty masks are skipped. Needs n^3 volumes with n a multiple of 32 (at
+ * most 256) and group <= the plan's batch. Between begin and finish like
+ * gtomx_pick_update; the values it writes are those of gtomx_pick_update on
+ * the same orientations, restricted to the tiles. */
+int gtomx_pick_update_tiles(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft,
+                            float projectoroversample, gtomx_int3 dimsprojector,
+                            const gtomx_complex* d_experimentalft, const float* d_ctf,
+                            size_t ctf_volume_stride, gtomx_int3 dimsvolume, unsigned nvolumes,
+                            const gtomx_float3* h_angles, unsigned ngroups, unsigned group,
+                            const uint8_t* h_tilemasks, float* d_bestcorrelation,
+                            int32_t* d_bestindex, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Learned orientation proposals (no reference counterpart)                  */
+/* ------------------------------------------------------------------------ */
+
+/* A 3D CNN over the 32^3 tiles of a volume (4 conv 3^3 layers of 16 channels
+ * with ReLU and average pooling, a linear head) scoring 500 coarse directions
+ * and one "no peak" class; bf16 weights and activations, fp32 accumulation,
+ * forward and backward on the matrix cores. Parameters are
+ * gtomx_pnet_param_count() floats (fp32 masters). Trained on the search's own
+ * argmax labels (direction = orientation index / in-plane steps). */
+typedef struct gtomx_pnet gtomx_pnet;
+int gtomx_pnet_param_count(void);
+int gtomx_pnet_create(gtomx_pnet** net, unsigned max_tiles);
+int gtomx_pnet_destroy(gtomx_pnet* net);
+/* Synchronous copies of all parameters; set also resets the optimiser state. */
+int gtomx_pnet_set_params(gtomx_pnet* net, const float* h_params, size_t count, void* stream);
+int gtomx_pnet_get_params(gtomx_pnet* net, float* h_params, size_t count, void* stream);
+/* Network input: n standardised 32^3 float tiles, or every tile of an n^3
+ * volume (n a multiple of 32) standardised by its mean and std. */
+int gtomx_pnet_load_tiles(gtomx_pnet* net, const float* d_tiles, unsigned n, void* stream);
+int gtomx_pnet_load_volume(gtomx_pnet* net, const float* d_volume, gtomx_int3 dims, void* stream);
+/* Logits of the first n loaded tiles (d_logits optional: n x 512 floats, 501 used). */
+int gtomx_pnet_forward(gtomx_pnet* net, unsigned n, float* d_logits, void* stream);
+/* Softmax cross-entropy against d_labels (class per tile, 500 = no peak): the
+ * gradients of every parameter (optional) and per-tile losses (optional). */
+int gtomx_pnet_gradients(gtomx_pnet* net, const int32_t* d_labels, unsigned n, float* d_grads,
+                         float* d_loss, void* stream);
+/* One Adam step (0.9, 0.999, 1e-8) with L2 weight decay wd on the loaded tiles. */
+int gtomx_pnet_train_step(gtomx_pnet* net, const int32_t* d_labels, unsigned n, float lr, float wd,
+                          float* d_loss, void* stream);
+/* From the last forward pass: per tile the top-k directions, the top-1
+ * softmax probability and the top class. */
+int gtomx_pnet_propose(gtomx_pnet* net, unsigned topk, int32_t* d_dirs, float* d_top1,
+                       int32_t* d_class, void* stream);
+/* The head's GEMM alone (tests of the matrix-core operand layout). Synchronous. */
+int gtomx_pnet_debug_gemm(const float* d_a, const float* d_b, float* d_c, int M, int N, int K,
+                          void* stream);
+
+typedef struct {
+  unsigned tiles;           /* tiles over all volumes */
+  unsigned exhaustive;      /* top-1 probability below the confidence: every direction */
+  unsigned shortlisted;     /* a confident direction: its top-k directions */
+  unsigned skipped;         /* confidently no peak: not searched */
+  unsigned groups_searched; /* (volume, direction) groups run */
+} gtomx_learned_stats;
+
+/* The learned search (cfg. d_PickSubTomograms, include/Correlation.cuh:97-109,
+ * whose argmax it is trained on): h_angles holds ndirections (500) x npsi
+ * orientations, direction-major. The network scores every 32^3 tile of
+ * C2R(E_v); a tile whose top-1 probability is below `confidence` is searched
+ * over all orientations, a tile confidently without a peak is not searched,
+ * and any other tile over the npsi orientations of its top-k directions.
+ * Searched voxels hold the result of gtomx_pick_subtomograms restricted to
+ * their tile's orientations (same kernels, same values); voxels of tiles that
+ * were not searched hold 0 and index -1. Synchronises `stream` once per
+ * volume (the proposals choose the launches) and before returning. */
+int gtomx_pick_subtomograms_learned(gtomx_pick_plan* plan, gtomx_pnet* net,
+                                    const gtomx_complex* d_projectorft, float projectoroversample,
+                                    gtomx_int3 dimsprojector, const gtomx_complex* d_experimentalft,
+                                    const float* d_ctf, size_t ctf_volume_stride,
+                                    gtomx_int3 dimsvolume, unsigned nvolumes,
+                                    const gtomx_float3* h_angles, unsigned ndirections,
+                                    unsigned npsi, unsigned topk, float confidence,
+                                    float maskradius, float* d_bestcorrelation,
+                                    int32_t* d_bestindex, float* d_bestangle,
+                                    gtomx_learned_stats* h_stats, void* stream);
 
 #ifdef __cplusplus
 }

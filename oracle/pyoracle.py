@@ -69,6 +69,10 @@ def lib():
         L.orc_bspline_prefilter_f64.argtypes = [_p, _i, _i, _i]
         L.orc_rotate_cubic_f64.argtypes = [_p, _p, _i, _i, _i, _p]
         L.orc_gaussrand_f32.argtypes = [ctypes.c_uint, _i, _p, _i]
+        L.orc_pnet_param_count.restype = _i
+        L.orc_pnet_forward_f64.argtypes = [_p, _p, _i, _i, _p]
+        L.orc_pnet_gradients_f64.argtypes = [_p, _p, _p, _i, _i, _p, _p]
+        L.orc_pnet_gradients_f64.restype = _d
         _lib = L
     return _lib
 
@@ -505,3 +509,31 @@ def prob_picker(ref, mask, image, lowpass, anglestep, ctf=None, refid=0, best=No
                               None if c is None else _ptr(c), float(anglestep), int(refid), _ptr(b), _ptr(ang),
                               _ptr(bref), _ptr(top2), int(bool(circular)))
     return b, ang, bref, top2
+
+
+def pnet_param_count():
+    return int(lib().orc_pnet_param_count())
+
+
+def pnet_forward(params, tiles, emulate=False):
+    """Logits (n, 501) of the proposal network (oracle/pnet.cpp) for tiles (n, 32, 32, 32)."""
+    P = _c(params, np.float64)
+    T = _c(tiles, np.float64)
+    n = T.shape[0]
+    out = np.empty((n, 501), np.float64)
+    lib().orc_pnet_forward_f64(_ptr(P), _ptr(T), n, int(bool(emulate)), _ptr(out))
+    return out
+
+
+def pnet_gradients(params, tiles, labels, emulate=False, plant_unflipped=False):
+    """(mean loss, gradients (nparams,), per-tile losses (n,)) of the softmax cross-entropy.
+    plant_unflipped: a deliberate mistake in the data gradient (for the tests that must catch it)."""
+    P = _c(params, np.float64)
+    T = _c(tiles, np.float64)
+    L = _c(labels, np.int32)
+    n = T.shape[0]
+    g = np.empty(P.shape[0], np.float64)
+    losses = np.empty(n, np.float64)
+    flags = int(bool(emulate)) | (2 if plant_unflipped else 0)
+    tot = lib().orc_pnet_gradients_f64(_ptr(P), _ptr(T), _ptr(L), n, flags, _ptr(g), _ptr(losses))
+    return tot, g, losses
