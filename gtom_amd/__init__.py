@@ -116,6 +116,7 @@ _SIGS = {
     "gtomx_mrc_to_device_f32": [_P, _I, _SZ, _P, _P],
     "gtomx_norm_mean01std": [_P, _P, _SZ, _U, _P],
     "gtomx_norm_monolithic": [_P, _P, _SZ, _U, _P],
+    "gtomx_norm_monolithic_noalloc": [_P, _P, _SZ, _U, _P],
     "gtomx_sum_f32": [_P, _P, _SZ, _U, _P],
     "gtomx_sum_f64": [_P, _P, _SZ, _U, _P],
     "gtomx_sum_i32": [_P, _P, _SZ, _U, _P],
@@ -543,6 +544,17 @@ def d_NormMonolithic(x, batch=1, stream=None):
     out = torch.empty_like(x)
     check(lib().gtomx_norm_monolithic(_ptr(x), _ptr(out), x.numel() // batch, batch, _stream(stream)),
           "gtomx_norm_monolithic")
+    return out
+
+
+NOALLOC_MAX_BATCH = 256  # GTOMX_NOALLOC_MAX_BATCH
+
+
+def d_NormMonolithic_noalloc(x, out, batch=1, stream=None):
+    """d_NormMonolithic through the allocation-free entry point: writes `out` (a float32 CUDA
+    tensor shaped like x) without allocating device memory or synchronising the stream."""
+    check(lib().gtomx_norm_monolithic_noalloc(_ptr(x), _ptr(out), x.numel() // batch, batch, _stream(stream)),
+          "gtomx_norm_monolithic_noalloc")
     return out
 
 

@@ -16,12 +16,26 @@
 #include <cmath>
 #include <cstdint>
 
+#include <mutex>
+
 #include "runtime.hpp"
 
 namespace gx {
 namespace {
-
 constexpr int NB = 256;  // reduction workgroups per volume
+}  // namespace
+}  // namespace gx
+
+// Scratch of the allocation-free normalisation (gtomx_norm_monolithic_noalloc):
+// the double2 partial sums of up to GTOMX_NOALLOC_MAX_BATCH items, then their
+// float2 coefficients (two per double2 slot). One copy per GPU, held until the
+// process ends. In host code its name is not a device address: every use goes
+// through the pointer hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double2 scratch[(size_t)GTOMX_NOALLOC_MAX_BATCH * gx::NB +
+                                                 GTOMX_NOALLOC_MAX_BATCH / 2];
+
+namespace gx {
+namespace {
 
 __device__ double2 block_sum2(double2 v, double2* red) {
   for (int o = 32; o > 0; o >>= 1) {
@@ -150,31 +164,79 @@ template <class T> int sum(const T* in, T* out, size_t n, unsigned batch, void* 
   return ok();
 }
 
-template <bool COMPLEX> int mono(const float* in, float* out, size_t n, unsigned batch, void* stream) {
+int mono_check(const float* in, const float* out, size_t n, unsigned batch) {
   if (!in || !out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (n == 0 || batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "elements and batch must be positive");
   if (batch > 65535) return fail(GTOMX_ERR_UNSUPPORTED, "batch above 65535");
-  hipStream_t s = as_stream(stream);
-  const int nb = (int)std::min<size_t>(NB, (n + 255) / 256);
-  DevBuf part, coef;
-  GX_TRY(part.alloc((size_t)nb * batch * sizeof(double2)));
-  GX_TRY(coef.alloc((size_t)batch * sizeof(float2)));
-  hipLaunchKernelGGL(k_mono_partial<COMPLEX>, dim3(nb, batch), dim3(256), 0, s, in, n, part.as<double2>());
+  return GTOMX_OK;
+}
+
+int mono_parts(size_t n) { return (int)std::min<size_t>(NB, (n + 255) / 256); }
+
+// The three launches of the normalisation with caller-provided scratch: part
+// (mono_parts(n) * batch double2) and coef (batch float2).
+template <bool COMPLEX>
+int mono_launch(const float* in, float* out, size_t n, unsigned batch, double2* part, float2* coef, hipStream_t s) {
+  const int nb = mono_parts(n);
+  hipLaunchKernelGGL(k_mono_partial<COMPLEX>, dim3(nb, batch), dim3(256), 0, s, in, n, part);
   GX_LAUNCHED("k_mono_partial");
-  hipLaunchKernelGGL(k_mono_finish<COMPLEX>, dim3(batch), dim3(256), 0, s, part.as<double2>(), nb, n,
-                     coef.as<float2>());
+  hipLaunchKernelGGL(k_mono_finish<COMPLEX>, dim3(batch), dim3(256), 0, s, part, nb, n, coef);
   GX_LAUNCHED("k_mono_finish");
   const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_mono_apply<COMPLEX>, dim3(grid, batch), dim3(256), 0, s, in, out, n, coef.as<float2>());
+  hipLaunchKernelGGL(k_mono_apply<COMPLEX>, dim3(grid, batch), dim3(256), 0, s, in, out, n, coef);
   GX_LAUNCHED("k_mono_apply");
+  return GTOMX_OK;
+}
+
+template <bool COMPLEX> int mono(const float* in, float* out, size_t n, unsigned batch, void* stream) {
+  GX_TRY(mono_check(in, out, n, batch));
+  hipStream_t s = as_stream(stream);
+  DevBuf part, coef;
+  GX_TRY(part.alloc((size_t)mono_parts(n) * batch * sizeof(double2)));
+  GX_TRY(coef.alloc((size_t)batch * sizeof(float2)));
+  GX_TRY(mono_launch<COMPLEX>(in, out, n, batch, part.as<double2>(), coef.as<float2>(), s));
   GX_HIP(hipStreamSynchronize(s));  // the scratch is freed on return
   return ok();
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double2* scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_address(double2** out) {
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(GTOMX_ERR_UNSUPPORTED, "device ordinal above 63");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    scratch_ptr[dev] = static_cast<double2*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return GTOMX_OK;
 }
 
 }  // namespace
 }  // namespace gx
 
 using namespace gx;
+
+extern "C" int gtomx_norm_monolithic_noalloc(const float* d_input, float* d_output, size_t elements, unsigned batch,
+                                             void* stream) {
+  GX_TRY(mono_check(d_input, d_output, elements, batch));
+  if (batch > GTOMX_NOALLOC_MAX_BATCH)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch above GTOMX_NOALLOC_MAX_BATCH (256) for the allocation-free call");
+  double2* base = nullptr;
+  GX_TRY(scratch_address(&base));
+  // partials of item b at base + b * mono_parts(n); coefficients after all partials
+  float2* coef = reinterpret_cast<float2*>(base + (size_t)GTOMX_NOALLOC_MAX_BATCH * NB);
+  GX_TRY(mono_launch<false>(d_input, d_output, elements, batch, base, coef, as_stream(stream)));
+  return ok();
+}
 
 extern "C" int gtomx_norm_monolithic(const float* d_input, float* d_output, size_t elements, unsigned batch,
                                      void* stream) {

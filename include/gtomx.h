@@ -314,6 +314,18 @@ int gtomx_sum_i32(const int32_t* d_input, int32_t* d_output, size_t n, unsigned 
  * Synchronises the stream (its scratch is freed on return). */
 int gtomx_norm_monolithic(const float* d_input, float* d_output, size_t elements, unsigned batch, void* stream);
 
+/* No counterpart in the reference. gtomx_norm_monolithic for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops): the same kernels and result, with the partial sums and coefficients
+ * in a file-scope device array of about 1 MiB per GPU (held until the process
+ * ends) instead of per-call allocations, and no synchronisation of the stream.
+ * batch may be at most GTOMX_NOALLOC_MAX_BATCH (else GTOMX_ERR_INVALID_ARGUMENT).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (on
+ * different streams, or from several threads) race. */
+#define GTOMX_NOALLOC_MAX_BATCH 256
+int gtomx_norm_monolithic_noalloc(const float* d_input, float* d_output, size_t elements, unsigned batch,
+                                  void* stream);
+
 /* d_NormFTMonolithic, include/ImageManipulation.cuh:55 / src/ImageManipulation/Norm.cu:147-154
  * (kernel :476-509): each of `batch` complex arrays divided by
  * max(sqrt(sum |x|^2 / elements), 1e-20) (sum in double, the root rounded to
