@@ -143,6 +143,13 @@ _SIGS = {
     "gtomx_pick_update": [_P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P],
     "gtomx_pick_finish": [_P, _P, int3, _U, _F, _P, _P, _P, _P],
     "gtomx_pick_update_tiles": [_P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P, _P],
+    "gtomx_pick_update_split": [_P, ctypes.c_int, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P],
+    "gtomx_split_dft64": [_P, _P, _U, ctypes.c_int, _P],
+    "gtomx_mpick_create": [_P, ctypes.c_int, _P, _P, _U],
+    "gtomx_mpick_destroy": [_P],
+    "gtomx_mpick_info": [_P, _P, _P, _P],
+    "gtomx_mpick_run": [_P, _P, _P, _U, _P, _U, ctypes.c_int, _P],
+    "gtomx_mpick_tune": [ctypes.c_int, ctypes.c_int, _P, _P, _U, _P, _U, ctypes.c_int, _P, _P],
     "gtomx_pick_subtomograms_learned": [_P, _P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _U, _F, _F, _P,
                                         _P, _P, _P, _P],
     "gtomx_pnet_param_count": [],
@@ -905,6 +912,110 @@ def pick_finish(plan, exp_ft, maskradius, best, bestidx, bestangle=None, stream=
     nvol = exp_ft.numel() // (plan.nz * n * (n // 2 + 1))
     check(lib().gtomx_pick_finish(plan.handle, _ptr(exp_ft), int3(n, n, plan.nz), nvol, maskradius, _ptr(best),
                                   _ptr(bestidx), _ptr(bestangle), _stream(stream)), "gtomx_pick_finish")
+
+
+# operand types of the split-precision matrix-core stages (GTOMX_SPLIT_* / GTOMX_SINGLE_*)
+SPLIT_FP16, SPLIT_BF16, SINGLE_FP16, SINGLE_BF16 = 0, 1, 2, 3
+PRECISIONS = {"fp16": SPLIT_FP16, "bf16": SPLIT_BF16, "fp16-hi": SINGLE_FP16, "bf16-hi": SINGLE_BF16}
+
+
+def pick_update_split(plan, precision, projector_ft, oversample, exp_ft, ctf, angles, angle_offset, best, bestidx,
+                      ctf_volume_stride=0, stream=None):
+    """gtomx_pick_update_split: gtomx_pick_update on 64^3 volumes with the radix-16 stages of every
+    per-orientation inverse transform on the matrix cores in `precision` (a PRECISIONS value)."""
+    n, p = plan.n, plan.p
+    nvol = exp_ft.numel() // (plan.nz * n * (n // 2 + 1))
+    a = _angles_arr(angles)
+    check(lib().gtomx_pick_update_split(plan.handle, int(precision), _ptr(projector_ft), oversample, int3(p, p, p),
+                                        _ptr(exp_ft), _ptr(ctf), ctf_volume_stride, int3(n, n, plan.nz), nvol,
+                                        a.ctypes.data, a.shape[0], angle_offset, _ptr(best), _ptr(bestidx),
+                                        _stream(stream)), "gtomx_pick_update_split")
+
+
+def d_PickSubTomogramsSplit(plan, precision, projector_ft, oversample, exp_ft, ctf, angles, maskradius,
+                            ctf_volume_stride=0, stream=None):
+    """The exhaustive search (begin, update, finish) with the split-precision update; returns (best, bestidx)."""
+    import torch
+    n = plan.n
+    nvol = exp_ft.numel() // (plan.nz * n * (n // 2 + 1))
+    best = torch.empty((nvol, n, n, n), dtype=torch.float32, device=exp_ft.device)
+    idx = torch.empty((nvol, n, n, n), dtype=torch.int32, device=exp_ft.device)
+    pick_begin(plan, nvol, best, idx, stream=stream)
+    pick_update_split(plan, precision, projector_ft, oversample, exp_ft, ctf, angles, 0, best, idx,
+                      ctf_volume_stride=ctf_volume_stride, stream=stream)
+    pick_finish(plan, exp_ft, maskradius, best, idx, stream=stream)
+    return best, idx
+
+
+def split_dft64(x, precision, stream=None):
+    """Unnormalised inverse 64-point DFTs of the rows of a complex64 CUDA tensor (..., 64) through the
+    split-precision matrix-core transform; returns a new tensor."""
+    import torch
+    assert x.dtype == torch.complex64 and x.shape[-1] == 64 and x.is_contiguous()
+    out = torch.empty_like(x)
+    check(lib().gtomx_split_dft64(_ptr(x), _ptr(out), x.numel() // 64, int(precision), _stream(stream)),
+          "gtomx_split_dft64")
+    return out
+
+
+class Micrograph(ctypes.Structure):
+    """gtomx_micrograph."""
+    _fields_ = [("n", ctypes.c_int), ("d_spectrum", ctypes.c_void_p), ("d_ctf", ctypes.c_void_p),
+                ("maskradius", ctypes.c_float), ("d_best", ctypes.c_void_p), ("d_index", ctypes.c_void_p)]
+
+
+class MicrographBatch:
+    """gtomx_mpick: the 2D search over micrographs of several sides, one plan per side (batches from
+    `batches`, e.g. the autotuner's table), per-side orientation loops captured as hipGraphs."""
+
+    def __init__(self, p, sides, batches=None):
+        sides = [int(n) for n in sides]
+        h = _P()
+        s = (ctypes.c_int * len(sides))(*sides)
+        b = (ctypes.c_uint * len(sides))(*(batches or [0] * len(sides)))
+        check(lib().gtomx_mpick_create(ctypes.byref(h), int(p), s, b, len(sides)), "gtomx_mpick_create")
+        self.handle, self.p, self.sides = h, int(p), sides
+
+    def close(self):
+        if self.handle:
+            lib().gtomx_mpick_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        b = (ctypes.c_uint * len(self.sides))()
+        cap = ctypes.c_ulonglong()
+        rep = ctypes.c_ulonglong()
+        check(lib().gtomx_mpick_info(self.handle, b, ctypes.byref(cap), ctypes.byref(rep)), "gtomx_mpick_info")
+        return {"batches": dict(zip(self.sides, list(b))), "captures": cap.value, "replays": rep.value}
+
+    def run(self, projector_ft, images, angles, use_graphs=True, stream=None):
+        """images: list of dicts with 'spectrum' (complex64 (n, n/2+1)), 'ctf' (float32 (n, n/2+1)),
+        'maskradius', 'best' (float32 (n, n)) and 'index' (int32 (n, n)) CUDA tensors."""
+        arr = (Micrograph * len(images))()
+        for k, im in enumerate(images):
+            n = im["spectrum"].shape[-2]
+            arr[k] = Micrograph(n, _ptr(im["spectrum"]), _ptr(im["ctf"]), float(im["maskradius"]), _ptr(im["best"]),
+                                _ptr(im["index"]))
+        a = _angles_arr(angles)
+        check(lib().gtomx_mpick_run(self.handle, _ptr(projector_ft), arr, len(images), a.ctypes.data, a.shape[0],
+                                    1 if use_graphs else 0, _stream(stream)), "gtomx_mpick_run")
+
+
+def mpick_tune(p, n, projector_ft, batches, angles, repeats=1, stream=None):
+    """Milliseconds per orientation of the 2D search's orientation loop on an n x n image for each
+    candidate batch (gtomx_mpick_tune; infinity where the plan does not fit)."""
+    a = _angles_arr(angles)
+    b = (ctypes.c_uint * len(batches))(*[int(x) for x in batches])
+    out = (ctypes.c_float * len(batches))()
+    check(lib().gtomx_mpick_tune(int(p), int(n), _ptr(projector_ft), b, len(batches), a.ctypes.data, a.shape[0],
+                                 int(repeats), out, _stream(stream)), "gtomx_mpick_tune")
+    return [float(x) for x in out]
 
 
 class LearnedStats(ctypes.Structure):

@@ -208,15 +208,15 @@ __global__ void k_mask_fourier(float2* mft, const float* fm, size_t n, double* m
 // sigma = sqrt(max(0, M*conv2 - conv1^2)) / M with conv = C2R_u(...)/N, in double
 // (src/Generics/LocalStd.cu:98-109); optionally fused with DivideSafeByVector
 // of the best correlation (src/Correlation/SubTomograms.cu:117).
-// The convolutions are of the map minus its global mean mu (shift), which
+// The convolutions are of the map minus its global mean mu = mapsum / n, which
 // leaves M*conv2 - conv1^2 unchanged for any mask and keeps it from cancelling
 // in float when the map's offset is large against its local spread; the mean
 // gets mu back.
 __global__ void k_local_std_combine(const float* c1, const float* c2, const double* masksum, size_t n,
-                                    double invN, const double* shift, float* std_out, float* mean_out, float* best,
+                                    double invN, const double* mapsum, float* std_out, float* mean_out, float* best,
                                     const int32_t* idx, float* angle_out) {
   const double M = *masksum;
-  const double mu = shift ? *shift : 0.0;
+  const double mu = mapsum ? *mapsum * invN : 0.0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     double a = (double)c1[i] * invN, b = (double)c2[i] * invN;
     double v = fmax(M * b - a * a, 0.0);
@@ -392,9 +392,9 @@ int sum_f64(const float* d, size_t n, double* scratch, hipStream_t s) {
 }
 int local_std_combine(const float* c1, const float* c2, const double* d_masksum, size_t n, float* std_out,
                       float* mean_out, float* best, const int32_t* idx, float* angle_out, hipStream_t s,
-                      const double* d_shift) {
+                      const double* d_mapsum) {
   hipLaunchKernelGGL(k_local_std_combine, dim3(grid_for(n)), dim3(TPB), 0, s, c1, c2, d_masksum, n,
-                     1.0 / (double)n, d_shift, std_out, mean_out, best, idx, angle_out);
+                     1.0 / (double)n, d_mapsum, std_out, mean_out, best, idx, angle_out);
   GX_LAUNCHED("k_local_std_combine");
   return GTOMX_OK;
 }

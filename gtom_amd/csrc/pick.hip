@@ -29,6 +29,7 @@
 
 #include "dev_fft.hpp"
 #include "dev_project.hpp"
+#include "pick_plan.hpp"
 #include "runtime.hpp"
 
 namespace gx {
@@ -113,6 +114,21 @@ __device__ __forceinline__ float2 sample(__amdgpu_buffer_rsrc_t rs, const Centre
   return project_centred(rs, C, px, py, pz, in);
 }
 
+// Tile masks (the learned search): tmask[tz * nt + ty] holds one bit per
+// 32-voxel tile tx of the output row band (z in tile tz, y in tile ty), nt =
+// N / 32; nullptr searches everything. Pass 2 transforms only the planes z
+// whose tile layer has a tile set and writes only the rows y of set (tz, ty)
+// bands; pass 3 takes only those rows and updates only the voxels of set
+// tiles. The unit-RMS scale is always computed.
+template <int N> __device__ __forceinline__ bool layer_any(const uint8_t* tmask, int z) {
+  constexpr int NT = N / 32;
+  const uint8_t* m = tmask + (z / 32) * NT;
+  uint8_t a = 0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) a |= m[k];
+  return a != 0;
+}
+
 // Columns (x, y) with x^2 + y^2 > (n/2)^2 hold no bin of the sphere, so Q is
 // zero along all of z there and stays zero through the z transform: pass 1
 // writes nothing for them and pass 2 reads zeros in their place.
@@ -142,12 +158,13 @@ template <int R1> __device__ constexpr int zorder(int k) { return k % 2 == 0 ? k
 // (orientation, tile), so that no barrier is spent on a block reduction.
 template <int N> constexpr int pass1_waves() { return (ColShape<N>::TX * ColShape<N>::R2 + 63) / 64; }
 
-template <int N, class PJ>
+template <int N, class PJ, bool MASKED = false>
 __global__ __launch_bounds__(ColShape<N>::TX * ColShape<N>::R2, (Pass1Occupancy<N, PJ>::MIN_BLOCKS)) void
 k_pick_project_z(
     PJ P, const float* __restrict__ mats, int nb, const float2* __restrict__ E,
     const float* __restrict__ ctf, float2* __restrict__ G, float* __restrict__ partial,
-    const float2* __restrict__ tw, const int* __restrict__ tiles, int ntiles) {
+    const float2* __restrict__ tw, const int* __restrict__ tiles, int ntiles,
+    const uint8_t* __restrict__ tmask) {
   using L = GLayout<N>;
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = ColShape<N>::TX;
   constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
@@ -161,6 +178,14 @@ k_pick_project_z(
   __shared__ float2 twl[N];
   const int t = threadIdx.x, c = t % TX, j = t / TX;
   for (int i = t; i < N; i += NT) twl[i] = twiddle<-1, N>(tw, i);
+  // With tile masks, bit tz of `layers` says whether z layer tz (planes 32tz ..
+  // 32tz + 31) holds a searched tile; pass 2 neither reads nor transforms the
+  // other planes, so their outputs are not stored.
+  unsigned layers = 0xffffffffu;
+  if constexpr (MASKED) {
+    layers = 0;
+    for (int tz = 0; tz < N / 32; ++tz) layers |= layer_any<N>(tmask, 32 * tz) ? 1u << tz : 0u;
+  }
 
   const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
   constexpr unsigned HB = (unsigned)(NH * N * N);
@@ -247,6 +272,7 @@ k_pick_project_z(
 #pragma unroll
       for (int s = 0; s < R1; ++s) {
         const int u = s / R2, k2 = s % R2;  // output z = j*(R1/R2) + u + R1*k2
+        if (MASKED && !((layers >> ((zrow0 + u + R1 * k2) >> 5)) & 1u)) continue;
         const u32x2 vs = __builtin_bit_cast(u32x2, v[s]);
         __builtin_amdgcn_raw_buffer_store_b64(vs, rG, gbase8, (u + R1 * k2) * gstep * 8, CPOL_NT);
       }
@@ -332,21 +358,6 @@ __global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float
 template <int N> struct Pass2Tile {
   static constexpr int TX = (GLayout<N>::SPLIT && (N / 2) % 32 == 0) ? 32 : ColShape<N>::TX;
 };
-
-// Tile masks (the learned search): tmask[tz * nt + ty] holds one bit per
-// 32-voxel tile tx of the output row band (z in tile tz, y in tile ty), nt =
-// N / 32; nullptr searches everything. Pass 2 transforms only the planes z
-// whose tile layer has a tile set and writes only the rows y of set (tz, ty)
-// bands; pass 3 takes only those rows and updates only the voxels of set
-// tiles. The unit-RMS scale is always computed.
-template <int N> __device__ __forceinline__ bool layer_any(const uint8_t* tmask, int z) {
-  constexpr int NT = N / 32;
-  const uint8_t* m = tmask + (z / 32) * NT;
-  uint8_t a = 0;
-#pragma unroll
-  for (int k = 0; k < NT; ++k) a |= m[k];
-  return a != 0;
-}
 
 template <int N>
 __global__ __launch_bounds__(Pass2Tile<N>::TX * ColShape<N>::R2) void k_pick_y(
@@ -731,58 +742,7 @@ __global__ __launch_bounds__(TX * LineShape<L1>::R2) void k_pick2d_project_cols(
 
 }  // namespace gx
 
-// ---------------------------------------------------------------------------
-// Plan
-// ---------------------------------------------------------------------------
-struct gtomx_pick_plan {
-  int n = 0, p = 0;
-  int nz = 0;  // n for n^3 volumes, 1 for n x n images (the 2D branch)
-  unsigned B = 0;
-  int device = 0;
-  size_t H = 0, V = 0;
-  int ntiles = 0;        // pass-1 column tiles inside the disc (partial sums per orientation)
-  int* tiles = nullptr;  // their workgroup indices
-  int grid1[2] = {0, 0}; // resident pass-1 workgroups: reference layout, centred layout
-  float2* G = nullptr;
-  float* partial = nullptr;
-  float* corr = nullptr;  // B x HERM_BLOCKS plane corrections of the unit-RMS sums
-  float* scale = nullptr;
-  float* mats = nullptr;
-  size_t mats_cap = 0;   // orientations
-  float* h_mats = nullptr;  // pinned staging
-  size_t h_mats_cap = 0;
-  hipEvent_t mats_ev = nullptr;
-  bool mats_ev_recorded = false;
-  void* ls_mem = nullptr;
-  gx::LocalStdScratch ls{};
-  float* ebuf = nullptr;
-  float2* cproj = nullptr;  // centred projector layout
-  float* gbest = nullptr;   // 2D: per-orientation-group maxima (groups x V)
-  int32_t* gidx = nullptr;
-  int og = 0;               // 2D: orientations per group
-  int nparts = 0;           // 2D above 256: pass-1 workgroups per orientation
-  float2* scr = nullptr;    // 2D above 256: pass-1 output (B spectra)
-  size_t device_bytes = 0;
-  // tile masks of the grouped (learned) search: device copy and pinned staging
-  uint8_t* tmasks = nullptr;
-  size_t tmasks_cap = 0;
-  uint8_t* h_tmasks = nullptr;
-  size_t h_tmasks_cap = 0;
-  // proposals of the learned search (per tile: top-k directions, top-1 probability, class)
-  int32_t* ldirs = nullptr;
-  float* ltop1 = nullptr;
-  int32_t* lcls = nullptr;
-  size_t lcap = 0;  // tiles x topk
-  gx::StreamFence fence;  // the plan's buffers are reused by calls on any stream
-  // kernel timing
-  bool timing = false;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  struct Pending { int stage; size_t a, b; };
-  std::vector<Pending> pending;
-  double ms[3] = {0, 0, 0};
-  unsigned long long launches[3] = {0, 0, 0};
-};
+// The plan (struct gtomx_pick_plan) is declared in pick_plan.hpp.
 
 namespace gx {
 namespace {
@@ -909,8 +869,12 @@ int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float
   constexpr int RW = 256 / RowShape<N / 2>::R2;
   GX_TRY(timed(pl, 0, s, [&]() -> int {
     const int g = pl->grid1[std::is_same<PJ, CentredProjector>::value ? 1 : 0];
-    hipLaunchKernelGGL((k_pick_project_z<N, PJ>), dim3(g), dim3(TX * R2), 0, s, PS, mats, nb, E, ctf, pl->G,
-                       pl->partial, tw, pl->tiles, pl->ntiles);
+    if (tmask)
+      hipLaunchKernelGGL((k_pick_project_z<N, PJ, true>), dim3(g), dim3(TX * R2), 0, s, PS, mats, nb, E, ctf, pl->G,
+                         pl->partial, tw, pl->tiles, pl->ntiles, tmask);
+    else
+      hipLaunchKernelGGL((k_pick_project_z<N, PJ>), dim3(g), dim3(TX * R2), 0, s, PS, mats, nb, E, ctf, pl->G,
+                         pl->partial, tw, pl->tiles, pl->ntiles, nullptr);
     GX_LAUNCHED("k_pick_project_z");
     return GTOMX_OK;
   }));
@@ -1054,6 +1018,8 @@ int run_batch2d_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projecto
   return fail(GTOMX_ERR_UNSUPPORTED, "image side not supported");
 }
 
+}  // namespace
+
 int check_plan_dims(const gtomx_pick_plan* pl, gtomx_int3 dimsvolume, unsigned nvolumes) {
   if (!pl) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
   if (dimsvolume.x != pl->n || dimsvolume.y != pl->n || dimsvolume.z != pl->nz)
@@ -1160,6 +1126,23 @@ int check_update_args(const gtomx_pick_plan* plan, const gtomx_complex* d_projec
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "projectoroversample must be positive");
   return GTOMX_OK;
 }
+
+int launch_update_batches(gtomx_pick_plan* pl, const Projector& P, const CentredProjector* C, const float* mats,
+                          unsigned nangles, const float2* E, const float* ctf, float* best, int32_t* idx,
+                          int angle0, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  for (unsigned b0 = 0; b0 < nangles; b0 += pl->B) {
+    const int nb = (int)std::min<unsigned>(pl->B, nangles - b0);
+    if (pl->nz == 1)
+      GX_TRY(run_batch2d_n(pl, C, P, mats + 9 * (size_t)b0, nb, E, ctf, best, idx, angle0 + (int)b0, tw, s));
+    else
+      GX_TRY(run_batch_n(pl, C, P, mats + 9 * (size_t)b0, nb, E, ctf, best, idx, angle0 + (int)b0, tw, s));
+  }
+  return GTOMX_OK;
+}
+
+namespace {
 
 // The grouped search: group g (orientations g*group .. g*group + group - 1 of
 // h_angles, which are also their indices) runs as one batch over the tiles set

@@ -208,10 +208,10 @@ int mul_conj_inplace(float2* a, const float2* b, size_t n, hipStream_t s);
 int square_f32(const float* in, float* out, size_t n, hipStream_t s);
 // std = sqrt(max(0, M*c2/N - (c1/N)^2)) / M (double), optional mean = c1/(N*M);
 // with best != nullptr also best = |std| > 1e-15 ? best/std : 0 and the float index copy.
-// d_shift (or nullptr): a constant subtracted from the map before the convolutions, added back to the mean.
+// d_mapsum (or nullptr): the map's sum; its mean was subtracted before the convolutions and is added back to the mean.
 int local_std_combine(const float* c1, const float* c2, const double* d_masksum, size_t n,
                       float* std_out, float* mean_out, float* best, const int32_t* idx,
-                      float* angle_out, hipStream_t s, const double* d_shift = nullptr);
+                      float* angle_out, hipStream_t s, const double* d_mapsum = nullptr);
 
 // Local standard deviation chain (ops.hip). The scratch holds the mask FT,
 // two spectra, three real volumes and a few doubles.

@@ -651,6 +651,85 @@ int gtomx_pick_update_tiles(gtomx_pick_plan* plan, const gtomx_complex* d_projec
                             int32_t* d_bestindex, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Micrographs of uneven sizes in one call                                   */
+/* ------------------------------------------------------------------------ */
+
+/* One micrograph of a batch: its R2C spectrum (n x (n/2+1), as
+ * gtomx_fft_r2c writes it), its own CTF on the same half grid (e.g. from
+ * gtomx_ctf_2d, astigmatism included), the local-std radius, and the outputs
+ * (n x n best and int32 index, as gtomx_pick_subtomograms writes them). */
+typedef struct {
+  int n;
+  const gtomx_complex* d_spectrum;
+  const float* d_ctf;
+  float maskradius;
+  float* d_best;
+  int32_t* d_index;
+} gtomx_micrograph;
+
+/* The 2D branch of d_PickSubTomograms (src/Correlation/SubTomograms.cu:34-36,
+ * include/Correlation.cuh:97-109) over images of several sizes: one search plan
+ * per side in h_sides (h_batches: orientations per pass for each side, 0 for the
+ * plan default; normally the autotuner's table), each with device slots for one
+ * image. The object owns a stream of its own (default priority). */
+typedef struct gtomx_mpick gtomx_mpick;
+int gtomx_mpick_create(gtomx_mpick** out, int projector_side, const int* h_sides, const unsigned* h_batches,
+                       unsigned nsides);
+int gtomx_mpick_destroy(gtomx_mpick* mp);
+/* Batches per side (in h_sides order), graphs captured and graph replays so far. */
+int gtomx_mpick_info(const gtomx_mpick* mp, unsigned* h_batches, unsigned long long* captures,
+                     unsigned long long* replays);
+/* Searches every image over h_angles (oversample projector_side / n for an
+ * image of side n; indices are positions in h_angles) with the projector
+ * d_projectorft (projector_side^3 half spectrum, FFT order). Each image's result
+ * equals gtomx_pick_subtomograms on that image alone. With use_graphs the
+ * orientation loop of each side is captured once as a hipGraph (again when the
+ * projector or the orientation list changes) and replayed per image. The work
+ * runs on the object's stream, ordered after `stream`'s prior work, and
+ * `stream` waits for it; the call returns without synchronising. */
+int gtomx_mpick_run(gtomx_mpick* mp, const gtomx_complex* d_projectorft, const gtomx_micrograph* h_images,
+                    unsigned nimages, const gtomx_float3* h_angles, unsigned nangles, int use_graphs,
+                    void* stream);
+/* Autotuner measurement for one image side: for each candidate batch, the
+ * orientation loop over h_angles (one warm-up, then `repeats` timed runs with
+ * events on `stream`), reported as milliseconds per orientation (infinity for a
+ * batch whose plan does not fit). Synchronous. */
+int gtomx_mpick_tune(int projector_side, int n, const gtomx_complex* d_projectorft, const unsigned* h_batches,
+                     unsigned ncand, const gtomx_float3* h_angles, unsigned nangles, int repeats,
+                     float* h_ms_per_orientation, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Split-precision transforms on the matrix cores (no reference counterpart) */
+/* ------------------------------------------------------------------------ */
+
+/* Operand types of the radix-16 matrix-core stages: hi/lo split fp16 or bf16
+ * (hi*hi + hi*lo + lo*hi, fp32 accumulation), or hi alone (no compensation). */
+#define GTOMX_SPLIT_FP16 0
+#define GTOMX_SPLIT_BF16 1
+#define GTOMX_SINGLE_FP16 2
+#define GTOMX_SINGLE_BF16 3
+
+/* gtomx_pick_update (the search over h_angles between begin and finish) on 64^3
+ * volumes with every per-orientation inverse transform (the C2R of
+ * src/Correlation/SubTomograms.cu:81, which the reference runs through cuFFT,
+ * src/FFT/IFFT.cu:48-56) done as 64 = 16 x 4 with the radix-16 stage on the
+ * matrix cores in `precision`; projection, CTF, conj-multiply, unit-RMS scale
+ * and max/argmax as in gtomx_pick_update, so only the transform's arithmetic
+ * differs. Asynchronous on `stream`. */
+int gtomx_pick_update_split(gtomx_pick_plan* plan, int precision, const gtomx_complex* d_projectorft,
+                            float projectoroversample, gtomx_int3 dimsprojector,
+                            const gtomx_complex* d_experimentalft, const float* d_ctf,
+                            size_t ctf_volume_stride, gtomx_int3 dimsvolume, unsigned nvolumes,
+                            const gtomx_float3* h_angles, unsigned nangles, unsigned angle_offset,
+                            float* d_bestcorrelation, int32_t* d_bestindex, void* stream);
+
+/* The transform of gtomx_pick_update_split alone: nlines contiguous lines of 64
+ * complex values, out[k] = sum_n in[n] exp(+2 pi i k n / 64) (unnormalised
+ * inverse); in may equal out. Asynchronous on `stream`. */
+int gtomx_split_dft64(const gtomx_complex* d_in, gtomx_complex* d_out, unsigned nlines, int precision,
+                      void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Learned orientation proposals (no reference counterpart)                  */
 /* ------------------------------------------------------------------------ */
 

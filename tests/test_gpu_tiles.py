@@ -37,6 +37,7 @@ def test_grouped_update_equals_plain_update_per_tile(gx, cuda, n, box, ngroups, 
     nt = n // 32
     masks = (rng.random((1, ngroups, nt, nt, nt)) < 0.4)
     masks[0, 1] = False  # an empty group is skipped
+    masks[0, :, nt - 1, 0, nt - 1] = False  # and one tile is set in no group
     packed = np.zeros((1, ngroups, nt, nt), np.uint8)
     for tx in range(nt):
         packed |= (masks[..., tx].astype(np.uint8) << tx)
