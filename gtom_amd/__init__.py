@@ -754,13 +754,15 @@ def d_NormMean01Std(x, batch=1, out=None, stream=None):
 
 class Picker:
     """The Picker class (include/Correlation.cuh:136-164): initialize with a cubic
-    reference (z, y, x) and the image shape; set_image; perform_correlation."""
+    reference (z, y, x) and a volume shape, or a square reference (y, x) and an
+    image shape (the 2D branch: in-plane orientations); set_image; perform_correlation."""
 
     def __init__(self, ref, imageshape, refmask=None, batch=0, stream=None):
         nr = ref.shape[-1]
-        nz, ny, nx = imageshape
+        nrz = ref.shape[0] if ref.dim() == 3 else 1
+        nz, ny, nx = tuple(imageshape) if len(imageshape) == 3 else (1,) + tuple(imageshape)
         h = _P()
-        check(lib().gtomx_picker_create(ctypes.byref(h), _ptr(ref), int3(nr, nr, nr), _ptr(refmask),
+        check(lib().gtomx_picker_create(ctypes.byref(h), _ptr(ref), int3(nr, nr, nrz), _ptr(refmask),
                                         int3(nx, ny, nz), batch, _stream(stream)), "gtomx_picker_create")
         self.handle = h
         self.imageshape = tuple(imageshape)

@@ -460,9 +460,10 @@ int local_std_apply(const LocalStdScratch& sc, const float* map, gtomx_int3 d, f
 }
 
 int project_launch(const float2* vol, int p, float2* out, int n, unsigned rmax, const float* d_mats,
-                   unsigned batch, hipStream_t s, int nz) {
+                   unsigned batch, hipStream_t s, int nz, int volz) {
   if (nz < 0) nz = n;
-  Projector P{vol, p, p / 2 + 1, (p / 2 + 1) * p * p * 8};
+  if (volz < 0) volz = p;
+  Projector P{vol, p, p / 2 + 1, (p / 2 + 1) * p * volz * 8};
   size_t H = (size_t)(n / 2 + 1) * n * nz;
   unsigned gx_ = (unsigned)std::min<size_t>((H + TPB - 1) / TPB, 4096);
   hipLaunchKernelGGL(k_project, dim3(gx_, batch), dim3(TPB), 0, s, P, out, n, nz, (int)(rmax * rmax), d_mats,

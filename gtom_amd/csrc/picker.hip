@@ -183,6 +183,7 @@ int image_times_conj(const float2* E, float2* P, size_t h, unsigned batch, hipSt
 // ---------------------------------------------------------------------------
 struct gtomx_picker {
   int nr = 0;           // reference side
+  int nrz = 0;          // reference z extent: nr (3D) or 1 (2D)
   gtomx_int3 di{};      // image dims
   int device = 0;
   unsigned B = 0;
@@ -250,17 +251,22 @@ int gtomx_picker_create(gtomx_picker** out, const float* d_ref, gtomx_int3 dimsr
   if (!out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "picker pointer is NULL");
   *out = nullptr;
   if (!d_ref) return fail(GTOMX_ERR_INVALID_ARGUMENT, "d_ref is NULL");
-  if (dimsref.x != dimsref.y || dimsref.x != dimsref.z)
-    return fail(GTOMX_ERR_UNSUPPORTED, "the reference must be a cube (3D picking)");
-  if (dimsimage.z < 2) return fail(GTOMX_ERR_UNSUPPORTED, "2D images are not supported");
-  GX_TRY(check_fft_dims(3, dimsref));
-  GX_TRY(check_fft_dims(3, dimsimage));
+  // 3D: a cubic reference and a volume; 2D (the reference's ndims == 2 branch,
+  // src/Correlation/Picker.cu:35, 163-171): a square reference and an image
+  const bool is2d = dimsref.z == 1;
+  if (dimsref.x != dimsref.y || (!is2d && dimsref.x != dimsref.z))
+    return fail(GTOMX_ERR_UNSUPPORTED, "the reference must be a cube (3D) or a square (2D)");
+  if (is2d != (dimsimage.z == 1))
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference and image must have the same dimensionality");
+  GX_TRY(check_fft_dims(is2d ? 2 : 3, dimsref));
+  GX_TRY(check_fft_dims(is2d ? 2 : 3, dimsimage));
   if (dimsref.x > dimsimage.x || dimsref.y > dimsimage.y || dimsref.z > dimsimage.z)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "the reference is larger than the image");
   if (batch > 64) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 64");
   hipStream_t s = as_stream(stream);
   auto* pk = new gtomx_picker();
   pk->nr = dimsref.x;
+  pk->nrz = dimsref.z;
   pk->di = dimsimage;
   pk->B = batch ? batch : 8u;
   pk->Hr = elements_ft(dimsref);
@@ -335,7 +341,7 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
   if (!d_bestccf || !d_bestangle) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL output buffer");
   if (nangles > 0 && !h_angles) return fail(GTOMX_ERR_INVALID_ARGUMENT, "h_angles is NULL");
   hipStream_t s = as_stream(stream);
-  const gtomx_int3 dr{pk->nr, pk->nr, pk->nr};
+  const gtomx_int3 dr{pk->nr, pk->nr, pk->nrz};
   const unsigned grid_i = (unsigned)std::min<size_t>((pk->H * pk->B + 255) / 256, 65536);
   const unsigned grid_v = (unsigned)std::min<size_t>((pk->V + 255) / 256, 65536);
   GX_TRY(pk->fence.enter(s));
@@ -354,7 +360,8 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
     GX_HIP(hipEventRecord(pk->staged, s));
     pk->staged_recorded = true;
     // R = project(refft) * ctf; r = C2R_n(R)   (src/Correlation/Picker.cu:185-190)
-    GX_TRY(project_launch(pk->refft, pk->nr, pk->proj, pk->nr, (unsigned)(pk->nr / 2), pk->mats, nb, s));
+    GX_TRY(project_launch(pk->refft, pk->nr, pk->proj, pk->nr, (unsigned)(pk->nr / 2), pk->mats, nb, s, pk->nrz,
+                          pk->nrz));
     GX_TRY(gtomx_complex_multiply_by_vector(reinterpret_cast<const gtomx_complex*>(pk->proj), pk->ctf,
                                             reinterpret_cast<gtomx_complex*>(pk->proj), pk->Hr, nb, stream));
     GX_TRY(fft_c2r(pk->proj, pk->proj, pk->realref, dr, nb, 1.0f / (float)pk->Vr, s));
@@ -367,7 +374,8 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
     const int rw = rows_per_workgroup(pk->di.x);
     const int vby = rw > 0 ? (dr.y / 2 + 1 + rw - 1) / rw * rw : 0;
     const int vbz = 2 * (dr.z / 2 + 1) <= pk->di.z ? dr.z / 2 + 1 : 0;
-    const bool vbanded = pk->di.x <= 256 && pk->di.y <= 256 && pk->di.z <= 256 && vby > 0 && 2 * vby <= pk->di.y;
+    const bool vbanded = pk->nrz > 1 && pk->di.x <= 256 && pk->di.y <= 256 && pk->di.z <= 256 && vby > 0 &&
+                         2 * vby <= pk->di.y;
     if (vbanded) {
       GX_TRY(norm_stats(pk->padded, pk->V, nb, pk->nscr, s));
       GX_TRY(volume_r2c_banded(pk->padded, pk->pft, pk->di, nb, vby, vbz, s));
@@ -396,12 +404,21 @@ int gtomx_picker_correlate(gtomx_picker* pk, float anglestep, float* d_bestccf, 
   GX_TRY(picker_check(pk));
   if (!(anglestep > 0.f) || !std::isfinite(anglestep))
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "anglestep must be positive");
-  // GetEqualAngularSpacing((0, 2pi), (0, pi), (0, 2pi), anglestep) (src/Correlation/Picker.cu:162-163)
   const float PI2 = 6.283185307179586f, PI = 3.14159265358979f;
-  int n = 0;
-  GX_TRY(gtomx_equal_angular_spacing(0.f, PI2, 0.f, PI, 0.f, PI2, anglestep, nullptr, 0, &n));
-  std::vector<gtomx_float3> angles((size_t)n);
-  GX_TRY(gtomx_equal_angular_spacing(0.f, PI2, 0.f, PI, 0.f, PI2, anglestep, angles.data(), n, &n));
+  std::vector<gtomx_float3> angles;
+  if (pk->nrz == 1) {
+    // 2D: ceil(2 pi / anglestep) in-plane steps of 2 pi / steps (src/Correlation/Picker.cu:165-171), in float
+    const int steps = (int)std::ceil(PI2 / anglestep);
+    const float step = PI2 / (float)steps;
+    for (int a = 0; a < steps; ++a) angles.push_back(gtomx_float3{0.f, 0.f, (float)a * step});
+  } else {
+    // GetEqualAngularSpacing((0, 2pi), (0, pi), (0, 2pi), anglestep) (src/Correlation/Picker.cu:163-164)
+    int n = 0;
+    GX_TRY(gtomx_equal_angular_spacing(0.f, PI2, 0.f, PI, 0.f, PI2, anglestep, nullptr, 0, &n));
+    angles.resize((size_t)n);
+    GX_TRY(gtomx_equal_angular_spacing(0.f, PI2, 0.f, PI, 0.f, PI2, anglestep, angles.data(), n, &n));
+  }
+  const int n = (int)angles.size();
   GX_TRY(gtomx_picker_correlate_angles(pk, angles.data(), (unsigned)n, d_bestccf, d_bestangle, stream));
   // the angle list lives on this stack frame; the uploads have been staged through pinned memory
   return GTOMX_OK;

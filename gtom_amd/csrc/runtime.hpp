@@ -235,8 +235,10 @@ int local_std_apply(const LocalStdScratch& sc, const float* map, gtomx_int3 d, f
                     float* best, const int32_t* idx, float* angle_out, hipStream_t s);
 
 // Standalone projector launch (ops.hip).
-// nz = n: n^3 projections; nz = 1: n x n central sections (2D).
+// nz = n: n^3 projections; nz = 1: n x n central sections (2D). volz: the
+// volume's z extent (p for a 3D spectrum; 1 for a 2D image's spectrum, whose
+// in-plane rotations read the missing planes as 0).
 int project_launch(const float2* vol, int p, float2* out, int n, unsigned rmax, const float* d_mats,
-                   unsigned batch, hipStream_t s, int nz = -1);
+                   unsigned batch, hipStream_t s, int nz = -1, int volz = -1);
 
 }  // namespace gx

@@ -195,12 +195,16 @@ void c2r3(const cpx<T>* in, T* out, int nx, int ny, int nz) {
 // ---------------------------------------------------------------------------
 // nz = n gives the 3D projection (Project3DArraytoNDKernel<3>), nz = 1 the 2D
 // central section (Project3DArraytoNDKernel<2>: z = 0, sphere test x^2 + y^2).
+// pz is the volume's z extent: p for a 3D volume, 1 for a 2D image's spectrum
+// (the Picker's 2D branch, which rotates a 2D reference in plane); reads past
+// the volume are 0 (the reference reads them with weight 0 when pz = 1).
 template <class T>
 void project1(const cpx<T>* vol, int p, cpx<T>* out, int n, const double* M,
-              int rmax, int nz = -1) {
+              int rmax, int nz = -1, int pz = -1) {
   if (nz < 0) nz = n;
+  if (pz < 0) pz = p;
   int ph = p / 2 + 1, nh = n / 2 + 1;
-  long long total = (long long)ph * p * p;
+  long long total = (long long)ph * p * pz;
   int rmax2 = rmax * rmax;
   long long H = (long long)nh * n * nz;
   auto at = [&](long long xx, long long yy, long long zz) -> cpx<T> {
@@ -534,29 +538,32 @@ template <class T> void norm_mean01std(const T* in, T* out, long long n) {
 //    padded -> mean0/std1 (:226), P = R2C(padded) (:227),
 //    C = C2R_u(imageft * conj(P)) (:228-229), best/angle updated where
 //    C > best (PickingUpdateKernel :249-276).
-// One cubic reference of side nr, one image nx x ny x nz, ctf of
-// (nr/2+1)*nr*nr. best/idx/top2 are updated in place (the caller fills them),
-// idx receives the index into `angles`.
+// One reference of side nr (a cube, or with nrz = 1 a square image: the 2D
+// branch, whose orientations are in-plane steps, src/Correlation/Picker.cu:163-171,
+// and whose rotation is the 2D section of the rotated 2D spectrum), one image
+// nx x ny x nz (nz = 1 with a 2D reference), ctf of (nr/2+1)*nr*nrz.
+// best/idx/top2 are updated in place (the caller fills them), idx receives the
+// index into `angles`.
 template <class T>
-void picker(const T* ref, int nr, const T* image, int nx, int ny, int nz, const T* ctf, const double* angles,
-            int nangles, T* best, int32_t* idx, T* top2) {
+void picker(const T* ref, int nr, int nrz, const T* image, int nx, int ny, int nz, const T* ctf,
+            const double* angles, int nangles, T* best, int32_t* idx, T* top2) {
   const int nrh = nr / 2 + 1;
-  const long long Vr = (long long)nr * nr * nr, Hr = (long long)nrh * nr * nr;
+  const long long Vr = (long long)nr * nr * nrz, Hr = (long long)nrh * nr * nrz;
   const long long V = (long long)nx * ny * nz, H = (long long)(nx / 2 + 1) * ny * nz;
   std::vector<T> shifted(Vr), r(Vr), padded(V), C(V), img(V);
   std::vector<cpx<T>> refft(Hr), R(Hr), imageft(H), P(H);
-  remap_full2fullfft(ref, shifted.data(), nr, nr, nr);
-  r2c3(shifted.data(), refft.data(), nr, nr, nr);
+  remap_full2fullfft(ref, shifted.data(), nr, nr, nrz);
+  r2c3(shifted.data(), refft.data(), nr, nr, nrz);
   norm_mean01std(image, img.data(), V);
   r2c3(img.data(), imageft.data(), nx, ny, nz);
   for (int o = 0; o < nangles; ++o) {
     double M[9];
     projector_matrix(angles[3 * o], angles[3 * o + 1], angles[3 * o + 2], 1.0, M);
-    project1(refft.data(), nr, R.data(), nr, M, nr / 2);
+    project1(refft.data(), nr, R.data(), nr, M, nr / 2, nrz, nrz);
     for (long long i = 0; i < Hr; ++i) R[i] *= ctf[i];
-    c2r3(R.data(), r.data(), nr, nr, nr);
+    c2r3(R.data(), r.data(), nr, nr, nrz);
     for (long long i = 0; i < Vr; ++i) r[i] /= (T)Vr;
-    full_pad(r.data(), padded.data(), nr, nr, nr, nx, ny, nz);
+    full_pad(r.data(), padded.data(), nr, nr, nrz, nx, ny, nz);
     norm_mean01std(padded.data(), padded.data(), V);
     r2c3(padded.data(), P.data(), nx, ny, nz);
     for (long long i = 0; i < H; ++i) P[i] = imageft[i] * std::conj(P[i]);
@@ -1728,9 +1735,10 @@ void orc_full_pad_f64(const double* in, double* out, int ox, int oy, int oz, int
 
 void orc_norm_mean01std_f64(const double* in, double* out, long long n) { norm_mean01std(in, out, n); }
 
-void orc_picker_f64(const double* ref, int nr, const double* image, int nx, int ny, int nz, const double* ctf,
-                    const double* angles, int nangles, double* best, int32_t* idx, double* top2) {
-  picker(ref, nr, image, nx, ny, nz, ctf, angles, nangles, best, idx, top2);
+void orc_picker_f64(const double* ref, int nr, int nrz, const double* image, int nx, int ny, int nz,
+                    const double* ctf, const double* angles, int nangles, double* best, int32_t* idx,
+                    double* top2) {
+  picker(ref, nr, nrz, image, nx, ny, nz, ctf, angles, nangles, best, idx, top2);
 }
 
 // Returns the wall time of the orientation loop in seconds.

@@ -55,7 +55,7 @@ def lib():
         L.orc_peak_one_2d_f64.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
-        L.orc_picker_f64.argtypes = [_p, _i, _p, _i, _i, _i, _p, _p, _i, _p, _p, _p]
+        L.orc_picker_f64.argtypes = [_p, _i, _i, _p, _i, _i, _i, _p, _p, _i, _p, _p, _p]
         pick_args = [_p, _i, _d, _p, _p, _i, _i, _i, _p, _i, _d, _p, _p, _p, _p, _p]
         L.orc_pick_f64.argtypes = pick_args
         L.orc_pick_f64.restype = _d
@@ -407,17 +407,20 @@ def norm_mean01std(a):
 
 def picker(ref, image, ctf, angles):
     """The Picker pipeline (Initialize, SetImage, PerformCorrelation) over an explicit
-    angle list. Returns dict(best, idx, top2) with best/top2 started at -1e30."""
+    angle list: a cubic reference with a (z, y, x) image, or a square 2D reference
+    with a (y, x) image (the 2D branch). Returns dict(best, idx, top2) with best/top2
+    started at -1e30."""
     r = _c(ref, np.float64)
-    nr = r.shape[0]
+    nr = r.shape[-1]
+    nrz = r.shape[0] if r.ndim == 3 else 1
     im = _c(image, np.float64)
-    nz, ny, nx = im.shape
+    nz, ny, nx = im.shape if im.ndim == 3 else (1,) + im.shape
     c = _c(ctf, np.float64)
     A = _c(angles, np.float64).reshape(-1, 3)
     best = np.full(im.shape, -1e30)
     top2 = np.full(im.shape, -1e30)
     idx = np.zeros(im.shape, np.int32)
-    lib().orc_picker_f64(_ptr(r), nr, _ptr(im), nx, ny, nz, _ptr(c), _ptr(A), len(A), _ptr(best), _ptr(idx),
+    lib().orc_picker_f64(_ptr(r), nr, nrz, _ptr(im), nx, ny, nz, _ptr(c), _ptr(A), len(A), _ptr(best), _ptr(idx),
                          _ptr(top2))
     return dict(best=best, idx=idx, top2=top2)
 

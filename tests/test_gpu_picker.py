@@ -113,3 +113,57 @@ def test_picker_anglestep_finds_planted_copy(gx, orc, cuda):
     pos = np.unravel_index(np.argmax(best), best.shape)
     assert pos == (4, n - 3, 6)
     assert np.array_equal(ang[pos], angles[k])
+
+
+def _blob2d(n, seed):
+    rng = np.random.default_rng(seed)
+    g = np.arange(n) - n // 2
+    y, x = np.meshgrid(g, g, indexing="ij")
+    v = np.zeros((n, n))
+    for _ in range(4):
+        c = rng.uniform(-n / 6, n / 6, 2)
+        s = rng.uniform(1.0, 2.0, 2)
+        v += np.exp(-(((x - c[0]) / s[0]) ** 2 + ((y - c[1]) / s[1]) ** 2) / 2)
+    return v.astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,batch", [((64, 48), 0), ((256, 256), 5)])
+def test_picker_2d_matches_oracle(gx, orc, cuda, shape, batch):
+    """The 2D branch (square reference, image, in-plane orientations;
+    src/Correlation/Picker.cu:163-171) against the oracle, same tolerance as 3D."""
+    nr = 16
+    ref = _blob2d(nr, 13)
+    rng = np.random.default_rng(14)
+    image = rng.standard_normal(shape).astype(np.float32)
+    ctf = rng.uniform(-1, 1, (nr, nr // 2 + 1)).astype(np.float32)
+    angles = np.array([[0.0, 0.0, a] for a in np.arange(12) * (TAU / 12)], np.float32)
+    best, ang, img_after = _run_picker(gx, cuda, ref, image, ctf, angles=angles, batch=batch)
+    ref64 = orc.picker(ref.astype(np.float64), image.astype(np.float64), ctf.astype(np.float64),
+                       angles.astype(np.float64))
+    err = rel_linf(best, ref64["best"])
+    assert err <= 1e-4, err
+    sure = (ref64["best"] - ref64["top2"]) > 1e-4 * np.max(np.abs(ref64["best"]))
+    assert sure.sum() > 0.9 * sure.size
+    assert np.array_equal(ang[sure], angles[ref64["idx"][sure]])
+    norm = orc.norm_mean01std(image.astype(np.float64))
+    assert np.max(np.abs(img_after - norm ** 2)) <= 1e-5 * np.max(norm ** 2)
+
+
+def test_picker_2d_anglestep_finds_quarter_turn(gx, orc, cuda):
+    """PerformCorrelation(anglestep = 90 degrees) on 2D data runs the in-plane steps
+    0, 90, 180, 270 degrees: an exact quarter turn of the reference (index
+    arithmetic), planted at a shift, is found there with psi = 90 degrees."""
+    nr, n = 16, 64
+    ref = _blob2d(nr, 15)
+    f = np.fft.ifftshift(ref)
+    yy, xx = np.meshgrid(np.arange(nr), np.arange(nr), indexing="ij")
+    rot = np.fft.fftshift(f[xx, (-yy) % nr])
+    img = np.zeros((n, n), np.float32)
+    img[(n - nr) // 2:(n + nr) // 2, (n - nr) // 2:(n + nr) // 2] = rot
+    img = np.roll(np.fft.ifftshift(img), (6, -9), axis=(0, 1))
+    img = (img + 0.01 * np.random.default_rng(16).standard_normal(img.shape)).astype(np.float32)
+    best, ang, _ = _run_picker(gx, cuda, ref, img, np.ones((nr, nr // 2 + 1), np.float32),
+                               anglestep=float(np.pi / 2))
+    pos = np.unravel_index(np.argmax(best), best.shape)
+    assert pos == (6, n - 9)
+    assert np.array_equal(ang[pos], np.array([0.0, 0.0, np.float32(2 * np.pi) / 4], np.float32))
