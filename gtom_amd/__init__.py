@@ -146,6 +146,12 @@ _SIGS = {
     "gtomx_pick_update_split": [_P, ctypes.c_int, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P],
     "gtomx_split_dft64": [_P, _P, _U, ctypes.c_int, _P],
     "gtomx_mpick_create": [_P, ctypes.c_int, _P, _P, _U],
+    "gtomx_exact2d_weighting": [_P, int3, _P, _P, _P, _U, _F, ctypes.c_int, _U, _P],
+    "gtomx_tomopicker_create": [_P, _P, int3, _P, ctypes.c_int, int3, _U, _P],
+    "gtomx_tomopicker_destroy": [_P],
+    "gtomx_tomopicker_set_image": [_P, _P, _P, _P, _P],
+    "gtomx_tomopicker_correlate": [_P, _P, _P, int3, _F, _P],
+    "gtomx_tomopicker_correlate_angles": [_P, _P, _U, _P, _P, int3, _P, _P, _P, _P],
     "gtomx_mpick_destroy": [_P],
     "gtomx_mpick_info": [_P, _P, _P, _P],
     "gtomx_mpick_run": [_P, _P, _P, _U, _P, _U, ctypes.c_int, _P],
@@ -958,6 +964,66 @@ def split_dft64(x, precision, stream=None):
     check(lib().gtomx_split_dft64(_ptr(x), _ptr(out), x.numel() // 64, int(precision), _stream(stream)),
           "gtomx_split_dft64")
     return out
+
+
+def d_Exact2DWeighting(weights, angles, imageweights=None, indices=None, maxfreq=None, stream=None):
+    """d_Exact2DWeighting (src/Projection/Weighting.cu:21-55) in place on float32 CUDA filters
+    (batch, ny, nx/2+1) of the images `indices` (default all) of a tilt series."""
+    batch, ny, nh = weights.shape
+    nx = 2 * (nh - 1)
+    a = _angles_arr(angles)
+    ind = np.ascontiguousarray(np.arange(batch) if indices is None else indices, np.int32)
+    check(lib().gtomx_exact2d_weighting(_ptr(weights), int3(nx, ny, 1), ind.ctypes.data, a.ctypes.data,
+                                        _ptr(imageweights), a.shape[0], float(nx if maxfreq is None else maxfreq), 0,
+                                        batch, _stream(stream)), "gtomx_exact2d_weighting")
+    return weights
+
+
+class TomoPicker:
+    """The TomoPicker class (include/Correlation.cuh:209-257): initialize with a cubic reference and
+    mask (z, y, x) and the tilt-image shape (ny, nx) and count; set_image; perform_correlation."""
+
+    def __init__(self, ref, mask, imageshape, nimages, circular=True, stream=None):
+        nr = ref.shape[-1]
+        ny, nx = imageshape
+        h = _P()
+        check(lib().gtomx_tomopicker_create(ctypes.byref(h), _ptr(ref), int3(nr, nr, nr), _ptr(mask),
+                                            int(bool(circular)), int3(nx, ny, 1), int(nimages), _stream(stream)),
+              "gtomx_tomopicker_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().gtomx_tomopicker_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_image(self, images, tilt_angles, weights, stream=None):
+        """images (nimages, ny, nx) float32 CUDA, weighted in place as in the reference."""
+        a = _angles_arr(tilt_angles)
+        w = np.ascontiguousarray(weights, np.float32)
+        check(lib().gtomx_tomopicker_set_image(self.handle, _ptr(images), a.ctypes.data, w.ctypes.data,
+                                               _stream(stream)), "gtomx_tomopicker_set_image")
+
+    def perform_correlation(self, corr, corrangle=None, anglestep=None, angles=None, index=None, sum1=None,
+                            sum2=None, stream=None):
+        """Updates corr (z, y, x) (and corrangle (z, y, x, 3), index, sum1, sum2 when given) over
+        GetEqualAngularSpacing(anglestep) or an explicit orientation list."""
+        cz, cy, cx = corr.shape
+        if angles is not None:
+            a = _angles_arr(angles)
+            check(lib().gtomx_tomopicker_correlate_angles(self.handle, a.ctypes.data, a.shape[0], _ptr(corr),
+                                                          _ptr(corrangle), int3(cx, cy, cz), _ptr(index), _ptr(sum1),
+                                                          _ptr(sum2), _stream(stream)),
+                  "gtomx_tomopicker_correlate_angles")
+        else:
+            check(lib().gtomx_tomopicker_correlate(self.handle, _ptr(corr), _ptr(corrangle), int3(cx, cy, cz),
+                                                   float(anglestep), _stream(stream)), "gtomx_tomopicker_correlate")
 
 
 class Micrograph(ctypes.Structure):

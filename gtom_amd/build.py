@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgtomx.so")
-SOURCES = ["runtime.cpp", "fft.hip", "ops.hip", "pick.hip", "peaks.hip", "picker.hip", "mrc.hip", "diff2.hip", "peak_subvoxel.hip", "rotate.hip", "norm.hip", "resize.hip", "probpicker.hip", "learn.hip", "split.hip", "mpick.hip"]
+SOURCES = ["runtime.cpp", "fft.hip", "ops.hip", "pick.hip", "peaks.hip", "picker.hip", "mrc.hip", "diff2.hip", "peak_subvoxel.hip", "rotate.hip", "norm.hip", "resize.hip", "probpicker.hip", "learn.hip", "split.hip", "mpick.hip", "tomopicker.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GTOMX_ARCH", "gfx950")

@@ -344,7 +344,9 @@ int gtomx_norm_mean01std(const float* d_input, float* d_output, size_t elements,
 typedef struct gtomx_picker gtomx_picker;
 
 /* Picker::Initialize, include/Correlation.cuh:161 / src/Correlation/Picker.cu:30-116:
- * a cubic 3D reference d_ref (centred, real) for images of dimsimage. The
+ * a cubic 3D reference d_ref (centred, real) for volumes of dimsimage, or a
+ * square 2D reference (dimsref.z = 1) for images (dimsimage.z = 1; the
+ * reference's ndims == 2 branch, whose orientations are in-plane steps). The
  * reference is transformed once (R2C of its ifftshift). d_refmask is accepted
  * and ignored: the reference prepares it but never uses it
  * (src/Correlation/Picker.cu:56-88, 193-221). batch = orientations per pass
@@ -362,7 +364,8 @@ int gtomx_picker_set_image(gtomx_picker* picker, float* d_image, const float* d_
 
 /* Picker::PerformCorrelation, include/Correlation.cuh:163 /
  * src/Correlation/Picker.cu:158-247: for every orientation of
- * GetEqualAngularSpacing((0, 2pi), (0, pi), (0, 2pi), anglestep), with the
+ * GetEqualAngularSpacing((0, 2pi), (0, pi), (0, 2pi), anglestep) (2D: the
+ * ceil(2 pi / anglestep) in-plane steps (0, 0, k 2 pi / steps), :165-171), with the
  * rotation the reference left commented out done by the device-array
  * projector at the reference size:
  *   C = C2R(imageft * conj(R2C(norm(FullPad(C2R_n(project(refft) * ctf)))))),
@@ -374,6 +377,48 @@ int gtomx_picker_correlate(gtomx_picker* picker, float anglestep, float* d_bestc
 /* The same over an explicit orientation list (no reference counterpart). */
 int gtomx_picker_correlate_angles(gtomx_picker* picker, const gtomx_float3* h_angles, unsigned nangles,
                                   float* d_bestccf, gtomx_float3* d_bestangle, void* stream);
+
+/* d_Exact2DWeighting, include/Projection.cuh / src/Projection/Weighting.cu:21-55
+ * (kernel :92-133): multiplies batch half-spectrum filters (n/2+1 x n each, FFT
+ * order; zero-centred weights are refused) of the images h_indices[b] of a tilt
+ * series with (rot, tilt, psi) h_angles by w_b^2 / sum_k |sinc(A . normal_k)| w_k
+ * inside |f| <= maxfreq and 0 outside (d_imageweights NULL: all 1). Synchronous. */
+int gtomx_exact2d_weighting(float* d_weights, gtomx_int3 dimsimage, const int* h_indices,
+                            const gtomx_float3* h_angles, const float* d_imageweights, unsigned nimages,
+                            float maxfreq, int iszerocentered, unsigned batch, void* stream);
+
+/* The TomoPicker class, include/Correlation.cuh:209-257 / src/Correlation/TomoPicker.cu.
+ * The reference's constructor and Initialize throw and its tilt projections and
+ * correlation call are marked broken (TomoPicker.cu:23, 28, 223-229, 264-269, 297);
+ * this object runs the pipeline its code describes, with 2D central sections of
+ * the padded (2 nr)^3 spectra from the device-array projector and, per tilt
+ * image, the masked locally normalised correlation of d_CorrelateRealspace
+ * (src/Correlation/Realspace.cu:16-102) at every image position.
+ * create = Initialize (:26-104): reference and mask nr^3 (centred), circular mask
+ * (projected once) or not (every orientation), images nx x ny, 1..128 of them. */
+typedef struct gtomx_tomopicker gtomx_tomopicker;
+int gtomx_tomopicker_create(gtomx_tomopicker** picker, const float* d_ref, gtomx_int3 dimsref,
+                            const float* d_refmask, int ismaskcircular, gtomx_int3 dimsimage,
+                            unsigned nimages, void* stream);
+int gtomx_tomopicker_destroy(gtomx_tomopicker* picker);
+/* SetImage (:158-186): d_image (nimages x ny x nx) is weighted by
+ * d_Exact2DWeighting(h_imageangles, h_imageweights, maxfreq = nx) in place, as in
+ * the reference; its spectrum and the spectrum of its square are kept. */
+int gtomx_tomopicker_set_image(gtomx_tomopicker* picker, float* d_image, const gtomx_float3* h_imageangles,
+                               const float* h_imageweights, void* stream);
+/* PerformCorrelation (:188-305) over GetEqualAngularSpacing(anglestep):
+ * d_corr (dimscorr, caller-initialised) keeps, with strict '<', the maximum over
+ * orientations of each voxel's weighted mean of the tilt correlation maps at
+ * its projections (UpdateTomoCorrKernel, :311-358), d_corrangle the triple. */
+int gtomx_tomopicker_correlate(gtomx_tomopicker* picker, float* d_corr, gtomx_float3* d_corrangle,
+                               gtomx_int3 dimscorr, float anglestep, void* stream);
+/* The same over an explicit orientation list, with the optional orientation
+ * index and the sums of the mean and of its square over orientations (the
+ * reference's d_corrsum1/2, :202-203); any output but d_corr may be NULL. */
+int gtomx_tomopicker_correlate_angles(gtomx_tomopicker* picker, const gtomx_float3* h_angles,
+                                      unsigned nangles, float* d_corr, gtomx_float3* d_corrangle,
+                                      gtomx_int3 dimscorr, int32_t* d_corrindex, float* d_corrsum1,
+                                      float* d_corrsum2, void* stream);
 
 typedef struct gtomx_prob_picker gtomx_prob_picker;
 

@@ -580,6 +580,241 @@ void picker(const T* ref, int nr, int nrz, const T* image, int nx, int ny, int n
   }
 }
 
+void fft_resize(int op, const double* in, double* out, const int* od, const int* nd, int ncomp);
+
+// ---------------------------------------------------------------------------
+// TomoPicker (include/Correlation.cuh:209-257, src/Correlation/TomoPicker.cu)
+// as its code describes, with the projections its author left broken done by
+// the device-array projector (src/Relion/Project.cu:89-177, 2D sections):
+// ---------------------------------------------------------------------------
+// d_Exact2DWeighting (src/Projection/Weighting.cu:21-55, kernel :92-133) for
+// images in FFT order (iszerocentered false), maxfreq in pixels: filter[i]
+// (n/2+1 x n half spectrum per image, multiplied in place) at frequency f is
+// w_i^2 / sum_b |sinc(dot(A, normal_b))| w_b, A = the frequency -f of the
+// kernel's loop written in the image's tilt frame (rows 0 and 1 of
+// Matrix3Euler(angle_i) as the local axes), normal_b = row 2 of
+// Matrix3Euler(angle_b); 0 where |f| > maxfreq. sinc(x) = sin(pi x)/(pi x),
+// 1 for |x| <= 1e-8 (include/DeviceFunctions.cuh:19-25).
+void exact2d_weighting(double* w, int nx, int ny, const double* angles, const double* weights, int nimages,
+                       const int* indices, int batch, double maxfreq) {
+  std::vector<double> normals(3 * (size_t)nimages), B(9 * (size_t)nimages);
+  for (int i = 0; i < nimages; ++i) {
+    euler_matrix(angles[3 * i], angles[3 * i + 1], angles[3 * i + 2], &B[9 * (size_t)i]);
+    for (int c = 0; c < 3; ++c) normals[3 * (size_t)i + c] = B[9 * (size_t)i + 6 + c];
+  }
+  const int nh = nx / 2 + 1;
+  for (int b = 0; b < batch; ++b) {
+    const int index = indices[b];
+    const double* E = &B[9 * (size_t)index];
+    double* out = w + (size_t)b * nh * ny;
+    for (int idy = 0; idy < ny; ++idy) {
+      const int y = ny - 1 - ((idy + ny / 2 - 1) % ny);
+      for (int idx = 0; idx < nh; ++idx) {
+        const int x = nx / 2 - idx;
+        const double ax = (double)idx - nx / 2, ay = (double)idy - ny / 2;
+        double& dst = out[(size_t)y * nh + x];
+        if (std::sqrt(ax * ax + ay * ay) <= maxfreq) {
+          // localA * globalB2localB: the vector ax * row0 + ay * row1 of the image's Euler matrix
+          const double g[3] = {ax * E[0] + ay * E[3], ax * E[1] + ay * E[4], ax * E[2] + ay * E[5]};
+          double sum = 0;
+          for (int k = 0; k < nimages; ++k) {
+            const double d = g[0] * normals[3 * k] + g[1] * normals[3 * k + 1] + g[2] * normals[3 * k + 2];
+            const double sn = std::fabs(d) <= 1e-8 ? 1.0 : std::sin(d * M_PI) / (d * M_PI);
+            sum += std::fabs(sn) * weights[k];
+          }
+          dst *= weights[b] * weights[b] / sum;
+        } else {
+          dst *= 0.0;
+        }
+      }
+    }
+  }
+}
+
+// The whole TomoPicker in double. ref and mask: nr^3 (centred); images: nimages
+// x ny x nx, filtered in place as SetImage leaves them; tilt: nimages (rot,
+// tilt, psi); weights: nimages; angles: the search orientations. corr (dimscorr,
+// caller-initialised) keeps the maximum over orientations of the weighted mean
+// of the tilt images' correlation maps sampled at each voxel's projection,
+// with strict '<' (UpdateTomoCorrKernel, :311-358), corridx the orientation
+// index, corrsum1/2 the sums of that mean and its square over orientations,
+// corr2 (optional) the second-best value.
+// The correlation of one tilt image with one projected reference (the call the
+// reference marks BROKEN, :297) is the masked, locally normalised correlation
+// of d_CorrelateRealspace (src/Correlation/Realspace.cu:16-102) at every image
+// position x:  sum_m mask(m) ref(m) img(x + m) / (M sigma(x)), with ref
+// normalised to mean 0, std 1 under the binarised mask (d_NormMonolithic with a
+// mask, src/ImageManipulation/Norm.cu:286-333), M = sum mask, sigma(x) =
+// sqrt(max(0, M s2 - s1^2)) / M from the local sums s1, s2 of img and img^2
+// under the mask (:255-263), and 0 where sigma is not positive.
+void tomo_picker(const double* ref, const double* mask, int nr, bool circular, double* images, int nx, int ny,
+                 int nimages, const double* tilt, const double* weights, const double* angles, int nangles,
+                 int cx, int cy, int cz, double* corr, int32_t* corridx, double* corrsum1, double* corrsum2,
+                 double* corr2) {
+  const int p = 2 * nr, ph = p / 2 + 1;
+  const long long Vp3 = (long long)p * p * p, Hp3 = (long long)ph * p * p;
+  const long long Vp = (long long)p * p, Hp = (long long)ph * p, Vr = (long long)nr * nr;
+  const long long Vi = (long long)nx * ny, Hi = (long long)(nx / 2 + 1) * ny;
+  // Initialize: pad (centre p/2 <- nr/2), ifftshift, R2C (TomoPicker.cu:80-103)
+  auto prepare = [&](const double* v, std::vector<cpx<double>>& ft) {
+    std::vector<double> padded(Vp3, 0.0), shifted(Vp3);
+    const int off = p / 2 - nr / 2;
+    for (int z = 0; z < nr; ++z)
+      for (int y = 0; y < nr; ++y)
+        for (int x = 0; x < nr; ++x)
+          padded[((size_t)(z + off) * p + y + off) * p + x + off] = v[((size_t)z * nr + y) * nr + x];
+    remap_full2fullfft(padded.data(), shifted.data(), p, p, p);
+    ft.resize(Hp3);
+    r2c3(shifted.data(), ft.data(), p, p, p);
+  };
+  std::vector<cpx<double>> Pref, Pmask;
+  prepare(ref, Pref);
+  prepare(mask, Pmask);
+  // SetImage (:158-186): filter, filtered images, their spectra and the spectra of their squares
+  std::vector<double> filter(Hi * nimages, 1.0);
+  std::vector<int> ind(nimages);
+  for (int i = 0; i < nimages; ++i) ind[i] = i;
+  exact2d_weighting(filter.data(), nx, ny, tilt, weights, nimages, ind.data(), nimages, (double)nx);
+  std::vector<cpx<double>> imageft(Hi * nimages), image2ft(Hi * nimages);
+  std::vector<double> sq(Vi);
+  for (int i = 0; i < nimages; ++i) {
+    cpx<double>* F = &imageft[Hi * i];
+    double* im = images + Vi * i;
+    r2c3(im, F, nx, ny, 1);
+    for (long long k = 0; k < Hi; ++k) F[k] *= filter[Hi * i + k];
+    c2r3(F, im, nx, ny, 1);
+    for (long long k = 0; k < Vi; ++k) im[k] /= (double)Vi;
+    r2c3(im, F, nx, ny, 1);  // the spectrum of the image as stored (the product, Hermitian-projected)
+    for (long long k = 0; k < Vi; ++k) sq[k] = im[k] * im[k];
+    r2c3(sq.data(), &image2ft[Hi * i], nx, ny, 1);
+  }
+  // PerformCorrelation (:188-305)
+  std::vector<double> mask2d(Vr * nimages), masksum(nimages), s1(Vi * nimages), s2(Vi * nimages);
+  std::vector<double> imagecorr(Vi * nimages);
+  std::vector<cpx<double>> sec(Hp), tmpft(Hi);
+  std::vector<double> full(Vp), padded(Vi), num(Vi);
+  const int od[3] = {p, p, 1}, nd[3] = {nr, nr, 1}, id3[3] = {nr, nr, 1}, pd3[3] = {nx, ny, 1};
+  std::vector<double> M(9 * (size_t)nimages), T(6 * (size_t)nimages);
+  for (int i = 0; i < nimages; ++i) {
+    double A[9];
+    euler_matrix(tilt[3 * i], tilt[3 * i + 1], tilt[3 * i + 2], A);
+    for (int c = 0; c < 3; ++c) {
+      T[6 * (size_t)i + c] = A[c];          // row 0
+      T[6 * (size_t)i + 3 + c] = A[3 + c];  // row 1
+    }
+  }
+  for (long long v = 0; v < (long long)cx * cy * cz; ++v) corrsum1[v] = corrsum2[v] = 0.0;
+  for (int a = 0; a < nangles; ++a) {
+    double G[9];
+    euler_matrix(angles[3 * a], angles[3 * a + 1], angles[3 * a + 2], G);
+    for (int i = 0; i < nimages; ++i) {
+      double B[9];
+      euler_matrix(tilt[3 * i], tilt[3 * i + 1], tilt[3 * i + 2], B);
+      // transpose(Euler(angle)) * transpose(Euler(tilt_i))
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double acc = 0;
+          for (int k = 0; k < 3; ++k) acc += G[3 * k + r] * B[3 * c + k];
+          M[9 * (size_t)i + 3 * r + c] = acc;
+        }
+    }
+    if (a == 0 || !circular) {
+      for (int i = 0; i < nimages; ++i) {
+        project1(Pmask.data(), p, sec.data(), p, &M[9 * (size_t)i], p / 2, 1);
+        c2r3(sec.data(), full.data(), p, p, 1);
+        for (long long k = 0; k < Vp; ++k) full[k] /= (double)Vp;
+        double* m2 = &mask2d[Vr * i];
+        fft_resize(3, full.data(), m2, od, nd, 1);
+        double ms = 0;
+        for (long long k = 0; k < Vr; ++k) {
+          m2[k] = m2[k] >= 0.99 ? 1.0 : 0.0;  // d_Binarize (src/BinaryManipulation/Binarize.cu:30-36)
+          ms += m2[k];
+        }
+        masksum[i] = ms;
+        fft_resize(4, m2, padded.data(), id3, pd3, 1);
+        r2c3(padded.data(), tmpft.data(), nx, ny, 1);
+        std::vector<cpx<double>> prod(Hi);
+        for (long long k = 0; k < Hi; ++k) prod[k] = imageft[Hi * i + k] * std::conj(tmpft[k]);
+        c2r3(prod.data(), &s1[Vi * i], nx, ny, 1);
+        for (long long k = 0; k < Hi; ++k) prod[k] = image2ft[Hi * i + k] * std::conj(tmpft[k]);
+        c2r3(prod.data(), &s2[Vi * i], nx, ny, 1);
+        for (long long k = 0; k < Vi; ++k) {
+          s1[Vi * i + k] /= (double)Vi;
+          s2[Vi * i + k] /= (double)Vi;
+        }
+      }
+    }
+    for (int i = 0; i < nimages; ++i) {
+      project1(Pref.data(), p, sec.data(), p, &M[9 * (size_t)i], p / 2, 1);
+      c2r3(sec.data(), full.data(), p, p, 1);  // not renormalised (the 3-argument d_IFFTC2R, :272)
+      std::vector<double> r(Vr);
+      fft_resize(3, full.data(), r.data(), od, nd, 1);
+      const double* m2 = &mask2d[Vr * i];
+      double a1 = 0, a2 = 0, sm = 0;
+      for (long long k = 0; k < Vr; ++k) {
+        a1 += r[k] * m2[k];
+        a2 += r[k] * r[k] * m2[k];
+        sm += m2[k];
+      }
+      const double mean = a1 / sm, sd = std::sqrt(sm * a2 - a1 * a1) / sm;
+      const double isd = sd > 0 ? 1.0 / sd : 0.0;
+      for (long long k = 0; k < Vr; ++k) r[k] = (r[k] - mean) * isd * m2[k];
+      fft_resize(4, r.data(), padded.data(), id3, pd3, 1);
+      r2c3(padded.data(), tmpft.data(), nx, ny, 1);
+      std::vector<cpx<double>> prod(Hi);
+      for (long long k = 0; k < Hi; ++k) prod[k] = imageft[Hi * i + k] * std::conj(tmpft[k]);
+      c2r3(prod.data(), num.data(), nx, ny, 1);
+      const double Mi = masksum[i];
+      for (long long k = 0; k < Vi; ++k) {
+        const double a = s1[Vi * i + k], b = s2[Vi * i + k];
+        const double sig = std::sqrt(std::max(0.0, Mi * b - a * a)) / Mi;
+        imagecorr[Vi * i + k] = sig > 0 ? num[k] / (double)Vi / (Mi * sig) : 0.0;
+      }
+    }
+    // back-projection (UpdateTomoCorrKernel, :311-358): bilinear interpolation of
+    // the linear-filtered texture with clamped addressing, texel centres at +0.5
+    for (int z = 0; z < cz; ++z)
+      for (int y = 0; y < cy; ++y)
+        for (int x = 0; x < cx; ++x) {
+          const double pos[3] = {(double)x - cx / 2, (double)y - cy / 2, (double)z - cz / 2};
+          double sum = 0, samples = 0;
+          for (int n = 0; n < nimages; ++n) {
+            const double* Tn = &T[6 * (size_t)n];
+            const double px = Tn[0] * pos[0] + Tn[1] * pos[1] + Tn[2] * pos[2] + (nx / 2 + 0.5);
+            const double py = Tn[3] * pos[0] + Tn[4] * pos[1] + Tn[5] * pos[2] + (ny / 2 + 0.5);
+            if (px >= 0 && px <= nx && py >= 0 && py <= ny) {
+              const double u = px - 0.5, v = py - 0.5;
+              const double fu = std::floor(u), fv = std::floor(v);
+              const double tu = u - fu, tv = v - fv;
+              auto at = [&](long long xi, long long yi) {
+                xi = std::min<long long>(std::max<long long>(xi, 0), nx - 1);
+                yi = std::min<long long>(std::max<long long>(yi, 0), ny - 1);
+                return imagecorr[Vi * n + yi * nx + xi];
+              };
+              const long long x0 = (long long)fu, y0 = (long long)fv;
+              const double val = (1 - tv) * ((1 - tu) * at(x0, y0) + tu * at(x0 + 1, y0)) +
+                                 tv * ((1 - tu) * at(x0, y0 + 1) + tu * at(x0 + 1, y0 + 1));
+              sum += val * weights[n];
+              samples += weights[n];
+            }
+          }
+          if (samples > 0) {
+            sum /= samples;
+            const size_t o = ((size_t)z * cy + y) * cx + x;
+            if (corr[o] < sum) {
+              if (corr2) corr2[o] = corr[o];
+              corr[o] = sum;
+              corridx[o] = a;
+            } else if (corr2 && corr2[o] < sum) {
+              corr2[o] = sum;
+            }
+            corrsum1[o] += sum;
+            corrsum2[o] += sum * sum;
+          }
+        }
+  }
+}
+
 // d_PickSubTomogramsDiff2 (src/Correlation/SubTomograms.cu:165-209, kernel
 // PickSubTomogramsDiff2Kernel :211-313), literally in double: for every voxel
 // (ix, iy, iz) of dimsrelevant the shift
@@ -1742,6 +1977,19 @@ void orc_picker_f64(const double* ref, int nr, int nrz, const double* image, int
 }
 
 // Returns the wall time of the orientation loop in seconds.
+void orc_exact2d_weighting_f64(double* w, int nx, int ny, const double* angles, const double* weights, int nimages,
+                               const int* indices, int batch, double maxfreq) {
+  exact2d_weighting(w, nx, ny, angles, weights, nimages, indices, batch, maxfreq);
+}
+
+void orc_tomo_picker_f64(const double* ref, const double* mask, int nr, int circular, double* images, int nx, int ny,
+                         int nimages, const double* tilt, const double* weights, const double* angles, int nangles,
+                         int cx, int cy, int cz, double* corr, int32_t* corridx, double* corrsum1, double* corrsum2,
+                         double* corr2) {
+  tomo_picker(ref, mask, nr, circular != 0, images, nx, ny, nimages, tilt, weights, angles, nangles, cx, cy, cz, corr,
+              corridx, corrsum1, corrsum2, corr2);
+}
+
 double orc_pick_f64(const double* projft, int p, double oversample, const double* expft,
                     const double* ctf, int ctf_per_volume, int n, int nvol,
                     const double* angles, int nangles, double maskradius, double* best_out,

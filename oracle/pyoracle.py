@@ -56,6 +56,9 @@ def lib():
         L.orc_subpixel_max_f64.argtypes = [_p, _p, _i, _i, _i, _i]
         L.orc_norm_mean01std_f64.argtypes = [_p, _p, _ll]
         L.orc_picker_f64.argtypes = [_p, _i, _i, _p, _i, _i, _i, _p, _p, _i, _p, _p, _p]
+        L.orc_exact2d_weighting_f64.argtypes = [_p, _i, _i, _p, _p, _i, _p, _i, _d]
+        L.orc_tomo_picker_f64.argtypes = [_p, _p, _i, _i, _p, _i, _i, _i, _p, _p, _p, _i, _i, _i, _i, _p, _p, _p,
+                                          _p, _p]
         pick_args = [_p, _i, _d, _p, _p, _i, _i, _i, _p, _i, _d, _p, _p, _p, _p, _p]
         L.orc_pick_f64.argtypes = pick_args
         L.orc_pick_f64.restype = _d
@@ -403,6 +406,43 @@ def norm_mean01std(a):
     out = np.empty_like(v)
     lib().orc_norm_mean01std_f64(_ptr(v), _ptr(out), v.size)
     return out
+
+
+def exact2d_weighting(nx, ny, angles, weights=None, indices=None, maxfreq=None, start=None):
+    """d_Exact2DWeighting (src/Projection/Weighting.cu:21-55) for FFT-order images: the
+    (batch, ny, nx/2+1) filters of the images `indices` (default all) of a tilt series with
+    (rot, tilt, psi) `angles`, multiplied into `start` (default ones)."""
+    A = _c(angles, np.float64).reshape(-1, 3)
+    n = len(A)
+    w = _c(np.ones(n) if weights is None else weights, np.float64)
+    ind = np.ascontiguousarray(np.arange(n) if indices is None else indices, np.int32)
+    out = _c(np.ones((len(ind), ny, nx // 2 + 1)) if start is None else start, np.float64).copy()
+    lib().orc_exact2d_weighting_f64(_ptr(out), nx, ny, _ptr(A), _ptr(w), n, _ptr(ind), len(ind),
+                                    float(nx if maxfreq is None else maxfreq))
+    return out
+
+
+def tomo_picker(ref, mask, circular, images, tilt, weights, angles, dimscorr, corr=None):
+    """The TomoPicker pipeline (Initialize, SetImage, PerformCorrelation over `angles`).
+    images (nimages, ny, nx) are returned filtered as SetImage leaves them. Returns
+    dict(corr, idx, sum1, sum2, images); corr starts at -1e30 unless given."""
+    r = _c(ref, np.float64)
+    m = _c(mask, np.float64)
+    nr = r.shape[-1]
+    im = _c(images, np.float64).copy()
+    nimg, ny, nx = im.shape
+    T = _c(tilt, np.float64).reshape(-1, 3)
+    w = _c(weights, np.float64)
+    A = _c(angles, np.float64).reshape(-1, 3)
+    cx, cy, cz = dimscorr
+    c = np.full((cz, cy, cx), -1e30) if corr is None else _c(corr, np.float64).copy()
+    idx = np.full((cz, cy, cx), -1, np.int32)
+    s1 = np.zeros((cz, cy, cx))
+    s2 = np.zeros((cz, cy, cx))
+    c2 = np.full((cz, cy, cx), -1e30)
+    lib().orc_tomo_picker_f64(_ptr(r), _ptr(m), nr, int(bool(circular)), _ptr(im), nx, ny, nimg, _ptr(T), _ptr(w),
+                              _ptr(A), len(A), cx, cy, cz, _ptr(c), _ptr(idx), _ptr(s1), _ptr(s2), _ptr(c2))
+    return dict(corr=c, idx=idx, sum1=s1, sum2=s2, images=im, corr2=c2)
 
 
 def picker(ref, image, ctf, angles):
