@@ -42,8 +42,14 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--search", choices=["plain", "learned"], default="plain",
+                    help="learned: the learned-proposal search against the exhaustive one on held-out blocks of the "
+                         "learned configuration (one JSON line of its own)")
+    ap.add_argument("--learned-blocks", type=int, default=4,
+                    help="held-out blocks timed by the learned comparison")
     ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
-                                      "prob_picker_micrograph", "pick2d_micrograph"], default=None,
+                                      "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split"],
+                    default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
 
@@ -169,6 +175,54 @@ def cpu_baseline(w, target_seconds):
                       f"orientation loop {r['loop_seconds']:.2f} s"}
 
 
+def learned_comparison(nblocks):
+    """The learned search (gtomx_pick_subtomograms_learned with the committed trained network)
+    against the exhaustive search on held-out blocks of the learned configuration (256^3, 48^3
+    template, 9000 orientations, SNR 0.3): per block both are timed end to end on the host clock
+    (the learned time includes the network and its one synchronisation), and the exhaustive
+    search's peaks (d_LocalPeaks above the workload's threshold) are recalled when the learned
+    map holds the same value and orientation there. None without trained weights."""
+    import torch
+    from gtom_amd import proposal as pr
+    from gtom_amd import train_proposal as tp
+    if not os.path.exists(tp.WEIGHTS):
+        log("no trained proposal network committed; learned comparison skipped")
+        return None
+    w = tp.Workload()
+    net = pr.ProposalNet(max_tiles=512, params=np.load(tp.WEIGHTS)["params"])
+    seeds = [tp.HELDOUT_SEEDS[0] + k for k in range(nblocks + 1)]
+    tp.recall(w, net, seeds[:1])  # warm-up (plans, kernels)
+    res = tp.recall(w, net, seeds[1:])
+    net.close()
+    ex = sum(r["exhaustive_s"] for r in res)
+    le = sum(r["learned_s"] for r in res)
+    peaks = sum(r["peaks"] for r in res)
+    found = sum(r["found"] for r in res)
+    vo = len(res) * tp.N ** 3 * tp.NDIRS * tp.NPSI
+    stats = {k: sum(r["stats"][k] for r in res) for k in res[0]["stats"]}
+    torch.cuda.synchronize()
+    return {"workload": f"{len(res)} held-out 256^3 blocks (seeds {seeds[1]}..{seeds[-1]}), 20 copies of the "
+                        f"48^3 template at SNR {tp.SNR}, 9000 orientations, top-16 directions per 32^3 tile, "
+                        "exhaustive fallback below confidence 0.5",
+            "exhaustive_vo_per_s": vo / ex, "learned_exhaustive_equivalent_vo_per_s": vo / le,
+            "learned_over_plain": ex / le, "target_ratio": 5.0, "recall": found / max(peaks, 1),
+            "peaks": peaks, "recalled": found, "tiles": stats, "exhaustive_s": round(ex, 3),
+            "learned_s": round(le, 3)}
+
+
+def row_learned(args):
+    cmp_ = learned_comparison(args.learned_blocks)
+    if cmp_ is None:
+        return 1
+    out = {"row": "learned", "metric": METRIC, "value": cmp_["learned_exhaustive_equivalent_vo_per_s"],
+           "unit": "voxel-orientations/s (exhaustive-equivalent)", "n_gpus": 1, "steps": args.learned_blocks,
+           "warmup": 1, "ms_per_step": cmp_["learned_s"] * 1e3 / args.learned_blocks, "higher_is_better": True,
+           "dtype": "f32 search, bf16 network", "data": "synthetic", "config": {"workload": cmp_["workload"]},
+           "learned": cmp_}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def _blob(box, rng, n=10):
     g = np.arange(box) - box // 2
     z, y, x = np.meshgrid(g, g, g, indexing="ij")
@@ -195,10 +249,12 @@ def _timed_steps(args, step):
     return (time.perf_counter() - t0) / args.steps
 
 
-def _row_line(args, row, metric, unit, work, sec, config, cpu):
+def _row_line(args, row, metric, unit, work, sec, config, cpu, extra=None):
     out = {"row": row, "metric": metric, "value": work / sec, "unit": unit, "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "dtype": "f32",
            "data": "synthetic", "config": config}
+    if extra:
+        out.update(extra)
     if cpu is not None and not args.no_cpu_baseline:
         from oracle import pyoracle as po
         cw, ct, sample = cpu()
@@ -319,6 +375,175 @@ def row_pick2d_micrograph(args):
                      "pixel-orientations/s", work, sec,
                      {"workload": "d_PickSubTomograms (2D)", "images": nimg, "image": [n, n], "template": [box] * 3,
                       "orientations": len(angles)}, cpu)
+
+
+def micrograph_workload(seed, nmic=64, sides=(1024, 2048, 3072, 4096), box=64, copies=200, snr=0.02):
+    """The uneven micrograph batch: nmic micrographs of sides drawn uniformly from `sides`, each
+    with its own astigmatic CTF (defocus U(0.8, 3.5) um, astigmatism U(0, 100) nm at a random
+    angle, 1.0 A/px, 300 kV, Cs 2.7 mm) and `copies` planted projections of a box^3 random-blob
+    reference (projections at random orientations through the reference's own projector), the
+    signal scaled for `snr` (signal variance inside the support of the projection over the noise
+    variance, before the CTF) and CTF-filtered, in unit Gaussian noise. Returns (projector, p, image dicts, sides)."""
+    import torch
+    import gtom_amd as gx
+    from gtom_amd import synth
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    p = 2 * box
+    t = synth.blob_template(box, np.random.default_rng(seed + 1), radius=min(14.0, box / 2 - 6))
+    projft = gx.prepare_projector(torch.from_numpy(t).to(dev), p)
+    amp = synth.snr_amplitude(t.sum(axis=0) / np.sqrt(box), snr)
+    ns = [int(rng.choice(sides)) for _ in range(nmic)]
+    images = []
+    gen = torch.Generator(device=dev)
+    for k, n in enumerate(ns):
+        angs = np.array([synth.random_orientation(rng) for _ in range(copies)], np.float32)
+        sec = gx.d_rlnProject(projft, p, p, angles=angs, ndims=2)  # (copies, p, p/2+1) central sections
+        proj = torch.fft.fftshift(torch.fft.irfft2(sec, s=(p, p)), dim=(-2, -1)) / np.sqrt(box)
+        sig = torch.zeros((n, n), device=dev)
+        pos = rng.integers(0, n - p, (copies, 2))
+        for c in range(copies):
+            sig[pos[c, 1]:pos[c, 1] + p, pos[c, 0]:pos[c, 0] + p] += proj[c]
+        par = gx.CtfParamsFull.make(pixelsize=1e-10, Cs=2.7e-3, voltage=300e3,
+                                    defocus=-float(rng.uniform(0.8e-6, 3.5e-6)),
+                                    defocusdelta=float(rng.uniform(0, 100e-9)),
+                                    astigmatismangle=float(rng.uniform(0, np.pi)), amplitude=0.07)
+        ctf = gx.ctf_2d([par], n, device=dev)[0].contiguous()
+        sig = torch.fft.irfft2(torch.fft.rfft2(sig) * ctf, s=(n, n)) * amp
+        gen.manual_seed(seed * 1000 + k)
+        img = sig + torch.randn((n, n), generator=gen, device=dev)
+        E = gx.d_FFTR2C(img[None], ndims=2)[0].contiguous()
+        del img, sig
+        images.append({"spectrum": E, "ctf": ctf, "maskradius": n / 8, "best": torch.empty((n, n), device=dev),
+                       "index": torch.empty((n, n), dtype=torch.int32, device=dev)})
+    return projft, p, images, ns
+
+
+def row_micrographs(args):
+    """Batching: 64 micrographs of uneven sides (1024^2 .. 4096^2), each with its own astigmatic
+    CTF, 200 planted projections of a 64^3 random-blob reference at SNR 0.02, searched over 4,000
+    orientations (500 Fibonacci directions x 8 in-plane steps) in one gtomx_mpick_run: one plan per
+    side with the autotuner's batch (gtom_amd/tuning/micrograph_batches.json), each side's
+    orientation loop captured once as a hipGraph and replayed per micrograph."""
+    import torch
+    import gtom_amd as gx
+    from gtom_amd import tune_micrographs as tm
+    from oracle import pyoracle as po
+    t0 = time.time()
+    projft, p, images, ns = micrograph_workload(args.seed)
+    log(f"micrograph batch built in {time.time() - t0:.1f} s: sides {sorted(set(ns))}, "
+        f"{sum(n * n for n in ns):.3g} pixels")
+    table = tm.load_table()
+    sides = sorted(set(ns))
+    batches = [table.get(n, 0) for n in sides]
+    mb = gx.MicrographBatch(p, sides, batches)
+    angles = gx.fibonacci_angles(500, 8)
+    stream = torch.cuda.current_stream()
+    sec = _timed_steps(args, lambda: mb.run(projft, images, angles, use_graphs=True, stream=stream))
+    info = mb.info()
+    # graphs against eager launches on the smallest side, where launch overhead weighs most
+    small = [im for im, n in zip(images, ns) if n == sides[0]][:4]
+    mb.run(projft, small, angles, use_graphs=False, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mb.run(projft, small, angles, use_graphs=False, stream=stream)
+    torch.cuda.synchronize()
+    eager = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    mb.run(projft, small, angles, use_graphs=True, stream=stream)
+    torch.cuda.synchronize()
+    graphed = time.perf_counter() - t1
+    mb.close()
+    work = sum(n * n for n in ns) * len(angles)
+    # algorithmic bytes per pixel-orientation of the 2D search at batch B (the 2D analogue of
+    # 16h + (8h + 20)/B, h = (n/2+1)/n): summed over the micrographs at their batches
+    algo = 0.0
+    for n in ns:
+        h = (n // 2 + 1) / n
+        b = info["batches"][n]
+        algo += n * n * len(angles) * (16 * h + (8 * h + 20) / b)
+    achieved = algo / sec / 1e9
+
+    def cpu():
+        k = 4
+        im = images[ns.index(sides[0])]
+        n = sides[0]
+        r = po.pick(projft.cpu().numpy(), p, p / n, im["spectrum"][None].cpu().numpy(), im["ctf"].cpu().numpy(), n,
+                    angles[:k].astype(np.float64), n / 8, ndims=2)
+        return n * n * k, r["loop_seconds"], (f"one {n}^2 micrograph, the first {k} orientations, FP64 oracle, "
+                                              f"orientation loop {r['loop_seconds']:.1f} s")
+    out_cfg = {"workload": "64 uneven micrographs (1024^2..4096^2, per-micrograph astigmatic CTF, 200 planted "
+                           "projections of a 64^3 reference at SNR 0.02) x 4,000 orientations in one batched call",
+               "micrographs": len(ns), "sides": {str(n): ns.count(n) for n in sides}, "orientations": len(angles),
+               "batches": {str(k): v for k, v in info["batches"].items()}, "graphs_captured": info["captures"],
+               "graph_replays": info["replays"],
+               "small_side_check": {"side": sides[0], "micrographs": len(small), "eager_s": round(eager, 3),
+                                    "graphs_s": round(graphed, 3)}}
+    rc = _row_line(args, "micrographs", "micrograph pixel-orientations/s, batched uneven micrographs",
+                   "pixel-orientations/s", work, sec, out_cfg, cpu, extra={
+                       "roofline": {"bound": "hbm", "kernel": "whole batched search (every kernel of every batch)",
+                                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                                    "algorithmic_bytes_per_po": round(algo / work, 3)}})
+    return rc
+
+
+def row_split(args):
+    """Precision: the 64^3 configuration (3 planted copies, 588 orientations, CTF) searched with
+    every per-orientation inverse transform's radix-16 stage on the matrix cores, fp16 and bf16
+    hi/lo split (and hi alone), against the FP64 oracle and beside the fp32 search."""
+    import torch
+    import gtom_amd as gx
+    from gtom_amd import synth
+    from oracle import pyoracle as po
+    dev = torch.device("cuda:0")
+    w = synth.cfg1(seed=1)
+    n, p, s = w["n"], w["p"], w["oversample"]
+    projft = gx.prepare_projector(torch.from_numpy(w["template"]).to(dev), p)
+    E = gx.d_FFTR2C(torch.from_numpy(w["tomogram"][None]).to(dev))
+    ctf_np = po.ctf3(n, 10e-10, 300e3, 2.7e-3, -3e-6, 0.07).astype(np.float32)
+    ctf = torch.from_numpy(ctf_np).to(dev)
+    angles = w["angles"]
+    plan = gx.PickPlan(p, n, 64)
+    ref = po.pick(projft.cpu().numpy().astype(np.complex128), p, s, E.cpu().numpy().astype(np.complex128),
+                  ctf_np.astype(np.float64), n, angles.astype(np.float64), w["maskradius"])
+    scale = np.abs(ref["best"]).max()
+    sure = (ref["top1"] - ref["top2"]) > 1e-4 * np.abs(ref["top1"]).max()
+    results = {}
+    for name in ("fp32", "fp16", "bf16", "fp16-hi", "bf16-hi"):
+        if name == "fp32":
+            step = lambda: gx.d_PickSubTomograms(plan, projft, s, E, ctf, angles, w["maskradius"])
+        else:
+            prec = gx.PRECISIONS[name]
+            step = lambda prec=prec: gx.d_PickSubTomogramsSplit(plan, prec, projft, s, E, ctf, angles,
+                                                                w["maskradius"])
+        best, idx = step()
+        torch.cuda.synchronize()
+        b, i = best.cpu().numpy(), idx.cpu().numpy()
+        sec = _timed_steps(args, step)
+        results[name] = {"ms_per_step": round(sec * 1e3, 3), "vo_per_s": n ** 3 * len(angles) / sec,
+                         "max_cc_error_vs_fp64": float(np.abs(b - ref["best"]).max() / scale),
+                         "argmax_agreement": float((i[sure] == ref["idx"][sure]).mean()),
+                         "argmax_voxels": int(sure.sum())}
+        log(f"{name}: {results[name]}")
+    plan.close()
+    h = (n // 2 + 1) / n
+    algo = 16 * h + (8 * h + 20) / 64
+    f16 = results["fp16"]
+    out = {"row": "split", "metric": "voxel-orientations/s, 64^3 template match, split-precision matrix-core "
+                                     "transforms", "value": f16["vo_per_s"], "unit": "voxel-orientations/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": f16["ms_per_step"],
+           "higher_is_better": True, "dtype": "fp16 hi/lo MFMA (radix-16), fp32 accumulation and VALU stages",
+           "data": "synthetic",
+           "config": {"workload": "64^3 tomogram, 3 planted copies of a 32^3 blob template (projector 64^3), 588 "
+                                  "orientations (30-degree GetEqualAngularSpacing), CTF 3 um", "batch": 64},
+           "precisions": results,
+           "roofline": {"bound": "hbm", "kernel": "split-precision search pipeline",
+                        "achieved": round(f16["vo_per_s"] * algo / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(f16["vo_per_s"] * algo / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes_per_vo": round(algo, 3)}}
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def row_diff2(args):
@@ -524,11 +749,14 @@ def main():
     if args.copy_probe:
         copy_probe()
         return 0
+    if args.search == "learned":
+        return row_learned(args)
     if args.row is not None:
         return {"prob_picker": row_prob_picker, "picker": row_picker, "pick2d": row_pick2d, "diff2": row_diff2,
                 "local_peaks": row_local_peaks, "fft_micrograph": row_fft_micrograph,
                 "prob_picker_micrograph": row_prob_picker_micrograph,
-                "pick2d_micrograph": row_pick2d_micrograph}[args.row](args)
+                "pick2d_micrograph": row_pick2d_micrograph, "micrographs": row_micrographs,
+                "split": row_split}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth
@@ -594,8 +822,10 @@ def main():
                       "blocks": nvol, "block": n, "orientations": len(angles), "batch": plan.batch,
                       "search": "plain"},
            "roofline": roof, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
     plan.close()
+    # the learned-vs-plain comparison of the metric, on its own workload (see learned_comparison)
+    out["learned"] = learned_comparison(args.learned_blocks) if args.learned_blocks > 0 else None
+    print(json.dumps(out), flush=True)
     return 0
 
 

@@ -48,16 +48,20 @@ __device__ __forceinline__ float sincf_ref(float x) {
   return fabsf(x) <= 1e-4f ? 1.f : sinf(x * 3.14159265358979f) / (x * 3.14159265358979f);
 }
 
-// Exact2DWeightingKernel for FFT-order weights: rows b of d_weights (nh x ny each)
-// for images indices[b]; basis[9 i..] = Matrix3Euler(angle_i) row-major.
+// Exact2DWeightingKernel for FFT-order weights: filter b of d_weights (nh x ny)
+// in the frame and with the weight of image b, as the reference launches it
+// (h_globalB2localB[b] and currentimage = b, src/Projection/Weighting.cu:40-46;
+// its h_indices[b] picks a normal the kernel never reads); basis[9 i..] =
+// Matrix3Euler(angle_i) row-major.
 __global__ __launch_bounds__(256) void k_exact2d_weighting(float* __restrict__ w, int nx, int ny,
                                                            const int* __restrict__ indices,
                                                            const float* __restrict__ basis,
                                                            const float* __restrict__ iw, int nimages, float maxfreq) {
   const int nh = nx / 2 + 1;
   const int b = blockIdx.y;
-  const float* E = basis + 9 * indices[b];
-  const float wc = iw[indices[b]];
+  (void)indices;
+  const float* E = basis + 9 * b;
+  const float wc = iw[b];
   float* out = w + (size_t)b * nh * ny;
   for (int id = blockIdx.x * 256 + threadIdx.x; id < nh * ny; id += gridDim.x * 256) {
     const int idx = id % nh, idy = id / nh;
@@ -392,6 +396,7 @@ int gtomx_exact2d_weighting(float* d_weights, gtomx_int3 dimsimage, const int* h
   if (iszerocentered) return fail(GTOMX_ERR_UNSUPPORTED, "zero-centred weights are not built (FFT order only)");
   if (nimages == 0 || batch == 0 || dimsimage.x < 2 || dimsimage.y < 1 || dimsimage.z != 1)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "bad dims, nimages or batch");
+  if (batch > nimages) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch item b takes image b's frame: batch <= nimages");
   for (unsigned b = 0; b < batch; ++b)
     if (h_indices[b] < 0 || h_indices[b] >= (int)nimages) return fail(GTOMX_ERR_INVALID_ARGUMENT, "index range");
   return exact2d_weighting_launch(d_weights, dimsimage.x, dimsimage.y, h_indices, h_angles, d_imageweights, nimages,

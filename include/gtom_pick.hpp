@@ -496,7 +496,8 @@ class CTFTiltParams {
 
 // class Picker, include/Correlation.cuh:136-164. The reference's members
 // (cuFFT plans, textures, scratch) live inside the library object; the public
-// dimensions are kept. 3D only.
+// dimensions are kept. 3D volumes, or 2D images with a square 2D reference
+// (in-plane orientations).
 class Picker {
  public:
   int3 dimsref{}, dimsimage{};
@@ -533,6 +534,55 @@ class Picker {
 
  private:
   gtomx_picker* handle_ = nullptr;
+};
+
+// class TomoPicker, include/Correlation.cuh:209-257: the tilt-series picker
+// (reference, mask and the images' tilt geometry; correlation maps of every
+// tilt image back-projected into a volume). The reference's public buffers
+// (textures, spectra, plans) live inside the library object; the dimensions
+// and flags are kept.
+class TomoPicker {
+ public:
+  int3 dimsref{}, dimsrefpadded{};
+  int2 dimsimage{};
+  uint nimages = 0;
+  bool ismaskcircular = true;
+
+  TomoPicker() = default;
+  TomoPicker(const TomoPicker&) = delete;
+  TomoPicker& operator=(const TomoPicker&) = delete;
+  ~TomoPicker() { gtomx_tomopicker_destroy(handle_); }
+
+  void Initialize(tfloat* _d_ref, int3 _dimsref, tfloat* _d_refmask, bool _ismaskcircular, int2 _dimsimage,
+                  uint _nimages) {
+    gtomx_tomopicker_destroy(handle_);
+    handle_ = nullptr;
+    dimsref = _dimsref;
+    dimsrefpadded = int3{2 * _dimsref.x, 2 * _dimsref.y, 2 * _dimsref.z};
+    dimsimage = _dimsimage;
+    ismaskcircular = _ismaskcircular;
+    nimages = _nimages;
+    detail::check(gtomx_tomopicker_create(&handle_, _d_ref, detail::i3(_dimsref), _d_refmask, _ismaskcircular ? 1 : 0,
+                                          gtomx_int3{_dimsimage.x, _dimsimage.y, 1}, _nimages, nullptr),
+                  "TomoPicker::Initialize");
+    detail::sync();
+  }
+  // Weights _d_image in place (d_Exact2DWeighting of the tilt geometry), as the reference does.
+  void SetImage(tfloat* _d_image, tfloat3* _h_imageangles, tfloat* _h_imageweights) {
+    detail::check(gtomx_tomopicker_set_image(handle_, _d_image, reinterpret_cast<const gtomx_float3*>(_h_imageangles),
+                                             _h_imageweights, nullptr),
+                  "TomoPicker::SetImage");
+    detail::sync();
+  }
+  void PerformCorrelation(tfloat* d_corr, tfloat3* d_corrangle, int3 dimscorr, tfloat anglestep) {
+    detail::check(gtomx_tomopicker_correlate(handle_, d_corr, reinterpret_cast<gtomx_float3*>(d_corrangle),
+                                             detail::i3(dimscorr), anglestep, nullptr),
+                  "TomoPicker::PerformCorrelation");
+    detail::sync();
+  }
+
+ private:
+  gtomx_tomopicker* handle_ = nullptr;
 };
 
 // class ProbabilisticPicker, include/Correlation.cuh:166-207, circular and

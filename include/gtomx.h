@@ -380,9 +380,11 @@ int gtomx_picker_correlate_angles(gtomx_picker* picker, const gtomx_float3* h_an
 
 /* d_Exact2DWeighting, include/Projection.cuh / src/Projection/Weighting.cu:21-55
  * (kernel :92-133): multiplies batch half-spectrum filters (n/2+1 x n each, FFT
- * order; zero-centred weights are refused) of the images h_indices[b] of a tilt
- * series with (rot, tilt, psi) h_angles by w_b^2 / sum_k |sinc(A . normal_k)| w_k
- * inside |f| <= maxfreq and 0 outside (d_imageweights NULL: all 1). Synchronous. */
+ * order; zero-centred weights are refused) by w_b^2 / sum_k |sinc(A . normal_k)| w_k
+ * inside |f| <= maxfreq and 0 outside, A the frequency in the frame of image b of
+ * the tilt series with (rot, tilt, psi) h_angles (d_imageweights NULL: all 1).
+ * As in the reference, filter b takes image b's frame and weight (batch <=
+ * nimages); h_indices is validated but, as there, selects nothing used. Synchronous. */
 int gtomx_exact2d_weighting(float* d_weights, gtomx_int3 dimsimage, const int* h_indices,
                             const gtomx_float3* h_angles, const float* d_imageweights, unsigned nimages,
                             float maxfreq, int iszerocentered, unsigned batch, void* stream);

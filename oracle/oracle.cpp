@@ -588,13 +588,15 @@ void fft_resize(int op, const double* in, double* out, const int* od, const int*
 // the device-array projector (src/Relion/Project.cu:89-177, 2D sections):
 // ---------------------------------------------------------------------------
 // d_Exact2DWeighting (src/Projection/Weighting.cu:21-55, kernel :92-133) for
-// images in FFT order (iszerocentered false), maxfreq in pixels: filter[i]
-// (n/2+1 x n half spectrum per image, multiplied in place) at frequency f is
-// w_i^2 / sum_b |sinc(dot(A, normal_b))| w_b, A = the frequency -f of the
-// kernel's loop written in the image's tilt frame (rows 0 and 1 of
-// Matrix3Euler(angle_i) as the local axes), normal_b = row 2 of
-// Matrix3Euler(angle_b); 0 where |f| > maxfreq. sinc(x) = sin(pi x)/(pi x),
-// 1 for |x| <= 1e-8 (include/DeviceFunctions.cuh:19-25).
+// images in FFT order (iszerocentered false), maxfreq in pixels: filter b
+// (n/2+1 x n half spectrum, multiplied in place) at frequency f is
+// w_b^2 / sum_k |sinc(dot(A, normal_k))| w_k, A = the frequency -f of the
+// kernel's loop written in the frame of image b (rows 0 and 1 of
+// Matrix3Euler(angle_b) as the local axes), normal_k = row 2 of
+// Matrix3Euler(angle_k); 0 where |f| > maxfreq. As in the reference, batch item
+// b takes the frame (h_globalB2localB[b], :40) and weight (currentimage = b) of
+// image b; indices[b] only selects a normal the kernel never uses. sinc(x) =
+// sin(pi x)/(pi x), 1 for |x| <= 1e-8 (include/DeviceFunctions.cuh:19-25).
 void exact2d_weighting(double* w, int nx, int ny, const double* angles, const double* weights, int nimages,
                        const int* indices, int batch, double maxfreq) {
   std::vector<double> normals(3 * (size_t)nimages), B(9 * (size_t)nimages);
@@ -604,8 +606,8 @@ void exact2d_weighting(double* w, int nx, int ny, const double* angles, const do
   }
   const int nh = nx / 2 + 1;
   for (int b = 0; b < batch; ++b) {
-    const int index = indices[b];
-    const double* E = &B[9 * (size_t)index];
+    (void)indices;
+    const double* E = &B[9 * (size_t)b];
     double* out = w + (size_t)b * nh * ny;
     for (int idy = 0; idy < ny; ++idy) {
       const int y = ny - 1 - ((idy + ny / 2 - 1) % ny);

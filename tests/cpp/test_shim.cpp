@@ -360,10 +360,70 @@ static void errors_throw() {
   std::printf("ok errors_throw\n");
 }
 
+// The TomoPicker class: one untilted tilt image holding the reference's
+// projection along z at a known pixel; the voxels above that pixel hold the
+// best correlation (near 1) with the identity orientation.
+static void tomopicker_finds_projection() {
+  const int nr = 16, n = 64;
+  std::vector<float> ref((size_t)nr * nr * nr), mask(ref.size(), 1.f);
+  for (int z = 0; z < nr; ++z)
+    for (int y = 0; y < nr; ++y)
+      for (int x = 0; x < nr; ++x) {
+        double dx = x - nr / 2 - 1.0, dy = y - nr / 2 + 2.0, dz = z - nr / 2;
+        ref[((size_t)z * nr + y) * nr + x] = (float)(std::exp(-(dx * dx / 3 + dy * dy / 6 + dz * dz / 2) / 2) +
+                                                     0.7 * std::exp(-((dx - 3) * (dx - 3) + dy * dy + dz * dz) / 2));
+      }
+  std::vector<float> image((size_t)n * n, 0.f);
+  const int px = 37, py = 22;
+  std::mt19937 rng(11);
+  std::normal_distribution<float> noise(0.f, 0.05f);
+  for (auto& v : image) v = noise(rng);
+  for (int z = 0; z < nr; ++z)
+    for (int y = 0; y < nr; ++y)
+      for (int x = 0; x < nr; ++x)
+        image[(size_t)(y - nr / 2 + py) * n + (x - nr / 2 + px)] += ref[((size_t)z * nr + y) * nr + x];
+  float* d_ref = dev<float>(ref.size());
+  float* d_mask = dev<float>(mask.size());
+  float* d_image = dev<float>(image.size());
+  hipMemcpy(d_ref, ref.data(), ref.size() * sizeof(float), hipMemcpyHostToDevice);
+  hipMemcpy(d_mask, mask.data(), mask.size() * sizeof(float), hipMemcpyHostToDevice);
+  hipMemcpy(d_image, image.data(), image.size() * sizeof(float), hipMemcpyHostToDevice);
+  const int cx = 32, cy = 32, cz = 2;
+  std::vector<float> c0((size_t)cx * cy * cz, -1e30f);
+  float* d_corr = dev<float>(c0.size());
+  hipMemcpy(d_corr, c0.data(), c0.size() * sizeof(float), hipMemcpyHostToDevice);
+  tfloat3* d_angle = dev<tfloat3>(c0.size());
+  hipMemset(d_angle, 0xff, c0.size() * sizeof(tfloat3));
+  tfloat3 tilt{0.f, 0.f, 0.f};
+  tfloat weight = 1.f;
+
+  TomoPicker tp;
+  tp.Initialize(d_ref, make_int3(nr, nr, nr), d_mask, true, int2{n, n}, 1);
+  EXPECT(tp.dimsrefpadded.x == 2 * nr && tp.nimages == 1, "TomoPicker dims");
+  tp.SetImage(d_image, &tilt, &weight);
+  tp.PerformCorrelation(d_corr, d_angle, make_int3(cx, cy, cz), 90.f * 3.14159265f / 180.f);
+  std::vector<float> corr(c0.size());
+  std::vector<tfloat3> ang(c0.size());
+  hipMemcpy(corr.data(), d_corr, corr.size() * sizeof(float), hipMemcpyDeviceToHost);
+  hipMemcpy(ang.data(), d_angle, ang.size() * sizeof(tfloat3), hipMemcpyDeviceToHost);
+  const int vx = px + cx / 2 - n / 2, vy = py + cy / 2 - n / 2;
+  for (int z = 0; z < cz; ++z) {
+    size_t arg = (size_t)z * cx * cy;
+    for (size_t i = arg; i < (size_t)(z + 1) * cx * cy; ++i)
+      if (corr[i] > corr[arg]) arg = i;
+    const int ax = (int)(arg % cx), ay = (int)((arg / cx) % cy);
+    EXPECT(ax == vx && ay == vy && corr[arg] > 0.97f, "TomoPicker peak %.3f at (%d,%d) in z %d", corr[arg], ax, ay, z);
+    EXPECT(ang[arg].x == 0.f && ang[arg].y == 0.f && ang[arg].z == 0.f, "TomoPicker angle");
+  }
+  std::printf("ok tomopicker_finds_projection\n");
+  hipFree(d_ref); hipFree(d_mask); hipFree(d_image); hipFree(d_corr); hipFree(d_angle);
+}
+
 int main() {
   fft_roundtrip();
   pick_finds_shift();
   picker_finds_copy();
+  tomopicker_finds_projection();
   prob_picker_matches_c_abi();
   mrc_round_trip();
   ctf_simulate_known_answers();

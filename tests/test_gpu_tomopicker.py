@@ -27,7 +27,7 @@ def test_exact2d_weighting_matches_oracle(gx, orc, cuda):
     tilts[:, 0] = np.linspace(0, 0.3, 9)  # some rot as well
     w = np.random.default_rng(3).uniform(0.5, 1.5, 9).astype(np.float32)
     start = np.random.default_rng(4).uniform(0.5, 2.0, (4, ny, nx // 2 + 1)).astype(np.float32)
-    ind = [0, 4, 7, 8]
+    ind = [6, 4, 7, 8]  # as in the reference, filter b uses image b's frame and weight whatever the index
     got = torch.from_numpy(start).to(cuda)
     gx.d_Exact2DWeighting(got, tilts, imageweights=torch.from_numpy(w).to(cuda), indices=ind, maxfreq=30.0)
     want = orc.exact2d_weighting(nx, ny, tilts.astype(np.float64), w.astype(np.float64), ind, 30.0,
