@@ -791,14 +791,15 @@ def main():
     plan.enable_timing(True)
     plan.timings(reset=True)
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
+    # one event per step boundary on the search stream: the mean is the metric, the median is kept beside it
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    evs[0].record(stream)
     for i in range(args.steps):
         step()
-    ev1.record(stream)
+        evs[i + 1].record(stream)
     torch.cuda.synchronize()
-    total_ms = ev0.elapsed_time(ev1)
+    step_ms = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
+    total_ms = evs[0].elapsed_time(evs[-1])
     ms, cnt = plan.timings(reset=True)
     ms_per_step = total_ms / args.steps
     work = nvol * n ** 3 * len(angles)
@@ -821,7 +822,7 @@ def main():
                                   "directions x 18 psi), per-block 3D CTF, local-std radius 24",
                       "blocks": nvol, "block": n, "orientations": len(angles), "batch": plan.batch,
                       "search": "plain"},
-           "roofline": roof, "cpu_baseline": cpu}
+           "median_ms_per_step": float(np.median(step_ms)), "roofline": roof, "cpu_baseline": cpu}
     plan.close()
     # the learned-vs-plain comparison of the metric, on its own workload (see learned_comparison)
     out["learned"] = learned_comparison(args.learned_blocks) if args.learned_blocks > 0 else None

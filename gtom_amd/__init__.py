@@ -168,7 +168,7 @@ _SIGS = {
     "gtomx_pnet_forward": [_P, _U, _P, _P],
     "gtomx_pnet_gradients": [_P, _P, _U, _P, _P, _P],
     "gtomx_pnet_train_step": [_P, _P, _U, _F, _F, _P, _P],
-    "gtomx_pnet_propose": [_P, _U, _P, _P, _P, _P],
+    "gtomx_pnet_propose": [_P, _U, _P, _P, _P, _P, _P, _P],
     "gtomx_pnet_debug_gemm": [_P, _P, _P, _I, _I, _I, _P],
 }
 _RESTYPES = {"gtomx_version": ctypes.c_char_p, "gtomx_status_string": ctypes.c_char_p,

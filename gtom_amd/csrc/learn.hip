@@ -377,13 +377,15 @@ __global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ in, in
 }
 
 // ---------------------------------------------------------------------------
-// Proposals: softmax top-1 probability and class over the 501 classes, and the
+// Proposals: softmax top-1 probability and class over the 501 classes, the
 // top-k of the 500 direction logits (larger logit first, ties to the lower
-// index). One wave per row.
+// index), the summed probability of those k directions (mass) and the
+// probability of "no peak" (pnp). One wave per row.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_propose(const float* __restrict__ logits, int n, int topk,
                                                  int* __restrict__ dirs, float* __restrict__ top1,
-                                                 int* __restrict__ cls) {
+                                                 int* __restrict__ cls, float* __restrict__ mass,
+                                                 float* __restrict__ pnp) {
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= n) return;
@@ -417,10 +419,12 @@ __global__ __launch_bounds__(256) void k_propose(const float* __restrict__ logit
   if (lane == 0) {
     top1[r] = 1.f / s;
     cls[r] = mk;
+    if (pnp) pnp[r] = expf(lr[NDIR] - mx) / s;
   }
   // the background class takes no part in the direction ranking
   for (int q = 0; q < 8; ++q)
     if (lane + 64 * q >= NDIR) v[q] = -INFINITY;
+  float msum = 0.f;
   for (int t = 0; t < topk; ++t) {
     float bm = -INFINITY;
     int bk = 0x7fffffff;
@@ -439,10 +443,12 @@ __global__ __launch_bounds__(256) void k_propose(const float* __restrict__ logit
       }
     }
     if (lane == 0) dirs[(size_t)r * topk + t] = bk < NDIR ? bk : -1;
+    if (bk < NDIR) msum += expf(bm - mx) / s;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
       if (lane + 64 * q == bk) v[q] = -INFINITY;
   }
+  if (lane == 0 && mass) mass[r] = msum;
 }
 
 // ---------------------------------------------------------------------------
@@ -891,9 +897,10 @@ int gtomx_pnet_train_step(gtomx_pnet* nt, const int32_t* d_labels, unsigned n, f
 }
 
 // Proposals from the last forward pass: per tile the top-k directions (d_dirs, n x topk), the softmax
-// probability of the top class (d_top1) and that class (d_class; 500 = no peak).
+// probability of the top class (d_top1) and that class (d_class; 500 = no peak), and optionally the
+// summed probability of the k directions (d_mass) and the probability of "no peak" (d_nopeak).
 int gtomx_pnet_propose(gtomx_pnet* nt, unsigned topk, int32_t* d_dirs, float* d_top1, int32_t* d_class,
-                       void* stream) {
+                       float* d_mass, float* d_nopeak, void* stream) {
   GX_TRY(pnet_check(nt));
   if (nt->nlast == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "no forward pass has run");
   if (topk == 0 || topk > (unsigned)NDIR) return fail(GTOMX_ERR_INVALID_ARGUMENT, "topk must be in [1, 500]");
@@ -901,7 +908,7 @@ int gtomx_pnet_propose(gtomx_pnet* nt, unsigned topk, int32_t* d_dirs, float* d_
   hipStream_t s = as_stream(stream);
   GX_TRY(nt->fence.enter(s));
   hipLaunchKernelGGL(k_propose, dim3((nt->nlast + 3) / 4), dim3(256), 0, s, nt->logits, (int)nt->nlast, (int)topk,
-                     d_dirs, d_top1, d_class);
+                     d_dirs, d_top1, d_class, d_mass, d_nopeak);
   GX_LAUNCHED("k_propose");
   return nt->fence.leave(s);
 }

@@ -766,6 +766,8 @@ void plan_release(gtomx_pick_plan* pl) {
   dev_free(pl->ldirs);
   dev_free(pl->ltop1);
   dev_free(pl->lcls);
+  dev_free(pl->lmass);
+  dev_free(pl->lpnp);
   if (pl->h_tmasks) (void)hipHostFree(pl->h_tmasks);
   if (pl->h_mats) (void)hipHostFree(pl->h_mats);
   if (pl->mats_ev) (void)hipEventDestroy(pl->mats_ev);
@@ -1495,25 +1497,30 @@ int gtomx_pick_subtomograms_learned(gtomx_pick_plan* plan, gtomx_pnet* net, cons
   const size_t ntl = (size_t)nt * nt * nt, mbytes = (size_t)nt * nt;
   GX_TRY(plan->fence.enter(s));
   if (plan->lcap < ntl * topk) {
-    dev_free(plan->ldirs);
-    dev_free(plan->ltop1);
-    dev_free(plan->lcls);
+    for (void* q : {(void*)plan->ldirs, (void*)plan->ltop1, (void*)plan->lcls, (void*)plan->lmass, (void*)plan->lpnp})
+      dev_free(q);
     plan->ldirs = nullptr;
     plan->ltop1 = nullptr;
     plan->lcls = nullptr;
+    plan->lmass = nullptr;
+    plan->lpnp = nullptr;
     plan->lcap = 0;
-    void *a = nullptr, *b = nullptr, *c = nullptr;
+    void *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
     GX_TRY(dev_alloc(&a, ntl * topk * sizeof(int32_t)));
     plan->ldirs = static_cast<int32_t*>(a);
     GX_TRY(dev_alloc(&b, ntl * sizeof(float)));
     plan->ltop1 = static_cast<float*>(b);
     GX_TRY(dev_alloc(&c, ntl * sizeof(int32_t)));
     plan->lcls = static_cast<int32_t*>(c);
+    GX_TRY(dev_alloc(&d, ntl * sizeof(float)));
+    plan->lmass = static_cast<float*>(d);
+    GX_TRY(dev_alloc(&e, ntl * sizeof(float)));
+    plan->lpnp = static_cast<float*>(e);
     plan->lcap = ntl * topk;
   }
   std::vector<uint8_t> masks((size_t)nvolumes * ndirections * mbytes, 0), searched((size_t)nvolumes * ntl, 0);
-  std::vector<int32_t> dirs(ntl * topk), cls(ntl);
-  std::vector<float> top1(ntl);
+  std::vector<int32_t> dirs(ntl * topk);
+  std::vector<float> mass(ntl), pnp(ntl);
   gtomx_learned_stats st{};
   for (unsigned v = 0; v < nvolumes; ++v) {
     // the network reads the block in real space: C2R_u(E_v) (its standardisation removes the scale)
@@ -1521,21 +1528,25 @@ int gtomx_pick_subtomograms_learned(gtomx_pick_plan* plan, gtomx_pnet* net, cons
     GX_TRY(fft_c2r(Ev, plan->G, plan->ebuf, dimsvolume, 1, 1.0f, s));
     GX_TRY(gtomx_pnet_load_volume(net, plan->ebuf, dimsvolume, stream));
     GX_TRY(gtomx_pnet_forward(net, (unsigned)ntl, nullptr, stream));
-    GX_TRY(gtomx_pnet_propose(net, topk, plan->ldirs, plan->ltop1, plan->lcls, stream));
+    GX_TRY(gtomx_pnet_propose(net, topk, plan->ldirs, plan->ltop1, plan->lcls, plan->lmass, plan->lpnp, stream));
     GX_HIP(hipMemcpyAsync(dirs.data(), plan->ldirs, ntl * topk * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GX_HIP(hipMemcpyAsync(top1.data(), plan->ltop1, ntl * sizeof(float), hipMemcpyDeviceToHost, s));
-    GX_HIP(hipMemcpyAsync(cls.data(), plan->lcls, ntl * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(mass.data(), plan->lmass, ntl * sizeof(float), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(pnp.data(), plan->lpnp, ntl * sizeof(float), hipMemcpyDeviceToHost, s));
     GX_HIP(hipStreamSynchronize(s));
     uint8_t* mv = masks.data() + (size_t)v * ndirections * mbytes;
     for (size_t t = 0; t < ntl; ++t) {
       const int tz = (int)(t / mbytes), ty = (int)((t / nt) % nt), tx = (int)(t % nt);
       const size_t mi = (size_t)tz * nt + ty;
       const uint8_t bit = (uint8_t)(1u << tx);
-      if (top1[t] < confidence) {  // not confident: every direction
+      // the tile's proposal is "no peak" or its top-k directions, whichever the network gives the
+      // larger probability; below `confidence` the tile falls back to every direction
+      const bool none = pnp[t] >= mass[t];
+      const float conf = none ? pnp[t] : mass[t];
+      if (conf < confidence) {  // not confident: every direction
         for (unsigned g = 0; g < ndirections; ++g) mv[g * mbytes + mi] |= bit;
         searched[v * ntl + t] = 1;
         st.exhaustive++;
-      } else if (cls[t] == (int)ndirections) {  // confidently no peak: not searched
+      } else if (none) {  // confidently no peak: not searched
         st.skipped++;
       } else {
         for (unsigned k = 0; k < topk; ++k) {

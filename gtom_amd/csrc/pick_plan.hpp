@@ -50,6 +50,8 @@ struct gtomx_pick_plan {
   int32_t* ldirs = nullptr;
   float* ltop1 = nullptr;
   int32_t* lcls = nullptr;
+  float* lmass = nullptr;  // summed probability of each tile's top-k directions
+  float* lpnp = nullptr;   // probability of "no peak"
   size_t lcap = 0;  // tiles x topk
   gx::StreamFence fence;  // the plan's buffers are reused by calls on any stream
   // kernel timing

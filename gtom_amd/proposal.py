@@ -128,15 +128,18 @@ class ProposalNet:
               "gtomx_pnet_train_step")
         return loss
 
-    def propose(self, n, topk=16, stream=None):
-        """From the last forward pass: (directions (n, topk), top-1 probability (n,), top class (n,))."""
+    def propose(self, n, topk=16, stream=None, full=False):
+        """From the last forward pass: (directions (n, topk), top-1 probability (n,), top class (n,)),
+        with full=True also (summed probability of the top-k directions (n,), P(no peak) (n,))."""
         import torch
         dirs = torch.empty((n, topk), dtype=torch.int32, device="cuda")
         top1 = torch.empty(n, dtype=torch.float32, device="cuda")
         cls = torch.empty(n, dtype=torch.int32, device="cuda")
-        check(lib().gtomx_pnet_propose(self.handle, topk, _ptr(dirs), _ptr(top1), _ptr(cls), _stream(stream)),
-              "gtomx_pnet_propose")
-        return dirs, top1, cls
+        mass = torch.empty(n, dtype=torch.float32, device="cuda")
+        pnp = torch.empty(n, dtype=torch.float32, device="cuda")
+        check(lib().gtomx_pnet_propose(self.handle, topk, _ptr(dirs), _ptr(top1), _ptr(cls), _ptr(mass), _ptr(pnp),
+                                       _stream(stream)), "gtomx_pnet_propose")
+        return (dirs, top1, cls, mass, pnp) if full else (dirs, top1, cls)
 
 
 def debug_gemm(a, b, stream=None):

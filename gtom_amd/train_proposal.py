@@ -206,15 +206,19 @@ def train(args):
             tl += float(loss.mean()) if b % 50 == 0 else 0.0
             step += 1
         metrics = evaluate_tiles(net, VX, VY)
-        metrics.update(epoch=ep + 1, seconds=round(time.time() - t0, 1))
+        metrics.update(epoch=ep + 1, seconds=round(time.time() - t0, 1), train_loss=round(tl / max(1, b // 50 + 1), 4))
         history.append(metrics)
         log(json.dumps(metrics))
-    params = net.get_params()
-    np.savez_compressed(args.out, params=params, history=json.dumps(history),
-                        config=json.dumps(dict(peak_threshold=PEAK_THRESHOLD, peak_extent=PEAK_EXTENT,
-                                               label_margin=LABEL_MARGIN, blocks=len(seeds), epochs=args.epochs,
-                                               batch=bs, lr=args.lr, wd=args.wd)))
-    log(f"wrote {args.out}")
+        # written after every epoch, so a run cut short still leaves its last network
+        params = net.get_params()
+        np.savez_compressed(args.out, params=params, history=json.dumps(history),
+                            config=json.dumps(dict(snr=SNR, peak_threshold=PEAK_THRESHOLD, peak_extent=PEAK_EXTENT,
+                                                   label_margin=LABEL_MARGIN, blocks=len(seeds), epochs=ep + 1,
+                                                   batch=bs, lr=args.lr, wd=args.wd)))
+        log(f"wrote {args.out} (epoch {ep + 1})")
+        if args.max_minutes > 0 and time.time() - t0 > 60 * args.max_minutes:
+            log(f"time budget of {args.max_minutes} min reached after epoch {ep + 1}")
+            break
 
 
 def evaluate_tiles(net, VX, VY, topk=16, conf=0.5):
@@ -231,12 +235,15 @@ def evaluate_tiles(net, VX, VY, topk=16, conf=0.5):
         y = VY[b0:b0 + 512].cpu().numpy()
         k = net.load_tiles(x)
         net.forward(k)
-        dirs, top1, cls = net.propose(k, topk)
-        dirs, top1, cls = dirs.cpu().numpy(), top1.cpu().numpy(), cls.cpu().numpy()
+        dirs, _, _, mass, pnp = net.propose(k, topk, full=True)
+        dirs, mass, pnp = dirs.cpu().numpy(), mass.cpu().numpy(), pnp.cpu().numpy()
         for i in range(k):
-            if top1[i] < conf:
+            # the decision of gtomx_pick_subtomograms_learned: "no peak" or the top-k directions,
+            # whichever is more probable, exhaustive when that probability is below conf
+            none = pnp[i] >= mass[i]
+            if max(pnp[i], mass[i]) < conf:
                 d = "exhaustive"
-            elif cls[i] == pr.NO_PEAK:
+            elif none:
                 d = "skip"
             else:
                 d = "short"
@@ -313,6 +320,7 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--wd", type=float, default=1e-4)
     t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--max-minutes", type=float, default=0.0, help="stop after the epoch that passes this budget")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--first", type=int, default=HELDOUT_SEEDS[0])

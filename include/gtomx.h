@@ -807,9 +807,10 @@ int gtomx_pnet_gradients(gtomx_pnet* net, const int32_t* d_labels, unsigned n, f
 int gtomx_pnet_train_step(gtomx_pnet* net, const int32_t* d_labels, unsigned n, float lr, float wd,
                           float* d_loss, void* stream);
 /* From the last forward pass: per tile the top-k directions, the top-1
- * softmax probability and the top class. */
+ * softmax probability and the top class, and (each optional) the summed
+ * probability of the top-k directions and the probability of "no peak". */
 int gtomx_pnet_propose(gtomx_pnet* net, unsigned topk, int32_t* d_dirs, float* d_top1,
-                       int32_t* d_class, void* stream);
+                       int32_t* d_class, float* d_mass, float* d_nopeak, void* stream);
 /* The head's GEMM alone (tests of the matrix-core operand layout). Synchronous. */
 int gtomx_pnet_debug_gemm(const float* d_a, const float* d_b, float* d_c, int M, int N, int K,
                           void* stream);
@@ -825,9 +826,11 @@ typedef struct {
 /* The learned search (cfg. d_PickSubTomograms, include/Correlation.cuh:97-109,
  * whose argmax it is trained on): h_angles holds ndirections (500) x npsi
  * orientations, direction-major. The network scores every 32^3 tile of
- * C2R(E_v); a tile whose top-1 probability is below `confidence` is searched
- * over all orientations, a tile confidently without a peak is not searched,
- * and any other tile over the npsi orientations of its top-k directions.
+ * C2R(E_v). A tile's proposal is "no peak" or its top-k directions, whichever
+ * has the larger probability under the network (P(no peak) against the summed
+ * probability of the k directions); a proposal below `confidence` falls back to
+ * every orientation, a confident "no peak" tile is not searched, and any other
+ * tile is searched over the npsi orientations of its top-k directions.
  * Searched voxels hold the result of gtomx_pick_subtomograms restricted to
  * their tile's orientations (same kernels, same values); voxels of tiles that
  * were not searched hold 0 and index -1. Synchronises `stream` once per

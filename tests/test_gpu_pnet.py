@@ -9,8 +9,9 @@ causes) separates them:
 - logits to 2e-3 normwise, per-tile losses to 1e-3;
 - every parameter array's gradient to 2e-2 of its largest entry;
 - the top-k directions equal numpy's ranking of the logits (ties to the lower
-  index, the "no peak" class excluded) and the top-1 probability the softmax
-  maximum;
+  index, the "no peak" class excluded), the top-1 probability the softmax
+  maximum, the top-k mass the summed softmax of those directions and P(no peak)
+  the softmax of class 500;
 - Adam steps on one batch drive its loss down (the network can fit labels).
 Tiles are 32^3; the network has no counterpart in the reference."""
 import numpy as np
@@ -87,14 +88,15 @@ def test_proposals_rank_the_logits(gx, cuda):
     net = pr.ProposalNet(max_tiles=40, params=_params(12))
     net.load_tiles(torch.from_numpy(_tiles(rng, 40)).to(cuda))
     logits = to_np(net.forward(40)).astype(np.float64)
-    dirs, top1, cls = net.propose(40, topk=16)
-    dirs, top1, cls = to_np(dirs), to_np(top1), to_np(cls)
+    dirs, top1, cls, mass, pnp = net.propose(40, topk=16, full=True)
+    dirs, top1, cls, mass, pnp = to_np(dirs), to_np(top1), to_np(cls), to_np(mass), to_np(pnp)
     for r in range(40):
         order = sorted(range(pr.NDIR), key=lambda k: (-logits[r, k], k))[:16]
         assert list(dirs[r]) == order
         p = np.exp(logits[r] - logits[r].max())
         p /= p.sum()
         assert cls[r] == int(np.argmax(p)) and abs(top1[r] - p.max()) <= 1e-5
+        assert abs(mass[r] - p[order].sum()) <= 1e-5 and abs(pnp[r] - p[pr.NO_PEAK]) <= 1e-6
     net.close()
 
 
