@@ -423,7 +423,7 @@ def row_micrographs(args):
     """Batching: 64 micrographs of uneven sides (1024^2 .. 4096^2), each with its own astigmatic
     CTF, 200 planted projections of a 64^3 random-blob reference at SNR 0.02, searched over 4,000
     orientations (500 Fibonacci directions x 8 in-plane steps) in one gtomx_mpick_run: one plan per
-    side with the autotuner's batch (gtom_amd/tuning/micrograph_batches.json), each side's
+    side with the autotuner's batch and row kernel (gtom_amd/tuning/micrograph_batches.json), each side's
     orientation loop captured once as a hipGraph and replayed per micrograph."""
     import torch
     import gtom_amd as gx
@@ -435,8 +435,9 @@ def row_micrographs(args):
         f"{sum(n * n for n in ns):.3g} pixels")
     table = tm.load_table()
     sides = sorted(set(ns))
-    batches = [table.get(n, 0) for n in sides]
-    mb = gx.MicrographBatch(p, sides, batches)
+    batches = [table.get(n, (0, 0))[0] for n in sides]
+    variants = [table.get(n, (0, 0))[1] for n in sides]
+    mb = gx.MicrographBatch(p, sides, batches, variants)
     angles = gx.fibonacci_angles(500, 8)
     stream = torch.cuda.current_stream()
     sec = _timed_steps(args, lambda: mb.run(projft, images, angles, use_graphs=True, stream=stream))
@@ -475,7 +476,8 @@ def row_micrographs(args):
     out_cfg = {"workload": "64 uneven micrographs (1024^2..4096^2, per-micrograph astigmatic CTF, 200 planted "
                            "projections of a 64^3 reference at SNR 0.02) x 4,000 orientations in one batched call",
                "micrographs": len(ns), "sides": {str(n): ns.count(n) for n in sides}, "orientations": len(angles),
-               "batches": {str(k): v for k, v in info["batches"].items()}, "graphs_captured": info["captures"],
+               "batches": {str(k): v for k, v in info["batches"].items()},
+               "row_variants": {str(k): v for k, v in info["variants"].items()}, "graphs_captured": info["captures"],
                "graph_replays": info["replays"],
                "small_side_check": {"side": sides[0], "micrographs": len(small), "eager_s": round(eager, 3),
                                     "graphs_s": round(graphed, 3)}}

@@ -135,6 +135,7 @@ _SIGS = {
     "gtomx_rotate2d_ft": [_P, _P, int3, _P, _F, _I, _I, _U, _P],
     "gtomx_pick_plan_create": [ctypes.POINTER(_P), int3, int3, _U],
     "gtomx_pick_plan_destroy": [_P],
+    "gtomx_pick_plan_set_rows_variant": [_P, ctypes.c_int],
     "gtomx_pick_plan_info": [_P, ctypes.POINTER(_U), ctypes.POINTER(_SZ)],
     "gtomx_pick_plan_enable_timing": [_P, _I],
     "gtomx_pick_plan_timings": [_P, _P, _P, _I],
@@ -145,7 +146,7 @@ _SIGS = {
     "gtomx_pick_update_tiles": [_P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P, _P],
     "gtomx_pick_update_split": [_P, ctypes.c_int, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _P, _P, _P],
     "gtomx_split_dft64": [_P, _P, _U, ctypes.c_int, _P],
-    "gtomx_mpick_create": [_P, ctypes.c_int, _P, _P, _U],
+    "gtomx_mpick_create": [_P, ctypes.c_int, _P, _P, _P, _U],
     "gtomx_exact2d_weighting": [_P, int3, _P, _P, _P, _U, _F, ctypes.c_int, _U, _P],
     "gtomx_tomopicker_create": [_P, _P, int3, _P, ctypes.c_int, int3, _U, _P],
     "gtomx_tomopicker_destroy": [_P],
@@ -153,9 +154,9 @@ _SIGS = {
     "gtomx_tomopicker_correlate": [_P, _P, _P, int3, _F, _P],
     "gtomx_tomopicker_correlate_angles": [_P, _P, _U, _P, _P, int3, _P, _P, _P, _P],
     "gtomx_mpick_destroy": [_P],
-    "gtomx_mpick_info": [_P, _P, _P, _P],
+    "gtomx_mpick_info": [_P, _P, _P, _P, _P],
     "gtomx_mpick_run": [_P, _P, _P, _U, _P, _U, ctypes.c_int, _P],
-    "gtomx_mpick_tune": [ctypes.c_int, ctypes.c_int, _P, _P, _U, _P, _U, ctypes.c_int, _P, _P],
+    "gtomx_mpick_tune": [ctypes.c_int, ctypes.c_int, _P, _P, _P, _U, _P, _U, ctypes.c_int, _P, _P],
     "gtomx_pick_subtomograms_learned": [_P, _P, _P, _F, int3, _P, _P, _SZ, int3, _U, _P, _U, _U, _U, _F, _F, _P,
                                         _P, _P, _P, _P],
     "gtomx_pnet_param_count": [],
@@ -597,6 +598,10 @@ def d_LocalStd(vol, radius, with_mean=False, fouriermask=None, stream=None):
     return (sd, mean) if with_mean else sd
 
 
+ROWS_PLAIN = 0  # GTOMX_ROWS_PLAIN
+ROWS_PREFETCH = 1  # GTOMX_ROWS_PREFETCH
+
+
 class PickPlan:
     """Owns a gtomx_pick_plan (scratch for d_PickSubTomograms on n^3 volumes)."""
 
@@ -625,6 +630,10 @@ class PickPlan:
 
     def enable_timing(self, on=True):
         check(lib().gtomx_pick_plan_enable_timing(self.handle, int(bool(on))), "enable_timing")
+
+    def set_rows_variant(self, variant):
+        """ROWS_PLAIN or ROWS_PREFETCH: the row-pass kernel of the 2D search on images above 256."""
+        check(lib().gtomx_pick_plan_set_rows_variant(self.handle, int(variant)), "gtomx_pick_plan_set_rows_variant")
 
     def timings(self, reset=True):
         ms = (ctypes.c_double * 3)()
@@ -1034,14 +1043,16 @@ class Micrograph(ctypes.Structure):
 
 class MicrographBatch:
     """gtomx_mpick: the 2D search over micrographs of several sides, one plan per side (batches from
-    `batches`, e.g. the autotuner's table), per-side orientation loops captured as hipGraphs."""
+    `batches` and row variants from `variants`, e.g. the autotuner's table), per-side orientation loops
+    captured as hipGraphs."""
 
-    def __init__(self, p, sides, batches=None):
+    def __init__(self, p, sides, batches=None, variants=None):
         sides = [int(n) for n in sides]
         h = _P()
         s = (ctypes.c_int * len(sides))(*sides)
         b = (ctypes.c_uint * len(sides))(*(batches or [0] * len(sides)))
-        check(lib().gtomx_mpick_create(ctypes.byref(h), int(p), s, b, len(sides)), "gtomx_mpick_create")
+        v = (ctypes.c_int * len(sides))(*(variants or [ROWS_PLAIN] * len(sides)))
+        check(lib().gtomx_mpick_create(ctypes.byref(h), int(p), s, b, v, len(sides)), "gtomx_mpick_create")
         self.handle, self.p, self.sides = h, int(p), sides
 
     def close(self):
@@ -1057,10 +1068,12 @@ class MicrographBatch:
 
     def info(self):
         b = (ctypes.c_uint * len(self.sides))()
+        v = (ctypes.c_int * len(self.sides))()
         cap = ctypes.c_ulonglong()
         rep = ctypes.c_ulonglong()
-        check(lib().gtomx_mpick_info(self.handle, b, ctypes.byref(cap), ctypes.byref(rep)), "gtomx_mpick_info")
-        return {"batches": dict(zip(self.sides, list(b))), "captures": cap.value, "replays": rep.value}
+        check(lib().gtomx_mpick_info(self.handle, b, v, ctypes.byref(cap), ctypes.byref(rep)), "gtomx_mpick_info")
+        return {"batches": dict(zip(self.sides, list(b))), "variants": dict(zip(self.sides, list(v))),
+                "captures": cap.value, "replays": rep.value}
 
     def run(self, projector_ft, images, angles, use_graphs=True, stream=None):
         """images: list of dicts with 'spectrum' (complex64 (n, n/2+1)), 'ctf' (float32 (n, n/2+1)),
@@ -1075,13 +1088,15 @@ class MicrographBatch:
                                     1 if use_graphs else 0, _stream(stream)), "gtomx_mpick_run")
 
 
-def mpick_tune(p, n, projector_ft, batches, angles, repeats=1, stream=None):
+def mpick_tune(p, n, projector_ft, batches, angles, repeats=1, stream=None, variants=None):
     """Milliseconds per orientation of the 2D search's orientation loop on an n x n image for each
-    candidate batch (gtomx_mpick_tune; infinity where the plan does not fit)."""
+    candidate (batches[c], variants[c] or ROWS_PLAIN) (gtomx_mpick_tune; infinity where the plan does
+    not fit)."""
     a = _angles_arr(angles)
     b = (ctypes.c_uint * len(batches))(*[int(x) for x in batches])
+    v = (ctypes.c_int * len(batches))(*(variants or [ROWS_PLAIN] * len(batches)))
     out = (ctypes.c_float * len(batches))()
-    check(lib().gtomx_mpick_tune(int(p), int(n), _ptr(projector_ft), b, len(batches), a.ctypes.data, a.shape[0],
+    check(lib().gtomx_mpick_tune(int(p), int(n), _ptr(projector_ft), b, v, len(batches), a.ctypes.data, a.shape[0],
                                  int(repeats), out, _stream(stream)), "gtomx_mpick_tune")
     return [float(x) for x in out]
 

@@ -232,10 +232,10 @@ template <int L1, int L2> struct LongRow {
 // the entries j + R2 k1 it read itself), so a pass needs no other LDS.
 template <int LN, int NL, int RWG, int DIR, int PITCH, class Pos, class Out>
 __device__ __forceinline__ void long_row_pass(float2* buf, float2* sink, const float2* __restrict__ tw, Pos pos,
-                                              Out store) {
+                                              Out store, int t) {
   constexpr int R1 = ColShape<LN>::R1, R2 = ColShape<LN>::R2, ITEMS = RWG * NL * R2,
                 ROUNDS = (ITEMS + 255) / 256;
-  const int t = threadIdx.x, j = t % R2;
+  const int j = t % R2;
   for (int rd = 0; rd < ROUNDS; ++rd) {
     const int item = rd * 256 + t;
     const bool active = item < ITEMS;
@@ -258,16 +258,16 @@ __device__ __forceinline__ void long_row_pass(float2* buf, float2* sink, const f
 }
 
 template <int L1, int L2, int RWG, int DIR>
-__device__ __forceinline__ void long_row_passes(float2* buf, float2* sink, const float2* __restrict__ tw) {
+__device__ __forceinline__ void long_row_passes(float2* buf, float2* sink, const float2* __restrict__ tw, int t) {
   using LR = LongRow<L1, L2>;
   // pass A: the L2 lines {n2 + L2 n1} of L1 points, times W_NC^{n2 k1}
   long_row_pass<L1, L2, RWG, DIR, LR::PITCH>(
       buf, sink, tw, [](int n2, int i) { return LR::phys(n2 + L2 * i); },
-      [&](int n2, int k1, float2 x) { return cmul(x, twiddle_rt<DIR>(tw, LR::NC, n2 * k1)); });
+      [&](int n2, int k1, float2 x) { return cmul(x, twiddle_rt<DIR>(tw, LR::NC, n2 * k1)); }, t);
   __syncthreads();
   // pass B: the L1 contiguous groups of L2 points
   long_row_pass<L2, L1, RWG, DIR, LR::PITCH>(
-      buf, sink, tw, [](int k1, int i) { return LR::phys(L2 * k1 + i); }, [](int, int, float2 x) { return x; });
+      buf, sink, tw, [](int k1, int i) { return LR::phys(L2 * k1 + i); }, [](int, int, float2 x) { return x; }, t);
   __syncthreads();
 }
 
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_rows_r2c_long(const float* __restrict__
   const float2* src = reinterpret_cast<const float2*>(in) + row0 * NC;
   for (int i = t; i < nr * NC; i += 256) buf[(i / NC) * LR::PITCH + LR::phys(i % NC)] = src[i];
   __syncthreads();
-  long_row_passes<L1, L2, RWG, -1>(buf, sink, tw);
+  long_row_passes<L1, L2, RWG, -1>(buf, sink, tw, t);
   float2* dst = out + row0 * NH;
   for (int i = t; i < nr * NH; i += 256) {
     const int r = i / NH, k = i % NH;
@@ -315,31 +315,38 @@ __global__ __launch_bounds__(256) void k_rows_r2c_long(const float* __restrict__
   }
 }
 
+// Half spectrum -> Z for the pair (k, NC - k) of a row: a = X[k], b = X[NC - k]
+// (the DC and Nyquist bins at k = 0 taken as real); both results into LDS.
+template <int L1, int L2>
+__device__ __forceinline__ void long_row_z_pair(float2* row, int k, float2 a, float2 b,
+                                                const float2* __restrict__ tw) {
+  using LR = LongRow<L1, L2>;
+  constexpr int NC = LR::NC;
+  if (k == 0) {
+    a.y = 0.f;
+    b.y = 0.f;
+  }
+  const float2 ac = cconj(a), bc = cconj(b);
+  row[LR::phys(k)] = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle_rt<1>(tw, LR::N, k), csub(a, bc))));
+  if (k > 0)
+    row[LR::phys(NC - k)] = cadd(cadd(b, ac), mul_pj<1>(cmul(twiddle_rt<1>(tw, LR::N, NC - k), csub(b, ac))));
+}
+
 // Loads nr rows of half spectra (pitch NH in `src`) into LDS and leaves their
-// inverse Z (z[m] = x[2m] + i x[2m+1], unscaled) at zpos(m) of each row.
+// inverse Z (z[m] = x[2m] + i x[2m+1], unscaled) at zpos(m) of each row. Each
+// thread reads its pairs (k, NC - k) from memory and writes Z directly (both
+// reads coalesce: k rises and NC - k falls along the wavefront).
 template <int L1, int L2, int RWG>
 __device__ __forceinline__ void long_row_c2r_core(float2* buf, float2* sink, const float2* __restrict__ src, int nr,
-                                                  const float2* __restrict__ tw) {
+                                                  const float2* __restrict__ tw, int t) {
   using LR = LongRow<L1, L2>;
   constexpr int NC = LR::NC, NH = LR::NH, NP = NC / 2 + 1;
-  const int t = threadIdx.x;
-  for (int i = t; i < nr * NH; i += 256) buf[(i / NH) * LR::PITCH + LR::phys(i % NH)] = src[i];
-  __syncthreads();
-  for (int i = t; i < nr * NP; i += 256) {  // half spectrum -> Z, one thread per pair (k, NC - k)
+  for (int i = t; i < nr * NP; i += 256) {
     const int r = i / NP, k = i % NP;
-    float2* row = buf + r * LR::PITCH;
-    float2 a = row[LR::phys(k)], b = row[LR::phys(NC - k)];
-    if (k == 0) {
-      a.y = 0.f;
-      b.y = 0.f;
-    }
-    const float2 ac = cconj(a), bc = cconj(b);
-    row[LR::phys(k)] = cadd(cadd(a, bc), mul_pj<1>(cmul(twiddle_rt<1>(tw, LR::N, k), csub(a, bc))));
-    if (k > 0)
-      row[LR::phys(NC - k)] = cadd(cadd(b, ac), mul_pj<1>(cmul(twiddle_rt<1>(tw, LR::N, NC - k), csub(b, ac))));
+    long_row_z_pair<L1, L2>(buf + r * LR::PITCH, k, src[r * NH + k], src[r * NH + NC - k], tw);
   }
   __syncthreads();
-  long_row_passes<L1, L2, RWG, 1>(buf, sink, tw);
+  long_row_passes<L1, L2, RWG, 1>(buf, sink, tw, t);
 }
 
 template <int L1, int L2, int RWG>
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(256) void k_rows_c2r_long(const float2* __restrict_
   const int t = threadIdx.x;
   int nr;
   const long long row0 = long_row0<RWG>(nrows, per, band, &nr);
-  long_row_c2r_core<L1, L2, RWG>(buf, sink, in + row0 * NH, nr, tw);
+  long_row_c2r_core<L1, L2, RWG>(buf, sink, in + row0 * NH, nr, tw, t);
   float2* dst = reinterpret_cast<float2*>(out) + row0 * NC;
   for (int i = t; i < nr * NC; i += 256) dst[i] = cscale(buf[(i / NC) * LR::PITCH + LR::zpos(i % NC)], scale);
 }
@@ -361,14 +368,18 @@ __global__ __launch_bounds__(256) void k_rows_c2r_long(const float2* __restrict_
 // The search's row pass for images above 256: the row C2R of every
 // orientation of a batch (spectra H apart in G), each value times scale[o],
 // folded into best / index in orientation order (strict >, first index on
-// ties). The workgroup's rows keep their running maxima in registers.
-template <int L1, int L2, int RWG>
+// ties). The workgroup's rows keep their running maxima in registers. With PF
+// the pairs of orientation o + 1 are loaded into registers while orientation
+// o is transformed (the loads are in flight across the passes; more VGPRs, so
+// fewer resident waves: the autotuner chooses per image side).
+template <int L1, int L2, int RWG, bool PF>
 __global__ __launch_bounds__(256) void k_pick_rows_max_long(const float2* __restrict__ G, int nb, long long H,
                                                             int nrows, int angle0, const float* __restrict__ scale,
                                                             float* __restrict__ best, int32_t* __restrict__ bidx,
                                                             const float2* __restrict__ tw) {
   using LR = LongRow<L1, L2>;
-  constexpr int NC = LR::NC, NH = LR::NH, Q = (RWG * NC + 255) / 256;
+  constexpr int NC = LR::NC, NH = LR::NH, NP = NC / 2 + 1, Q = (RWG * NC + 255) / 256,
+                QP = (RWG * NP + 255) / 256;
   __shared__ float2 buf[RWG * LR::PITCH];
   __shared__ float2 sink[1];
   const int t = threadIdx.x;
@@ -388,14 +399,44 @@ __global__ __launch_bounds__(256) void k_pick_rows_max_long(const float2* __rest
     bi[2 * q] = a.x;
     bi[2 * q + 1] = a.y;
   }
+  float2 pa[PF ? QP : 1], pb[PF ? QP : 1];  // prefetched pairs (k, NC - k) of the next orientation
+  auto fetch = [&](int o) {
+    const float2* src = G + (size_t)o * H + (size_t)row0 * NH;
+#pragma unroll
+    for (int q = 0; q < QP; ++q) {
+      const int i = t + 256 * q;
+      if (i < nr * NP) {
+        const int r = i / NP, k = i % NP;
+        pa[q] = src[r * NH + k];
+        pb[q] = src[r * NH + NC - k];
+      }
+    }
+  };
+  if constexpr (PF) fetch(0);
   for (int o = 0; o < nb; ++o) {
     const float sc = scale[o];
+    // The transform's LDS addresses and twiddles are recomputed from an opaque
+    // copy of the thread index in each iteration, rather than hoisted out of
+    // the loop into registers (which costs occupancy).
+    int tt = t;
+    asm volatile("" : "+v"(tt));
     __syncthreads();  // the previous orientation's reads of buf are done
-    long_row_c2r_core<L1, L2, RWG>(buf, sink, G + (size_t)o * H + (size_t)row0 * NH, nr, tw);
+    if constexpr (PF) {
+#pragma unroll
+      for (int q = 0; q < QP; ++q) {
+        const int i = tt + 256 * q;
+        if (i < nr * NP) long_row_z_pair<L1, L2>(buf + (i / NP) * LR::PITCH, i % NP, pa[q], pb[q], tw);
+      }
+      if (o + 1 < nb) fetch(o + 1);
+      __syncthreads();
+      long_row_passes<L1, L2, RWG, 1>(buf, sink, tw, tt);
+    } else {
+      long_row_c2r_core<L1, L2, RWG>(buf, sink, G + (size_t)o * H + (size_t)row0 * NH, nr, tw, tt);
+    }
     const int32_t oi = angle0 + o;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int i = t + 256 * q;
+      const int i = tt + 256 * q;
       if (i < nr * NC) {
         const float2 z = buf[(i / NC) * LR::PITCH + LR::zpos(i % NC)];
         const float c0 = z.x * sc, c1 = z.y * sc;
@@ -673,7 +714,7 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned b
 // The search's row pass above 256 (k_pick_rows_max_long): nrows rows per
 // spectrum, nb spectra H apart.
 int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, int angle0, const float* scale,
-                       float* best, int32_t* idx, hipStream_t s) {
+                       float* best, int32_t* idx, hipStream_t s, int variant) {
   const float2* tw = nullptr;
   GX_TRY(twiddles(&tw));
   auto go = [&](auto kernel, int rwg) -> int {
@@ -684,7 +725,8 @@ int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, i
   };
   switch (n / 2) {
 #define GX_LONG(NC, A, B, W) \
-  case NC: return go(k_pick_rows_max_long<A, B, W>, W);
+  case NC:                   \
+    return variant ? go(k_pick_rows_max_long<A, B, W, true>, W) : go(k_pick_rows_max_long<A, B, W, false>, W);
     GX_LONG(256, 16, 16, 8)
     GX_LONG(384, 16, 24, 8)
     GX_LONG(512, 16, 32, 4)

@@ -151,7 +151,8 @@ using namespace gx;
 
 extern "C" {
 
-int gtomx_mpick_create(gtomx_mpick** out, int p, const int* h_sides, const unsigned* h_batches, unsigned nsides) {
+int gtomx_mpick_create(gtomx_mpick** out, int p, const int* h_sides, const unsigned* h_batches,
+                       const int* h_variants, unsigned nsides) {
   if (!out) return fail(GTOMX_ERR_INVALID_ARGUMENT, "object pointer is NULL");
   *out = nullptr;
   if (!h_sides || nsides == 0 || nsides > 64) return fail(GTOMX_ERR_INVALID_ARGUMENT, "1 to 64 image sides");
@@ -177,6 +178,7 @@ int gtomx_mpick_create(gtomx_mpick** out, int p, const int* h_sides, const unsig
     SizeSlot& sl = mp->slots.back();
     sl.n = n;
     int st = gtomx_pick_plan_create(&sl.plan, gtomx_int3{p, p, p}, gtomx_int3{n, n, 1}, h_batches ? h_batches[k] : 0);
+    if (st == GTOMX_OK && h_variants) st = gtomx_pick_plan_set_rows_variant(sl.plan, h_variants[k]);
     const size_t H = (size_t)(n / 2 + 1) * n, V = (size_t)n * n;
     void* d = nullptr;
     if (st == GTOMX_OK && (st = dev_alloc(&d, H * sizeof(float2))) == GTOMX_OK) sl.E = static_cast<float2*>(d);
@@ -198,11 +200,13 @@ int gtomx_mpick_destroy(gtomx_mpick* mp) {
   return GTOMX_OK;
 }
 
-int gtomx_mpick_info(const gtomx_mpick* mp, unsigned* h_batches, unsigned long long* captures,
+int gtomx_mpick_info(const gtomx_mpick* mp, unsigned* h_batches, int* h_variants, unsigned long long* captures,
                      unsigned long long* replays) {
   if (!mp) return fail(GTOMX_ERR_INVALID_ARGUMENT, "object is NULL");
-  if (h_batches)
-    for (size_t k = 0; k < mp->slots.size(); ++k) h_batches[k] = mp->slots[k].plan->B;
+  for (size_t k = 0; k < mp->slots.size(); ++k) {
+    if (h_batches) h_batches[k] = mp->slots[k].plan->B;
+    if (h_variants) h_variants[k] = mp->slots[k].plan->rows_variant;
+  }
   if (captures) *captures = mp->captures;
   if (replays) *replays = mp->replays;
   return GTOMX_OK;
@@ -260,9 +264,9 @@ int gtomx_mpick_run(gtomx_mpick* mp, const gtomx_complex* d_projectorft, const g
   return GTOMX_OK;
 }
 
-int gtomx_mpick_tune(int p, int n, const gtomx_complex* d_projectorft, const unsigned* h_batches, unsigned ncand,
-                     const gtomx_float3* h_angles, unsigned nangles, int repeats, float* h_ms_per_orientation,
-                     void* stream) {
+int gtomx_mpick_tune(int p, int n, const gtomx_complex* d_projectorft, const unsigned* h_batches,
+                     const int* h_variants, unsigned ncand, const gtomx_float3* h_angles, unsigned nangles,
+                     int repeats, float* h_ms_per_orientation, void* stream) {
   if (!d_projectorft || !h_batches || !h_angles || !h_ms_per_orientation || ncand == 0 || nangles == 0 ||
       repeats < 1)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "bad autotuner arguments");
@@ -282,6 +286,11 @@ int gtomx_mpick_tune(int p, int n, const gtomx_complex* d_projectorft, const uns
     gtomx_pick_plan* plan = nullptr;
     if (gtomx_pick_plan_create(&plan, gtomx_int3{p, p, p}, gtomx_int3{n, n, 1}, h_batches[c]) != GTOMX_OK)
       continue;  // a batch that does not fit is skipped (its time stays infinite)
+    if (h_variants && gtomx_pick_plan_set_rows_variant(plan, h_variants[c]) != GTOMX_OK) {
+      gtomx_pick_plan_destroy(plan);
+      result = GTOMX_ERR_INVALID_ARGUMENT;
+      break;
+    }
     const float oversample = (float)p / (float)n;
     Projector P;
     CentredProjector C{};

@@ -999,7 +999,7 @@ int run_batch2d_any(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const
   }));
   return timed(pl, 2, s, [&]() -> int {
     GX_TRY(image_cols_pass2_inverse(n, pl->scr, pl->G, (unsigned)nb, s));
-    return pick_rows_max_long(n, pl->G, nb, (long long)pl->H, n, angle0, pl->scale, best, idx, s);
+    return pick_rows_max_long(n, pl->G, nb, (long long)pl->H, n, angle0, pl->scale, best, idx, s, pl->rows_variant);
   });
 }
 
@@ -1333,6 +1333,14 @@ int gtomx_pick_plan_destroy(gtomx_pick_plan* plan) {
   if (!plan) return GTOMX_OK;
   // Buffers may still be in use by the last call's work (plan_release waits for it).
   plan_release(plan);
+  return GTOMX_OK;
+}
+
+int gtomx_pick_plan_set_rows_variant(gtomx_pick_plan* plan, int variant) {
+  if (!plan) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (variant != GTOMX_ROWS_PLAIN && variant != GTOMX_ROWS_PREFETCH)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "rows variant must be GTOMX_ROWS_PLAIN or GTOMX_ROWS_PREFETCH");
+  plan->rows_variant = variant;
   return GTOMX_OK;
 }
 

@@ -56,6 +56,7 @@ struct gtomx_pick_plan {
   gx::StreamFence fence;  // the plan's buffers are reused by calls on any stream
   // kernel timing
   bool timing = false;
+  int rows_variant = 0;  // GTOMX_ROWS_*: the row-pass kernel of images above 256
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   struct Pending { int stage; size_t a, b; };

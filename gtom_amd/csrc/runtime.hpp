@@ -166,9 +166,10 @@ int volume_r2c_banded(const float* in, float2* out, gtomx_int3 dims, unsigned nb
 int fft_c2r_cols_inplace(float2* data, gtomx_int3 dims, unsigned batch, hipStream_t s);
 // For n x n images with n > 256: the row C2R of nb spectra (H apart in G, in
 // place after fft_c2r_cols_inplace), times scale[o], folded into best / idx in
-// orientation order from angle0 (strict >).
+// orientation order from angle0 (strict >). variant 1 prefetches the next
+// orientation's spectrum rows into registers during each transform.
 int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, int angle0, const float* scale,
-                       float* best, int32_t* idx, hipStream_t s);
+                       float* best, int32_t* idx, hipStream_t s, int variant = 0);
 
 // Fourier-space resizing (resize.hip): op is one of the GTOMX_FFT_* values.
 constexpr int FFT_CROP = GTOMX_FFT_CROP, FFT_PAD = GTOMX_FFT_PAD, FFT_FULL_CROP = GTOMX_FFT_FULL_CROP,

@@ -1,7 +1,8 @@
 """Autotuner of the micrograph batch (gtomx_mpick): for each image side it times
 the 2D search's orientation loop (gtomx_mpick_tune: one warm-up, then timed
-runs with HIP events) at every candidate batch that fits, and writes the fastest
-batch per side to gtom_amd/tuning/micrograph_batches.json, which
+runs with HIP events) at every candidate batch that fits, with each row-pass
+kernel (GTOMX_ROWS_PLAIN / GTOMX_ROWS_PREFETCH), and writes the fastest
+(batch, variant) per side to gtom_amd/tuning/micrograph_batches.json, which
 MicrographBatch users (bench.py's micrograph row) read.
 
     python -m gtom_amd.tune_micrographs [--sides 1024 2048 3072 4096] [--out PATH]
@@ -23,12 +24,12 @@ CANDIDATES = {1024: [16, 32, 64, 128, 240], 2048: [8, 16, 32, 64], 3072: [4, 8, 
 
 
 def load_table(path=TABLE):
-    """{side: batch} from the committed table, or {} when there is none."""
+    """{side: (batch, rows variant)} from the committed table, or {} when there is none."""
     if not os.path.exists(path):
         return {}
     with open(path) as f:
         js = json.load(f)
-    return {int(k): int(v["batch"]) for k, v in js["sides"].items()}
+    return {int(k): (int(v["batch"]), int(v.get("variant", 0))) for k, v in js["sides"].items()}
 
 
 def tune(sides, box=64, orientations=64, repeats=2, log=print):
@@ -42,15 +43,18 @@ def tune(sides, box=64, orientations=64, repeats=2, log=print):
     angles = gx.fibonacci_angles(orientations // 8, 8)
     out = {}
     for n in sides:
-        cands = CANDIDATES.get(n, [8, 16, 32, 64])
+        batches = CANDIDATES.get(n, [8, 16, 32, 64])
+        variants = [gx.ROWS_PLAIN, gx.ROWS_PREFETCH] if n > 256 else [gx.ROWS_PLAIN]
+        cands = [(b, v) for v in variants for b in batches]
         t0 = time.time()
-        ms = gx.mpick_tune(p, n, projft, cands, angles, repeats=repeats)
+        ms = gx.mpick_tune(p, n, projft, [b for b, _ in cands], angles, repeats=repeats,
+                           variants=[v for _, v in cands])
         best = int(np.argmin(ms))
-        out[str(n)] = {"batch": int(cands[best]), "ms_per_orientation": {str(c): (None if not np.isfinite(m) else
-                                                                                  round(m, 5))
-                                                                          for c, m in zip(cands, ms)}}
-        log(f"[tune] {n}^2: {dict(zip(cands, [round(m, 4) for m in ms]))} ms/orientation -> batch {cands[best]} "
-            f"({time.time() - t0:.1f} s)")
+        out[str(n)] = {"batch": int(cands[best][0]), "variant": int(cands[best][1]),
+                       "ms_per_orientation": {f"{b}/{v}": (None if not np.isfinite(m) else round(m, 5))
+                                              for (b, v), m in zip(cands, ms)}}
+        log(f"[tune] {n}^2: {dict(zip([f'{b}/{v}' for b, v in cands], [round(m, 4) for m in ms]))} "
+            f"ms/orientation (batch/variant) -> {cands[best]} ({time.time() - t0:.1f} s)")
     return out
 
 

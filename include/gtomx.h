@@ -627,6 +627,16 @@ int gtomx_pick_plan_destroy(gtomx_pick_plan* plan);
 /* Orientations per pass and device bytes owned by the plan. */
 int gtomx_pick_plan_info(const gtomx_pick_plan* plan, unsigned* batch, size_t* device_bytes);
 
+/* Row-pass kernel of the 2D search on images above 256 (no reference
+ * counterpart; the results are identical, only the schedule differs):
+ * GTOMX_ROWS_PLAIN loads each orientation's spectrum rows when it starts,
+ * GTOMX_ROWS_PREFETCH loads the next orientation's rows into registers while
+ * the current one is transformed (more registers per thread, fewer resident
+ * waves). The micrograph autotuner picks one per image side. */
+#define GTOMX_ROWS_PLAIN 0
+#define GTOMX_ROWS_PREFETCH 1
+int gtomx_pick_plan_set_rows_variant(gtomx_pick_plan* plan, int variant);
+
 /* Kernel timing: when enabled, every launch of the search kernels is bracketed
  * by HIP events on the plan's stream; gtomx_pick_plan_timings synchronises and
  * returns, per stage k in {0: project+z, 1: plane correction + y, 2: x+max},
@@ -717,14 +727,16 @@ typedef struct {
 /* The 2D branch of d_PickSubTomograms (src/Correlation/SubTomograms.cu:34-36,
  * include/Correlation.cuh:97-109) over images of several sizes: one search plan
  * per side in h_sides (h_batches: orientations per pass for each side, 0 for the
- * plan default; normally the autotuner's table), each with device slots for one
+ * plan default; h_variants: GTOMX_ROWS_* per side, NULL for GTOMX_ROWS_PLAIN;
+ * normally both from the autotuner's table), each with device slots for one
  * image. The object owns a stream of its own (default priority). */
 typedef struct gtomx_mpick gtomx_mpick;
 int gtomx_mpick_create(gtomx_mpick** out, int projector_side, const int* h_sides, const unsigned* h_batches,
-                       unsigned nsides);
+                       const int* h_variants, unsigned nsides);
 int gtomx_mpick_destroy(gtomx_mpick* mp);
-/* Batches per side (in h_sides order), graphs captured and graph replays so far. */
-int gtomx_mpick_info(const gtomx_mpick* mp, unsigned* h_batches, unsigned long long* captures,
+/* Batches and row variants per side (in h_sides order; either may be NULL),
+ * graphs captured and graph replays so far. */
+int gtomx_mpick_info(const gtomx_mpick* mp, unsigned* h_batches, int* h_variants, unsigned long long* captures,
                      unsigned long long* replays);
 /* Searches every image over h_angles (oversample projector_side / n for an
  * image of side n; indices are positions in h_angles) with the projector
@@ -737,13 +749,14 @@ int gtomx_mpick_info(const gtomx_mpick* mp, unsigned* h_batches, unsigned long l
 int gtomx_mpick_run(gtomx_mpick* mp, const gtomx_complex* d_projectorft, const gtomx_micrograph* h_images,
                     unsigned nimages, const gtomx_float3* h_angles, unsigned nangles, int use_graphs,
                     void* stream);
-/* Autotuner measurement for one image side: for each candidate batch, the
+/* Autotuner measurement for one image side: for each candidate (batch
+ * h_batches[c], row variant h_variants[c] or GTOMX_ROWS_PLAIN when NULL), the
  * orientation loop over h_angles (one warm-up, then `repeats` timed runs with
  * events on `stream`), reported as milliseconds per orientation (infinity for a
  * batch whose plan does not fit). Synchronous. */
 int gtomx_mpick_tune(int projector_side, int n, const gtomx_complex* d_projectorft, const unsigned* h_batches,
-                     unsigned ncand, const gtomx_float3* h_angles, unsigned nangles, int repeats,
-                     float* h_ms_per_orientation, void* stream);
+                     const int* h_variants, unsigned ncand, const gtomx_float3* h_angles, unsigned nangles,
+                     int repeats, float* h_ms_per_orientation, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Split-precision transforms on the matrix cores (no reference counterpart) */

@@ -73,7 +73,9 @@ def test_mpick_equals_each_image_alone(gx, cuda, use_graphs):
     images = _images(gx, cuda, sides, seed=5)
     angles = gx.fibonacci_angles(10, 6)
     batches = {256: 24, 512: 16, 768: 8}
-    mb = gx.MicrographBatch(p, [256, 512, 768], [batches[n] for n in (256, 512, 768)])
+    # the long-row sides run the prefetching row kernel here, the plain one in _alone: same results
+    mb = gx.MicrographBatch(p, [256, 512, 768], [batches[n] for n in (256, 512, 768)],
+                            [gx.ROWS_PLAIN, gx.ROWS_PREFETCH, gx.ROWS_PREFETCH])
     for call in range(2):
         for im in images:
             im["best"].fill_(float("nan"))
@@ -85,13 +87,15 @@ def test_mpick_equals_each_image_alone(gx, cuda, use_graphs):
             assert np.array_equal(to_np(im["index"]), wi), (call, k, sides[k])
     info = mb.info()
     mb.close()
+    assert info["variants"] == {256: gx.ROWS_PLAIN, 512: gx.ROWS_PREFETCH, 768: gx.ROWS_PREFETCH}, info
     if use_graphs:
         assert info["captures"] == 3 and info["replays"] == 2 * len(sides), info
     else:
         assert info["captures"] == 0 and info["replays"] == 0, info
 
 
-def test_mpick_uneven_set_matches_oracle(gx, orc, cuda):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_mpick_uneven_set_matches_oracle(gx, orc, cuda, variant):
     import torch
     box = 32
     p = 2 * box
@@ -99,7 +103,7 @@ def test_mpick_uneven_set_matches_oracle(gx, orc, cuda):
     sides = [2048, 1024]
     images = _images(gx, cuda, sides, seed=8)
     angles = gx.fibonacci_angles(8, 5)
-    mb = gx.MicrographBatch(p, [1024, 2048])
+    mb = gx.MicrographBatch(p, [1024, 2048], variants=[variant, variant])
     mb.run(projft, images, angles)
     torch.cuda.synchronize()
     mb.close()
@@ -111,7 +115,7 @@ def test_mpick_uneven_set_matches_oracle(gx, orc, cuda):
         best, idx = to_np(im["best"]), to_np(im["index"])
         err = rel_linf(best, ref["best"][0])
         agree, cnt = argmax_agreement(idx, ref["idx"][0], ref["top1"][0], ref["top2"][0], CC_TOL)
-        print(f"mpick {n}^2: max-CC error {err:.2e}, argmax agreement {agree} on {cnt} pixels")
+        print(f"mpick {n}^2 rows variant {variant}: max-CC error {err:.2e}, argmax agreement {agree} on {cnt} pixels")
         assert err <= CC_TOL, (n, err)
         assert cnt > 0 and agree == 1.0, (n, agree, cnt)
 
@@ -121,6 +125,15 @@ def test_mpick_tune_times_every_candidate(gx, cuda):
     box = 16
     p = 2 * box
     projft = gx.prepare_projector(torch.from_numpy(_blob(box, 4)).to(cuda), p)
-    ms = gx.mpick_tune(p, 1024, projft, [4, 8, 16], gx.fibonacci_angles(4, 4), repeats=1)
-    print("mpick tune 1024^2 (ms per orientation, batches 4/8/16):", ms)
+    ms = gx.mpick_tune(p, 1024, projft, [4, 8, 16, 8], gx.fibonacci_angles(4, 4), repeats=1,
+                       variants=[gx.ROWS_PLAIN] * 3 + [gx.ROWS_PREFETCH])
+    print("mpick tune 1024^2 (ms per orientation, batches 4/8/16 plain, 8 prefetch):", ms)
     assert all(np.isfinite(m) and m > 0 for m in ms), ms
+
+
+def test_rows_variant_rejects_unknown_value(gx, cuda):
+    plan = gx.PickPlan(32, 512, 8, ndims=2)
+    plan.set_rows_variant(gx.ROWS_PREFETCH)
+    with pytest.raises(RuntimeError, match="rows variant"):
+        plan.set_rows_variant(2)
+    plan.close()
