@@ -26,6 +26,7 @@ import numpy as np
 
 METRIC = "voxel-orientations/s, 256³ tomo template match; learned vs plain at ≥99% recall"
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense
 STAGES = ("k_pick_project_z", "k_pick_y", "k_pick_x_max")
 
 
@@ -48,7 +49,7 @@ def parse():
     ap.add_argument("--learned-blocks", type=int, default=4,
                     help="held-out blocks timed by the learned comparison")
     ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
-                                      "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split"],
+                                      "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split", "pnet"],
                     default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
@@ -548,6 +549,64 @@ def row_split(args):
     return 0
 
 
+def pnet_flops_per_tile():
+    """Multiply-adds x 2 of one 32^3 tile through the proposal network (forward): four 3^3
+    convolutions (1->16 channels at 32^3, 16->16 at 16^3, 8^3, 8^3) and the 1024 x 501 head."""
+    conv = lambda side, ci: side ** 3 * 16 * ci * 27 * 2
+    fwd = conv(32, 1) + conv(16, 16) + conv(8, 16) + conv(8, 16) + 1024 * 501 * 2
+    # training: forward, data gradients of every layer but the first, weight gradients of all
+    train = 2 * fwd + (fwd - conv(32, 1))
+    return fwd, train
+
+
+def row_pnet(args):
+    """The orientation-proposal network's kernels (bf16 matrix-core implicit GEMMs): inference on
+    every 32^3 tile of a 256^3 block (standardisation, forward, top-16 proposals; 512 tiles a step)
+    and Adam steps on batches of 256 tiles. Matrix-core FLOP rates are reported against the dense
+    bf16 peak; the network is small enough that activations' memory traffic and the launches of its
+    15 kernels, not the matrix cores, set its speed."""
+    import torch
+    from gtom_amd import proposal as pr
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed)
+    net = pr.ProposalNet(max_tiles=512, seed=1)
+    vol = torch.from_numpy(rng.standard_normal((256, 256, 256), dtype=np.float32)).to(dev)
+    x = torch.from_numpy(rng.standard_normal((256, 32, 32, 32), dtype=np.float32)).to(dev)
+    y = torch.from_numpy(rng.integers(0, pr.NCLS, 256).astype(np.int32)).to(dev)
+    stream = torch.cuda.current_stream()
+
+    def infer():
+        k = net.load_volume(vol, stream=stream)
+        net.forward(k, stream=stream)
+        net.propose(k, 16, stream=stream, full=True)
+
+    def train():
+        net.load_tiles(x, stream=stream)
+        net.train_step(y, 1e-4, stream=stream)
+
+    reps = 20
+    sec_i = _timed_steps(args, lambda: [infer() for _ in range(reps)]) / reps
+    sec_t = _timed_steps(args, lambda: [train() for _ in range(reps)]) / reps
+    net.close()
+    fwd, trn = pnet_flops_per_tile()
+    tf_i = 512 * fwd / sec_i / 1e12
+    tf_t = 256 * trn / sec_t / 1e12
+    out = {"row": "pnet", "metric": "32^3 tiles/s through the orientation-proposal network (inference)",
+           "value": 512 / sec_i, "unit": "tiles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": sec_i * 1e3, "higher_is_better": True,
+           "dtype": "bf16 matrix-core operands, fp32 accumulation", "data": "synthetic (random tiles and labels)",
+           "config": {"workload": "inference: every 32^3 tile of one 256^3 block (512 tiles) with top-16 "
+                                  "proposals; training: Adam steps on 256-tile batches",
+                      "flops_per_tile_forward": fwd, "flops_per_tile_training": trn},
+           "training": {"tiles_per_s": 256 / sec_t, "ms_per_step": round(sec_t * 1e3, 3),
+                        "tflops": round(tf_t, 2)},
+           "roofline": {"bound": "mfma", "kernel": "whole inference pass", "achieved": round(tf_i, 2),
+                        "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tf_i / BF16_MFMA_PEAK_TFLOPS, 5), "traffic": None}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def row_diff2(args):
     """d_PickSubTomogramsDiff2: 8 volumes of 64^3, a 32^3 template (projector 64^3,
     oversample 1), relevant shifts 12^3, 500 orientations (50 x 10)."""
@@ -758,7 +817,7 @@ def main():
                 "local_peaks": row_local_peaks, "fft_micrograph": row_fft_micrograph,
                 "prob_picker_micrograph": row_prob_picker_micrograph,
                 "pick2d_micrograph": row_pick2d_micrograph, "micrographs": row_micrographs,
-                "split": row_split}[args.row](args)
+                "split": row_split, "pnet": row_pnet}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth

@@ -9,7 +9,7 @@
 //   score = sum|R_a ctf|^2 + sum|E|^2 - 2 Re X_a(d),  X_a(d) = sum_k Q_a(k) e^{-2 pi i k.d/n},
 // Q_a = R_a ctf conj(E), and X_a is separable along the three axes, so it is
 // three small batched complex contractions of Q_a with phase matrices (x, then
-// y, then z) over the relevant shifts only. The two norms are accumulated in
+// y, then z) over the relevant shifts only, each a GEMM on the matrix cores. The two norms are accumulated in
 // double, the score is formed in double and stored as float.
 #include <cmath>
 #include <vector>
@@ -68,23 +68,49 @@ __global__ __launch_bounds__(256) void k_diff2_sum(const double* __restrict__ pa
 }
 
 // out[r1*os1 + r2*os2 + j] = sum_k in[r1*is1 + r2*is2 + k*isk] * W[k*nj + j]
-// for r1 < n1, r2 < n2, j < nj, k < nk (complex).
-__global__ __launch_bounds__(256) void k_contract(const float2* __restrict__ in, float2* __restrict__ out,
-                                                  const float2* __restrict__ W, int n2, int nj, int nk,
-                                                  long long is1, long long is2, long long isk, long long os1,
-                                                  long long os2, long long total) {
-  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
-    const int j = (int)(t % nj);
-    const long long r = t / nj;
-    const long long r1 = r / n2, r2 = r % n2;
-    const float2* src = in + r1 * is1 + r2 * is2;
-    float re = 0.f, im = 0.f;
-    for (int k = 0; k < nk; ++k) {
-      const float2 a = src[k * isk], w = W[k * nj + j];
-      re = fmaf(a.x, w.x, fmaf(-a.y, w.y, re));
-      im = fmaf(a.x, w.y, fmaf(a.y, w.x, im));
-    }
-    out[r1 * os1 + r2 * os2 + j] = make_float2(re, im);
+// (complex) for rows r = r1 * n2 + r2 < rows, j < nj, k < nk, on the matrix
+// cores (v_mfma_f32_16x16x4_f32, fp32 operands and accumulation). The complex
+// product is the real product of A' (rows x 2K: re, im of each k interleaved)
+// and, per output part, B' (2K x 16): for the real part rows (W_re, -W_im),
+// for the imaginary part (W_im, W_re). A wave computes 16 rows x 16 complex columns (two
+// accumulator tiles) over K in steps of 2 complex (4 real) k. Operand lanes:
+// A row l % 16, B column l % 16, real k = 4 step + l / 16 for both; result
+// row 4 (l / 16) + e, column l % 16. Rows, columns and k past the ends
+// contribute zeros and store nothing. Grid: x = row tiles / 4 (four waves per
+// workgroup), y = column tiles.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_contract_mfma(const float2* __restrict__ in, float2* __restrict__ out,
+                                                       const float2* __restrict__ W, int n2, int nj, int nk,
+                                                       long long is1, long long is2, long long isk, long long os1,
+                                                       long long os2, long long rows) {
+  const int lane = threadIdx.x % 64;
+  const long long tile = (long long)blockIdx.x * 4 + threadIdx.x / 64;
+  const int c = lane % 16, q = lane / 16;  // operand row / column, real k within the step
+  const long long r = tile * 16 + c;        // this lane's A row
+  const bool rok = r < rows;
+  const float2* src = in + (rok ? (r / n2) * is1 + (r % n2) * is2 : 0);
+  const int j = blockIdx.y * 16 + c;  // this lane's B column
+  const bool jok = j < nj;
+  const int kq = q >> 1;   // complex k offset within the step
+  const bool im = q & 1;   // imaginary half of the complex k
+  f32x4 acc_re = {0.f, 0.f, 0.f, 0.f}, acc_im = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < nk; k0 += 2) {
+    const int k = k0 + kq;
+    const bool kok = k < nk;
+    const float2 av = (rok && kok) ? src[k * isk] : make_float2(0.f, 0.f);
+    const float2 w = (jok && kok) ? W[(size_t)k * nj + j] : make_float2(0.f, 0.f);
+    const float a_ = im ? av.y : av.x;
+    const float bre = im ? -w.y : w.x;
+    const float bim = im ? w.x : w.y;
+    acc_re = __builtin_amdgcn_mfma_f32_16x16x4f32(a_, bre, acc_re, 0, 0, 0);
+    acc_im = __builtin_amdgcn_mfma_f32_16x16x4f32(a_, bim, acc_im, 0, 0, 0);
+  }
+  if (!jok) return;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const long long ro = tile * 16 + 4 * q + e;
+    if (ro < rows) out[(ro / n2) * os1 + (ro % n2) * os2 + j] = make_float2(acc_re[e], acc_im[e]);
   }
 }
 
@@ -128,7 +154,19 @@ std::vector<float2> phases(int nk, int r, int n) {
   return w;
 }
 
-unsigned grid_of(long long total) { return (unsigned)std::min<long long>((total + 255) / 256, 65536); }
+// out[r1*os1 + r2*os2 + j] = sum_k in[r1*is1 + r2*is2 + k*isk] W[k*nj + j] over
+// n1 x n2 rows on the matrix cores.
+int contract(const float2* in, float2* out, const float2* W, long long n1, int n2, int nj, int nk, long long is1,
+             long long is2, long long isk, long long os1, long long os2, hipStream_t s) {
+  const long long rows = n1 * n2;
+  const long long tiles = (rows + 15) / 16;
+  const long long gx_ = (tiles + 3) / 4;
+  if (gx_ > 0x7fffffffLL) return fail(GTOMX_ERR_UNSUPPORTED, "contraction too large");
+  hipLaunchKernelGGL(k_contract_mfma, dim3((unsigned)gx_, (unsigned)((nj + 15) / 16)), dim3(256), 0, s, in, out, W,
+                     n2, nj, nk, is1, is2, isk, os1, os2, rows);
+  GX_LAUNCHED("k_contract_mfma");
+  return GTOMX_OK;
+}
 
 }  // namespace
 }  // namespace gx
@@ -201,22 +239,14 @@ extern "C" int gtomx_pick_subtomograms_diff2(const gtomx_complex* d_projectorft,
       hipLaunchKernelGGL(k_diff2_sum, dim3(1), dim3(256), 0, s, eparts.as<double>(), NBLK, esum.as<double>());
       GX_LAUNCHED("k_diff2_sum");
       // x: Q[a][kz][ky][kx] -> T1[a][kz][ky][dx]
-      long long tot = (long long)nb * n * n * rx;
-      hipLaunchKernelGGL(k_contract, dim3(grid_of(tot)), dim3(256), 0, s, R.as<float2>(), T1.as<float2>(),
-                         Wx.as<float2>(), 1, rx, nh, (long long)nh, 0LL, 1LL, (long long)rx, 0LL, tot);
-      GX_LAUNCHED("k_contract");
+      GX_TRY(contract(R.as<float2>(), T1.as<float2>(), Wx.as<float2>(), (long long)nb * n * n, 1, rx, nh,
+                      (long long)nh, 0LL, 1LL, (long long)rx, 0LL, s));
       // y: T1 rows (a*n + kz, dx) over ky -> T2[a][kz][dx][dy]
-      tot = (long long)nb * n * rx * ry;
-      hipLaunchKernelGGL(k_contract, dim3(grid_of(tot)), dim3(256), 0, s, T1.as<float2>(), T2.as<float2>(),
-                         Wy.as<float2>(), rx, ry, n, (long long)n * rx, 1LL, (long long)rx, (long long)rx * ry,
-                         (long long)ry, tot);
-      GX_LAUNCHED("k_contract");
+      GX_TRY(contract(T1.as<float2>(), T2.as<float2>(), Wy.as<float2>(), (long long)nb * n, rx, ry, n,
+                      (long long)n * rx, 1LL, (long long)rx, (long long)rx * ry, (long long)ry, s));
       // z: T2 rows (a, dx*ry + dy) over kz -> T3[a][dx][dy][dz]
-      tot = (long long)nb * rx * ry * rz;
-      hipLaunchKernelGGL(k_contract, dim3(grid_of(tot)), dim3(256), 0, s, T2.as<float2>(), T3.as<float2>(),
-                         Wz.as<float2>(), rx * ry, rz, n, (long long)n * rx * ry, 1LL, (long long)rx * ry,
-                         (long long)Vr, (long long)rz, tot);
-      GX_LAUNCHED("k_contract");
+      GX_TRY(contract(T2.as<float2>(), T3.as<float2>(), Wz.as<float2>(), (long long)nb, rx * ry, rz, n,
+                      (long long)n * rx * ry, 1LL, (long long)rx * ry, (long long)Vr, (long long)rz, s));
       hipLaunchKernelGGL(k_diff2_update, dim3((unsigned)((Vr + 255) / 256)), dim3(256), 0, s, T3.as<float2>(),
                          sums.as<double>(), esum.as<double>(), rx, ry, rz, (int)nb, (int)b0,
                          d_bestdiff2 + (size_t)v * Vr, d_bestangle + (size_t)v * Vr);

@@ -91,3 +91,43 @@ def test_benchmark_call_shape_matches_oracle(gx, orc, cuda):
         assert cnt > 0 and agree == 1.0, (v, agree, cnt)
     got_idx = to_np(idx)
     assert got_idx.min() >= off and got_idx.max() <= off + 39 and (got_idx >= 8992).any()
+
+
+def test_benchmark_call_shape_matches_oracle(gx, orc, cuda):
+    """The benchmark's own call shape: the two 256^3 blocks of the workload with one CTF each
+    (gtomx_ctf_3d, ctf_volume_stride = H), batch 32, and the last 40 of the 9000 orientations at
+    their own indices: the full batch 8960..8991 and the final partial batch 8992..8999, as the
+    whole-list call splits them (8960 = 280 x 32). The oracle gets the same CTF array per volume."""
+    import torch
+    from gtom_amd import synth
+    w = synth.cfg2(seed=9)
+    n, p, s = w["n"], w["p"], w["oversample"]
+    H = (n // 2 + 1) * n * n
+    first = 8960
+    angles = w["angles"][first:]
+    assert len(w["angles"]) == 9000 and len(angles) == 40
+    projft = gx.prepare_projector(torch.from_numpy(w["template"]).to(cuda), p)
+    exp_ft = gx.d_FFTR2C(torch.from_numpy(w["blocks"]).to(cuda))
+    ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=w["pixelsize"], Cs=2.7e-3, voltage=300e3, defocus=-float(d),
+                                       amplitude=0.07) for d in w["defoci"]], n, device=cuda)
+    plan = gx.PickPlan(p, n)
+    assert plan.batch == 32
+    nvol = w["blocks"].shape[0]
+    best = torch.empty((nvol, n, n, n), dtype=torch.float32, device=cuda)
+    idx = torch.empty((nvol, n, n, n), dtype=torch.int32, device=cuda)
+    gx.pick_begin(plan, nvol, best, idx)
+    gx.pick_update(plan, projft, s, exp_ft, ctf, angles, first, best, idx, ctf_volume_stride=H)
+    gx.pick_finish(plan, exp_ft, w["maskradius"], best, idx)
+    torch.cuda.synchronize()
+    plan.close()
+    ref = orc.pick(to_np(projft).astype(np.complex128), p, s, to_np(exp_ft).astype(np.complex128),
+                   to_np(ctf).reshape(nvol, -1).astype(np.float64), n, angles.astype(np.float64), w["maskradius"],
+                   ctf_per_volume=True)
+    b, i = to_np(best), to_np(idx)
+    assert i.min() >= first and i.max() < 9000
+    for v in range(nvol):
+        err = rel_linf(b[v], ref["best"][v])
+        agree, cnt = argmax_agreement(i[v] - first, ref["idx"][v], ref["top1"][v], ref["top2"][v], 1e-4)
+        print(f"block {v}: max-CC error {err:.2e}, argmax agreement {agree} on {cnt} voxels")
+        assert err <= 1e-4, (v, err)
+        assert cnt > 0 and agree == 1.0, (v, agree, cnt)
