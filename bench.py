@@ -10,8 +10,10 @@ d_PickSubTomograms call over both blocks and all orientations, with the block
 spectra, CTFs and projector already resident in HBM.
 
 Prints one JSON line. `value` is voxel-orientations per second (2 * 256^3 *
-9000 per step); `roofline` describes the kernel that takes the most time, from
-HIP events recorded around every launch on the stream the search runs on;
+9000 per step); `roofline` is the search pipeline's: the algorithmic bytes of one
+batch of orientations over one block divided by the summed time of the batch's
+kernels, from HIP events recorded around every launch on the stream the search
+runs on, with each kernel's own figures as a breakdown;
 `cpu_baseline` times the oracle's float32 restatement on this host's cores over
 a bounded sample (one block, the first K orientations).
 """
@@ -182,7 +184,10 @@ def learned_comparison(nblocks):
     template, 9000 orientations, SNR 0.3): per block both are timed end to end on the host clock
     (the learned time includes the network and its one synchronisation), and the exhaustive
     search's peaks (d_LocalPeaks above the workload's threshold) are recalled when the learned
-    map holds the same value and orientation there. None without trained weights."""
+    map holds the same value and orientation there. Two operating points: the shortlist point
+    (top-16 directions, fallback below confidence 0.5) and the recall point (a tile is skipped only
+    when the network gives "no peak" at least the recall confidence, every other tile is searched
+    over every direction); the metric's ratio is the recall point's. None without trained weights."""
     import torch
     from gtom_amd import proposal as pr
     from gtom_amd import train_proposal as tp
@@ -193,22 +198,31 @@ def learned_comparison(nblocks):
     net = pr.ProposalNet(max_tiles=512, params=np.load(tp.WEIGHTS)["params"])
     seeds = [tp.HELDOUT_SEEDS[0] + k for k in range(nblocks + 1)]
     tp.recall(w, net, seeds[:1])  # warm-up (plans, kernels)
-    res = tp.recall(w, net, seeds[1:])
+    points = {}
+    for name, (topk, conf) in (("shortlist", (tp.SHORTLIST_TOPK, tp.SHORTLIST_CONFIDENCE)),
+                               ("recall", (tp.RECALL_TOPK, tp.RECALL_CONFIDENCE))):
+        res = tp.recall(w, net, seeds[1:], topk, conf)
+        ex = sum(r["exhaustive_s"] for r in res)
+        le = sum(r["learned_s"] for r in res)
+        peaks = sum(r["peaks"] for r in res)
+        found = sum(r["found"] for r in res)
+        vo = len(res) * tp.N ** 3 * tp.NDIRS * tp.NPSI
+        points[name] = {"topk": topk, "confidence": conf, "exhaustive_vo_per_s": vo / ex,
+                        "learned_exhaustive_equivalent_vo_per_s": vo / le, "learned_over_plain": ex / le,
+                        "recall": found / max(peaks, 1), "peaks": peaks, "recalled": found,
+                        "tiles": {k: sum(r["stats"][k] for r in res) for k in res[0]["stats"]},
+                        "exhaustive_s": round(ex, 3), "learned_s": round(le, 3)}
+        log(f"learned ({name}): {points[name]}")
     net.close()
-    ex = sum(r["exhaustive_s"] for r in res)
-    le = sum(r["learned_s"] for r in res)
-    peaks = sum(r["peaks"] for r in res)
-    found = sum(r["found"] for r in res)
-    vo = len(res) * tp.N ** 3 * tp.NDIRS * tp.NPSI
-    stats = {k: sum(r["stats"][k] for r in res) for k in res[0]["stats"]}
     torch.cuda.synchronize()
-    return {"workload": f"{len(res)} held-out 256^3 blocks (seeds {seeds[1]}..{seeds[-1]}), 20 copies of the "
-                        f"48^3 template at SNR {tp.SNR}, 9000 orientations, top-16 directions per 32^3 tile, "
-                        "exhaustive fallback below confidence 0.5",
-            "exhaustive_vo_per_s": vo / ex, "learned_exhaustive_equivalent_vo_per_s": vo / le,
-            "learned_over_plain": ex / le, "target_ratio": 5.0, "recall": found / max(peaks, 1),
-            "peaks": peaks, "recalled": found, "tiles": stats, "exhaustive_s": round(ex, 3),
-            "learned_s": round(le, 3)}
+    head = points["recall"]
+    return {"workload": f"{nblocks} held-out 256^3 blocks (seeds {seeds[1]}..{seeds[-1]}), 20 copies of the "
+                        f"48^3 template at SNR {tp.SNR}, 9000 orientations, proposals per 32^3 tile",
+            "exhaustive_vo_per_s": head["exhaustive_vo_per_s"],
+            "learned_exhaustive_equivalent_vo_per_s": head["learned_exhaustive_equivalent_vo_per_s"],
+            "learned_over_plain": head["learned_over_plain"], "target_ratio": 5.0, "recall": head["recall"],
+            "peaks": head["peaks"], "recalled": head["recalled"], "exhaustive_s": head["exhaustive_s"],
+            "learned_s": head["learned_s"], "operating_points": points}
 
 
 def row_learned(args):

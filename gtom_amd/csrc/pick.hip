@@ -739,6 +739,64 @@ __global__ __launch_bounds__(TX * LineShape<L1>::R2) void k_pick2d_project_cols(
   if (t == 0) partial[((size_t)o * gridDim.x + blockIdx.x) * L2 + n2] = tot;
 }
 
+// ---------------------------------------------------------------------------
+// Volumes above 256 per side (n^3, n in {512, 768, 1024}), on the generic
+// transforms: pass 1 writes Q_o = E * conj(CTF R_o) over the whole half
+// spectrum of every orientation of the batch (zero outside the sphere and at
+// DC) with one sum of |R|^2 per workgroup and orientation; the x = 0 / n/2
+// plane correction and the scale as in the fused form; then the generic
+// column inverse along z and y in place and the long-row pass (x C2R, scale,
+// in-order max/argmax, pick_rows_max_long).
+// ---------------------------------------------------------------------------
+constexpr int P3_BINS = 4;  // consecutive-in-x bins per thread (stride 256)
+
+template <class PJ>
+__global__ __launch_bounds__(256) void k_pick3d_project_any(PJ P, const float* __restrict__ mats, int nb,
+                                                            const float2* __restrict__ E,
+                                                            const float* __restrict__ ctf, float2* __restrict__ G,
+                                                            float* __restrict__ partial, int n, long long H) {
+  __shared__ float red[4];
+  const int NH = n / 2 + 1, HALF = n / 2;
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * 256 * P3_BINS + t;
+  const __amdgpu_buffer_rsrc_t rs = sampler_rsrc(P);
+  float2 e[P3_BINS];
+  float cf[P3_BINS];
+  bool live[P3_BINS], ins[P3_BINS];
+  float fx[P3_BINS], fy[P3_BINS], fz[P3_BINS];
+#pragma unroll
+  for (int k = 0; k < P3_BINS; ++k) {
+    const long long i = base + 256LL * k;
+    live[k] = i < H;
+    const long long ii = live[k] ? i : 0;
+    const int ix = (int)(ii % NH), iy = (int)((ii / NH) % n), iz = (int)(ii / ((long long)NH * n));
+    const int y = signed_freq(iy, n), z = signed_freq(iz, n);
+    ins[k] = live[k] && (long long)ix * ix + (long long)y * y + (long long)z * z <= (long long)HALF * HALF;
+    fx[k] = (float)ix;
+    fy[k] = (float)y;
+    fz[k] = (float)z;
+    e[k] = ins[k] && ii != 0 ? E[ii] : make_float2(0.f, 0.f);  // DC of Q zeroed
+    cf[k] = ins[k] ? (ctf ? ctf[ii] : 1.f) : 0.f;
+  }
+  for (int o = 0; o < nb; ++o) {
+    float M[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) M[k] = mats[9 * o + k];
+    float acc = 0.f;
+    float2* Go = G + (size_t)o * H;
+#pragma unroll
+    for (int k = 0; k < P3_BINS; ++k) {
+      const float2 r = cscale(sample(rs, P, fmaf(M[2], fz[k], fmaf(M[1], fy[k], M[0] * fx[k])),
+                                     fmaf(M[5], fz[k], fmaf(M[4], fy[k], M[3] * fx[k])),
+                                     fmaf(M[8], fz[k], fmaf(M[7], fy[k], M[6] * fx[k])), ins[k]),
+                              cf[k]);
+      acc += r.x * r.x + r.y * r.y;
+      if (live[k]) Go[base + 256LL * k] = cmulc(e[k], r);
+    }
+    const float tot = block_sum(acc, red);
+    if (t == 0) partial[(size_t)o * gridDim.x + blockIdx.x] = tot;
+  }
+}
 
 }  // namespace gx
 
@@ -898,12 +956,57 @@ int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float
   return GTOMX_OK;
 }
 
+// Volumes above 256: pass 1 over the whole half spectrum, the plane correction
+// and the scale, the generic column inverse along z and y in place, then the
+// long-row pass folds every orientation into best / index in order.
+int herm_norm3_any(int n, const Projector& P, const float* mats, const float* ctf, float* corr, int nb,
+                   hipStream_t s) {
+  switch (n) {
+#define GX_CASE(N)                                                                                    \
+  case N:                                                                                             \
+    hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, corr); \
+    GX_LAUNCHED("k_pick_herm_norm");                                                                  \
+    return GTOMX_OK;
+    GX_CASE(512) GX_CASE(768) GX_CASE(1024)
+#undef GX_CASE
+  }
+  return fail(GTOMX_ERR_UNSUPPORTED, "volume side not supported");
+}
+
+template <class PJ>
+int run_batch3d_any(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float* mats, int nb,
+                    const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, hipStream_t s) {
+  const int n = pl->n;
+  const long long H = (long long)pl->H;
+  GX_TRY(timed(pl, 0, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick3d_project_any<PJ>, dim3((unsigned)pl->nparts), dim3(256), 0, s, PS, mats, nb, E, ctf,
+                       pl->G, pl->partial, n, H);
+    GX_LAUNCHED("k_pick3d_project_any");
+    return GTOMX_OK;
+  }));
+  GX_TRY(herm_norm3_any(n, P, mats, ctf, pl->corr, nb, s));
+  GX_TRY(timed(pl, 1, s, [&]() -> int {
+    hipLaunchKernelGGL(k_pick_scale, dim3(nb), dim3(256), 0, s, pl->partial, pl->nparts, pl->corr, HERM_BLOCKS,
+                       (double)pl->H, pl->scale);
+    GX_LAUNCHED("k_pick_scale");
+    return fft_c2r_cols_inplace(pl->G, gtomx_int3{n, n, n}, (unsigned)nb, s);
+  }));
+  return timed(pl, 2, s, [&]() -> int {
+    return pick_rows_max_long(n, pl->G, nb, H, n * n, angle0, pl->scale, best, idx, s, pl->rows_variant);
+  });
+}
+
 // Pass 1 samples the centred layout when every sampled q stays within
 // |q| < p/2 + 1/2 (oversample * n/2 <= p/2 + 1/4, which leaves room for the
 // rounding of q), and the reference's layout otherwise.
 int run_batch_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projector& P, const float* mats, int nb,
                 const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
                 hipStream_t s, const uint8_t* tmask = nullptr) {
+  if (pl->n > 256) {
+    if (tmask) return fail(GTOMX_ERR_UNSUPPORTED, "tile masks need volumes of side at most 256");
+    return C ? run_batch3d_any(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, s)
+             : run_batch3d_any(pl, P, P, mats, nb, E, ctf, best, idx, angle0, s);
+  }
   switch (pl->n) {
 #define GX_CASE(N)                                                                            \
   case N:                                                                                     \
@@ -1165,20 +1268,35 @@ int update_tiles(gtomx_pick_plan* plan, const gtomx_complex* d_projectorft, floa
   int st = GTOMX_OK;
   const bool centred = setup_projector(plan, d_projectorft, oversample, s, &P, &C, &st);
   GX_TRY(st);
+  // Consecutive groups with identical masks are one contiguous range of orientations and run
+  // in batches of the plan's size (the in-order update is the same); a mask with every tile
+  // set runs the unmasked kernels.
+  const uint8_t full = (uint8_t)((1u << nt) - 1u);
   unsigned run = 0;
   for (unsigned v = 0; v < nvolumes; ++v) {
     const float2* Ev = reinterpret_cast<const float2*>(d_expft) + (size_t)v * plan->H;
     const float* cv = d_ctf ? d_ctf + (size_t)v * ctf_volume_stride : nullptr;
     float* bv = d_best + (size_t)v * plan->V;
     int32_t* iv = d_idx + (size_t)v * plan->V;
-    for (unsigned g = 0; g < ngroups; ++g) {
+    for (unsigned g = 0; g < ngroups;) {
       const uint8_t* hm = h_masks + ((size_t)v * ngroups + g) * mbytes;
-      bool any = false;
-      for (size_t k = 0; k < mbytes && !any; ++k) any = hm[k] != 0;
-      if (!any) continue;
-      ++run;
-      GX_TRY(run_batch_n(plan, centred ? &C : nullptr, P, plan->mats + 9 * (size_t)g * group, (int)group, Ev, cv,
-                         bv, iv, (int)(g * group), tw, s, plan->tmasks + ((size_t)v * ngroups + g) * mbytes));
+      bool any = false, all = true;
+      for (size_t k = 0; k < mbytes; ++k) {
+        any |= hm[k] != 0;
+        all &= hm[k] == full;
+      }
+      unsigned g1 = g + 1;
+      while (g1 < ngroups && std::memcmp(hm, h_masks + ((size_t)v * ngroups + g1) * mbytes, mbytes) == 0) ++g1;
+      if (any) {
+        run += g1 - g;
+        const uint8_t* dm = all ? nullptr : plan->tmasks + ((size_t)v * ngroups + g) * mbytes;
+        for (unsigned o0 = g * group, o1 = g1 * group; o0 < o1; o0 += plan->B) {
+          const int nb = (int)std::min<unsigned>(plan->B, o1 - o0);
+          GX_TRY(run_batch_n(plan, centred ? &C : nullptr, P, plan->mats + 9 * (size_t)o0, nb, Ev, cv, bv, iv,
+                             (int)o0, tw, s, dm));
+        }
+      }
+      g = g1;
     }
   }
   if (groups_run) *groups_run = run;
@@ -1223,9 +1341,12 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   if (dv.x != dv.y || (!is2d && dv.x != dv.z))
     return fail(GTOMX_ERR_UNSUPPORTED, "volumes must be cubes (or images squares)");
   const bool big2d = is2d && dv.x > 256;  // micrographs: the generic-transform form of the 2D branch
-  if (big2d ? !fft_len_ok_any(dv.x) : !fft_len_ok(dv.x))
+  // volumes above 256: the generic-transform form (sides with a long-row pass)
+  const bool big3d = !is2d && (dv.x == 512 || dv.x == 768 || dv.x == 1024);
+  if (big2d ? !fft_len_ok_any(dv.x) : (!big3d && !fft_len_ok(dv.x)))
     return fail(GTOMX_ERR_UNSUPPORTED,
-                "volume side must be 2^a or 3*2^a in [8, 256]; image side also 2^a up to 4096 or 3*2^a up to 3072");
+                "volume side must be 2^a or 3*2^a in [8, 256], or 512, 768 or 1024; image side also 2^a up to 4096 "
+                "or 3*2^a up to 3072");
   if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 512)
     return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 512");
   if (batch > 256) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 256");
@@ -1238,16 +1359,19 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   pl->V = elements(dv);
   // above 256 the default batch keeps the spectra near 1 GiB (16 orientations at 4096^2)
   const unsigned big_b = (unsigned)std::max<size_t>(1, std::min<size_t>(240, (1ull << 30) / (pl->H * sizeof(float2))));
-  pl->B = batch ? batch : (is2d ? (big2d ? big_b : 240u) : (dv.x >= 128 ? 32u : 16u));
+  // volumes above 256: spectra of about 8 GiB per batch (15 orientations at 512^3, 1 at 1024^3)
+  const unsigned big3_b = (unsigned)std::max<size_t>(1, std::min<size_t>(32, (8ull << 30) / (pl->H * sizeof(float2))));
+  pl->B = batch ? batch : (is2d ? (big2d ? big_b : 240u) : (big3d ? big3_b : (dv.x >= 128 ? 32u : 16u)));
   pl->og = dv.x >= 128 ? 2 : 1;
   int L1 = 0, L2 = 0;
   if (big2d && fft_len_split(dv.x, &L1, &L2) != GTOMX_OK) {
     delete pl;
     return fail(GTOMX_ERR_UNSUPPORTED, "image side has no two-pass split");
   }
-  pl->nparts = big2d ? (dv.x / 2 + 1 + 63) / 64 * L2 : 0;
-  const std::vector<int> tiles = is2d ? std::vector<int>{0} : tiles_for(pl->n);
-  pl->ntiles = big2d ? 1 : (is2d ? g2_tiles_for(pl->n) : (int)tiles.size());
+  pl->nparts = big2d ? (dv.x / 2 + 1 + 63) / 64 * L2
+                     : (big3d ? (int)((pl->H + 256 * P3_BINS - 1) / (256 * P3_BINS)) : 0);
+  const std::vector<int> tiles = (is2d || big3d) ? std::vector<int>{0} : tiles_for(pl->n);
+  pl->ntiles = (big2d || big3d) ? 1 : (is2d ? g2_tiles_for(pl->n) : (int)tiles.size());
   int s = GTOMX_OK;
   hipError_t he = hipGetDevice(&pl->device);
   if (he != hipSuccess) {
@@ -1261,12 +1385,12 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
     if (s == GTOMX_OK) pl->device_bytes += bytes;
   };
   // The batch buffer also serves as the C2R workspace in pick_finish (H elements).
-  const size_t gstride = big2d ? pl->H : (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n));
+  const size_t gstride = (big2d || big3d) ? pl->H : (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n));
   alloc(&d, std::max((size_t)pl->B * gstride, pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
   // up to 4 pass-1 waves per tile; above 256, one sum per pass-1 workgroup
-  alloc(&d, (size_t)pl->B * (big2d ? (size_t)pl->nparts : (size_t)pl->ntiles * 4) * sizeof(float));
+  alloc(&d, (size_t)pl->B * ((big2d || big3d) ? (size_t)pl->nparts : (size_t)pl->ntiles * 4) * sizeof(float));
   pl->partial = static_cast<float*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->ntiles * sizeof(int));

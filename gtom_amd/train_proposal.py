@@ -21,6 +21,7 @@ strongest peak within LABEL_MARGIN voxels of its box, or "no peak".
     python -m gtom_amd.train_proposal label --first S --count N --out labels.npz
     python -m gtom_amd.train_proposal train --labels a.npz [b.npz ...] --out weights.npz
     python -m gtom_amd.train_proposal eval --weights weights.npz --first S --count N
+    python -m gtom_amd.train_proposal scores --weights weights.npz --first S --count N
 
 Peak recall (eval): an exhaustive peak is recalled when the learned search's
 map holds the exhaustive value at that voxel (the argmax orientation was
@@ -46,6 +47,15 @@ SNR = 0.3
 PEAK_EXTENT = 12
 PEAK_THRESHOLD = 7.5
 LABEL_MARGIN = 4
+# Operating points of the learned search. Shortlist: a tile takes its top-16 directions or "no peak",
+# whichever the network makes more probable, and every direction below confidence 0.5. Recall: with
+# top-k = all 500 directions the summed direction probability is 1 - P(no peak), so a tile is
+# skipped only when P(no peak) reaches the confidence and every other tile is searched over every
+# direction; the confidence is the lowest at which every exhaustive peak of the calibration
+# blocks (CALIBRATION_SEEDS, 418 peaks) lies in a searched tile (0.86 loses one).
+SHORTLIST_TOPK, SHORTLIST_CONFIDENCE = 16, 0.5
+RECALL_TOPK, RECALL_CONFIDENCE = 500, 0.87
+CALIBRATION_SEEDS = (300_000, 20)
 TRAIN_SEEDS = (100_000, 2000)
 HELDOUT_SEEDS = (200_000, 50)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "proposal_weights.npz")
@@ -290,6 +300,64 @@ def recall(w, net, seeds, topk=16, conf=0.5):
     return res
 
 
+def block_scores(w, net, seed, topk=16):
+    """One block: the exhaustive search's peaks [(x, y, z, value, direction)] and, per 32^3 tile
+    (tz, ty, tx order), the network's P(no peak) and top-k directions."""
+    blk, _, _, df = w.block(seed)
+    E, ctf = w.spectra(blk[None], [df])
+    bx, ix = w.exhaustive(E, ctf)
+    pk = w.peaks(bx[0])[0]
+    b, iv = bx[0].cpu().numpy(), ix[0].cpu().numpy()
+    k = net.load_tiles(_standardised_tiles(blk).contiguous())
+    net.forward(k)
+    dirs, _, _, _, pnp = net.propose(k, topk, full=True)
+    peaks = [(int(x), int(y), int(z), float(b[z, y, x]), int(iv[z, y, x]) // NPSI) for x, y, z in pk]
+    return peaks, pnp.cpu().numpy(), dirs.cpu().numpy()
+
+
+def tile_of(x, y, z, n=N):
+    nt = n // 32
+    return ((z // 32) * nt + y // 32) * nt + x // 32
+
+
+def detection_auc(blocks):
+    """Area under the ROC of 1 - P(no peak) separating the tiles that hold an exhaustive peak from
+    the tiles with no peak within LABEL_MARGIN voxels (0.5 for a scorer without information)."""
+    pos, neg = [], []
+    for peaks, pnp, _ in blocks:
+        hold = np.zeros(len(pnp), bool)
+        near = np.zeros(len(pnp), bool)
+        for x, y, z, _, _ in peaks:
+            hold[tile_of(x, y, z)] = True
+            for dz in (-LABEL_MARGIN, 0, LABEL_MARGIN):
+                for dy in (-LABEL_MARGIN, 0, LABEL_MARGIN):
+                    for dx in (-LABEL_MARGIN, 0, LABEL_MARGIN):
+                        xx, yy, zz = x + dx, y + dy, z + dz
+                        if 0 <= xx < N and 0 <= yy < N and 0 <= zz < N:
+                            near[tile_of(xx, yy, zz)] = True
+        pos.append(1 - pnp[hold])
+        neg.append(1 - pnp[~near])
+    pos, neg = np.concatenate(pos), np.concatenate(neg)
+    return float((pos[:, None] > neg[None, :]).mean() + 0.5 * (pos[:, None] == neg[None, :]).mean())
+
+
+def scores(args):
+    """Per block: the exhaustive search's peaks and the network's per-tile P(no peak) and top-k
+    directions -- what the operating points are chosen from (every decision of
+    gtomx_pick_subtomograms_learned is a function of these)."""
+    from . import proposal as pr
+    w = Workload()
+    net = pr.ProposalNet(max_tiles=512, params=np.load(args.weights)["params"])
+    out = []
+    for seed in range(args.first, args.first + args.count):
+        peaks, pnp, dirs = block_scores(w, net, seed, args.topk)
+        out.append({"seed": seed, "peaks": [list(p) for p in peaks], "p_no_peak": [round(float(v), 5) for v in pnp],
+                    "top_dirs": dirs.tolist()})
+        log(f"scored block {seed}: {len(peaks)} peaks")
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
 def evaluate(args):
     from . import proposal as pr
     w = Workload()
@@ -327,8 +395,13 @@ def main(argv=None):
     e.add_argument("--count", type=int, default=HELDOUT_SEEDS[1])
     e.add_argument("--topk", type=int, default=16)
     e.add_argument("--confidence", type=float, default=0.5)
+    c = sub.add_parser("scores")
+    c.add_argument("--weights", default=WEIGHTS)
+    c.add_argument("--first", type=int, default=HELDOUT_SEEDS[0])
+    c.add_argument("--count", type=int, default=HELDOUT_SEEDS[1])
+    c.add_argument("--topk", type=int, default=16)
     args = ap.parse_args(argv)
-    return {"label": label, "train": train, "eval": evaluate}[args.cmd](args)
+    return {"label": label, "train": train, "eval": evaluate, "scores": scores}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -17,7 +17,8 @@
  *    gtomx_last_error() gives a human-readable message for the calling thread.
  *  - Supported transform sizes: every axis 2^a in [8, 4096] or 3*2^a in
  *    [12, 3072] (an axis of extent 1 is allowed for 2D and 1D data). The
- *    search (gtomx_pick_*) takes volume sides up to 256 and image sides up
+ *    search (gtomx_pick_*) takes volume sides up to 256 (fused passes) and
+ *    512, 768 or 1024 (through the generic transforms), and image sides up
  *    to 4096 (above 256 through the generic transforms); the transforms and
  *    the pickers built on them take the full range, strided axes above 256 in
  *    two passes with stream-ordered scratch (hipMallocAsync).
