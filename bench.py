@@ -330,11 +330,15 @@ def row_pick2d(args):
     E = gx.d_FFTR2C(torch.from_numpy(imgs).to(dev), ndims=2)
     ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=5e-10, Cs=2.7e-3, voltage=300e3, defocus=-2e-6, amplitude=0.07)],
                     n, device=dev, ndims=2)
-    plan = gx.PickPlan(p, n, 0, ndims=2)
+    from gtom_amd import tune_micrographs as tm
+    batch, variant = tm.load_table().get(n, (0, gx.ROWS_PLAIN))  # the autotuner's choice for this side
+    plan = gx.PickPlan(p, n, batch, ndims=2)
+    plan.set_rows_variant(variant)
     best = torch.empty((nimg, n, n), device=dev)
     idx = torch.empty((nimg, n, n), dtype=torch.int32, device=dev)
     sec = _timed_steps(args, lambda: gx.d_PickSubTomograms(plan, projft, p / n, E, ctf, angles, n / 8, best=best,
                                                             bestidx=idx))
+    batch = plan.batch
     plan.close()
     work = nimg * n * n * len(angles)
 
@@ -349,7 +353,7 @@ def row_pick2d(args):
     return _row_line(args, "pick2d", "image pixel-orientations/s, d_PickSubTomograms 2D branch",
                      "pixel-orientations/s", work, sec,
                      {"workload": "d_PickSubTomograms (2D)", "images": nimg, "image": [n, n], "template": [box] * 3,
-                      "orientations": len(angles)}, cpu)
+                      "orientations": len(angles), "batch": batch, "rows_variant": variant}, cpu)
 
 
 def row_pick2d_micrograph(args):
@@ -370,11 +374,15 @@ def row_pick2d_micrograph(args):
     E = gx.d_FFTR2C(torch.from_numpy(imgs).to(dev), ndims=2)
     ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=1e-10, Cs=2.7e-3, voltage=300e3, defocus=-2e-6, amplitude=0.07)],
                     n, device=dev, ndims=2)
-    plan = gx.PickPlan(p, n, 0, ndims=2)
+    from gtom_amd import tune_micrographs as tm
+    batch, variant = tm.load_table().get(n, (0, gx.ROWS_PLAIN))  # the autotuner's choice for this side
+    plan = gx.PickPlan(p, n, batch, ndims=2)
+    plan.set_rows_variant(variant)
     best = torch.empty((nimg, n, n), device=dev)
     idx = torch.empty((nimg, n, n), dtype=torch.int32, device=dev)
     sec = _timed_steps(args, lambda: gx.d_PickSubTomograms(plan, projft, p / n, E, ctf, angles, n / 8, best=best,
                                                             bestidx=idx))
+    batch = plan.batch
     plan.close()
     work = nimg * n * n * len(angles)
 
@@ -389,7 +397,7 @@ def row_pick2d_micrograph(args):
     return _row_line(args, "pick2d_micrograph", "micrograph pixel-orientations/s, d_PickSubTomograms 2D branch",
                      "pixel-orientations/s", work, sec,
                      {"workload": "d_PickSubTomograms (2D)", "images": nimg, "image": [n, n], "template": [box] * 3,
-                      "orientations": len(angles)}, cpu)
+                      "orientations": len(angles), "batch": batch, "rows_variant": variant}, cpu)
 
 
 def micrograph_workload(seed, nmic=64, sides=(1024, 2048, 3072, 4096), box=64, copies=200, snr=0.02):

@@ -600,6 +600,7 @@ def d_LocalStd(vol, radius, with_mean=False, fouriermask=None, stream=None):
 
 ROWS_PLAIN = 0  # GTOMX_ROWS_PLAIN
 ROWS_PREFETCH = 1  # GTOMX_ROWS_PREFETCH
+ROWS_ONE = 2  # GTOMX_ROWS_ONE (images of side 2048, 3072 or 4096)
 
 
 class PickPlan:
@@ -632,7 +633,8 @@ class PickPlan:
         check(lib().gtomx_pick_plan_enable_timing(self.handle, int(bool(on))), "enable_timing")
 
     def set_rows_variant(self, variant):
-        """ROWS_PLAIN or ROWS_PREFETCH: the row-pass kernel of the 2D search on images above 256."""
+        """ROWS_PLAIN, ROWS_PREFETCH or ROWS_ONE (sides 2048, 3072, 4096): the row-pass kernel of the 2D search on images
+        above 256."""
         check(lib().gtomx_pick_plan_set_rows_variant(self.handle, int(variant)), "gtomx_pick_plan_set_rows_variant")
 
     def timings(self, reset=True):

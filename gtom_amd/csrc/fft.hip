@@ -461,6 +461,174 @@ __global__ __launch_bounds__(256) void k_pick_rows_max_long(const float2* __rest
   }
 }
 
+// The row pass with one row per workgroup of NC / 8 threads (NC = L1 * L2
+// complex points, L2 = 32; rows of 2048, 3072 and 4096 reals): the transform
+// and in-order update of k_pick_rows_max_long, with every LDS address a
+// per-thread base plus compile-time offsets (the row buffer keeps LongRow's
+// layout: element 32 a + b at 36 a + b), the twiddles of the Z pairs
+// (exp(-2 pi i m / 2 NC), m < NC) and of the two-step stages in LDS tables
+// staged once per workgroup, and pass A's output twiddles in registers.
+// Thread t: pass A line la = t / R2a (points la + 32 i) with index ja; pass B
+// line lb = t / R2b (points 32 lb + i) with index jb; the running maxima of
+// pairs t + T q, q < 8.
+template <int L1, int L2>
+__global__ __launch_bounds__(L1 * L2 / 8) void k_pick_rows_max_one(const float2* __restrict__ G, int nb, long long H,
+                                                                   int angle0, const float* __restrict__ scale,
+                                                                   float* __restrict__ best,
+                                                                   int32_t* __restrict__ bidx,
+                                                                   const float2* __restrict__ tw) {
+  using LR = LongRow<L1, L2>;
+  constexpr int NC = LR::NC, NH = LR::NH, N = LR::N, T = NC / 8, Q = 8;
+  using RA = ColShape<L1>;
+  using RB = ColShape<L2>;
+  constexpr int R1a = RA::R1, R2a = RA::R2, R1b = RB::R1, R2b = RB::R2;
+  constexpr int ITEMS_A = L2 * R2a, ITEMS_B = L1 * R2b;
+  static_assert(L2 == 32 && LR::phys(32 * 5 + 7) == 36 * 5 + 7, "row layout: element 32 a + b at 36 a + b");
+  static_assert(ITEMS_B == T && ITEMS_A <= T && ITEMS_A % 64 == 0, "pass B one line item per thread");
+  static_assert(T % L1 == 0 && T / L1 * 8 <= 32, "update addresses");
+  __shared__ float2 buf[LR::PITCH];
+  __shared__ float2 tz[NC];  // exp(-2 pi i m / N), m < NC
+  __shared__ float2 ta[L1];  // exp(-2 pi i m / L1)
+  __shared__ float2 tb[L2];  // exp(-2 pi i m / L2)
+  const int t = threadIdx.x;
+  const long long row = blockIdx.x;
+  for (int m = t; m < NC; m += T) tz[m] = twiddle<-1, N>(tw, m);
+  if (t < L1) ta[t] = twiddle<-1, L1>(tw, t);
+  if (t < L2) tb[t] = twiddle<-1, L2>(tw, t);
+  // pass A's output twiddles conj(W_NC^{la k1}), the same for every orientation
+  float2 twa[R1a];
+  {
+    const int la = t / R2a, ja = t % R2a;
+#pragma unroll
+    for (int sl = 0; sl < R1a; ++sl) twa[sl] = twiddle<1, N>(tw, 2 * la * out_index<R1a, R2a>(ja, sl));
+  }
+  float2* bp = reinterpret_cast<float2*>(best) + row * NC;  // values 2m, 2m+1 of the row at m
+  int2* ip = reinterpret_cast<int2*>(bidx) + row * NC;
+  float bv[2 * Q];
+  int32_t bi[2 * Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const float2 b = bp[t + T * q];
+    const int2 a = ip[t + T * q];
+    bv[2 * q] = b.x;
+    bv[2 * q + 1] = b.y;
+    bi[2 * q] = a.x;
+    bi[2 * q + 1] = a.y;
+  }
+  for (int o = 0; o < nb; ++o) {
+    const float sc = scale[o];
+    const float2* src = G + (size_t)o * H + row * NH;
+    // bases from an opaque copy of the thread index in each iteration (not
+    // hoisted into registers across the loop); the modulo keeps its range known
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    tt %= T;
+    __syncthreads();  // the previous orientation's reads of buf are done (and the tables are written)
+    {
+      // Z[k] and Z[NC - k] from X[k], X[NC - k], k = tt + T q (X[0], X[NC] real); thread 0 also k = NC / 2
+      const int pz = tt + 4 * (tt >> 5);                // phys(tt)
+      const int pm = (NC - tt) + 4 * ((NC - tt) >> 5);  // phys(NC - tt)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = tt + T * q;
+        float2 a = src[k], b = src[NC - k];
+        if (q == 0 && tt == 0) {
+          a.y = 0.f;
+          b.y = 0.f;
+        }
+        const float2 ac = cconj(a), bc = cconj(b);
+        buf[pz + (T + T / 8) * q] = cadd(cadd(a, bc), mul_pj<1>(cmul(cconj(tz[k]), csub(a, bc))));
+        if (q > 0 || tt > 0)
+          buf[pm - (T + T / 8) * q] = cadd(cadd(b, ac), mul_pj<1>(cmul(cconj(tz[NC - k]), csub(b, ac))));
+      }
+      if (tt == 0) {
+        const float2 a = src[NC / 2], ac = cconj(a);
+        buf[LR::phys(NC / 2)] = cadd(cadd(a, ac), mul_pj<1>(cmul(cconj(tz[NC / 2]), csub(a, ac))));
+      }
+    }
+    __syncthreads();
+    if (tt < ITEMS_A) {  // whole waves
+      // pass A: line la of L1 points as R1a x R2a, then times conj(W_NC^{la k1})
+      const int la = tt / R2a, ja = tt % R2a;
+      const int bA = 36 * ja + la, bA2 = 36 * R1a * ja + la;
+      float2 v[R1a];
+#pragma unroll
+      for (int m = 0; m < R1a; ++m) v[m] = buf[bA + 36 * R2a * m];
+      dft<R1a, 1>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < R1a; ++k1) v[k1] = cmul(v[k1], cconj(ta[ja * k1]));
+#pragma unroll
+      for (int k1 = 0; k1 < R1a; ++k1) buf[bA + 36 * R2a * k1] = v[k1];
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < R1a / R2a; ++u) {
+        float2 z[R2a];
+#pragma unroll
+        for (int jj = 0; jj < R2a; ++jj) z[jj] = buf[bA2 + 36 * R2a * u + 36 * jj];
+        dft<R2a, 1>(z);
+#pragma unroll
+        for (int k2 = 0; k2 < R2a; ++k2) v[u * R2a + k2] = z[k2];
+      }
+      __syncthreads();  // the exchange reads are done before the line's entries change
+#pragma unroll
+      for (int sl = 0; sl < R1a; ++sl) buf[36 * out_index<R1a, R2a>(ja, sl) + la] = cmul(v[sl], twa[sl]);
+    } else {
+      __syncthreads();  // the two barriers of the waves with a line
+      __syncthreads();
+    }
+    __syncthreads();
+    {
+      // pass B: line lb (32 contiguous points) as R1b x R2b
+      const int lb = tt / R2b, jb = tt % R2b;
+      const int bB = 36 * lb + jb, bB2 = 36 * lb + R1b * jb, bB3 = 36 * lb + (R1b / R2b) * jb;
+      float2 v[R1b];
+#pragma unroll
+      for (int m = 0; m < R1b; ++m) v[m] = buf[bB + R2b * m];
+      dft<R1b, 1>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < R1b; ++k1) v[k1] = cmul(v[k1], cconj(tb[jb * k1]));
+#pragma unroll
+      for (int k1 = 0; k1 < R1b; ++k1) buf[bB + R2b * k1] = v[k1];
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < R1b / R2b; ++u) {
+        float2 z[R2b];
+#pragma unroll
+        for (int jj = 0; jj < R2b; ++jj) z[jj] = buf[bB2 + R2b * u + jj];
+        dft<R2b, 1>(z);
+#pragma unroll
+        for (int k2 = 0; k2 < R2b; ++k2) v[u * R2b + k2] = z[k2];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < R1b / R2b; ++u)
+#pragma unroll
+        for (int k2 = 0; k2 < R2b; ++k2) buf[bB3 + u + R1b * k2] = v[u * R2b + k2];
+    }
+    __syncthreads();
+    const int32_t oi = angle0 + o;
+    const int bM = 36 * (tt % L1) + tt / L1;  // Z[tt + T q] at bM + (T / L1) q
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const float2 z = buf[bM + (T / L1) * q];
+      const float c0 = z.x * sc, c1 = z.y * sc;
+      if (c0 > bv[2 * q]) {
+        bv[2 * q] = c0;
+        bi[2 * q] = oi;
+      }
+      if (c1 > bv[2 * q + 1]) {
+        bv[2 * q + 1] = c1;
+        bi[2 * q + 1] = oi;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    bp[t + T * q] = make_float2(bv[2 * q], bv[2 * q + 1]);
+    ip[t + T * q] = make_int2(bi[2 * q], bi[2 * q + 1]);
+  }
+}
+
 // Zeroes the rows of half spectra outside a RowBand (rows of NH bins, nrows
 // in all: every plane of every volume).
 __global__ __launch_bounds__(256) void k_zero_outside_band(float2* __restrict__ data, long long nrows, int NH,
@@ -723,6 +891,19 @@ int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, i
     GX_LAUNCHED("k_pick_rows_max_long");
     return GTOMX_OK;
   };
+  if (variant == GTOMX_ROWS_ONE) {  // one row per workgroup of NC / 8 threads, LDS twiddles
+    auto one = [&](auto kernel, int threads) -> int {
+      hipLaunchKernelGGL(kernel, dim3((unsigned)nrows), dim3(threads), 0, s, G, nb, H, angle0, scale, best, idx, tw);
+      GX_LAUNCHED("k_pick_rows_max_one");
+      return GTOMX_OK;
+    };
+    switch (n) {
+      case 2048: return one(k_pick_rows_max_one<32, 32>, 128);
+      case 3072: return one(k_pick_rows_max_one<48, 32>, 192);
+      case 4096: return one(k_pick_rows_max_one<64, 32>, 256);
+      default: return fail(GTOMX_ERR_UNSUPPORTED, "the one-row pass takes rows of 2048, 3072 or 4096 reals");
+    }
+  }
   switch (n / 2) {
 #define GX_LONG(NC, A, B, W) \
   case NC:                   \

@@ -1462,8 +1462,11 @@ int gtomx_pick_plan_destroy(gtomx_pick_plan* plan) {
 
 int gtomx_pick_plan_set_rows_variant(gtomx_pick_plan* plan, int variant) {
   if (!plan) return fail(GTOMX_ERR_INVALID_ARGUMENT, "plan is NULL");
-  if (variant != GTOMX_ROWS_PLAIN && variant != GTOMX_ROWS_PREFETCH)
-    return fail(GTOMX_ERR_INVALID_ARGUMENT, "rows variant must be GTOMX_ROWS_PLAIN or GTOMX_ROWS_PREFETCH");
+  if (variant != GTOMX_ROWS_PLAIN && variant != GTOMX_ROWS_PREFETCH && variant != GTOMX_ROWS_ONE)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT,
+                "rows variant must be GTOMX_ROWS_PLAIN, GTOMX_ROWS_PREFETCH or GTOMX_ROWS_ONE");
+  if (variant == GTOMX_ROWS_ONE && !(plan->nz == 1 && (plan->n == 2048 || plan->n == 3072 || plan->n == 4096)))
+    return fail(GTOMX_ERR_UNSUPPORTED, "GTOMX_ROWS_ONE takes images of side 2048, 3072 or 4096");
   plan->rows_variant = variant;
   return GTOMX_OK;
 }

@@ -1,7 +1,7 @@
 """Autotuner of the micrograph batch (gtomx_mpick): for each image side it times
 the 2D search's orientation loop (gtomx_mpick_tune: one warm-up, then timed
 runs with HIP events) at every candidate batch that fits, with each row-pass
-kernel (GTOMX_ROWS_PLAIN / GTOMX_ROWS_PREFETCH), and writes the fastest
+kernel (GTOMX_ROWS_PLAIN / GTOMX_ROWS_PREFETCH / GTOMX_ROWS_ONE at 2048, 3072, 4096), and writes the fastest
 (batch, variant) per side to gtom_amd/tuning/micrograph_batches.json, which
 MicrographBatch users (bench.py's micrograph row) read.
 
@@ -45,6 +45,8 @@ def tune(sides, box=64, orientations=64, repeats=2, log=print):
     for n in sides:
         batches = CANDIDATES.get(n, [8, 16, 32, 64])
         variants = [gx.ROWS_PLAIN, gx.ROWS_PREFETCH] if n > 256 else [gx.ROWS_PLAIN]
+        if n in (2048, 3072, 4096):
+            variants.append(gx.ROWS_ONE)
         cands = [(b, v) for v in variants for b in batches]
         t0 = time.time()
         ms = gx.mpick_tune(p, n, projft, [b for b, _ in cands], angles, repeats=repeats,

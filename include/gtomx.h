@@ -629,13 +629,17 @@ int gtomx_pick_plan_destroy(gtomx_pick_plan* plan);
 int gtomx_pick_plan_info(const gtomx_pick_plan* plan, unsigned* batch, size_t* device_bytes);
 
 /* Row-pass kernel of the 2D search on images above 256 (no reference
- * counterpart; the results are identical, only the schedule differs):
+ * counterpart; the same operations in another schedule, so the results agree
+ * up to the rounding of contracted multiply-adds):
  * GTOMX_ROWS_PLAIN loads each orientation's spectrum rows when it starts,
  * GTOMX_ROWS_PREFETCH loads the next orientation's rows into registers while
  * the current one is transformed (more registers per thread, fewer resident
- * waves). The micrograph autotuner picks one per image side. */
+ * waves), GTOMX_ROWS_ONE (images of side 2048, 3072 or 4096) runs one row per
+ * workgroup with every twiddle from LDS tables staged once per workgroup.
+ * The micrograph autotuner picks one per image side. */
 #define GTOMX_ROWS_PLAIN 0
 #define GTOMX_ROWS_PREFETCH 1
+#define GTOMX_ROWS_ONE 2
 int gtomx_pick_plan_set_rows_variant(gtomx_pick_plan* plan, int variant);
 
 /* Kernel timing: when enabled, every launch of the search kernels is bracketed

@@ -135,5 +135,7 @@ def test_rows_variant_rejects_unknown_value(gx, cuda):
     plan = gx.PickPlan(32, 512, 8, ndims=2)
     plan.set_rows_variant(gx.ROWS_PREFETCH)
     with pytest.raises(RuntimeError, match="rows variant"):
-        plan.set_rows_variant(2)
+        plan.set_rows_variant(3)
+    with pytest.raises(RuntimeError, match="4096"):
+        plan.set_rows_variant(gx.ROWS_ONE)  # the one-row pass takes 2048^2, 3072^2 and 4096^2 images only
     plan.close()

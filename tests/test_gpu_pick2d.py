@@ -43,7 +43,8 @@ def _images(orc, gx, cuda, n, box, nimg, angles, seed):
     return projft, P, np.stack(imgs).astype(np.float32)
 
 
-def _run(gx, orc, cuda, n, box, nimg, nang, batch, seed, ctf_per_image=False, oversample=None):
+def _run(gx, orc, cuda, n, box, nimg, nang, batch, seed, ctf_per_image=False, oversample=None, rows_variant=0,
+         oracle=True):
     import torch
     angles = orc.equal_angular_spacing((0, TAU), (0, np.pi), (0, TAU), np.deg2rad(30.0))[:nang]
     projft, P, imgs = _images(orc, gx, cuda, n, box, nimg, angles, seed)
@@ -54,10 +55,14 @@ def _run(gx, orc, cuda, n, box, nimg, nang, batch, seed, ctf_per_image=False, ov
     ctf = gx.ctf_3d(params, n, device=cuda, ndims=2)
     stride = ctf[0].numel() if ctf_per_image else 0
     plan = gx.PickPlan(p, n, batch, ndims=2)
+    if rows_variant:
+        plan.set_rows_variant(rows_variant)
     ov = p / n if oversample is None else oversample
     best, idx = gx.d_PickSubTomograms(plan, projft, ov, E, ctf, angles, n / 8, ctf_volume_stride=stride)
     torch.cuda.synchronize()
     plan.close()
+    if not oracle:
+        return to_np(best), to_np(idx), None
     C = to_np(ctf).astype(np.float64)
     ref = orc.pick(P, p, ov, to_np(E).astype(np.complex128), C if ctf_per_image else C[0], n,
                    angles.astype(np.float64), n / 8, ctf_per_volume=ctf_per_image, ndims=2)
@@ -96,6 +101,21 @@ def test_pick2d_micrograph_sizes_match_oracle(gx, orc, cuda, n, nang, ctf_per_im
     best, idx, ref = _run(gx, orc, cuda, n, 32, nimg=2 if ctf_per_image else 1, nang=nang, batch=0, seed=n,
                           ctf_per_image=ctf_per_image)
     _check(best, idx, ref)
+
+
+@pytest.mark.parametrize("n", [2048, 3072, 4096])
+def test_pick2d_one_row_pass_matches_oracle(gx, orc, cuda, n):
+    """The one-row-per-workgroup row pass with LDS twiddles (GTOMX_ROWS_ONE) against the oracle,
+    and against the plain row pass (the same operations, compiled into a different kernel: equal
+    up to the rounding of contracted multiply-adds), over two batches (4 + 2)."""
+    best, idx, ref = _run(gx, orc, cuda, n, 32, nimg=1, nang=6, batch=4, seed=11, rows_variant=gx.ROWS_ONE)
+    _check(best, idx, ref)
+    b0, i0, _ = _run(gx, orc, cuda, n, 32, nimg=1, nang=6, batch=4, seed=11, oracle=False)
+    err = rel_linf(best, b0)
+    agree, cnt = argmax_agreement(idx, i0, ref["top1"], ref["top2"], 1e-5)
+    print(f"{n}^2 one-row vs plain row pass: max difference {err:.2e}, argmax agreement {agree} on {cnt} pixels")
+    assert err <= 1e-6
+    assert cnt > 0 and agree == 1.0
 
 
 def test_pick2d_micrograph_uncentred_projector_matches_oracle(gx, orc, cuda):
