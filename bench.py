@@ -51,7 +51,8 @@ def parse():
     ap.add_argument("--learned-blocks", type=int, default=4,
                     help="held-out blocks timed by the learned comparison")
     ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
-                                      "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split", "pnet"],
+                                      "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split", "pnet",
+                                      "pick_large"],
                     default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
@@ -311,6 +312,42 @@ def row_picker(args):
     return _row_line(args, "picker", "image voxel-orientations/s, Picker 3D", "voxel-orientations/s",
                      vox * len(angles), sec, {"workload": "Picker.PerformCorrelation", "image": list(di),
                                               "ref": [nr] * 3, "anglestep": step, "orientations": len(angles)}, cpu)
+
+
+def row_pick_large(args):
+    """d_PickSubTomograms on a 512^3 volume (the generic-transform form above 256): one volume of
+    noise with 8 planted copies of a 48^3 blob template (projector 96^3, oversample 96/512), a CTF
+    (defocus 3 um, 10 A/px), the first 1,152 orientations of the 500-direction x 18 set (batch 15),
+    local-std radius 24."""
+    import torch
+    import gtom_amd as gx
+    from gtom_amd import synth
+    rng = np.random.default_rng(args.seed)
+    n, box = 512, 48
+    p = 2 * box
+    dev = torch.device("cuda:0")
+    template = synth.blob_template(box, rng, radius=14.0)
+    angles = gx.fibonacci_angles(500, 18)[:1152]
+    pos = [tuple(int(v) for v in rng.integers(box, n - box, 3)) for _ in range(8)]
+    ori = [tuple(float(a) for a in angles[int(k)]) for k in rng.integers(0, len(angles), 8)]
+    tomo = (synth.plant((n, n, n), template, pos, ori, 0.3) + rng.standard_normal((n, n, n), dtype=np.float32))
+    E = gx.d_FFTR2C(torch.from_numpy(tomo.astype(np.float32)[None]).to(dev))
+    del tomo
+    ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=10e-10, Cs=2.7e-3, voltage=300e3, defocus=-3e-6, amplitude=0.07)],
+                    n, device=dev)
+    projft = gx.prepare_projector(torch.from_numpy(template).to(dev), p)
+    plan = gx.PickPlan(p, n)
+    best = torch.empty((1, n, n, n), device=dev)
+    idx = torch.empty((1, n, n, n), dtype=torch.int32, device=dev)
+    sec = _timed_steps(args, lambda: gx.d_PickSubTomograms(plan, projft, p / n, E, ctf, angles, 24.0, best=best,
+                                                            bestidx=idx))
+    batch = plan.batch
+    plan.close()
+    work = n ** 3 * len(angles)
+    return _row_line(args, "pick_large", "voxel-orientations/s, d_PickSubTomograms on a 512^3 volume",
+                     "voxel-orientations/s", work, sec,
+                     {"workload": "one 512^3 volume, 48^3 template, 1152 orientations", "volume": [n] * 3,
+                      "orientations": len(angles), "batch": batch}, None)
 
 
 def row_pick2d(args):
@@ -839,7 +876,7 @@ def main():
                 "local_peaks": row_local_peaks, "fft_micrograph": row_fft_micrograph,
                 "prob_picker_micrograph": row_prob_picker_micrograph,
                 "pick2d_micrograph": row_pick2d_micrograph, "micrographs": row_micrographs,
-                "split": row_split, "pnet": row_pnet}[args.row](args)
+                "split": row_split, "pnet": row_pnet, "pick_large": row_pick_large}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth

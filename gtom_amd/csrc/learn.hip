@@ -777,7 +777,7 @@ int gtomx_pnet_create(gtomx_pnet** out, unsigned max_tiles) {
   // zero parameters and optimiser state until gtomx_pnet_set_params
   for (float* q : {nt->p, nt->g, nt->m, nt->v}) he = he == hipSuccess ? hipMemset(q, 0, NPARAM * sizeof(float)) : he;
   if (he == hipSuccess) he = hipMemset(nt->logits, 0, R * CPAD * 4);
-  if (he == hipSuccess) he = hipDeviceSynchronize();
+  if (he == hipSuccess) he = hipStreamSynchronize(nullptr);  // the clears ran on the null stream
   if (he != hipSuccess) {
     pnet_release(nt);
     return hip_fail(he, "network set-up");

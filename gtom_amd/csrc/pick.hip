@@ -732,7 +732,11 @@ __global__ __launch_bounds__(TX * LineShape<L1>::R2) void k_pick2d_project_cols(
 #pragma unroll
     for (int s = 0; s < R1; ++s) {
       const int k1 = out_index<R1, R2>(j, s);
-      So[(size_t)(n2 + L2 * k1) * NH + ix] = cmul(v[s], twiddle_rt<1>(tw, n, n2 * k1));
+      // streaming stores: the spectra are read once, by the second pass
+      const float2 w = cmul(v[s], twiddle_rt<1>(tw, n, n2 * k1));
+      float* dst = reinterpret_cast<float*>(So + (size_t)(n2 + L2 * k1) * NH + ix);
+      __builtin_nontemporal_store(w.x, dst);
+      __builtin_nontemporal_store(w.y, dst + 1);
     }
   }
   const float tot = block_sum(acc, red);

@@ -94,8 +94,8 @@ def test_mpick_equals_each_image_alone(gx, cuda, use_graphs):
         assert info["captures"] == 0 and info["replays"] == 0, info
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-def test_mpick_uneven_set_matches_oracle(gx, orc, cuda, variant):
+@pytest.mark.parametrize("variants", [(0, 0), (1, 1), (0, 2)])
+def test_mpick_uneven_set_matches_oracle(gx, orc, cuda, variants):
     import torch
     box = 32
     p = 2 * box
@@ -103,7 +103,7 @@ def test_mpick_uneven_set_matches_oracle(gx, orc, cuda, variant):
     sides = [2048, 1024]
     images = _images(gx, cuda, sides, seed=8)
     angles = gx.fibonacci_angles(8, 5)
-    mb = gx.MicrographBatch(p, [1024, 2048], variants=[variant, variant])
+    mb = gx.MicrographBatch(p, [1024, 2048], variants=list(variants))
     mb.run(projft, images, angles)
     torch.cuda.synchronize()
     mb.close()
@@ -115,7 +115,7 @@ def test_mpick_uneven_set_matches_oracle(gx, orc, cuda, variant):
         best, idx = to_np(im["best"]), to_np(im["index"])
         err = rel_linf(best, ref["best"][0])
         agree, cnt = argmax_agreement(idx, ref["idx"][0], ref["top1"][0], ref["top2"][0], CC_TOL)
-        print(f"mpick {n}^2 rows variant {variant}: max-CC error {err:.2e}, argmax agreement {agree} on {cnt} pixels")
+        print(f"mpick {n}^2 rows variants {variants}: max-CC error {err:.2e}, argmax agreement {agree} on {cnt} pixels")
         assert err <= CC_TOL, (n, err)
         assert cnt > 0 and agree == 1.0, (n, agree, cnt)
 
