@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--search", choices=["plain", "learned"], default="plain",
                     help="learned: the learned-proposal search against the exhaustive one on held-out blocks of the "
                          "learned configuration (one JSON line of its own)")
-    ap.add_argument("--learned-blocks", type=int, default=4,
+    ap.add_argument("--learned-blocks", type=int, default=10,
                     help="held-out blocks timed by the learned comparison")
     ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
                                       "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split", "pnet",

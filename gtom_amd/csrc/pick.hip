@@ -795,7 +795,12 @@ __global__ __launch_bounds__(256) void k_pick3d_project_any(PJ P, const float* _
                                      fmaf(M[8], fz[k], fmaf(M[7], fy[k], M[6] * fx[k])), ins[k]),
                               cf[k]);
       acc += r.x * r.x + r.y * r.y;
-      if (live[k]) Go[base + 256LL * k] = cmulc(e[k], r);
+      if (live[k]) {  // streaming stores: each spectrum is read once, by the column inverse
+        const float2 q = cmulc(e[k], r);
+        float* dst = reinterpret_cast<float*>(Go + base + 256LL * k);
+        __builtin_nontemporal_store(q.x, dst);
+        __builtin_nontemporal_store(q.y, dst + 1);
+      }
     }
     const float tot = block_sum(acc, red);
     if (t == 0) partial[(size_t)o * gridDim.x + blockIdx.x] = tot;
