@@ -112,6 +112,35 @@ template <int DIR> __device__ __forceinline__ void dft16(float2* a) {
     a[k + 8] = csub(e[k], o[k]);
   }
 }
+// DFT-32 as two DFT-16 of the even and odd points and one radix-2 stage with W32^k.
+template <int DIR> __device__ __forceinline__ void dft32(float2* a) {
+  float2 e[16], o[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    e[i] = a[2 * i];
+    o[i] = a[2 * i + 1];
+  }
+  dft16<DIR>(e);
+  dft16<DIR>(o);
+  // cos(pi k / 16), sin(pi k / 16), k = 0..15
+  constexpr float C[16] = {1.f, 0.98078528040323044913f, 0.92387953251128675613f, 0.83146961230254523708f,
+                           0.70710678118654752440f, 0.55557023301960222474f, 0.38268343236508977173f,
+                           0.19509032201612826785f, 0.f, -0.19509032201612826785f, -0.38268343236508977173f,
+                           -0.55557023301960222474f, -0.70710678118654752440f, -0.83146961230254523708f,
+                           -0.92387953251128675613f, -0.98078528040323044913f};
+  constexpr float S[16] = {0.f, 0.19509032201612826785f, 0.38268343236508977173f, 0.55557023301960222474f,
+                           0.70710678118654752440f, 0.83146961230254523708f, 0.92387953251128675613f,
+                           0.98078528040323044913f, 1.f, 0.98078528040323044913f, 0.92387953251128675613f,
+                           0.83146961230254523708f, 0.70710678118654752440f, 0.55557023301960222474f,
+                           0.38268343236508977173f, 0.19509032201612826785f};
+#pragma unroll
+  for (int k = 1; k < 16; ++k) o[k] = k == 8 ? mul_pj<DIR>(o[k]) : cmul(o[k], make_float2(C[k], DIR * S[k]));
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    a[k] = cadd(e[k], o[k]);
+    a[k + 16] = csub(e[k], o[k]);
+  }
+}
 template <int DIR> __device__ __forceinline__ void dft3(float2& a0, float2& a1, float2& a2) {
   constexpr float s3 = 0.86602540378443864676f;  // sin(2 pi / 3)
   const float2 t = cadd(a1, a2);
@@ -166,6 +195,8 @@ template <int R, int DIR> __device__ __forceinline__ void dft(float2* a) {
     dft16<DIR>(a);
   } else if constexpr (R == 24) {
     dft_mixed<8, 3, DIR>(a);
+  } else if constexpr (R == 32) {
+    dft32<DIR>(a);
   } else {
     static_assert(R == 16, "unsupported radix");
   }
@@ -186,6 +217,8 @@ template <> struct ColShape<96>  { static constexpr int R1 = 24, R2 = 4,  TX = 1
 template <> struct ColShape<128> { static constexpr int R1 = 16, R2 = 8,  TX = 16; };
 template <> struct ColShape<192> { static constexpr int R1 = 24, R2 = 8,  TX = 16; };
 template <> struct ColShape<256> { static constexpr int R1 = 16, R2 = 16, TX = 16; };
+// one-pass columns of the generic transforms at 512 (64 KB of exchange)
+template <> struct ColShape<512> { static constexpr int R1 = 32, R2 = 16, TX = 16; };
 
 // Row transforms (contiguous axis, complex length NC): R2 threads per row,
 // R1 points per thread.

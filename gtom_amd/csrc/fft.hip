@@ -27,7 +27,7 @@ namespace gx {
 template <int N> constexpr int fft_col_tx() { return (N == 128 || N == 256) ? 32 : ColShape<N>::TX; }
 
 template <int N, int DIR>
-__global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
+__global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2, (N == 512 ? 2 : 1)) void k_cols(
     float2* __restrict__ data, int width, long long axis_stride, long long nplanes, long long P1,
     long long S0, long long S1, const float2* __restrict__ tw) {
   constexpr int R1 = ColShape<N>::R1, R2 = ColShape<N>::R2, TX = fft_col_tx<N>();
@@ -35,6 +35,11 @@ __global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
   const int t = threadIdx.x, c = t % TX, j = t / TX;
   const int col = blockIdx.x * TX + c;
   const bool valid = col < width;
+  // at 512 the stage twiddles come from an LDS copy (31 per thread would otherwise be held in registers)
+  constexpr bool LDS_TW = N == 512;
+  __shared__ float2 twl[LDS_TW ? N : 1];
+  if constexpr (LDS_TW)
+    for (int i = t; i < N; i += TX * R2) twl[i] = twiddle<-1, N>(tw, i);
   for (long long pl = blockIdx.y; pl < nplanes; pl += gridDim.y) {
     const long long base = (pl / P1) * S1 + (pl % P1) * S0 + col;
     float2 v[R1];
@@ -42,7 +47,11 @@ __global__ __launch_bounds__(fft_col_tx<N>() * ColShape<N>::R2) void k_cols(
     for (int m = 0; m < R1; ++m)
       v[m] = valid ? data[base + (long long)(j + R2 * m) * axis_stride] : make_float2(0.f, 0.f);
     __syncthreads();
-    fft_two_step<N, R1, R2, DIR>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
+    if constexpr (LDS_TW)
+      fft_two_step<N, R1, R2, DIR>(v, j, twl, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; },
+                                   TwiddleLDS{});
+    else
+      fft_two_step<N, R1, R2, DIR>(v, j, tw, [&](int k1, int jj) { return &xch[(k1 * R2 + jj) * TX + c]; });
     if (valid) {
 #pragma unroll
       for (int s = 0; s < R1; ++s) data[base + (long long)out_index<R1, R2>(j, s) * axis_stride] = v[s];
@@ -660,6 +669,7 @@ int launch_cols_n(int n, float2* data, int width, long long axis_stride, long lo
   case N:          \
     return go(k_cols<N, DIR>, fft_col_tx<N>(), ColShape<N>::R2);
     GX_FOR_SIZES(GX_CASE)
+    GX_CASE(512)
 #undef GX_CASE
     default: return fail(GTOMX_ERR_UNSUPPORTED, "column FFT length not supported");
   }
@@ -756,7 +766,10 @@ int launch_lines(int n, const float2* src, float2* dst, const LineMap& p, const 
 template <int DIR>
 int cols_axis(int L, float2* data, float2* scratch, int width, long long as, long long nplanes, long long P1,
               long long S0, long long S1, const float2* tw, hipStream_t s, int band = 0) {
-  if (L <= 256) return launch_cols_n<DIR>(L, data, width, as, nplanes, P1, S0, S1, tw, s);
+  // one pass up to 256 and at 512 (the exchange of 16 columns fits LDS), two otherwise or
+  // with a band
+  if (L <= 256 || (L == 512 && band == 0))
+    return launch_cols_n<DIR>(L, data, width, as, nplanes, P1, S0, S1, tw, s);
   int L1, L2;
   if (!split_len(L, &L1, &L2) || !scratch) return fail(GTOMX_ERR_INTERNAL, "axis length not split");
   if ((long long)L * as >= (1LL << 31)) return fail(GTOMX_ERR_UNSUPPORTED, "axis span of 2^31 elements or more");

@@ -24,7 +24,7 @@ def test_full_size_block_matches_oracle(gx, orc, cuda):
     projft = gx.prepare_projector(torch.from_numpy(w["template"]).to(cuda), p)
     exp_ft = gx.d_FFTR2C(torch.from_numpy(w["blocks"]).to(cuda))
     ctf = torch.from_numpy(ctf_np).to(cuda)
-    plan = gx.PickPlan(p, n)  # the benchmark's batch of 32
+    plan = gx.PickPlan(p, n, 32)  # a full batch of 32 and a partial one of 8
     best, idx = gx.d_PickSubTomograms(plan, projft, s, exp_ft, ctf, angles, w["maskradius"])
     torch.cuda.synchronize()
     plan.close()
@@ -50,7 +50,7 @@ def test_full_size_planted_copy_found(gx, cuda):
     projft = gx.prepare_projector(torch.from_numpy(template).to(cuda), p)
     exp_ft = gx.d_FFTR2C(torch.from_numpy(tomo[None]).to(cuda))
     ctf = torch.ones((n, n, n // 2 + 1), device=cuda)
-    plan = gx.PickPlan(p, n)
+    plan = gx.PickPlan(p, n, 32)
     best, idx = gx.d_PickSubTomograms(plan, projft, p / n, exp_ft, ctf, angles, 24.0)
     torch.cuda.synchronize()
     plan.close()
@@ -60,7 +60,7 @@ def test_full_size_planted_copy_found(gx, cuda):
     assert to_np(idx)[0, z, y, x] == target
 
 
-def test_benchmark_call_shape_matches_oracle(gx, orc, cuda):
+def test_tail_batches_with_per_volume_ctfs_match_oracle(gx, orc, cuda):
     import torch
     from gtom_amd import synth
     w = synth.cfg2(seed=7)
@@ -74,7 +74,7 @@ def test_benchmark_call_shape_matches_oracle(gx, orc, cuda):
     projft = gx.prepare_projector(torch.from_numpy(w["template"]).to(cuda), p)
     exp_ft = gx.d_FFTR2C(torch.from_numpy(w["blocks"]).to(cuda))
     ctf = torch.from_numpy(ctf_np).to(cuda)
-    plan = gx.PickPlan(p, n)
+    plan = gx.PickPlan(p, n, 32)  # a full batch 8960..8991 and a partial one 8992..8999
     assert plan.batch == 32 and (len(angles) % plan.batch) == 8
     best = torch.empty((2, n, n, n), dtype=torch.float32, device=cuda)
     idx = torch.empty((2, n, n, n), dtype=torch.int32, device=cuda)
@@ -95,9 +95,9 @@ def test_benchmark_call_shape_matches_oracle(gx, orc, cuda):
 
 def test_benchmark_call_shape_matches_oracle(gx, orc, cuda):
     """The benchmark's own call shape: the two 256^3 blocks of the workload with one CTF each
-    (gtomx_ctf_3d, ctf_volume_stride = H), batch 32, and the last 40 of the 9000 orientations at
-    their own indices: the full batch 8960..8991 and the final partial batch 8992..8999, as the
-    whole-list call splits them (8960 = 280 x 32). The oracle gets the same CTF array per volume."""
+    (gtomx_ctf_3d, ctf_volume_stride = H), the default batch of 128, and the last 40 of the 9000
+    orientations at their own indices: the final partial batch 8960..8999, as the whole-list call
+    splits them (8960 = 70 x 128). The oracle gets the same CTF array per volume."""
     import torch
     from gtom_amd import synth
     w = synth.cfg2(seed=9)
@@ -111,7 +111,7 @@ def test_benchmark_call_shape_matches_oracle(gx, orc, cuda):
     ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=w["pixelsize"], Cs=2.7e-3, voltage=300e3, defocus=-float(d),
                                        amplitude=0.07) for d in w["defoci"]], n, device=cuda)
     plan = gx.PickPlan(p, n)
-    assert plan.batch == 32
+    assert plan.batch == 128 and 9000 % plan.batch == len(angles)
     nvol = w["blocks"].shape[0]
     best = torch.empty((nvol, n, n, n), dtype=torch.float32, device=cuda)
     idx = torch.empty((nvol, n, n, n), dtype=torch.int32, device=cuda)
