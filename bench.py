@@ -52,7 +52,7 @@ def parse():
                     help="held-out blocks timed by the learned comparison")
     ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
                                       "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split", "pnet",
-                                      "pick_large"],
+                                      "pick_large", "tomo_picker"],
                     default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
@@ -285,7 +285,6 @@ def row_picker(args):
     (648 orientations). One step is PerformCorrelation over all of them."""
     import torch
     import gtom_amd as gx
-    from oracle import pyoracle as po
     rng = np.random.default_rng(args.seed)
     nr, di, step = 32, (128, 128, 128), 0.5
     ref = _blob(nr, rng)
@@ -298,12 +297,13 @@ def row_picker(args):
     pk.set_image(dimg, dctf)
     best = torch.full(di, -1e30, device=dev)
     ang = torch.zeros(di + (3,), device=dev)
-    angles = po.equal_angular_spacing((0, 2 * np.pi), (0, np.pi), (0, 2 * np.pi), step)
+    angles = gx.equal_angular_spacing((0, 2 * np.pi), (0, np.pi), (0, 2 * np.pi), step)
     sec = _timed_steps(args, lambda: pk.perform_correlation(best, ang, anglestep=step))
     pk.close()
     vox = int(np.prod(di))
 
     def cpu():
+        from oracle import pyoracle as po
         k = 40
         t0 = time.perf_counter()
         po.picker(ref.astype(np.float64), image.astype(np.float64), ctf.astype(np.float64), angles[:k])
@@ -312,6 +312,61 @@ def row_picker(args):
     return _row_line(args, "picker", "image voxel-orientations/s, Picker 3D", "voxel-orientations/s",
                      vox * len(angles), sec, {"workload": "Picker.PerformCorrelation", "image": list(di),
                                               "ref": [nr] * 3, "anglestep": step, "orientations": len(angles)}, cpu)
+
+
+def row_tomo_picker(args):
+    """The TomoPicker class: 41 tilt images of 512^2 (tilt -60 ... 60 degrees, weights 0.5 ... 1), a
+    32^3 blob reference with a spherical mask of radius 14 (circular), a 512 x 512 x 128 correlation
+    volume, the 588 orientations of anglestep 30 degrees. One step is PerformCorrelation over all of
+    them, writing the maximum, its angle and index and the two sums."""
+    import torch
+    import gtom_amd as gx
+    rng = np.random.default_rng(args.seed)
+    nr, nx, ny, nimg, dc = 32, 512, 512, 41, (512, 512, 128)
+    g = np.arange(nr) - nr // 2
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    ref = _blob(nr, rng)
+    mask = (x * x + y * y + z * z <= 14 * 14).astype(np.float32)
+    images = rng.standard_normal((nimg, ny, nx)).astype(np.float32)
+    tilts = np.array([[0.0, np.deg2rad(t), 0.0] for t in np.linspace(-60.0, 60.0, nimg)], np.float32)
+    w = rng.uniform(0.5, 1.0, nimg).astype(np.float32)
+    angles = gx.equal_angular_spacing((0, 2 * np.pi), (0, np.pi), (0, 2 * np.pi), np.deg2rad(30.0))
+    dev = torch.device("cuda:0")
+    tp = gx.TomoPicker(torch.from_numpy(ref).to(dev), torch.from_numpy(mask).to(dev), (ny, nx), nimg)
+    tp.set_image(torch.from_numpy(images).to(dev), tilts, w)
+    shape = (dc[2], dc[1], dc[0])
+    corr = torch.empty(shape, device=dev)
+    ang = torch.empty(shape + (3,), device=dev)
+    idx = torch.empty(shape, dtype=torch.int32, device=dev)
+    s1, s2 = torch.empty_like(corr), torch.empty_like(corr)
+
+    def step():
+        corr.fill_(-1e30)
+        tp.perform_correlation(corr, ang, angles=angles, index=idx, sum1=s1, sum2=s2)
+    sec = _timed_steps(args, step)
+    tp.close()
+    vox = int(np.prod(dc))
+    # the update's HBM bytes per voxel and orientation: maximum, sums read and written (20 B); the
+    # angle and index are written only where the maximum rises, and the 41 maps (43 MB) stay in cache
+    achieved = vox * len(angles) * 20 / sec / 1e9
+
+    def cpu():
+        from oracle import pyoracle as po
+        k = 2
+        t0 = time.perf_counter()
+        po.tomo_picker(ref.astype(np.float64), mask.astype(np.float64), True, images.astype(np.float64),
+                       tilts.astype(np.float64), w.astype(np.float64), angles[:k].astype(np.float64), dc)
+        ct = time.perf_counter() - t0
+        return vox * k, ct, (f"same images, reference and volume, the first {k} orientations, FP64 oracle, "
+                             f"{ct:.1f} s including its set-up")
+    return _row_line(args, "tomo_picker", "correlation-volume voxel-orientations/s, TomoPicker",
+                     "voxel-orientations/s", vox * len(angles), sec,
+                     {"workload": "TomoPicker.PerformCorrelation", "tilt_images": [nimg, ny, nx], "ref": [nr] * 3,
+                      "volume": list(dc), "anglestep_deg": 30, "orientations": len(angles)}, cpu,
+                     extra={"roofline": {"bound": "hbm", "kernel": "whole step over the update's bytes (20 B per "
+                                                                   "voxel-orientation)",
+                                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}})
 
 
 def row_pick_large(args):
@@ -553,27 +608,34 @@ def row_micrographs(args):
 def row_split(args):
     """Precision: the 64^3 configuration (3 planted copies, 588 orientations, CTF) searched with
     every per-orientation inverse transform's radix-16 stage on the matrix cores, fp16 and bf16
-    hi/lo split (and hi alone), against the FP64 oracle and beside the fp32 search."""
+    hi/lo split (and hi alone), beside the fp32 search. The CPU baseline leg runs the FP64 oracle on
+    the whole workload; its result is also the checker of every precision's error (none without it)."""
     import torch
     import gtom_amd as gx
     from gtom_amd import synth
-    from oracle import pyoracle as po
     dev = torch.device("cuda:0")
     w = synth.cfg1(seed=1)
     n, p, s = w["n"], w["p"], w["oversample"]
     projft = gx.prepare_projector(torch.from_numpy(w["template"]).to(dev), p)
     E = gx.d_FFTR2C(torch.from_numpy(w["tomogram"][None]).to(dev))
-    ctf_np = po.ctf3(n, 10e-10, 300e3, 2.7e-3, -3e-6, 0.07).astype(np.float32)
-    ctf = torch.from_numpy(ctf_np).to(dev)
+    ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=10e-10, Cs=2.7e-3, voltage=300e3, defocus=-3e-6, amplitude=0.07)],
+                    n, device=dev)
     angles = w["angles"]
+    ref = cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+        t0 = time.perf_counter()
+        ref = po.pick(projft.cpu().numpy().astype(np.complex128), p, s, E.cpu().numpy().astype(np.complex128),
+                      ctf.cpu().numpy()[0].astype(np.float64), n, angles.astype(np.float64), w["maskradius"])
+        ct = time.perf_counter() - t0
+        cpu = {"value": n ** 3 * len(angles) / ct, "unit": "voxel-orientations/s", "cores": po.num_threads(),
+               "kind": "port", "sample": f"the whole workload, FP64 oracle, {ct:.1f} s"}
+        scale = np.abs(ref["best"]).max()
+        sure = (ref["top1"] - ref["top2"]) > 1e-4 * np.abs(ref["top1"]).max()
     # 64 orientations per batch: the split pipeline's fastest (5.66 ms per call against 5.78 at 256);
     # the fp32 search is timed at that batch and, as "fp32-default", at the library's default (256)
     plan = gx.PickPlan(p, n, args.batch or 64)
     plan_default = gx.PickPlan(p, n)
-    ref = po.pick(projft.cpu().numpy().astype(np.complex128), p, s, E.cpu().numpy().astype(np.complex128),
-                  ctf_np.astype(np.float64), n, angles.astype(np.float64), w["maskradius"])
-    scale = np.abs(ref["best"]).max()
-    sure = (ref["top1"] - ref["top2"]) > 1e-4 * np.abs(ref["top1"]).max()
     results = {}
     for name in ("fp32", "fp32-default", "fp16", "bf16", "fp16-hi", "bf16-hi"):
         if name.startswith("fp32"):
@@ -587,10 +649,11 @@ def row_split(args):
         torch.cuda.synchronize()
         b, i = best.cpu().numpy(), idx.cpu().numpy()
         sec = _timed_steps(args, step)
-        results[name] = {"ms_per_step": round(sec * 1e3, 3), "vo_per_s": n ** 3 * len(angles) / sec,
-                         "max_cc_error_vs_fp64": float(np.abs(b - ref["best"]).max() / scale),
-                         "argmax_agreement": float((i[sure] == ref["idx"][sure]).mean()),
-                         "argmax_voxels": int(sure.sum())}
+        results[name] = {"ms_per_step": round(sec * 1e3, 3), "vo_per_s": n ** 3 * len(angles) / sec}
+        if ref is not None:
+            results[name].update({"max_cc_error_vs_fp64": float(np.abs(b - ref["best"]).max() / scale),
+                                  "argmax_agreement": float((i[sure] == ref["idx"][sure]).mean()),
+                                  "argmax_voxels": int(sure.sum())})
         log(f"{name}: {results[name]}")
     batch = plan.batch
     plan.close()
@@ -610,6 +673,8 @@ def row_split(args):
                         "achieved": round(f16["vo_per_s"] * algo / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(f16["vo_per_s"] * algo / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                         "algorithmic_bytes_per_vo": round(algo, 3)}}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     return 0
 
@@ -882,7 +947,8 @@ def main():
                 "local_peaks": row_local_peaks, "fft_micrograph": row_fft_micrograph,
                 "prob_picker_micrograph": row_prob_picker_micrograph,
                 "pick2d_micrograph": row_pick2d_micrograph, "micrographs": row_micrographs,
-                "split": row_split, "pnet": row_pnet, "pick_large": row_pick_large}[args.row](args)
+                "split": row_split, "pnet": row_pnet, "pick_large": row_pick_large,
+                "tomo_picker": row_tomo_picker}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth

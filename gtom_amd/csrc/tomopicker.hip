@@ -167,36 +167,77 @@ __global__ __launch_bounds__(256) void k_local_cc(float* __restrict__ num, const
 // UpdateTomoCorrKernel (src/Correlation/TomoPicker.cu:311-358) with the texture
 // fetch as bilinear interpolation of CC_n at (px - 0.5, py - 0.5) with clamped
 // indices (unnormalised coordinates clamp); T: nimg x (row 0, row 1) of Matrix3Euler(tilt_n).
+// The tilt rows and weights are staged in LDS (at most 128 images); every image
+// is sampled without a divergent branch (indices clamped, weight 0 where the
+// point falls outside, which adds exactly 0 to both sums), with 32-bit byte
+// offsets within an image (images of at most 4096^2). An image whose second
+// row is exactly (0, 1, 0) -- a tilt about y alone -- maps every voxel to
+// py = y - cy/2 + ny/2 + 0.5, so its sample lies on a pixel row (tv = 0) and
+// only the two pixels of that row are read: the general formula's second row
+// enters with weight exactly 0 there.
+__device__ __forceinline__ bool in_range(float p, float n) { return fminf(p, n - p) >= 0.f; }
+
 __global__ __launch_bounds__(256) void k_tomo_update(const float* __restrict__ cc, int nx, int ny, int nimg,
                                                      const float* __restrict__ T, const float* __restrict__ iw,
                                                      int cx, int cy, int cz, float* __restrict__ corr,
                                                      float3* __restrict__ corrangle, int32_t* __restrict__ corridx,
                                                      float* __restrict__ sum1, float* __restrict__ sum2, float3 angle,
                                                      int aindex) {
+  // image n: (T0, T1, T2, T3), (T4, T5, weight, 1 if the second row is (0, 1, 0) else 0)
+  __shared__ float4 sT[2 * 128];
+  for (int i = threadIdx.x; i < 2 * nimg; i += 256) {
+    const int n = i >> 1;
+    const float* Tn = T + 6 * n;
+    const bool ytilt = Tn[3] == 0.f && Tn[4] == 1.f && Tn[5] == 0.f;
+    sT[i] = (i & 1) ? make_float4(Tn[4], Tn[5], iw[n], ytilt ? 1.f : 0.f) : make_float4(Tn[0], Tn[1], Tn[2], Tn[3]);
+  }
+  __syncthreads();
   const size_t total = (size_t)cx * cy * cz;
   const size_t vi = (size_t)nx * ny;
+  const float fnx = (float)nx, fny = (float)ny, fx1 = (float)(nx - 1), fy1 = (float)(ny - 1);
+  const float icx = (float)(nx / 2) + 0.5f, icy = (float)(ny / 2) + 0.5f;
   for (size_t o = blockIdx.x * (size_t)256 + threadIdx.x; o < total; o += (size_t)gridDim.x * 256) {
     const int x = (int)(o % cx), y = (int)((o / cx) % cy), z = (int)(o / ((size_t)cx * cy));
     const float px0 = (float)x - (float)(cx / 2), py0 = (float)y - (float)(cy / 2), pz0 = (float)z - (float)(cz / 2);
-    const float icx = (float)(nx / 2) + 0.5f, icy = (float)(ny / 2) + 0.5f;
+    // the y-tilt images' row: py = py0 + icy, v = py - 0.5 an integer
+    const float pyt = py0 + icy;
+    const bool yin = in_range(pyt, fny);
+    const float vt = pyt - 0.5f;
+    const unsigned rt = __umul24((unsigned)__builtin_amdgcn_fmed3f(vt, 0.f, fy1), (unsigned)nx);
     float sum = 0.f, samples = 0.f;
     for (int n = 0; n < nimg; ++n) {
-      const float* Tn = T + 6 * n;
-      const float px = Tn[0] * px0 + Tn[1] * py0 + Tn[2] * pz0 + icx;
-      const float py = Tn[3] * px0 + Tn[4] * py0 + Tn[5] * pz0 + icy;
-      if (px >= 0.f && px <= (float)nx && py >= 0.f && py <= (float)ny) {
-        const float u = px - 0.5f, v = py - 0.5f;
-        const float fu = floorf(u), fv = floorf(v);
-        const float tu = u - fu, tv = v - fv;
-        const int x0 = (int)fu, y0 = (int)fv;
-        const int xa = min(max(x0, 0), nx - 1), xb = min(max(x0 + 1, 0), nx - 1);
-        const int ya = min(max(y0, 0), ny - 1), yb = min(max(y0 + 1, 0), ny - 1);
-        const float* c = cc + vi * n;
-        const float val = (1.f - tv) * ((1.f - tu) * c[(size_t)ya * nx + xa] + tu * c[(size_t)ya * nx + xb]) +
-                          tv * ((1.f - tu) * c[(size_t)yb * nx + xa] + tu * c[(size_t)yb * nx + xb]);
-        sum += val * iw[n];
-        samples += iw[n];
+      const float4 ta = sT[2 * n], tb = sT[2 * n + 1];
+      const char* c = reinterpret_cast<const char*>(cc + vi * n);
+      const float px = ta.x * px0 + ta.y * py0 + ta.z * pz0 + icx;
+      const float u = px - 0.5f;
+      const float fu = floorf(u);
+      const float tu = u - fu;
+      const unsigned xa = (unsigned)__builtin_amdgcn_fmed3f(fu, 0.f, fx1);
+      const unsigned xb = (unsigned)__builtin_amdgcn_fmed3f(fu + 1.f, 0.f, fx1);
+      float val;
+      bool inside;
+      if (__builtin_amdgcn_readfirstlane(__float_as_int(tb.w)) != 0) {
+        const float a00 = *reinterpret_cast<const float*>(c + 4u * (rt + xa));
+        const float a01 = *reinterpret_cast<const float*>(c + 4u * (rt + xb));
+        val = (1.f - tu) * a00 + tu * a01;
+        inside = in_range(px, fnx) & yin;
+      } else {
+        const float py = ta.w * px0 + tb.x * py0 + tb.y * pz0 + icy;
+        const float v = py - 0.5f;
+        const float fv = floorf(v);
+        const float tv = v - fv;
+        const unsigned ra = __umul24((unsigned)__builtin_amdgcn_fmed3f(fv, 0.f, fy1), (unsigned)nx);
+        const unsigned rb = __umul24((unsigned)__builtin_amdgcn_fmed3f(fv + 1.f, 0.f, fy1), (unsigned)nx);
+        const float a00 = *reinterpret_cast<const float*>(c + 4u * (ra + xa));
+        const float a01 = *reinterpret_cast<const float*>(c + 4u * (ra + xb));
+        const float a10 = *reinterpret_cast<const float*>(c + 4u * (rb + xa));
+        const float a11 = *reinterpret_cast<const float*>(c + 4u * (rb + xb));
+        val = (1.f - tv) * ((1.f - tu) * a00 + tu * a01) + tv * ((1.f - tu) * a10 + tu * a11);
+        inside = in_range(px, fnx) & in_range(py, fny);
       }
+      const float w = inside ? tb.z : 0.f;
+      sum += val * w;
+      samples += w;
     }
     if (samples > 0.f) {
       sum /= samples;
