@@ -1,14 +1,21 @@
 // Batched R2C / C2R transforms of 3D, 2D and 1D volumes (axes 2^a up to 4096
-// or 3*2^a up to 3072, from 8) for gfx950. Replace the cuFFT plans wrapped by
-// d_FFTR2C and d_IFFTC2R (src/FFT/FFT.cu:7-42, src/FFT/IFFT.cu:8-56). Axes up
-// to 256 (rows up to 256 reals) take one pass each; longer ones two.
+// or 3*2^a up to 3072, from 8; the standalone entry points also take any other
+// extent up to 2048 by the chirp-z form at the end of this file) for gfx950.
+// Replace the cuFFT plans wrapped by d_FFTR2C and d_IFFTC2R (src/FFT/FFT.cu:7-42,
+// src/FFT/IFFT.cu:8-56). Axes up to 256 (rows up to 256 reals) take one pass
+// each; longer ones two.
 //
 // Layout: real (nx, ny, nz) x-fastest; half spectrum (nx/2+1, ny, nz).
 // R2C = rows (real -> half, via an nx/2-point complex FFT) then y then z.
 // C2R = z then y (in place on a copy) then rows (half -> real). The row C2R
 // uses only the real parts of the x = 0 and x = nx/2 bins, which makes the
 // result the real part of the Hermitian-completed inverse.
+#include <algorithm>
+#include <complex>
 #include <cstdint>
+#include <map>
+#include <mutex>
+#include <vector>
 #include <type_traits>
 
 #include "dev_fft.hpp"
@@ -1035,28 +1042,291 @@ int fft_c2r_cols_inplace(float2* data, gtomx_int3 d, unsigned batch, hipStream_t
   return yz_axes(data, scr.p, d, batch, 1, tw, s);
 }
 
+// ---------------------------------------------------------------------------
+// Axes of any length L <= 2048 that the kernels above do not take (odd
+// lengths, other prime factors), by Bluestein's chirp-z algorithm: with
+// c[j] = exp(-i pi j^2 / L), the forward DFT is
+//   X[k] = c[k] sum_j (x[j] c[j]) conj(c[k - j]),
+// a circular convolution of length M (the power of two >= 2L - 1, at least 8)
+// done with the native M-point column transforms: load a[j] = x[j] c[j]
+// (zero for j >= L), forward M-point DFT, times B[k] / M with B the DFT of
+// b[m] = conj(c[m]) (m in (-L, L), wrapped), inverse M-point DFT, times c[k].
+// The inverse DFT uses conj(c) for c and c for b. The lines of one axis are
+// gathered into an M x (lines) block (line index fastest, the layout the
+// column kernels take), in chunks that keep the block within 2^31 elements
+// and 256 MB.
+
+// Line l of an axis: its first point at (l / P) * S1 + (l % P) * S0, its
+// points es apart.
+struct AxisLines {
+  long long P, S0, S1, es;
+};
+__device__ __forceinline__ long long line_base(const AxisLines& a, long long l) {
+  return (l / a.P) * a.S1 + (l % a.P) * a.S0;
+}
+
+// MODE 0: complex points; 1: real points (the x rows of an R2C); 2: a half
+// spectrum of nh points completed as X[k] = conj(X[L - k]) for k >= nh (the x
+// rows of a C2R). DIR -1 multiplies by c, +1 by conj(c).
+template <int MODE, int DIR>
+__global__ __launch_bounds__(256) void k_blue_load(const void* __restrict__ src, AxisLines a, long long l0, int nl,
+                                                   int L, int nh, int M, const float2* __restrict__ chirp,
+                                                   float2* __restrict__ W) {
+  const long long total = (long long)M * nl;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int j = (int)(i / nl), l = (int)(i % nl);
+    float2 v = make_float2(0.f, 0.f);
+    if (j < L) {
+      const long long b = line_base(a, l0 + l);
+      if constexpr (MODE == 0) {
+        v = static_cast<const float2*>(src)[b + j * a.es];
+      } else if constexpr (MODE == 1) {
+        v = make_float2(static_cast<const float*>(src)[b + j * a.es], 0.f);
+      } else {
+        const float2* X = static_cast<const float2*>(src);
+        v = j < nh ? X[b + j * a.es] : cconj(X[b + (L - j) * a.es]);
+      }
+      const float2 c = chirp[j];
+      v = cmul(v, DIR < 0 ? c : cconj(c));
+    }
+    W[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_blue_mul(float2* __restrict__ W, int nl, int M,
+                                                  const float2* __restrict__ B) {
+  const long long total = (long long)M * nl;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < total; i += (long long)gridDim.x * 256)
+    W[i] = cmul(W[i], B[i / nl]);
+}
+
+// MODE 0: complex points k < L; 1: the half spectrum k < nh; 2: the real part
+// times scale (k < L).
+template <int MODE, int DIR>
+__global__ __launch_bounds__(256) void k_blue_store(const float2* __restrict__ W, AxisLines a, long long l0, int nl,
+                                                    int nout, const float2* __restrict__ chirp, void* __restrict__ dst,
+                                                    float scale) {
+  const long long total = (long long)nout * nl;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int k = (int)(i / nl), l = (int)(i % nl);
+    const float2 c = chirp[k];
+    const float2 v = cmul(W[i], DIR < 0 ? c : cconj(c));
+    const long long o = line_base(a, l0 + l) + k * a.es;
+    if constexpr (MODE == 2)
+      static_cast<float*>(dst)[o] = v.x * scale;
+    else
+      static_cast<float2*>(dst)[o] = v;
+  }
+}
+
+// Chirp and kernel spectra of one length, per device, computed once in long
+// double: chirp[j] = exp(-i pi (j^2 mod 2L) / L), bf / bi = the M-point DFTs
+// of the forward / inverse convolution kernels, divided by M.
+struct BlueTables {
+  int M = 0;
+  float2* chirp = nullptr;
+  float2* bf = nullptr;
+  float2* bi = nullptr;
+};
+std::mutex g_blue_mu;
+std::map<std::pair<int, int>, BlueTables> g_blue;
+
+int blue_tables(int L, BlueTables* out) {
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_blue_mu);
+  auto it = g_blue.find({dev, L});
+  if (it != g_blue.end()) {
+    *out = it->second;
+    return GTOMX_OK;
+  }
+  int M = 8;
+  while (M < 2 * L - 1) M *= 2;
+  if (M > 4096) return fail(GTOMX_ERR_UNSUPPORTED, "lengths without a native transform must be at most 2048");
+  const long double pi = 3.141592653589793238462643383279502884L;
+  std::vector<std::complex<long double>> c(L), bf(M), bi(M);
+  for (int j = 0; j < L; ++j) {
+    const long long r = ((long long)j * j) % (2LL * L);
+    c[j] = std::polar(1.0L, -pi * (long double)r / (long double)L);
+  }
+  std::vector<std::complex<long double>> kf(M, 0.0L), ki(M, 0.0L);
+  for (int m = 0; m < L; ++m) {
+    kf[m] = std::conj(c[m]);
+    ki[m] = c[m];
+    if (m > 0) {
+      kf[M - m] = std::conj(c[m]);
+      ki[M - m] = c[m];
+    }
+  }
+  std::vector<std::complex<long double>> w(M);
+  for (int t = 0; t < M; ++t) w[t] = std::polar(1.0L, -2.0L * pi * (long double)t / (long double)M);
+  for (int k = 0; k < M; ++k) {
+    std::complex<long double> sf = 0.0L, si = 0.0L;
+    for (int m = 0; m < M; ++m) {
+      const std::complex<long double>& e = w[(int)(((long long)m * k) % M)];
+      sf += kf[m] * e;
+      si += ki[m] * e;
+    }
+    bf[k] = sf / (long double)M;
+    bi[k] = si / (long double)M;
+  }
+  std::vector<float2> h((size_t)L + 2 * (size_t)M);
+  for (int j = 0; j < L; ++j) h[j] = make_float2((float)c[j].real(), (float)c[j].imag());
+  for (int k = 0; k < M; ++k) {
+    h[L + k] = make_float2((float)bf[k].real(), (float)bf[k].imag());
+    h[L + M + k] = make_float2((float)bi[k].real(), (float)bi[k].imag());
+  }
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, h.size() * sizeof(float2));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(chirp tables)");
+  e = hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "hipMemcpy(chirp tables)");
+  }
+  BlueTables t;
+  t.M = M;
+  t.chirp = static_cast<float2*>(d);
+  t.bf = t.chirp + L;
+  t.bi = t.bf + M;
+  g_blue[{dev, L}] = t;
+  *out = t;
+  return GTOMX_OK;
+}
+
+unsigned blue_grid(long long n) { return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 65536)); }
+
+// One axis of nlines lines of length L by the chirp-z form: IN / OUT are the
+// load and store modes above, src / dst laid out by as / ad (they may be the
+// same array: every chunk is loaded before it is stored).
+template <int IN, int OUT, int DIR>
+int blue_axis(const void* src, AxisLines as, void* dst, AxisLines ad, long long nlines, int L, int nh, float scale,
+              hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  BlueTables t;
+  GX_TRY(blue_tables(L, &t));
+  const int M = t.M;
+  const long long cap = std::max<long long>(1, std::min<long long>(((1LL << 31) - 1) / M, (32LL << 20) / M));
+  const int chunk = (int)std::min<long long>(nlines, cap);
+  StreamScratch W, W2;
+  GX_TRY(W.take((size_t)M * chunk, s));
+  if (M > 512) GX_TRY(W2.take((size_t)M * chunk, s));
+  const int nout = OUT == 1 ? nh : L;
+  for (long long l0 = 0; l0 < nlines; l0 += chunk) {
+    const int nl = (int)std::min<long long>(chunk, nlines - l0);
+    hipLaunchKernelGGL((k_blue_load<IN, DIR>), dim3(blue_grid((long long)M * nl)), dim3(256), 0, s, src, as, l0, nl,
+                       L, nh, M, t.chirp, W.p);
+    GX_LAUNCHED("k_blue_load");
+    GX_TRY(cols_axis<-1>(M, W.p, W2.p, nl, nl, 1, 1LL << 62, (long long)M * nl, 0, tw, s));
+    hipLaunchKernelGGL(k_blue_mul, dim3(blue_grid((long long)M * nl)), dim3(256), 0, s, W.p, nl, M,
+                       DIR < 0 ? t.bf : t.bi);
+    GX_LAUNCHED("k_blue_mul");
+    GX_TRY(cols_axis<1>(M, W.p, W2.p, nl, nl, 1, 1LL << 62, (long long)M * nl, 0, tw, s));
+    hipLaunchKernelGGL((k_blue_store<OUT, DIR>), dim3(blue_grid((long long)nout * nl)), dim3(256), 0, s, W.p, ad,
+                       l0, nl, nout, t.chirp, dst, scale);
+    GX_LAUNCHED("k_blue_store");
+  }
+  return GTOMX_OK;
+}
+
+// The y and z axes of half spectra, each native or by the chirp-z form.
+int yz_axes_any(float2* data, gtomx_int3 d, unsigned batch, int dir, hipStream_t s) {
+  const float2* tw = nullptr;
+  GX_TRY(twiddles(&tw));
+  const long long nh = d.x / 2 + 1;
+  StreamScratch scr;
+  if ((d.y > 256 && fft_len_ok_any(d.y)) || (d.z > 256 && fft_len_ok_any(d.z)))
+    GX_TRY(scr.take(elements_ft(d) * batch, s));
+  auto yax = [&](auto DIRC) -> int {
+    constexpr int D = decltype(DIRC)::value;
+    if (fft_len_ok_any(d.y))
+      return cols_axis<D>(d.y, data, scr.p, (int)nh, nh, (long long)d.z * batch, 1LL << 62, nh * d.y, 0, tw, s);
+    const AxisLines a{nh, 1, nh * d.y, nh};
+    return blue_axis<0, 0, D>(data, a, data, a, nh * d.z * batch, d.y, 0, 1.f, s);
+  };
+  auto zax = [&](auto DIRC) -> int {
+    constexpr int D = decltype(DIRC)::value;
+    if (fft_len_ok_any(d.z))
+      return cols_axis<D>(d.z, data, scr.p, (int)nh, nh * d.y, (long long)d.y * batch, d.y, nh, nh * d.y * d.z, tw,
+                          s);
+    const AxisLines a{nh * d.y, 1, nh * d.y * d.z, nh * d.y};
+    return blue_axis<0, 0, D>(data, a, data, a, nh * d.y * batch, d.z, 0, 1.f, s);
+  };
+  using F = std::integral_constant<int, -1>;
+  using B = std::integral_constant<int, 1>;
+  if (dir < 0) {
+    if (d.y > 1) GX_TRY(yax(F{}));
+    if (d.z > 1) GX_TRY(zax(F{}));
+  } else {
+    if (d.z > 1) GX_TRY(zax(B{}));
+    if (d.y > 1) GX_TRY(yax(B{}));
+  }
+  return GTOMX_OK;
+}
+
+bool dims_native(gtomx_int3 d) {
+  return fft_len_ok_any(d.x) && (d.y == 1 || fft_len_ok_any(d.y)) && (d.z == 1 || fft_len_ok_any(d.z));
+}
+
+int fft_r2c_any(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStream_t s) {
+  if (dims_native(d)) return fft_r2c(in, out, d, batch, s);
+  const long long rows = (long long)d.y * d.z * batch, nh = d.x / 2 + 1;
+  if (fft_len_ok_any(d.x)) {
+    const float2* tw = nullptr;
+    GX_TRY(twiddles(&tw));
+    if (d.x > 256)
+      GX_TRY(launch_rows_long<true>(d.x, in, out, rows, 1.f, tw, s));
+    else
+      GX_TRY(launch_rows_r2c(d.x, in, out, rows, tw, s));
+  } else {
+    GX_TRY((blue_axis<1, 1, -1>(in, AxisLines{1LL << 62, d.x, 0, 1}, out, AxisLines{1LL << 62, nh, 0, 1}, rows, d.x,
+                                (int)nh, 1.f, s)));
+  }
+  return yz_axes_any(out, d, batch, -1, s);
+}
+
+// work holds a copy of the half spectra (it is transformed in place).
+int fft_c2r_any(float2* work, float* out, gtomx_int3 d, unsigned batch, float scale, hipStream_t s) {
+  GX_TRY(yz_axes_any(work, d, batch, 1, s));
+  const long long rows = (long long)d.y * d.z * batch, nh = d.x / 2 + 1;
+  if (fft_len_ok_any(d.x)) {
+    const float2* tw = nullptr;
+    GX_TRY(twiddles(&tw));
+    if (d.x > 256) return launch_rows_long<false>(d.x, work, out, rows, scale, tw, s);
+    return launch_rows_c2r(d.x, work, out, rows, scale, tw, s);
+  }
+  return blue_axis<2, 2, 1>(work, AxisLines{1LL << 62, nh, 0, 1}, out, AxisLines{1LL << 62, d.x, 0, 1}, rows, d.x,
+                            (int)nh, scale, s);
+}
+
 }  // namespace gx
 
 using namespace gx;
 
 extern "C" int gtomx_fft_r2c(const float* d_input, gtomx_complex* d_output, int ndims, gtomx_int3 dims,
                              unsigned batch, void* stream) {
-  GX_TRY(check_fft_dims(ndims, dims));
+  GX_TRY(check_fft_dims_any(ndims, dims));
   if (!d_input || !d_output) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be positive");
-  return fft_r2c(d_input, reinterpret_cast<float2*>(d_output), dims, batch, as_stream(stream));
+  return fft_r2c_any(d_input, reinterpret_cast<float2*>(d_output), dims, batch, as_stream(stream));
 }
 
 extern "C" int gtomx_ifft_c2r(const gtomx_complex* d_input, float* d_output, int ndims, gtomx_int3 dims,
                               unsigned batch, int renormalize, void* stream) {
-  GX_TRY(check_fft_dims(ndims, dims));
+  GX_TRY(check_fft_dims_any(ndims, dims));
   if (!d_input || !d_output) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be positive");
   DevBuf work;
   GX_TRY(work.alloc(elements_ft(dims) * batch * sizeof(float2)));
   hipStream_t s = as_stream(stream);
   float scale = renormalize ? (float)(1.0 / (double)elements(dims)) : 1.0f;
-  GX_TRY(fft_c2r(reinterpret_cast<const float2*>(d_input), work.as<float2>(), d_output, dims, batch, scale, s));
+  if (dims_native(dims)) {
+    GX_TRY(fft_c2r(reinterpret_cast<const float2*>(d_input), work.as<float2>(), d_output, dims, batch, scale, s));
+  } else {
+    GX_HIP(hipMemcpyAsync(work.p, d_input, elements_ft(dims) * batch * sizeof(float2), hipMemcpyDeviceToDevice, s));
+    GX_TRY(fft_c2r_any(work.as<float2>(), d_output, dims, batch, scale, s));
+  }
   // The workspace is released when this call returns; wait for its last use.
   GX_HIP(hipStreamSynchronize(s));
   return GTOMX_OK;

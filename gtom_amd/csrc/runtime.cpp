@@ -105,6 +105,19 @@ int check_fft_dims(int ndims, gtomx_int3 d) {
   return GTOMX_OK;
 }
 
+int check_fft_dims_any(int ndims, gtomx_int3 d) {
+  if (ndims < 1 || ndims > 3) return fail(GTOMX_ERR_INVALID_ARGUMENT, "ndims must be 1, 2 or 3");
+  if (d.x < 1 || d.y < 1 || d.z < 1) return fail(GTOMX_ERR_INVALID_ARGUMENT, "dims must be positive");
+  int want = 3 - (d.z == 1) - (d.z == 1 && d.y == 1);
+  if (ndims != want)
+    return fail(GTOMX_ERR_INVALID_ARGUMENT, "ndims does not match dims (trailing extents must be 1)");
+  auto ok = [](int n) { return fft_len_ok_any(n) || n <= 2048; };
+  if (!ok(d.x) || !ok(d.y) || !ok(d.z))
+    return fail(GTOMX_ERR_UNSUPPORTED,
+                "extents must be at most 2048, or 2^a up to 4096, or 3*2^a up to 3072");
+  return GTOMX_OK;
+}
+
 }  // namespace gx
 
 using namespace gx;

@@ -68,6 +68,9 @@ inline bool fft_len_ok_any(int n) {
 
 // Validates dims for an ndims-dimensional transform; returns GTOMX_OK or an error.
 int check_fft_dims(int ndims, gtomx_int3 dims);
+// The same for the standalone transforms, which also take any extent up to
+// 2048 that has no native transform (by the chirp-z form).
+int check_fft_dims_any(int ndims, gtomx_int3 dims);
 
 inline size_t elements(gtomx_int3 d) { return (size_t)d.x * d.y * d.z; }
 inline size_t elements_ft(gtomx_int3 d) { return (size_t)(d.x / 2 + 1) * d.y * d.z; }

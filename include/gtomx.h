@@ -88,14 +88,17 @@ int gtomx_projector_matrices(const gtomx_float3* h_angles, unsigned n, float ove
 /* ------------------------------------------------------------------------ */
 
 /* d_FFTR2C, include/FFT.cuh:70 / src/FFT/FFT.cu:7-42: unnormalised forward
- * transform of `batch` real volumes into half spectra. */
+ * transform of `batch` real volumes into half spectra. Extents: 2^a up to 4096
+ * and 3*2^a up to 3072 natively, any other extent up to 2048 by the chirp-z
+ * (Bluestein) form over the native power-of-two transforms. */
 int gtomx_fft_r2c(const float* d_input, gtomx_complex* d_output, int ndims, gtomx_int3 dims,
                   unsigned batch, void* stream);
 
 /* d_IFFTC2R, include/FFT.cuh:150 / src/FFT/IFFT.cu:8-56: inverse transform of
  * half spectra; renormalize != 0 multiplies by 1/N. The result is the real
  * part of the inverse DFT of the Hermitian-completed spectrum. The input is
- * not modified (the reference's cuFFT call may clobber it). */
+ * not modified (the reference's cuFFT call may clobber it). Extents as for
+ * gtomx_fft_r2c. */
 int gtomx_ifft_c2r(const gtomx_complex* d_input, float* d_output, int ndims, gtomx_int3 dims,
                    unsigned batch, int renormalize, void* stream);
 

@@ -75,9 +75,11 @@ def test_validation_errors_without_device_work(gx):
     L = gx.lib()
     i3 = gx.int3
     # unsupported transform length, bad ndims, null buffers, zero batch
-    assert L.gtomx_fft_r2c(None, None, 3, i3(100, 100, 100), 1, None) == gx.ERR_UNSUPPORTED
-    # axes above 256: 2^a up to 4096 and 3 * 2^a up to 3072 pass the size check
-    for n, ok in [(4096, True), (3072, True), (384, True), (8192, False), (6144, False), (320, False)]:
+    assert L.gtomx_fft_r2c(None, None, 3, i3(100, 100, 2049), 1, None) == gx.ERR_UNSUPPORTED
+    # the standalone transforms take 2^a up to 4096, 3 * 2^a up to 3072 and any other extent up
+    # to 2048 (chirp-z); 100 and 320 now pass the size check and stop at the NULL buffer
+    for n, ok in [(4096, True), (3072, True), (384, True), (8192, False), (6144, False), (320, True), (100, True),
+                  (7, True), (2048, True), (2049, False), (2500, False)]:
         st = L.gtomx_fft_r2c(None, None, 2, i3(n, 16, 1), 1, None)
         assert st == (gx.ERR_INVALID_ARGUMENT if ok else gx.ERR_UNSUPPORTED), n
         assert ("NULL" in gx.last_error()) == ok

@@ -1,0 +1,85 @@
+"""R2C / C2R transforms with extents that have no native kernel (odd lengths,
+other prime factors, up to 2048) by the chirp-z form, alone and mixed with
+native axes, against numpy's FP64 pocketfft (R2C, and C2R of Hermitian
+spectra) and the oracle's real-part C2R (non-Hermitian half spectra). cuFFT
+behind d_FFTR2C / d_IFFTC2R (src/FFT/FFT.cu:17-42, src/FFT/IFFT.cu:8-56) plans
+any extent. Tolerance 4e-6 normwise, as for the native transforms."""
+import numpy as np
+import pytest
+
+from tests._util import rel_linf, to_np
+
+pytestmark = pytest.mark.gpu
+
+TOL = 4e-6
+
+SHAPES = [  # (nz, ny, nx); x, y, z each native (N) or chirp-z (C)
+    (1, 1, 5),  # C
+    (1, 1, 100),  # C
+    (1, 1, 2047),  # C, the longest chirp-z length (4096-point convolution)
+    (1, 75, 100),  # C, C
+    (1, 64, 100),  # C x, N y
+    (1, 100, 64),  # N x, C y
+    (1, 7, 4096),  # N x (two-pass rows), C y
+    (20, 30, 50),  # C, C, C
+    (27, 64, 36),  # C x, N y, C z
+    (40, 8, 200),  # C x, N y, C z
+    (3, 2, 1000),  # tiny y and z
+]
+
+
+def _nd(shape):
+    nz, ny, nx = shape
+    return 3 if nz > 1 else (2 if ny > 1 else 1)
+
+
+def _dev(torch, a, shape, cuda):
+    """a is (batch, nz, ny, n); the transform's own rank drops the unit axes."""
+    nz, ny = shape[:2]
+    keep = {3: (nz, ny), 2: (ny,), 1: ()}[_nd(shape)]
+    return torch.from_numpy(a).to(cuda).reshape(a.shape[:1] + keep + a.shape[-1:])
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_fft_any_r2c_matches_numpy(gx, cuda, shape):
+    import torch
+    nz, ny, nx = shape
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz)
+    x = rng.standard_normal((2, nz, ny, nx)).astype(np.float32)
+    X = gx.d_FFTR2C(_dev(torch, x, shape, cuda), ndims=_nd(shape))
+    ref = np.fft.rfftn(x.astype(np.float64), axes=(1, 2, 3))
+    err = rel_linf(to_np(X).reshape(ref.shape), ref)
+    print(f"R2C {shape}: {err:.2e}")
+    assert err < TOL
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_fft_any_roundtrip(gx, cuda, shape):
+    """C2R of the R2C (renormalised) gives the input back: numpy's irfftn of the same spectrum."""
+    import torch
+    nz, ny, nx = shape
+    rng = np.random.default_rng(11 * nx + ny + nz)
+    x = rng.standard_normal((2, nz, ny, nx)).astype(np.float32)
+    X = gx.d_FFTR2C(_dev(torch, x, shape, cuda), ndims=_nd(shape))
+    y = gx.d_IFFTC2R(X, nx, ndims=_nd(shape), renormalize=True)
+    err = rel_linf(to_np(y).reshape(x.shape), x.astype(np.float64))
+    print(f"round trip {shape}: {err:.2e}")
+    assert err < TOL
+
+
+@pytest.mark.parametrize("shape", [(1, 30, 51), (5, 12, 70), (1, 1, 9)])
+def test_fft_any_c2r_nonhermitian(gx, orc, cuda, shape):
+    """Arbitrary half spectra: the real part of the Hermitian-completed inverse (the imaginary
+    parts of the x = 0 and, for even widths, x = nx/2 bins are not read), input unmodified."""
+    import torch
+    nz, ny, nx = shape
+    rng = np.random.default_rng(nx * 5 + ny)
+    X = (rng.standard_normal((1, nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((1, nz, ny, nx // 2 + 1)))
+    X = X.astype(np.complex64)
+    Xd = _dev(torch, X, shape, cuda)
+    x = gx.d_IFFTC2R(Xd, nx, ndims=_nd(shape), renormalize=False)
+    ref = orc.c2r(X[0].astype(np.complex128), nx)
+    err = rel_linf(to_np(x).reshape(ref.shape), ref)
+    print(f"C2R {shape}: {err:.2e}")
+    assert err < TOL
+    assert np.array_equal(to_np(Xd).reshape(X.shape), X)
