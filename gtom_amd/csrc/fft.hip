@@ -870,7 +870,11 @@ int yz_axes(float2* data, float2* scr, gtomx_int3 d, unsigned batch, int dir, co
   return GTOMX_OK;
 }
 
+int fft_r2c_any(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStream_t s);
+int fft_c2r_any(float2* work, float* out, gtomx_int3 d, unsigned batch, float scale, hipStream_t s);
+
 int fft_r2c(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStream_t s) {
+  if (!dims_native(d)) return fft_r2c_any(in, out, d, batch, s);
   const float2* tw = nullptr;
   GX_TRY(twiddles(&tw));
   const long long rows = (long long)d.y * d.z * batch;
@@ -890,6 +894,7 @@ int fft_c2r(const float2* in, float2* work, float* out, gtomx_int3 d, unsigned b
   const size_t H = elements_ft(d) * batch;
   if (cols < 0 || cols > d.x / 2 + 1) return fail(GTOMX_ERR_INTERNAL, "column count outside the half spectrum");
   if (work != in) GX_HIP(hipMemcpyAsync(work, in, H * sizeof(float2), hipMemcpyDeviceToDevice, s));
+  if (!dims_native(d)) return fft_c2r_any(work, out, d, batch, scale, s);
   StreamScratch scr;
   if (d.y > 256 || d.z > 256) GX_TRY(scr.take(H, s));
   GX_TRY(yz_axes(work, scr.p, d, batch, 1, tw, s, cols));
@@ -1266,13 +1271,13 @@ int yz_axes_any(float2* data, gtomx_int3 d, unsigned batch, int dir, hipStream_t
 }
 
 bool dims_native(gtomx_int3 d) {
-  return fft_len_ok_any(d.x) && (d.y == 1 || fft_len_ok_any(d.y)) && (d.z == 1 || fft_len_ok_any(d.z));
+  return row_len_native(d.x) && (d.y == 1 || fft_len_ok_any(d.y)) && (d.z == 1 || fft_len_ok_any(d.z));
 }
 
 int fft_r2c_any(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipStream_t s) {
   if (dims_native(d)) return fft_r2c(in, out, d, batch, s);
   const long long rows = (long long)d.y * d.z * batch, nh = d.x / 2 + 1;
-  if (fft_len_ok_any(d.x)) {
+  if (row_len_native(d.x)) {
     const float2* tw = nullptr;
     GX_TRY(twiddles(&tw));
     if (d.x > 256)
@@ -1290,7 +1295,7 @@ int fft_r2c_any(const float* in, float2* out, gtomx_int3 d, unsigned batch, hipS
 int fft_c2r_any(float2* work, float* out, gtomx_int3 d, unsigned batch, float scale, hipStream_t s) {
   GX_TRY(yz_axes_any(work, d, batch, 1, s));
   const long long rows = (long long)d.y * d.z * batch, nh = d.x / 2 + 1;
-  if (fft_len_ok_any(d.x)) {
+  if (row_len_native(d.x)) {
     const float2* tw = nullptr;
     GX_TRY(twiddles(&tw));
     if (d.x > 256) return launch_rows_long<false>(d.x, work, out, rows, scale, tw, s);
@@ -1298,6 +1303,13 @@ int fft_c2r_any(float2* work, float* out, gtomx_int3 d, unsigned batch, float sc
   }
   return blue_axis<2, 2, 1>(work, AxisLines{1LL << 62, nh, 0, 1}, out, AxisLines{1LL << 62, d.x, 0, 1}, rows, d.x,
                             (int)nh, scale, s);
+}
+
+int rows_c2r_any_inplace(float2* G, int n, int nb, long long H, hipStream_t s) {
+  const long long nh = n / 2 + 1, nn = (long long)n * n;
+  if (n % 2 || H != nh * nn) return fail(GTOMX_ERR_INTERNAL, "rows_c2r_any_inplace takes n^3 half spectra, n even");
+  return blue_axis<2, 2, 1>(G, AxisLines{1LL << 62, nh, 0, 1}, G, AxisLines{nn, n, 2 * H, 1}, (long long)nb * nn, n,
+                            (int)nh, 1.f, s);
 }
 
 }  // namespace gx
@@ -1309,7 +1321,7 @@ extern "C" int gtomx_fft_r2c(const float* d_input, gtomx_complex* d_output, int 
   GX_TRY(check_fft_dims_any(ndims, dims));
   if (!d_input || !d_output) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   if (batch == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be positive");
-  return fft_r2c_any(d_input, reinterpret_cast<float2*>(d_output), dims, batch, as_stream(stream));
+  return fft_r2c(d_input, reinterpret_cast<float2*>(d_output), dims, batch, as_stream(stream));
 }
 
 extern "C" int gtomx_ifft_c2r(const gtomx_complex* d_input, float* d_output, int ndims, gtomx_int3 dims,
@@ -1321,12 +1333,7 @@ extern "C" int gtomx_ifft_c2r(const gtomx_complex* d_input, float* d_output, int
   GX_TRY(work.alloc(elements_ft(dims) * batch * sizeof(float2)));
   hipStream_t s = as_stream(stream);
   float scale = renormalize ? (float)(1.0 / (double)elements(dims)) : 1.0f;
-  if (dims_native(dims)) {
-    GX_TRY(fft_c2r(reinterpret_cast<const float2*>(d_input), work.as<float2>(), d_output, dims, batch, scale, s));
-  } else {
-    GX_HIP(hipMemcpyAsync(work.p, d_input, elements_ft(dims) * batch * sizeof(float2), hipMemcpyDeviceToDevice, s));
-    GX_TRY(fft_c2r_any(work.as<float2>(), d_output, dims, batch, scale, s));
-  }
+  GX_TRY(fft_c2r(reinterpret_cast<const float2*>(d_input), work.as<float2>(), d_output, dims, batch, scale, s));
   // The workspace is released when this call returns; wait for its last use.
   GX_HIP(hipStreamSynchronize(s));
   return GTOMX_OK;

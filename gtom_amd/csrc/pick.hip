@@ -311,10 +311,13 @@ template <int N> std::vector<int> disc_tiles() {
 // ---------------------------------------------------------------------------
 constexpr int HERM_BLOCKS = 32;
 
-template <int N>
+// NT = 0: the side is the run-time n.
+template <int NT>
 __global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float* __restrict__ mats,
-                                                        const float* __restrict__ ctf, float* __restrict__ corr) {
-  constexpr int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
+                                                        const float* __restrict__ ctf, float* __restrict__ corr,
+                                                        int n) {
+  const int N = NT ? NT : n;
+  const int NH = N / 2 + 1, HALF = N / 2, RMAX2 = HALF * HALF;
   __shared__ float red[4];
   const int o = blockIdx.y;
   float M[9];
@@ -949,7 +952,7 @@ int run_batch(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const float
   }));
   // stage 1 of the timings: the plane correction and pass 2
   GX_TRY(timed(pl, 1, s, [&]() -> int {
-    hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr);
+    hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, pl->corr, N);
     GX_LAUNCHED("k_pick_herm_norm");
     hipLaunchKernelGGL(k_pick_y<N>, dim3(ColMap<N, Pass2Tile<N>::TX>::NWG, nb), dim3(Pass2Tile<N>::TX * R2), 0, s,
                        pl->G, pl->partial, pl->ntiles, pl->corr, pl->scale, tw, tmask);
@@ -973,13 +976,36 @@ int herm_norm3_any(int n, const Projector& P, const float* mats, const float* ct
   switch (n) {
 #define GX_CASE(N)                                                                                    \
   case N:                                                                                             \
-    hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, corr); \
+    hipLaunchKernelGGL(k_pick_herm_norm<N>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, corr, N); \
     GX_LAUNCHED("k_pick_herm_norm");                                                                  \
     return GTOMX_OK;
     GX_CASE(512) GX_CASE(768) GX_CASE(1024)
 #undef GX_CASE
   }
-  return fail(GTOMX_ERR_UNSUPPORTED, "volume side not supported");
+  hipLaunchKernelGGL(k_pick_herm_norm<0>, dim3(HERM_BLOCKS, nb), dim3(256), 0, s, P, mats, ctf, corr, n);
+  GX_LAUNCHED("k_pick_herm_norm");
+  return GTOMX_OK;
+}
+
+// The in-order update of the generic form: the real maps of nb orientations
+// (`stride` floats apart), each times scale[o], folded into best / index
+// (strict >, the first orientation winning ties), as k_pick_rows_max_long does.
+__global__ __launch_bounds__(256) void k_pick_max_maps(const float* __restrict__ maps, long long stride, int nb,
+                                                       const float* __restrict__ scale, long long V, int angle0,
+                                                       float* __restrict__ best, int32_t* __restrict__ idx) {
+  for (long long v = blockIdx.x * 256LL + threadIdx.x; v < V; v += (long long)gridDim.x * 256) {
+    float b = best[v];
+    int32_t i = idx[v];
+    for (int o = 0; o < nb; ++o) {
+      const float c = maps[o * stride + v] * scale[o];
+      if (c > b) {
+        b = c;
+        i = angle0 + o;
+      }
+    }
+    best[v] = b;
+    idx[v] = i;
+  }
 }
 
 template <class PJ>
@@ -998,9 +1024,19 @@ int run_batch3d_any(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const
     hipLaunchKernelGGL(k_pick_scale, dim3(nb), dim3(256), 0, s, pl->partial, pl->nparts, pl->corr, HERM_BLOCKS,
                        (double)pl->H, pl->scale);
     GX_LAUNCHED("k_pick_scale");
+    if (pl->gen3d) return yz_axes_any(pl->G, gtomx_int3{n, n, n}, (unsigned)nb, 1, s);
     return fft_c2r_cols_inplace(pl->G, gtomx_int3{n, n, n}, (unsigned)nb, s);
   }));
   return timed(pl, 2, s, [&]() -> int {
+    if (pl->gen3d) {
+      // the rows' C2R into the first n^3 floats of each spectrum's slot, then the in-order update
+      GX_TRY(rows_c2r_any_inplace(pl->G, n, nb, H, s));
+      const long long V = (long long)n * n * n;
+      hipLaunchKernelGGL(k_pick_max_maps, dim3((unsigned)std::min<long long>((V + 255) / 256, 16384)), dim3(256), 0,
+                         s, reinterpret_cast<const float*>(pl->G), 2 * H, nb, pl->scale, V, angle0, best, idx);
+      GX_LAUNCHED("k_pick_max_maps");
+      return GTOMX_OK;
+    }
     return pick_rows_max_long(n, pl->G, nb, H, n * n, angle0, pl->scale, best, idx, s, pl->rows_variant);
   });
 }
@@ -1011,8 +1047,8 @@ int run_batch3d_any(gtomx_pick_plan* pl, const PJ& PS, const Projector& P, const
 int run_batch_n(gtomx_pick_plan* pl, const CentredProjector* C, const Projector& P, const float* mats, int nb,
                 const float2* E, const float* ctf, float* best, int32_t* idx, int angle0, const float2* tw,
                 hipStream_t s, const uint8_t* tmask = nullptr) {
-  if (pl->n > 256) {
-    if (tmask) return fail(GTOMX_ERR_UNSUPPORTED, "tile masks need volumes of side at most 256");
+  if (pl->n > 256 || pl->gen3d) {
+    if (tmask) return fail(GTOMX_ERR_UNSUPPORTED, "tile masks need volumes of side 2^a or 3*2^a, at most 256");
     return C ? run_batch3d_any(pl, *C, P, mats, nb, E, ctf, best, idx, angle0, s)
              : run_batch3d_any(pl, P, P, mats, nb, E, ctf, best, idx, angle0, s);
   }
@@ -1352,10 +1388,13 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   const bool big2d = is2d && dv.x > 256;  // micrographs: the generic-transform form of the 2D branch
   // volumes above 256: the generic-transform form (sides with a long-row pass)
   const bool big3d = !is2d && (dv.x == 512 || dv.x == 768 || dv.x == 1024);
-  if (big2d ? !fft_len_ok_any(dv.x) : (!big3d && !fft_len_ok(dv.x)))
+  // other even volume sides from 8 to 1024: the same form on the chirp-z transforms
+  const bool gen3d = !is2d && !big3d && !fft_len_ok(dv.x) && dv.x % 2 == 0 && dv.x >= 8 && dv.x <= 1024;
+  const bool any3d = big3d || gen3d;
+  if (big2d ? !row_len_native(dv.x) : (!any3d && !fft_len_ok(dv.x)))
     return fail(GTOMX_ERR_UNSUPPORTED,
-                "volume side must be 2^a or 3*2^a in [8, 256], or 512, 768 or 1024; image side also 2^a up to 4096 "
-                "or 3*2^a up to 3072");
+                "volume side must be even, from 8 to 1024; image side 2^a or 3*2^a from 8 to 256, or 512, 768, "
+                "1024, 1536, 2048, 3072 or 4096");
   if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 512)
     return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 512");
   if (batch > 256) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 256");
@@ -1374,7 +1413,8 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   // in pass 1, the running maxima in pass 3 -- spread over more orientations; 8.7 GB of spectra at 256^3);
   // smaller volumes: 256 (their launches are short, so their number matters most: 1.89 ms for 588
   // orientations of 64^3 at 256 against 2.68 ms at 16)
-  pl->B = batch ? batch : (is2d ? (big2d ? big_b : 240u) : (big3d ? big3_b : (dv.x >= 128 ? 128u : 256u)));
+  pl->B = batch ? batch : (is2d ? (big2d ? big_b : 240u) : (any3d ? big3_b : (dv.x >= 128 ? 128u : 256u)));
+  pl->gen3d = gen3d;
   pl->og = dv.x >= 128 ? 2 : 1;
   int L1 = 0, L2 = 0;
   if (big2d && fft_len_split(dv.x, &L1, &L2) != GTOMX_OK) {
@@ -1382,9 +1422,9 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
     return fail(GTOMX_ERR_UNSUPPORTED, "image side has no two-pass split");
   }
   pl->nparts = big2d ? (dv.x / 2 + 1 + 63) / 64 * L2
-                     : (big3d ? (int)((pl->H + 256 * P3_BINS - 1) / (256 * P3_BINS)) : 0);
-  const std::vector<int> tiles = (is2d || big3d) ? std::vector<int>{0} : tiles_for(pl->n);
-  pl->ntiles = (big2d || big3d) ? 1 : (is2d ? g2_tiles_for(pl->n) : (int)tiles.size());
+                     : (any3d ? (int)((pl->H + 256 * P3_BINS - 1) / (256 * P3_BINS)) : 0);
+  const std::vector<int> tiles = (is2d || any3d) ? std::vector<int>{0} : tiles_for(pl->n);
+  pl->ntiles = (big2d || any3d) ? 1 : (is2d ? g2_tiles_for(pl->n) : (int)tiles.size());
   int s = GTOMX_OK;
   hipError_t he = hipGetDevice(&pl->device);
   if (he != hipSuccess) {
@@ -1398,12 +1438,12 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
     if (s == GTOMX_OK) pl->device_bytes += bytes;
   };
   // The batch buffer also serves as the C2R workspace in pick_finish (H elements).
-  const size_t gstride = (big2d || big3d) ? pl->H : (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n));
+  const size_t gstride = (big2d || any3d) ? pl->H : (is2d ? g2_stride_for(pl->n) : g_stride_for(pl->n));
   alloc(&d, std::max((size_t)pl->B * gstride, pl->H) * sizeof(float2));
   pl->G = static_cast<float2*>(d);
   d = nullptr;
   // up to 4 pass-1 waves per tile; above 256, one sum per pass-1 workgroup
-  alloc(&d, (size_t)pl->B * ((big2d || big3d) ? (size_t)pl->nparts : (size_t)pl->ntiles * 4) * sizeof(float));
+  alloc(&d, (size_t)pl->B * ((big2d || any3d) ? (size_t)pl->nparts : (size_t)pl->ntiles * 4) * sizeof(float));
   pl->partial = static_cast<float*>(d);
   d = nullptr;
   alloc(&d, (size_t)pl->ntiles * sizeof(int));

@@ -38,6 +38,7 @@ struct gtomx_pick_plan {
   float* gbest = nullptr;   // 2D: per-orientation-group maxima (groups x V)
   int32_t* gidx = nullptr;
   int og = 0;               // 2D: orientations per group
+  bool gen3d = false;       // 3D volumes whose side has no fused or long-row search: chirp-z transforms
   int nparts = 0;           // 2D above 256: pass-1 workgroups per orientation
   float2* scr = nullptr;    // 2D above 256: pass-1 output (B spectra)
   size_t device_bytes = 0;

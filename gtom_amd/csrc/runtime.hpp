@@ -63,6 +63,12 @@ inline bool fft_len_ok(int n) {
 inline bool fft_len_ok_any(int n) {
   return fft_len_ok(n) || (n > 256 && ((is_pow2(n) && n <= 4096) || (n % 3 == 0 && is_pow2(n / 3) && n <= 3072)));
 }
+// Row (x) extents of the native transforms: one pass up to 256, the long-row
+// kernels at 512, 768, 1024, 1536, 2048, 3072 and 4096 (other x extents,
+// 384 among them, take the chirp-z form).
+inline bool row_len_native(int n) {
+  return fft_len_ok(n) || n == 512 || n == 768 || n == 1024 || n == 1536 || n == 2048 || n == 3072 || n == 4096;
+}
 // Expands X(n) for every supported transform length.
 #define GX_FOR_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
@@ -136,6 +142,7 @@ struct DevBuf {
 // Batched 3D/2D transforms (dims with dims.z == 1 for 2D). `work` must hold
 // elements_ft(dims) * batch complex values for c2r (the input is copied
 // there first so it is left unmodified); r2c needs no workspace.
+// Extents without a native transform (at most 2048) go through the chirp-z form.
 int fft_r2c(const float* in, float2* out, gtomx_int3 dims, unsigned batch, hipStream_t s);
 // cols > 0 declares the columns x >= cols of every half spectrum zero: the y
 // and z inverses skip them (the result is the same).
@@ -173,6 +180,16 @@ int fft_c2r_cols_inplace(float2* data, gtomx_int3 dims, unsigned batch, hipStrea
 // orientation's spectrum rows into registers during each transform.
 int pick_rows_max_long(int n, const float2* G, int nb, long long H, int nrows, int angle0, const float* scale,
                        float* best, int32_t* idx, hipStream_t s, int variant = 0);
+// True when every axis of dims has a native transform.
+bool dims_native(gtomx_int3 dims);
+// The inverse along z then y of nb half spectra in place, each axis native or
+// by the chirp-z form (dir -1 forward, +1 inverse).
+int yz_axes_any(float2* data, gtomx_int3 dims, unsigned batch, int dir, hipStream_t s);
+// The x C2R (real part, unscaled) of the nb * n * n rows of nb n^3 half spectra
+// H apart in G, by the chirp-z form, written in place: the real map of
+// spectrum o fills the first n^3 floats of its slot (every row is read before
+// the rows it overwrites are written). n even, at most 2048.
+int rows_c2r_any_inplace(float2* G, int n, int nb, long long H, hipStream_t s);
 
 // Fourier-space resizing (resize.hip): op is one of the GTOMX_FFT_* values.
 constexpr int FFT_CROP = GTOMX_FFT_CROP, FFT_PAD = GTOMX_FFT_PAD, FFT_FULL_CROP = GTOMX_FFT_FULL_CROP,

@@ -624,7 +624,12 @@ typedef struct gtomx_pick_plan gtomx_pick_plan;
  * p = dimsprojector.x. The plan owns every device buffer the search needs
  * (the reference allocates them per call, src/Correlation/SubTomograms.cu:44-56).
  * batch is the number of orientations processed per pass (0 selects the
- * default: 32 or 16 for volumes, 240 for images as in the reference). */
+ * default: 256 for volumes below 128, 128 up to 256, 32 or fewer above
+ * (spectra of about 8 GiB), 240 for images as in the reference, fewer above
+ * 256 (about 1 GiB)). Volume sides: 2^a and 3*2^a from 8 to 256 (fused
+ * passes), 512, 768 and 1024 (long-row pass), and every other even side from
+ * 8 to 1024 (the same form on chirp-z transforms). Image sides: 2^a and 3*2^a
+ * from 8 to 256, and 512, 768, 1024, 1536, 2048, 3072 and 4096. */
 int gtomx_pick_plan_create(gtomx_pick_plan** plan, gtomx_int3 dimsprojector,
                            gtomx_int3 dimsvolume, unsigned batch);
 int gtomx_pick_plan_destroy(gtomx_pick_plan* plan);

@@ -25,6 +25,8 @@ SHAPES = [  # (nz, ny, nx); x, y, z each native (N) or chirp-z (C)
     (27, 64, 36),  # C x, N y, C z
     (40, 8, 200),  # C x, N y, C z
     (3, 2, 1000),  # tiny y and z
+    (1, 96, 384),  # C x (384 has native columns but no native row kernel), N y
+    (6, 384, 384),  # C x, N y (two-pass), C z
 ]
 
 
