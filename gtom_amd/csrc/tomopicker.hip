@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_tomo_update(const float* __restrict__ c
         const float a00 = *reinterpret_cast<const float*>(c + 4u * (rt + xa));
         const float a01 = *reinterpret_cast<const float*>(c + 4u * (rt + xb));
         val = (1.f - tu) * a00 + tu * a01;
-        inside = in_range(px, fnx) & yin;
+        inside = static_cast<int>(in_range(px, fnx)) & static_cast<int>(yin);
       } else {
         const float py = ta.w * px0 + tb.x * py0 + tb.y * pz0 + icy;
         const float v = py - 0.5f;
@@ -233,11 +233,82 @@ __global__ __launch_bounds__(256) void k_tomo_update(const float* __restrict__ c
         const float a10 = *reinterpret_cast<const float*>(c + 4u * (rb + xa));
         const float a11 = *reinterpret_cast<const float*>(c + 4u * (rb + xb));
         val = (1.f - tv) * ((1.f - tu) * a00 + tu * a01) + tv * ((1.f - tu) * a10 + tu * a11);
-        inside = in_range(px, fnx) & in_range(py, fny);
+        inside = static_cast<int>(in_range(px, fnx)) & static_cast<int>(in_range(py, fny));
       }
       const float w = inside ? tb.z : 0.f;
       sum += val * w;
       samples += w;
+    }
+    if (samples > 0.f) {
+      sum /= samples;
+      if (corr[o] < sum) {
+        corr[o] = sum;
+        if (corrangle) corrangle[o] = angle;
+        if (corridx) corridx[o] = aindex;
+      }
+      if (sum1) sum1[o] += sum;
+      if (sum2) sum2[o] += sum * sum;
+    }
+  }
+}
+
+// The same update for a tilt series about one in-plane axis: every image's
+// second row is (a, b, 0), so py = a px0 + b py0 + ny/2 + 0.5 is one value per
+// voxel for all images (its row pair, fraction and range test are computed
+// once), and the voxels are dealt to lanes along the direction that keeps py
+// nearly constant: with |a| <= |b| a wave runs along x over the sheared rows
+// y = (j + round(-a x / b)) mod cy (else along y over x = (j + round(-b y / a))
+// mod cx), so its samples fall on one or two image rows instead of up to 64 |a|.
+// Each voxel is still visited once, and the sample and sums are those of
+// k_tomo_update.
+__global__ __launch_bounds__(256) void k_tomo_update_axis(const float* __restrict__ cc, int nx, int ny, int nimg,
+                                                          const float* __restrict__ T, const float* __restrict__ iw,
+                                                          int cx, int cy, int cz, float* __restrict__ corr,
+                                                          float3* __restrict__ corrangle,
+                                                          int32_t* __restrict__ corridx, float* __restrict__ sum1,
+                                                          float* __restrict__ sum2, float3 angle, int aindex,
+                                                          int xmajor, float slope) {
+  __shared__ float4 sT[128];  // image n: (T0, T1, T2, weight)
+  for (int n = threadIdx.x; n < nimg; n += 256) sT[n] = make_float4(T[6 * n], T[6 * n + 1], T[6 * n + 2], iw[n]);
+  __syncthreads();
+  const float a = T[3], b = T[4], c = T[5];
+  const size_t total = (size_t)cx * cy * cz;
+  const size_t vi = (size_t)nx * ny;
+  const int cu = xmajor ? cx : cy, cv = xmajor ? cy : cx;
+  const float fnx = (float)nx, fny = (float)ny, fx1 = (float)(nx - 1), fy1 = (float)(ny - 1);
+  const float icx = (float)(nx / 2) + 0.5f, icy = (float)(ny / 2) + 0.5f;
+  for (size_t q = blockIdx.x * (size_t)256 + threadIdx.x; q < total; q += (size_t)gridDim.x * 256) {
+    const int u = (int)(q % cu), j = (int)((q / cu) % cv), z = (int)(q / ((size_t)cu * cv));
+    int w = (j + (int)rintf(slope * (float)u)) % cv;
+    w += w < 0 ? cv : 0;
+    const int x = xmajor ? u : w, y = xmajor ? w : u;
+    const size_t o = ((size_t)z * cy + y) * cx + x;
+    const float px0 = (float)x - (float)(cx / 2), py0 = (float)y - (float)(cy / 2), pz0 = (float)z - (float)(cz / 2);
+    const float py = a * px0 + b * py0 + c * pz0 + icy;
+    const bool yin = in_range(py, fny);
+    const float v = py - 0.5f;
+    const float fv = floorf(v);
+    const float tv = v - fv;
+    const unsigned ra = __umul24((unsigned)__builtin_amdgcn_fmed3f(fv, 0.f, fy1), (unsigned)nx);
+    const unsigned rb = __umul24((unsigned)__builtin_amdgcn_fmed3f(fv + 1.f, 0.f, fy1), (unsigned)nx);
+    float sum = 0.f, samples = 0.f;
+    for (int n = 0; n < nimg; ++n) {
+      const float4 ta = sT[n];
+      const char* cb = reinterpret_cast<const char*>(cc + vi * n);
+      const float px = ta.x * px0 + ta.y * py0 + ta.z * pz0 + icx;
+      const float uu = px - 0.5f;
+      const float fu = floorf(uu);
+      const float tu = uu - fu;
+      const unsigned xa = (unsigned)__builtin_amdgcn_fmed3f(fu, 0.f, fx1);
+      const unsigned xb = (unsigned)__builtin_amdgcn_fmed3f(fu + 1.f, 0.f, fx1);
+      const float a00 = *reinterpret_cast<const float*>(cb + 4u * (ra + xa));
+      const float a01 = *reinterpret_cast<const float*>(cb + 4u * (ra + xb));
+      const float a10 = *reinterpret_cast<const float*>(cb + 4u * (rb + xa));
+      const float a11 = *reinterpret_cast<const float*>(cb + 4u * (rb + xb));
+      const float val = (1.f - tv) * ((1.f - tu) * a00 + tu * a01) + tv * ((1.f - tu) * a10 + tu * a11);
+      const float wgt = (static_cast<int>(in_range(px, fnx)) & static_cast<int>(yin)) ? ta.w : 0.f;
+      sum += val * wgt;
+      samples += wgt;
     }
     if (samples > 0.f) {
       sum /= samples;
@@ -307,6 +378,10 @@ struct gtomx_tomopicker {
   float* iw = nullptr;  // image weights (device)
   std::vector<gtomx_float3> tilt;
   float* T = nullptr;   // nimg x 6 tilt rows (device)
+  // every tilt about y alone (second rows (0, 1, 0)), or every second row the same (a, b, 0): a
+  // tilt series about one in-plane axis (rot, tilt, psi = 0)
+  bool ytilt = false, common_row = false;
+  float row_a = 0.f, row_b = 1.f;
   float* mats = nullptr;  // nimg x 9 (device)
   float* h_mats = nullptr;  // pinned staging
   hipEvent_t staged = nullptr;
@@ -416,6 +491,15 @@ int tomo_angle(gtomx_tomopicker* tp, const gtomx_float3& angle, int a, bool mask
                      N);
   GX_LAUNCHED("k_local_cc");
   const size_t V = (size_t)dc.x * dc.y * dc.z;
+  if (tp->common_row && !tp->ytilt) {
+    const bool xmajor = std::fabs(tp->row_a) <= std::fabs(tp->row_b);
+    const float slope = xmajor ? -tp->row_a / tp->row_b : -tp->row_b / tp->row_a;
+    hipLaunchKernelGGL(k_tomo_update_axis, dim3(grid_for_n(V)), dim3(256), 0, s, tp->cc, tp->nx, tp->ny, (int)N,
+                       tp->T, tp->iw, dc.x, dc.y, dc.z, corr, corrangle, corridx, sum1, sum2,
+                       make_float3(angle.x, angle.y, angle.z), a, (int)xmajor, slope);
+    GX_LAUNCHED("k_tomo_update_axis");
+    return GTOMX_OK;
+  }
   hipLaunchKernelGGL(k_tomo_update, dim3(grid_for_n(V)), dim3(256), 0, s, tp->cc, tp->nx, tp->ny, (int)N, tp->T,
                      tp->iw, dc.x, dc.y, dc.z, corr, corrangle, corridx, sum1, sum2,
                      make_float3(angle.x, angle.y, angle.z), a);
@@ -536,6 +620,13 @@ int gtomx_tomopicker_set_image(gtomx_tomopicker* tp, float* d_image, const gtomx
     euler_rowmajor(tp->tilt[i], E);
     for (int k = 0; k < 6; ++k) T[6 * (size_t)i + k] = (float)E[k];  // rows 0 and 1 (TomoPicker.cu:193-196)
   }
+  tp->common_row = true;
+  for (unsigned i = 0; i < N; ++i)
+    tp->common_row = tp->common_row && T[6 * (size_t)i + 5] == 0.f && T[6 * (size_t)i + 3] == T[3] &&
+                     T[6 * (size_t)i + 4] == T[4];
+  tp->ytilt = tp->common_row && T[3] == 0.f && T[4] == 1.f;
+  tp->row_a = T[3];
+  tp->row_b = T[4];
   GX_HIP(hipMemcpyAsync(tp->T, T.data(), T.size() * sizeof(float), hipMemcpyHostToDevice, s));
   GX_HIP(hipMemcpyAsync(tp->iw, h_imageweights, N * sizeof(float), hipMemcpyHostToDevice, s));
   // filter = Exact2DWeighting(1; tilts, weights, maxfreq = nx) (:168-176); images <- C2R_n(R2C(images) * filter)

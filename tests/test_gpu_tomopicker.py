@@ -62,11 +62,13 @@ def _blob(nr, seed):
     return v.astype(np.float32)
 
 
-@pytest.mark.parametrize("circular,axis", [(True, "y"), (False, "y"), (True, "rotated"), (False, "rotated")])
+@pytest.mark.parametrize("circular,axis", [(True, "y"), (False, "y"), (True, "rotated"), (False, "rotated"),
+                                           (True, "common"), (False, "common-steep")])
 def test_tomopicker_matches_oracle(gx, orc, cuda, circular, axis):
     """axis "y": every tilt about y alone (the update reads one pixel row per image); "rotated":
     three of the five tilt axes turned in-plane by 0.1 ... 0.3 rad (the general bilinear form) and
-    two about y."""
+    two about y; "common" / "common-steep": every axis turned by 0.25 / 1.2 rad (one second row for
+    all images: lanes along x / along y over sheared rows)."""
     import torch
     nr, nx, ny, nimg = 16, 64, 64, 5
     ref = _blob(nr, 5)
@@ -76,6 +78,10 @@ def test_tomopicker_matches_oracle(gx, orc, cuda, circular, axis):
     tilts = _tilts(nimg, 40.0)
     if axis == "rotated":
         tilts[[0, 2, 3], 0] = [0.1, 0.2, 0.3]
+    elif axis == "common":
+        tilts[:, 0] = 0.25
+    elif axis == "common-steep":
+        tilts[:, 0] = 1.2
     w = rng.uniform(0.5, 1.0, nimg).astype(np.float32)
     angles = orc.equal_angular_spacing((0, TAU), (0, np.pi), (0, TAU), np.deg2rad(60.0))[:10].astype(np.float32)
     dc = (32, 24, 16)
