@@ -715,7 +715,7 @@ int gtomx_local_std(const float* d_map, gtomx_int3 dims, const float* d_fourierm
                     float* d_std, float* d_mean, void* stream) {
   if (!d_map || !d_std) return fail(GTOMX_ERR_INVALID_ARGUMENT, "NULL buffer");
   int nd = dims.z > 1 ? 3 : (dims.y > 1 ? 2 : 1);
-  GX_TRY(check_fft_dims(nd, dims));
+  GX_TRY(check_fft_dims_any(nd, dims));  // its transforms take every extent up to 2048
   DevBuf sb;
   GX_TRY(sb.alloc(local_std_scratch_bytes(dims)));
   LocalStdScratch sc = local_std_carve(sb.p, dims);

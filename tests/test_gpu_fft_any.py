@@ -85,3 +85,20 @@ def test_fft_any_c2r_nonhermitian(gx, orc, cuda, shape):
     print(f"C2R {shape}: {err:.2e}")
     assert err < TOL
     assert np.array_equal(to_np(Xd).reshape(X.shape), X)
+
+
+@pytest.mark.parametrize("shape,r", [((30, 40, 50), 6.0), ((1, 75, 100), 9.0), ((21, 33, 45), 5.0)])
+def test_local_std_any_extent_matches_oracle(gx, orc, cuda, shape, r):
+    """d_LocalStd (src/Generics/LocalStd.cu:10-118) on extents without a native transform, odd ones
+    included, against the FP64 oracle (its transforms are exact DFTs at these sizes)."""
+    import torch
+    rng = np.random.default_rng(sum(shape))
+    m = (3.0 + rng.standard_normal(shape)).astype(np.float32)
+    x = torch.from_numpy(m).to(cuda)
+    sd, mean = gx.d_LocalStd(x, r, with_mean=True)
+    torch.cuda.synchronize()
+    rsd, rmean = orc.local_std(m.astype(np.float64), r, with_mean=True)
+    err = rel_linf(to_np(sd).reshape(rsd.shape), rsd)
+    print(f"local std {shape}: {err:.2e}")
+    assert err < 1e-5
+    assert rel_linf(to_np(mean).reshape(rmean.shape), rmean) < 1e-5
