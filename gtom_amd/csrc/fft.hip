@@ -1052,7 +1052,7 @@ int fft_c2r_cols_inplace(float2* data, gtomx_int3 d, unsigned batch, hipStream_t
 // lengths, other prime factors), by Bluestein's chirp-z algorithm: with
 // c[j] = exp(-i pi j^2 / L), the forward DFT is
 //   X[k] = c[k] sum_j (x[j] c[j]) conj(c[k - j]),
-// a circular convolution of length M (the power of two >= 2L - 1, at least 8)
+// a circular convolution of length M (the shortest native length >= 2L - 1)
 // done with the native M-point column transforms: load a[j] = x[j] c[j]
 // (zero for j >= L), forward M-point DFT, times B[k] / M with B the DFT of
 // b[m] = conj(c[m]) (m in (-L, L), wrapped), inverse M-point DFT, times c[k].
@@ -1145,9 +1145,10 @@ int blue_tables(int L, BlueTables* out) {
     *out = it->second;
     return GTOMX_OK;
   }
+  // the shortest native length (2^a or 3 * 2^a, at least 8) that holds the linear convolution
+  if (L > 2048) return fail(GTOMX_ERR_UNSUPPORTED, "lengths without a native transform must be at most 2048");
   int M = 8;
-  while (M < 2 * L - 1) M *= 2;
-  if (M > 4096) return fail(GTOMX_ERR_UNSUPPORTED, "lengths without a native transform must be at most 2048");
+  while (M < 2 * L - 1 || !fft_len_ok_any(M)) ++M;
   const long double pi = 3.141592653589793238462643383279502884L;
   std::vector<std::complex<long double>> c(L), bf(M), bi(M);
   for (int j = 0; j < L; ++j) {
@@ -1216,7 +1217,7 @@ int blue_axis(const void* src, AxisLines as, void* dst, AxisLines ad, long long 
   const int chunk = (int)std::min<long long>(nlines, cap);
   StreamScratch W, W2;
   GX_TRY(W.take((size_t)M * chunk, s));
-  if (M > 512) GX_TRY(W2.take((size_t)M * chunk, s));
+  if (M > 256 && M != 512) GX_TRY(W2.take((size_t)M * chunk, s));  // the two-pass column lengths
   const int nout = OUT == 1 ? nh : L;
   for (long long l0 = 0; l0 < nlines; l0 += chunk) {
     const int nl = (int)std::min<long long>(chunk, nlines - l0);

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 4e-6
 
-SHAPES = [  # (nz, ny, nx); x, y, z each native (N) or chirp-z (C)
+SHAPES = [  # (nz, ny, nx); x, y, z each native (N) or chirp-z (C); the convolution length M is the
+    # shortest 2^a or 3 * 2^a >= 2L - 1
     (1, 1, 5),  # C
     (1, 1, 100),  # C
     (1, 1, 2047),  # C, the longest chirp-z length (4096-point convolution)
@@ -26,6 +27,8 @@ SHAPES = [  # (nz, ny, nx); x, y, z each native (N) or chirp-z (C)
     (40, 8, 200),  # C x, N y, C z
     (3, 2, 1000),  # tiny y and z
     (1, 96, 384),  # C x (384 has native columns but no native row kernel), N y
+    (1, 150, 320),  # C x and C y on 3 * 2^a convolution lengths: 768 and 384 (two-pass columns)
+    (1, 2, 1500),  # C x on a 3072-point convolution
     (6, 384, 384),  # C x, N y (two-pass), C z
 ]
 
