@@ -52,7 +52,7 @@ def parse():
                     help="held-out blocks timed by the learned comparison")
     ap.add_argument("--row", choices=["prob_picker", "picker", "pick2d", "diff2", "local_peaks", "fft_micrograph",
                                       "prob_picker_micrograph", "pick2d_micrograph", "micrographs", "split", "pnet",
-                                      "pick_large", "tomo_picker"],
+                                      "pick_large", "tomo_picker", "pick_generic"],
                     default=None,
                     help="measure a row next to the search instead (one JSON line of its own)")
     return ap.parse_args()
@@ -403,6 +403,51 @@ def row_pick_large(args):
                      "voxel-orientations/s", work, sec,
                      {"workload": "one 512^3 volume, 48^3 template, 1152 orientations", "volume": [n] * 3,
                       "orientations": len(angles), "batch": batch}, None)
+
+
+def row_pick_generic(args):
+    """d_PickSubTomograms on a 200^3 volume, a side without a fused search (the long-row form on
+    chirp-z transforms): one volume of noise with 4 planted copies of a 32^3 blob template
+    (projector 64^3, oversample 64/200), a CTF (defocus 3 um, 10 A/px), the first 576 orientations of
+    the 500-direction x 18 set, local-std radius 16."""
+    import torch
+    import gtom_amd as gx
+    from gtom_amd import synth
+    rng = np.random.default_rng(args.seed)
+    n, box = 200, 32
+    p = 2 * box
+    dev = torch.device("cuda:0")
+    template = synth.blob_template(box, rng, radius=10.0)
+    angles = gx.fibonacci_angles(500, 18)[:576]
+    pos = [tuple(int(v) for v in rng.integers(box, n - box, 3)) for _ in range(4)]
+    ori = [tuple(float(a) for a in angles[int(k)]) for k in rng.integers(0, len(angles), 4)]
+    tomo = (synth.plant((n, n, n), template, pos, ori, 0.3) + rng.standard_normal((n, n, n), dtype=np.float32))
+    tomo = tomo.astype(np.float32)
+    E = gx.d_FFTR2C(torch.from_numpy(tomo[None]).to(dev))
+    ctf = gx.ctf_3d([gx.CtfParams.make(pixelsize=10e-10, Cs=2.7e-3, voltage=300e3, defocus=-3e-6, amplitude=0.07)],
+                    n, device=dev)
+    projft = gx.prepare_projector(torch.from_numpy(template).to(dev), p)
+    plan = gx.PickPlan(p, n)
+    best = torch.empty((1, n, n, n), device=dev)
+    idx = torch.empty((1, n, n, n), dtype=torch.int32, device=dev)
+    sec = _timed_steps(args, lambda: gx.d_PickSubTomograms(plan, projft, p / n, E, ctf, angles, 16.0, best=best,
+                                                            bestidx=idx))
+    batch = plan.batch
+    plan.close()
+    work = n ** 3 * len(angles)
+
+    def cpu():
+        from oracle import pyoracle as po
+        k = 2
+        t0 = time.perf_counter()
+        po.pick(projft.cpu().numpy().astype(np.complex128), p, p / n, E.cpu().numpy().astype(np.complex128),
+                ctf.cpu().numpy()[0].astype(np.float64), n, angles[:k].astype(np.float64), 16.0)
+        ct = time.perf_counter() - t0
+        return n ** 3 * k, ct, f"same volume, the first {k} orientations, FP64 oracle, {ct:.1f} s"
+    return _row_line(args, "pick_generic", "voxel-orientations/s, d_PickSubTomograms on a 200^3 volume (chirp-z form)",
+                     "voxel-orientations/s", work, sec,
+                     {"workload": "one 200^3 volume, 32^3 template, 576 orientations", "volume": [n] * 3,
+                      "orientations": len(angles), "batch": batch}, cpu)
 
 
 def row_pick2d(args):
@@ -948,7 +993,7 @@ def main():
                 "prob_picker_micrograph": row_prob_picker_micrograph,
                 "pick2d_micrograph": row_pick2d_micrograph, "micrographs": row_micrographs,
                 "split": row_split, "pnet": row_pnet, "pick_large": row_pick_large,
-                "tomo_picker": row_tomo_picker}[args.row](args)
+                "tomo_picker": row_tomo_picker, "pick_generic": row_pick_generic}[args.row](args)
     import torch
     import gtom_amd as gx
     from gtom_amd import synth
