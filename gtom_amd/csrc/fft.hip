@@ -1066,8 +1066,11 @@ int fft_c2r_cols_inplace(float2* data, gtomx_int3 d, unsigned batch, hipStream_t
 struct AxisLines {
   long long P, S0, S1, es;
 };
+// (line indices below 2^32: 32-bit division; P above 2^31 means no period)
 __device__ __forceinline__ long long line_base(const AxisLines& a, long long l) {
-  return (l / a.P) * a.S1 + (l % a.P) * a.S0;
+  if (a.P > 0x7fffffffLL) return l * a.S0;
+  const unsigned u = (unsigned)l, p = (unsigned)a.P;
+  return (long long)(u / p) * a.S1 + (long long)(u % p) * a.S0;
 }
 
 // MODE 0: complex points; 1: real points (the x rows of an R2C); 2: a half
@@ -1121,6 +1124,77 @@ __global__ __launch_bounds__(256) void k_blue_store(const float2* __restrict__ W
       static_cast<float*>(dst)[o] = v.x * scale;
     else
       static_cast<float2*>(dst)[o] = v;
+  }
+}
+
+// The row modes (1, 2: points contiguous, es = 1) through a 64 x 64 LDS tile:
+// the rows are read along the row and the block written along the line index,
+// both coalesced. Grid (ceil(M / 64), ceil(nl / 64)) for the load,
+// (ceil(nout / 64), ceil(nl / 64)) for the store.
+template <int MODE, int DIR>
+__global__ __launch_bounds__(256) void k_blue_load_rows(const void* __restrict__ src, AxisLines a, long long l0,
+                                                        int nl, int L, int nh, int M,
+                                                        const float2* __restrict__ chirp, float2* __restrict__ W) {
+  __shared__ float tx[64][65], ty[64][65];  // split: a column read is conflict-free
+  __shared__ long long lbase[64];
+  const int jb = blockIdx.x * 64, lb = blockIdx.y * 64, t = threadIdx.x;
+  if (t < 64) lbase[t] = line_base(a, l0 + lb + t);
+  __syncthreads();
+  for (int r = 0; r < 16; ++r) {
+    const int ll = (t >> 6) + 4 * r, jj = t & 63;
+    const int l = lb + ll, j = jb + jj;
+    float2 v = make_float2(0.f, 0.f);
+    if (l < nl && j < L) {
+      const long long b = lbase[ll];
+      if constexpr (MODE == 1) {
+        v = make_float2(static_cast<const float*>(src)[b + j], 0.f);
+      } else {
+        const float2* X = static_cast<const float2*>(src);
+        v = j < nh ? X[b + j] : cconj(X[b + (L - j)]);
+      }
+      const float2 c = chirp[j];
+      v = cmul(v, DIR < 0 ? c : cconj(c));
+    }
+    tx[ll][jj] = v.x;
+    ty[ll][jj] = v.y;
+  }
+  __syncthreads();
+  for (int r = 0; r < 16; ++r) {
+    const int jj = (t >> 6) + 4 * r, ll = t & 63;
+    const int l = lb + ll, j = jb + jj;
+    if (l < nl && j < M) W[(long long)j * nl + l] = make_float2(tx[ll][jj], ty[ll][jj]);
+  }
+}
+
+template <int MODE, int DIR>
+__global__ __launch_bounds__(256) void k_blue_store_rows(const float2* __restrict__ W, AxisLines a, long long l0,
+                                                         int nl, int nout, const float2* __restrict__ chirp,
+                                                         void* __restrict__ dst, float scale) {
+  __shared__ float tx[64][65], ty[64][65];  // split: a column read is conflict-free
+  __shared__ long long lbase[64];
+  const int kb = blockIdx.x * 64, lb = blockIdx.y * 64, t = threadIdx.x;
+  if (t < 64) lbase[t] = line_base(a, l0 + lb + t);
+  for (int r = 0; r < 16; ++r) {
+    const int kk = (t >> 6) + 4 * r, ll = t & 63;
+    const int l = lb + ll, k = kb + kk;
+    if (l < nl && k < nout) {
+      const float2 c = chirp[k];
+      const float2 v = cmul(W[(long long)k * nl + l], DIR < 0 ? c : cconj(c));
+      tx[ll][kk] = v.x;
+      ty[ll][kk] = v.y;
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < 16; ++r) {
+    const int ll = (t >> 6) + 4 * r, kk = t & 63;
+    const int l = lb + ll, k = kb + kk;
+    if (l < nl && k < nout) {
+      const long long o = lbase[ll] + k;
+      if constexpr (MODE == 2)
+        static_cast<float*>(dst)[o] = tx[ll][kk] * scale;
+      else
+        static_cast<float2*>(dst)[o] = make_float2(tx[ll][kk], ty[ll][kk]);
+    }
   }
 }
 
@@ -1213,7 +1287,10 @@ int blue_axis(const void* src, AxisLines as, void* dst, AxisLines ad, long long 
   BlueTables t;
   GX_TRY(blue_tables(L, &t));
   const int M = t.M;
-  const long long cap = std::max<long long>(1, std::min<long long>(((1LL << 31) - 1) / M, (32LL << 20) / M));
+  // chunks within 2^31 elements, 256 MB and 65535 tiles of 64 lines
+  const long long cap =
+      std::max<long long>(1, std::min({((1LL << 31) - 1) / M, (32LL << 20) / M, 65535LL * 64}));
+  if (nlines >= (1LL << 32)) return fail(GTOMX_ERR_UNSUPPORTED, "2^32 or more lines on one axis");
   const int chunk = (int)std::min<long long>(nlines, cap);
   StreamScratch W, W2;
   GX_TRY(W.take((size_t)M * chunk, s));
@@ -1221,17 +1298,31 @@ int blue_axis(const void* src, AxisLines as, void* dst, AxisLines ad, long long 
   const int nout = OUT == 1 ? nh : L;
   for (long long l0 = 0; l0 < nlines; l0 += chunk) {
     const int nl = (int)std::min<long long>(chunk, nlines - l0);
-    hipLaunchKernelGGL((k_blue_load<IN, DIR>), dim3(blue_grid((long long)M * nl)), dim3(256), 0, s, src, as, l0, nl,
-                       L, nh, M, t.chirp, W.p);
-    GX_LAUNCHED("k_blue_load");
+    if constexpr (IN == 0) {
+      hipLaunchKernelGGL((k_blue_load<IN, DIR>), dim3(blue_grid((long long)M * nl)), dim3(256), 0, s, src, as, l0,
+                         nl, L, nh, M, t.chirp, W.p);
+      GX_LAUNCHED("k_blue_load");
+    } else {
+      if (as.es != 1) return fail(GTOMX_ERR_INTERNAL, "row loads take contiguous lines");
+      hipLaunchKernelGGL((k_blue_load_rows<IN, DIR>), dim3((M + 63) / 64, (nl + 63) / 64), dim3(256), 0, s, src, as,
+                         l0, nl, L, nh, M, t.chirp, W.p);
+      GX_LAUNCHED("k_blue_load_rows");
+    }
     GX_TRY(cols_axis<-1>(M, W.p, W2.p, nl, nl, 1, 1LL << 62, (long long)M * nl, 0, tw, s));
     hipLaunchKernelGGL(k_blue_mul, dim3(blue_grid((long long)M * nl)), dim3(256), 0, s, W.p, nl, M,
                        DIR < 0 ? t.bf : t.bi);
     GX_LAUNCHED("k_blue_mul");
     GX_TRY(cols_axis<1>(M, W.p, W2.p, nl, nl, 1, 1LL << 62, (long long)M * nl, 0, tw, s));
-    hipLaunchKernelGGL((k_blue_store<OUT, DIR>), dim3(blue_grid((long long)nout * nl)), dim3(256), 0, s, W.p, ad,
-                       l0, nl, nout, t.chirp, dst, scale);
-    GX_LAUNCHED("k_blue_store");
+    if constexpr (OUT == 0) {
+      hipLaunchKernelGGL((k_blue_store<OUT, DIR>), dim3(blue_grid((long long)nout * nl)), dim3(256), 0, s, W.p, ad,
+                         l0, nl, nout, t.chirp, dst, scale);
+      GX_LAUNCHED("k_blue_store");
+    } else {
+      if (ad.es != 1) return fail(GTOMX_ERR_INTERNAL, "row stores take contiguous lines");
+      hipLaunchKernelGGL((k_blue_store_rows<OUT, DIR>), dim3((nout + 63) / 64, (nl + 63) / 64), dim3(256), 0, s, W.p,
+                         ad, l0, nl, nout, t.chirp, dst, scale);
+      GX_LAUNCHED("k_blue_store_rows");
+    }
   }
   return GTOMX_OK;
 }
