@@ -538,8 +538,9 @@ int gtomx_tomopicker_create(gtomx_tomopicker** out, const float* d_ref, gtomx_in
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "1 to 128 tilt images (c_tiltmatrices, TomoPicker.cu:19)");
   if (dimsimage.z != 1) return fail(GTOMX_ERR_INVALID_ARGUMENT, "dimsimage is 2D");
   const int p = 2 * dimsref.x;
-  GX_TRY(check_fft_dims(3, gtomx_int3{p, p, p}));
-  GX_TRY(check_fft_dims(2, dimsimage));
+  // every transform here is fft_r2c / fft_c2r, which take any extent up to 2048
+  GX_TRY(check_fft_dims_any(3, gtomx_int3{p, p, p}));
+  GX_TRY(check_fft_dims_any(2, dimsimage));
   if (dimsref.x > dimsimage.x || dimsref.x > dimsimage.y)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "the reference is larger than the images");
   hipStream_t s = as_stream(stream);

@@ -111,6 +111,39 @@ def test_tomopicker_matches_oracle(gx, orc, cuda, circular, axis):
     assert np.array_equal(to_np(ang)[sure], angles[ref64["idx"][sure]])
 
 
+def test_tomopicker_non_native_sizes_match_oracle(gx, orc, cuda):
+    """A 10^3 reference (padded projector 20^3) and 50 x 36 tilt images: extents without a native
+    transform, which take the chirp-z form; the same tolerances as above."""
+    import torch
+    nr, nx, ny, nimg = 10, 50, 36, 4
+    ref = _blob(nr, 15)
+    mask = _sphere(nr, 4)
+    rng = np.random.default_rng(16)
+    images = rng.standard_normal((nimg, ny, nx)).astype(np.float32)
+    tilts = _tilts(nimg, 45.0)
+    tilts[:, 0] = 0.2
+    w = rng.uniform(0.5, 1.0, nimg).astype(np.float32)
+    angles = orc.equal_angular_spacing((0, TAU), (0, np.pi), (0, TAU), np.deg2rad(60.0))[:6].astype(np.float32)
+    dc = (30, 20, 12)
+    tp = gx.TomoPicker(torch.from_numpy(ref).to(cuda), torch.from_numpy(mask).to(cuda), (ny, nx), nimg)
+    img = torch.from_numpy(images).to(cuda)
+    tp.set_image(img, tilts, w)
+    corr = torch.full((dc[2], dc[1], dc[0]), -1e30, device=cuda)
+    idx = torch.full(corr.shape, -1, dtype=torch.int32, device=cuda)
+    tp.perform_correlation(corr, angles=angles, index=idx)
+    torch.cuda.synchronize()
+    tp.close()
+    ref64 = orc.tomo_picker(ref.astype(np.float64), mask.astype(np.float64), True, images.astype(np.float64),
+                            tilts.astype(np.float64), w.astype(np.float64), angles.astype(np.float64), dc)
+    assert rel_linf(to_np(img), ref64["images"]) <= 1e-5
+    err = rel_linf(to_np(corr), ref64["corr"])
+    print(f"TomoPicker 10^3 / 50 x 36: max error {err:.2e}")
+    assert err <= 1e-4, err
+    sure = (ref64["corr"] - ref64["corr2"]) > 1e-4 * np.abs(ref64["corr"]).max()
+    assert sure.sum() > 0.5 * sure.size
+    assert np.array_equal(to_np(idx)[sure], ref64["idx"][sure])
+
+
 def test_tomopicker_anglestep_finds_planted_projection(gx, orc, cuda):
     """PerformCorrelation(anglestep) on one untilted image holding the reference's
     projection along z (numpy's sum) at a pixel: the voxels above it hold the
