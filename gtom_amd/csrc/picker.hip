@@ -258,8 +258,9 @@ int gtomx_picker_create(gtomx_picker** out, const float* d_ref, gtomx_int3 dimsr
     return fail(GTOMX_ERR_UNSUPPORTED, "the reference must be a cube (3D) or a square (2D)");
   if (is2d != (dimsimage.z == 1))
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference and image must have the same dimensionality");
-  GX_TRY(check_fft_dims(is2d ? 2 : 3, dimsref));
-  GX_TRY(check_fft_dims(is2d ? 2 : 3, dimsimage));
+  // extents without a native transform take the chirp-z form (and full, not banded, transforms)
+  GX_TRY(check_fft_dims_any(is2d ? 2 : 3, dimsref));
+  GX_TRY(check_fft_dims_any(is2d ? 2 : 3, dimsimage));
   if (dimsref.x > dimsimage.x || dimsref.y > dimsimage.y || dimsref.z > dimsimage.z)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "the reference is larger than the image");
   if (batch > 64) return fail(GTOMX_ERR_INVALID_ARGUMENT, "batch must be <= 64");
@@ -371,7 +372,7 @@ int gtomx_picker_correlate_angles(gtomx_picker* pk, const gtomx_float3* h_angles
     // (FFT order); where the row kernels allow, the R2C runs over that band
     // of rows and the normalisation moves into the product (same values up
     // to rounding).
-    const int rw = rows_per_workgroup(pk->di.x);
+    const int rw = dims_native(pk->di) ? rows_per_workgroup(pk->di.x) : 0;
     const int vby = rw > 0 ? (dr.y / 2 + 1 + rw - 1) / rw * rw : 0;
     const int vbz = 2 * (dr.z / 2 + 1) <= pk->di.z ? dr.z / 2 + 1 : 0;
     const bool vbanded = pk->nrz > 1 && pk->di.x <= 256 && pk->di.y <= 256 && pk->di.z <= 256 && vby > 0 &&

@@ -551,7 +551,7 @@ int gtomx_prob_picker_create(gtomx_prob_picker** out, const float* d_refs, gtomx
   if (nrefs == 0) return fail(GTOMX_ERR_INVALID_ARGUMENT, "nrefs must be positive");
   const int nd = dimsimage.z > 1 ? 3 : 2;
   if ((dimsref.z > 1 ? 3 : 2) != nd) return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference and image dimensionality differ");
-  GX_TRY(check_fft_dims(nd, dimsimage));
+  GX_TRY(check_fft_dims_any(nd, dimsimage));  // non-native extents: chirp-z, full transforms
   if (dimsref.x < 1 || dimsref.y < 1 || dimsref.z < 1 || dimsref.x > dimsimage.x || dimsref.y > dimsimage.y ||
       dimsref.z > dimsimage.z)
     return fail(GTOMX_ERR_INVALID_ARGUMENT, "reference dims must be positive and fit the image");
@@ -825,11 +825,12 @@ int gtomx_prob_picker_correlate(gtomx_prob_picker* pp, unsigned n, float anglest
     // survive the mask, so the transforms skip the rest (the same values
     // where they are used).
     const int n2d = pp->di.x;
-    const bool banded = pp->ndims == 2 && pp->circular && n2d > 256 && pp->di.y == n2d;
+    const bool native = dims_native(pp->di);
+    const bool banded = native && pp->ndims == 2 && pp->circular && n2d > 256 && pp->di.y == n2d;
     const int band = std::min(n2d / 2, (pp->dr.y / 2 + 1 + 7) / 8 * 8);
     // the same for volumes and images up to 256 per axis: bands in y (whole
     // row blocks of the row kernels) and, for volumes, in z
-    const int rw = rows_per_workgroup(pp->di.x);
+    const int rw = native ? rows_per_workgroup(pp->di.x) : 0;
     const int vby = rw > 0 ? (pp->dr.y / 2 + 1 + rw - 1) / rw * rw : 0;
     const int vbz = pp->ndims == 3 && 2 * (pp->dr.z / 2 + 1) <= pp->di.z ? pp->dr.z / 2 + 1 : 0;
     const bool vbanded = !banded && pp->circular && pp->di.x <= 256 && pp->di.y <= 256 && pp->di.z <= 256 &&

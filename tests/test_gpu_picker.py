@@ -167,3 +167,23 @@ def test_picker_2d_anglestep_finds_quarter_turn(gx, orc, cuda):
     pos = np.unravel_index(np.argmax(best), best.shape)
     assert pos == (6, n - 9)
     assert np.array_equal(ang[pos], np.array([0.0, 0.0, np.float32(2 * np.pi) / 4], np.float32))
+
+
+def test_picker_non_native_sizes_match_oracle(gx, orc, cuda):
+    """A 10^3 reference and a 20 x 30 x 36 image: extents without a native transform take the
+    chirp-z form with full (not banded) transforms; the same bounds as above."""
+    nr = 10
+    ref = _blob(nr, 13)
+    rng = np.random.default_rng(14)
+    image = rng.standard_normal((20, 30, 36)).astype(np.float32)
+    ctf = rng.uniform(-1, 1, (nr, nr, nr // 2 + 1)).astype(np.float32)
+    angles = orc.equal_angular_spacing((0, TAU), (0, np.pi), (0, TAU), np.deg2rad(60.0))[:11]
+    best, ang, _ = _run_picker(gx, cuda, ref, image, ctf, angles=angles, batch=4)
+    ref64 = orc.picker(ref.astype(np.float64), image.astype(np.float64), ctf.astype(np.float64),
+                       angles.astype(np.float64))
+    err = rel_linf(best, ref64["best"])
+    print(f"Picker 10^3 / 20 x 30 x 36: max error {err:.2e}")
+    assert err <= 1e-4, err
+    sure = ref64["best"] - ref64["top2"] > 1e-4 * np.max(np.abs(ref64["best"]))
+    assert sure.sum() > 0.9 * sure.size
+    assert np.array_equal(ang[sure], angles[ref64["idx"][sure]])

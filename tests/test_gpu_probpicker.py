@@ -229,3 +229,27 @@ def test_bench_size_3d_matches_oracle(gx, orc, cuda):
     sel = (ob - top2) > 1e-4 * np.abs(ob).max()
     assert sel.mean() > 0.9
     assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_non_native_sizes_match_oracle(gx, orc, cuda, nd):
+    """Images of 100 x 90 and volumes of 30 x 30 x 40 with references of 20^2 / 10^3: extents without
+    a native transform take the chirp-z form with full (not banded) transforms."""
+    rng = np.random.default_rng(70 + nd)
+    if nd == 2:
+        di, dr, lp, step = (90, 100), (20, 20), 10, 0.5
+    else:
+        di, dr, lp, step = (40, 30, 30), (10, 10, 10), 5, 1.2
+    refs = rng.standard_normal((1,) + dr).astype(np.float32)
+    masks = _disc(dr, dr[0] / 2 - 1)[None]
+    image = rng.standard_normal(di).astype(np.float32)
+    b, a, r, nang = _run_gpu(gx, cuda, refs, masks, image, lp, step, None, 0)
+    orc.gaussrand(0, reset=True)
+    ob, oa, _, top2 = orc.prob_picker(refs[0].astype(np.float64), masks[0].astype(np.float64),
+                                      image.astype(np.float64), lp, step, refid=0)
+    err = rel_linf(to_np(b).astype(np.float64), ob)
+    print(f"ProbabilisticPicker {di} / {dr}: max error {err:.2e} over {nang} orientations")
+    assert err <= 1e-4
+    sel = (ob - top2) > 1e-4 * np.abs(ob).max()
+    assert sel.mean() > 0.9
+    assert np.array_equal(to_np(a)[sel], oa[sel].astype(np.float32))
