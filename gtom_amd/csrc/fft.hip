@@ -1399,7 +1399,7 @@ int fft_c2r_any(float2* work, float* out, gtomx_int3 d, unsigned batch, float sc
 
 int rows_c2r_any_inplace(float2* G, int n, int nb, long long H, hipStream_t s) {
   const long long nh = n / 2 + 1, nn = (long long)n * n;
-  if (n % 2 || H != nh * nn) return fail(GTOMX_ERR_INTERNAL, "rows_c2r_any_inplace takes n^3 half spectra, n even");
+  if (H != nh * nn) return fail(GTOMX_ERR_INTERNAL, "rows_c2r_any_inplace takes n^3 half spectra");
   return blue_axis<2, 2, 1>(G, AxisLines{1LL << 62, nh, 0, 1}, G, AxisLines{nn, n, 2 * H, 1}, (long long)nb * nn, n,
                             (int)nh, 1.f, s);
 }

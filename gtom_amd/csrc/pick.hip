@@ -342,8 +342,9 @@ __global__ __launch_bounds__(256) void k_pick_herm_norm(Projector P, const float
     if (b == 0) h = make_float2(0.f, 0.f);  // DC
     acc += (h.x * h.x + h.y * h.y) - (R.x * R.x + R.y * R.y);
   }
-  // the x = n/2 plane meets the sphere only at (n/2, 0, 0), its own mirror: Herm R = Re R
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  // the x = n/2 plane meets the sphere only at (n/2, 0, 0), its own mirror: Herm R = Re R (odd n have
+  // no such plane)
+  if ((N & 1) == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
     const float c = ctf ? ctf[HALF] : 1.f;
     const float2 R = cscale(project_at(rs, P.p, P.ph, M[0] * HALF, M[3] * HALF, M[6] * HALF), c);
     acc -= R.y * R.y;
@@ -1388,12 +1389,12 @@ int gtomx_pick_plan_create(gtomx_pick_plan** out, gtomx_int3 dp, gtomx_int3 dv, 
   const bool big2d = is2d && dv.x > 256;  // micrographs: the generic-transform form of the 2D branch
   // volumes above 256: the generic-transform form (sides with a long-row pass)
   const bool big3d = !is2d && (dv.x == 512 || dv.x == 768 || dv.x == 1024);
-  // other even volume sides from 8 to 1024: the same form on the chirp-z transforms
-  const bool gen3d = !is2d && !big3d && !fft_len_ok(dv.x) && dv.x % 2 == 0 && dv.x >= 8 && dv.x <= 1024;
+  // every other volume side from 8 to 1024: the same form on the chirp-z transforms
+  const bool gen3d = !is2d && !big3d && !fft_len_ok(dv.x) && dv.x >= 8 && dv.x <= 1024;
   const bool any3d = big3d || gen3d;
   if (big2d ? !row_len_native(dv.x) : (!any3d && !fft_len_ok(dv.x)))
     return fail(GTOMX_ERR_UNSUPPORTED,
-                "volume side must be even, from 8 to 1024; image side 2^a or 3*2^a from 8 to 256, or 512, 768, "
+                "volume side must be from 8 to 1024; image side 2^a or 3*2^a from 8 to 256, or 512, 768, "
                 "1024, 1536, 2048, 3072 or 4096");
   if (dp.x != dp.y || dp.x != dp.z || dp.x < 2 || dp.x % 2 || dp.x > 512)
     return fail(GTOMX_ERR_UNSUPPORTED, "projector must be an even cube of side <= 512");

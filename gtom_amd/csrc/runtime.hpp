@@ -188,7 +188,7 @@ int yz_axes_any(float2* data, gtomx_int3 dims, unsigned batch, int dir, hipStrea
 // The x C2R (real part, unscaled) of the nb * n * n rows of nb n^3 half spectra
 // H apart in G, by the chirp-z form, written in place: the real map of
 // spectrum o fills the first n^3 floats of its slot (every row is read before
-// the rows it overwrites are written). n even, at most 2048.
+// the rows it overwrites are written). n at most 2048.
 int rows_c2r_any_inplace(float2* G, int n, int nb, long long H, hipStream_t s);
 
 // Fourier-space resizing (resize.hip): op is one of the GTOMX_FFT_* values.

@@ -627,8 +627,8 @@ typedef struct gtomx_pick_plan gtomx_pick_plan;
  * default: 256 for volumes below 128, 128 up to 256, 32 or fewer above
  * (spectra of about 8 GiB), 240 for images as in the reference, fewer above
  * 256 (about 1 GiB)). Volume sides: 2^a and 3*2^a from 8 to 256 (fused
- * passes), 512, 768 and 1024 (long-row pass), and every other even side from
- * 8 to 1024 (the same form on chirp-z transforms). Image sides: 2^a and 3*2^a
+ * passes), 512, 768 and 1024 (long-row pass), and every other side from 8 to
+ * 1024, odd ones included (the same form on chirp-z transforms). Image sides: 2^a and 3*2^a
  * from 8 to 256, and 512, 768, 1024, 1536, 2048, 3072 and 4096. */
 int gtomx_pick_plan_create(gtomx_pick_plan** plan, gtomx_int3 dimsprojector,
                            gtomx_int3 dimsvolume, unsigned batch);

@@ -91,9 +91,9 @@ def test_validation_errors_without_device_work(gx):
     h = ctypes.c_void_p()
     assert L.gtomx_pick_plan_create(ctypes.byref(h), i3(96, 96, 96), i3(64, 64, 32), 0) == gx.ERR_UNSUPPORTED
     assert L.gtomx_pick_plan_create(ctypes.byref(h), i3(95, 95, 95), i3(64, 64, 64), 0) == gx.ERR_UNSUPPORTED
-    # the fused 3D search stops at 256 and the generic form takes every other even side up to 1024
-    # (odd sides and sides above 1024 are refused); images (dims.z == 1) go to 4096 or 3 * 2^10
-    assert L.gtomx_pick_plan_create(ctypes.byref(h), i3(96, 96, 96), i3(385, 385, 385), 0) == gx.ERR_UNSUPPORTED
+    # the fused 3D search stops at 256 and the generic form takes every other side up to 1024 (sides
+    # above 1024 are refused); images (dims.z == 1) go to 4096 or 3 * 2^10
+    assert L.gtomx_pick_plan_create(ctypes.byref(h), i3(96, 96, 96), i3(1025, 1025, 1025), 0) == gx.ERR_UNSUPPORTED
     assert L.gtomx_pick_plan_create(ctypes.byref(h), i3(96, 96, 96), i3(1026, 1026, 1026), 0) == gx.ERR_UNSUPPORTED
     assert L.gtomx_pick_plan_create(ctypes.byref(h), i3(96, 96, 96), i3(2048, 2048, 2048), 0) == gx.ERR_UNSUPPORTED
     assert L.gtomx_pick_plan_create(ctypes.byref(h), i3(96, 96, 96), i3(8192, 8192, 1), 0) == gx.ERR_UNSUPPORTED

@@ -1,11 +1,11 @@
-"""d_PickSubTomograms on volume sides without a fused search (every other even
-side up to 1024: 40, 100, 200, 384, ...): pass 1 over the whole half spectrum,
+"""d_PickSubTomograms on volume sides without a fused search (every other side
+up to 1024: 40, 45, 63, 100, 200, 384, ...; odd sides have no x = n/2 plane): pass 1 over the whole half spectrum,
 the plane correction and scale as in the fused form, the z and y inverses and
 the row C2R by the chirp-z transforms (native where the length has one), and
 the in-order max/argmax over the real maps. The reference takes any size cuFFT
 plans (src/Correlation/SubTomograms.cu:19-125).
 
-- 40^3 and 100^3 against the FP64 oracle over two batches, with a CTF: max-CC
+- 40^3, 100^3 and the odd 45^3 and 63^3 against the FP64 oracle over two batches, with a CTF: max-CC
   within 1e-4 normwise, argmax identical wherever the oracle's top-2 margin
   exceeds 1e-4 * max|top1|.
 - 200^3 and 384^3, a size-independent property: a copy planted at a search
@@ -24,7 +24,7 @@ def _volume(rng, n, template, pos, angle, amplitude, noise):
     return (sig + noise * rng.standard_normal((n, n, n), dtype=np.float32)).astype(np.float32)
 
 
-@pytest.mark.parametrize("n,box,nang,batch", [(40, 10, 5, 3), (100, 16, 4, 3)])
+@pytest.mark.parametrize("n,box,nang,batch", [(40, 10, 5, 3), (100, 16, 4, 3), (45, 10, 4, 3), (63, 12, 3, 2)])
 def test_generic_side_matches_oracle(gx, orc, cuda, n, box, nang, batch):
     import torch
     from gtom_amd import synth
